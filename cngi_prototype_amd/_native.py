@@ -1,0 +1,356 @@
+"""ctypes binding of libngcasa_hip.so, the gfx950 gridding library.
+
+Every entry point of ``include/ngcasa_gridding.h`` is wrapped here.  Large
+arrays travel as torch CUDA (HIP) tensors; their device pointers and the
+current torch stream are handed to the library.  There is no fallback: if the
+library is missing or the device is unavailable, the call raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libngcasa_hip.so")
+
+# constants mirrored from include/ngcasa_gridding.h
+NGC_OK = 0
+ERRORS = {-1: "NGC_ERR_ARG", -2: "NGC_ERR_HIP", -3: "NGC_ERR_UNSUPPORTED",
+          -4: "NGC_ERR_FIELD", -5: "NGC_ERR_FFT"}
+GRID_IMAGE, GRID_PSF, GRID_DENSITY, GRID_WEIGHT = 0, 1, 2, 3
+PREC_FP64, PREC_FP64_TILED, PREC_FP32_KAHAN = 0, 1, 2
+PRECISIONS = {"fp64": PREC_FP64, "fp64_tiled": PREC_FP64_TILED, "fp32_kahan": PREC_FP32_KAHAN}
+C64, C128, F32, F64 = 0, 1, 2, 3
+WEIGHT_UNIFORM, WEIGHT_BRIGGS = 1, 2
+
+EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_version",
+           "ngc_standard_grid", "ngc_briggs_factors", "ngc_imaging_weight_degrid",
+           "ngc_aperture_grid", "ngc_grid_to_image", "ngc_normalize_mosaic")
+
+
+class NativeError(RuntimeError):
+    """A libngcasa_hip call returned an error status."""
+
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed with {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+_vp, _i64, _i32, _d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+
+_PROTOS = {
+    "ngc_context_create": (_i32, [_i32, ctypes.POINTER(_vp)]),
+    "ngc_context_destroy": (_i32, [_vp]),
+    "ngc_last_error": (ctypes.c_char_p, []),
+    "ngc_version": (ctypes.c_char_p, []),
+    "ngc_standard_grid": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _vp,
+                                 _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
+                                 _i64, _i64, _d, _d, _i64, _i64, _i32, _vp, _vp]),
+    "ngc_briggs_factors": (_i32, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i32, _d, _vp]),
+    "ngc_imaging_weight_degrid": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp,
+                                         _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
+                                         _d, _d, _vp, _i32]),
+    "ngc_aperture_grid": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _vp,
+                                 _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64,
+                                 _i64, _i64, _i64, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
+                                 _i64, _i64, _i64, _d, _d, _i64, _i64, _i32, _vp, _vp]),
+    "ngc_grid_to_image": (_i32, [_vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64, _i64, _vp, _vp,
+                                 _vp, _vp]),
+    "ngc_normalize_mosaic": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64,
+                                    _i64, _d, _i32]),
+}
+
+
+def load_library(path=LIB_PATH):
+    """Load libngcasa_hip.so and declare its prototypes.  Raises if missing."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"libngcasa_hip.so not found at {path}; build it with "
+                "`python -m cngi_prototype_amd.build` (there is no CPU fallback)")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def _check(fn, rc):
+    if rc != NGC_OK:
+        msg = load_library().ngc_last_error().decode(errors="replace")
+        raise NativeError(fn, rc, msg)
+
+
+class Context:
+    """Owns one ngc_context (device workspace, rocFFT plans) on one device."""
+
+    def __init__(self, device=0):
+        lib = load_library()
+        h = _vp()
+        _check("ngc_context_create", lib.ngc_context_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = int(device)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                load_library().ngc_context_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+_contexts = {}
+
+
+def context(device=None):
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device is visible; the MI355X gridding path needs a GPU")
+    if device is None:
+        device = torch.cuda.current_device()
+    if device not in _contexts:
+        _contexts[device] = Context(device)
+    return _contexts[device]
+
+
+# ---------------------------------------------------------------------------
+# helpers
+# ---------------------------------------------------------------------------
+def _stream():
+    import torch
+    return _vp(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return None if t is None else _vp(t.data_ptr())
+
+
+def _host(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data_as(_vp)
+
+
+def _dt_vis(t):
+    import torch
+    if t.dtype == torch.complex64:
+        return C64
+    if t.dtype == torch.complex128:
+        return C128
+    raise TypeError(f"visibilities must be complex64 or complex128, got {t.dtype}")
+
+
+def _dt_w(t):
+    import torch
+    if t.dtype == torch.float32:
+        return F32
+    if t.dtype == torch.float64:
+        return F64
+    raise TypeError(f"weights must be float32 or float64, got {t.dtype}")
+
+
+def _dt_grid(t):
+    import torch
+    return {torch.complex64: C64, torch.complex128: C128, torch.float32: F32,
+            torch.float64: F64}[t.dtype]
+
+
+def _dev(t, name):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a device (HIP) tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
+
+
+# ---------------------------------------------------------------------------
+# entry points
+# ---------------------------------------------------------------------------
+def standard_grid(uvw, weight, freq_chan, chan_map, pol_map, cgk_1d, n_imag_chan, n_uv,
+                  delta_lm, support, oversampling, mode, precision, vis=None, flag=None,
+                  grid=None, sum_weight=None, accumulate=False, n_imag_pol=None):
+    """ngc_standard_grid on device tensors.  Returns (grid, sum_weight)."""
+    import torch
+    ctx = context()
+    _dev(uvw, "uvw")
+    _dev(weight, "weight")
+    n_time, n_baseline, n_chan, n_pol = weight.shape
+    if tuple(uvw.shape[:2]) != (n_time, n_baseline) or uvw.shape[2] < 2:
+        raise ValueError("uvw shape does not match weight")
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    n_imag_pol = n_pol if n_imag_pol is None else int(n_imag_pol)
+    n_u, n_v = int(n_uv[0]), int(n_uv[1])
+    single = prec == PREC_FP32_KAHAN
+    if mode == GRID_IMAGE:
+        gdt = torch.complex64 if single else torch.complex128
+        _dev(vis, "vis")
+        if tuple(vis.shape) != tuple(weight.shape):
+            raise ValueError("vis shape does not match weight")
+    else:
+        gdt = torch.float32 if single else torch.float64
+    shape = (int(n_imag_chan), n_imag_pol, n_u, n_v)
+    if grid is None:
+        grid = torch.empty(shape, dtype=gdt, device=weight.device)
+    elif tuple(grid.shape) != shape or grid.dtype != gdt:
+        raise ValueError(f"grid must be {shape} {gdt}")
+    if sum_weight is None:
+        sum_weight = torch.empty(shape[:2], dtype=torch.float64, device=weight.device)
+    if flag is not None:
+        _dev(flag, "flag")
+        if flag.dtype != torch.uint8 or tuple(flag.shape) != tuple(weight.shape):
+            raise ValueError("flag must be uint8 shaped like weight")
+    fr, frp = _host(freq_chan, np.float64)
+    cm, cmp_ = _host(chan_map, np.int32)
+    pm, pmp = _host(pol_map, np.int32)
+    cg, cgp = _host(cgk_1d, np.float64)
+    lib = load_library()
+    rc = lib.ngc_standard_grid(
+        ctx.handle, _stream(), int(mode), prec, _ptr(uvw), _ptr(vis),
+        _dt_vis(vis) if vis is not None else C64, _ptr(weight), _dt_w(weight), _ptr(flag),
+        frp, cmp_, pmp, cgp, len(cg), n_time, n_baseline, n_chan, n_pol, int(n_imag_chan),
+        n_imag_pol, n_u, n_v, float(delta_lm[0]), float(delta_lm[1]), int(support),
+        int(oversampling), int(bool(accumulate)), _ptr(grid), _ptr(sum_weight))
+    _check("ngc_standard_grid", rc)
+    return grid, sum_weight
+
+
+def briggs_factors(density_grid, sum_weight, weighting, robust=0.5):
+    """ngc_briggs_factors.  Returns host float64 [2, n_imag_chan, n_imag_pol]."""
+    ctx = context()
+    _dev(density_grid, "density_grid")
+    _dev(sum_weight, "sum_weight")
+    nc, npol, nu, nv = density_grid.shape
+    out = np.zeros((2, nc, npol), dtype=np.float64)
+    w = {"uniform": WEIGHT_UNIFORM, "briggs": WEIGHT_BRIGGS}[weighting]
+    rc = load_library().ngc_briggs_factors(ctx.handle, _stream(), _ptr(density_grid),
+                                           _ptr(sum_weight), nc, npol, nu, nv, w,
+                                           float(robust), out.ctypes.data_as(_vp))
+    _check("ngc_briggs_factors", rc)
+    return out
+
+
+def imaging_weight_degrid(uvw, natural_weight, density_grid, factors, freq_chan, chan_map,
+                          pol_map, delta_lm, flag=None, out=None):
+    """ngc_imaging_weight_degrid.  Returns the imaging-weight tensor."""
+    import torch
+    ctx = context()
+    _dev(uvw, "uvw")
+    _dev(natural_weight, "natural_weight")
+    _dev(density_grid, "density_grid")
+    n_time, n_baseline, n_chan, n_pol = natural_weight.shape
+    nc, npol, nu, nv = density_grid.shape
+    if out is None:
+        out = torch.empty_like(natural_weight)
+    fac, facp = _host(factors, np.float64)
+    fr, frp = _host(freq_chan, np.float64)
+    cm, cmp_ = _host(chan_map, np.int32)
+    pm, pmp = _host(pol_map, np.int32)
+    rc = load_library().ngc_imaging_weight_degrid(
+        ctx.handle, _stream(), _ptr(uvw), _ptr(natural_weight), _dt_w(natural_weight),
+        _ptr(flag), frp, cmp_, pmp, _ptr(density_grid), facp, n_time, n_baseline, n_chan,
+        n_pol, nc, npol, nu, nv, float(delta_lm[0]), float(delta_lm[1]), _ptr(out),
+        _dt_w(out))
+    _check("ngc_imaging_weight_degrid", rc)
+    return out
+
+
+def aperture_grid(uvw, weight, field, field_id, freq_chan, chan_map, pol_map,
+                  cf_baseline_map, cf_chan_map, cf_pol_map, conv_kernel, support,
+                  phase_gradient, n_imag_chan, n_uv, delta_lm, oversampling, mode,
+                  precision="fp64", vis=None, flag=None, grid=None, sum_weight=None,
+                  accumulate=False):
+    """ngc_aperture_grid on device tensors.  Returns (complex grid, sum_weight)."""
+    import torch
+    ctx = context()
+    for t, n in ((uvw, "uvw"), (weight, "weight"), (field, "field"),
+                 (conv_kernel, "conv_kernel"), (phase_gradient, "phase_gradient")):
+        _dev(t, n)
+    if field.dtype != torch.int32:
+        raise TypeError("field must be int32")
+    if conv_kernel.dtype != torch.float64 or phase_gradient.dtype != torch.complex128:
+        raise TypeError("conv_kernel must be float64 and phase_gradient complex128")
+    n_time, n_baseline, n_chan, n_pol = weight.shape
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    n_cfb, n_cfc, n_cfp, k_u, k_v = conv_kernel.shape
+    if tuple(phase_gradient.shape[1:]) != (k_u, k_v):
+        raise ValueError("phase_gradient must be [n_field, K_u, K_v]")
+    n_u, n_v = int(n_uv[0]), int(n_uv[1])
+    shape = (int(n_imag_chan), n_pol, n_u, n_v)
+    gdt = torch.complex64 if prec == PREC_FP32_KAHAN else torch.complex128
+    if grid is None:
+        grid = torch.empty(shape, dtype=gdt, device=weight.device)
+    if sum_weight is None:
+        sum_weight = torch.empty(shape[:2], dtype=torch.float64, device=weight.device)
+    fid, fidp = _host(field_id, np.int32)
+    fr, frp = _host(freq_chan, np.float64)
+    cm, cmp_ = _host(chan_map, np.int32)
+    pm, pmp = _host(pol_map, np.int32)
+    b, bp = _host(cf_baseline_map, np.int32)
+    c, cp = _host(cf_chan_map, np.int32)
+    p, pp = _host(cf_pol_map, np.int32)
+    s, sp = _host(support, np.int32)
+    rc = load_library().ngc_aperture_grid(
+        ctx.handle, _stream(), int(mode), prec, _ptr(uvw), _ptr(vis),
+        _dt_vis(vis) if vis is not None else C64, _ptr(weight), _dt_w(weight), _ptr(flag),
+        _ptr(field), fidp, len(fid), frp, cmp_, pmp, bp, cp, pp, _ptr(conv_kernel), n_cfb,
+        n_cfc, n_cfp, k_u, k_v, sp, _ptr(phase_gradient), n_time, n_baseline, n_chan, n_pol,
+        int(n_imag_chan), n_pol, n_u, n_v, float(delta_lm[0]), float(delta_lm[1]),
+        int(oversampling[0]), int(oversampling[1]), int(bool(accumulate)), _ptr(grid),
+        _ptr(sum_weight))
+    if rc == -4:
+        raise IndexError("a row's FIELD_ID is not in the gcf's field_id list")
+    _check("ngc_aperture_grid", rc)
+    return grid, sum_weight
+
+
+def grid_to_image(grid, image_size, sum_weight=None, corr=None):
+    """ngc_grid_to_image.  grid [..., n_u, n_v] (leading axes are planes).
+    corr = (corr_l, corr_m) host arrays or None.  Returns float64
+    [planes..., n_l, n_m] device tensor."""
+    import torch
+    ctx = context()
+    _dev(grid, "grid")
+    lead = tuple(grid.shape[:-2])
+    n_u, n_v = grid.shape[-2:]
+    n_planes = int(np.prod(lead)) if lead else 1
+    n_l, n_m = int(image_size[0]), int(image_size[1])
+    image = torch.empty(lead + (n_l, n_m), dtype=torch.float64, device=grid.device)
+    if sum_weight is not None:
+        _dev(sum_weight, "sum_weight")
+        if sum_weight.numel() != n_planes or sum_weight.dtype != torch.float64:
+            raise ValueError("sum_weight must be float64 with one entry per plane")
+    cl = cm = None
+    clp = cmp_ = None
+    if corr is not None:
+        cl, clp = _host(corr[0], np.float64)
+        cm, cmp_ = _host(corr[1], np.float64)
+    rc = load_library().ngc_grid_to_image(ctx.handle, _stream(), _ptr(grid), _dt_grid(grid),
+                                          n_planes, n_u, n_v, n_l, n_m, _ptr(sum_weight), clp,
+                                          cmp_, _ptr(image))
+    _check("ngc_grid_to_image", rc)
+    return image
+
+
+def normalize_mosaic(image, sum_weight, ps_corr, norm_pb, pb, oversampling, pb_limit,
+                     single_precision):
+    """ngc_normalize_mosaic, in place on a float64 [planes..., n_l, n_m] tensor."""
+    ctx = context()
+    _dev(image, "image")
+    n_l, n_m = image.shape[-2:]
+    n_planes = image.numel() // (n_l * n_m)
+    rc = load_library().ngc_normalize_mosaic(
+        ctx.handle, _stream(), _ptr(image), _ptr(sum_weight), _ptr(ps_corr), _ptr(norm_pb),
+        _ptr(pb), n_planes, n_l, n_m, int(oversampling[0]), int(oversampling[1]),
+        float(pb_limit), int(bool(single_precision)))
+    _check("ngc_normalize_mosaic", rc)
+    return image

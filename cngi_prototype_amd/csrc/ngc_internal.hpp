@@ -1,0 +1,174 @@
+// Internal declarations of libngcasa_hip: context, error reporting, device
+// helpers shared by the gridding kernels.  Written for gfx950 (CDNA4):
+// 64-lane wavefronts, 160 KB LDS per CU, FP64 atomics in LDS and L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ngcasa_gridding.h"
+
+namespace ngc {
+
+constexpr double kCLight = 299792458.0;
+
+// ---------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------
+void set_error(const std::string &msg);
+
+struct Status {
+    int code = NGC_OK;
+    explicit operator bool() const { return code == NGC_OK; }
+};
+
+#define NGC_HIP(call)                                                           \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            ::ngc::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return NGC_ERR_HIP;                                                 \
+        }                                                                       \
+    } while (0)
+
+#define NGC_CHECK(cond, code, msg)     \
+    do {                               \
+        if (!(cond)) {                 \
+            ::ngc::set_error(msg);     \
+            return (code);             \
+        }                              \
+    } while (0)
+
+#define NGC_TRY(expr)            \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_ != NGC_OK)       \
+            return rc_;          \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// context: device, workspace arena, cached FFT plans
+// ---------------------------------------------------------------------------
+struct FftPlan;
+
+struct Buffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace ngc
+
+struct ngc_context {
+    int device = 0;
+    std::mutex mu;
+    hipStream_t last_stream = nullptr;
+    bool has_last_stream = false;
+    // named workspace buffers, grown on demand, freed with the context
+    std::map<std::string, ngc::Buffer> ws;
+    // rocFFT plans keyed by (n_u, n_v, batch, double_precision)
+    std::map<std::tuple<int64_t, int64_t, int64_t, int>, ngc::FftPlan *> plans;
+    int *d_error = nullptr;  // device error flag
+};
+
+namespace ngc {
+
+// Returns a device buffer of at least `bytes` under `name`; contents are
+// undefined.  The context must be locked.
+int workspace(ngc_context *ctx, const char *name, size_t bytes, void **out);
+
+// Switch the context to `stream`, synchronising the previous stream if it
+// differs (workspace buffers are reused across calls).
+int enter_stream(ngc_context *ctx, hipStream_t stream);
+
+// Copy a small host array to a named device workspace buffer (synchronous
+// w.r.t. the host buffer).
+int upload(ngc_context *ctx, hipStream_t stream, const char *name,
+           const void *host, size_t bytes, void **dev);
+
+int fft_destroy_plans(ngc_context *ctx);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// uv scale per channel, in the reference's operation order:
+// -(freq*delta*n_uv)/c  (_standard_grid.py:274-276)
+inline void uv_scale_host(const double *freq, int64_t n_chan, double delta_l,
+                          double delta_m, int64_t n_u, int64_t n_v,
+                          std::vector<double> &out) {
+    out.resize(2 * n_chan);
+    for (int64_t c = 0; c < n_chan; c++) {
+        out[c] = -((freq[c] * delta_l) * (double)n_u) / kCLight;
+        out[n_chan + c] = -((freq[c] * delta_m) * (double)n_v) / kCLight;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+template <typename T> struct VisLoad;
+template <> struct VisLoad<float2> {
+    __device__ static inline double2 ld(const void *p, int64_t i) {
+        float2 v = reinterpret_cast<const float2 *>(p)[i];
+        return make_double2(v.x, v.y);
+    }
+};
+template <> struct VisLoad<double2> {
+    __device__ static inline double2 ld(const void *p, int64_t i) {
+        return reinterpret_cast<const double2 *>(p)[i];
+    }
+};
+template <typename T> __device__ inline double ld_weight(const void *p, int64_t i) {
+    return (double)reinterpret_cast<const T *>(p)[i];
+}
+
+// Pixel position of one (row, channel) sample, with the reference's exact
+// FP64 operation order and rounding (_standard_grid.py:299-324):
+//   u = uvw_u*scale; u_pos = u + n_u//2; centre = int(u_pos+0.5) (truncation);
+//   offset index = floor((centre - u_pos)*os + 0.5).
+// No multiply-add is fused here, so the integer results match the reference
+// bit for bit.  Returns false when u or v is NaN.
+struct Pix {
+    int64_t uc, vc;        // centre cell
+    int64_t ucj, vcj;      // centre cell of the conjugate position
+    int64_t du, dv;        // oversampled offset index
+};
+
+__device__ inline bool pixel_position(double uvw_u, double uvw_v, double s0,
+                                      double s1, int64_t n_u, int64_t n_v,
+                                      int64_t os, Pix &p) {
+#pragma clang fp contract(off)
+    const double u = uvw_u * s0;
+    const double v = uvw_v * s1;
+    if (isnan(u) || isnan(v)) return false;
+    const double cu = (double)(n_u / 2), cv = (double)(n_v / 2);
+    const double u_pos = u + cu;
+    const double v_pos = v + cv;
+    const double u_pos_conj = -u + cu;
+    const double v_pos_conj = -v + cv;
+    p.uc = (int64_t)(u_pos + 0.5);
+    p.vc = (int64_t)(v_pos + 0.5);
+    p.ucj = (int64_t)(u_pos_conj + 0.5);
+    p.vcj = (int64_t)(v_pos_conj + 0.5);
+    const double uo = (double)p.uc - u_pos;
+    const double vo = (double)p.vc - v_pos;
+    p.du = (int64_t)floor(uo * (double)os + 0.5);
+    p.dv = (int64_t)floor(vo * (double)os + 0.5);
+    return true;
+}
+
+__device__ inline int64_t iabs64(int64_t x) { return x < 0 ? -x : x; }
+
+// wave-level reduction of a double (64 lanes)
+__device__ inline double wave_sum(double x) {
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+}  // namespace ngc

@@ -1,0 +1,233 @@
+/*
+ * ngcasa_gridding.h — C ABI of the MI355X gridding library (libngcasa_hip.so).
+ *
+ * This is the lower drop-in boundary of ngCASA's imaging hot path.  Each entry
+ * point replaces one numba kernel (or one numpy/dask step) of the reference
+ * and keeps its argument meaning:
+ *
+ *   ngc_standard_grid          _standard_grid_jit
+ *                              ngcasa/imaging/_imaging_utils/_standard_grid.py:243-371
+ *                              (called by _standard_grid_numpy_wrap :123-177 and
+ *                              _standard_grid_psf_numpy_wrap :180-236)
+ *   ngc_imaging_weight_degrid  _standard_imaging_weight_degrid_jit
+ *                              _standard_grid.py:467-518 (wrapper :443-464)
+ *   ngc_briggs_factors         calculate_briggs_parms
+ *                              ngcasa/imaging/make_imaging_weight.py:198-216
+ *   ngc_aperture_grid          _aperture_grid_jit  _aperture_grid.py:377-513
+ *                              (wrappers :294-371) and, with NGC_GRID_WEIGHT,
+ *                              _aperture_weight_grid_jit  _aperture_grid.py:181-291
+ *                              (wrapper :146-178)
+ *   ngc_grid_to_image          ifftshift/ifft2/fftshift + _remove_padding +
+ *                              correct_image: make_psf.py:117-131,
+ *                              make_image.py:116-130, make_mosaic_pb.py:119-129,
+ *                              make_image_with_gcf.py:143-147
+ *   ngc_normalize_mosaic       _normalize 'forward'  _imaging_utils/_normalize.py:17-89
+ *
+ * Conventions
+ *   - Large arrays (uvw, visibilities, weights, flags, grids, images) are
+ *     DEVICE pointers on the context's device.  Small parameter arrays
+ *     (freq_chan, chan_map, pol_map, cgk_1d, cf maps, field ids, supports,
+ *     briggs factors) are HOST pointers; the library copies them.
+ *   - Array layouts are the reference's, C order: visibilities and weights
+ *     [n_time, n_baseline, n_chan, n_pol], uvw [n_time, n_baseline, 3] in
+ *     metres, grids [n_imag_chan, n_imag_pol, n_u, n_v] (the gridder's own
+ *     orientation, before the reference's moveaxis), complex values
+ *     interleaved (re, im).
+ *   - Flags: the reference marks flagged samples by NaN (cngi/vis/apply_flags.py:49-53).
+ *     NaN is honoured exactly as the reference does; an optional u8 mask
+ *     [n_time, n_baseline, n_chan, n_pol] (nonzero = flagged) is treated the
+ *     same as a NaN weight.  Pass NULL for none.
+ *   - `accumulate` = 0 overwrites grid and sum_weight; 1 adds to them (the
+ *     reference's in-place convention with caller-zeroed outputs).
+ *   - Calls are ordered on `stream` (a hipStream_t; NULL = default stream) and
+ *     return before the work completes, except where a result must come back
+ *     to the host (ngc_briggs_factors) or a host-visible error must be checked.
+ *   - Every function returns 0 on success and a negative NGC_ERR_* code on
+ *     failure; ngc_last_error() gives the message (thread-local).  No C++
+ *     exception crosses this boundary.
+ *   - A context owns the device workspace, the cached kernel tables and the
+ *     rocFFT plans.  Calls on one context are serialised internally; separate
+ *     contexts may be used from separate threads.
+ */
+#ifndef NGCASA_GRIDDING_H
+#define NGCASA_GRIDDING_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define NGC_OK 0
+#define NGC_ERR_ARG (-1)         /* invalid argument (shape, dtype, range) */
+#define NGC_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define NGC_ERR_UNSUPPORTED (-3) /* valid input that this build does not handle */
+#define NGC_ERR_FIELD (-4)       /* a row's field id is absent from field_id */
+#define NGC_ERR_FFT (-5)         /* rocFFT failed */
+
+/* gridding modes */
+#define NGC_GRID_IMAGE 0   /* complex grid of vis*weight (make_image, make_image_with_gcf) */
+#define NGC_GRID_PSF 1     /* grid of weight (make_psf, make_psf_with_gcf) */
+#define NGC_GRID_DENSITY 2 /* weight density: nearest cell and its conjugate (make_imaging_weight) */
+#define NGC_GRID_WEIGHT 3  /* A-term weight image at the grid centre (make_mosaic_pb) */
+
+/* precision forms */
+#define NGC_PREC_FP64 0       /* plain FP64 drop-in: reference loop, FP64 atomics */
+#define NGC_PREC_FP64_TILED 1 /* uv-tiled binning, FP64 LDS accumulation */
+#define NGC_PREC_FP32_KAHAN 2 /* uv-tiled binning, FP32 LDS accumulation with
+                                 compensated (Kahan-Neumaier) folding; FP64 for
+                                 coordinates and sum_weight */
+
+/* element types */
+#define NGC_C64 0
+#define NGC_C128 1
+#define NGC_F32 2
+#define NGC_F64 3
+
+/* weighting for ngc_briggs_factors (make_imaging_weight.py:198-213) */
+#define NGC_WEIGHT_UNIFORM 1
+#define NGC_WEIGHT_BRIGGS 2
+
+typedef struct ngc_context ngc_context;
+
+/* Create a context on HIP device `device`. */
+int ngc_context_create(int device, ngc_context **ctx);
+int ngc_context_destroy(ngc_context *ctx);
+/* Message of the last failure on this thread ("" if none). */
+const char *ngc_last_error(void);
+/* Library version string. */
+const char *ngc_version(void);
+
+/*
+ * Standard (prolate-spheroidal) gridder: replaces _standard_grid_jit.
+ *   mode       NGC_GRID_IMAGE, NGC_GRID_PSF or NGC_GRID_DENSITY
+ *   precision  NGC_PREC_*; grid element type follows it:
+ *              FP64 forms: complex128 (IMAGE) / float64 (PSF, DENSITY);
+ *              FP32_KAHAN: complex64 (IMAGE) / float32 (PSF, DENSITY)
+ *   vis        device [T,B,C,P] of vis_dtype (NGC_C64|NGC_C128); NULL unless IMAGE
+ *   weight     device [T,B,C,P] of weight_dtype (NGC_F32|NGC_F64)
+ *   chan_map   host int32 [n_chan] -> image channel; pol_map host int32 [n_pol]
+ *   cgk_1d     host float64 [cgk_len] oversampled half kernel
+ *              (_gridding_convolutional_kernels.py:151-157); density mode uses {1.0}
+ *   delta_l/m  cell size in radians with the reference's sign (cell[0] negated)
+ *   grid       device out [n_imag_chan, n_imag_pol, n_u, n_v]
+ *   sum_weight device float64 out [n_imag_chan, n_imag_pol]
+ */
+int ngc_standard_grid(ngc_context *ctx, void *stream, int mode, int precision,
+                      const double *uvw, const void *vis, int vis_dtype,
+                      const void *weight, int weight_dtype, const uint8_t *flag,
+                      const double *freq_chan, const int32_t *chan_map,
+                      const int32_t *pol_map, const double *cgk_1d,
+                      int64_t cgk_len, int64_t n_time, int64_t n_baseline,
+                      int64_t n_chan, int64_t n_pol, int64_t n_imag_chan,
+                      int64_t n_imag_pol, int64_t n_u, int64_t n_v,
+                      double delta_l, double delta_m, int64_t support,
+                      int64_t oversampling, int accumulate, void *grid,
+                      double *sum_weight);
+
+/*
+ * Briggs/uniform factors from a density grid: replaces calculate_briggs_parms.
+ *   density_grid  device float64 [n_imag_chan, n_imag_pol, n_u, n_v]
+ *   sum_weight    device float64 [n_imag_chan, n_imag_pol]
+ *   factors       HOST float64 out [2, n_imag_chan, n_imag_pol]
+ * Synchronises `stream`.
+ */
+int ngc_briggs_factors(ngc_context *ctx, void *stream, const double *density_grid,
+                       const double *sum_weight, int64_t n_imag_chan,
+                       int64_t n_imag_pol, int64_t n_u, int64_t n_v,
+                       int weighting, double robust, double *factors);
+
+/*
+ * Imaging-weight degridder: replaces _standard_imaging_weight_degrid_jit.
+ *   density_grid    device float64 [n_imag_chan, n_imag_pol, n_u, n_v]
+ *   briggs_factors  host float64 [2, n_imag_chan, n_imag_pol]
+ *   natural_weight  device [T,B,C,P] of weight_dtype
+ *   imaging_weight  device out [T,B,C,P] of out_dtype (NGC_F32|NGC_F64)
+ */
+int ngc_imaging_weight_degrid(ngc_context *ctx, void *stream, const double *uvw,
+                              const void *natural_weight, int weight_dtype,
+                              const uint8_t *flag, const double *freq_chan,
+                              const int32_t *chan_map, const int32_t *pol_map,
+                              const double *density_grid,
+                              const double *briggs_factors, int64_t n_time,
+                              int64_t n_baseline, int64_t n_chan, int64_t n_pol,
+                              int64_t n_imag_chan, int64_t n_imag_pol,
+                              int64_t n_u, int64_t n_v, double delta_l,
+                              double delta_m, void *imaging_weight,
+                              int out_dtype);
+
+/*
+ * Aperture (mosaic A-term) gridder: replaces _aperture_grid_jit (modes IMAGE
+ * and PSF) and _aperture_weight_grid_jit (mode NGC_GRID_WEIGHT).
+ *   field            device int32 [T,B] field id per row (<= -1 skips the row)
+ *   field_id         host int32 [n_field]: the gcf's field ids; position = index
+ *                    into phase_gradient
+ *   cf_*_map         host int32 maps [n_baseline], [n_chan], [n_pol]
+ *   conv_kernel      device float64 [n_cfb, n_cfc, n_cfp, k_u, k_v]
+ *                    (CONV_KERNEL, or WEIGHT_CONV_KERNEL for NGC_GRID_WEIGHT)
+ *   support          host int32 [n_cfb, n_cfc, n_cfp, 2]
+ *   phase_gradient   device complex128 [n_field, k_u, k_v]
+ *   grid             device out complex128 [n_imag_chan, n_imag_pol, n_u, n_v]
+ *                    (complex64 for NGC_PREC_FP32_KAHAN)
+ * Returns NGC_ERR_FIELD if a gridded row's field id is not in field_id (the
+ * reference raises IndexError); this check synchronises `stream`.
+ */
+int ngc_aperture_grid(ngc_context *ctx, void *stream, int mode, int precision,
+                      const double *uvw, const void *vis, int vis_dtype,
+                      const void *weight, int weight_dtype, const uint8_t *flag,
+                      const int32_t *field, const int32_t *field_id,
+                      int64_t n_field, const double *freq_chan,
+                      const int32_t *chan_map, const int32_t *pol_map,
+                      const int32_t *cf_baseline_map, const int32_t *cf_chan_map,
+                      const int32_t *cf_pol_map, const double *conv_kernel,
+                      int64_t n_cfb, int64_t n_cfc, int64_t n_cfp, int64_t k_u,
+                      int64_t k_v, const int32_t *support,
+                      const double *phase_gradient, int64_t n_time,
+                      int64_t n_baseline, int64_t n_chan, int64_t n_pol,
+                      int64_t n_imag_chan, int64_t n_imag_pol, int64_t n_u,
+                      int64_t n_v, double delta_l, double delta_m,
+                      int64_t os_u, int64_t os_v, int accumulate, void *grid,
+                      double *sum_weight);
+
+/*
+ * Grid -> image: ifftshift, unnormalised inverse 2-D FFT (rocFFT), fftshift,
+ * crop of the centre n_l x n_m, real part x (n_u*n_v), then division by
+ * sum_weight (0 -> 1) and by corr_l[l]*corr_m[m] when those are given.
+ *   grid        device [n_planes, n_u, n_v] of grid_dtype (C64, C128, F32, F64);
+ *               single-precision grids are transformed in FP32, double in FP64
+ *   sum_weight  device float64 [n_planes] or NULL
+ *   corr_l/m    host float64 [n_l] / [n_m] or NULL (grdsf correction factors)
+ *   image       device float64 out [n_planes, n_l, n_m]
+ */
+int ngc_grid_to_image(ngc_context *ctx, void *stream, const void *grid,
+                      int grid_dtype, int64_t n_planes, int64_t n_u,
+                      int64_t n_v, int64_t n_l, int64_t n_m,
+                      const double *sum_weight, const double *corr_l,
+                      const double *corr_m, double *image);
+
+/*
+ * Mosaic normalisation, 'forward' direction of _normalize
+ * (_imaging_utils/_normalize.py:39-89):
+ *   image = (image / sum_weight') / (sinc_l*sinc_m * ps_corr * norm_pb),
+ *   zero where pb < pb_limit (pb_limit > 0), optional float32 round trip.
+ *   image      device float64 in/out [n_planes, n_l, n_m]
+ *   sum_weight device float64 [n_planes] (0 -> 1)
+ *   ps_corr    device float64 [n_l, n_m] (PS_CORR_IMAGE) or NULL (ones)
+ *   norm_pb    device float64 [n_planes, n_l, n_m]: PB for flat_noise,
+ *              WEIGHT_PB for flat_sky, NULL for 'none'
+ *   pb         device float64 [n_planes, n_l, n_m] for the pb_limit mask, or NULL
+ *   os_u/os_v  GCF oversampling for the sinc correction
+ */
+int ngc_normalize_mosaic(ngc_context *ctx, void *stream, double *image,
+                         const double *sum_weight, const double *ps_corr,
+                         const double *norm_pb, const double *pb,
+                         int64_t n_planes, int64_t n_l, int64_t n_m,
+                         int64_t os_u, int64_t os_v, double pb_limit,
+                         int single_precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NGCASA_GRIDDING_H */
