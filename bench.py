@@ -1,0 +1,221 @@
+"""Benchmark: standard gridding of the C2 workload on one MI355X.
+
+Workload (Briggs robust=0.5 standard gridding, FP32-Kahan vs FP64):
+  50-antenna compact array (1225 baselines), log-normal radii <= 3 km,
+  2000 integrations x 4 s at 100 GHz, 64 channels x 4 pols (XX,XY,YX,YY)
+  complex64, 200 point sources + 5 Gaussian blobs, 10% random flags;
+  2048 x 2048 cube at 0.1", padded grid 2457 x 2457 (256 planes), PS kernel
+  support 7, oversampling 100.
+
+One step = one ngc_standard_grid call (image mode, chosen FP32-Kahan form)
+over all 627.2M visibility samples resident in HBM, including the uv-tile
+binning.  Briggs imaging weights are computed once, on the GPU, before the
+timed region (also timed and reported).  The dirty-image error of the chosen
+form is measured against the FP64 form on the same inputs, per plane.
+
+Prints one JSON line (see the keys below).  --gpus N > 1 is refused: this
+build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(n_vis, n_rows, n_planes, npad):
+    """Image-pass algorithmic HBM bytes: c64 vis + f32 weight + u8 flag per
+    sample, f64 uvw per row, one c64 grid write per cell."""
+    return n_vis * 13 + n_rows * 24 + n_planes * npad * npad * 8
+
+
+def cpu_baseline(cfg, data, npad, delta, cgk, n_chan_sample):
+    """The oracle (FP64 C restatement of the reference loop) timed on this
+    host, one thread, on the first n_chan_sample channels of the workload."""
+    import oracle
+    oracle.build()
+    t0 = time.time()
+    uvw = data["uvw"].cpu().numpy()
+    vis = data["vis"][:, :, :n_chan_sample].cpu().numpy().astype(np.complex128)
+    w = data["imaging_weight"][:, :, :n_chan_sample].cpu().numpy().astype(np.float64)
+    flag = data["flag"][:, :, :n_chan_sample].cpu().numpy().astype(bool)
+    w[flag] = np.nan
+    fr = data["freq"][:n_chan_sample]
+    log(f"cpu baseline: host copy {time.time() - t0:.1f}s")
+    t0 = time.time()
+    grid, sw, used = oracle.standard_grid_chunked(vis, uvw, w, fr, cgk, [npad, npad], delta, 7,
+                                                  100, do_psf=False, time_chunk=uvw.shape[0],
+                                                  n_threads=1)
+    dt = time.time() - t0
+    n_unflagged = int((~flag).sum())
+    del grid
+    return dict(value=round(n_unflagged / dt / 1e6, 3), unit="Mvis/s", cores=int(used),
+                kind="port",
+                sample=f"FP64 C oracle (reference loop order), 1 thread, image mode, first "
+                       f"{n_chan_sample} of {cfg.n_chan} channels ({n_unflagged} unflagged "
+                       f"samples), {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error-check", action="store_true")
+    ap.add_argument("--cpu-chans", type=int, default=6)
+    ap.add_argument("--config", default="C2")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU path); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from cngi_prototype_amd import _native as nat
+    from cngi_prototype_amd import synthetic as syn
+    from cngi_prototype_amd.imaging._imaging_utils._gridding_convolutional_kernels import \
+        cropped_correction_factors, prolate_spheroidal_kernel_1d
+
+    torch.cuda.set_device(0)
+    cfg = syn.CONFIGS[args.config]
+    N = cfg.image_size
+    npad = int(1.2 * N)
+    cell = cfg.cell_arcsec * syn.ARCSEC
+    delta = np.array([-cell, cell])
+    cgk = prolate_spheroidal_kernel_1d(100, 7)
+    C, P = cfg.n_chan, cfg.n_pol
+
+    t0 = time.time()
+    data = syn.device_dataset(cfg)
+    torch.cuda.synchronize()
+    log(f"generated {cfg.n_vis / 1e6:.1f}M samples in HBM in {time.time() - t0:.1f}s")
+    uvw, vis, w, flag = data["uvw"], data["vis"], data["weight"], data["flag"]
+    fr = data["freq"]
+    chan_map = np.arange(C)
+    pol_map = np.arange(P)
+
+    # Briggs 0.5 imaging weights (density grid + factors + degrid), on the GPU
+    torch.cuda.synchronize()
+    tb = time.time()
+    dens, dsw = nat.standard_grid(uvw, w, fr, chan_map, pol_map, np.ones(1), C, [N, N], delta,
+                                  1, 0, nat.GRID_DENSITY, "fp64_tiled", flag=flag)
+    fac = nat.briggs_factors(dens, dsw, "briggs", 0.5)
+    iw = nat.imaging_weight_degrid(uvw, w, dens, fac, fr, chan_map, pol_map, delta, flag=flag)
+    torch.cuda.synchronize()
+    t_briggs = time.time() - tb
+    del dens
+    data["imaging_weight"] = iw
+    log(f"briggs weights: {t_briggs * 1e3:.1f} ms")
+
+    n_rows = cfg.n_time * cfg.n_baseline
+    n_unflagged = int((flag == 0).sum().item())
+    n_planes = C * P
+    grid = torch.empty((C, P, npad, npad), dtype=torch.complex64, device="cuda")
+    sw = torch.empty((C, P), dtype=torch.float64, device="cuda")
+
+    def step():
+        nat.standard_grid(uvw, iw, fr, chan_map, pol_map, cgk, C, [npad, npad], delta, 7, 100,
+                          nat.GRID_IMAGE, "fp32_kahan", vis=vis, flag=flag, grid=grid,
+                          sum_weight=sw)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    nat.profile_enable(True)
+    phases = {}
+    t0 = time.time()
+    for _ in range(args.steps):
+        step()
+        for name, ms in nat.profile_read():
+            phases.setdefault(name, []).append(ms)
+    torch.cuda.synchronize()
+    elapsed = time.time() - t0
+    nat.profile_enable(False)
+    ms_step = elapsed / args.steps * 1e3
+    value = n_unflagged / (elapsed / args.steps) / 1e6
+    phase_ms = {k: float(np.mean(v)) for k, v in phases.items()}
+    log(f"step {ms_step:.2f} ms, phases {phase_ms}")
+
+    abytes = algorithmic_bytes(cfg.n_vis, n_rows, n_planes, npad)
+    t_kernel = phase_ms.get("std_grid_tiled", float("nan")) / 1e3
+    achieved = abytes / t_kernel / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="std_grid_tiled_kernel<IMAGE,4,float>",
+                    algorithmic_bytes=int(abytes), kernel_ms=round(t_kernel * 1e3, 3),
+                    whole_call_gbs=round(abytes / (ms_step / 1e3) / 1e9, 1))
+
+    # error of the chosen form against FP64 on the same inputs, per plane
+    err = None
+    if not args.no_error_check:
+        te = time.time()
+        corr = cropped_correction_factors([npad, npad], [N, N])
+        step()
+        img32 = nat.grid_to_image(grid, [N, N], sum_weight=sw, corr=corr)
+        del grid
+        g64, sw64 = nat.standard_grid(uvw, iw, fr, chan_map, pol_map, cgk, C, [npad, npad],
+                                      delta, 7, 100, nat.GRID_IMAGE, "fp64_tiled", vis=vis,
+                                      flag=flag)
+        img64 = nat.grid_to_image(g64, [N, N], sum_weight=sw64, corr=corr)
+        del g64
+        d = (img32 - img64).reshape(n_planes, -1)
+        r = img64.reshape(n_planes, -1)
+        rel = (torch.linalg.norm(d, dim=1) / torch.linalg.norm(r, dim=1)).cpu().numpy()
+        pk = (d.abs().max(dim=1).values / r.abs().max(dim=1).values).cpu().numpy()
+        err = dict(rel_rms_max=float(rel.max()), rel_rms_median=float(np.median(rel)),
+                   max_abs_over_peak=float(pk.max()), planes=int(n_planes),
+                   sum_weight_rel=float(((sw - sw64).abs() / sw64.abs()).max().item()))
+        del img32, img64
+        torch.cuda.synchronize()
+        log(f"error check {time.time() - te:.1f}s: {err}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(cfg, data, npad, delta, cgk, args.cpu_chans)
+            log(f"cpu baseline {cpu}")
+        except Exception as e:  # report, never fake
+            log(f"cpu baseline failed: {e!r}")
+
+    line = {
+        "metric": "Mvis/s gridded (std+mosaic) at <=1e-5 rel err vs FP64; % of HBM roofline",
+        "value": round(value, 1),
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{cfg.name}: standard gridding, Briggs 0.5 weights, "
+                               f"{cfg.n_ant} antennas x {cfg.n_time} integrations, "
+                               f"{C} chan x {P} pol, {N}^2 image (padded {npad}^2), "
+                               f"support 7, oversampling 100, 10% flags",
+                   "samples": int(cfg.n_vis), "unflagged": n_unflagged,
+                   "planes": int(n_planes), "form": "fp32_kahan (uv-tiled, LDS, Kahan fold)"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "error_vs_fp64": err,
+        "phases_ms": phase_ms,
+        "briggs_ms": round(t_briggs * 1e3, 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

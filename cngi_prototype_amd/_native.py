@@ -26,7 +26,8 @@ WEIGHT_UNIFORM, WEIGHT_BRIGGS = 1, 2
 
 EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_version",
            "ngc_standard_grid", "ngc_briggs_factors", "ngc_imaging_weight_degrid",
-           "ngc_aperture_grid", "ngc_grid_to_image", "ngc_normalize_mosaic")
+           "ngc_aperture_grid", "ngc_grid_to_image", "ngc_normalize_mosaic",
+           "ngc_profile_enable", "ngc_profile_read")
 
 
 class NativeError(RuntimeError):
@@ -60,6 +61,8 @@ _PROTOS = {
                                  _i64, _i64, _i64, _d, _d, _i64, _i64, _i32, _vp, _vp]),
     "ngc_grid_to_image": (_i32, [_vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64, _i64, _vp, _vp,
                                  _vp, _vp]),
+    "ngc_profile_enable": (_i32, [_vp, _i32]),
+    "ngc_profile_read": (_i32, [_vp, ctypes.c_char_p, _i64, _vp, _i64]),
     "ngc_normalize_mosaic": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64,
                                     _i64, _d, _i32]),
 }
@@ -354,3 +357,21 @@ def normalize_mosaic(image, sum_weight, ps_corr, norm_pb, pb, oversampling, pb_l
         float(pb_limit), int(bool(single_precision)))
     _check("ngc_normalize_mosaic", rc)
     return image
+
+
+def profile_enable(enable=True):
+    """Record per-phase HIP-event timings of subsequent calls."""
+    ctx = context()
+    _check("ngc_profile_enable", load_library().ngc_profile_enable(ctx.handle, int(bool(enable))))
+
+
+def profile_read():
+    """Per-phase durations (ms) of the last profiled call: list of (name, ms)."""
+    ctx = context()
+    names = ctypes.create_string_buffer(4096)
+    ms = np.zeros(64, dtype=np.float64)
+    n = load_library().ngc_profile_read(ctx.handle, names, 4096, ms.ctypes.data_as(_vp), 64)
+    if n < 0:
+        _check("ngc_profile_read", n)
+    parts = names.value.decode().split(";") if n > 0 else []
+    return list(zip(parts, ms[:n].tolist()))

@@ -154,11 +154,13 @@ extern "C" int ngc_aperture_grid(
               NGC_ERR_ARG, "mode must be IMAGE, PSF or WEIGHT");
     NGC_CHECK(precision == NGC_PREC_FP64, NGC_ERR_UNSUPPORTED,
               "aperture gridder: only NGC_PREC_FP64 is built");
-    NGC_CHECK(uvw && weight && field && field_id && freq_chan && chan_map && pol_map &&
-                  cf_baseline_map && cf_chan_map && cf_pol_map && conv_kernel && support &&
-                  phase_gradient && grid && sum_weight,
+    const int64_t n_samples = n_time * n_baseline * n_chan;
+    NGC_CHECK(field_id && freq_chan && chan_map && pol_map && cf_baseline_map && cf_chan_map &&
+                  cf_pol_map && conv_kernel && support && phase_gradient && grid && sum_weight,
               NGC_ERR_ARG, "NULL array");
-    NGC_CHECK(mode != NGC_GRID_IMAGE || vis, NGC_ERR_ARG, "image mode needs vis");
+    NGC_CHECK((uvw && weight && field) || n_samples == 0, NGC_ERR_ARG,
+              "NULL uvw, weight or field");
+    NGC_CHECK(mode != NGC_GRID_IMAGE || vis || n_samples == 0, NGC_ERR_ARG, "image mode needs vis");
     NGC_CHECK(n_field >= 1 && n_cfb >= 1 && n_cfc >= 1 && n_cfp >= 1 && k_u >= 1 && k_v >= 1,
               NGC_ERR_ARG, "bad kernel dimensions");
     NGC_CHECK(os_u >= 1 && os_v >= 1, NGC_ERR_ARG, "oversampling must be >= 1");

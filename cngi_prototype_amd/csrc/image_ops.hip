@@ -135,6 +135,8 @@ static int grid_to_image_impl(ngc_context *ctx, hipStream_t s, const void *grid,
     void *buf;
     NGC_TRY(workspace(ctx, "fft_buffer", cells * elem, &buf));
     const int nb = blocks_for(cells);
+    profile_new_call(ctx, s);
+    phase_begin(ctx, s, "ifftshift");
     switch (dt) {
     case NGC_C64:
         ifftshift_kernel<float, float, 2><<<nb, 256, 0, s>>>((const float *)grid, (float *)buf,
@@ -154,6 +156,7 @@ static int grid_to_image_impl(ngc_context *ctx, hipStream_t s, const void *grid,
         break;
     }
     NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
     FftPlan *plan;
     NGC_TRY(get_plan(ctx, n_u, n_v, n_planes, dbl, &plan));
     void *work = nullptr;
@@ -161,8 +164,10 @@ static int grid_to_image_impl(ngc_context *ctx, hipStream_t s, const void *grid,
     rocfft_execution_info_set_stream(plan->info, s);
     if (plan->work_bytes) rocfft_execution_info_set_work_buffer(plan->info, work, plan->work_bytes);
     void *bufs[1] = {buf};
+    phase_begin(ctx, s, "rocfft_inverse");
     rocfft_status st = rocfft_execute(plan->plan, bufs, nullptr, plan->info);
     NGC_CHECK(st == rocfft_status_success, NGC_ERR_FFT, "rocfft_execute failed");
+    phase_end(ctx, s);
     double *d_corr = nullptr;
     if (corr_l) {
         std::vector<double> c(n_l + n_m);
@@ -173,6 +178,7 @@ static int grid_to_image_impl(ngc_context *ctx, hipStream_t s, const void *grid,
         d_corr = (double *)p;
     }
     const int nb2 = blocks_for(n_planes * n_l * n_m);
+    phase_begin(ctx, s, "crop_correct");
     if (dbl)
         crop_correct_kernel<double><<<nb2, 256, 0, s>>>((const double *)buf, n_planes, n_u, n_v,
                                                         n_l, n_m, sum_weight, d_corr, image);
@@ -180,6 +186,7 @@ static int grid_to_image_impl(ngc_context *ctx, hipStream_t s, const void *grid,
         crop_correct_kernel<float><<<nb2, 256, 0, s>>>((const float *)buf, n_planes, n_u, n_v,
                                                        n_l, n_m, sum_weight, d_corr, image);
     NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
     return NGC_OK;
 }
 
@@ -371,9 +378,10 @@ int ngc_imaging_weight_degrid(ngc_context *ctx, void *stream, const double *uvw,
                               int64_t n_v, double delta_l, double delta_m,
                               void *imaging_weight, int out_dtype) {
     NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
-    NGC_CHECK(uvw && natural_weight && density_grid && briggs_factors && imaging_weight &&
-                  freq_chan && chan_map && pol_map,
-              NGC_ERR_ARG, "NULL array");
+    NGC_CHECK(density_grid && briggs_factors && freq_chan && chan_map && pol_map, NGC_ERR_ARG,
+              "NULL array");
+    NGC_CHECK((uvw && natural_weight && imaging_weight) || n_time * n_baseline * n_chan == 0,
+              NGC_ERR_ARG, "NULL uvw, weight or output");
     NGC_CHECK(weight_dtype == NGC_F32 || weight_dtype == NGC_F64, NGC_ERR_ARG, "weight dtype");
     NGC_CHECK(out_dtype == NGC_F32 || out_dtype == NGC_F64, NGC_ERR_ARG, "out dtype");
     for (int64_t c = 0; c < n_chan; c++)

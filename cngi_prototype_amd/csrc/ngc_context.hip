@@ -1,4 +1,6 @@
 // Context management, error reporting and workspace for libngcasa_hip.
+#include <cstring>
+
 #include "ngc_internal.hpp"
 
 namespace ngc {
@@ -55,9 +57,81 @@ int upload(ngc_context *ctx, hipStream_t stream, const char *name,
     return NGC_OK;
 }
 
+static hipEvent_t pool_event(ngc_context *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void profile_new_call(ngc_context *ctx, hipStream_t s) {
+    if (!ctx->profile) return;
+    for (auto &m : ctx->marks) {
+        ctx->event_pool.push_back(m.start);
+        ctx->event_pool.push_back(m.stop);
+    }
+    ctx->marks.clear();
+    ctx->profile_stream = s;
+}
+
+void phase_begin(ngc_context *ctx, hipStream_t s, const char *name) {
+    if (!ctx->profile) return;
+    ngc_profile_mark m;
+    m.name = name;
+    m.start = pool_event(ctx);
+    m.stop = pool_event(ctx);
+    if (m.start) (void)hipEventRecord(m.start, s);
+    ctx->marks.push_back(m);
+}
+
+void phase_end(ngc_context *ctx, hipStream_t s) {
+    if (!ctx->profile || ctx->marks.empty()) return;
+    if (ctx->marks.back().stop) (void)hipEventRecord(ctx->marks.back().stop, s);
+}
+
 }  // namespace ngc
 
 extern "C" {
+
+int ngc_profile_enable(ngc_context *ctx, int enable) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->profile = enable != 0;
+    return NGC_OK;
+}
+
+int ngc_profile_read(ngc_context *ctx, char *names, int64_t names_len, double *ms, int64_t max) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->marks.empty()) {
+        if (names && names_len > 0) names[0] = 0;
+        return 0;
+    }
+    NGC_HIP(hipSetDevice(ctx->device));
+    NGC_HIP(hipStreamSynchronize(ctx->profile_stream));
+    std::string all;
+    int64_t n = 0;
+    for (auto &m : ctx->marks) {
+        if (n >= max) break;
+        float t = 0.f;
+        if (m.start && m.stop) NGC_HIP(hipEventElapsedTime(&t, m.start, m.stop));
+        if (ms) ms[n] = (double)t;
+        if (!all.empty()) all += ";";
+        all += m.name;
+        n++;
+    }
+    if (names && names_len > 0) {
+        size_t k = std::min<size_t>(all.size(), (size_t)names_len - 1);
+        memcpy(names, all.data(), k);
+        names[k] = 0;
+    }
+    return (int)n;
+}
+
 
 const char *ngc_last_error(void) { return ngc::g_last_error.c_str(); }
 
@@ -89,6 +163,11 @@ int ngc_context_destroy(ngc_context *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     ngc::fft_destroy_plans(ctx);
+    for (auto &m : ctx->marks) {
+        if (m.start) (void)hipEventDestroy(m.start);
+        if (m.stop) (void)hipEventDestroy(m.stop);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto &kv : ctx->ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     if (ctx->d_error) (void)hipFree(ctx->d_error);

@@ -66,8 +66,17 @@ struct Buffer {
 
 }  // namespace ngc
 
+struct ngc_profile_mark {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
 struct ngc_context {
     int device = 0;
+    bool profile = false;
+    hipStream_t profile_stream = nullptr;
+    std::vector<ngc_profile_mark> marks;   // phases of the last call
+    std::vector<hipEvent_t> event_pool;
     std::mutex mu;
     hipStream_t last_stream = nullptr;
     bool has_last_stream = false;
@@ -94,6 +103,12 @@ int upload(ngc_context *ctx, hipStream_t stream, const char *name,
            const void *host, size_t bytes, void **dev);
 
 int fft_destroy_plans(ngc_context *ctx);
+
+// Phase timing: begin a new call (clears the previous marks), then bracket
+// phases with phase_begin/phase_end.  No-ops unless profiling is enabled.
+void profile_new_call(ngc_context *ctx, hipStream_t s);
+void phase_begin(ngc_context *ctx, hipStream_t s, const char *name);
+void phase_end(ngc_context *ctx, hipStream_t s);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

@@ -500,8 +500,11 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
               NGC_ERR_ARG, "mode must be IMAGE, PSF or DENSITY");
     NGC_CHECK(precision >= NGC_PREC_FP64 && precision <= NGC_PREC_FP32_KAHAN, NGC_ERR_ARG,
               "unknown precision");
-    NGC_CHECK(a.uvw && a.weight && grid && sum_weight, NGC_ERR_ARG, "NULL array");
-    NGC_CHECK(mode != NGC_GRID_IMAGE || a.vis, NGC_ERR_ARG, "image mode needs vis");
+    const int64_t n_samples = a.n_time * a.n_baseline * a.n_chan;
+    NGC_CHECK(grid && sum_weight, NGC_ERR_ARG, "NULL grid or sum_weight");
+    NGC_CHECK((a.uvw && a.weight) || n_samples == 0, NGC_ERR_ARG, "NULL uvw or weight");
+    NGC_CHECK(mode != NGC_GRID_IMAGE || a.vis || n_samples == 0, NGC_ERR_ARG,
+              "image mode needs vis");
     NGC_CHECK(a.vis_dtype == NGC_C64 || a.vis_dtype == NGC_C128 || mode != NGC_GRID_IMAGE,
               NGC_ERR_ARG, "vis dtype must be C64 or C128");
     NGC_CHECK(a.weight_dtype == NGC_F32 || a.weight_dtype == NGC_F64, NGC_ERR_ARG,
@@ -545,6 +548,7 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     const int64_t n_rc = a.n_time * a.n_baseline * a.n_chan;
 
     const bool tiled = precision != NGC_PREC_FP64 && mode != NGC_GRID_DENSITY;
+    profile_new_call(ctx, s);
     if (!tiled) {
         if (single) {
             // FP32 grid requested for a density grid: the density gridder is
@@ -561,6 +565,7 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
         const size_t shmem = sw_lds ? n_sw * sizeof(double) : 0;
         const int blocks = launch_blocks(n_rc);
         double *g = (double *)grid;
+        phase_begin(ctx, s, "std_grid_plain");
         if (mode == NGC_GRID_IMAGE)
             std_grid_plain_kernel<NGC_GRID_IMAGE><<<blocks, kBlock, shmem, s>>>(
                 a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
@@ -571,6 +576,7 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
             std_grid_plain_kernel<NGC_GRID_DENSITY><<<blocks, kBlock, shmem, s>>>(
                 a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
         NGC_HIP(hipGetLastError());
+        phase_end(ctx, s);
         return NGC_OK;
     }
 
@@ -602,11 +608,14 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     NGC_HIP(hipMemsetAsync(d_counts, 0, (n_bins + 1) * sizeof(unsigned long long), s));
 
     const int blocks = launch_blocks(n_rc);
+    phase_begin(ctx, s, "bin_count");
     if (n_rc > 0) {
         std_bin_kernel<false><<<blocks, kBlock, 0, s>>>(a, g, (unsigned long long *)d_counts,
                                                           nullptr);
         NGC_HIP(hipGetLastError());
     }
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "bin_scan");
     size_t tmp_bytes = 0;
     NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes,
                                              (unsigned long long *)d_counts,
@@ -615,6 +624,7 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp, tmp_bytes,
                                              (unsigned long long *)d_counts,
                                              (unsigned long long *)d_off, n_bins + 1, s));
+    phase_end(ctx, s);
     unsigned long long total = 0;
     NGC_HIP(hipMemcpyAsync(&total, (unsigned long long *)d_off + n_bins, sizeof(total),
                            hipMemcpyDeviceToHost, s));
@@ -623,14 +633,22 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     // cursors start at the offsets
     NGC_HIP(hipMemcpyAsync(d_counts, d_off, (n_bins + 1) * sizeof(unsigned long long),
                            hipMemcpyDeviceToDevice, s));
+    phase_begin(ctx, s, "bin_fill");
     if (n_rc > 0) {
         std_bin_kernel<true><<<blocks, kBlock, 0, s>>>(a, g, (unsigned long long *)d_counts,
                                                          (int4 *)d_ent);
         NGC_HIP(hipGetLastError());
     }
+    phase_end(ctx, s);
     if (!accumulate) NGC_HIP(hipMemsetAsync(sum_weight, 0, n_sw * sizeof(double), s));
 
     dim3 gridDim((unsigned)n_bins);
+    phase_begin(ctx, s, "std_grid_tiled");
+    struct PhaseGuard {
+        ngc_context *c;
+        hipStream_t st;
+        ~PhaseGuard() { phase_end(c, st); }
+    } guard{ctx, s};
     const unsigned long long *off = (const unsigned long long *)d_off;
     const int4 *ent = (const int4 *)d_ent;
     if (single) {

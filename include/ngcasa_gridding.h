@@ -226,6 +226,18 @@ int ngc_normalize_mosaic(ngc_context *ctx, void *stream, double *image,
                          int64_t os_u, int64_t os_v, double pb_limit,
                          int single_precision);
 
+/*
+ * Phase timing (instrumentation; no counterpart in the reference, which has
+ * no timing inside its kernels).  When enabled, each gridding call records
+ * HIP events on its stream around its phases; ngc_profile_read synchronises
+ * that stream and returns up to `max` (name, milliseconds) pairs of the last
+ * call: names are written as a ';'-separated list into `names` (capacity
+ * `names_len`), durations into `ms`.  Returns the number of phases.
+ */
+int ngc_profile_enable(ngc_context *ctx, int enable);
+int ngc_profile_read(ngc_context *ctx, char *names, int64_t names_len, double *ms,
+                     int64_t max);
+
 #ifdef __cplusplus
 }
 #endif

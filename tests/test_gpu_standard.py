@@ -80,7 +80,7 @@ def test_grid_to_image_matches_oracle(gpu, precision):
     corr = cropped_correction_factors([npad, npad], [N, N])
     img = nat.grid_to_image(_dev(g_ref).to(gdt), [N, N], sum_weight=_dev(sw_ref), corr=corr)
     img = np.moveaxis(img.cpu().numpy(), [0, 1], [-2, -1])
-    tol = 1e-12 if precision == "fp64" else 1e-6
+    tol = 1e-12 if precision == "fp64" else 1e-5  # FP32 FFT: north-star image bound
     for ch in range(img.shape[2]):
         for p in range(img.shape[3]):
             assert rel_err(img[:, :, ch, p], img_ref[:, :, ch, p]) < tol
