@@ -76,6 +76,7 @@ def main():
     ap.add_argument("--no-error-check", action="store_true")
     ap.add_argument("--cpu-chans", type=int, default=6)
     ap.add_argument("--config", default="C2")
+    ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); "
@@ -127,7 +128,7 @@ def main():
 
     def step():
         nat.standard_grid(uvw, iw, fr, chan_map, pol_map, cgk, C, [npad, npad], delta, 7, 100,
-                          nat.GRID_IMAGE, "fp32_kahan", vis=vis, flag=flag, grid=grid,
+                          nat.GRID_IMAGE, args.precision, vis=vis, flag=flag, grid=grid,
                           sum_weight=sw)
 
     for _ in range(args.warmup):
@@ -153,7 +154,8 @@ def main():
     achieved = abytes / t_kernel / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="std_grid_tiled_kernel<IMAGE,4,float>",
+                    kernel="std_grid_tiled_kernel<IMAGE,4,float,%s>" % (
+                        "true" if args.precision == "fp32_kahan" else "false"),
                     algorithmic_bytes=int(abytes), kernel_ms=round(t_kernel * 1e3, 3),
                     whole_call_gbs=round(abytes / (ms_step / 1e3) / 1e9, 1))
 
@@ -201,13 +203,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
+        "precision": args.precision,
         "data": "synthetic",
         "config": {"workload": f"{cfg.name}: standard gridding, Briggs 0.5 weights, "
                                f"{cfg.n_ant} antennas x {cfg.n_time} integrations, "
                                f"{C} chan x {P} pol, {N}^2 image (padded {npad}^2), "
                                f"support 7, oversampling 100, 10% flags",
                    "samples": int(cfg.n_vis), "unflagged": n_unflagged,
-                   "planes": int(n_planes), "form": "fp32_kahan (uv-tiled, LDS, Kahan fold)"},
+                   "planes": int(n_planes),
+                   "form": f"{args.precision} (uv-tiled bins, one wavefront per 32x32 tile, "
+                           f"register runs flushed into an LDS subgrid)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "error_vs_fp64": err,

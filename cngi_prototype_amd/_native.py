@@ -19,8 +19,9 @@ NGC_OK = 0
 ERRORS = {-1: "NGC_ERR_ARG", -2: "NGC_ERR_HIP", -3: "NGC_ERR_UNSUPPORTED",
           -4: "NGC_ERR_FIELD", -5: "NGC_ERR_FFT"}
 GRID_IMAGE, GRID_PSF, GRID_DENSITY, GRID_WEIGHT = 0, 1, 2, 3
-PREC_FP64, PREC_FP64_TILED, PREC_FP32_KAHAN = 0, 1, 2
-PRECISIONS = {"fp64": PREC_FP64, "fp64_tiled": PREC_FP64_TILED, "fp32_kahan": PREC_FP32_KAHAN}
+PREC_FP64, PREC_FP64_TILED, PREC_FP32_KAHAN, PREC_FP32 = 0, 1, 2, 3
+PRECISIONS = {"fp64": PREC_FP64, "fp64_tiled": PREC_FP64_TILED, "fp32_kahan": PREC_FP32_KAHAN,
+              "fp32": PREC_FP32}
 C64, C128, F32, F64 = 0, 1, 2, 3
 WEIGHT_UNIFORM, WEIGHT_BRIGGS = 1, 2
 
@@ -193,7 +194,7 @@ def standard_grid(uvw, weight, freq_chan, chan_map, pol_map, cgk_1d, n_imag_chan
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     n_imag_pol = n_pol if n_imag_pol is None else int(n_imag_pol)
     n_u, n_v = int(n_uv[0]), int(n_uv[1])
-    single = prec == PREC_FP32_KAHAN
+    single = prec in (PREC_FP32_KAHAN, PREC_FP32)
     if mode == GRID_IMAGE:
         gdt = torch.complex64 if single else torch.complex128
         _dev(vis, "vis")

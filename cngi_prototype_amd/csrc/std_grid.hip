@@ -19,6 +19,9 @@
 //    sum_weight stay FP64.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "ngc_internal.hpp"
 
 namespace ngc {
@@ -161,25 +164,45 @@ std_grid_plain_kernel(StdArgs a, const double *cgk, int64_t cgk_len, double *gri
 }
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
 // binning: deterministic counting sort of samples into (image chan, tile) bins
 // ---------------------------------------------------------------------------
-// Samples are enumerated channel-major, then baseline, then time, so one
-// wavefront usually covers 64 consecutive integrations of one baseline track.
+// A wavefront covers a block of (t_lanes integrations) x (ch_lanes channels)
+// of one baseline: for each integration the ch_lanes channels are adjacent in
+// memory, so the visibility and weight reads are coalesced, and the
+// integrations of one channel land in the same tile, so the wave's entries
+// reach each bin as a contiguous run in time order.
 struct BinGeom {
-    int64_t ntu, ntv;  // tiles per axis
+    int64_t ntu, ntv;      // tiles per axis
+    int ch_log2;           // log2(ch_lanes)
+    int64_t n_tb, n_cb;    // time blocks, channel blocks
 };
 
-__device__ inline bool sample_of(const StdArgs &a, int64_t i, int64_t &rc, Pix &px,
-                                 int64_t &a_chan) {
-    const int64_t bt = a.n_baseline * a.n_time;
-    const int64_t ch = i / bt;
-    const int64_t rem = i - ch * bt;
-    const int64_t b = rem / a.n_time;
-    const int64_t t = rem - b * a.n_time;
-    const int64_t row = t * a.n_baseline + b;
-    rc = row * a.n_chan + ch;
-    if (!pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
-                        a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os, px))
+struct Sample {
+    int64_t t, b, ch, rc;  // rc = (t*n_baseline + b)*n_chan + ch
+};
+
+__device__ inline bool sample_at(const StdArgs &a, const BinGeom &g, int64_t wave_id, int lane,
+                                 Sample &sm) {
+    const int chl = 1 << g.ch_log2;
+    const int tl = 64 >> g.ch_log2;
+    const int64_t cb = wave_id % g.n_cb;
+    const int64_t r = wave_id / g.n_cb;
+    const int64_t tb = r % g.n_tb;
+    const int64_t b = r / g.n_tb;
+    sm.t = tb * tl + (lane >> g.ch_log2);
+    sm.ch = cb * chl + (lane & (chl - 1));
+    sm.b = b;
+    if (b >= a.n_baseline || sm.t >= a.n_time || sm.ch >= a.n_chan) return false;
+    sm.rc = (sm.t * a.n_baseline + b) * a.n_chan + sm.ch;
+    return true;
+}
+
+// centre cell, offsets and on-grid test of a sample; false when it is not gridded
+__device__ inline bool place(const StdArgs &a, const Sample &sm, Pix &px) {
+    const int64_t row = sm.t * a.n_baseline + sm.b;
+    if (!pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[sm.ch],
+                        a.uv_scale[a.n_chan + sm.ch], a.n_u, a.n_v, a.os, px))
         return false;
     const int64_t sc = a.support / 2;
     if (!((px.uc + sc < a.n_u) && (px.vc + sc < a.n_v) && (px.uc - sc >= 0) &&
@@ -187,14 +210,13 @@ __device__ inline bool sample_of(const StdArgs &a, int64_t i, int64_t &rc, Pix &
         return false;
     if (a.flag) {
         bool any = false;
-        for (int p = 0; p < a.n_pol; p++) any |= (a.flag[rc * a.n_pol + p] == 0);
+        for (int p = 0; p < a.n_pol; p++) any |= (a.flag[sm.rc * a.n_pol + p] == 0);
         if (!any) return false;
     }
-    a_chan = a.chan_map[ch];
     return true;
 }
 
-// k-th overlapped tile (k = 0..3) of a valid sample, or -1
+// k-th overlapped tile (k = 0..3) of a placed sample, or -1
 __device__ inline int64_t tile_key(const StdArgs &a, const BinGeom &g, const Pix &px,
                                    int64_t a_chan, int k) {
     const int64_t sc = a.support / 2;
@@ -205,18 +227,75 @@ __device__ inline int64_t tile_key(const StdArgs &a, const BinGeom &g, const Pix
     return (a_chan * g.ntu + tu) * g.ntv + tv;
 }
 
-template <bool FILL>
+// Entry record of one sample in one bin: packed centre cell and offsets, then
+// the weighted visibilities of every pol (ACC precision), padded to 16 B.
+template <int NP, int NC, typename ACC> struct Entry {
+    int uv;      // u0 | v0 << 16
+    int duv;     // du & 0xffff | dv << 16
+    ACC wd[NP][NC];
+};
+template <int NP, int NC, typename ACC> struct EntrySize {
+    static constexpr int raw = (int)sizeof(Entry<NP, NC, ACC>);
+    static constexpr int bytes = (raw + 15) / 16 * 16;
+};
+
+// Count pass (FILL = false) and fill pass (FILL = true).  The fill pass also
+// accumulates sum_weight in FP64: sel_weight * norm(du, dv) per valid pol,
+// the reference's per-sample term (_standard_grid.py:366).
+template <bool FILL, int MODE, int NP, typename ACC>
 __global__ void __launch_bounds__(kBlock)
-std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, int4 *entries) {
-    const int64_t n = a.n_time * a.n_baseline * a.n_chan;
+std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
+               const double *norm_tab, int norm_half, double *sum_weight, int sw_in_lds) {
+    constexpr int NC = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
+    using E = Entry<NP, NC, ACC>;
+    constexpr int EB = EntrySize<NP, NC, ACC>::bytes;
+    extern __shared__ double s_sw[];
+    const int64_t n_sw = a.n_imag_chan * a.n_imag_pol;
+    if (FILL && sw_in_lds) {
+        for (int64_t i = threadIdx.x; i < n_sw; i += blockDim.x) s_sw[i] = 0.0;
+        __syncthreads();
+    }
     const int lane = threadIdx.x & 63;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n;
-         i0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = i0 + threadIdx.x;
-        int64_t rc = 0, a_chan = 0;
+    const int64_t n_waves = a.n_baseline * g.n_tb * g.n_cb;
+    const int waves_per_block = blockDim.x >> 6;
+    const int nw = 2 * norm_half + 1;
+    for (int64_t w0 = (int64_t)blockIdx.x * waves_per_block; w0 < n_waves;
+         w0 += (int64_t)gridDim.x * waves_per_block) {
+        const int64_t wave_id = w0 + (threadIdx.x >> 6);
+        Sample sm{};
         Pix px{};
-        const bool ok = (i < n) && sample_of(a, i, rc, px, a_chan);
+        const bool ok = (wave_id < n_waves) && sample_at(a, g, wave_id, lane, sm) && place(a, sm, px);
+        const int64_t a_chan = ok ? a.chan_map[sm.ch] : 0;
+        E rec;
+        if (FILL && ok) {
+            rec.uv = (int)((px.uc & 0xffff) | (px.vc << 16));
+            rec.duv = (int)((px.du & 0xffff) | ((px.dv & 0xffff) << 16));
+            const int ndu = min(max((int)px.du + norm_half, 0), nw - 1);
+            const int ndv = min(max((int)px.dv + norm_half, 0), nw - 1);
+            const double norm = norm_tab[ndu * nw + ndv];
+            const int64_t base = sm.rc * NP;
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                const double wv = weight_of(a, base, p);
+                double wr = wv, wi = 0.0;
+                if (MODE == NGC_GRID_IMAGE) {
+                    const double2 v = load_vis(a.vis, base + p, a.vis_dtype);
+                    wr = v.x * wv;
+                    wi = v.y * wv;
+                }
+                const bool use = !(isnan(wr) || isnan(wi)) && !(wr == 0.0 && wi == 0.0);
+                rec.wd[p][0] = use ? (ACC)wr : (ACC)0;
+                if (NC == 2) rec.wd[p][NC - 1] = use ? (ACC)wi : (ACC)0;
+                if (use) {
+                    const int64_t swi = a_chan * a.n_imag_pol + p;
+                    if (sw_in_lds)
+                        atomicAdd(&s_sw[swi], wv * norm);
+                    else
+                        atomicAdd(&sum_weight[swi], wv * norm);
+                }
+            }
+        }
         for (int k = 0; k < 4; k++) {
             int64_t key = ok ? tile_key(a, g, px, a_chan, k) : -1;
             uint64_t pending = __ballot(key >= 0);
@@ -231,14 +310,7 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, int4 *entries) 
                     basev = __shfl(basev, leader, 64);
                     if (key == lkey) {
                         const unsigned long long pos = basev + __popcll(same & below);
-                        const int64_t hu = px.uc / kTile, hv = px.vc / kTile;
-                        const int64_t hkey = (a_chan * g.ntu + hu) * g.ntv + hv;
-                        int4 e;
-                        e.x = (int)rc;
-                        e.y = (int)((px.uc & 0xffff) | (px.vc << 16));
-                        e.z = (int)((px.du & 0xffff) | ((px.dv & 0xffff) << 16));
-                        e.w = (hkey == lkey) ? 1 : 0;
-                        entries[pos] = e;
+                        *reinterpret_cast<E *>(entries + pos * EB) = rec;
                     }
                 }
                 if (key == lkey) key = -1;
@@ -246,182 +318,180 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, int4 *entries) 
             }
         }
     }
+    if (FILL && sw_in_lds) {
+        __syncthreads();
+        for (int64_t i = threadIdx.x; i < n_sw; i += blockDim.x)
+            if (s_sw[i] != 0.0) atomicAdd(&sum_weight[i], s_sw[i]);
+    }
 }
 
 // ---------------------------------------------------------------------------
-// tiled gridding kernel: one workgroup per (image channel, tile) bin
+// tiled gridding kernel: one wavefront per (image channel, 32x32 tile) bin
 // ---------------------------------------------------------------------------
-template <typename ACC> struct AccTraits;
-template <> struct AccTraits<float> {
-    static constexpr bool kKahan = true;
-    static constexpr int kPad = 1;
-};
-template <> struct AccTraits<double> {
-    static constexpr bool kKahan = false;
-    static constexpr int kPad = 1;
-};
-
+// Work distribution after J. W. Romein, "An efficient work-distribution
+// strategy for gridding radio-telescope data on GPUs" (ICS 2012), applied to
+// an LDS subgrid: lane (a, b) of the wavefront owns every tile cell with
+// u = a (mod 8) and v = b (mod 8).  A support <= 8 window covers each residue
+// pair at most once, so for every sample each lane has at most one tap, and
+// no two lanes ever touch the same cell.  A lane keeps the running sum of its
+// current cell in registers and writes it to the LDS subgrid (plain
+// read-modify-write, no atomics) only when its cell changes; consecutive
+// integrations of one baseline move slowly across the grid, so most taps
+// never leave the register file.  With KAHAN the LDS subgrid carries a
+// Kahan-Neumaier compensation word per value and every flush is a
+// compensated add.  The tile is written to HBM once at the end.
 template <int NP, int NC, typename ACC> struct Stage {
-    int u0, v0, du, dv;
+    int lu0, lv0;          // tap window origin relative to the tile
     ACC wd[NP][NC];
+    ACC ku[8], kv[8];
 };
 
-template <int MODE, int NP, typename ACC>
-__global__ void __launch_bounds__(kBlock)
+constexpr int kStage = 32;      // entries staged per round
+constexpr int kRow = kTile + 1; // LDS row stride (words)
+
+template <int MODE, int NP, typename ACC, bool KAHAN>
+__global__ void __launch_bounds__(64, 4)
 std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *offsets,
-                      const int4 *entries, const double *cgk, int cgk_len,
-                      const double *norm_tab, int norm_half, void *grid_out,
-                      double *sum_weight, int accumulate) {
+                      const char *entries, const double *cgk, int cgk_len, void *grid_out,
+                      int accumulate) {
     constexpr int NC = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
-    constexpr int RS = kTile * NC + AccTraits<ACC>::kPad;  // LDS row stride
-    constexpr int F = NP * kTile * kTile * NC;              // accumulators per tile
-    constexpr int PER = F / kBlock;                          // owned per thread
-    static_assert(F % kBlock == 0, "tile must split evenly over the workgroup");
+    constexpr int PLANE = kTile * kRow;
+    constexpr int NPC = NP * NC;
+    using E = Entry<NP, NC, ACC>;
+    constexpr int EB = EntrySize<NP, NC, ACC>::bytes;
 
-    __shared__ ACC s_acc[NP * kTile * RS];
-    __shared__ Stage<NP, NC, ACC> s_stage[kBlock / 64][64];
+    __shared__ ACC s_acc[NPC * PLANE];
+    __shared__ ACC s_cmp[KAHAN ? NPC * PLANE : 1];
+    __shared__ Stage<NP, NC, ACC> s_stage[kStage];
     __shared__ ACC s_k[512];
-    __shared__ double s_sw[kBlock / 64][NP];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lane = threadIdx.x;
     const int64_t bin = blockIdx.x;
     const int64_t tiles = g.ntu * g.ntv;
     const int64_t a_chan = bin / tiles;
     const int64_t trem = bin - a_chan * tiles;
-    const int64_t u_lo = (trem / g.ntv) * kTile;
-    const int64_t v_lo = (trem % g.ntv) * kTile;
+    const int u_lo = (int)((trem / g.ntv) * kTile);
+    const int v_lo = (int)((trem % g.ntv) * kTile);
+    const int64_t start = (int64_t)offsets[bin], end = (int64_t)offsets[bin + 1];
 
-    for (int i = tid; i < NP * kTile * RS; i += kBlock) s_acc[i] = (ACC)0;
-    for (int i = tid; i < cgk_len && i < 512; i += kBlock) s_k[i] = (ACC)cgk[i];
-
-    ACC ksum[PER], kcomp[PER];
-#pragma unroll
-    for (int j = 0; j < PER; j++) { ksum[j] = 0; kcomp[j] = 0; }
-    double swp[NP];
-#pragma unroll
-    for (int p = 0; p < NP; p++) swp[p] = 0.0;
+#pragma unroll 4
+    for (int i = lane; i < NPC * PLANE; i += 64) {
+        s_acc[i] = (ACC)0;
+        if (KAHAN) s_cmp[i] = (ACC)0;
+    }
+    if (end > start)
+        for (int i = lane; i < 512; i += 64) s_k[i] = i < cgk_len ? (ACC)cgk[i] : (ACC)0;
     __syncthreads();
 
     const int sc = (int)(a.support / 2);
     const int S = (int)a.support;
     const int os = (int)a.os;
-    const int ta = lane & 7, tb = lane >> 3;
-    const bool tap_on = (ta < S) && (tb < S);
-    const int iu = ta - sc, iv = tb - sc;
-    const int nw = 2 * norm_half + 1;
+    const int ra = lane & 7, rb = lane >> 3;  // owned residues
 
-    const int64_t start = (int64_t)offsets[bin], end = (int64_t)offsets[bin + 1];
-    for (int64_t b0 = start; b0 < end; b0 += kBlock) {
-        const int64_t e = b0 + wave * 64 + lane;
-        if (e < end) {
-            const int4 en = entries[e];
-            const int64_t rc = (int64_t)(unsigned)en.x;
-            Stage<NP, NC, ACC> st;
-            st.u0 = en.y & 0xffff;
-            st.v0 = (en.y >> 16) & 0xffff;
-            st.du = (int)(short)(en.z & 0xffff);
-            st.dv = (int)(short)((en.z >> 16) & 0xffff);
-            const bool home = en.w & 1;
-            const int64_t base = rc * NP;
-            const int ndu = min(max(st.du + norm_half, 0), nw - 1);
-            const int ndv = min(max(st.dv + norm_half, 0), nw - 1);
-            const double norm = home ? norm_tab[ndu * nw + ndv] : 0.0;
+    int cur_u = -64, cur_v = -64;             // current cell (tile-relative)
+    ACC acc[NP][NC];
 #pragma unroll
-            for (int p = 0; p < NP; p++) {
-                const double w = weight_of(a, base, p);
-                double wr = w, wi = 0.0;
-                if (MODE == NGC_GRID_IMAGE) {
-                    const double2 v = load_vis(a.vis, base + p, a.vis_dtype);
-                    wr = v.x * w;
-                    wi = v.y * w;
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[p][c] = (ACC)0;
+
+    auto flush = [&]() {
+        if ((unsigned)cur_u < (unsigned)kTile && (unsigned)cur_v < (unsigned)kTile) {
+            const int o = cur_u * kRow + cur_v;
+#pragma unroll
+            for (int p = 0; p < NP; p++)
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    ACC *dst = &s_acc[(p * NC + c) * PLANE + o];
+                    const ACC x = acc[p][c];
+                    if (KAHAN) {
+                        const ACC s = *dst;
+                        const ACC t = s + x;
+                        ACC *cp = &s_cmp[(p * NC + c) * PLANE + o];
+                        if (fabs(s) >= fabs(x))
+                            *cp += (s - t) + x;
+                        else
+                            *cp += (x - t) + s;
+                        *dst = t;
+                    } else {
+                        *dst = *dst + x;
+                    }
                 }
-                const bool use = !(isnan(wr) || isnan(wi)) && !(wr == 0.0 && wi == 0.0);
-                st.wd[p][0] = use ? (ACC)wr : (ACC)0;
-                if (NC == 2) st.wd[p][NC - 1] = use ? (ACC)wi : (ACC)0;
-                if (use && home) swp[p] += w * norm;
+        }
+    };
+
+    for (int64_t b0 = start; b0 < end; b0 += kStage) {
+        const int64_t e = b0 + lane;
+        if (lane < kStage && e < end) {
+            const E en = *reinterpret_cast<const E *>(entries + e * EB);
+            Stage<NP, NC, ACC> st;
+            const int u0 = en.uv & 0xffff, v0 = (en.uv >> 16) & 0xffff;
+            const int du = (int)(short)(en.duv & 0xffff);
+            const int dv = (int)(short)((en.duv >> 16) & 0xffff);
+            st.lu0 = u0 - sc - u_lo;
+            st.lv0 = v0 - sc - v_lo;
+#pragma unroll
+            for (int p = 0; p < NP; p++)
+#pragma unroll
+                for (int c = 0; c < NC; c++) st.wd[p][c] = en.wd[p][c];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int ku = abs(os * (i - sc) + du);
+                const int kv = abs(os * (i - sc) + dv);
+                st.ku[i] = (i < S && ku < cgk_len) ? s_k[ku] : (ACC)0;
+                st.kv[i] = (i < S && kv < cgk_len) ? s_k[kv] : (ACC)0;
             }
-            s_stage[wave][lane] = st;
+            s_stage[lane] = st;
         }
         __syncthreads();
-        const int64_t nh64 = end - (b0 + wave * 64);
-        const int nh = nh64 > 64 ? 64 : (nh64 < 0 ? 0 : (int)nh64);
+        const int64_t nh64 = end - b0;
+        const int nh = nh64 > kStage ? kStage : (int)nh64;
         for (int j = 0; j < nh; j++) {
-            const Stage<NP, NC, ACC> st = s_stage[wave][j];
-            if (tap_on) {
-                const int lu = st.u0 + iu - (int)u_lo;
-                const int lv = st.v0 + iv - (int)v_lo;
-                if ((unsigned)lu < (unsigned)kTile && (unsigned)lv < (unsigned)kTile) {
-                    const int ku = abs(os * iu + st.du);
-                    const int kv = abs(os * iv + st.dv);
-                    const ACC kw = (ku < cgk_len ? s_k[ku] : (ACC)0) * (kv < cgk_len ? s_k[kv] : (ACC)0);
-                    ACC *cell = &s_acc[lu * RS + lv * NC];
+            const Stage<NP, NC, ACC> &st = s_stage[j];
+            const int iu = (ra - st.lu0) & 7;
+            const int iv = (rb - st.lv0) & 7;
+            if (iu < S && iv < S) {
+                const int u = st.lu0 + iu, v = st.lv0 + iv;
+                if (u != cur_u || v != cur_v) {
+                    flush();
+                    cur_u = u;
+                    cur_v = v;
 #pragma unroll
                     for (int p = 0; p < NP; p++)
 #pragma unroll
-                        for (int c = 0; c < NC; c++)
-                            atomicAdd(cell + p * kTile * RS + c, kw * st.wd[p][c]);
+                        for (int c = 0; c < NC; c++) acc[p][c] = (ACC)0;
                 }
+                const ACC w = st.ku[iu] * st.kv[iv];
+#pragma unroll
+                for (int p = 0; p < NP; p++)
+#pragma unroll
+                    for (int c = 0; c < NC; c++) acc[p][c] += w * st.wd[p][c];
             }
         }
         __syncthreads();
-        if (AccTraits<ACC>::kKahan) {
-            // fold the batch's LDS partial sums into compensated registers
-#pragma unroll
-            for (int j = 0; j < PER; j++) {
-                const int k = tid + j * kBlock;
-                const int c = k % NC;
-                const int lv = (k / NC) % kTile;
-                const int lu = (k / (NC * kTile)) % kTile;
-                const int p = k / (NC * kTile * kTile);
-                ACC *src = &s_acc[(p * kTile + lu) * RS + lv * NC + c];
-                const ACC x = *src;
-                *src = (ACC)0;
-                const ACC s = ksum[j];
-                const ACC t = s + x;
-                if (fabs(s) >= fabs(x))
-                    kcomp[j] += (s - t) + x;
-                else
-                    kcomp[j] += (x - t) + s;
-                ksum[j] = t;
-            }
-            __syncthreads();
-        }
     }
+    flush();
+    __syncthreads();
 
     // write the tile: every cell of the grid is written exactly once
-#pragma unroll
-    for (int j = 0; j < PER; j++) {
-        const int k = tid + j * kBlock;
+    constexpr int F = NPC * kTile * kTile;
+#pragma unroll 4
+    for (int k = lane; k < F; k += 64) {
         const int c = k % NC;
         const int lv = (k / NC) % kTile;
         const int lu = (k / (NC * kTile)) % kTile;
         const int p = k / (NC * kTile * kTile);
         const int64_t gu = u_lo + lu, gv = v_lo + lv;
         if (gu >= a.n_u || gv >= a.n_v) continue;
-        ACC val;
-        if (AccTraits<ACC>::kKahan)
-            val = ksum[j] + kcomp[j];
-        else
-            val = s_acc[(p * kTile + lu) * RS + lv * NC + c];
+        const int o = (p * NC + c) * PLANE + lu * kRow + lv;
+        ACC val = s_acc[o];
+        if (KAHAN) val = val + s_cmp[o];
         const int64_t gi = (((a_chan * a.n_imag_pol + p) * a.n_u + gu) * a.n_v + gv) * NC + c;
         ACC *gp = reinterpret_cast<ACC *>(grid_out) + gi;
         if (accumulate)
             *gp = *gp + val;
         else
             *gp = val;
-    }
-
-    // sum_weight: FP64 partials of the samples whose home tile this is
-#pragma unroll
-    for (int p = 0; p < NP; p++) {
-        const double s = wave_sum(swp[p]);
-        if (lane == 0) s_sw[wave][p] = s;
-    }
-    __syncthreads();
-    if (tid < NP) {
-        double s = 0.0;
-        for (int w = 0; w < kBlock / 64; w++) s += s_sw[w][tid];
-        if (s != 0.0) atomicAdd(&sum_weight[a_chan * a.n_imag_pol + tid], s);
     }
 }
 
@@ -460,37 +530,6 @@ static void norm_table(const double *cgk, int64_t cgk_len, int64_t support, int6
         }
 }
 
-template <int MODE, typename ACC>
-static int launch_tiled(int64_t np, dim3 grid, hipStream_t s, const StdArgs &a,
-                        const BinGeom &g, const unsigned long long *off,
-                        const int4 *ent, const double *cgk, int cgk_len,
-                        const double *ntab, int half, void *out, double *sw,
-                        int accumulate) {
-    switch (np) {
-    case 1:
-        std_grid_tiled_kernel<MODE, 1, ACC><<<grid, kBlock, 0, s>>>(
-            a, g, off, ent, cgk, cgk_len, ntab, half, out, sw, accumulate);
-        break;
-    case 2:
-        std_grid_tiled_kernel<MODE, 2, ACC><<<grid, kBlock, 0, s>>>(
-            a, g, off, ent, cgk, cgk_len, ntab, half, out, sw, accumulate);
-        break;
-    case 3:
-        std_grid_tiled_kernel<MODE, 3, ACC><<<grid, kBlock, 0, s>>>(
-            a, g, off, ent, cgk, cgk_len, ntab, half, out, sw, accumulate);
-        break;
-    case 4:
-        std_grid_tiled_kernel<MODE, 4, ACC><<<grid, kBlock, 0, s>>>(
-            a, g, off, ent, cgk, cgk_len, ntab, half, out, sw, accumulate);
-        break;
-    default:
-        set_error("tiled gridder supports 1..4 polarisations");
-        return NGC_ERR_UNSUPPORTED;
-    }
-    NGC_HIP(hipGetLastError());
-    return NGC_OK;
-}
-
 static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int precision,
                               StdArgs a, const double *freq, const int32_t *chan_map,
                               const int32_t *pol_map, const double *cgk, int64_t cgk_len,
@@ -498,7 +537,7 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
                               double delta_l, double delta_m) {
     NGC_CHECK(mode == NGC_GRID_IMAGE || mode == NGC_GRID_PSF || mode == NGC_GRID_DENSITY,
               NGC_ERR_ARG, "mode must be IMAGE, PSF or DENSITY");
-    NGC_CHECK(precision >= NGC_PREC_FP64 && precision <= NGC_PREC_FP32_KAHAN, NGC_ERR_ARG,
+    NGC_CHECK(precision >= NGC_PREC_FP64 && precision <= NGC_PREC_FP32, NGC_ERR_ARG,
               "unknown precision");
     const int64_t n_samples = a.n_time * a.n_baseline * a.n_chan;
     NGC_CHECK(grid && sum_weight, NGC_ERR_ARG, "NULL grid or sum_weight");
@@ -541,7 +580,8 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     a.chan_map = (const int *)d_cmap;
     a.pol_map = (const int *)d_pmap;
 
-    const bool single = (precision == NGC_PREC_FP32_KAHAN);
+    const bool single = (precision == NGC_PREC_FP32_KAHAN || precision == NGC_PREC_FP32);
+    const bool compensated = (precision == NGC_PREC_FP32_KAHAN);
     const int64_t cells = a.n_imag_chan * a.n_imag_pol * a.n_u * a.n_v;
     const int64_t nc = (mode == NGC_GRID_IMAGE) ? 2 : 1;
     const int64_t n_sw = a.n_imag_chan * a.n_imag_pol;
@@ -586,7 +626,6 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     NGC_CHECK(pol_identity, NGC_ERR_UNSUPPORTED, "tiled gridder needs pol_map = arange(n_pol)");
     NGC_CHECK(a.n_pol <= 4, NGC_ERR_UNSUPPORTED, "tiled gridder supports up to 4 pols");
     NGC_CHECK(a.n_u < 65536 && a.n_v < 65536, NGC_ERR_UNSUPPORTED, "grid axis >= 65536");
-    NGC_CHECK(n_rc < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED, "more than 2^31 samples per call");
     NGC_CHECK(a.os < 30000, NGC_ERR_UNSUPPORTED, "oversampling too large");
     NGC_CHECK(cgk_len <= 512, NGC_ERR_UNSUPPORTED, "cgk_1d longer than 512");
 
@@ -599,78 +638,97 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     BinGeom g;
     g.ntu = ceil_div(a.n_u, kTile);
     g.ntv = ceil_div(a.n_v, kTile);
+    // two channels per integration and 32 integrations per wavefront: 64-byte
+    // contiguous visibility reads, and runs of 32 integrations per bin
+    int ch_log2 = 0;
+    while ((1 << (ch_log2 + 1)) <= std::min<int64_t>(a.n_chan, 2)) ch_log2++;
+    g.ch_log2 = ch_log2;
+    g.n_cb = ceil_div(a.n_chan, int64_t(1) << ch_log2);
+    g.n_tb = ceil_div(a.n_time, int64_t(64) >> ch_log2);
     const int64_t n_bins = a.n_imag_chan * g.ntu * g.ntv;
     NGC_CHECK(n_bins < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED, "too many tiles");
+    const int64_t n_waves = a.n_baseline * g.n_tb * g.n_cb;
+    int64_t bin_blocks = ceil_div(n_waves, kBlock / 64);
+    if (bin_blocks > (1 << 20)) bin_blocks = 1 << 20;
+    if (bin_blocks < 1) bin_blocks = 1;
 
     void *d_counts, *d_off, *d_scan_tmp, *d_ent;
     NGC_TRY(workspace(ctx, "bin_counts", (n_bins + 1) * sizeof(unsigned long long), &d_counts));
     NGC_TRY(workspace(ctx, "bin_offsets", (n_bins + 1) * sizeof(unsigned long long), &d_off));
     NGC_HIP(hipMemsetAsync(d_counts, 0, (n_bins + 1) * sizeof(unsigned long long), s));
-
-    const int blocks = launch_blocks(n_rc);
-    phase_begin(ctx, s, "bin_count");
-    if (n_rc > 0) {
-        std_bin_kernel<false><<<blocks, kBlock, 0, s>>>(a, g, (unsigned long long *)d_counts,
-                                                          nullptr);
-        NGC_HIP(hipGetLastError());
-    }
-    phase_end(ctx, s);
-    phase_begin(ctx, s, "bin_scan");
-    size_t tmp_bytes = 0;
-    NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes,
-                                             (unsigned long long *)d_counts,
-                                             (unsigned long long *)d_off, n_bins + 1, s));
-    NGC_TRY(workspace(ctx, "scan_tmp", tmp_bytes, &d_scan_tmp));
-    NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp, tmp_bytes,
-                                             (unsigned long long *)d_counts,
-                                             (unsigned long long *)d_off, n_bins + 1, s));
-    phase_end(ctx, s);
-    unsigned long long total = 0;
-    NGC_HIP(hipMemcpyAsync(&total, (unsigned long long *)d_off + n_bins, sizeof(total),
-                           hipMemcpyDeviceToHost, s));
-    NGC_HIP(hipStreamSynchronize(s));
-    NGC_TRY(workspace(ctx, "bin_entries", (size_t)(total + 1) * sizeof(int4), &d_ent));
-    // cursors start at the offsets
-    NGC_HIP(hipMemcpyAsync(d_counts, d_off, (n_bins + 1) * sizeof(unsigned long long),
-                           hipMemcpyDeviceToDevice, s));
-    phase_begin(ctx, s, "bin_fill");
-    if (n_rc > 0) {
-        std_bin_kernel<true><<<blocks, kBlock, 0, s>>>(a, g, (unsigned long long *)d_counts,
-                                                         (int4 *)d_ent);
-        NGC_HIP(hipGetLastError());
-    }
-    phase_end(ctx, s);
     if (!accumulate) NGC_HIP(hipMemsetAsync(sum_weight, 0, n_sw * sizeof(double), s));
+    const int sw_lds = n_sw <= 4096 ? 1 : 0;
+    const size_t sw_shmem = sw_lds ? n_sw * sizeof(double) : 0;
 
-    dim3 gridDim((unsigned)n_bins);
-    phase_begin(ctx, s, "std_grid_tiled");
-    struct PhaseGuard {
-        ngc_context *c;
-        hipStream_t st;
-        ~PhaseGuard() { phase_end(c, st); }
-    } guard{ctx, s};
-    const unsigned long long *off = (const unsigned long long *)d_off;
-    const int4 *ent = (const int4 *)d_ent;
-    if (single) {
-        if (mode == NGC_GRID_IMAGE)
-            return launch_tiled<NGC_GRID_IMAGE, float>(a.n_pol, gridDim, s, a, g, off, ent,
-                                                       (const double *)d_cgk, (int)cgk_len,
-                                                       (const double *)d_ntab, half, grid,
-                                                       sum_weight, accumulate);
-        return launch_tiled<NGC_GRID_PSF, float>(a.n_pol, gridDim, s, a, g, off, ent,
-                                                 (const double *)d_cgk, (int)cgk_len,
-                                                 (const double *)d_ntab, half, grid,
-                                                 sum_weight, accumulate);
-    }
+    // one instantiation per (mode, pols, precision)
+    int rc = NGC_OK;
+    auto run = [&](auto mode_c, auto np_c, auto acc_c) -> int {
+        constexpr int MODE = decltype(mode_c)::value;
+        constexpr int NP = decltype(np_c)::value;
+        using ACC = typename decltype(acc_c)::type;
+        constexpr int NC = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
+        constexpr int EB = EntrySize<NP, NC, ACC>::bytes;
+        unsigned long long *counts = (unsigned long long *)d_counts;
+        phase_begin(ctx, s, "bin_count");
+        if (n_waves > 0) {
+            std_bin_kernel<false, MODE, NP, ACC><<<(int)bin_blocks, kBlock, 0, s>>>(
+                a, g, counts, nullptr, (const double *)d_ntab, half, sum_weight, 0);
+            NGC_HIP(hipGetLastError());
+        }
+        phase_end(ctx, s);
+        phase_begin(ctx, s, "bin_scan");
+        size_t tmp_bytes = 0;
+        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts,
+                                                 (unsigned long long *)d_off, n_bins + 1, s));
+        NGC_TRY(workspace(ctx, "scan_tmp", tmp_bytes, &d_scan_tmp));
+        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp, tmp_bytes, counts,
+                                                 (unsigned long long *)d_off, n_bins + 1, s));
+        phase_end(ctx, s);
+        unsigned long long total = 0;
+        NGC_HIP(hipMemcpyAsync(&total, (unsigned long long *)d_off + n_bins, sizeof(total),
+                               hipMemcpyDeviceToHost, s));
+        NGC_HIP(hipStreamSynchronize(s));
+        NGC_TRY(workspace(ctx, "bin_entries", (size_t)(total + 1) * EB, &d_ent));
+        // cursors start at the offsets
+        NGC_HIP(hipMemcpyAsync(d_counts, d_off, (n_bins + 1) * sizeof(unsigned long long),
+                               hipMemcpyDeviceToDevice, s));
+        phase_begin(ctx, s, "bin_fill");
+        if (n_waves > 0) {
+            std_bin_kernel<true, MODE, NP, ACC><<<(int)bin_blocks, kBlock, sw_shmem, s>>>(
+                a, g, counts, (char *)d_ent, (const double *)d_ntab, half, sum_weight, sw_lds);
+            NGC_HIP(hipGetLastError());
+        }
+        phase_end(ctx, s);
+        phase_begin(ctx, s, "std_grid_tiled");
+        if (compensated)
+            std_grid_tiled_kernel<MODE, NP, ACC, true><<<(unsigned)n_bins, 64, 0, s>>>(
+                a, g, (const unsigned long long *)d_off, (const char *)d_ent,
+                (const double *)d_cgk, (int)cgk_len, grid, accumulate);
+        else
+            std_grid_tiled_kernel<MODE, NP, ACC, false><<<(unsigned)n_bins, 64, 0, s>>>(
+                a, g, (const unsigned long long *)d_off, (const char *)d_ent,
+                (const double *)d_cgk, (int)cgk_len, grid, accumulate);
+        NGC_HIP(hipGetLastError());
+        phase_end(ctx, s);
+        return NGC_OK;
+    };
+    using I0 = std::integral_constant<int, NGC_GRID_IMAGE>;
+    using I1 = std::integral_constant<int, NGC_GRID_PSF>;
+    struct F32 { using type = float; };
+    struct F64 { using type = double; };
+    auto by_pol = [&](auto mode_c, auto acc_c) -> int {
+        switch (a.n_pol) {
+        case 1: return run(mode_c, std::integral_constant<int, 1>{}, acc_c);
+        case 2: return run(mode_c, std::integral_constant<int, 2>{}, acc_c);
+        case 3: return run(mode_c, std::integral_constant<int, 3>{}, acc_c);
+        default: return run(mode_c, std::integral_constant<int, 4>{}, acc_c);
+        }
+    };
     if (mode == NGC_GRID_IMAGE)
-        return launch_tiled<NGC_GRID_IMAGE, double>(a.n_pol, gridDim, s, a, g, off, ent,
-                                                    (const double *)d_cgk, (int)cgk_len,
-                                                    (const double *)d_ntab, half, grid,
-                                                    sum_weight, accumulate);
-    return launch_tiled<NGC_GRID_PSF, double>(a.n_pol, gridDim, s, a, g, off, ent,
-                                              (const double *)d_cgk, (int)cgk_len,
-                                              (const double *)d_ntab, half, grid, sum_weight,
-                                              accumulate);
+        rc = single ? by_pol(I0{}, F32{}) : by_pol(I0{}, F64{});
+    else
+        rc = single ? by_pol(I1{}, F32{}) : by_pol(I1{}, F64{});
+    return rc;
 }
 
 }  // namespace ngc

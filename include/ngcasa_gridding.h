@@ -75,9 +75,11 @@ extern "C" {
 /* precision forms */
 #define NGC_PREC_FP64 0       /* plain FP64 drop-in: reference loop, FP64 atomics */
 #define NGC_PREC_FP64_TILED 1 /* uv-tiled binning, FP64 LDS accumulation */
-#define NGC_PREC_FP32_KAHAN 2 /* uv-tiled binning, FP32 LDS accumulation with
-                                 compensated (Kahan-Neumaier) folding; FP64 for
-                                 coordinates and sum_weight */
+#define NGC_PREC_FP32_KAHAN 2 /* uv-tiled binning, FP32 register runs flushed
+                                 into an LDS subgrid with compensated
+                                 (Kahan-Neumaier) sums; FP64 for coordinates
+                                 and sum_weight */
+#define NGC_PREC_FP32 3       /* as FP32_KAHAN without the compensation words */
 
 /* element types */
 #define NGC_C64 0
@@ -104,7 +106,7 @@ const char *ngc_version(void);
  *   mode       NGC_GRID_IMAGE, NGC_GRID_PSF or NGC_GRID_DENSITY
  *   precision  NGC_PREC_*; grid element type follows it:
  *              FP64 forms: complex128 (IMAGE) / float64 (PSF, DENSITY);
- *              FP32_KAHAN: complex64 (IMAGE) / float32 (PSF, DENSITY)
+ *              FP32 forms: complex64 (IMAGE) / float32 (PSF, DENSITY)
  *   vis        device [T,B,C,P] of vis_dtype (NGC_C64|NGC_C128); NULL unless IMAGE
  *   weight     device [T,B,C,P] of weight_dtype (NGC_F32|NGC_F64)
  *   chan_map   host int32 [n_chan] -> image channel; pol_map host int32 [n_pol]
