@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--cpu-chans", type=int, default=6)
     ap.add_argument("--config", default="C2")
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
+    ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 16, 32],
+                    help="uv-tile rows of the gridder (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); "
@@ -90,6 +92,8 @@ def main():
         cropped_correction_factors, prolate_spheroidal_kernel_1d
 
     torch.cuda.set_device(0)
+    if args.tile_rows:
+        nat.set_tile_rows(args.tile_rows)
     cfg = syn.CONFIGS[args.config]
     N = cfg.image_size
     npad = int(1.2 * N)
@@ -166,19 +170,27 @@ def main():
         corr = cropped_correction_factors([npad, npad], [N, N])
         step()
         img32 = nat.grid_to_image(grid, [N, N], sum_weight=sw, corr=corr)
-        del grid
         g64, sw64 = nat.standard_grid(uvw, iw, fr, chan_map, pol_map, cgk, C, [npad, npad],
                                       delta, 7, 100, nat.GRID_IMAGE, "fp64_tiled", vis=vis,
                                       flag=flag)
         img64 = nat.grid_to_image(g64, [N, N], sum_weight=sw64, corr=corr)
-        del g64
+        # split the error: gridding alone (FP32 grid through the FP64 FFT)
+        g32 = torch.empty_like(g64)
+        step()
+        g32.copy_(grid)
+        img32_f64fft = nat.grid_to_image(g32, [N, N], sum_weight=sw, corr=corr)
+        del g32
+        grid_rel = float((torch.linalg.norm((img32_f64fft - img64).reshape(n_planes, -1), dim=1) /
+                          torch.linalg.norm(img64.reshape(n_planes, -1), dim=1)).max().item())
+        del img32_f64fft, g64
         d = (img32 - img64).reshape(n_planes, -1)
         r = img64.reshape(n_planes, -1)
         rel = (torch.linalg.norm(d, dim=1) / torch.linalg.norm(r, dim=1)).cpu().numpy()
         pk = (d.abs().max(dim=1).values / r.abs().max(dim=1).values).cpu().numpy()
         err = dict(rel_rms_max=float(rel.max()), rel_rms_median=float(np.median(rel)),
                    max_abs_over_peak=float(pk.max()), planes=int(n_planes),
-                   sum_weight_rel=float(((sw - sw64).abs() / sw64.abs()).max().item()))
+                   sum_weight_rel=float(((sw - sw64).abs() / sw64.abs()).max().item()),
+                   gridding_only_rel_rms_max=grid_rel)
         del img32, img64
         torch.cuda.synchronize()
         log(f"error check {time.time() - te:.1f}s: {err}")
@@ -218,6 +230,7 @@ def main():
         "error_vs_fp64": err,
         "phases_ms": phase_ms,
         "briggs_ms": round(t_briggs * 1e3, 2),
+        "tile_rows": args.tile_rows or 32,
     }
     print(json.dumps(line), flush=True)
 

@@ -28,7 +28,7 @@ WEIGHT_UNIFORM, WEIGHT_BRIGGS = 1, 2
 EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_version",
            "ngc_standard_grid", "ngc_briggs_factors", "ngc_imaging_weight_degrid",
            "ngc_aperture_grid", "ngc_grid_to_image", "ngc_normalize_mosaic",
-           "ngc_profile_enable", "ngc_profile_read")
+           "ngc_profile_enable", "ngc_profile_read", "ngc_set_tile_rows")
 
 
 class NativeError(RuntimeError):
@@ -63,6 +63,7 @@ _PROTOS = {
     "ngc_grid_to_image": (_i32, [_vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64, _i64, _vp, _vp,
                                  _vp, _vp]),
     "ngc_profile_enable": (_i32, [_vp, _i32]),
+    "ngc_set_tile_rows": (_i32, [_vp, _i32]),
     "ngc_profile_read": (_i32, [_vp, ctypes.c_char_p, _i64, _vp, _i64]),
     "ngc_normalize_mosaic": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64,
                                     _i64, _d, _i32]),
@@ -376,3 +377,9 @@ def profile_read():
         _check("ngc_profile_read", n)
     parts = names.value.decode().split(";") if n > 0 else []
     return list(zip(parts, ms[:n].tolist()))
+
+
+def set_tile_rows(rows):
+    """Tile rows (16 or 32) of the uv-tiled gridders on the current device."""
+    ctx = context()
+    _check("ngc_set_tile_rows", load_library().ngc_set_tile_rows(ctx.handle, int(rows)))

@@ -97,6 +97,14 @@ void phase_end(ngc_context *ctx, hipStream_t s) {
 
 extern "C" {
 
+int ngc_set_tile_rows(ngc_context *ctx, int tile_rows) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    NGC_CHECK(tile_rows == 16 || tile_rows == 32, NGC_ERR_ARG, "tile_rows must be 16 or 32");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tile_rows = tile_rows;
+    return NGC_OK;
+}
+
 int ngc_profile_enable(ngc_context *ctx, int enable) {
     NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
     std::lock_guard<std::mutex> lock(ctx->mu);

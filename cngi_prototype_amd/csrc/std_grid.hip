@@ -174,6 +174,7 @@ std_grid_plain_kernel(StdArgs a, const double *cgk, int64_t cgk_len, double *gri
 // reach each bin as a contiguous run in time order.
 struct BinGeom {
     int64_t ntu, ntv;      // tiles per axis
+    int tu;                // tile rows (u extent); the v extent is kTile
     int ch_log2;           // log2(ch_lanes)
     int64_t n_tb, n_cb;    // time blocks, channel blocks
 };
@@ -220,7 +221,7 @@ __device__ inline bool place(const StdArgs &a, const Sample &sm, Pix &px) {
 __device__ inline int64_t tile_key(const StdArgs &a, const BinGeom &g, const Pix &px,
                                    int64_t a_chan, int k) {
     const int64_t sc = a.support / 2;
-    const int64_t tu0 = (px.uc - sc) / kTile, tu1 = (px.uc + sc) / kTile;
+    const int64_t tu0 = (px.uc - sc) / g.tu, tu1 = (px.uc + sc) / g.tu;
     const int64_t tv0 = (px.vc - sc) / kTile, tv1 = (px.vc + sc) / kTile;
     const int64_t tu = tu0 + (k & 1), tv = tv0 + (k >> 1);
     if (tu > tu1 || tv > tv1) return -1;
@@ -326,7 +327,7 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
 }
 
 // ---------------------------------------------------------------------------
-// tiled gridding kernel: one wavefront per (image channel, 32x32 tile) bin
+// tiled gridding kernel: one wavefront per (image channel, TU x 32 tile) bin
 // ---------------------------------------------------------------------------
 // Work distribution after J. W. Romein, "An efficient work-distribution
 // strategy for gridding radio-telescope data on GPUs" (ICS 2012), applied to
@@ -340,29 +341,39 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
 // never leave the register file.  With KAHAN the LDS subgrid carries a
 // Kahan-Neumaier compensation word per value and every flush is a
 // compensated add.  The tile is written to HBM once at the end.
-template <int NP, int NC, typename ACC> struct Stage {
-    int lu0, lv0;          // tap window origin relative to the tile
-    ACC wd[NP][NC];
-    ACC ku[8], kv[8];
-};
+//
+// Per-sample work is done once per sample, by the lane that stages it: the
+// 8 u and 8 v tap weights (into LDS) and a 16-bit mask of the residue rows
+// and columns whose cell differs from the previous sample's.  The broadcast
+// loop then needs, per lane and sample, two lane-indexed weight reads, one
+// mask test and the multiply-adds.  TU (16 or 32 rows) trades LDS per
+// wavefront, hence occupancy, against the share of samples binned to two
+// tiles.
+constexpr int kBatch = 64;
+constexpr int kGroup = 4;
+constexpr int kRow = kTile + 1;  // LDS row stride (words)
 
-constexpr int kStage = 32;      // entries staged per round
-constexpr int kRow = kTile + 1; // LDS row stride (words)
+// cell of residue r for a window starting at w: the first u >= w with u = r (mod 8)
+__device__ inline int residue_cell(int w, int r) { return w + ((r - w) & 7); }
 
-template <int MODE, int NP, typename ACC, bool KAHAN>
-__global__ void __launch_bounds__(64, 4)
-std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *offsets,
-                      const char *entries, const double *cgk, int cgk_len, void *grid_out,
-                      int accumulate) {
+template <int MODE, int NP, typename ACC, bool KAHAN, int TU>
+__global__ void __launch_bounds__(64)
+std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict__ offsets,
+                      const char *__restrict__ entries, const double *__restrict__ cgk,
+                      int cgk_len, void *grid_out, int accumulate) {
     constexpr int NC = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
-    constexpr int PLANE = kTile * kRow;
+    constexpr int PLANE = TU * kRow;
     constexpr int NPC = NP * NC;
     using E = Entry<NP, NC, ACC>;
     constexpr int EB = EntrySize<NP, NC, ACC>::bytes;
+    struct WD {
+        ACC v[NPC];
+    };
 
     __shared__ ACC s_acc[NPC * PLANE];
     __shared__ ACC s_cmp[KAHAN ? NPC * PLANE : 1];
-    __shared__ Stage<NP, NC, ACC> s_stage[kStage];
+    __shared__ ACC s_w[kBatch][16];   // u weights [0..7], v weights [8..15]
+    __shared__ WD s_wd[kBatch];
     __shared__ ACC s_k[512];
 
     const int lane = threadIdx.x;
@@ -370,7 +381,7 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *offsets,
     const int64_t tiles = g.ntu * g.ntv;
     const int64_t a_chan = bin / tiles;
     const int64_t trem = bin - a_chan * tiles;
-    const int u_lo = (int)((trem / g.ntv) * kTile);
+    const int u_lo = (int)((trem / g.ntv) * TU);
     const int v_lo = (int)((trem % g.ntv) * kTile);
     const int64_t start = (int64_t)offsets[bin], end = (int64_t)offsets[bin + 1];
 
@@ -387,85 +398,117 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *offsets,
     const int S = (int)a.support;
     const int os = (int)a.os;
     const int ra = lane & 7, rb = lane >> 3;  // owned residues
+    const int my_bits = (1 << ra) | (1 << (8 + rb));
 
-    int cur_u = -64, cur_v = -64;             // current cell (tile-relative)
-    ACC acc[NP][NC];
+    int cur_u = -64, cur_v = -64;  // current cell (tile-relative), off-tile = none
+    ACC acc[NPC];
 #pragma unroll
-    for (int p = 0; p < NP; p++)
-#pragma unroll
-        for (int c = 0; c < NC; c++) acc[p][c] = (ACC)0;
+    for (int q = 0; q < NPC; q++) acc[q] = (ACC)0;
 
     auto flush = [&]() {
-        if ((unsigned)cur_u < (unsigned)kTile && (unsigned)cur_v < (unsigned)kTile) {
+        if ((unsigned)cur_u < (unsigned)TU && (unsigned)cur_v < (unsigned)kTile) {
             const int o = cur_u * kRow + cur_v;
 #pragma unroll
-            for (int p = 0; p < NP; p++)
-#pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    ACC *dst = &s_acc[(p * NC + c) * PLANE + o];
-                    const ACC x = acc[p][c];
-                    if (KAHAN) {
-                        const ACC s = *dst;
-                        const ACC t = s + x;
-                        ACC *cp = &s_cmp[(p * NC + c) * PLANE + o];
-                        if (fabs(s) >= fabs(x))
-                            *cp += (s - t) + x;
-                        else
-                            *cp += (x - t) + s;
-                        *dst = t;
-                    } else {
-                        *dst = *dst + x;
-                    }
+            for (int q = 0; q < NPC; q++) {
+                ACC *dst = &s_acc[q * PLANE + o];
+                const ACC x = acc[q];
+                if (KAHAN) {
+                    const ACC s = *dst;
+                    const ACC t = s + x;
+                    ACC *cp = &s_cmp[q * PLANE + o];
+                    if (fabs(s) >= fabs(x))
+                        *cp += (s - t) + x;
+                    else
+                        *cp += (x - t) + s;
+                    *dst = t;
+                } else {
+                    *dst = *dst + x;
                 }
+            }
         }
     };
 
-    for (int64_t b0 = start; b0 < end; b0 += kStage) {
+    int prev_lu0 = -1000, prev_lv0 = -1000;  // window of the last sample of the previous batch
+    for (int64_t b0 = start; b0 < end; b0 += kBatch) {
+        // staging: lane j unpacks sample b0 + j
         const int64_t e = b0 + lane;
-        if (lane < kStage && e < end) {
+        const bool valid = e < end;
+        int lu0 = -1000, lv0 = -1000;
+        if (valid) {
             const E en = *reinterpret_cast<const E *>(entries + e * EB);
-            Stage<NP, NC, ACC> st;
-            const int u0 = en.uv & 0xffff, v0 = (en.uv >> 16) & 0xffff;
+            lu0 = (en.uv & 0xffff) - sc - u_lo;
+            lv0 = ((en.uv >> 16) & 0xffff) - sc - v_lo;
             const int du = (int)(short)(en.duv & 0xffff);
             const int dv = (int)(short)((en.duv >> 16) & 0xffff);
-            st.lu0 = u0 - sc - u_lo;
-            st.lv0 = v0 - sc - v_lo;
-#pragma unroll
-            for (int p = 0; p < NP; p++)
-#pragma unroll
-                for (int c = 0; c < NC; c++) st.wd[p][c] = en.wd[p][c];
 #pragma unroll
             for (int i = 0; i < 8; i++) {
                 const int ku = abs(os * (i - sc) + du);
                 const int kv = abs(os * (i - sc) + dv);
-                st.ku[i] = (i < S && ku < cgk_len) ? s_k[ku] : (ACC)0;
-                st.kv[i] = (i < S && kv < cgk_len) ? s_k[kv] : (ACC)0;
+                s_w[lane][i] = (i < S && ku < cgk_len) ? s_k[ku] : (ACC)0;
+                s_w[lane][8 + i] = (i < S && kv < cgk_len) ? s_k[kv] : (ACC)0;
             }
-            s_stage[lane] = st;
+            WD w;
+#pragma unroll
+            for (int p = 0; p < NP; p++)
+#pragma unroll
+                for (int c = 0; c < NC; c++) w.v[p * NC + c] = en.wd[p][c];
+            s_wd[lane] = w;
+        } else {
+            // padding samples carry zero weight and data
+#pragma unroll
+            for (int i = 0; i < 16; i++) s_w[lane][i] = (ACC)0;
+            WD w;
+#pragma unroll
+            for (int q = 0; q < NPC; q++) w.v[q] = (ACC)0;
+            s_wd[lane] = w;
         }
+        // previous sample's window (lane 0 takes the last one of the previous batch)
+        int plu0 = __shfl_up(lu0, 1, 64), plv0 = __shfl_up(lv0, 1, 64);
+        if (lane == 0) {
+            plu0 = prev_lu0;
+            plv0 = prev_lv0;
+        }
+        int chg = 0;
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            chg |= (residue_cell(plu0, r) != residue_cell(lu0, r)) ? (1 << r) : 0;
+            chg |= (residue_cell(plv0, r) != residue_cell(lv0, r)) ? (1 << (8 + r)) : 0;
+        }
+        const int64_t nleft = end - b0;
+        const int nh = nleft >= kBatch ? kBatch : (int)nleft;
+        prev_lu0 = __shfl(lu0, nh - 1, 64);
+        prev_lv0 = __shfl(lv0, nh - 1, 64);
         __syncthreads();
-        const int64_t nh64 = end - b0;
-        const int nh = nh64 > kStage ? kStage : (int)nh64;
-        for (int j = 0; j < nh; j++) {
-            const Stage<NP, NC, ACC> &st = s_stage[j];
-            const int iu = (ra - st.lu0) & 7;
-            const int iv = (rb - st.lv0) & 7;
-            if (iu < S && iv < S) {
-                const int u = st.lu0 + iu, v = st.lv0 + iv;
-                if (u != cur_u || v != cur_v) {
+
+        const int ng = (nh + kGroup - 1) / kGroup;
+        for (int gi = 0; gi < ng; gi++) {
+            int iu[kGroup], iv[kGroup], hit[kGroup];
+            ACC wt[kGroup];
+#pragma unroll
+            for (int jj = 0; jj < kGroup; jj++) {
+                const int j = gi * kGroup + jj;
+                const int hu = __builtin_amdgcn_readlane(lu0, j);
+                const int hv = __builtin_amdgcn_readlane(lv0, j);
+                const int hc = __builtin_amdgcn_readlane(chg, j);
+                iu[jj] = (ra - hu) & 7;
+                iv[jj] = (rb - hv) & 7;
+                hit[jj] = hc & my_bits;
+                wt[jj] = s_w[j][iu[jj]] * s_w[j][8 + iv[jj]];
+                iu[jj] += hu;  // now the lane's cell
+                iv[jj] += hv;
+            }
+#pragma unroll
+            for (int jj = 0; jj < kGroup; jj++) {
+                const WD w = s_wd[gi * kGroup + jj];
+                if (hit[jj]) {
                     flush();
-                    cur_u = u;
-                    cur_v = v;
+                    cur_u = iu[jj];
+                    cur_v = iv[jj];
 #pragma unroll
-                    for (int p = 0; p < NP; p++)
-#pragma unroll
-                        for (int c = 0; c < NC; c++) acc[p][c] = (ACC)0;
+                    for (int q = 0; q < NPC; q++) acc[q] = (ACC)0;
                 }
-                const ACC w = st.ku[iu] * st.kv[iv];
 #pragma unroll
-                for (int p = 0; p < NP; p++)
-#pragma unroll
-                    for (int c = 0; c < NC; c++) acc[p][c] += w * st.wd[p][c];
+                for (int q = 0; q < NPC; q++) acc[q] += wt[jj] * w.v[q];
             }
         }
         __syncthreads();
@@ -474,13 +517,13 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *offsets,
     __syncthreads();
 
     // write the tile: every cell of the grid is written exactly once
-    constexpr int F = NPC * kTile * kTile;
+    constexpr int F = NPC * TU * kTile;
 #pragma unroll 4
     for (int k = lane; k < F; k += 64) {
         const int c = k % NC;
         const int lv = (k / NC) % kTile;
-        const int lu = (k / (NC * kTile)) % kTile;
-        const int p = k / (NC * kTile * kTile);
+        const int lu = (k / (NC * kTile)) % TU;
+        const int p = k / (NC * kTile * TU);
         const int64_t gu = u_lo + lu, gv = v_lo + lv;
         if (gu >= a.n_u || gv >= a.n_v) continue;
         const int o = (p * NC + c) * PLANE + lu * kRow + lv;
@@ -636,7 +679,8 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     NGC_TRY(upload(ctx, s, "norm_tab", ntab.data(), ntab.size() * sizeof(double), &d_ntab));
 
     BinGeom g;
-    g.ntu = ceil_div(a.n_u, kTile);
+    g.tu = ctx->tile_rows == 16 ? 16 : 32;
+    g.ntu = ceil_div(a.n_u, g.tu);
     g.ntv = ceil_div(a.n_v, kTile);
     // two channels per integration and 32 integrations per wavefront: 64-byte
     // contiguous visibility reads, and runs of 32 integrations per bin
@@ -700,14 +744,21 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
         }
         phase_end(ctx, s);
         phase_begin(ctx, s, "std_grid_tiled");
-        if (compensated)
-            std_grid_tiled_kernel<MODE, NP, ACC, true><<<(unsigned)n_bins, 64, 0, s>>>(
-                a, g, (const unsigned long long *)d_off, (const char *)d_ent,
-                (const double *)d_cgk, (int)cgk_len, grid, accumulate);
+        const unsigned long long *offp = (const unsigned long long *)d_off;
+        const char *entp = (const char *)d_ent;
+        const double *cgkp = (const double *)d_cgk;
+        if (compensated && g.tu == 16)
+            std_grid_tiled_kernel<MODE, NP, ACC, true, 16><<<(unsigned)n_bins, 64, 0, s>>>(
+                a, g, offp, entp, cgkp, (int)cgk_len, grid, accumulate);
+        else if (compensated)
+            std_grid_tiled_kernel<MODE, NP, ACC, true, 32><<<(unsigned)n_bins, 64, 0, s>>>(
+                a, g, offp, entp, cgkp, (int)cgk_len, grid, accumulate);
+        else if (g.tu == 16)
+            std_grid_tiled_kernel<MODE, NP, ACC, false, 16><<<(unsigned)n_bins, 64, 0, s>>>(
+                a, g, offp, entp, cgkp, (int)cgk_len, grid, accumulate);
         else
-            std_grid_tiled_kernel<MODE, NP, ACC, false><<<(unsigned)n_bins, 64, 0, s>>>(
-                a, g, (const unsigned long long *)d_off, (const char *)d_ent,
-                (const double *)d_cgk, (int)cgk_len, grid, accumulate);
+            std_grid_tiled_kernel<MODE, NP, ACC, false, 32><<<(unsigned)n_bins, 64, 0, s>>>(
+                a, g, offp, entp, cgkp, (int)cgk_len, grid, accumulate);
         NGC_HIP(hipGetLastError());
         phase_end(ctx, s);
         return NGC_OK;

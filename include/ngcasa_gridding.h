@@ -229,6 +229,13 @@ int ngc_normalize_mosaic(ngc_context *ctx, void *stream, double *image,
                          int single_precision);
 
 /*
+ * Tile shape of the uv-tiled gridders: 16 or 32 grid rows per tile (the tile
+ * is always 32 columns wide).  Set by the autotuner (no counterpart in the
+ * reference, whose gridder has no tiles).  Default 32.
+ */
+int ngc_set_tile_rows(ngc_context *ctx, int tile_rows);
+
+/*
  * Phase timing (instrumentation; no counterpart in the reference, which has
  * no timing inside its kernels).  When enabled, each gridding call records
  * HIP events on its stream around its phases; ngc_profile_read synchronises
