@@ -342,14 +342,16 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
 // Kahan-Neumaier compensation word per value and every flush is a
 // compensated add.  The tile is written to HBM once at the end.
 //
-// Per-sample work is done once per sample, by the lane that stages it: the
-// 8 u and 8 v tap weights (into LDS) and a 16-bit mask of the residue rows
-// and columns whose cell differs from the previous sample's.  The broadcast
-// loop then needs, per lane and sample, two lane-indexed weight reads, one
-// mask test and the multiply-adds.  TU (16 or 32 rows) trades LDS per
-// wavefront, hence occupancy, against the share of samples binned to two
-// tiles.
-constexpr int kBatch = 64;
+// Per-sample work is done once per sample by the lane that stages it, in
+// per-residue layout: for residue r the u tap weight of the lane owning u = r
+// (mod 8), with the sign bit set when that residue's cell differs from the
+// previous sample's (the weights are non-negative), the same for v, and the
+// residue's cell coordinates.  In the broadcast loop a lane then reads its
+// two words at a fixed offset, ORs the sign bits to decide a flush, and
+// multiplies the magnitudes (abs is an operand modifier).  TU (16 or 32 rows)
+// trades LDS per wavefront, hence occupancy, against the share of samples
+// binned to two tiles.
+constexpr int kBatch = 32;
 constexpr int kGroup = 4;
 constexpr int kRow = kTile + 1;  // LDS row stride (words)
 
@@ -372,7 +374,9 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
 
     __shared__ ACC s_acc[NPC * PLANE];
     __shared__ ACC s_cmp[KAHAN ? NPC * PLANE : 1];
-    __shared__ ACC s_w[kBatch][16];   // u weights [0..7], v weights [8..15]
+    __shared__ ACC s_wu[kBatch][8];    // u weight per residue, sign = cell changed
+    __shared__ ACC s_wv[kBatch][8];    // v weight per residue, sign = cell changed
+    __shared__ int s_cell[kBatch][8];  // cell_u(r) | cell_v(r) << 16 (tile-relative + 64)
     __shared__ WD s_wd[kBatch];
     __shared__ ACC s_k[512];
 
@@ -398,7 +402,6 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     const int S = (int)a.support;
     const int os = (int)a.os;
     const int ra = lane & 7, rb = lane >> 3;  // owned residues
-    const int my_bits = (1 << ra) | (1 << (8 + rb));
 
     int cur_u = -64, cur_v = -64;  // current cell (tile-relative), off-tile = none
     ACC acc[NPC];
@@ -430,49 +433,43 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
 
     int prev_lu0 = -1000, prev_lv0 = -1000;  // window of the last sample of the previous batch
     for (int64_t b0 = start; b0 < end; b0 += kBatch) {
-        // staging: lane j unpacks sample b0 + j
+        // staging: lane j < kBatch unpacks sample b0 + j
         const int64_t e = b0 + lane;
-        const bool valid = e < end;
-        int lu0 = -1000, lv0 = -1000;
+        const bool valid = lane < kBatch && e < end;
+        int lu0 = -1000, lv0 = -1000, du = 0, dv = 0;
         if (valid) {
             const E en = *reinterpret_cast<const E *>(entries + e * EB);
             lu0 = (en.uv & 0xffff) - sc - u_lo;
             lv0 = ((en.uv >> 16) & 0xffff) - sc - v_lo;
-            const int du = (int)(short)(en.duv & 0xffff);
-            const int dv = (int)(short)((en.duv >> 16) & 0xffff);
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const int ku = abs(os * (i - sc) + du);
-                const int kv = abs(os * (i - sc) + dv);
-                s_w[lane][i] = (i < S && ku < cgk_len) ? s_k[ku] : (ACC)0;
-                s_w[lane][8 + i] = (i < S && kv < cgk_len) ? s_k[kv] : (ACC)0;
-            }
+            du = (int)(short)(en.duv & 0xffff);
+            dv = (int)(short)((en.duv >> 16) & 0xffff);
             WD w;
 #pragma unroll
             for (int p = 0; p < NP; p++)
 #pragma unroll
                 for (int c = 0; c < NC; c++) w.v[p * NC + c] = en.wd[p][c];
             s_wd[lane] = w;
-        } else {
-            // padding samples carry zero weight and data
-#pragma unroll
-            for (int i = 0; i < 16; i++) s_w[lane][i] = (ACC)0;
-            WD w;
-#pragma unroll
-            for (int q = 0; q < NPC; q++) w.v[q] = (ACC)0;
-            s_wd[lane] = w;
         }
-        // previous sample's window (lane 0 takes the last one of the previous batch)
         int plu0 = __shfl_up(lu0, 1, 64), plv0 = __shfl_up(lv0, 1, 64);
         if (lane == 0) {
             plu0 = prev_lu0;
             plv0 = prev_lv0;
         }
-        int chg = 0;
+        if (lane < kBatch) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-            chg |= (residue_cell(plu0, r) != residue_cell(lu0, r)) ? (1 << r) : 0;
-            chg |= (residue_cell(plv0, r) != residue_cell(lv0, r)) ? (1 << (8 + r)) : 0;
+            for (int r = 0; r < 8; r++) {
+                const int cu = residue_cell(lu0, r), cv = residue_cell(lv0, r);
+                const int iu = cu - lu0, iv = cv - lv0;
+                const int ku = abs(os * (iu - sc) + du);
+                const int kv = abs(os * (iv - sc) + dv);
+                ACC wu = (valid && iu < S && ku < cgk_len) ? s_k[ku] : (ACC)0;
+                ACC wv = (valid && iv < S && kv < cgk_len) ? s_k[kv] : (ACC)0;
+                if (cu != residue_cell(plu0, r)) wu = -wu - (ACC)0;  // sign bit set, even for 0
+                if (cv != residue_cell(plv0, r)) wv = -wv - (ACC)0;
+                s_wu[lane][r] = wu;
+                s_wv[lane][r] = wv;
+                s_cell[lane][r] = ((cu + 64) & 0xffff) | ((cv + 64) << 16);
+            }
         }
         const int64_t nleft = end - b0;
         const int nh = nleft >= kBatch ? kBatch : (int)nleft;
@@ -482,33 +479,26 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
 
         const int ng = (nh + kGroup - 1) / kGroup;
         for (int gi = 0; gi < ng; gi++) {
-            int iu[kGroup], iv[kGroup], hit[kGroup];
-            ACC wt[kGroup];
+            ACC wu[kGroup], wv[kGroup];
 #pragma unroll
             for (int jj = 0; jj < kGroup; jj++) {
-                const int j = gi * kGroup + jj;
-                const int hu = __builtin_amdgcn_readlane(lu0, j);
-                const int hv = __builtin_amdgcn_readlane(lv0, j);
-                const int hc = __builtin_amdgcn_readlane(chg, j);
-                iu[jj] = (ra - hu) & 7;
-                iv[jj] = (rb - hv) & 7;
-                hit[jj] = hc & my_bits;
-                wt[jj] = s_w[j][iu[jj]] * s_w[j][8 + iv[jj]];
-                iu[jj] += hu;  // now the lane's cell
-                iv[jj] += hv;
+                wu[jj] = s_wu[gi * kGroup + jj][ra];
+                wv[jj] = s_wv[gi * kGroup + jj][rb];
             }
 #pragma unroll
             for (int jj = 0; jj < kGroup; jj++) {
-                const WD w = s_wd[gi * kGroup + jj];
-                if (hit[jj]) {
+                const int j = gi * kGroup + jj;
+                const WD w = s_wd[j];
+                if (signbit(wu[jj]) || signbit(wv[jj])) {
                     flush();
-                    cur_u = iu[jj];
-                    cur_v = iv[jj];
+                    cur_u = (s_cell[j][ra] & 0xffff) - 64;
+                    cur_v = (s_cell[j][rb] >> 16) - 64;
 #pragma unroll
                     for (int q = 0; q < NPC; q++) acc[q] = (ACC)0;
                 }
+                const ACC wt = fabs(wu[jj]) * fabs(wv[jj]);
 #pragma unroll
-                for (int q = 0; q < NPC; q++) acc[q] += wt[jj] * w.v[q];
+                for (int q = 0; q < NPC; q++) acc[q] += wt * w.v[q];
             }
         }
         __syncthreads();
