@@ -28,7 +28,8 @@ WEIGHT_UNIFORM, WEIGHT_BRIGGS = 1, 2
 EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_version",
            "ngc_standard_grid", "ngc_briggs_factors", "ngc_imaging_weight_degrid",
            "ngc_aperture_grid", "ngc_grid_to_image", "ngc_normalize_mosaic",
-           "ngc_profile_enable", "ngc_profile_read", "ngc_set_tile_rows")
+           "ngc_profile_enable", "ngc_profile_read", "ngc_set_tile_rows",
+           "ngc_gcf_patterns", "ngc_gcf_resize", "ngc_phase_gradient")
 
 
 class NativeError(RuntimeError):
@@ -64,6 +65,11 @@ _PROTOS = {
                                  _vp, _vp]),
     "ngc_profile_enable": (_i32, [_vp, _i32]),
     "ngc_set_tile_rows": (_i32, [_vp, _i32]),
+    "ngc_gcf_patterns": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _i32, _i32, _d,
+                                _d, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "ngc_gcf_resize": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
+                              _vp]),
+    "ngc_phase_gradient": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _vp]),
     "ngc_profile_read": (_i32, [_vp, ctypes.c_char_p, _i64, _vp, _i64]),
     "ngc_normalize_mosaic": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64,
                                     _i64, _d, _i32]),
@@ -383,3 +389,61 @@ def set_tile_rows(rows):
     """Tile rows (16 or 32) of the uv-tiled gridders on the current device."""
     ctx = context()
     _check("ngc_set_tile_rows", load_library().ngc_set_tile_rows(ctx.handle, int(rows)))
+
+
+def gcf_patterns(pb_freq, dish, blockage, pairs, casa, ipower, cell_lm, n_uv):
+    """ngc_gcf_patterns.  Returns (device float64 [n_pairs, n_freq, n_u, n_v],
+    abs_max, abs_min, centre_row, centre_col) with the host arrays shaped
+    [n_pairs, n_freq(, n)]."""
+    import torch
+    ctx = context()
+    fr, frp = _host(pb_freq, np.float64)
+    di, dip = _host(dish, np.float64)
+    bl, blp = _host(blockage, np.float64)
+    pr, prp = _host(pairs, np.int32)
+    n_pairs, n_freq = len(pr), len(fr)
+    n_u, n_v = int(n_uv[0]), int(n_uv[1])
+    out = torch.empty((n_pairs, n_freq, n_u, n_v), dtype=torch.float64, device="cuda")
+    mx = np.zeros(n_pairs * n_freq)
+    mn = np.zeros(n_pairs * n_freq)
+    row = np.zeros((n_pairs * n_freq, n_u))
+    col = np.zeros((n_pairs * n_freq, n_v))
+    rc = load_library().ngc_gcf_patterns(
+        ctx.handle, _stream(), frp, n_freq, dip, blp, len(di), prp, n_pairs, int(bool(casa)),
+        int(ipower), float(cell_lm[0]), float(cell_lm[1]), n_u, n_v, _ptr(out),
+        mx.ctypes.data_as(_vp), mn.ctypes.data_as(_vp), row.ctypes.data_as(_vp),
+        col.ctypes.data_as(_vp))
+    _check("ngc_gcf_patterns", rc)
+    sh = (n_pairs, n_freq)
+    return out, mx.reshape(sh), mn.reshape(sh), row.reshape(sh + (n_u,)), col.reshape(sh + (n_v,))
+
+
+def gcf_resize(full, support, oversampling, resize_conv_size):
+    """ngc_gcf_resize on [..., n_u, n_v] planes; support [..., 2]."""
+    import torch
+    ctx = context()
+    _dev(full, "full")
+    lead = tuple(full.shape[:-2])
+    n_items = int(np.prod(lead)) if lead else 1
+    k_u, k_v = int(resize_conv_size[0]), int(resize_conv_size[1])
+    sp, spp = _host(np.asarray(support).reshape(n_items, 2), np.int32)
+    out = torch.empty(lead + (k_u, k_v), dtype=torch.float64, device=full.device)
+    rc = load_library().ngc_gcf_resize(ctx.handle, _stream(), _ptr(full), n_items,
+                                       full.shape[-2], full.shape[-1], spp,
+                                       int(oversampling[0]), int(oversampling[1]), k_u, k_v,
+                                       _ptr(out))
+    _check("ngc_gcf_resize", rc)
+    return out
+
+
+def phase_gradient(pix, resize_conv_size):
+    """ngc_phase_gradient: device complex128 [n_field, K_u, K_v]."""
+    import torch
+    ctx = context()
+    px, pxp = _host(np.asarray(pix).reshape(-1, 2), np.float64)
+    k_u, k_v = int(resize_conv_size[0]), int(resize_conv_size[1])
+    out = torch.empty((len(px), k_u, k_v), dtype=torch.complex128, device="cuda")
+    rc = load_library().ngc_phase_gradient(ctx.handle, _stream(), pxp, len(px), k_u, k_v,
+                                           _ptr(out))
+    _check("ngc_phase_gradient", rc)
+    return out

@@ -32,10 +32,10 @@ int fft_destroy_plans(ngc_context *ctx) {
 }
 
 static int get_plan(ngc_context *ctx, int64_t n_u, int64_t n_v, int64_t batch, bool dbl,
-                    FftPlan **out) {
+                    FftPlan **out, bool forward = false) {
     std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
     NGC_CHECK(g_rocfft_setup == rocfft_status_success, NGC_ERR_FFT, "rocfft_setup failed");
-    auto key = std::make_tuple(n_u, n_v, batch, dbl ? 1 : 0);
+    auto key = std::make_tuple(n_u, n_v, batch, (dbl ? 1 : 0) + (forward ? 2 : 0));
     auto it = ctx->plans.find(key);
     if (it != ctx->plans.end()) {
         *out = it->second;
@@ -44,7 +44,8 @@ static int get_plan(ngc_context *ctx, int64_t n_u, int64_t n_v, int64_t batch, b
     FftPlan *p = new FftPlan();
     size_t lengths[2] = {(size_t)n_v, (size_t)n_u};  // fastest axis first
     rocfft_status st = rocfft_plan_create(
-        &p->plan, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+        &p->plan, rocfft_placement_inplace,
+        forward ? rocfft_transform_type_complex_forward : rocfft_transform_type_complex_inverse,
         dbl ? rocfft_precision_double : rocfft_precision_single, 2, lengths, (size_t)batch,
         nullptr);
     if (st != rocfft_status_success) {
@@ -109,6 +110,22 @@ __global__ void crop_correct_kernel(const T *buf, int64_t n_planes, int64_t n_u,
         if (corr) x = x / (corr[l] * corr[n_l + m]);
         image[i] = x;
     }
+}
+
+// In-place batched 2-D complex FFT of `buf` ([batch, n_u, n_v], interleaved),
+// unnormalised, forward or inverse.
+int fft_execute(ngc_context *ctx, hipStream_t s, void *buf, int64_t n_u, int64_t n_v,
+                int64_t batch, bool dbl, bool forward) {
+    FftPlan *plan;
+    NGC_TRY(get_plan(ctx, n_u, n_v, batch, dbl, &plan, forward));
+    void *work = nullptr;
+    if (plan->work_bytes) NGC_TRY(workspace(ctx, "fft_work", plan->work_bytes, &work));
+    rocfft_execution_info_set_stream(plan->info, s);
+    if (plan->work_bytes) rocfft_execution_info_set_work_buffer(plan->info, work, plan->work_bytes);
+    void *bufs[1] = {buf};
+    rocfft_status st = rocfft_execute(plan->plan, bufs, nullptr, plan->info);
+    NGC_CHECK(st == rocfft_status_success, NGC_ERR_FFT, "rocfft_execute failed");
+    return NGC_OK;
 }
 
 static int blocks_for(int64_t n) {

@@ -104,6 +104,8 @@ int upload(ngc_context *ctx, hipStream_t stream, const char *name,
            const void *host, size_t bytes, void **dev);
 
 int fft_destroy_plans(ngc_context *ctx);
+int fft_execute(ngc_context *ctx, hipStream_t s, void *buf, int64_t n_u, int64_t n_v,
+                int64_t batch, bool dbl, bool forward);
 
 // Phase timing: begin a new call (clears the previous marks), then bracket
 // phases with phase_begin/phase_end.  No-ops unless profiling is enabled.

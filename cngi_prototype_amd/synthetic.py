@@ -71,6 +71,12 @@ CONFIGS = {
     "C3": Config("C3", 50, "lognormal", 3000.0, -23.0234, -23.0, 1000, 8.0, 100e9, 2e6, 256, 2,
                  1024, 0.2, 100, flag_per_plane_max=0.6, spectral_index=True, fov_deg=0.02,
                  seed=3),
+    # 25-pointing heterogeneous mosaic: 12 x 7 m + 40 x 12 m, 230 GHz, 4 h of 10 s dumps
+    "C4": Config("C4", 52, "lognormal", 1500.0, -23.0234, -23.0, 1440, 10.0, 230e9, 15.625e6,
+                 32, 2, 1024, 0.15, 60, fov_deg=0.012, lognormal_mu=np.log(150.0),
+                 lognormal_sigma=0.7, seed=4,
+                 extra=dict(n_field=25, dish=(7.0, 12.0), n_dish=(12, 40),
+                            fields_per_dump=1)),
 }
 
 
@@ -139,6 +145,50 @@ def sky(cfg):
         blobs.append((rng.uniform(-half, half), rng.uniform(-half, half),
                       float(np.exp(rng.uniform(lo, hi))), float(rng.uniform(0.2, 1.0) * half / 10)))
     return dict(l=l, m=m, flux=s, alpha=alpha, blobs=blobs)
+
+
+def dish_types(cfg):
+    """Dish-type index of every antenna (type-0 antennas first, so that
+    type(ant1) <= type(ant2) on every baseline) and the diameters."""
+    counts = cfg.extra.get("n_dish", (cfg.n_ant,))
+    dish = np.asarray(cfg.extra.get("dish", (12.0,)), dtype=np.float64)
+    types = np.concatenate([np.full(n, k, dtype=int) for k, n in enumerate(counts)])
+    assert len(types) == cfg.n_ant
+    return types, dish
+
+
+def mosaic_fields(cfg, phase_dir=None):
+    """Pointing centres [n_field, 2] (ra, dec radians) on a hexagonal grid
+    spaced by lambda/(2 D_max) at the lowest frequency, centred on the
+    phase centre (rows of 5 for 25 fields)."""
+    n_field = int(cfg.extra.get("n_field", 1))
+    ra0, dec0 = phase_center(cfg) if phase_dir is None else phase_dir
+    _, dish = dish_types(cfg)
+    step = C_LIGHT / cfg.f0 / (2 * dish.max())
+    side = int(np.ceil(np.sqrt(n_field)))
+    pts = []
+    for r in range(side):
+        for c in range(side):
+            pts.append(((c - (side - 1) / 2 + (0.5 if r % 2 else 0.0) - 0.25) * step,
+                        (r - (side - 1) / 2) * step * np.sqrt(3) / 2))
+    pts = np.array(pts[:n_field])
+    return np.stack([ra0 + pts[:, 0] / np.cos(dec0), dec0 + pts[:, 1]], 1)
+
+
+def field_schedule(cfg):
+    """FIELD_ID of every integration: the fields are visited in turn, an
+    equal number of dumps each (the remainder spread over the first)."""
+    n_field = int(cfg.extra.get("n_field", 1))
+    per = max(1, cfg.n_time // n_field)
+    return np.minimum(np.arange(cfg.n_time) // per, n_field - 1).astype(np.int32)
+
+
+def airy_voltage(r_rad, freq, dish):
+    """Unblocked Airy voltage pattern 2 J1(x)/x, x = pi D r / lambda."""
+    from scipy.special import j1
+    x = np.pi * dish * np.asarray(r_rad)[..., None] * np.asarray(freq) / C_LIGHT
+    xs = np.where(x == 0, 1.0, x)
+    return np.where(x == 0, 1.0, 2 * j1(xs) / xs)
 
 
 def freqs(cfg):
@@ -233,6 +283,62 @@ def make_mxds(cfg, vis=None, uvw=None, weight=None, flag=None, apply_flags=True)
         attrs={"data_groups": [{"0": {"id": "0", "data": "DATA", "uvw": "UVW", "flag": "FLAG",
                                       "weight": "DATA_WEIGHT"}}]})
     mxds = Dataset(attrs={"xds0": xds})
+    return mxds
+
+
+def make_mosaic_mxds(cfg, apply_flags=True):
+    """make_mxds for a mosaic: every integration points at one field
+    (FIELD_ID [time, baseline]), visibilities are phase-referenced to the
+    common phase centre and each source is attenuated by the product of the
+    two antennas' Airy voltage patterns at its distance from that field's
+    centre.  Adds mxds.attrs['FIELD'] (PHASE_DIR [n_field, 1, 2], coord
+    field_id) as read by make_gridding_convolution_function."""
+    from .xds import Dataset
+    enu = antenna_enu(cfg)
+    uvw = uvw_tracks(enu, cfg.lat_deg, cfg.dec_deg, cfg.n_time, cfg.dt)
+    fr = freqs(cfg)
+    sk = sky(cfg)
+    ra0, dec0 = phase_center(cfg)
+    fields = mosaic_fields(cfg)
+    sched = field_schedule(cfg)
+    types, dish = dish_types(cfg)
+    a1, a2 = baseline_pairs(cfg.n_ant)
+    # source directions from (l, m) about the phase centre (inverse SIN)
+    l, m = sk["l"], sk["m"]
+    n = np.sqrt(1 - l * l - m * m)
+    dec_s = np.arcsin(m * np.cos(dec0) + n * np.sin(dec0))
+    ra_s = ra0 + np.arctan2(l, n * np.cos(dec0) - m * np.sin(dec0))
+    T, B, C, P = cfg.n_time, cfg.n_baseline, cfg.n_chan, cfg.n_pol
+    lam = fr / C_LIGHT
+    vis = np.zeros((T, B, C, P), dtype=np.complex128)
+    for f in range(len(fields)):
+        rows = np.nonzero(sched == f)[0]
+        if len(rows) == 0:
+            continue
+        raf, decf = fields[f]
+        cosr = (np.sin(dec_s) * np.sin(decf) +
+                np.cos(dec_s) * np.cos(decf) * np.cos(ra_s - raf))
+        r = np.arccos(np.clip(cosr, -1, 1))                               # [S]
+        vp = np.stack([airy_voltage(r, fr, d) for d in dish])              # [n_dish, S, C]
+        g = vp[types[a1]] * vp[types[a2]]                                  # [B, S, C]
+        u = uvw[rows, :, 0, None] * lam
+        v = uvw[rows, :, 1, None] * lam
+        w = uvw[rows, :, 2, None] * lam
+        I = np.zeros((len(rows), B, C), dtype=np.complex128)
+        for k in range(len(l)):
+            I += (sk["flux"][k] * g[None, :, k, :] *
+                  np.exp(2j * np.pi * (u * l[k] + v * m[k] + w * (n[k] - 1))))
+        vis[rows, :, :, 0] = I
+        vis[rows, :, :, P - 1] = I
+    rng = np.random.default_rng(cfg.seed + 2000)
+    vis += cfg.noise * (rng.standard_normal(vis.shape) + 1j * rng.standard_normal(vis.shape))
+    mxds = make_mxds(cfg, vis=vis, uvw=uvw, apply_flags=apply_flags)
+    xds = mxds.attrs["xds0"]
+    from .xds import DataArray
+    xds["FIELD_ID"] = DataArray(np.repeat(sched[:, None], B, axis=1), dims=["time", "baseline"])
+    field = Dataset(data_vars={"PHASE_DIR": (["field_id", "poly_id", "d2"], fields[:, None, :])},
+                    coords={"field_id": (["field_id"], np.arange(len(fields)))})
+    mxds.attrs["FIELD"] = field
     return mxds
 
 

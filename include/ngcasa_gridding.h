@@ -229,6 +229,53 @@ int ngc_normalize_mosaic(ngc_context *ctx, void *stream, double *image,
                          int single_precision);
 
 /*
+ * A-term convolution-function patterns: replaces make_baseline_patterns and
+ * the real part of fftshift(fft2(ifftshift(.))) in
+ * make_gridding_convolution_function (make_gridding_convolution_function.py:
+ * 181-191, 245-246, 394-412; patterns _make_pb_symmetric.py:135-235).
+ *   pb_freq      host [n_freq] pattern frequencies (Hz)
+ *   dish, blockage host [n_dish] diameters (m)
+ *   pairs        host int32 [n_pairs][2] dish-type pairs
+ *   casa         1: CASA Airy (casa_airy), 0: obscured Airy (airy)
+ *   ipower       1: voltage-pattern products (CONV_KERNEL), 2: power (WEIGHT_CONV_KERNEL)
+ *   cell_l/m     pattern cell = image cell (radians, cell[0] negated) x GCF oversampling
+ *   out          device float64 [n_pairs, n_freq, n_u, n_v]
+ *   abs_max/min  host [n_pairs*n_freq] max and min of |out| per plane, or NULL
+ *   centre_row   host [n_pairs*n_freq][n_u]: out[..][:, n_v//2], or NULL
+ *   centre_col   host [n_pairs*n_freq][n_v]: out[..][n_u//2, :], or NULL
+ * Synchronises `stream` (the support cut is decided on the host from these).
+ */
+int ngc_gcf_patterns(ngc_context *ctx, void *stream, const double *pb_freq, int64_t n_freq,
+                     const double *dish, const double *blockage, int64_t n_dish,
+                     const int32_t *pairs, int64_t n_pairs, int casa, int ipower,
+                     double cell_l, double cell_m, int64_t n_u, int64_t n_v, double *out,
+                     double *abs_max, double *abs_min, double *centre_row,
+                     double *centre_col);
+
+/*
+ * Crop and normalisation of resize_and_calc_support
+ * (make_gridding_convolution_function.py:361-391): the centre k_u x k_v window
+ * of each pattern plane divided by Re(sum over the centred (support+1)*os
+ * window) / (os_u*os_v).
+ *   full     device float64 [n_items, n_u, n_v]
+ *   support  host int32 [n_items][2]
+ *   out      device float64 [n_items, k_u, k_v]
+ */
+int ngc_gcf_resize(ngc_context *ctx, void *stream, const double *full, int64_t n_items,
+                   int64_t n_u, int64_t n_v, const int32_t *support, int64_t os_u,
+                   int64_t os_v, int64_t k_u, int64_t k_v, double *out);
+
+/*
+ * Phase-gradient table of make_phase_gradient
+ * (make_gridding_convolution_function.py:349-359):
+ *   out[f][x][y] = exp(i (x*pix[f][0] + y*pix[f][1])), x in [-k_u//2, k_u - k_u//2)
+ *   pix  host float64 [n_field][2] (radians per oversampled kernel pixel)
+ *   out  device complex128 [n_field, k_u, k_v]
+ */
+int ngc_phase_gradient(ngc_context *ctx, void *stream, const double *pix, int64_t n_field,
+                       int64_t k_u, int64_t k_v, double *out);
+
+/*
  * Tile shape of the uv-tiled gridders: 16 or 32 grid rows per tile (the tile
  * is always 32 columns wide).  Set by the autotuner (no counterpart in the
  * reference, whose gridder has no tiles).  Default 32.

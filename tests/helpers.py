@@ -30,3 +30,22 @@ def small_case(name="C1", n_time=24, n_chan=3, n_pol=2, image_size=128, cell=Non
     delta = np.array([-cell_rad, cell_rad])
     return dict(cfg=cfg, uvw=uvw, freq=fr, vis=vis, weight=weight, npad=npad,
                 image_size=image_size, delta=delta, cell_arcsec=cell_rad / ARCSEC)
+
+
+def mosaic_case(n_time=40, n_chan=4, n_pol=2, n_field=4, image_size=160, cell_arcsec=0.3,
+                seed=4, flag_frac=0.05):
+    """Small heterogeneous mosaic: 3 x 7 m + 5 x 12 m antennas within 150 m,
+    n_field pointings visited in turn, 230 GHz.  Returns the mxds, the dish
+    types and the grid geometry used by the mosaic parity tests."""
+    base = syn.CONFIGS["C4"]
+    cfg = syn.scaled(base, n_ant=8, r_max=150.0, lognormal_mu=np.log(50.0),
+                     extra=dict(n_field=n_field, dish=(7.0, 12.0), n_dish=(3, 5)),
+                     n_time=n_time, dt=60.0, n_chan=n_chan, df=50e6, n_pol=n_pol,
+                     image_size=image_size, cell_arcsec=cell_arcsec, n_src=6, fov_deg=0.01,
+                     flag_frac=flag_frac, seed=seed)
+    mxds = syn.make_mosaic_mxds(cfg)
+    types, dish = syn.dish_types(cfg)
+    npad = int(1.2 * image_size)
+    delta = np.array([-cell_arcsec, cell_arcsec]) * ARCSEC
+    return dict(cfg=cfg, mxds=mxds, types=types, dish=dish, npad=npad, image_size=image_size,
+                delta=delta, phase_center=syn.phase_center(cfg))

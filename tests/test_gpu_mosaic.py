@@ -1,0 +1,180 @@
+"""Mosaic (A-term) path on the GPU against the oracle: GCF generation
+(patterns, FFT, support cut, crop/normalise, maps), phase gradients, the
+aperture gridder in image/PSF/weight modes, and the make_mosaic_pb /
+make_image_with_gcf / make_psf_with_gcf chain."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+from helpers import mosaic_case
+
+pytestmark = pytest.mark.gpu
+
+OS = np.array([10, 10])
+MAX_SUPPORT = np.array([15, 15])
+
+
+def _dev(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return t if dtype is None else t.to(dtype)
+
+
+@pytest.fixture(scope="module")
+def case():
+    return mosaic_case()
+
+
+def _oracle_gcf(c, blockage, casa=True):
+    xds = c["mxds"].attrs["xds0"]
+    bl = np.stack([xds.ANTENNA1.values, xds.ANTENNA2.values], 1)
+    npad = np.array([c["npad"], c["npad"]])
+    g = oracle.make_gcf(xds.coords["chan"].values, xds.dims["pol"], c["types"], bl, c["dish"],
+                        blockage, c["delta"], npad, OS, MAX_SUPPORT, casa=casa)
+    fpd = np.asarray(c["mxds"].attrs["FIELD"]["PHASE_DIR"].values)[:, 0, :]
+    g["PHASE_GRADIENT"] = oracle.phase_gradient(fpd, c["phase_center"], c["delta"], npad, OS,
+                                                (MAX_SUPPORT + 1) * OS)
+    return g
+
+
+def _parms(c, blockage, function="casa_airy"):
+    gcf_parms = dict(function=function, list_dish_diameters=list(c["dish"]),
+                     list_blockage_diameters=list(blockage), unique_ant_indx=list(c["types"]),
+                     phase_center=c["phase_center"], oversampling=OS, max_support=MAX_SUPPORT)
+    grid_parms = dict(image_size=[c["image_size"]] * 2, cell_size=[c["delta"][1] / (np.pi / 648000)] * 2,
+                      fft_padding=1.2, chan_mode="cube", phase_center=c["phase_center"])
+    return gcf_parms, grid_parms
+
+
+@pytest.mark.parametrize("function,blockage", [("casa_airy", (0.75, 0.75)), ("airy", (0.75, 0.75)),
+                                               ("casa_airy", (0.0, 0.0))])
+def test_gcf_matches_oracle(gpu, case, function, blockage):
+    from cngi_prototype_amd.imaging import make_gridding_convolution_function
+    gcf_parms, grid_parms = _parms(case, blockage, function)
+    gcf = make_gridding_convolution_function(case["mxds"], gcf_parms, grid_parms, {"xds": "xds0"})
+    ref = _oracle_gcf(case, np.array(blockage), casa=function == "casa_airy")
+    assert np.array_equal(np.asarray(gcf["SUPPORT"].values), ref["SUPPORT"])
+    for k in ("CF_BASELINE_MAP", "CF_CHAN_MAP", "CF_POL_MAP"):
+        assert np.array_equal(np.asarray(gcf[k].values), ref[k]), k
+    for k in ("CONV_KERNEL", "WEIGHT_CONV_KERNEL", "PHASE_GRADIENT"):
+        got = gcf[k].data.cpu().numpy()
+        assert got.shape == ref[k].shape, k
+        assert rel_err(got, ref[k]) < 1e-11, k
+    assert np.allclose(gcf.attrs["oversampling"], OS)
+
+
+@pytest.mark.parametrize("mode", ["image", "psf", "weight"])
+@pytest.mark.parametrize("chan_mode", ["cube", "continuum"])
+def test_aperture_grid_matches_oracle(gpu, case, mode, chan_mode):
+    import torch
+    import cngi_prototype_amd._native as nat
+    ref = _oracle_gcf(case, np.array([0.75, 0.75]))
+    xds = case["mxds"].attrs["xds0"]
+    vis = np.asarray(xds.DATA.values)
+    w = np.asarray(xds.DATA_WEIGHT.values)
+    uvw = np.asarray(xds.UVW.values)
+    field = np.asarray(xds.FIELD_ID.values)
+    fid = np.arange(len(case["mxds"].attrs["FIELD"].coords["field_id"].values))
+    fr = xds.coords["chan"].values
+    npad = [case["npad"]] * 2
+    kname = "WEIGHT_CONV_KERNEL" if mode == "weight" else "CONV_KERNEL"
+    if mode == "weight":
+        g_ref, sw_ref = oracle.aperture_weight_grid(
+            uvw, w, field, fid, ref["CF_BASELINE_MAP"], ref["CF_CHAN_MAP"], ref["CF_POL_MAP"],
+            ref[kname], ref["SUPPORT"], ref["PHASE_GRADIENT"], fr, npad, case["delta"], OS,
+            chan_mode=chan_mode)
+    else:
+        g_ref, sw_ref = oracle.aperture_grid(
+            vis, uvw, w, field, fid, ref["CF_BASELINE_MAP"], ref["CF_CHAN_MAP"],
+            ref["CF_POL_MAP"], ref[kname], ref["SUPPORT"], ref["PHASE_GRADIENT"], fr, npad,
+            case["delta"], OS, do_psf=(mode == "psf"), chan_mode=chan_mode)
+    n_chan, n_pol = w.shape[2:]
+    cmap = np.arange(n_chan) if chan_mode == "cube" else np.zeros(n_chan, dtype=int)
+    n_imag = n_chan if chan_mode == "cube" else 1
+    native_mode = {"image": nat.GRID_IMAGE, "psf": nat.GRID_PSF, "weight": nat.GRID_WEIGHT}[mode]
+    g, sw = nat.aperture_grid(
+        _dev(uvw), _dev(w), _dev(field, torch.int32), fid, fr, cmap, np.arange(n_pol),
+        ref["CF_BASELINE_MAP"], ref["CF_CHAN_MAP"], ref["CF_POL_MAP"], _dev(ref[kname]),
+        ref["SUPPORT"], _dev(ref["PHASE_GRADIENT"]), n_imag, npad, case["delta"], OS,
+        native_mode, "fp64", vis=_dev(vis) if mode == "image" else None)
+    assert rel_err(g.cpu().numpy(), g_ref) < 1e-12
+    assert rel_err(sw.cpu().numpy(), sw_ref) < 1e-12
+
+
+def test_aperture_grid_unknown_field_raises(gpu, case):
+    import torch
+    import cngi_prototype_amd._native as nat
+    ref = _oracle_gcf(case, np.array([0.75, 0.75]))
+    xds = case["mxds"].attrs["xds0"]
+    w = np.asarray(xds.DATA_WEIGHT.values)
+    field = np.array(xds.FIELD_ID.values, copy=True)
+    field[3, 5] = 99
+    n_chan, n_pol = w.shape[2:]
+    with pytest.raises(IndexError):
+        nat.aperture_grid(
+            _dev(xds.UVW.values), _dev(w), _dev(field, torch.int32), np.arange(4),
+            xds.coords["chan"].values, np.arange(n_chan), np.arange(n_pol),
+            ref["CF_BASELINE_MAP"], ref["CF_CHAN_MAP"], ref["CF_POL_MAP"],
+            _dev(ref["CONV_KERNEL"]), ref["SUPPORT"], _dev(ref["PHASE_GRADIENT"]), n_chan,
+            [case["npad"]] * 2, case["delta"], OS, nat.GRID_PSF, "fp64")
+    with pytest.raises(IndexError):
+        oracle.aperture_grid(None, xds.UVW.values, w, field, np.arange(4),
+                             ref["CF_BASELINE_MAP"], ref["CF_CHAN_MAP"], ref["CF_POL_MAP"],
+                             ref["CONV_KERNEL"], ref["SUPPORT"], ref["PHASE_GRADIENT"],
+                             xds.coords["chan"].values, [case["npad"]] * 2, case["delta"], OS,
+                             do_psf=True)
+
+
+@pytest.mark.parametrize("norm_type,single", [("flat_sky", True), ("flat_noise", False),
+                                              ("none", False)])
+def test_mosaic_chain_matches_oracle(gpu, case, norm_type, single):
+    from cngi_prototype_amd.imaging import (make_gridding_convolution_function,
+                                            make_image_with_gcf, make_imaging_weight,
+                                            make_mosaic_pb, make_psf_with_gcf)
+    from cngi_prototype_amd.xds import Dataset
+    blockage = (0.75, 0.75)
+    gcf_parms, grid_parms = _parms(case, blockage)
+    mxds = make_imaging_weight(case["mxds"], {"weighting": "natural"}, {}, {"xds": "xds0"})
+    vsel = {"xds": "xds0"}
+    gcf = make_gridding_convolution_function(mxds, gcf_parms, grid_parms, vsel)
+    img = make_mosaic_pb(mxds, gcf, Dataset(), vsel, {}, grid_parms)
+    norm = dict(norm_type=norm_type, single_precision=single, pb_limit=0.2)
+    img = make_image_with_gcf(mxds, gcf, img, grid_parms, norm, vsel, {})
+    img = make_psf_with_gcf(mxds, gcf, img, grid_parms, norm, vsel, {})
+
+    # oracle chain
+    ref = _oracle_gcf(case, np.array(blockage))
+    xds = mxds.attrs["xds0"]
+    uvw, vis = xds.UVW.values, xds.DATA.values
+    w = np.asarray(xds.DATA_WEIGHT.values)
+    field, fid = xds.FIELD_ID.values, np.arange(4)
+    fr = xds.coords["chan"].values
+    N, npad = case["image_size"], [case["npad"]] * 2
+    maps = (ref["CF_BASELINE_MAP"], ref["CF_CHAN_MAP"], ref["CF_POL_MAP"])
+    wg, wsw = oracle.aperture_weight_grid(uvw, w, field, fid, *maps, ref["WEIGHT_CONV_KERNEL"],
+                                          ref["SUPPORT"], ref["PHASE_GRADIENT"], fr, npad,
+                                          case["delta"], OS)
+    swc = wsw.copy()
+    swc[swc == 0] = 1
+    weight_pb = oracle.grid_to_image(wg, [N, N]) / swc[None, None]
+    pb = np.sqrt(np.abs(weight_pb))
+    out = {}
+    for psf in (False, True):
+        g, sw = oracle.aperture_grid(vis, uvw, w, field, fid, *maps, ref["CONV_KERNEL"],
+                                     ref["SUPPORT"], ref["PHASE_GRADIENT"], fr, npad,
+                                     case["delta"], OS, do_psf=psf)
+        im = oracle.normalize_mosaic(oracle.grid_to_image(g, [N, N]), sw, np.ones((N, N)), pb,
+                                     weight_pb, OS, norm_type, 0.2, single)
+        if psf:
+            im = im / im[N // 2:N // 2 + 1, N // 2:N // 2 + 1]
+        out[psf] = (im, sw)
+    tol = 1e-6 if single else 1e-10
+    assert rel_err(np.asarray(img["WEIGHT_PB"].values)[:, :, 0], weight_pb) < 1e-10
+    assert rel_err(np.asarray(img["PB"].values)[:, :, 0], pb) < 1e-10
+    assert rel_err(np.asarray(img["WEIGHT_PB_SUM_WEIGHT"].values)[0], wsw) < 1e-12
+    assert rel_err(np.asarray(img["IMAGE"].values)[:, :, 0], out[False][0]) < tol
+    assert rel_err(np.asarray(img["SUM_WEIGHT"].values)[0], out[False][1]) < 1e-12
+    assert rel_err(np.asarray(img["PSF"].values)[:, :, 0], out[True][0]) < tol
+    assert rel_err(np.asarray(img["PSF_SUM_WEIGHT"].values)[0], out[True][1]) < 1e-12
+    assert "PSF_FIT" in img
