@@ -159,7 +159,7 @@ def dish_types(cfg):
 
 def mosaic_fields(cfg, phase_dir=None):
     """Pointing centres [n_field, 2] (ra, dec radians) on a hexagonal grid
-    spaced by lambda/(2 D_max) at the lowest frequency, centred on the
+    spaced by lambda/(2 D_max) at the lowest frequency, whose mean is the
     phase centre (rows of 5 for 25 fields)."""
     n_field = int(cfg.extra.get("n_field", 1))
     ra0, dec0 = phase_center(cfg) if phase_dir is None else phase_dir
@@ -172,15 +172,15 @@ def mosaic_fields(cfg, phase_dir=None):
             pts.append(((c - (side - 1) / 2 + (0.5 if r % 2 else 0.0) - 0.25) * step,
                         (r - (side - 1) / 2) * step * np.sqrt(3) / 2))
     pts = np.array(pts[:n_field])
+    pts -= pts.mean(axis=0)
     return np.stack([ra0 + pts[:, 0] / np.cos(dec0), dec0 + pts[:, 1]], 1)
 
 
 def field_schedule(cfg):
-    """FIELD_ID of every integration: the fields are visited in turn, an
-    equal number of dumps each (the remainder spread over the first)."""
+    """FIELD_ID of every integration: the fields are visited in turn, in
+    contiguous blocks whose lengths differ by at most one dump."""
     n_field = int(cfg.extra.get("n_field", 1))
-    per = max(1, cfg.n_time // n_field)
-    return np.minimum(np.arange(cfg.n_time) // per, n_field - 1).astype(np.int32)
+    return ((np.arange(cfg.n_time) * n_field) // cfg.n_time).astype(np.int32)
 
 
 def airy_voltage(r_rad, freq, dish):

@@ -1,0 +1,633 @@
+"""Pins of the oracle (CPU, no GPU).
+
+The reference ships no golden vectors for the gridding path, so each oracle
+function is checked here against something that does not restate its own
+formula: the in-source SIN known-answer case (tests/golden), the eigen-relation
+that defines the prolate spheroidal function, brute force in another
+formulation, and closed forms (footprints, sums, point sources, Briggs
+limits, annular-aperture Hankel transforms, hand-computed integer cases).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+C = oracle.C_LIGHT
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    oracle.build()
+
+
+# ---------------------------------------------------------------------------
+# prolate spheroidal function and tables
+# ---------------------------------------------------------------------------
+def test_ps_known_value_at_origin():
+    """Schwab's SPHFN (m = 6, alpha = 1) at 0 is 0.99999967 to 8 digits."""
+    g, c = oracle.prolate_spheroidal_function(np.array([0.0]))
+    assert abs(g[0] - 0.99999967) < 5e-9
+    assert c[0] == g[0]
+
+
+def test_ps_eigen_relation():
+    """The spheroidal function is its own Fourier transform under the weight
+    (1 - eta^2)^alpha: lambda psi(x) = int_-1^1 (1-eta^2) psi(eta)
+    exp(i c x eta) d eta with c = pi m / 2.  A wrong coefficient or branch
+    breaks this at the 1e-3 level; the rational fit holds it to ~2e-6."""
+    cpar = np.pi * 6 / 2
+    eta = np.linspace(-1, 1, 100001)
+    g, _ = oracle.prolate_spheroidal_function(np.abs(eta))
+    kern = (1 - eta ** 2) * g
+    xs = np.linspace(0.0, 0.9, 19)
+    ft = np.array([np.trapezoid(kern * np.cos(cpar * x * eta), eta) for x in xs])
+    gx, _ = oracle.prolate_spheroidal_function(xs)
+    ratio = ft / gx
+    assert np.max(np.abs(ratio / ratio[0] - 1)) < 1e-5
+
+
+def test_ps_kernel_table_layout():
+    """cgk_1D: os*(S//2) samples of (1-u^2) psi(u) at u = k/(S//2*os), then
+    a zero tail of os samples (so every tap index up to os*(S//2)+os/2 is
+    defined)."""
+    k = oracle.ps_kernel_1d(100, 7)
+    assert k.shape == (400,)
+    assert np.all(k[300:] == 0)
+    assert abs(k[0] - 0.99999967) < 5e-9
+    assert np.all(np.diff(k[:300]) < 0)
+    # half-way sample: u = 0.5
+    g, cf = oracle.prolate_spheroidal_function(np.array([0.5]))
+    assert k[150] == cf[0]
+
+
+def test_correction_image_matches_product_factors():
+    """The GPU path's separable correction factors equal the oracle's
+    correction image, cropped, as an outer product."""
+    from cngi_prototype_amd.imaging._imaging_utils._gridding_convolutional_kernels import \
+        cropped_correction_factors
+    for npad, n in (([614, 614], [512, 512]), ([99, 120], [80, 101])):
+        img = oracle.remove_padding(oracle.ps_correction_image(npad), n)
+        cu, cv = cropped_correction_factors(npad, n)
+        assert np.array_equal(np.outer(cu, cv), img)
+        assert img[n[0] // 2, n[1] // 2] == oracle.prolate_spheroidal_function(
+            np.array([0.0]))[0][0] ** 2
+
+
+# ---------------------------------------------------------------------------
+# standard gridder: brute force by cell distance, closed forms
+# ---------------------------------------------------------------------------
+def _brute_standard(vis, uvw, w, freq, cgk, n_uv, delta, support, os_, mode):
+    """Per-cell formulation: a cell x takes the kernel at the nearest
+    oversampled distance |floor((x - u_pos) os + 0.5)| from the sample, for
+    the S cells nearest int(u_pos + 0.5).  mode 'psf', 'image' or
+    'density' (conjugate point too, w averaged over the first two pols)."""
+    T, B, NC, P = w.shape
+    n0, n1 = n_uv
+    grid = np.zeros((NC, P, n0, n1), dtype=np.complex128)
+    sw = np.zeros((NC, P))
+    half = support // 2
+    flat = grid.reshape(-1)
+    for t in range(T):
+        for b in range(B):
+            for c in range(NC):
+                su = -(freq[c] * delta[0] * n0) / C
+                sv = -(freq[c] * delta[1] * n1) / C
+                u, v = uvw[t, b, 0] * su, uvw[t, b, 1] * sv
+                if math.isnan(u) or math.isnan(v):
+                    continue
+                up, vp = u + n0 // 2, v + n1 // 2
+                u0, v0 = int(up + 0.5), int(vp + 0.5)
+                if not (u0 - half >= 0 and v0 - half >= 0 and u0 + half < n0 and v0 + half < n1):
+                    continue
+                uc, vc = int(-u + n0 // 2 + 0.5), int(-v + n1 // 2 + 0.5)
+                for p in range(P):
+                    if mode == "density" and P >= 2:
+                        wd = (w[t, b, c, 0] + w[t, b, c, 1]) / 2
+                        sel = wd
+                    elif mode == "image":
+                        sel = w[t, b, c, p]
+                        wd = vis[t, b, c, p] * sel
+                    else:
+                        sel = wd = w[t, b, c, p]
+                    if np.isnan(wd) or wd == 0:
+                        continue
+                    norm = 0.0
+                    for x in range(u0 - half, u0 - half + support):
+                        ku = abs(math.floor((x - up) * os_ + 0.5)) if os_ else 0
+                        for y in range(v0 - half, v0 - half + support):
+                            kv = abs(math.floor((y - vp) * os_ + 0.5)) if os_ else 0
+                            tap = cgk[ku] * cgk[kv]
+                            grid[c, p, x, y] += tap * wd
+                            norm += tap
+                            if mode == "density":
+                                # the reference does not bounds-check the
+                                # conjugate cell: C-order flat offset
+                                off = ((c * P + p) * n0 + uc + (x - u0)) * n1 + vc + (y - v0)
+                                if 0 <= off < flat.size:
+                                    flat[off] += tap * wd
+                    sw[c, p] += sel * norm * (2 if mode == "density" else 1)
+    return grid, sw
+
+
+def _tiny_case(seed, T=5, B=9, NC=2, P=2, n=40, fill=0.45, nan_frac=0.1):
+    rng = np.random.default_rng(seed)
+    freq = 1.0e9 + np.arange(NC) * 3e7
+    # |u| <= 700 m: the sample positions reach fill * n/2 cells from the centre
+    cell = fill * 0.98 / (2 * 700.0 * freq.max() / C)
+    uvw = rng.uniform(-700, 700, size=(T, B, 3))
+    uvw[0, 0, 0] = np.nan
+    vis = rng.standard_normal((T, B, NC, P)) + 1j * rng.standard_normal((T, B, NC, P))
+    w = rng.uniform(0.2, 2.0, size=(T, B, NC, P))
+    w[rng.uniform(size=w.shape) < nan_frac] = np.nan
+    w[1, 1, 0, 0] = 0.0
+    return vis, uvw, w, freq, np.array([-cell, cell])
+
+
+@pytest.mark.parametrize("mode", ["psf", "image"])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_standard_grid_brute_force(mode, seed):
+    vis, uvw, w, freq, delta = _tiny_case(seed, fill=0.55)
+    cgk = oracle.ps_kernel_1d(100, 7)
+    n = [40, 36]
+    g, sw = oracle.standard_grid(vis, uvw, w, freq, cgk, n, delta, 7, 100, do_psf=mode == "psf",
+                                 complex_grid=True)
+    gb, swb = _brute_standard(vis, uvw, w, freq, cgk, n, delta, 7, 100, mode)
+    assert np.abs(swb).sum() > 0
+    assert np.allclose(g, gb, rtol=0, atol=1e-12 * np.abs(gb).max())
+    assert np.allclose(sw, swb, rtol=1e-13)
+
+
+@pytest.mark.parametrize("seed", [0, 3])
+def test_density_grid_brute_force(seed):
+    """Nearest-cell density gridding with the separately rounded conjugate
+    cell; some conjugate cells fall on index n (the reference's unchecked
+    write) and land one row further in C order."""
+    vis, uvw, w, freq, delta = _tiny_case(seed, fill=1.0, T=8)
+    n = [24, 24]
+    g, sw = oracle.standard_grid(None, uvw, w, freq, np.ones(1), n, delta, 1, 0, do_psf=True,
+                                 do_imaging_weight=True)
+    gb, swb = _brute_standard(None, uvw, w, freq, np.ones(1), n, delta, 1, 0, "density")
+    assert np.allclose(g, gb.real, rtol=0, atol=1e-13)
+    assert np.allclose(sw, swb, rtol=1e-14)
+
+
+def test_single_visibility_footprint_is_outer_product():
+    """One sample: the 7x7 footprint is cgk(|os i + du|) cgk(|os j + dv|) w,
+    and sum_weight = w * sum of the footprint (PSF mode: grid sum =
+    sum_weight)."""
+    cgk = oracle.ps_kernel_1d(100, 7)
+    n = [32, 32]
+    freq = np.array([1.4e9])
+    cell = 1e-3
+    u_cells, v_cells = 3.27, -5.61           # position relative to the centre, cells
+    uvw = np.zeros((1, 1, 3))
+    uvw[0, 0, 0] = -u_cells * C / (freq[0] * -cell * n[0])
+    uvw[0, 0, 1] = -v_cells * C / (freq[0] * cell * n[1])
+    w = np.full((1, 1, 1, 1), 1.7)
+    g, sw = oracle.standard_grid(None, uvw, w, freq, cgk, n, [-cell, cell], 7, 100, do_psf=True)
+    up, vp = 16 + u_cells, 16 + v_cells
+    u0, v0 = int(up + 0.5), int(vp + 0.5)
+    du = math.floor((u0 - up) * 100 + 0.5)
+    dv = math.floor((v0 - vp) * 100 + 0.5)
+    tu = np.array([cgk[abs(100 * i + du)] for i in range(-3, 4)])
+    tv = np.array([cgk[abs(100 * j + dv)] for j in range(-3, 4)])
+    expect = np.zeros(n)
+    expect[u0 - 3:u0 + 4, v0 - 3:v0 + 4] = 1.7 * np.outer(tu, tv)
+    assert np.allclose(g[0, 0], expect, rtol=1e-14, atol=0)
+    assert np.isclose(sw[0, 0], g.sum(), rtol=1e-13)
+
+
+def test_psf_grid_sum_equals_sum_weight():
+    vis, uvw, w, freq, delta = _tiny_case(5, T=10, fill=0.5)
+    cgk = oracle.ps_kernel_1d(100, 7)
+    g, sw = oracle.standard_grid(None, uvw, w, freq, cgk, [48, 48], delta, 7, 100, do_psf=True)
+    assert np.allclose(g.sum(axis=(2, 3)), sw, rtol=1e-12)
+
+
+def _point_source_case(l_cells, m_cells, n=64, npad=76):
+    """Visibilities of a 1 Jy point source at direction cosines (l, m) =
+    (l_cells, m_cells) * cell with V = exp(+2 pi i (u l + v m)), on a dense
+    random uv coverage."""
+    rng = np.random.default_rng(11)
+    freq = np.array([1.4e9])
+    cell = 20 * np.pi / (180 * 3600)
+    lam = C / freq[0]
+    umax = 0.45 / cell                     # wavelengths, inside the padded grid
+    T, B = 40, 50
+    uvw = np.zeros((T, B, 3))
+    uvw[..., :2] = rng.uniform(-umax, umax, size=(T, B, 2)) * lam
+    l, m = l_cells * cell, m_cells * cell
+    u = uvw[..., 0] / lam
+    v = uvw[..., 1] / lam
+    vis = np.exp(2j * np.pi * (u * l + v * m))[..., None, None]
+    w = np.ones(vis.shape)
+    return vis, uvw, w, freq, np.array([-cell, cell]), n, npad
+
+
+def test_point_source_lands_on_its_sky_pixel():
+    """A source at (l, m) appears at the image pixel whose SIN sky
+    coordinates (make_empty_sky_image) are the source's, with peak flux ~1
+    after the sum-of-weights and grdsf corrections."""
+    from cngi_prototype_amd.image import make_empty_sky_image
+    from cngi_prototype_amd.xds import Dataset
+    vis, uvw, w, freq, delta, n, npad = _point_source_case(5.0, -9.0)
+    cgk = oracle.ps_kernel_1d(100, 7)
+    g, sw = oracle.standard_grid(vis, uvw, w, freq, cgk, [npad, npad], delta, 7, 100,
+                                 do_psf=False)
+    img = oracle.correct_standard_image(oracle.grid_to_image(g, [n, n]), sw, [npad, npad], [n, n])
+    pk = np.unravel_index(np.argmax(img[:, :, 0, 0]), (n, n))
+    assert abs(img[pk][0, 0] - 1.0) < 2e-3
+    pc = np.array([0.3, -0.4])
+    xds = make_empty_sky_image(Dataset(), pc, [n, n], delta, [1.4e9], [1e6], [9], [0.0])
+    ra = np.asarray(xds.coords["right_ascension"].values)[pk]
+    dec = np.asarray(xds.coords["declination"].values)[pk]
+    # the pixel's direction cosines about the phase centre are the source's
+    l_pix = np.cos(dec) * np.sin(ra - pc[0])
+    m_pix = np.sin(dec) * np.cos(pc[1]) - np.cos(dec) * np.sin(pc[1]) * np.cos(ra - pc[0])
+    assert np.isclose(l_pix, 5.0 * delta[1], rtol=1e-9)
+    assert np.isclose(m_pix, -9.0 * delta[1], rtol=1e-9)
+
+
+def test_psf_peak_is_inverse_grdsf_squared():
+    """PSF centre = (sum grid / sum_weight) / (grdsf(0)^2) = 1/grdsf(0)^2 for
+    any coverage (the correction image is an outer product of two grdsf
+    factors)."""
+    vis, uvw, w, freq, delta, n, npad = _point_source_case(0, 0)
+    cgk = oracle.ps_kernel_1d(100, 7)
+    g, sw = oracle.standard_grid(None, uvw, w, freq, cgk, [npad, npad], delta, 7, 100,
+                                 do_psf=True)
+    psf = oracle.correct_standard_image(oracle.grid_to_image(g, [n, n]), sw, [npad, npad], [n, n])
+    g0 = oracle.prolate_spheroidal_function(np.array([0.0]))[0][0]
+    assert np.isclose(psf[n // 2, n // 2, 0, 0], 1 / g0 ** 2, rtol=1e-12)
+
+
+def test_image_of_hermitian_data_is_real():
+    """Gridding V at (u, v) together with V* at (-u, -v) gives a Hermitian
+    grid about the centre, whose transform is real."""
+    rng = np.random.default_rng(2)
+    freq = np.array([1.0e9])
+    T, B = 6, 8
+    uvw = rng.uniform(-500, 500, size=(T, B, 3))
+    uvw2 = np.concatenate([uvw, -uvw], axis=1)
+    v1 = rng.standard_normal((T, B, 1, 1)) + 1j * rng.standard_normal((T, B, 1, 1))
+    vis = np.concatenate([v1, np.conj(v1)], axis=1)
+    w = np.ones(vis.shape)
+    n = 64
+    cell = 0.8 / (2 * 600.0 * freq[0] / C)
+    cgk = oracle.ps_kernel_1d(100, 7)
+    g, _ = oracle.standard_grid(vis, uvw2, w, freq, cgk, [n, n], [-cell, cell], 7, 100,
+                                do_psf=False)
+    gg = np.moveaxis(g, [0, 1], [-2, -1])[:, :, 0, 0]
+    img = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(gg)))
+    assert np.abs(img.imag).max() < 1e-12 * np.abs(img.real).max()
+
+
+# ---------------------------------------------------------------------------
+# Briggs weighting
+# ---------------------------------------------------------------------------
+def _half_plane_case(seed=4):
+    """Samples only at u > 0, so no conjugate cell coincides with a direct
+    one."""
+    rng = np.random.default_rng(seed)
+    freq = np.array([1.0e9, 1.02e9])
+    T, B = 12, 15
+    uvw = rng.uniform(-800, 800, size=(T, B, 3))
+    uvw[..., 0] = np.abs(uvw[..., 0]) + 60
+    w = rng.uniform(0.5, 2.0, size=(T, B, 2, 1))
+    n = 32
+    cell = 1.0 / (2 * 900.0 * freq.max() / C)
+    return uvw, w, freq, np.array([-cell, cell]), n
+
+
+def _briggs(uvw, w, freq, delta, n, weighting, robust):
+    g, sw = oracle.standard_grid(None, uvw, w, freq, np.ones(1), [n, n], delta, 1, 0, do_psf=True,
+                                 do_imaging_weight=True)
+    gi = np.moveaxis(g, [0, 1], [-2, -1])
+    f = oracle.briggs_factors(gi, sw, weighting, robust)
+    return oracle.imaging_weight_degrid(gi, uvw, w, f, freq, [n, n], delta), g, f
+
+
+def test_uniform_weights_sum_to_one_per_cell():
+    uvw, w, freq, delta, n = _half_plane_case()
+    iw, g, f = _briggs(uvw, w, freq, delta, n, "uniform", 0)
+    assert np.all(f[0] == 1) and np.all(f[1] == 0)
+    for c in range(2):
+        su = -(freq[c] * delta[0] * n) / C
+        sv = -(freq[c] * delta[1] * n) / C
+        u0 = (uvw[..., 0] * su + n // 2 + 0.5).astype(int)
+        v0 = (uvw[..., 1] * sv + n // 2 + 0.5).astype(int)
+        acc = np.zeros((n, n))
+        np.add.at(acc, (u0.ravel(), v0.ravel()), iw[:, :, c, 0].ravel())
+        occupied = acc > 0
+        assert np.allclose(acc[occupied], 1.0, rtol=1e-12)
+
+
+def test_briggs_robust_limits():
+    """robust = +2 tends to natural weights, robust = -2 to uniform ones
+    (Briggs 1995): w_img = w / (f0 g + 1) with f0 = (5 10^-R)^2 / (sum g^2 /
+    sum w)."""
+    uvw, w, freq, delta, n = _half_plane_case()
+    nat2, _, _ = _briggs(uvw, w, freq, delta, n, "briggs", 2.0)
+    assert np.max(np.abs(nat2 / w - 1)) < 0.05
+    uni, _, _ = _briggs(uvw, w, freq, delta, n, "uniform", 0)
+    bm2, _, f = _briggs(uvw, w, freq, delta, n, "briggs", -2.0)
+    assert np.max(np.abs(bm2 * f[0][None, None] / uni - 1)) < 0.05
+
+
+def test_degrid_off_grid_and_two_pol_mean():
+    """Off-grid samples get imaging weight 0; with two pols both pols get
+    the mean natural weight divided by the density (which was built from the
+    same mean), so the pols agree and, in uniform mode, sum to 1 per cell."""
+    uvw, w, freq, delta, n = _half_plane_case()
+    uvw = uvw.copy()
+    uvw[0, 0, :2] = 1e7
+    w2 = np.concatenate([w, 2 * w], axis=3)
+    iw, g, f = _briggs(uvw, w2, freq, delta, n, "uniform", 0)
+    assert np.all(iw[0, 0] == 0)
+    assert np.array_equal(iw[..., 1], iw[..., 0])
+    n_uv = np.array([n, n])
+    su = -(freq[0] * delta[0] * n) / C
+    sv = -(freq[0] * delta[1] * n) / C
+    u0 = (uvw[..., 0] * su + n // 2 + 0.5).astype(int)
+    v0 = (uvw[..., 1] * sv + n // 2 + 0.5).astype(int)
+    inside = (u0 >= 0) & (u0 < n_uv[0]) & (v0 >= 0) & (v0 < n_uv[1])
+    acc = np.zeros((n, n))
+    np.add.at(acc, (u0[inside], v0[inside]), iw[:, :, 0, 0][inside])
+    assert np.allclose(acc[acc > 0], 1.0, rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# aperture gridders
+# ---------------------------------------------------------------------------
+def _separable_ps_kernel(os_, K):
+    """CK[k_u, k_v] = cgk(|k_u - K//2|) cgk(|k_v - K//2|) from the PS table."""
+    cgk = oracle.ps_kernel_1d(os_, 7)
+    idx = np.abs(np.arange(K) - K // 2)
+    k1 = np.where(idx < len(cgk), cgk[np.minimum(idx, len(cgk) - 1)], 0.0)
+    return np.outer(k1, k1)
+
+
+@pytest.mark.parametrize("do_psf", [True, False])
+def test_aperture_grid_reduces_to_standard(do_psf):
+    """With a separable prolate-spheroidal CK, unit phase gradient and
+    SUPPORT = 7 the aperture gridder's taps (os i + floor(offset os + 0.5) +
+    K//2) are the standard gridder's: the grids agree, and so does the PSF
+    sum of weights (the image-mode one is w Re(norm^2) instead)."""
+    vis, uvw, w, freq, delta = _tiny_case(7, T=6, fill=0.35)
+    os_ = 100
+    K = 800
+    ck = _separable_ps_kernel(os_, K)[None, None, None]
+    T, B, NC, P = w.shape
+    sup = np.full((1, 1, 1, 2), 7)
+    pg = np.ones((1, K, K), dtype=np.complex128)
+    field = np.zeros((T, B), dtype=int)
+    n = [44, 44]
+    ga, swa = oracle.aperture_grid(vis, uvw, w, field, [0], np.zeros(B, int), np.zeros(NC, int),
+                                   np.zeros(P, int), ck, sup, pg, freq, n, delta, [os_, os_],
+                                   do_psf=do_psf)
+    gs, sws = oracle.standard_grid(vis, uvw, w, freq, oracle.ps_kernel_1d(os_, 7), n, delta, 7,
+                                   os_, do_psf=do_psf, complex_grid=True)
+    assert np.abs(gs).max() > 0
+    assert np.allclose(ga, gs, rtol=0, atol=1e-13 * np.abs(gs).max())
+    if do_psf:
+        assert np.allclose(swa, sws, rtol=1e-13)
+
+
+def test_aperture_weight_grid_closed_form():
+    """The weight gridder puts every sample's taps at the grid centre: grid
+    = (sum of valid w) * CK*PG at os i + K//2, sum_weight = W Re(sum taps)."""
+    rng = np.random.default_rng(8)
+    T, B, NC, P = 4, 6, 2, 1
+    K, os_, S = 60, 5, 5
+    ck = rng.uniform(0.1, 1.0, size=(1, 1, 1, K, K))
+    pg = np.exp(1j * rng.uniform(0, 2 * np.pi, size=(1, K, K)))
+    uvw = rng.uniform(-300, 300, size=(T, B, 3))
+    uvw[0, 1, 1] = np.nan
+    w = rng.uniform(0.5, 2, size=(T, B, NC, P))
+    w[2, 2, 1, 0] = np.nan
+    freq = np.array([1e9, 1.1e9])
+    n = [40, 40]
+    delta = np.array([-1e-4, 1e-4])
+    g, sw = oracle.aperture_weight_grid(uvw, w, np.zeros((T, B), int), [0], np.zeros(B, int),
+                                        np.zeros(NC, int), np.zeros(P, int), ck,
+                                        np.full((1, 1, 1, 2), S), pg, freq, n, delta, [os_, os_])
+    taps = (ck[0, 0, 0] * pg[0])[[os_ * i + K // 2 for i in range(-(S // 2), S - S // 2)]][
+        :, [os_ * j + K // 2 for j in range(-(S // 2), S - S // 2)]]
+    for c in range(NC):
+        valid = ~np.isnan(w[:, :, c, 0]) & ~np.isnan(uvw[:, :, 1])
+        W = w[:, :, c, 0][valid].sum()
+        expect = np.zeros(n, dtype=np.complex128)
+        expect[20 - S // 2:20 - S // 2 + S, 20 - S // 2:20 - S // 2 + S] = W * taps
+        assert np.allclose(g[c, 0], expect, rtol=1e-12, atol=0)
+        assert np.isclose(sw[c, 0], W * taps.sum().real, rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# A-term GCF
+# ---------------------------------------------------------------------------
+def test_airy_pattern_is_annulus_transform():
+    """The 'airy' branch (blockage b) is the normalised far field of an
+    annular aperture D/2 > rho > b/2: E(theta) = int J0(2 pi rho theta /
+    lambda) rho d rho / int rho d rho, evaluated here by quadrature."""
+    from scipy.special import j0
+    f = np.array([1.0e11])
+    D, b = 12.0, 0.75
+    n = 64
+    cell = np.array([-2e-6, 2e-6])
+    p = oracle.airy_rorder(f, 1, [D], [b], 1, cell, [n, n], [n // 2, n // 2], casa=False)
+    lam = C / f[0]
+    rho = np.linspace(b / 2, D / 2, 20001)
+    for (i, j) in ((32, 40), (37, 45), (20, 9), (32, 33)):
+        theta = math.hypot((i - 32) * cell[0], (j - 32) * cell[1])
+        e = np.trapezoid(j0(2 * np.pi * rho * theta / lam) * rho, rho) / np.trapezoid(rho, rho)
+        assert abs(p[0, 0, 0, i, j] - e) < 1e-8
+    assert p[0, 0, 0, 32, 32] == 1.0
+
+
+def test_airy_pattern_limits():
+    """CASA branch: centre 1, and with blockage -> 0 it tends to the
+    unblocked Airy pattern 2 J1(r)/r; ipower squares the pattern."""
+    from scipy.special import j1
+    f = np.array([2.3e11])
+    n = 48
+    cell = np.array([-1e-6, 1e-6])
+    c = [n // 2, n // 2]
+    p1 = oracle.airy_rorder(f, 2, [7.0], [0.0], 1, cell, [n, n], c)
+    p2 = oracle.airy_rorder(f, 2, [7.0], [0.0], 2, cell, [n, n], c)
+    pb = oracle.airy_rorder(f, 1, [7.0], [1e-4], 1, cell, [n, n], c)
+    x = np.arange(-c[0], n - c[0]) * cell[0]
+    y = np.arange(-c[1], n - c[1]) * cell[1]
+    r = np.hypot(x[:, None], y[None, :]) * (2 * np.pi * f[0] / C) * 7.0 / 2
+    r[c[0], c[1]] = 1.0
+    ref = 2 * j1(r) / r
+    ref[c[0], c[1]] = 1.0
+    assert np.allclose(p1[0, 0, 0], ref, rtol=1e-13, atol=1e-15)
+    assert np.allclose(p1[0, 0, 1], ref, rtol=1e-13, atol=1e-15)
+    assert np.allclose(p2[0, 0, 0], ref ** 2, rtol=1e-13, atol=1e-15)
+    assert np.allclose(pb[0, 0, 0], ref, atol=1e-6)
+
+
+def test_cf_maps_hand_cases():
+    """Dish-type pairs (i <= j) in row-major order; a baseline whose types
+    come reversed keeps the default pair 0.  Channel bins: n = int(floor(BW /
+    (f_max tol)) + 0.5), centres at min + (k + 1/2) BW/n, nearest centre."""
+    from cngi_prototype_amd.imaging._imaging_utils._gcf_maps import cf_baseline_map, cf_chan_map
+    types = np.array([0, 0, 1, 2])
+    bl = np.array([[0, 1], [0, 2], [1, 3], [2, 3], [2, 0], [3, 3]])
+    m, pairs = oracle.cf_baseline_map(types, bl, 3)
+    assert pairs.tolist() == [[0, 0], [0, 1], [0, 2], [1, 1], [1, 2], [2, 2]]
+    assert m.tolist() == [0, 1, 2, 4, 0, 5]
+    m2, pairs2 = cf_baseline_map(types, bl, 3)
+    assert np.array_equal(m, m2) and np.array_equal(pairs, pairs2)
+    # BW 1.8 GHz, f_max 101.8 GHz: 1.8 / 0.509 = 3.54 -> 3 bins centred at
+    # 100.3, 100.9 and 101.5 GHz (no channel equidistant from two centres)
+    fr = np.array([100.0, 100.1, 100.5, 100.65, 101.0, 101.25, 101.8]) * 1e9
+    m, pb = oracle.cf_chan_map(fr, 0.005)
+    assert len(pb) == 3
+    assert np.allclose(pb, 100e9 + 0.6e9 * (np.arange(3) + 0.5))
+    assert m.tolist() == [0, 0, 0, 1, 1, 2, 2]
+    m2, pb2 = cf_chan_map(fr, 0.005)
+    assert np.array_equal(m, m2) and np.allclose(pb, pb2, rtol=0, atol=0)
+    m, pb = oracle.cf_chan_map(fr, 0.5)              # one bin
+    assert len(pb) == 1 and np.all(m == 0)
+    m, pb = oracle.cf_chan_map(fr[:3], 1e-5)         # more bins than channels
+    assert np.array_equal(m, np.arange(3)) and np.array_equal(pb, fr[:3])
+
+
+def test_conv_size_hand_case():
+    """A Gaussian weight kernel of width sigma = 10 samples, os = 10: the
+    +x walk stops at the first sample <= 0.025 max, 28 samples out
+    (exp(-28^2/200) = 0.0198 < 0.025 < exp(-27^2/200) = 0.0261), so S =
+    (int(0.5 + 2.8) + 1) * 2 + 1 = 9; sigma 12 along y gives 34 -> S = 9
+    as well; the result is square at the larger one."""
+    from cngi_prototype_amd.imaging._imaging_utils._gcf_maps import conv_support
+    n = 128
+    x = np.arange(n) - n // 2
+    for sx, sy, expect in ((10, 10, 9), (10, 12, 9), (10, 16, 11)):
+        k = np.exp(-(x[:, None] ** 2) / (2 * sx ** 2) - (x[None, :] ** 2) / (2 * sy ** 2))
+        s = oracle.calc_conv_size(k, [n, n], 0.025, [10, 10], [15, 15])
+        assert s == [expect, expect]
+        s2 = conv_support(k[:, n // 2], k[n // 2, :], np.abs(k).max(), np.abs(k).min(), [n, n],
+                          0.025, [10, 10], [15, 15])
+        assert s2 == s
+    k = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / (2 * 30 ** 2))
+    with pytest.raises(AssertionError):
+        oracle.calc_conv_size(k, [n, n], 0.025, [10, 10], [15, 15])
+
+
+def test_gcf_kernels_normalised_over_support_window():
+    """After the crop, each kernel sums to os_u*os_v over its centre
+    (S+1)*os window, and the weight kernel's support follows the cut."""
+    fr = np.array([2.3e11])
+    types = np.array([0, 1])
+    bl = np.array([[0, 1], [0, 0], [1, 1]])
+    npad = np.array([192, 192])
+    cell = np.array([-0.3, 0.3]) * np.pi / (180 * 3600)
+    g = oracle.make_gcf(fr, 1, types, bl, [7.0, 12.0], [0.0, 0.75], cell, npad)
+    K = g["CONV_KERNEL"].shape[-1]
+    for key in ("CONV_KERNEL", "WEIGHT_CONV_KERNEL"):
+        for p in range(3):
+            s = g["SUPPORT"][p, 0, 0, 0]
+            e = (s + 1) * 10
+            st = K // 2 - e // 2
+            win = g[key][p, 0, 0, st:st + e, st:st + e]
+            assert np.isclose(win.sum(), 100.0, rtol=1e-12)
+    # bigger dishes, wider kernels: 7-7 < 7-12 < 12-12
+    s = g["SUPPORT"][:, 0, 0, 0]
+    assert s[0] <= s[1] <= s[2]
+
+
+def test_sin_projection_known_answer():
+    """casacore toPixelWCS case quoted in the reference source."""
+    with open(os.path.join(HERE, "golden", "sin_projection_kat.json")) as f:
+        kat = json.load(f)
+    d = np.pi / 180
+    ra, dec = np.array(kat["world_deg"]) * d
+    ra0, dec0 = np.array(kat["crval_deg"]) * d
+    cell = np.array(kat["cdelt_deg"]) * d
+    dx, dy = oracle.sin_world_to_pixel_offset(ra, dec, ra0, dec0, cell)
+    assert abs(kat["crpix"][0] + dx - kat["pixel"][0]) < 1e-9
+    assert abs(kat["crpix"][1] + dy - kat["pixel"][1]) < 1e-9
+    from cngi_prototype_amd.imaging._imaging_utils._gcf_maps import field_pixel_offsets
+    off = field_pixel_offsets([[ra, dec]], [ra0, dec0], cell)
+    assert np.allclose(off[0], [dx, dy], rtol=0, atol=1e-12)
+
+
+def test_sin_projection_round_trip():
+    """make_empty_sky_image's inverse SIN and the field offset are inverse
+    maps: a pixel's (ra, dec) projects back to its (l, m)/cell."""
+    from cngi_prototype_amd.image import make_empty_sky_image
+    from cngi_prototype_amd.xds import Dataset
+    pc = np.array([1.1, -0.9])
+    cell = np.array([-3e-5, 3e-5])
+    xds = make_empty_sky_image(Dataset(), pc, [40, 50], cell, [1e9], [1e6], [9], [0.0])
+    ra = np.asarray(xds.coords["right_ascension"].values)
+    dec = np.asarray(xds.coords["declination"].values)
+    assert ra[20, 25] == pc[0] and np.isclose(dec[20, 25], pc[1], rtol=0, atol=1e-15)
+    dx, dy = oracle.sin_world_to_pixel_offset(ra, dec, pc[0], pc[1], cell)
+    ii, jj = np.meshgrid(np.arange(40) - 20, np.arange(50) - 25, indexing="ij")
+    assert np.allclose(dx, ii, atol=1e-9) and np.allclose(dy, jj, atol=1e-9)
+
+
+def test_phase_gradient_properties():
+    """PG = exp(i (x px_u + y px_v)), px = -dpix 2 pi / (Npad os): unit
+    modulus, 1 at the kernel centre, separable, and a field dpix pixels off
+    winds the phase by -2 pi dpix / Npad over os kernel samples."""
+    pc = np.array([0.2, 0.5])
+    cell = np.array([-1e-5, 1e-5])
+    npad = np.array([200, 200])
+    os_ = np.array([10, 10])
+    K = np.array([160, 160])
+    fields = np.array([[pc[0], pc[1]], [pc[0] + 3e-4, pc[1] - 2e-4]])
+    pg = oracle.phase_gradient(fields, pc, cell, npad, os_, K)
+    assert pg.shape == (2, 160, 160)
+    assert np.allclose(np.abs(pg), 1, atol=1e-15)
+    assert np.all(pg[0] == 1)
+    assert pg[1, 80, 80] == 1
+    assert np.allclose(pg[1], np.outer(pg[1, :, 80], pg[1, 80, :]), atol=1e-14)
+    dx, dy = oracle.sin_world_to_pixel_offset(fields[1, 0], fields[1, 1], pc[0], pc[1], cell)
+    step = np.angle(pg[1, 90, 80] / pg[1, 80, 80])
+    expect = np.angle(np.exp(-1j * 2 * np.pi * dx / npad[0]))
+    assert np.isclose(step, expect, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# post-grid steps
+# ---------------------------------------------------------------------------
+def test_grid_to_image_of_single_cell():
+    """A single grid cell at the centre + (du, dv) transforms to a plane wave
+    exp(2 pi i (du l + dv m) / Npad) scaled so that its real part peaks at 1
+    at the image centre (the inverse FFT times Npad^2)."""
+    npad, n = 50, 40
+    g = np.zeros((1, 1, npad, npad), dtype=np.complex128)
+    du, dv = 3, -2
+    g[0, 0, npad // 2 + du, npad // 2 + dv] = 1.0
+    img = oracle.grid_to_image(g, [n, n])[:, :, 0, 0]
+    li = np.arange(n) - n // 2
+    expect = np.cos(2 * np.pi * (du * li[:, None] + dv * li[None, :]) / npad)
+    assert np.allclose(img, expect, atol=1e-13)
+
+
+def test_normalize_mosaic_closed_form():
+    """flat_sky: (I / sw) / (sinc_u sinc_v * PS * W); zero where PB <
+    pb_limit; sinc over the unpadded size and the oversampling; optional
+    float32 rounding."""
+    rng = np.random.default_rng(3)
+    n = [16, 20]
+    img = rng.standard_normal((16, 20, 1, 1))
+    pb = rng.uniform(0.0, 1.0, size=(16, 20, 1, 1))
+    wpb = pb ** 2 + 0.01
+    sw = np.array([[0.0]])
+    out = oracle.normalize_mosaic(img, sw, np.ones(n), pb, wpb, [4, 5], "flat_sky", 0.2, False)
+    i, j = 3, 17
+    s = np.sinc((i - 8) / (16 * 4)) * np.sinc((j - 10) / (20 * 5))
+    assert np.isclose(out[i, j, 0, 0], 0.0 if pb[i, j, 0, 0] < 0.2 else
+                      img[i, j, 0, 0] / (s * wpb[i, j, 0, 0]), rtol=1e-14)
+    assert np.all(out[pb < 0.2] == 0)
+    out32 = oracle.normalize_mosaic(img, sw, np.ones(n), pb, wpb, [4, 5], "flat_sky", 0.2, True)
+    assert np.array_equal(out32, out.astype(np.float32).astype(np.float64))
