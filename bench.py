@@ -40,6 +40,19 @@ def algorithmic_bytes(n_vis, n_rows, n_planes, npad):
     return n_vis * 13 + n_rows * 24 + n_planes * npad * npad * 8
 
 
+def measured_traffic(precision, config):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same
+    counter's reading for a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"std_grid_tiled_{config}_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+        return rec[precision]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(cfg, data, npad, delta, cgk, n_chan_sample):
     """The oracle (FP64 C restatement of the reference loop) timed on this
     host, one thread, on the first n_chan_sample channels of the workload."""
@@ -78,7 +91,10 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 16, 32],
-                    help="uv-tile rows of the gridder (0: library default)")
+                    help="uv-tile rows of the gridder (0: library default, 16)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the steps, run the grid's ifftshift (a plain copy kernel of "
+                         "known size) so that a counter run sees a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); "
@@ -148,6 +164,12 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.time() - t0
     nat.profile_enable(False)
+    if args.copy_calibration:
+        # ifftshift_kernel<float, float, 2> is a plain permuted copy of the
+        # complex64 grid: planes * Npad^2 * 8 bytes in and the same out
+        log(f"copy calibration: ifftshift of {n_planes * npad * npad * 8} bytes in and out")
+        nat.grid_to_image(grid, [N, N], sum_weight=sw)
+        torch.cuda.synchronize()
     ms_step = elapsed / args.steps * 1e3
     value = n_unflagged / (elapsed / args.steps) / 1e6
     phase_ms = {k: float(np.mean(v)) for k, v in phases.items()}
@@ -156,8 +178,9 @@ def main():
     abytes = algorithmic_bytes(cfg.n_vis, n_rows, n_planes, npad)
     t_kernel = phase_ms.get("std_grid_tiled", float("nan")) / 1e3
     achieved = abytes / t_kernel / 1e9
+    traffic = measured_traffic(args.precision, args.config)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                     kernel="std_grid_tiled_kernel<IMAGE,4,float,%s>" % (
                         "true" if args.precision == "fp32_kahan" else "false"),
                     algorithmic_bytes=int(abytes), kernel_ms=round(t_kernel * 1e3, 3),
@@ -223,14 +246,15 @@ def main():
                                f"support 7, oversampling 100, 10% flags",
                    "samples": int(cfg.n_vis), "unflagged": n_unflagged,
                    "planes": int(n_planes),
-                   "form": f"{args.precision} (uv-tiled bins, one wavefront per 32x32 tile, "
-                           f"register runs flushed into an LDS subgrid)"},
+                   "form": f"{args.precision} (uv-tiled bins, one wavefront per "
+                           f"{args.tile_rows or 16}x32-cell tile, lane-owned residue "
+                           f"classes, register runs flushed into an LDS subgrid)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "error_vs_fp64": err,
         "phases_ms": phase_ms,
         "briggs_ms": round(t_briggs * 1e3, 2),
-        "tile_rows": args.tile_rows or 32,
+        "tile_rows": args.tile_rows or 16,
     }
     print(json.dumps(line), flush=True)
 

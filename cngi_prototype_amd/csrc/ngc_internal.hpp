@@ -85,7 +85,7 @@ struct ngc_context {
     // rocFFT plans keyed by (n_u, n_v, batch, double_precision)
     std::map<std::tuple<int64_t, int64_t, int64_t, int>, ngc::FftPlan *> plans;
     int *d_error = nullptr;  // device error flag
-    int tile_rows = 32;      // uv-tile rows of the tiled gridder (16 or 32)
+    int tile_rows = 16;      // uv-tile rows of the tiled gridder (16 or 32)
 };
 
 namespace ngc {

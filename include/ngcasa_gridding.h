@@ -278,7 +278,7 @@ int ngc_phase_gradient(ngc_context *ctx, void *stream, const double *pix, int64_
 /*
  * Tile shape of the uv-tiled gridders: 16 or 32 grid rows per tile (the tile
  * is always 32 columns wide).  Set by the autotuner (no counterpart in the
- * reference, whose gridder has no tiles).  Default 32.
+ * reference, whose gridder has no tiles).  Default 16.
  */
 int ngc_set_tile_rows(ngc_context *ctx, int tile_rows);
 

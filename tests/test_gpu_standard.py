@@ -26,18 +26,31 @@ def _dev(a, dtype=None):
     return t.to(dtype) if dtype is not None else t
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp64_tiled", "fp32_kahan"])
+@pytest.mark.parametrize("rows", [16, 32])
+@pytest.mark.parametrize("precision", ["fp64", "fp64_tiled", "fp32_kahan", "fp32"])
 @pytest.mark.parametrize("mode", ["psf", "image"])
-def test_standard_grid_matches_oracle(gpu, precision, mode):
+def test_standard_grid_matches_oracle(gpu, precision, mode, rows):
     import torch
     nat = _native()
+    if precision == "fp64" and rows == 32:
+        pytest.skip("the plain FP64 form has no tiles")
+    nat.set_tile_rows(rows)
+    try:
+        _standard_case(nat, precision, mode)
+    finally:
+        nat.set_tile_rows(16)
+
+
+def _standard_case(nat, precision, mode):
+    import torch
     c = small_case(n_time=24, n_chan=3, n_pol=2, nan_frac=0.05)
     cgk = oracle.ps_kernel_1d(100, 7)
     npad = c["npad"]
     g_ref, sw_ref = oracle.standard_grid(c["vis"], c["uvw"], c["weight"], c["freq"], cgk,
                                          [npad, npad], c["delta"], 7, 100, do_psf=(mode == "psf"))
-    vis_dt = torch.complex64 if precision == "fp32_kahan" else torch.complex128
-    w_dt = torch.float32 if precision == "fp32_kahan" else torch.float64
+    single = precision in ("fp32_kahan", "fp32")
+    vis_dt = torch.complex64 if single else torch.complex128
+    w_dt = torch.float32 if single else torch.float64
     vis = _dev(c["vis"], vis_dt) if mode == "image" else None
     n_chan = c["weight"].shape[2]
     g, sw = nat.standard_grid(_dev(c["uvw"]), _dev(c["weight"], w_dt), c["freq"],
@@ -48,7 +61,7 @@ def test_standard_grid_matches_oracle(gpu, precision, mode):
     torch.cuda.synchronize()
     g = g.cpu().numpy()
     sw = sw.cpu().numpy()
-    if precision == "fp32_kahan":
+    if single:
         # inputs were rounded to complex64/float32: compare against the
         # oracle on the same rounded inputs
         visr = c["vis"].astype(np.complex64).astype(np.complex128)
