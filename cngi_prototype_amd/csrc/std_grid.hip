@@ -8,15 +8,17 @@
 //    support^2 taps in the reference's loop order and scatters with FP64
 //    atomics into the grid in HBM.
 //
-//  * uv-tiled (NGC_PREC_FP64_TILED, NGC_PREC_FP32_KAHAN): samples are binned
-//    by (image channel, 32x32 uv tile) with a deterministic, order-preserving
-//    counting sort (wave-aggregated atomics keep runs of consecutive
-//    integrations of one baseline together).  One workgroup owns one tile: it
-//    streams the tile's bin, accumulates every tap into an LDS subgrid (one
-//    lane per tap of an 8x8 tap window) and writes each grid cell to HBM once.
-//    In FP32 the LDS subgrid is folded into per-thread Kahan-Neumaier
-//    accumulators after every batch of 256 samples.  Pixel positions and
-//    sum_weight stay FP64.
+//  * uv-tiled (NGC_PREC_FP64_TILED, NGC_PREC_FP32_KAHAN, NGC_PREC_FP32): a
+//    count pass and a fill pass bin every (row, channel) sample into the
+//    (image channel, TU x 32 cell) tiles its footprint touches with an
+//    order-preserving counting sort (wave-aggregated atomics keep runs of
+//    consecutive integrations of one baseline together).  The fill pass
+//    stores per sample the cell, the sub-cell offsets and the weighted
+//    visibilities of all pols in the accumulation type, and adds the FP64
+//    sum_weight term.  One 64-lane wavefront then grids one tile into an LDS
+//    subgrid (lane-owned residue classes, register runs; see the kernel) and
+//    writes each grid cell to HBM once.  Pixel positions and sum_weight stay
+//    FP64; FP32_KAHAN compensates every LDS flush.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>

@@ -22,6 +22,10 @@
  *                              make_image.py:116-130, make_mosaic_pb.py:119-129,
  *                              make_image_with_gcf.py:143-147
  *   ngc_normalize_mosaic       _normalize 'forward'  _imaging_utils/_normalize.py:17-89
+ *   ngc_gcf_patterns           make_baseline_patterns + FFT of the a_term branch of
+ *                              make_gridding_convolution_function.py:181-191,245-246,394-412
+ *   ngc_gcf_resize             resize_and_calc_support  :361-391 (crop/normalise)
+ *   ngc_phase_gradient         make_phase_gradient  :331-359 (the table)
  *
  * Conventions
  *   - Large arrays (uvw, visibilities, weights, flags, grids, images) are
