@@ -297,7 +297,7 @@ def aperture_grid(uvw, weight, field, field_id, freq_chan, chan_map, pol_map,
         raise ValueError("phase_gradient must be [n_field, K_u, K_v]")
     n_u, n_v = int(n_uv[0]), int(n_uv[1])
     shape = (int(n_imag_chan), n_pol, n_u, n_v)
-    gdt = torch.complex64 if prec == PREC_FP32_KAHAN else torch.complex128
+    gdt = torch.complex64 if prec in (PREC_FP32_KAHAN, PREC_FP32) else torch.complex128
     if grid is None:
         grid = torch.empty(shape, dtype=gdt, device=weight.device)
     if sum_weight is None:

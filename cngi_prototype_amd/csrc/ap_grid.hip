@@ -152,8 +152,8 @@ extern "C" int ngc_aperture_grid(
     NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
     NGC_CHECK(mode == NGC_GRID_IMAGE || mode == NGC_GRID_PSF || mode == NGC_GRID_WEIGHT,
               NGC_ERR_ARG, "mode must be IMAGE, PSF or WEIGHT");
-    NGC_CHECK(precision == NGC_PREC_FP64, NGC_ERR_UNSUPPORTED,
-              "aperture gridder: only NGC_PREC_FP64 is built");
+    NGC_CHECK(precision >= NGC_PREC_FP64 && precision <= NGC_PREC_FP32, NGC_ERR_ARG,
+              "unknown precision");
     const int64_t n_samples = n_time * n_baseline * n_chan;
     NGC_CHECK(field_id && freq_chan && chan_map && pol_map && cf_baseline_map && cf_chan_map &&
                   cf_pol_map && conv_kernel && support && phase_gradient && grid && sum_weight,
@@ -206,11 +206,66 @@ extern "C" int ngc_aperture_grid(
     NGC_TRY(upload(ctx, s, "support", support, n_sup * sizeof(int32_t), &d_sup));
     NGC_TRY(upload(ctx, s, "field_id", field_id, n_field * sizeof(int32_t), &d_fid));
     const int64_t cells = n_imag_chan * n_imag_pol * n_u * n_v;
+    NGC_HIP(hipMemsetAsync(ctx->d_error, 0, sizeof(int), s));
+    if (precision != NGC_PREC_FP64) {
+        bool pol_identity = n_imag_pol == n_pol, cfp_uniform = true;
+        for (int64_t p = 0; p < n_pol; p++) {
+            pol_identity &= pol_map[p] == p;
+            cfp_uniform &= cf_pol_map[p] == cf_pol_map[0];
+        }
+        ApTiledCall c{};
+        c.mode = mode;
+        c.precision = precision;
+        c.uvw = uvw;
+        c.vis = vis;
+        c.vis_dtype = vis_dtype;
+        c.weight = weight;
+        c.weight_dtype = weight_dtype;
+        c.flag = flag;
+        c.field = field;
+        c.d_field_id = (const int *)d_fid;
+        c.n_field = n_field;
+        c.d_uv_scale = (const double *)d_scale;
+        c.d_chan_map = (const int *)d_cmap;
+        c.d_cfb = (const int *)d_cfb;
+        c.d_cfc = (const int *)d_cfc;
+        c.d_support = (const int *)d_sup;
+        c.n_cfc = n_cfc;
+        c.n_cfp = n_cfp;
+        c.cfp0 = cf_pol_map[0];
+        c.n_cf = n_cfb * n_cfc * n_cfp;
+        c.conv_kernel = conv_kernel;
+        c.phase_gradient = phase_gradient;
+        c.k_u = k_u;
+        c.k_v = k_v;
+        c.n_time = n_time;
+        c.n_baseline = n_baseline;
+        c.n_chan = n_chan;
+        c.n_pol = n_pol;
+        c.n_imag_chan = n_imag_chan;
+        c.n_imag_pol = n_imag_pol;
+        c.n_u = n_u;
+        c.n_v = n_v;
+        c.os_u = os_u;
+        c.os_v = os_v;
+        c.max_support = ms;
+        c.pol_identity = pol_identity;
+        c.cfp_uniform = cfp_uniform;
+        c.accumulate = accumulate;
+        c.grid = grid;
+        c.sum_weight = sum_weight;
+        profile_new_call(ctx, s);
+        NGC_TRY(ap_tiled_run(ctx, s, c));
+        int err = 0;
+        NGC_HIP(hipMemcpyAsync(&err, ctx->d_error, sizeof(int), hipMemcpyDeviceToHost, s));
+        NGC_HIP(hipStreamSynchronize(s));
+        NGC_CHECK(err == 0, NGC_ERR_FIELD, "a row's field id is not in field_id");
+        return NGC_OK;
+    }
     if (!accumulate) {
         NGC_HIP(hipMemsetAsync(grid, 0, cells * 2 * sizeof(double), s));
         NGC_HIP(hipMemsetAsync(sum_weight, 0, n_imag_chan * n_imag_pol * sizeof(double), s));
     }
-    NGC_HIP(hipMemsetAsync(ctx->d_error, 0, sizeof(int), s));
     ApArgs a{};
     a.uvw = uvw;
     a.vis = vis;

@@ -1,0 +1,950 @@
+// Tiled aperture (mosaic A-term) gridder for gfx950: the NGC_PREC_FP64_TILED,
+// NGC_PREC_FP32_KAHAN and NGC_PREC_FP32 forms behind ngc_aperture_grid().
+//
+// Replaces _aperture_grid_jit (ngcasa/imaging/_imaging_utils/_aperture_grid.py:377-513)
+// and _aperture_weight_grid_jit (:181-291).
+//
+// Image and PSF modes:
+//  1. ap_norm_kernel: the complex tap sum norm = sum CK*PG of every
+//     (convolution function, field, sub-cell offset pair), in the reference's
+//     tap order and FP64, so every sample's sum_weight term (w Re(norm^2) or
+//     w Re(norm), :508-511) is the plain path's to the last bit.
+//  2. ap_taps_kernel: the kernel re-laid per sub-cell offset pair as an R x R
+//     block (R = max support), so the taps of one sample are contiguous.
+//     ap_pg_kernel: the phase gradient split into its u and v factors
+//     (PG[x][y] = PG[x][c]*PG[c][y] for the reference's exp(i(x px + y py)));
+//     a table that does not factor is refused.
+//  3. ap_bin_kernel (count, scan, fill): every gridded (row, channel) goes to
+//     the bin of the (image channel, T x T tile) holding its centre cell, in
+//     time order per baseline; the fill pass adds the FP64 sum_weight term.
+//  4. ap_tiled_kernel: one workgroup of R*R lanes per bin.  Lane (a, b) owns
+//     every cell of the tile's halo region (T + R - 1 cells square) with
+//     u = a, v = b (mod R); a footprint of support <= R meets each residue
+//     pair at most once, so a lane has at most one tap per sample and no two
+//     lanes share a cell (J. W. Romein, ICS 2012).  The lane keeps its
+//     cell's running sum in registers and flushes it into the LDS region
+//     (compensated with FP32_KAHAN) when its cell changes.  The region is
+//     written to a scratch slot of the bin.
+//  5. ap_merge_kernel: every grid cell sums, in a fixed order and FP64, the
+//     (at most four) regions that cover it, and is written exactly once.
+//
+// Weight mode: every sample lands on the same centre taps of its (image
+// channel, pol, convolution function, field) key, so the weight grid is
+// sum_key W_key * CK*PG[centre taps] with W_key the key's sum of imaging
+// weights: ap_wsum_kernel reduces W per key in FP64 (and adds w Re(norm) to
+// sum_weight), ap_wstamp_kernel stamps the keys.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <type_traits>
+
+#include "ngc_internal.hpp"
+
+namespace ngc {
+
+constexpr int kApBatch = 32;   // samples staged per batch
+constexpr int kApBinBlock = 256;
+
+struct ApT {
+    const double *uvw;
+    const void *vis;
+    const void *weight;
+    const uint8_t *flag;
+    const int *field;      // [T, B]
+    const int *field_id;   // [n_field]
+    const double *uv_scale;  // [2][n_chan]
+    const int *chan_map, *cfb, *cfc;
+    const int *support;    // [n_cf][2]
+    int64_t n_field, n_cfc, n_cfp, cfp0, n_cf;
+    int64_t n_time, n_baseline, n_chan, n_pol, n_imag_chan, n_imag_pol, n_u, n_v;
+    int64_t os_u, os_v, k_u, k_v;
+    int R, T, W, hl;
+    int64_t ntu, ntv;
+    int ch_log2;
+    int64_t n_tb, n_cb;
+    int half_u, half_v;    // sub-cell offset index range [-half, half]
+    int vis_dtype, weight_dtype;
+    int *error;
+};
+
+__device__ inline double apt_weight(const ApT &a, int64_t i) {
+    double w = a.weight_dtype == NGC_F32 ? (double)((const float *)a.weight)[i]
+                                         : ((const double *)a.weight)[i];
+    if (a.flag && a.flag[i]) w = __longlong_as_double(0x7ff8000000000000LL);
+    return w;
+}
+
+__device__ inline double2 apt_vis(const ApT &a, int64_t i) {
+    if (a.vis_dtype == NGC_C64) {
+        const float2 v = ((const float2 *)a.vis)[i];
+        return make_double2(v.x, v.y);
+    }
+    return ((const double2 *)a.vis)[i];
+}
+
+// ---------------------------------------------------------------------------
+// tables
+// ---------------------------------------------------------------------------
+__global__ void ap_norm_kernel(ApT a, const double *ck, const double2 *pg, double2 *ntab) {
+#pragma clang fp contract(off)
+    const int nwu = 2 * a.half_u + 1, nwv = 2 * a.half_v + 1;
+    const int64_t n = a.n_cf * a.n_field * nwu * nwv;
+    const int64_t kplane = a.k_u * a.k_v;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ov = i % nwv - a.half_v;
+        const int64_t ou = (i / nwv) % nwu - a.half_u;
+        const int64_t f = (i / ((int64_t)nwu * nwv)) % a.n_field;
+        const int64_t cf = i / ((int64_t)nwu * nwv * a.n_field);
+        const int64_t su = a.support[cf * 2], sv = a.support[cf * 2 + 1];
+        const int64_t scu = su / 2, scv = sv / 2;
+        const int64_t u_off = ou + a.k_u / 2, v_off = ov + a.k_v / 2;
+        double nr = 0.0, ni = 0.0;
+        for (int64_t iv = -scv; iv < sv - scv; iv++) {
+            const int64_t cf_v = a.os_v * iv + v_off;
+            for (int64_t iu = -scu; iu < su - scu; iu++) {
+                const int64_t cf_u = a.os_u * iu + u_off;
+                if (cf_u < 0 || cf_u >= a.k_u || cf_v < 0 || cf_v >= a.k_v) continue;
+                const int64_t k = cf_u * a.k_v + cf_v;
+                const double kv = ck[cf * kplane + k];
+                const double2 g = pg[f * kplane + k];
+                nr = nr + kv * g.x;
+                ni = ni + kv * g.y;
+            }
+        }
+        ntab[i] = make_double2(nr, ni);
+    }
+}
+
+// ckr[cf][ou][ov][ru][rv] = CK[cf][os_u (ru - su/2) + ou + k_u/2][...], zero
+// outside the support
+template <typename ACC>
+__global__ void ap_taps_kernel(ApT a, const double *ck, ACC *ckr) {
+    const int nwu = 2 * a.half_u + 1, nwv = 2 * a.half_v + 1;
+    const int R = a.R;
+    const int64_t n = a.n_cf * nwu * nwv * R * R;
+    const int64_t kplane = a.k_u * a.k_v;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int rv = (int)(i % R), ru = (int)((i / R) % R);
+        const int64_t o = i / ((int64_t)R * R);
+        const int64_t ov = o % nwv - a.half_v;
+        const int64_t ou = (o / nwv) % nwu - a.half_u;
+        const int64_t cf = o / ((int64_t)nwu * nwv);
+        const int64_t su = a.support[cf * 2], sv = a.support[cf * 2 + 1];
+        double v = 0.0;
+        if (ru < su && rv < sv) {
+            const int64_t cf_u = a.os_u * (ru - su / 2) + ou + a.k_u / 2;
+            const int64_t cf_v = a.os_v * (rv - sv / 2) + ov + a.k_v / 2;
+            if (cf_u >= 0 && cf_u < a.k_u && cf_v >= 0 && cf_v < a.k_v)
+                v = ck[cf * kplane + cf_u * a.k_v + cf_v];
+        }
+        ckr[i] = (ACC)v;
+    }
+}
+
+template <typename ACC2>
+__global__ void ap_pg_kernel(ApT a, const double2 *pg, ACC2 *pgu, ACC2 *pgv, int *bad) {
+#pragma clang fp contract(off)
+    const int64_t cu = a.k_u / 2, cv = a.k_v / 2;
+    const int64_t kplane = a.k_u * a.k_v;
+    const int64_t n = a.n_field * kplane;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = i / kplane, r = i - f * kplane;
+        const int64_t x = r / a.k_v, y = r - x * a.k_v;
+        const double2 gx = pg[f * kplane + x * a.k_v + cv];
+        const double2 gy = pg[f * kplane + cu * a.k_v + y];
+        const double2 g = pg[i];
+        const double pr = gx.x * gy.x - gx.y * gy.y, pi = gx.x * gy.y + gx.y * gy.x;
+        if (fabs(pr - g.x) > 1e-9 || fabs(pi - g.y) > 1e-9) atomicOr(bad, 1);
+        if (y == cv) {
+            ACC2 o;
+            o.x = gx.x;
+            o.y = gx.y;
+            pgu[f * a.k_u + x] = o;
+        }
+        if (x == cu) {
+            ACC2 o;
+            o.x = gy.x;
+            o.y = gy.y;
+            pgv[f * a.k_v + y] = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// binning
+// ---------------------------------------------------------------------------
+template <int NP, int NW, typename ACC> struct ApEntry {
+    int uv;    // uc | vc << 16
+    int off;   // (ou & 0xffff) | ov << 16
+    int meta;  // cf | field index << 16
+    int pad;
+    ACC wd[NP][NW];
+};
+template <int NP, int NW, typename ACC> struct ApEntrySize {
+    static constexpr int bytes = ((int)sizeof(ApEntry<NP, NW, ACC>) + 15) / 16 * 16;
+};
+
+struct ApSample {
+    int64_t t, b, ch, row, rc;
+};
+
+__device__ inline bool ap_sample_at(const ApT &a, int64_t wave_id, int lane, ApSample &sm) {
+    const int chl = 1 << a.ch_log2;
+    const int tl = 64 >> a.ch_log2;
+    const int64_t cb = wave_id % a.n_cb;
+    const int64_t r = wave_id / a.n_cb;
+    const int64_t tb = r % a.n_tb;
+    const int64_t b = r / a.n_tb;
+    sm.t = tb * tl + (lane >> a.ch_log2);
+    sm.ch = cb * chl + (lane & (chl - 1));
+    sm.b = b;
+    if (b >= a.n_baseline || sm.t >= a.n_time || sm.ch >= a.n_chan) return false;
+    sm.row = sm.t * a.n_baseline + b;
+    sm.rc = sm.row * a.n_chan + sm.ch;
+    return true;
+}
+
+// field index, centre cell and offsets of a sample; false when not gridded
+__device__ inline bool ap_place(const ApT &a, const ApSample &sm, int &fidx, Pix &px, Pix &pxv) {
+    const int fval = a.field[sm.row];
+    if (!(fval > -1)) return false;
+    fidx = -1;
+    for (int64_t f = 0; f < a.n_field; f++)
+        if (a.field_id[f] == fval) {
+            fidx = (int)f;
+            break;
+        }
+    if (fidx < 0) {
+        atomicOr(a.error, 1);
+        return false;
+    }
+    const double *uvw = a.uvw + sm.row * 3;
+    if (!pixel_position(uvw[0], uvw[1], a.uv_scale[sm.ch], a.uv_scale[a.n_chan + sm.ch], a.n_u,
+                        a.n_v, a.os_u, px))
+        return false;
+    pixel_position(uvw[0], uvw[1], a.uv_scale[sm.ch], a.uv_scale[a.n_chan + sm.ch], a.n_u, a.n_v,
+                   a.os_v, pxv);
+    const int64_t ms = a.R;
+    if (!((px.uc + ms < a.n_u) && (px.vc + ms < a.n_v) && (px.uc - ms >= 0) &&
+          (px.vc - ms >= 0)))
+        return false;
+    if (a.flag) {
+        bool any = false;
+        for (int p = 0; p < a.n_pol; p++) any |= (a.flag[sm.rc * a.n_pol + p] == 0);
+        if (!any) return false;
+    }
+    return true;
+}
+
+template <bool FILL, int MODE, int NP, typename ACC>
+__global__ void __launch_bounds__(kApBinBlock)
+ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *ntab,
+              double *sum_weight, int sw_in_lds) {
+#pragma clang fp contract(off)
+    constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
+    using E = ApEntry<NP, NW, ACC>;
+    constexpr int EB = ApEntrySize<NP, NW, ACC>::bytes;
+    extern __shared__ double s_sw[];
+    const int64_t n_sw = a.n_imag_chan * a.n_imag_pol;
+    if (FILL && sw_in_lds) {
+        for (int64_t i = threadIdx.x; i < n_sw; i += blockDim.x) s_sw[i] = 0.0;
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t n_waves = a.n_baseline * a.n_tb * a.n_cb;
+    const int waves_per_block = blockDim.x >> 6;
+    const int nwu = 2 * a.half_u + 1, nwv = 2 * a.half_v + 1;
+    for (int64_t w0 = (int64_t)blockIdx.x * waves_per_block; w0 < n_waves;
+         w0 += (int64_t)gridDim.x * waves_per_block) {
+        const int64_t wave_id = w0 + (threadIdx.x >> 6);
+        ApSample sm{};
+        Pix px{}, pxv{};
+        int fidx = 0;
+        const bool ok = (wave_id < n_waves) && ap_sample_at(a, wave_id, lane, sm) &&
+                        ap_place(a, sm, fidx, px, pxv);
+        int64_t key = -1;
+        E rec;
+        if (ok) {
+            const int64_t a_chan = a.chan_map[sm.ch];
+            key = (a_chan * a.ntu + px.uc / a.T) * a.ntv + pxv.vc / a.T;
+            if (FILL) {
+                const int64_t cf = ((int64_t)a.cfb[sm.b] * a.n_cfc + a.cfc[sm.ch]) * a.n_cfp +
+                                   a.cfp0;
+                // |offset| <= 1/2 cell, so |ou| <= os_u/2 < half_u; the clamp
+                // only bounds the table indices
+                const int ou = (int)min(max(px.du, (int64_t)-a.half_u), (int64_t)a.half_u);
+                const int ov = (int)min(max(pxv.dv, (int64_t)-a.half_v), (int64_t)a.half_v);
+                rec.uv = (int)((px.uc & 0xffff) | (pxv.vc << 16));
+                rec.off = (ou & 0xffff) | (ov << 16);
+                rec.meta = (int)(cf | ((int64_t)fidx << 16));
+                rec.pad = 0;
+                const double2 nm =
+                    ntab[((cf * a.n_field + fidx) * nwu + ou + a.half_u) * nwv + ov + a.half_v];
+                const int64_t base = sm.rc * a.n_pol;
+#pragma unroll
+                for (int p = 0; p < NP; p++) {
+                    const double w = apt_weight(a, base + p);
+                    double wr = w, wi = 0.0;
+                    if (MODE == NGC_GRID_IMAGE) {
+                        const double2 v = apt_vis(a, base + p);
+                        wr = v.x * w;
+                        wi = v.y * w;
+                    }
+                    const bool use = !(isnan(wr) || isnan(wi)) && !(wr == 0.0 && wi == 0.0);
+                    rec.wd[p][0] = use ? (ACC)wr : (ACC)0;
+                    if (NW == 2) rec.wd[p][NW - 1] = use ? (ACC)wi : (ACC)0;
+                    if (use) {
+                        const double term = MODE == NGC_GRID_IMAGE
+                                                ? w * (nm.x * nm.x - nm.y * nm.y)
+                                                : w * nm.x;
+                        const int64_t swi = a_chan * a.n_imag_pol + p;
+                        if (sw_in_lds)
+                            atomicAdd(&s_sw[swi], term);
+                        else
+                            atomicAdd(&sum_weight[swi], term);
+                    }
+                }
+            }
+        }
+        uint64_t pending = __ballot(key >= 0);
+        while (pending) {
+            const int leader = __ffsll((long long)pending) - 1;
+            const int64_t lkey = __shfl(key, leader, 64);
+            const uint64_t same = __ballot(key == lkey);
+            unsigned long long basev = 0;
+            if (lane == leader)
+                basev = atomicAdd(&counts[lkey], (unsigned long long)__popcll(same));
+            if (FILL) {
+                basev = __shfl(basev, leader, 64);
+                if (key == lkey) {
+                    const unsigned long long pos = basev + __popcll(same & below);
+                    *reinterpret_cast<E *>(entries + pos * EB) = rec;
+                }
+            }
+            if (key == lkey) key = -1;
+            pending &= ~same;
+        }
+    }
+    if (FILL && sw_in_lds) {
+        __syncthreads();
+        for (int64_t i = threadIdx.x; i < n_sw; i += blockDim.x)
+            if (s_sw[i] != 0.0) atomicAdd(&sum_weight[i], s_sw[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tiled gridding kernel
+// ---------------------------------------------------------------------------
+template <typename ACC> struct Cplx {
+    ACC x, y;
+};
+
+template <int MODE, int NP, typename ACC, bool KAHAN>
+__global__ void __launch_bounds__(1024)
+ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
+                const char *__restrict__ entries, const ACC *__restrict__ ckr,
+                const Cplx<ACC> *__restrict__ pgu, const Cplx<ACC> *__restrict__ pgv,
+                typename std::conditional<KAHAN, double, ACC>::type *__restrict__ scratch) {
+    constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
+    constexpr int NACC = NP * 2;  // the grid is complex in every mode
+    using E = ApEntry<NP, NW, ACC>;
+    constexpr int EB = ApEntrySize<NP, NW, ACC>::bytes;
+    using SC = typename std::conditional<KAHAN, double, ACC>::type;
+    const int R = a.R, W = a.W;
+    const int PLANE = W * (W + 1);
+
+    extern __shared__ char smem_raw[];
+    ACC *s_acc = reinterpret_cast<ACC *>(smem_raw);
+    ACC *s_cmp = s_acc + NACC * PLANE;
+    char *p = reinterpret_cast<char *>(s_acc + (KAHAN ? 2 : 1) * NACC * PLANE);
+    p = reinterpret_cast<char *>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
+    int4 *s_meta = reinterpret_cast<int4 *>(p);          // [kApBatch]
+    int4 *s_fo = s_meta + kApBatch;                      // [kApBatch]
+    ACC *s_wd = reinterpret_cast<ACC *>(s_fo + kApBatch);  // [kApBatch][NP*NW]
+    Cplx<ACC> *s_pu = reinterpret_cast<Cplx<ACC> *>(s_wd + kApBatch * NP * NW);  // [kApBatch][R]
+    Cplx<ACC> *s_pv = s_pu + kApBatch * R;                                        // [kApBatch][R]
+
+    const int tid = threadIdx.x;
+    const int64_t bin = blockIdx.x;
+    const int64_t start = (int64_t)offsets[bin], end = (int64_t)offsets[bin + 1];
+    if (end <= start) return;
+    const int64_t tiles = a.ntu * a.ntv;
+    const int64_t trem = bin % tiles;
+    const int org_u = (int)((trem / a.ntv) * a.T) - a.hl;
+    const int org_v = (int)((trem % a.ntv) * a.T) - a.hl;
+
+    for (int i = tid; i < NACC * PLANE; i += blockDim.x) {
+        s_acc[i] = (ACC)0;
+        if (KAHAN) s_cmp[i] = (ACC)0;
+    }
+
+    const bool active = tid < R * R;
+    const int ra = active ? tid / R : 0, rb = active ? tid % R : 0;
+    const int nwv = 2 * a.half_v + 1;
+    int cur_u = -1, cur_v = -1;
+    ACC acc[NACC];
+#pragma unroll
+    for (int q = 0; q < NACC; q++) acc[q] = (ACC)0;
+
+    auto flush = [&]() {
+        if (cur_u >= 0) {
+            const int o = cur_u * (W + 1) + cur_v;
+#pragma unroll
+            for (int q = 0; q < NACC; q++) {
+                ACC *dst = &s_acc[q * PLANE + o];
+                const ACC x = acc[q];
+                if (KAHAN) {
+                    const ACC s = *dst;
+                    const ACC t = s + x;
+                    ACC *cp = &s_cmp[q * PLANE + o];
+                    if (fabs(s) >= fabs(x))
+                        *cp += (s - t) + x;
+                    else
+                        *cp += (x - t) + s;
+                    *dst = t;
+                } else {
+                    *dst = *dst + x;
+                }
+            }
+        }
+    };
+
+    for (int64_t b0 = start; b0 < end; b0 += kApBatch) {
+        const int nh = (int)min((int64_t)kApBatch, end - b0);
+        __syncthreads();  // previous batch consumed (and the zeroing done)
+        if (tid < kApBatch) {
+            int4 m = make_int4(0, 0, 0, 0), fo = make_int4(0, 0, 0, 0);
+            if (tid < nh) {
+                const E en = *reinterpret_cast<const E *>(entries + (b0 + tid) * EB);
+                const int uc = en.uv & 0xffff, vc = (en.uv >> 16) & 0xffff;
+                const int ou = (int)(short)(en.off & 0xffff), ov = (int)(short)(en.off >> 16);
+                const int cf = en.meta & 0xffff, fidx = (en.meta >> 16) & 0xffff;
+                const int su = a.support[cf * 2], sv = a.support[cf * 2 + 1];
+                const int us = uc - su / 2 - org_u, vs = vc - sv / 2 - org_v;
+                m.x = us | ((us % R) << 16);
+                m.y = vs | ((vs % R) << 16);
+                m.z = su | (sv << 16);
+                m.w = (cf * (2 * a.half_u + 1) + ou + a.half_u) * nwv + ov + a.half_v;
+                fo = make_int4(fidx, ou, ov, su | (sv << 16));
+#pragma unroll
+                for (int q = 0; q < NP; q++)
+#pragma unroll
+                    for (int c = 0; c < NW; c++) s_wd[tid * NP * NW + q * NW + c] = en.wd[q][c];
+            }
+            s_meta[tid] = m;
+            s_fo[tid] = fo;
+        }
+        __syncthreads();
+        // per-sample phase-gradient factors of the R residues along u and v
+        for (int idx = tid; idx < nh * 2 * R; idx += blockDim.x) {
+            const int j = idx / (2 * R), q = idx - j * 2 * R;
+            const int axis = q >= R, r = axis ? q - R : q;
+            const int4 fo = s_fo[j];
+            Cplx<ACC> val;
+            val.x = (ACC)0;
+            val.y = (ACC)0;
+            if (axis == 0) {
+                const int su = fo.w & 0xffff;
+                if (r < su) val = pgu[(int64_t)fo.x * a.k_u + a.os_u * (r - su / 2) + fo.y +
+                                      a.k_u / 2];
+                s_pu[j * R + r] = val;
+            } else {
+                const int sv = fo.w >> 16;
+                if (r < sv) val = pgv[(int64_t)fo.x * a.k_v + a.os_v * (r - sv / 2) + fo.z +
+                                      a.k_v / 2];
+                s_pv[j * R + r] = val;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            for (int j = 0; j < nh; j++) {
+                const int4 m = s_meta[j];
+                const int us = m.x & 0xffff, usm = m.x >> 16;
+                const int vs = m.y & 0xffff, vsm = m.y >> 16;
+                int ru = ra - usm;
+                ru += ru < 0 ? R : 0;
+                int rv = rb - vsm;
+                rv += rv < 0 ? R : 0;
+                const int su = m.z & 0xffff, sv = m.z >> 16;
+                if (ru < su && rv < sv) {
+                    const ACC k = ckr[((int64_t)m.w * R + ru) * R + rv];
+                    const Cplx<ACC> pu = s_pu[j * R + ru], pv = s_pv[j * R + rv];
+                    const ACC pr = pu.x * pv.x - pu.y * pv.y;
+                    const ACC pi = pu.x * pv.y + pu.y * pv.x;
+                    const ACC tr = k * pr, ti = k * pi;
+                    const int cu = us + ru, cv = vs + rv;
+                    if (cu != cur_u || cv != cur_v) {
+                        flush();
+                        cur_u = cu;
+                        cur_v = cv;
+#pragma unroll
+                        for (int q = 0; q < NACC; q++) acc[q] = (ACC)0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < NP; q++) {
+                        if (NW == 2) {
+                            const ACC wr = s_wd[j * NP * NW + q * NW];
+                            const ACC wi = s_wd[j * NP * NW + q * NW + 1];
+                            acc[2 * q] += tr * wr - ti * wi;
+                            acc[2 * q + 1] += tr * wi + ti * wr;
+                        } else {
+                            const ACC w = s_wd[j * NP * NW + q];
+                            acc[2 * q] += tr * w;
+                            acc[2 * q + 1] += ti * w;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    flush();
+    __syncthreads();
+    // the region goes to this bin's scratch slot: [NACC][W][W]
+    SC *out = scratch + bin * (int64_t)NACC * W * W;
+    for (int i = tid; i < NACC * W * W; i += blockDim.x) {
+        const int q = i / (W * W), r = i - q * W * W;
+        const int lu = r / W, lv = r - lu * W;
+        const int o = q * PLANE + lu * (W + 1) + lv;
+        SC v = (SC)s_acc[o];
+        if (KAHAN) v += (SC)s_cmp[o];
+        out[i] = v;
+    }
+}
+
+// Every grid cell sums the regions that cover it (home tile first, then the
+// neighbours in fixed order) and is written once.
+template <typename SC, typename OUT>
+__global__ void ap_merge_kernel(ApT a, const unsigned long long *__restrict__ offsets,
+                                const SC *__restrict__ scratch, OUT *grid, int accumulate) {
+    const int64_t n = a.n_imag_chan * a.n_imag_pol * a.n_u * a.n_v;
+    const int W = a.W, T = a.T;
+    const int NACC = (int)a.n_imag_pol * 2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = i % a.n_v;
+        const int64_t u = (i / a.n_v) % a.n_u;
+        const int64_t pol = (i / (a.n_u * a.n_v)) % a.n_imag_pol;
+        const int64_t ch = i / (a.n_u * a.n_v * a.n_imag_pol);
+        double re = 0.0, im = 0.0;
+        const int64_t tu0 = u / T, tv0 = v / T;
+        for (int du = 0; du < 3; du++) {
+            const int64_t tu = du == 0 ? tu0 : (du == 1 ? tu0 - 1 : tu0 + 1);
+            if (tu < 0 || tu >= a.ntu) continue;
+            const int64_t lu = u - (tu * T - a.hl);
+            if (lu < 0 || lu >= W) continue;
+            for (int dv = 0; dv < 3; dv++) {
+                const int64_t tv = dv == 0 ? tv0 : (dv == 1 ? tv0 - 1 : tv0 + 1);
+                if (tv < 0 || tv >= a.ntv) continue;
+                const int64_t lv = v - (tv * T - a.hl);
+                if (lv < 0 || lv >= W) continue;
+                const int64_t bin = (ch * a.ntu + tu) * a.ntv + tv;
+                if (offsets[bin + 1] <= offsets[bin]) continue;
+                const SC *s = scratch + bin * (int64_t)NACC * W * W + lu * W + lv;
+                re += (double)s[(2 * pol) * (int64_t)W * W];
+                im += (double)s[(2 * pol + 1) * (int64_t)W * W];
+            }
+        }
+        if (accumulate) {
+            grid[2 * i] = (OUT)((double)grid[2 * i] + re);
+            grid[2 * i + 1] = (OUT)((double)grid[2 * i + 1] + im);
+        } else {
+            grid[2 * i] = (OUT)re;
+            grid[2 * i + 1] = (OUT)im;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// weight mode
+// ---------------------------------------------------------------------------
+// W[key] with key = ((a_chan*n_imag_pol + p)*n_cf + cf)*n_field + f
+__global__ void __launch_bounds__(256)
+ap_wsum_kernel(ApT a, const double2 *ntab, double *wkey, double *sum_weight, int keys_in_lds,
+               int64_t n_keys) {
+#pragma clang fp contract(off)
+    extern __shared__ double s_w[];
+    if (keys_in_lds) {
+        for (int64_t i = threadIdx.x; i < n_keys; i += blockDim.x) s_w[i] = 0.0;
+        __syncthreads();
+    }
+    const int nwu = 2 * a.half_u + 1, nwv = 2 * a.half_v + 1;
+    const int64_t n_rc = a.n_time * a.n_baseline * a.n_chan;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_rc;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        ApSample sm;
+        sm.ch = i % a.n_chan;
+        sm.row = i / a.n_chan;
+        sm.b = sm.row % a.n_baseline;
+        sm.t = sm.row / a.n_baseline;
+        sm.rc = i;
+        int fidx = 0;
+        Pix px{}, pxv{};
+        if (!ap_place(a, sm, fidx, px, pxv)) continue;
+        const int64_t cf = ((int64_t)a.cfb[sm.b] * a.n_cfc + a.cfc[sm.ch]) * a.n_cfp + a.cfp0;
+        const int64_t a_chan = a.chan_map[sm.ch];
+        // centre taps: the zero offset row of the norm table
+        const double2 nm = ntab[((cf * a.n_field + fidx) * nwu + a.half_u) * nwv + a.half_v];
+        for (int p = 0; p < a.n_pol; p++) {
+            const double w = apt_weight(a, i * a.n_pol + p);
+            if (isnan(w) || w == 0.0) continue;
+            const int64_t key = ((a_chan * a.n_imag_pol + p) * a.n_cf + cf) * a.n_field + fidx;
+            if (keys_in_lds)
+                atomicAdd(&s_w[key], w);
+            else
+                atomicAdd(&wkey[key], w);
+            atomicAdd(&sum_weight[a_chan * a.n_imag_pol + p], w * nm.x);
+        }
+    }
+    if (keys_in_lds) {
+        __syncthreads();
+        for (int64_t i = threadIdx.x; i < n_keys; i += blockDim.x)
+            if (s_w[i] != 0.0) atomicAdd(&wkey[i], s_w[i]);
+    }
+}
+
+// one thread per (a_chan, pol, centre tap): sum_key W * CK*PG at the tap
+template <typename OUT>
+__global__ void ap_wstamp_kernel(ApT a, const double *ck, const double2 *pg, const double *wkey,
+                                 OUT *grid, int accumulate) {
+#pragma clang fp contract(off)
+    const int R = a.R;
+    const int64_t n = a.n_imag_chan * a.n_imag_pol * R * R;
+    const int64_t kplane = a.k_u * a.k_v;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int rv = (int)(i % R), ru = (int)((i / R) % R);
+        const int64_t cp = i / ((int64_t)R * R);  // a_chan * n_imag_pol + pol
+        const int64_t iu = ru - R / 2, iv = rv - R / 2;
+        double re = 0.0, im = 0.0;
+        bool touched = false;
+        for (int64_t cf = 0; cf < a.n_cf; cf++) {
+            const int64_t su = a.support[cf * 2], sv = a.support[cf * 2 + 1];
+            if (iu < -(su / 2) || iu >= su - su / 2 || iv < -(sv / 2) || iv >= sv - sv / 2)
+                continue;
+            const int64_t k = (a.os_u * iu + a.k_u / 2) * a.k_v + a.os_v * iv + a.k_v / 2;
+            for (int64_t f = 0; f < a.n_field; f++) {
+                const double wk = wkey[(cp * a.n_cf + cf) * a.n_field + f];
+                if (wk == 0.0) continue;
+                const double kv = ck[cf * kplane + k];
+                const double2 g = pg[f * kplane + k];
+                re += (kv * g.x) * wk;
+                im += (kv * g.y) * wk;
+                touched = true;
+            }
+        }
+        if (!touched) continue;
+        const int64_t cell = (cp * a.n_u + a.n_u / 2 + iu) * a.n_v + a.n_v / 2 + iv;
+        if (accumulate) {
+            grid[2 * cell] = (OUT)((double)grid[2 * cell] + re);
+            grid[2 * cell + 1] = (OUT)((double)grid[2 * cell + 1] + im);
+        } else {
+            grid[2 * cell] = (OUT)re;
+            grid[2 * cell + 1] = (OUT)im;
+        }
+    }
+}
+
+__global__ void ap_zero_kernel(double *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+template <typename T> __global__ void ap_zero_grid_kernel(T *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (T)0;
+}
+
+static int grid_blocks(int64_t n, int bs = 256) {
+    int64_t b = ceil_div(n, bs);
+    if (b > (1 << 20)) b = 1 << 20;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
+    const bool single = c.precision != NGC_PREC_FP64_TILED;
+    const bool kahan = c.precision == NGC_PREC_FP32_KAHAN;
+    NGC_CHECK(c.pol_identity, NGC_ERR_UNSUPPORTED,
+              "tiled aperture gridder needs pol_map = arange(n_pol)");
+    NGC_CHECK(c.cfp_uniform, NGC_ERR_UNSUPPORTED,
+              "tiled aperture gridder needs one convolution-function pol for all pols");
+    NGC_CHECK(c.n_pol >= 1 && c.n_pol <= 4 && c.n_pol != 3, NGC_ERR_UNSUPPORTED,
+              "tiled aperture gridder supports 1, 2 or 4 pols");
+    NGC_CHECK(c.max_support >= 1 && c.max_support <= 32, NGC_ERR_UNSUPPORTED,
+              "tiled aperture gridder supports max(SUPPORT) <= 32");
+    NGC_CHECK(c.n_u < 32768 && c.n_v < 32768, NGC_ERR_UNSUPPORTED, "grid axis >= 32768");
+    NGC_CHECK(c.n_cf < 65536 && c.n_field < 65536, NGC_ERR_UNSUPPORTED,
+              "too many convolution functions or fields");
+    NGC_CHECK(c.os_u < 16384 && c.os_v < 16384, NGC_ERR_UNSUPPORTED, "oversampling too large");
+
+    ApT a{};
+    a.uvw = c.uvw;
+    a.vis = c.vis;
+    a.weight = c.weight;
+    a.flag = c.flag;
+    a.field = c.field;
+    a.field_id = c.d_field_id;
+    a.uv_scale = c.d_uv_scale;
+    a.chan_map = c.d_chan_map;
+    a.cfb = c.d_cfb;
+    a.cfc = c.d_cfc;
+    a.support = c.d_support;
+    a.n_field = c.n_field;
+    a.n_cfc = c.n_cfc;
+    a.n_cfp = c.n_cfp;
+    a.cfp0 = c.cfp0;
+    a.n_cf = c.n_cf;
+    a.n_time = c.n_time;
+    a.n_baseline = c.n_baseline;
+    a.n_chan = c.n_chan;
+    a.n_pol = c.n_pol;
+    a.n_imag_chan = c.n_imag_chan;
+    a.n_imag_pol = c.n_imag_pol;
+    a.n_u = c.n_u;
+    a.n_v = c.n_v;
+    a.os_u = c.os_u;
+    a.os_v = c.os_v;
+    a.k_u = c.k_u;
+    a.k_v = c.k_v;
+    a.R = (int)c.max_support;
+    a.hl = a.R / 2;
+    a.half_u = (int)(c.os_u / 2 + 1);
+    a.half_v = (int)(c.os_v / 2 + 1);
+    a.vis_dtype = c.vis_dtype;
+    a.weight_dtype = c.weight_dtype;
+    a.error = ctx->d_error;
+    int ch_log2 = 0;
+    while ((1 << (ch_log2 + 1)) <= std::min<int64_t>(c.n_chan, 8)) ch_log2++;
+    a.ch_log2 = ch_log2;
+    a.n_cb = ceil_div(c.n_chan, int64_t(1) << ch_log2);
+    a.n_tb = ceil_div(c.n_time, int64_t(64) >> ch_log2);
+
+    const int64_t n_sw = c.n_imag_chan * c.n_imag_pol;
+    const int64_t n_rc = c.n_time * c.n_baseline * c.n_chan;
+    const int64_t cells = n_sw * c.n_u * c.n_v;
+    const int nwu = 2 * a.half_u + 1, nwv = 2 * a.half_v + 1;
+
+    // FP64 tap-sum table (sum_weight) for every mode
+    void *d_ntab;
+    const int64_t n_ntab = c.n_cf * c.n_field * nwu * nwv;
+    NGC_TRY(workspace(ctx, "ap_ntab", n_ntab * sizeof(double2), &d_ntab));
+    if (!c.accumulate) {
+        ap_zero_kernel<<<grid_blocks(n_sw), 256, 0, s>>>(c.sum_weight, n_sw);
+        NGC_HIP(hipGetLastError());
+    }
+    phase_begin(ctx, s, "ap_tables");
+    ap_norm_kernel<<<grid_blocks(n_ntab), 256, 0, s>>>(a, c.conv_kernel,
+                                                       (const double2 *)c.phase_gradient,
+                                                       (double2 *)d_ntab);
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+
+    if (c.mode == NGC_GRID_WEIGHT) {
+        const int64_t n_keys = n_sw * c.n_cf * c.n_field;
+        void *d_w;
+        NGC_TRY(workspace(ctx, "ap_wkey", n_keys * sizeof(double), &d_w));
+        ap_zero_kernel<<<grid_blocks(n_keys), 256, 0, s>>>((double *)d_w, n_keys);
+        NGC_HIP(hipGetLastError());
+        const int in_lds = n_keys <= 4096 ? 1 : 0;
+        phase_begin(ctx, s, "ap_weight_sum");
+        if (n_rc > 0) {
+            ap_wsum_kernel<<<grid_blocks(n_rc, 256) > 4096 ? 4096 : grid_blocks(n_rc, 256), 256,
+                             in_lds ? n_keys * sizeof(double) : 0, s>>>(
+                a, (const double2 *)d_ntab, (double *)d_w, c.sum_weight, in_lds, n_keys);
+            NGC_HIP(hipGetLastError());
+        }
+        phase_end(ctx, s);
+        phase_begin(ctx, s, "ap_weight_stamp");
+        if (single) {
+            if (!c.accumulate) {
+                ap_zero_grid_kernel<float><<<grid_blocks(2 * cells), 256, 0, s>>>(
+                    (float *)c.grid, 2 * cells);
+                NGC_HIP(hipGetLastError());
+            }
+            ap_wstamp_kernel<float><<<grid_blocks(n_sw * a.R * a.R), 256, 0, s>>>(
+                a, c.conv_kernel, (const double2 *)c.phase_gradient, (const double *)d_w,
+                (float *)c.grid, c.accumulate);
+        } else {
+            if (!c.accumulate) {
+                ap_zero_grid_kernel<double><<<grid_blocks(2 * cells), 256, 0, s>>>(
+                    (double *)c.grid, 2 * cells);
+                NGC_HIP(hipGetLastError());
+            }
+            ap_wstamp_kernel<double><<<grid_blocks(n_sw * a.R * a.R), 256, 0, s>>>(
+                a, c.conv_kernel, (const double2 *)c.phase_gradient, (const double *)d_w,
+                (double *)c.grid, c.accumulate);
+        }
+        NGC_HIP(hipGetLastError());
+        phase_end(ctx, s);
+        return NGC_OK;
+    }
+
+    // tile edge: the largest of 32, 24, 16 whose workgroup LDS lets two
+    // workgroups share a CU
+    const int acc_bytes = single ? 4 : 8;
+    const int nacc = 2 * (int)c.n_pol;
+    const int nw = c.mode == NGC_GRID_IMAGE ? 2 : 1;
+    auto lds_for = [&](int T) {
+        const int W = T + a.R - 1;
+        size_t b = (size_t)(kahan ? 2 : 1) * nacc * W * (W + 1) * acc_bytes;
+        b = (b + 15) / 16 * 16;
+        b += 2 * kApBatch * sizeof(int4) + (size_t)kApBatch * c.n_pol * nw * acc_bytes +
+             2 * (size_t)kApBatch * a.R * 2 * acc_bytes;
+        return b;
+    };
+    int T = 16;
+    for (int cand : {32, 24, 16})
+        if (lds_for(cand) <= 78 * 1024) {
+            T = cand;
+            break;
+        }
+    const size_t lds = lds_for(T);
+    NGC_CHECK(lds <= 160 * 1024, NGC_ERR_UNSUPPORTED, "aperture tile does not fit in LDS");
+    a.T = T;
+    a.W = T + a.R - 1;
+    a.ntu = ceil_div(c.n_u, T);
+    a.ntv = ceil_div(c.n_v, T);
+    const int64_t n_bins = c.n_imag_chan * a.ntu * a.ntv;
+    NGC_CHECK(n_bins < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED, "too many tiles");
+
+    // re-laid taps and the separable phase-gradient factors
+    void *d_ckr, *d_pgu, *d_pgv;
+    const int64_t n_ckr = c.n_cf * nwu * nwv * a.R * a.R;
+    NGC_TRY(workspace(ctx, "ap_ckr", n_ckr * acc_bytes, &d_ckr));
+    NGC_TRY(workspace(ctx, "ap_pgu", c.n_field * c.k_u * 2 * acc_bytes, &d_pgu));
+    NGC_TRY(workspace(ctx, "ap_pgv", c.n_field * c.k_v * 2 * acc_bytes, &d_pgv));
+    const int64_t n_pg = c.n_field * c.k_u * c.k_v;
+    NGC_HIP(hipMemsetAsync(ctx->d_error + 1, 0, sizeof(int), s));
+    phase_begin(ctx, s, "ap_tables");
+    if (single) {
+        ap_taps_kernel<float><<<grid_blocks(n_ckr), 256, 0, s>>>(a, c.conv_kernel,
+                                                                 (float *)d_ckr);
+        ap_pg_kernel<float2><<<grid_blocks(n_pg), 256, 0, s>>>(
+            a, (const double2 *)c.phase_gradient, (float2 *)d_pgu, (float2 *)d_pgv,
+            ctx->d_error + 1);
+    } else {
+        ap_taps_kernel<double><<<grid_blocks(n_ckr), 256, 0, s>>>(a, c.conv_kernel,
+                                                                  (double *)d_ckr);
+        ap_pg_kernel<double2><<<grid_blocks(n_pg), 256, 0, s>>>(
+            a, (const double2 *)c.phase_gradient, (double2 *)d_pgu, (double2 *)d_pgv,
+            ctx->d_error + 1);
+    }
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    int bad = 0;
+    NGC_HIP(hipMemcpyAsync(&bad, ctx->d_error + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    NGC_HIP(hipStreamSynchronize(s));
+    NGC_CHECK(bad == 0, NGC_ERR_UNSUPPORTED,
+              "tiled aperture gridder needs a phase gradient of the form exp(i(x px + y py))");
+
+    void *d_counts, *d_off, *d_scan_tmp, *d_ent, *d_scr;
+    NGC_TRY(workspace(ctx, "ap_bin_counts", (n_bins + 1) * sizeof(unsigned long long),
+                      &d_counts));
+    NGC_TRY(workspace(ctx, "ap_bin_offsets", (n_bins + 1) * sizeof(unsigned long long), &d_off));
+    NGC_HIP(hipMemsetAsync(d_counts, 0, (n_bins + 1) * sizeof(unsigned long long), s));
+    const int sw_lds = n_sw <= 4096 ? 1 : 0;
+    const size_t sw_shmem = sw_lds ? n_sw * sizeof(double) : 0;
+    const int64_t n_waves = c.n_baseline * a.n_tb * a.n_cb;
+    int64_t bin_blocks = ceil_div(n_waves, kApBinBlock / 64);
+    bin_blocks = std::max<int64_t>(1, std::min<int64_t>(bin_blocks, 1 << 20));
+
+    auto run = [&](auto mode_c, auto np_c, auto acc_c, auto kahan_c) -> int {
+        constexpr int MODE = decltype(mode_c)::value;
+        constexpr int NP = decltype(np_c)::value;
+        using ACC = typename decltype(acc_c)::type;
+        constexpr bool KAHAN = decltype(kahan_c)::value;
+        constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
+        constexpr int EB = ApEntrySize<NP, NW, ACC>::bytes;
+        using SC = typename std::conditional<KAHAN, double, ACC>::type;
+        unsigned long long *counts = (unsigned long long *)d_counts;
+        phase_begin(ctx, s, "ap_bin_count");
+        if (n_waves > 0) {
+            ap_bin_kernel<false, MODE, NP, ACC><<<(int)bin_blocks, kApBinBlock, 0, s>>>(
+                a, counts, nullptr, (const double2 *)d_ntab, c.sum_weight, 0);
+            NGC_HIP(hipGetLastError());
+        }
+        phase_end(ctx, s);
+        phase_begin(ctx, s, "ap_bin_scan");
+        size_t tmp_bytes = 0;
+        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts,
+                                                 (unsigned long long *)d_off, n_bins + 1, s));
+        NGC_TRY(workspace(ctx, "ap_scan_tmp", tmp_bytes, &d_scan_tmp));
+        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp, tmp_bytes, counts,
+                                                 (unsigned long long *)d_off, n_bins + 1, s));
+        phase_end(ctx, s);
+        unsigned long long total = 0;
+        NGC_HIP(hipMemcpyAsync(&total, (unsigned long long *)d_off + n_bins, sizeof(total),
+                               hipMemcpyDeviceToHost, s));
+        NGC_HIP(hipStreamSynchronize(s));
+        NGC_TRY(workspace(ctx, "ap_bin_entries", (size_t)(total + 1) * EB, &d_ent));
+        NGC_TRY(workspace(ctx, "ap_scratch",
+                          (size_t)n_bins * 2 * NP * a.W * a.W * sizeof(SC), &d_scr));
+        NGC_HIP(hipMemcpyAsync(d_counts, d_off, (n_bins + 1) * sizeof(unsigned long long),
+                               hipMemcpyDeviceToDevice, s));
+        phase_begin(ctx, s, "ap_bin_fill");
+        if (n_waves > 0) {
+            ap_bin_kernel<true, MODE, NP, ACC><<<(int)bin_blocks, kApBinBlock, sw_shmem, s>>>(
+                a, counts, (char *)d_ent, (const double2 *)d_ntab, c.sum_weight, sw_lds);
+            NGC_HIP(hipGetLastError());
+        }
+        phase_end(ctx, s);
+        const int threads = (a.R * a.R + 63) / 64 * 64;
+        phase_begin(ctx, s, "ap_grid_tiled");
+        if (total > 0) {
+            auto kern = ap_tiled_kernel<MODE, NP, ACC, KAHAN>;
+            NGC_HIP(hipFuncSetAttribute((const void *)kern,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            kern<<<(unsigned)n_bins, threads, lds, s>>>(
+                a, (const unsigned long long *)d_off, (const char *)d_ent, (const ACC *)d_ckr,
+                (const Cplx<ACC> *)d_pgu, (const Cplx<ACC> *)d_pgv, (SC *)d_scr);
+            NGC_HIP(hipGetLastError());
+        }
+        phase_end(ctx, s);
+        phase_begin(ctx, s, "ap_merge");
+        if (single)
+            ap_merge_kernel<SC, float><<<grid_blocks(cells), 256, 0, s>>>(
+                a, (const unsigned long long *)d_off, (const SC *)d_scr, (float *)c.grid,
+                c.accumulate);
+        else
+            ap_merge_kernel<SC, double><<<grid_blocks(cells), 256, 0, s>>>(
+                a, (const unsigned long long *)d_off, (const SC *)d_scr, (double *)c.grid,
+                c.accumulate);
+        NGC_HIP(hipGetLastError());
+        phase_end(ctx, s);
+        return NGC_OK;
+    };
+    using I0 = std::integral_constant<int, NGC_GRID_IMAGE>;
+    using I1 = std::integral_constant<int, NGC_GRID_PSF>;
+    struct F32 {
+        using type = float;
+    };
+    struct F64 {
+        using type = double;
+    };
+    using KT = std::true_type;
+    using KF = std::false_type;
+    auto by_prec = [&](auto mode_c, auto np_c) -> int {
+        if (c.precision == NGC_PREC_FP32_KAHAN) return run(mode_c, np_c, F32{}, KT{});
+        if (c.precision == NGC_PREC_FP32) return run(mode_c, np_c, F32{}, KF{});
+        return run(mode_c, np_c, F64{}, KF{});
+    };
+    auto by_pol = [&](auto mode_c) -> int {
+        switch (c.n_pol) {
+        case 1: return by_prec(mode_c, std::integral_constant<int, 1>{});
+        case 2: return by_prec(mode_c, std::integral_constant<int, 2>{});
+        default: return by_prec(mode_c, std::integral_constant<int, 4>{});
+        }
+    };
+    return c.mode == NGC_GRID_IMAGE ? by_pol(I0{}) : by_pol(I1{});
+}
+
+}  // namespace ngc

@@ -156,7 +156,7 @@ int ngc_context_create(int device, ngc_context **ctx) {
     NGC_HIP(hipSetDevice(device));
     ngc_context *c = new ngc_context();
     c->device = device;
-    hipError_t e = hipMalloc(&c->d_error, sizeof(int));
+    hipError_t e = hipMalloc(&c->d_error, 4 * sizeof(int));
     if (e != hipSuccess) {
         delete c;
         ngc::set_error(std::string("hipMalloc(error flag): ") + hipGetErrorString(e));

@@ -84,7 +84,7 @@ struct ngc_context {
     std::map<std::string, ngc::Buffer> ws;
     // rocFFT plans keyed by (n_u, n_v, batch, double_precision)
     std::map<std::tuple<int64_t, int64_t, int64_t, int>, ngc::FftPlan *> plans;
-    int *d_error = nullptr;  // device error flag
+    int *d_error = nullptr;  // device error flags [4]
     int tile_rows = 16;      // uv-tile rows of the tiled gridder (16 or 32)
 };
 
@@ -114,6 +114,32 @@ void phase_begin(ngc_context *ctx, hipStream_t s, const char *name);
 void phase_end(ngc_context *ctx, hipStream_t s);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// One validated ngc_aperture_grid call in a tiled precision form (device
+// copies of the small tables already uploaded).
+struct ApTiledCall {
+    int mode, precision;
+    const double *uvw;
+    const void *vis;
+    int vis_dtype;
+    const void *weight;
+    int weight_dtype;
+    const uint8_t *flag;
+    const int *field, *d_field_id;
+    int64_t n_field;
+    const double *d_uv_scale;
+    const int *d_chan_map, *d_cfb, *d_cfc, *d_support;
+    int64_t n_cfc, n_cfp, cfp0, n_cf;
+    const double *conv_kernel, *phase_gradient;
+    int64_t k_u, k_v;
+    int64_t n_time, n_baseline, n_chan, n_pol, n_imag_chan, n_imag_pol, n_u, n_v;
+    int64_t os_u, os_v, max_support;
+    bool pol_identity, cfp_uniform;
+    int accumulate;
+    void *grid;
+    double *sum_weight;
+};
+int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c);
 
 // uv scale per channel, in the reference's operation order:
 // -(freq*delta*n_uv)/c  (_standard_grid.py:274-276)

@@ -64,15 +64,22 @@ def test_gcf_matches_oracle(gpu, case, function, blockage):
     assert np.allclose(gcf.attrs["oversampling"], OS)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp64_tiled", "fp32_kahan", "fp32"])
 @pytest.mark.parametrize("mode", ["image", "psf", "weight"])
 @pytest.mark.parametrize("chan_mode", ["cube", "continuum"])
-def test_aperture_grid_matches_oracle(gpu, case, mode, chan_mode):
+def test_aperture_grid_matches_oracle(gpu, case, mode, chan_mode, precision):
     import torch
     import cngi_prototype_amd._native as nat
     ref = _oracle_gcf(case, np.array([0.75, 0.75]))
     xds = case["mxds"].attrs["xds0"]
     vis = np.asarray(xds.DATA.values)
     w = np.asarray(xds.DATA_WEIGHT.values)
+    single = precision in ("fp32_kahan", "fp32")
+    if single:
+        # the single-precision forms read complex64/float32 inputs: the oracle
+        # grids the same rounded values
+        vis = vis.astype(np.complex64).astype(np.complex128)
+        w = w.astype(np.float32).astype(np.float64)
     uvw = np.asarray(xds.UVW.values)
     field = np.asarray(xds.FIELD_ID.values)
     fid = np.arange(len(case["mxds"].attrs["FIELD"].coords["field_id"].values))
@@ -93,12 +100,15 @@ def test_aperture_grid_matches_oracle(gpu, case, mode, chan_mode):
     cmap = np.arange(n_chan) if chan_mode == "cube" else np.zeros(n_chan, dtype=int)
     n_imag = n_chan if chan_mode == "cube" else 1
     native_mode = {"image": nat.GRID_IMAGE, "psf": nat.GRID_PSF, "weight": nat.GRID_WEIGHT}[mode]
+    wd = _dev(w, torch.float32) if single else _dev(w)
+    vd = (_dev(vis, torch.complex64) if single else _dev(vis)) if mode == "image" else None
     g, sw = nat.aperture_grid(
-        _dev(uvw), _dev(w), _dev(field, torch.int32), fid, fr, cmap, np.arange(n_pol),
+        _dev(uvw), wd, _dev(field, torch.int32), fid, fr, cmap, np.arange(n_pol),
         ref["CF_BASELINE_MAP"], ref["CF_CHAN_MAP"], ref["CF_POL_MAP"], _dev(ref[kname]),
         ref["SUPPORT"], _dev(ref["PHASE_GRADIENT"]), n_imag, npad, case["delta"], OS,
-        native_mode, "fp64", vis=_dev(vis) if mode == "image" else None)
-    assert rel_err(g.cpu().numpy(), g_ref) < 1e-12
+        native_mode, precision, vis=vd)
+    assert g.dtype == (torch.complex64 if single else torch.complex128)
+    assert rel_err(g.cpu().numpy(), g_ref) < (1e-6 if single else 1e-12)
     assert rel_err(sw.cpu().numpy(), sw_ref) < 1e-12
 
 
@@ -178,3 +188,59 @@ def test_mosaic_chain_matches_oracle(gpu, case, norm_type, single):
     assert rel_err(np.asarray(img["PSF"].values)[:, :, 0], out[True][0]) < tol
     assert rel_err(np.asarray(img["PSF_SUM_WEIGHT"].values)[0], out[True][1]) < 1e-12
     assert "PSF_FIT" in img
+
+
+@pytest.mark.parametrize("precision", ["fp64_tiled", "fp32_kahan"])
+@pytest.mark.parametrize("n_pol", [1, 4])
+def test_aperture_tiled_large_support_and_pols(gpu, precision, n_pol):
+    """Supports up to 17 (max_support 20), 1 and 4 pols, 9 fields."""
+    import torch
+    import cngi_prototype_amd._native as nat
+    from helpers import mosaic_case
+    c = mosaic_case(n_pol=n_pol, n_field=9, image_size=200, cell_arcsec=0.45, n_time=36)
+    xds = c["mxds"].attrs["xds0"]
+    bl = np.stack([xds.ANTENNA1.values, xds.ANTENNA2.values], 1)
+    npad = np.array([c["npad"], c["npad"]])
+    ms = np.array([20, 20])
+    ref = oracle.make_gcf(xds.coords["chan"].values, n_pol, c["types"], bl, c["dish"],
+                          np.array([0.75, 0.75]), c["delta"], npad, OS, ms)
+    assert ref["SUPPORT"].max() == 17
+    fpd = np.asarray(c["mxds"].attrs["FIELD"]["PHASE_DIR"].values)[:, 0, :]
+    pg = oracle.phase_gradient(fpd, c["phase_center"], c["delta"], npad, OS, (ms + 1) * OS)
+    vis = np.asarray(xds.DATA.values).astype(np.complex64).astype(np.complex128)
+    w = np.asarray(xds.DATA_WEIGHT.values).astype(np.float32).astype(np.float64)
+    field = np.asarray(xds.FIELD_ID.values)
+    fid = np.arange(9)
+    fr = xds.coords["chan"].values
+    g_ref, sw_ref = oracle.aperture_grid(
+        vis, xds.UVW.values, w, field, fid, ref["CF_BASELINE_MAP"], ref["CF_CHAN_MAP"],
+        np.zeros(n_pol, int), ref["CONV_KERNEL"], ref["SUPPORT"], pg, fr, npad, c["delta"], OS,
+        do_psf=False)
+    single = precision == "fp32_kahan"
+    g, sw = nat.aperture_grid(
+        _dev(xds.UVW.values), _dev(w, torch.float32 if single else torch.float64),
+        _dev(field, torch.int32), fid, fr, np.arange(len(fr)), np.arange(n_pol),
+        ref["CF_BASELINE_MAP"], ref["CF_CHAN_MAP"], np.zeros(n_pol, int),
+        _dev(ref["CONV_KERNEL"]), ref["SUPPORT"], _dev(pg), len(fr), npad, c["delta"], OS,
+        nat.GRID_IMAGE, precision,
+        vis=_dev(vis, torch.complex64 if single else torch.complex128))
+    assert rel_err(g.cpu().numpy(), g_ref) < (1e-6 if single else 1e-12)
+    assert rel_err(sw.cpu().numpy(), sw_ref) < 1e-12
+
+
+def test_aperture_tiled_refuses_non_separable_phase_gradient(gpu, case):
+    import torch
+    import cngi_prototype_amd._native as nat
+    ref = _oracle_gcf(case, np.array([0.75, 0.75]))
+    xds = case["mxds"].attrs["xds0"]
+    w = np.asarray(xds.DATA_WEIGHT.values)
+    n_chan, n_pol = w.shape[2:]
+    rng = np.random.default_rng(0)
+    pg = np.exp(1j * rng.uniform(0, 6, size=ref["PHASE_GRADIENT"].shape))
+    with pytest.raises(nat.NativeError):
+        nat.aperture_grid(
+            _dev(xds.UVW.values), _dev(w, torch.float32), _dev(xds.FIELD_ID.values, torch.int32),
+            np.arange(4), xds.coords["chan"].values, np.arange(n_chan), np.arange(n_pol),
+            ref["CF_BASELINE_MAP"], ref["CF_CHAN_MAP"], ref["CF_POL_MAP"],
+            _dev(ref["CONV_KERNEL"]), ref["SUPPORT"], _dev(pg), n_chan, [case["npad"]] * 2,
+            case["delta"], OS, nat.GRID_PSF, "fp32")
