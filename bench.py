@@ -80,6 +80,116 @@ def cpu_baseline(cfg, data, npad, delta, cgk, n_chan_sample):
                        f"samples), {dt:.1f} s")
 
 
+def mosaic_leg(args):
+    """C4: 25-pointing heterogeneous mosaic (12 x 7 m + 40 x 12 m, 230 GHz, 32 chan x 2
+    pol, 1440 x 10 s dumps, 1024^2 at 0.09", GCF oversampling 20, supports 11/15/17).
+    Times the chosen form against the plain FP64 drop-in on the same inputs and
+    states the dirty-image error per plane."""
+    import torch
+    from cngi_prototype_amd import _native as nat
+    from cngi_prototype_amd import synthetic as syn
+    from cngi_prototype_amd.imaging import make_gridding_convolution_function
+    from cngi_prototype_amd.xds import Dataset
+
+    cfg = syn.CONFIGS["C4"]
+    N = cfg.image_size
+    npad = int(1.2 * N)
+    t0 = time.time()
+    d = syn.device_mosaic_dataset(cfg)
+    torch.cuda.synchronize()
+    log(f"mosaic: generated {cfg.n_vis / 1e6:.1f}M samples in {time.time() - t0:.1f}s")
+    pc = syn.phase_center(cfg)
+    dims4 = ["time", "baseline", "chan", "pol"]
+    xds = Dataset(
+        data_vars={"DATA": (dims4, d["vis"]), "UVW": (["time", "baseline", "uvw_index"], d["uvw"]),
+                   "DATA_WEIGHT": (dims4, d["weight"]), "FLAG": (dims4, d["flag"]),
+                   "ANTENNA1": (["baseline"], d["a1"]), "ANTENNA2": (["baseline"], d["a2"]),
+                   "FIELD_ID": (["time", "baseline"], d["field"])},
+        coords={"chan": (["chan"], d["freq"]), "pol": (["pol"], np.array([9, 12]))},
+        attrs={"data_groups": [{"0": {"id": "0", "data": "DATA", "uvw": "UVW", "flag": "FLAG",
+                                      "weight": "DATA_WEIGHT"}}]})
+    field = Dataset(data_vars={"PHASE_DIR": (["field_id", "poly_id", "d2"], d["fields"][:, None, :])},
+                    coords={"field_id": (["field_id"], np.arange(len(d["fields"])))})
+    mxds = Dataset(attrs={"xds0": xds, "FIELD": field})
+    gcf_parms = dict(function="casa_airy", list_dish_diameters=list(d["dish"]),
+                     list_blockage_diameters=[0.75, 0.75], unique_ant_indx=list(d["types"]),
+                     phase_center=pc, oversampling=[20, 20], max_support=[18, 18])
+    grid_parms = dict(image_size=[N, N], cell_size=[cfg.cell_arcsec] * 2, fft_padding=1.2,
+                      chan_mode="cube", phase_center=pc)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    gcf = make_gridding_convolution_function(mxds, gcf_parms, grid_parms, {"xds": "xds0"})
+    torch.cuda.synchronize()
+    t_gcf = time.time() - t0
+    sup = np.asarray(gcf["SUPPORT"].values)
+    log(f"mosaic: GCF in {t_gcf:.2f}s, supports {sup[:, 0, 0, 0].tolist()}")
+    cell = cfg.cell_arcsec * syn.ARCSEC
+    delta = np.array([-cell, cell])
+    C, P = cfg.n_chan, cfg.n_pol
+    cfb = np.asarray(gcf["CF_BASELINE_MAP"].values)
+    ck = gcf["CONV_KERNEL"].data
+    pg = gcf["PHASE_GRADIENT"].data
+    fid = np.arange(len(d["fields"]))
+
+    def call(precision, grid=None, sw=None):
+        return nat.aperture_grid(d["uvw"], d["weight"], d["field"], fid, d["freq"], np.arange(C),
+                                 np.arange(P), cfb, np.asarray(gcf["CF_CHAN_MAP"].values),
+                                 np.asarray(gcf["CF_POL_MAP"].values), ck, sup, pg, C,
+                                 [npad, npad], delta, [20, 20], nat.GRID_IMAGE, precision,
+                                 vis=d["vis"], grid=grid, sum_weight=sw)
+
+    grid = torch.empty((C, P, npad, npad), dtype=torch.complex64, device="cuda")
+    sw = torch.empty((C, P), dtype=torch.float64, device="cuda")
+    for _ in range(args.warmup):
+        call(args.precision, grid, sw)
+    torch.cuda.synchronize()
+    nat.profile_enable(True)
+    phases = {}
+    t0 = time.time()
+    for _ in range(args.steps):
+        call(args.precision, grid, sw)
+        for name, ms in nat.profile_read():
+            phases[name] = phases.get(name, 0.0) + ms / args.steps
+    torch.cuda.synchronize()
+    ms_step = (time.time() - t0) / args.steps * 1e3
+    nat.profile_enable(False)
+    n_vis = cfg.n_vis
+    log(f"mosaic chosen form {args.precision}: {ms_step:.2f} ms, phases {phases}")
+    # plain FP64 drop-in, once
+    torch.cuda.synchronize()
+    t0 = time.time()
+    g64, sw64 = call("fp64")
+    torch.cuda.synchronize()
+    ms_plain = (time.time() - t0) * 1e3
+    log(f"mosaic plain fp64: {ms_plain:.1f} ms")
+    img64 = nat.grid_to_image(g64, [N, N], sum_weight=sw64)
+    del g64
+    img32 = nat.grid_to_image(grid, [N, N], sum_weight=sw)
+    n_planes = C * P
+    dd = (img32 - img64).reshape(n_planes, -1)
+    rr = img64.reshape(n_planes, -1)
+    rel = (torch.linalg.norm(dd, dim=1) / torch.linalg.norm(rr, dim=1)).cpu().numpy()
+    del img32, img64
+    # algorithmic flops: per tap CK*PG (2) + one complex multiply-add per pol (8)
+    types = d["types"]
+    taps = np.asarray([sup[cfb[b], 0, 0, 0] ** 2 for b in range(cfg.n_baseline)], dtype=np.float64)
+    flops = float(taps.sum()) * cfg.n_time * C * (2 + 8 * P)
+    t_k = phases.get("ap_grid_tiled", float("nan")) / 1e3
+    return dict(workload=f"C4: 25-pointing mosaic, 12x7 m + 40x12 m, {cfg.n_time} x 10 s, "
+                         f"{C} chan x {P} pol at 230 GHz, {N}^2 at {cfg.cell_arcsec}\", "
+                         f"GCF os 20, supports {sorted(set(sup[:, 0, 0, 0].tolist()))}",
+                value=round(n_vis / (ms_step / 1e3) / 1e6, 1), unit="Mvis/s",
+                ms_per_step=round(ms_step, 3), precision=args.precision,
+                plain_fp64_mvis_s=round(n_vis / (ms_plain / 1e3) / 1e6, 2),
+                plain_fp64_ms=round(ms_plain, 1),
+                speedup_vs_plain_fp64=round(ms_plain / ms_step, 1),
+                error_vs_fp64=dict(rel_rms_max=float(rel.max()),
+                                   rel_rms_median=float(np.median(rel)), planes=int(n_planes)),
+                kernel_ms=round(t_k * 1e3, 3),
+                kernel_tflops=round(flops / t_k / 1e12, 2),
+                fp32_peak_tflops=157.3, phases_ms=phases, gcf_s=round(t_gcf, 2))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +197,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
+    ap.add_argument("--no-mosaic", action="store_true",
+                    help="skip the C4 mosaic leg (reported under 'mosaic')")
     ap.add_argument("--cpu-chans", type=int, default=6)
     ap.add_argument("--config", default="C2")
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
@@ -226,6 +338,17 @@ def main():
         except Exception as e:  # report, never fake
             log(f"cpu baseline failed: {e!r}")
 
+    mosaic = None
+    if not args.no_mosaic:
+        del data, uvw, vis, w, flag, iw, grid, sw
+        torch.cuda.empty_cache()
+        try:
+            mosaic = mosaic_leg(args)
+            log(f"mosaic {mosaic}")
+        except Exception as e:  # report, never fake
+            log(f"mosaic leg failed: {e!r}")
+            mosaic = {"error": repr(e)}
+
     line = {
         "metric": "Mvis/s gridded (std+mosaic) at <=1e-5 rel err vs FP64; % of HBM roofline",
         "value": round(value, 1),
@@ -255,6 +378,7 @@ def main():
         "phases_ms": phase_ms,
         "briggs_ms": round(t_briggs * 1e3, 2),
         "tile_rows": args.tile_rows or 16,
+        "mosaic": mosaic,
     }
     print(json.dumps(line), flush=True)
 

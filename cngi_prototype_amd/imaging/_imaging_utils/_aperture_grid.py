@@ -37,4 +37,4 @@ def grid_aperture(vis_xds, gcf_xds, grid_parms, sel_parms, mode):
         np.asarray(gcf_xds['CF_CHAN_MAP'].values), np.asarray(gcf_xds['CF_POL_MAP'].values),
         kernel, np.asarray(gcf_xds['SUPPORT'].values), pg, n_imag_chan,
         grid_parms['image_size_padded'], grid_parms['cell_size'], grid_parms['oversampling'],
-        native_mode, 'fp64', vis=vis)
+        native_mode, grid_parms.get('precision', 'fp64'), vis=vis)

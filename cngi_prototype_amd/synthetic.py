@@ -72,11 +72,12 @@ CONFIGS = {
                  1024, 0.2, 100, flag_per_plane_max=0.6, spectral_index=True, fov_deg=0.02,
                  seed=3),
     # 25-pointing heterogeneous mosaic: 12 x 7 m + 40 x 12 m, 230 GHz, 4 h of 10 s dumps
+    # (cell 0.09": the 12 m - 12 m GCF support is 17 at oversampling 20,
+    # max_support 18; 7 m - 7 m is 11 and 7 m - 12 m 15)
     "C4": Config("C4", 52, "lognormal", 1500.0, -23.0234, -23.0, 1440, 10.0, 230e9, 15.625e6,
-                 32, 2, 1024, 0.15, 60, fov_deg=0.012, lognormal_mu=np.log(150.0),
+                 32, 2, 1024, 0.09, 60, fov_deg=0.012, lognormal_mu=np.log(150.0),
                  lognormal_sigma=0.7, seed=4,
-                 extra=dict(n_field=25, dish=(7.0, 12.0), n_dish=(12, 40),
-                            fields_per_dump=1)),
+                 extra=dict(n_field=25, dish=(7.0, 12.0), n_dish=(12, 40))),
 }
 
 
@@ -413,3 +414,72 @@ def device_dataset(cfg, device="cuda", vis_dtype=None, weight_value=1.0, rows_pe
         vflat[r0:r1] = chunk.to(vis_dtype)
     out["vis"] = vis
     return out
+
+
+def device_mosaic_dataset(cfg, device="cuda", rows_per_chunk=4096):
+    """make_mosaic_mxds's data built in device memory for a large mosaic:
+    uvw (float64 [T, B, 3]), FIELD_ID (int32 [T, B]), vis (complex64), weight
+    (float32, ones), flag (uint8, none set).  Sources are attenuated by the
+    two antennas' Airy voltage patterns at their distance from the field
+    centre; phases are referenced to the common phase centre."""
+    import torch
+    enu = antenna_enu(cfg)
+    uvw_np = uvw_tracks(enu, cfg.lat_deg, cfg.dec_deg, cfg.n_time, cfg.dt)
+    T, B, C, P = cfg.n_time, cfg.n_baseline, cfg.n_chan, cfg.n_pol
+    fr = freqs(cfg)
+    sk = sky(cfg)
+    ra0, dec0 = phase_center(cfg)
+    fields = mosaic_fields(cfg)
+    sched = field_schedule(cfg)
+    types, dish = dish_types(cfg)
+    a1, a2 = baseline_pairs(cfg.n_ant)
+    l, m = sk["l"], sk["m"]
+    n = np.sqrt(1 - l * l - m * m)
+    dec_s = np.arcsin(m * np.cos(dec0) + n * np.sin(dec0))
+    ra_s = ra0 + np.arctan2(l, n * np.cos(dec0) - m * np.sin(dec0))
+    uvw = torch.from_numpy(uvw_np).to(device)
+    field = torch.from_numpy(np.repeat(sched[:, None], B, axis=1).astype(np.int32)).to(device)
+    vis = torch.empty((T, B, C, P), dtype=torch.complex64, device=device)
+    lam = torch.tensor(fr / C_LIGHT, dtype=torch.float32, device=device)
+    lk = torch.tensor(l, dtype=torch.float32, device=device)
+    mk = torch.tensor(m, dtype=torch.float32, device=device)
+    nk1 = torch.tensor(n - 1, dtype=torch.float32, device=device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(cfg.seed + 4000)
+    ta1 = torch.from_numpy(types[a1]).to(device)
+    ta2 = torch.from_numpy(types[a2]).to(device)
+    for f in range(len(fields)):
+        rows_t = np.nonzero(sched == f)[0]
+        if len(rows_t) == 0:
+            continue
+        raf, decf = fields[f]
+        cosr = (np.sin(dec_s) * np.sin(decf) +
+                np.cos(dec_s) * np.cos(decf) * np.cos(ra_s - raf))
+        r = np.arccos(np.clip(cosr, -1, 1))
+        vp = torch.tensor(np.stack([airy_voltage(r, fr, d) for d in dish]), dtype=torch.float32,
+                          device=device)                                   # [n_dish, S, C]
+        amp = (vp[ta1] * vp[ta2]) * torch.tensor(sk["flux"], dtype=torch.float32,
+                                                  device=device)[None, :, None]  # [B, S, C]
+        t0, t1 = int(rows_t[0]), int(rows_t[-1]) + 1
+        flat = uvw[t0:t1].reshape(-1, 3).float()
+        vflat = vis[t0:t1].view(-1, C, P)
+        nr = flat.shape[0]
+        for r0 in range(0, nr, rows_per_chunk):
+            r1 = min(nr, r0 + rows_per_chunk)
+            bidx = torch.arange(r0, r1, device=device) % B
+            u = flat[r0:r1, 0, None, None] * lam[None, :, None]            # [R, C, 1]
+            v = flat[r0:r1, 1, None, None] * lam[None, :, None]
+            w = flat[r0:r1, 2, None, None] * lam[None, :, None]
+            ph = 2 * np.pi * (u * lk + v * mk + w * nk1)                     # [R, C, S]
+            a = amp[bidx].transpose(1, 2)                                    # [R, C, S]
+            I = torch.complex((a * torch.cos(ph)).sum(-1), (a * torch.sin(ph)).sum(-1))
+            noise = torch.complex(torch.randn((r1 - r0, C, P), generator=gen, device=device),
+                                  torch.randn((r1 - r0, C, P), generator=gen, device=device))
+            chunk = noise * cfg.noise
+            chunk[:, :, 0] += I
+            chunk[:, :, P - 1] += I
+            vflat[r0:r1] = chunk.to(torch.complex64)
+    weight = torch.ones((T, B, C, P), dtype=torch.float32, device=device)
+    flag = torch.zeros((T, B, C, P), dtype=torch.uint8, device=device)
+    return dict(uvw=uvw, field=field, vis=vis, weight=weight, flag=flag, freq=fr,
+                fields=fields, types=types, dish=dish, a1=a1, a2=a2)
