@@ -199,6 +199,8 @@ def main():
     ap.add_argument("--no-error-check", action="store_true")
     ap.add_argument("--no-mosaic", action="store_true",
                     help="skip the C4 mosaic leg (reported under 'mosaic')")
+    ap.add_argument("--mosaic-only", action="store_true",
+                    help="run only the C4 mosaic leg and print its record")
     ap.add_argument("--cpu-chans", type=int, default=6)
     ap.add_argument("--config", default="C2")
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
@@ -220,6 +222,9 @@ def main():
         cropped_correction_factors, prolate_spheroidal_kernel_1d
 
     torch.cuda.set_device(0)
+    if args.mosaic_only:
+        print(json.dumps({"mosaic": mosaic_leg(args)}), flush=True)
+        return
     if args.tile_rows:
         nat.set_tile_rows(args.tile_rows)
     cfg = syn.CONFIGS[args.config]

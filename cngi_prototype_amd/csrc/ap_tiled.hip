@@ -43,6 +43,7 @@
 namespace ngc {
 
 constexpr int kApBatch = 32;   // samples staged per batch
+constexpr int kApGroup = 8;    // samples whose tap loads are issued together
 constexpr int kApBinBlock = 256;
 
 struct ApT {
@@ -339,6 +340,28 @@ ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *n
 // ---------------------------------------------------------------------------
 // tiled gridding kernel
 // ---------------------------------------------------------------------------
+// Byte offsets of the tiled kernel's dynamic LDS block (shared by the
+// kernel and the host's size computation).
+struct ApLds {
+    int fo, pos, base, ui, vi, q, pv, wd, bytes;
+};
+__host__ __device__ inline int ap_align16(int x) { return (x + 15) & ~15; }
+__host__ __device__ inline ApLds ap_lds_layout(int R, int W, int NP, int NW, int acc_bytes,
+                                               bool kahan) {
+    ApLds L;
+    const int nacc = 2 * NP;
+    L.fo = ap_align16((kahan ? 2 : 1) * nacc * W * (W + 1) * acc_bytes);
+    L.pos = L.fo + kApBatch * 16;
+    L.base = L.pos + kApBatch * 16;
+    L.ui = L.base + kApBatch * 4;
+    L.vi = L.ui + kApBatch * R * 4;
+    L.q = ap_align16(L.vi + kApBatch * R * 4);
+    L.pv = L.q + kApBatch * R * NP * 2 * acc_bytes;
+    L.wd = L.pv + kApBatch * R * 2 * acc_bytes;
+    L.bytes = ap_align16(L.wd + kApBatch * NP * NW * acc_bytes);
+    return L;
+}
+
 template <typename ACC> struct Cplx {
     ACC x, y;
 };
@@ -346,8 +369,9 @@ template <typename ACC> struct Cplx {
 template <int MODE, int NP, typename ACC, bool KAHAN>
 __global__ void __launch_bounds__(1024)
 ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
-                const char *__restrict__ entries, const ACC *__restrict__ ckr,
-                const Cplx<ACC> *__restrict__ pgu, const Cplx<ACC> *__restrict__ pgv,
+                const unsigned int *__restrict__ order, const char *__restrict__ entries,
+                const ACC *__restrict__ ckr, const Cplx<ACC> *__restrict__ pgu,
+                const Cplx<ACC> *__restrict__ pgv,
                 typename std::conditional<KAHAN, double, ACC>::type *__restrict__ scratch) {
     constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
     constexpr int NACC = NP * 2;  // the grid is complex in every mode
@@ -357,19 +381,32 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     const int R = a.R, W = a.W;
     const int PLANE = W * (W + 1);
 
-    extern __shared__ char smem_raw[];
+    // One dynamic LDS block, carved by integer byte offsets (no pointer round
+    // trip through integers, so every access stays an LDS access):
+    //   region  [NACC][W][W+1] (+ the same again for the compensation words)
+    //   s_fo    [kApBatch] int4   staging: field, offsets, supports
+    //   s_pos   [kApBatch] int4   staging: start cells and their residues
+    //   s_base  [kApBatch] uint   first tap index of the sample's R x R block
+    //   s_ui    [kApBatch][R] int u-residue a: cell_u*(W+1) | ru << 16, -1 = no tap
+    //   s_vi    [kApBatch][R] int v-residue b: cell_v | rv << 16, -1 = no tap
+    //   s_q     [kApBatch][R][NP] u phase factor x weighted visibility per pol
+    //   s_pv    [kApBatch][R] v phase factor
+    //   s_wd    [kApBatch][NP*NW]
+    extern __shared__ __align__(16) char smem_raw[];
+    const ApLds L = ap_lds_layout(R, W, NP, NW, (int)sizeof(ACC), KAHAN);
     ACC *s_acc = reinterpret_cast<ACC *>(smem_raw);
     ACC *s_cmp = s_acc + NACC * PLANE;
-    char *p = reinterpret_cast<char *>(s_acc + (KAHAN ? 2 : 1) * NACC * PLANE);
-    p = reinterpret_cast<char *>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
-    int4 *s_meta = reinterpret_cast<int4 *>(p);          // [kApBatch]
-    int4 *s_fo = s_meta + kApBatch;                      // [kApBatch]
-    ACC *s_wd = reinterpret_cast<ACC *>(s_fo + kApBatch);  // [kApBatch][NP*NW]
-    Cplx<ACC> *s_pu = reinterpret_cast<Cplx<ACC> *>(s_wd + kApBatch * NP * NW);  // [kApBatch][R]
-    Cplx<ACC> *s_pv = s_pu + kApBatch * R;                                        // [kApBatch][R]
+    int4 *s_fo = reinterpret_cast<int4 *>(smem_raw + L.fo);
+    int4 *s_pos = reinterpret_cast<int4 *>(smem_raw + L.pos);
+    unsigned int *s_base = reinterpret_cast<unsigned int *>(smem_raw + L.base);
+    int *s_ui = reinterpret_cast<int *>(smem_raw + L.ui);
+    int *s_vi = reinterpret_cast<int *>(smem_raw + L.vi);
+    Cplx<ACC> *s_q = reinterpret_cast<Cplx<ACC> *>(smem_raw + L.q);
+    Cplx<ACC> *s_pv = reinterpret_cast<Cplx<ACC> *>(smem_raw + L.pv);
+    ACC *s_wd = reinterpret_cast<ACC *>(smem_raw + L.wd);
 
     const int tid = threadIdx.x;
-    const int64_t bin = blockIdx.x;
+    const int64_t bin = order[blockIdx.x];
     const int64_t start = (int64_t)offsets[bin], end = (int64_t)offsets[bin + 1];
     if (end <= start) return;
     const int64_t tiles = a.ntu * a.ntv;
@@ -385,22 +422,21 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     const bool active = tid < R * R;
     const int ra = active ? tid / R : 0, rb = active ? tid % R : 0;
     const int nwv = 2 * a.half_v + 1;
-    int cur_u = -1, cur_v = -1;
+    int cur = -1;  // current cell: u*(W+1) + v in the region, -1 = none
     ACC acc[NACC];
 #pragma unroll
     for (int q = 0; q < NACC; q++) acc[q] = (ACC)0;
 
     auto flush = [&]() {
-        if (cur_u >= 0) {
-            const int o = cur_u * (W + 1) + cur_v;
+        if (cur >= 0) {
 #pragma unroll
             for (int q = 0; q < NACC; q++) {
-                ACC *dst = &s_acc[q * PLANE + o];
+                ACC *dst = &s_acc[q * PLANE + cur];
                 const ACC x = acc[q];
                 if (KAHAN) {
                     const ACC s = *dst;
                     const ACC t = s + x;
-                    ACC *cp = &s_cmp[q * PLANE + o];
+                    ACC *cp = &s_cmp[q * PLANE + cur];
                     if (fabs(s) >= fabs(x))
                         *cp += (s - t) + x;
                     else
@@ -416,87 +452,101 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     for (int64_t b0 = start; b0 < end; b0 += kApBatch) {
         const int nh = (int)min((int64_t)kApBatch, end - b0);
         __syncthreads();  // previous batch consumed (and the zeroing done)
-        if (tid < kApBatch) {
-            int4 m = make_int4(0, 0, 0, 0), fo = make_int4(0, 0, 0, 0);
-            if (tid < nh) {
-                const E en = *reinterpret_cast<const E *>(entries + (b0 + tid) * EB);
-                const int uc = en.uv & 0xffff, vc = (en.uv >> 16) & 0xffff;
-                const int ou = (int)(short)(en.off & 0xffff), ov = (int)(short)(en.off >> 16);
-                const int cf = en.meta & 0xffff, fidx = (en.meta >> 16) & 0xffff;
-                const int su = a.support[cf * 2], sv = a.support[cf * 2 + 1];
-                const int us = uc - su / 2 - org_u, vs = vc - sv / 2 - org_v;
-                m.x = us | ((us % R) << 16);
-                m.y = vs | ((vs % R) << 16);
-                m.z = su | (sv << 16);
-                m.w = (cf * (2 * a.half_u + 1) + ou + a.half_u) * nwv + ov + a.half_v;
-                fo = make_int4(fidx, ou, ov, su | (sv << 16));
+        if (tid < nh) {
+            const E en = *reinterpret_cast<const E *>(entries + (b0 + tid) * EB);
+            const int uc = en.uv & 0xffff, vc = (en.uv >> 16) & 0xffff;
+            const int ou = (int)(short)(en.off & 0xffff), ov = (int)(short)(en.off >> 16);
+            const int cf = en.meta & 0xffff, fidx = (en.meta >> 16) & 0xffff;
+            const int su = a.support[cf * 2], sv = a.support[cf * 2 + 1];
+            const int us = uc - su / 2 - org_u, vs = vc - sv / 2 - org_v;
+            s_fo[tid] = make_int4(fidx, ou, ov, su | (sv << 16));
+            s_pos[tid] = make_int4(us, us % R, vs, vs % R);
+            s_base[tid] = (unsigned)(((cf * (2 * a.half_u + 1) + ou + a.half_u) * nwv + ov +
+                                      a.half_v) * R * R);
 #pragma unroll
-                for (int q = 0; q < NP; q++)
+            for (int q = 0; q < NP; q++)
 #pragma unroll
-                    for (int c = 0; c < NW; c++) s_wd[tid * NP * NW + q * NW + c] = en.wd[q][c];
-            }
-            s_meta[tid] = m;
-            s_fo[tid] = fo;
+                for (int c = 0; c < NW; c++) s_wd[tid * NP * NW + q * NW + c] = en.wd[q][c];
         }
         __syncthreads();
-        // per-sample phase-gradient factors of the R residues along u and v
+        // per-sample, per-residue tables
         for (int idx = tid; idx < nh * 2 * R; idx += blockDim.x) {
             const int j = idx / (2 * R), q = idx - j * 2 * R;
-            const int axis = q >= R, r = axis ? q - R : q;
+            const bool axis_v = q >= R;
+            const int r = axis_v ? q - R : q;
             const int4 fo = s_fo[j];
-            Cplx<ACC> val;
-            val.x = (ACC)0;
-            val.y = (ACC)0;
-            if (axis == 0) {
+            const int4 ps = s_pos[j];
+            if (!axis_v) {
                 const int su = fo.w & 0xffff;
-                if (r < su) val = pgu[(int64_t)fo.x * a.k_u + a.os_u * (r - su / 2) + fo.y +
-                                      a.k_u / 2];
-                s_pu[j * R + r] = val;
+                int ru = r - ps.y;
+                ru += ru < 0 ? R : 0;
+                const bool valid = ru < su;
+                Cplx<ACC> pu;
+                pu.x = (ACC)0;
+                pu.y = (ACC)0;
+                if (valid) pu = pgu[(int64_t)fo.x * a.k_u + a.os_u * (ru - su / 2) + fo.y +
+                                    a.k_u / 2];
+                s_ui[j * R + r] = valid ? (((ps.x + ru) * (W + 1)) | (ru << 16)) : -1;
+#pragma unroll
+                for (int p = 0; p < NP; p++) {
+                    Cplx<ACC> qv;
+                    if (NW == 2) {
+                        const ACC wr = s_wd[j * NP * NW + p * NW];
+                        const ACC wi = s_wd[j * NP * NW + p * NW + 1];
+                        qv.x = pu.x * wr - pu.y * wi;
+                        qv.y = pu.x * wi + pu.y * wr;
+                    } else {
+                        const ACC w = s_wd[j * NP * NW + p];
+                        qv.x = pu.x * w;
+                        qv.y = pu.y * w;
+                    }
+                    s_q[(j * R + r) * NP + p] = qv;
+                }
             } else {
                 const int sv = fo.w >> 16;
-                if (r < sv) val = pgv[(int64_t)fo.x * a.k_v + a.os_v * (r - sv / 2) + fo.z +
-                                      a.k_v / 2];
-                s_pv[j * R + r] = val;
+                int rv = r - ps.w;
+                rv += rv < 0 ? R : 0;
+                const bool valid = rv < sv;
+                Cplx<ACC> pv;
+                pv.x = (ACC)0;
+                pv.y = (ACC)0;
+                if (valid) pv = pgv[(int64_t)fo.x * a.k_v + a.os_v * (rv - sv / 2) + fo.z +
+                                    a.k_v / 2];
+                s_vi[j * R + r] = valid ? ((ps.z + rv) | (rv << 16)) : -1;
+                s_pv[j * R + r] = pv;
             }
         }
         __syncthreads();
         if (active) {
+            // one sample of look-ahead on the tap load
+            int ui_n = s_ui[ra], vi_n = s_vi[rb];
+            ACC k_n = (ui_n | vi_n) >= 0
+                          ? ckr[s_base[0] + (unsigned)((ui_n >> 16) * R + (vi_n >> 16))]
+                          : (ACC)0;
             for (int j = 0; j < nh; j++) {
-                const int4 m = s_meta[j];
-                const int us = m.x & 0xffff, usm = m.x >> 16;
-                const int vs = m.y & 0xffff, vsm = m.y >> 16;
-                int ru = ra - usm;
-                ru += ru < 0 ? R : 0;
-                int rv = rb - vsm;
-                rv += rv < 0 ? R : 0;
-                const int su = m.z & 0xffff, sv = m.z >> 16;
-                if (ru < su && rv < sv) {
-                    const ACC k = ckr[((int64_t)m.w * R + ru) * R + rv];
-                    const Cplx<ACC> pu = s_pu[j * R + ru], pv = s_pv[j * R + rv];
-                    const ACC pr = pu.x * pv.x - pu.y * pv.y;
-                    const ACC pi = pu.x * pv.y + pu.y * pv.x;
-                    const ACC tr = k * pr, ti = k * pi;
-                    const int cu = us + ru, cv = vs + rv;
-                    if (cu != cur_u || cv != cur_v) {
-                        flush();
-                        cur_u = cu;
-                        cur_v = cv;
+                const int ui = ui_n, vi = vi_n;
+                const ACC k = k_n;
+                if (j + 1 < nh) {
+                    ui_n = s_ui[(j + 1) * R + ra];
+                    vi_n = s_vi[(j + 1) * R + rb];
+                    if ((ui_n | vi_n) >= 0)
+                        k_n = ckr[s_base[j + 1] + (unsigned)((ui_n >> 16) * R + (vi_n >> 16))];
+                }
+                if ((ui | vi) < 0) continue;
+                const int cell = (ui & 0xffff) + (vi & 0xffff);
+                const Cplx<ACC> pv = s_pv[j * R + rb];
+                const ACC tr = k * pv.x, ti = k * pv.y;
+                if (cell != cur) {
+                    flush();
+                    cur = cell;
 #pragma unroll
-                        for (int q = 0; q < NACC; q++) acc[q] = (ACC)0;
-                    }
+                    for (int q = 0; q < NACC; q++) acc[q] = (ACC)0;
+                }
 #pragma unroll
-                    for (int q = 0; q < NP; q++) {
-                        if (NW == 2) {
-                            const ACC wr = s_wd[j * NP * NW + q * NW];
-                            const ACC wi = s_wd[j * NP * NW + q * NW + 1];
-                            acc[2 * q] += tr * wr - ti * wi;
-                            acc[2 * q + 1] += tr * wi + ti * wr;
-                        } else {
-                            const ACC w = s_wd[j * NP * NW + q];
-                            acc[2 * q] += tr * w;
-                            acc[2 * q + 1] += ti * w;
-                        }
-                    }
+                for (int p = 0; p < NP; p++) {
+                    const Cplx<ACC> qv = s_q[(j * R + ra) * NP + p];
+                    acc[2 * p] += tr * qv.x - ti * qv.y;
+                    acc[2 * p + 1] += tr * qv.y + ti * qv.x;
                 }
             }
         }
@@ -660,6 +710,16 @@ template <typename T> __global__ void ap_zero_grid_kernel(T *p, int64_t n) {
         p[i] = (T)0;
 }
 
+__global__ void bin_size_kernel(const unsigned long long *offsets, int64_t n_bins,
+                                unsigned int *size, unsigned int *idx) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_bins;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long n = offsets[i + 1] - offsets[i];
+        size[i] = n > 0xffffffffull ? 0xffffffffu : (unsigned int)n;
+        idx[i] = (unsigned int)i;
+    }
+}
+
 static int grid_blocks(int64_t n, int bs = 256) {
     int64_t b = ceil_div(n, bs);
     if (b > (1 << 20)) b = 1 << 20;
@@ -670,6 +730,32 @@ static int grid_blocks(int64_t n, int bs = 256) {
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
+// Bins in decreasing order of their sample counts (the longest-processing-
+// time-first order: workgroups are dispatched roughly in block order, so the
+// heaviest tiles start first and the tail is short).
+int bin_order_desc(ngc_context *ctx, hipStream_t s, const unsigned long long *offsets,
+                   int64_t n_bins, const unsigned int **order) {
+    NGC_CHECK(n_bins < (int64_t(1) << 32), NGC_ERR_UNSUPPORTED, "too many bins");
+    void *d_k, *d_v, *d_ko, *d_vo, *d_tmp;
+    NGC_TRY(workspace(ctx, "order_keys", n_bins * 4, &d_k));
+    NGC_TRY(workspace(ctx, "order_vals", n_bins * 4, &d_v));
+    NGC_TRY(workspace(ctx, "order_keys_out", n_bins * 4, &d_ko));
+    NGC_TRY(workspace(ctx, "order_vals_out", n_bins * 4, &d_vo));
+    bin_size_kernel<<<grid_blocks(n_bins), 256, 0, s>>>(offsets, n_bins, (unsigned int *)d_k,
+                                                        (unsigned int *)d_v);
+    NGC_HIP(hipGetLastError());
+    size_t tmp = 0;
+    NGC_HIP(hipcub::DeviceRadixSort::SortPairsDescending(
+        nullptr, tmp, (const unsigned int *)d_k, (unsigned int *)d_ko, (const unsigned int *)d_v,
+        (unsigned int *)d_vo, (int)n_bins, 0, 32, s));
+    NGC_TRY(workspace(ctx, "order_tmp", tmp, &d_tmp));
+    NGC_HIP(hipcub::DeviceRadixSort::SortPairsDescending(
+        d_tmp, tmp, (const unsigned int *)d_k, (unsigned int *)d_ko, (const unsigned int *)d_v,
+        (unsigned int *)d_vo, (int)n_bins, 0, 32, s));
+    *order = (const unsigned int *)d_vo;
+    return NGC_OK;
+}
+
 int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     const bool single = c.precision != NGC_PREC_FP64_TILED;
     const bool kahan = c.precision == NGC_PREC_FP32_KAHAN;
@@ -794,12 +880,7 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     const int nacc = 2 * (int)c.n_pol;
     const int nw = c.mode == NGC_GRID_IMAGE ? 2 : 1;
     auto lds_for = [&](int T) {
-        const int W = T + a.R - 1;
-        size_t b = (size_t)(kahan ? 2 : 1) * nacc * W * (W + 1) * acc_bytes;
-        b = (b + 15) / 16 * 16;
-        b += 2 * kApBatch * sizeof(int4) + (size_t)kApBatch * c.n_pol * nw * acc_bytes +
-             2 * (size_t)kApBatch * a.R * 2 * acc_bytes;
-        return b;
+        return (size_t)ap_lds_layout(a.R, T + a.R - 1, (int)c.n_pol, nw, acc_bytes, kahan).bytes;
     };
     int T = 16;
     for (int cand : {32, 24, 16})
@@ -898,14 +979,19 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
         }
         phase_end(ctx, s);
         const int threads = (a.R * a.R + 63) / 64 * 64;
+        phase_begin(ctx, s, "ap_bin_order");
+        const unsigned int *order = nullptr;
+        NGC_TRY(bin_order_desc(ctx, s, (const unsigned long long *)d_off, n_bins, &order));
+        phase_end(ctx, s);
         phase_begin(ctx, s, "ap_grid_tiled");
         if (total > 0) {
             auto kern = ap_tiled_kernel<MODE, NP, ACC, KAHAN>;
             NGC_HIP(hipFuncSetAttribute((const void *)kern,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             kern<<<(unsigned)n_bins, threads, lds, s>>>(
-                a, (const unsigned long long *)d_off, (const char *)d_ent, (const ACC *)d_ckr,
-                (const Cplx<ACC> *)d_pgu, (const Cplx<ACC> *)d_pgv, (SC *)d_scr);
+                a, (const unsigned long long *)d_off, order, (const char *)d_ent,
+                (const ACC *)d_ckr, (const Cplx<ACC> *)d_pgu, (const Cplx<ACC> *)d_pgv,
+                (SC *)d_scr);
             NGC_HIP(hipGetLastError());
         }
         phase_end(ctx, s);

@@ -141,6 +141,11 @@ struct ApTiledCall {
 };
 int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c);
 
+// Bin indices sorted by decreasing sample count (offsets: n_bins + 1 exclusive
+// prefix sums).  The returned device array lives in the context workspace.
+int bin_order_desc(ngc_context *ctx, hipStream_t s, const unsigned long long *offsets,
+                   int64_t n_bins, const unsigned int **order);
+
 // uv scale per channel, in the reference's operation order:
 // -(freq*delta*n_uv)/c  (_standard_grid.py:274-276)
 inline void uv_scale_host(const double *freq, int64_t n_chan, double delta_l,
