@@ -363,6 +363,7 @@ __device__ inline int residue_cell(int w, int r) { return w + ((r - w) & 7); }
 template <int MODE, int NP, typename ACC, bool KAHAN, int TU>
 __global__ void __launch_bounds__(64)
 std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict__ offsets,
+                      const unsigned int *__restrict__ order,
                       const char *__restrict__ entries, const double *__restrict__ cgk,
                       int cgk_len, void *grid_out, int accumulate) {
     constexpr int NC = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
@@ -383,7 +384,7 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     __shared__ ACC s_k[512];
 
     const int lane = threadIdx.x;
-    const int64_t bin = blockIdx.x;
+    const int64_t bin = order[blockIdx.x];
     const int64_t tiles = g.ntu * g.ntv;
     const int64_t a_chan = bin / tiles;
     const int64_t trem = bin - a_chan * tiles;
@@ -735,22 +736,26 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
             NGC_HIP(hipGetLastError());
         }
         phase_end(ctx, s);
-        phase_begin(ctx, s, "std_grid_tiled");
         const unsigned long long *offp = (const unsigned long long *)d_off;
+        phase_begin(ctx, s, "bin_order");
+        const unsigned int *order = nullptr;
+        NGC_TRY(bin_order_desc(ctx, s, offp, n_bins, &order));
+        phase_end(ctx, s);
+        phase_begin(ctx, s, "std_grid_tiled");
         const char *entp = (const char *)d_ent;
         const double *cgkp = (const double *)d_cgk;
         if (compensated && g.tu == 16)
             std_grid_tiled_kernel<MODE, NP, ACC, true, 16><<<(unsigned)n_bins, 64, 0, s>>>(
-                a, g, offp, entp, cgkp, (int)cgk_len, grid, accumulate);
+                a, g, offp, order, entp, cgkp, (int)cgk_len, grid, accumulate);
         else if (compensated)
             std_grid_tiled_kernel<MODE, NP, ACC, true, 32><<<(unsigned)n_bins, 64, 0, s>>>(
-                a, g, offp, entp, cgkp, (int)cgk_len, grid, accumulate);
+                a, g, offp, order, entp, cgkp, (int)cgk_len, grid, accumulate);
         else if (g.tu == 16)
             std_grid_tiled_kernel<MODE, NP, ACC, false, 16><<<(unsigned)n_bins, 64, 0, s>>>(
-                a, g, offp, entp, cgkp, (int)cgk_len, grid, accumulate);
+                a, g, offp, order, entp, cgkp, (int)cgk_len, grid, accumulate);
         else
             std_grid_tiled_kernel<MODE, NP, ACC, false, 32><<<(unsigned)n_bins, 64, 0, s>>>(
-                a, g, offp, entp, cgkp, (int)cgk_len, grid, accumulate);
+                a, g, offp, order, entp, cgkp, (int)cgk_len, grid, accumulate);
         NGC_HIP(hipGetLastError());
         phase_end(ctx, s);
         return NGC_OK;
