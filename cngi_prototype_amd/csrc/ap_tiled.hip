@@ -387,7 +387,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     //   s_fo    [kApBatch] int4   staging: field, offsets, supports
     //   s_pos   [kApBatch] int4   staging: start cells and their residues
     //   s_base  [kApBatch] uint   first tap index of the sample's R x R block
-    //   s_ui    [kApBatch][R] int u-residue a: cell_u*(W+1) | ru << 16, -1 = no tap
+    //   s_ui    [kApBatch][R] int u-residue a: cell_u*(W+1) | ru*R << 16, -1 = no tap
     //   s_vi    [kApBatch][R] int v-residue b: cell_v | rv << 16, -1 = no tap
     //   s_q     [kApBatch][R][NP] u phase factor x weighted visibility per pol
     //   s_pv    [kApBatch][R] v phase factor
@@ -486,7 +486,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                 pu.y = (ACC)0;
                 if (valid) pu = pgu[(int64_t)fo.x * a.k_u + a.os_u * (ru - su / 2) + fo.y +
                                     a.k_u / 2];
-                s_ui[j * R + r] = valid ? (((ps.x + ru) * (W + 1)) | (ru << 16)) : -1;
+                s_ui[j * R + r] = valid ? (((ps.x + ru) * (W + 1)) | ((ru * R) << 16)) : -1;
 #pragma unroll
                 for (int p = 0; p < NP; p++) {
                     Cplx<ACC> qv;
@@ -520,9 +520,8 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
         if (active) {
             // one sample of look-ahead on the tap load
             int ui_n = s_ui[ra], vi_n = s_vi[rb];
-            ACC k_n = (ui_n | vi_n) >= 0
-                          ? ckr[s_base[0] + (unsigned)((ui_n >> 16) * R + (vi_n >> 16))]
-                          : (ACC)0;
+            ACC k_n = (ui_n | vi_n) >= 0 ? ckr[s_base[0] + (unsigned)((ui_n >> 16) + (vi_n >> 16))]
+                                         : (ACC)0;
             for (int j = 0; j < nh; j++) {
                 const int ui = ui_n, vi = vi_n;
                 const ACC k = k_n;
@@ -530,7 +529,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                     ui_n = s_ui[(j + 1) * R + ra];
                     vi_n = s_vi[(j + 1) * R + rb];
                     if ((ui_n | vi_n) >= 0)
-                        k_n = ckr[s_base[j + 1] + (unsigned)((ui_n >> 16) * R + (vi_n >> 16))];
+                        k_n = ckr[s_base[j + 1] + (unsigned)((ui_n >> 16) + (vi_n >> 16))];
                 }
                 if ((ui | vi) < 0) continue;
                 const int cell = (ui & 0xffff) + (vi & 0xffff);
@@ -809,7 +808,7 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     a.weight_dtype = c.weight_dtype;
     a.error = ctx->d_error;
     int ch_log2 = 0;
-    while ((1 << (ch_log2 + 1)) <= std::min<int64_t>(c.n_chan, 8)) ch_log2++;
+    while ((1 << (ch_log2 + 1)) <= std::min<int64_t>(c.n_chan, 2)) ch_log2++;
     a.ch_log2 = ch_log2;
     a.n_cb = ceil_div(c.n_chan, int64_t(1) << ch_log2);
     a.n_tb = ceil_div(c.n_time, int64_t(64) >> ch_log2);
