@@ -29,7 +29,9 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_standard_grid", "ngc_briggs_factors", "ngc_imaging_weight_degrid",
            "ngc_aperture_grid", "ngc_grid_to_image", "ngc_normalize_mosaic",
            "ngc_profile_enable", "ngc_profile_read", "ngc_set_tile_rows",
-           "ngc_gcf_patterns", "ngc_gcf_resize", "ngc_phase_gradient")
+           "ngc_gcf_patterns", "ngc_gcf_resize", "ngc_phase_gradient",
+           "ngc_grid_plan_create", "ngc_grid_plan_execute", "ngc_grid_plan_status",
+           "ngc_grid_plan_describe", "ngc_grid_plan_destroy")
 
 
 class NativeError(RuntimeError):
@@ -73,6 +75,14 @@ _PROTOS = {
     "ngc_profile_read": (_i32, [_vp, ctypes.c_char_p, _i64, _vp, _i64]),
     "ngc_normalize_mosaic": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64,
                                     _i64, _d, _i32]),
+    "ngc_grid_plan_create": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _vp,
+                                    _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
+                                    _i64, _i64, _d, _d, _i64, _i64, _i32, _vp, _vp, _i32,
+                                    ctypes.POINTER(_vp)]),
+    "ngc_grid_plan_execute": (_i32, [_vp, _vp]),
+    "ngc_grid_plan_status": (_i32, [_vp, _vp]),
+    "ngc_grid_plan_describe": (_i32, [_vp, ctypes.c_char_p, _i64]),
+    "ngc_grid_plan_destroy": (_i32, [_vp]),
 }
 
 
@@ -233,6 +243,72 @@ def standard_grid(uvw, weight, freq_chan, chan_map, pol_map, cgk_1d, n_imag_chan
         int(oversampling), int(bool(accumulate)), _ptr(grid), _ptr(sum_weight))
     _check("ngc_standard_grid", rc)
     return grid, sum_weight
+
+
+class GridPlan:
+    """A captured, autotuned ngc_standard_grid call (ngc_grid_plan_*): the
+    tensors given at creation are the plan's inputs and outputs; execute()
+    replays the whole tiled pipeline with one graph launch."""
+
+    def __init__(self, uvw, weight, freq_chan, chan_map, pol_map, cgk_1d, n_imag_chan, n_uv,
+                 delta_lm, support, oversampling, mode, precision, vis=None, flag=None,
+                 grid=None, sum_weight=None, autotune=True):
+        import torch
+        self._ctx = context()
+        self.handle = None
+        n_time, n_baseline, n_chan, n_pol = weight.shape
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        single = prec in (PREC_FP32_KAHAN, PREC_FP32)
+        n_u, n_v = int(n_uv[0]), int(n_uv[1])
+        if mode == GRID_IMAGE:
+            gdt = torch.complex64 if single else torch.complex128
+        else:
+            gdt = torch.float32 if single else torch.float64
+        shape = (int(n_imag_chan), n_pol, n_u, n_v)
+        self.grid = torch.empty(shape, dtype=gdt, device=weight.device) if grid is None else grid
+        self.sum_weight = (torch.empty(shape[:2], dtype=torch.float64, device=weight.device)
+                           if sum_weight is None else sum_weight)
+        # the plan holds device pointers: keep the tensors alive with it
+        self._keep = (uvw, weight, vis, flag)
+        fr, frp = _host(freq_chan, np.float64)
+        cm, cmp_ = _host(chan_map, np.int32)
+        pm, pmp = _host(pol_map, np.int32)
+        cg, cgp = _host(cgk_1d, np.float64)
+        h = _vp()
+        rc = load_library().ngc_grid_plan_create(
+            self._ctx.handle, _stream(), int(mode), prec, _ptr(uvw), _ptr(vis),
+            _dt_vis(vis) if vis is not None else C64, _ptr(weight), _dt_w(weight), _ptr(flag),
+            frp, cmp_, pmp, cgp, len(cg), n_time, n_baseline, n_chan, n_pol, int(n_imag_chan),
+            n_pol, n_u, n_v, float(delta_lm[0]), float(delta_lm[1]), int(support),
+            int(oversampling), 0, _ptr(self.grid), _ptr(self.sum_weight), int(bool(autotune)),
+            ctypes.byref(h))
+        _check("ngc_grid_plan_create", rc)
+        self.handle = h
+
+    def execute(self):
+        _check("ngc_grid_plan_execute",
+               load_library().ngc_grid_plan_execute(self.handle, _stream()))
+        return self.grid, self.sum_weight
+
+    def status(self):
+        _check("ngc_grid_plan_status", load_library().ngc_grid_plan_status(self.handle, _stream()))
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(4096)
+        _check("ngc_grid_plan_describe",
+               load_library().ngc_grid_plan_describe(self.handle, buf, 4096))
+        return buf.value.decode()
+
+    def close(self):
+        if self.handle is not None:
+            load_library().ngc_grid_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def briggs_factors(density_grid, sum_weight, weighting, robust=0.5):

@@ -156,6 +156,10 @@ int ngc_context_create(int device, ngc_context **ctx) {
     NGC_HIP(hipSetDevice(device));
     ngc_context *c = new ngc_context();
     c->device = device;
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        n_cu > 0)
+        c->n_cu = n_cu;
     hipError_t e = hipMalloc(&c->d_error, 4 * sizeof(int));
     if (e != hipSuccess) {
         delete c;

@@ -22,6 +22,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 
 #include "ngc_internal.hpp"
@@ -248,7 +249,8 @@ template <int NP, int NC, typename ACC> struct EntrySize {
 template <bool FILL, int MODE, int NP, typename ACC>
 __global__ void __launch_bounds__(kBlock)
 std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
-               const double *norm_tab, int norm_half, double *sum_weight, int sw_in_lds) {
+               unsigned long long cap, int *overflow, const double *norm_tab, int norm_half,
+               double *sum_weight, int sw_in_lds) {
     constexpr int NC = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
     using E = Entry<NP, NC, ACC>;
     constexpr int EB = EntrySize<NP, NC, ACC>::bytes;
@@ -313,7 +315,10 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
                     basev = __shfl(basev, leader, 64);
                     if (key == lkey) {
                         const unsigned long long pos = basev + __popcll(same & below);
-                        *reinterpret_cast<E *>(entries + pos * EB) = rec;
+                        if (pos < cap)
+                            *reinterpret_cast<E *>(entries + pos * EB) = rec;
+                        else
+                            atomicOr(overflow, 1);
                     }
                 }
                 if (key == lkey) key = -1;
@@ -363,9 +368,10 @@ __device__ inline int residue_cell(int w, int r) { return w + ((r - w) & 7); }
 template <int MODE, int NP, typename ACC, bool KAHAN, int TU>
 __global__ void __launch_bounds__(64)
 std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict__ offsets,
-                      const unsigned int *__restrict__ order,
-                      const char *__restrict__ entries, const double *__restrict__ cgk,
-                      int cgk_len, void *grid_out, int accumulate) {
+                      const unsigned int *__restrict__ order, int *queue, int n_items,
+                      unsigned long long cap, const char *__restrict__ entries,
+                      const double *__restrict__ cgk, int cgk_len, void *grid_out,
+                      int accumulate) {
     constexpr int NC = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
     constexpr int PLANE = TU * kRow;
     constexpr int NPC = NP * NC;
@@ -384,13 +390,22 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     __shared__ ACC s_k[512];
 
     const int lane = threadIdx.x;
-    const int64_t bin = order[blockIdx.x];
+    // persistent: each workgroup takes the next tile (largest bins first) from
+    // the queue until all n_items are done; every wave reaches the exit
+    for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(queue, 1);
+    item = __shfl(item, 0, 64);
+    if (item >= n_items) break;
+    __syncthreads();  // the previous tile's LDS reads are complete
+    const int64_t bin = order[item];
     const int64_t tiles = g.ntu * g.ntv;
     const int64_t a_chan = bin / tiles;
     const int64_t trem = bin - a_chan * tiles;
     const int u_lo = (int)((trem / g.ntv) * TU);
     const int v_lo = (int)((trem % g.ntv) * kTile);
-    const int64_t start = (int64_t)offsets[bin], end = (int64_t)offsets[bin + 1];
+    const int64_t start = (int64_t)min(offsets[bin], cap);
+    const int64_t end = (int64_t)min(offsets[bin + 1], cap);
 
 #pragma unroll 4
     for (int i = lane; i < NPC * PLANE; i += 64) {
@@ -529,6 +544,153 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         else
             *gp = val;
     }
+    }  // work queue
+}
+
+// ---------------------------------------------------------------------------
+// graph-safe helpers: fills, a three-pass exclusive scan, and the
+// largest-first bin order (bins grouped by floor(log2(count)), descending)
+// ---------------------------------------------------------------------------
+constexpr int kScanChunk = 1024;  // elements per scan block (256 threads x 4)
+constexpr int kBuckets = 40;
+
+template <typename T> __global__ void fill_kernel(T *p, int64_t n, T v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <typename T> __global__ void copy_kernel(T *dst, const T *src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__device__ inline unsigned long long block_exclusive_scan_256(unsigned long long x,
+                                                              unsigned long long *total) {
+    __shared__ unsigned long long s_w[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned long long incl = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned long long y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    unsigned long long before = 0, all = 0;
+    for (int i = 0; i < 4; i++) {
+        if (i < w) before += s_w[i];
+        all += s_w[i];
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - x;
+}
+
+__global__ void __launch_bounds__(256)
+scan_reduce_kernel(const unsigned long long *in, int64_t n, unsigned long long *block_sums) {
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * 4;
+    unsigned long long x = 0;
+    for (int k = 0; k < 4; k++)
+        if (base + k < n) x += in[base + k];
+    unsigned long long total;
+    block_exclusive_scan_256(x, &total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// one workgroup: exclusive scan of the block sums, in chunks of 1024
+__global__ void __launch_bounds__(256)
+scan_blocks_kernel(unsigned long long *block_sums, int64_t nb) {
+    unsigned long long carry = 0;
+    for (int64_t c0 = 0; c0 < nb; c0 += kScanChunk) {
+        const int64_t base = c0 + threadIdx.x * 4;
+        unsigned long long v[4], x = 0;
+        for (int k = 0; k < 4; k++) {
+            v[k] = base + k < nb ? block_sums[base + k] : 0ull;
+            x += v[k];
+        }
+        unsigned long long total;
+        unsigned long long e = block_exclusive_scan_256(x, &total) + carry;
+        for (int k = 0; k < 4; k++) {
+            if (base + k < nb) block_sums[base + k] = e;
+            e += v[k];
+        }
+        carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256)
+scan_apply_kernel(const unsigned long long *in, int64_t n, const unsigned long long *block_off,
+                  unsigned long long *out) {
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + threadIdx.x * 4;
+    unsigned long long v[4], x = 0;
+    for (int k = 0; k < 4; k++) {
+        v[k] = base + k < n ? in[base + k] : 0ull;
+        x += v[k];
+    }
+    unsigned long long total;
+    unsigned long long e = block_exclusive_scan_256(x, &total) + block_off[blockIdx.x];
+    for (int k = 0; k < 4; k++) {
+        if (base + k < n) out[base + k] = e;
+        e += v[k];
+    }
+}
+
+__device__ inline int bin_bucket(unsigned long long n) {
+    if (n == 0) return 0;
+    const int b = 64 - __clzll((long long)n);
+    return b < kBuckets ? b : kBuckets - 1;
+}
+
+// hist[b] = number of bins in bucket b (hist zeroed beforehand)
+__global__ void __launch_bounds__(256)
+order_hist_kernel(const unsigned long long *offsets, int64_t n_bins, unsigned int *hist) {
+    __shared__ unsigned int s_h[kBuckets];
+    for (int i = threadIdx.x; i < kBuckets; i += blockDim.x) s_h[i] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_bins;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&s_h[bin_bucket(offsets[i + 1] - offsets[i])], 1u);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kBuckets; i += blockDim.x)
+        if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// cursor[b] = first position of bucket b, buckets in decreasing order
+__global__ void order_prefix_kernel(const unsigned int *hist, unsigned int *cursor) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned int pos = 0;
+        for (int b = kBuckets - 1; b >= 0; b--) {
+            cursor[b] = pos;
+            pos += hist[b];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+order_scatter_kernel(const unsigned long long *offsets, int64_t n_bins, unsigned int *cursor,
+                     unsigned int *order) {
+    __shared__ unsigned int s_h[kBuckets];
+    __shared__ unsigned int s_base[kBuckets];
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n_bins;
+         i0 += (int64_t)gridDim.x * blockDim.x) {
+        for (int i = threadIdx.x; i < kBuckets; i += blockDim.x) s_h[i] = 0;
+        __syncthreads();
+        const int64_t i = i0 + threadIdx.x;
+        int b = 0;
+        unsigned int rank = 0;
+        if (i < n_bins) {
+            b = bin_bucket(offsets[i + 1] - offsets[i]);
+            rank = atomicAdd(&s_h[b], 1u);
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < kBuckets; k += blockDim.x)
+            s_base[k] = s_h[k] ? atomicAdd(&cursor[k], s_h[k]) : 0u;
+        __syncthreads();
+        if (i < n_bins) order[s_base[b] + rank] = (unsigned int)i;
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -566,11 +728,242 @@ static void norm_table(const double *cgk, int64_t cgk_len, int64_t support, int6
         }
 }
 
-static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int precision,
-                              StdArgs a, const double *freq, const int32_t *chan_map,
-                              const int32_t *pol_map, const double *cgk, int64_t cgk_len,
-                              int accumulate, void *grid, double *sum_weight,
-                              double delta_l, double delta_m) {
+// ---------------------------------------------------------------------------
+// tiled pipeline: count -> scan -> fill -> order -> persistent grid.  Every
+// step is a kernel launch (no memset, no host synchronisation between the
+// passes of one phase), so the same enqueue functions serve the eager call
+// and a captured hipGraph.
+// ---------------------------------------------------------------------------
+struct TiledCfg {
+    int tu = 16;        // tile rows (16 or 32); tiles are 32 columns wide
+    int ch_lanes = 2;   // channels per binning wavefront (1, 2, 4 or 8)
+};
+
+struct TiledGeom {
+    TiledCfg cfg;
+    BinGeom g;
+    int64_t n_bins, n_waves, bin_blocks, scan_nb;
+};
+
+struct TiledBufs {
+    unsigned long long *counts = nullptr, *offsets = nullptr, *block_sums = nullptr;
+    char *entries = nullptr;
+    unsigned long long cap = 0;           // entries capacity (records)
+    unsigned int *order = nullptr, *hist = nullptr;  // hist: [2][kBuckets] (counts, cursors)
+    int *flags = nullptr;                 // [0] work queue, [1] entries overflow
+    const double *ntab = nullptr, *cgk = nullptr;
+    int64_t cgk_len = 0;
+    int half = 0;
+};
+
+static TiledGeom tiled_geom(const StdArgs &a, const TiledCfg &cfg) {
+    TiledGeom t;
+    t.cfg = cfg;
+    t.g.tu = cfg.tu;
+    t.g.ntu = ceil_div(a.n_u, cfg.tu);
+    t.g.ntv = ceil_div(a.n_v, kTile);
+    int ch_log2 = 0;
+    while ((1 << (ch_log2 + 1)) <= std::min<int64_t>(a.n_chan, cfg.ch_lanes)) ch_log2++;
+    t.g.ch_log2 = ch_log2;
+    t.g.n_cb = ceil_div(a.n_chan, int64_t(1) << ch_log2);
+    t.g.n_tb = ceil_div(a.n_time, int64_t(64) >> ch_log2);
+    t.n_bins = a.n_imag_chan * t.g.ntu * t.g.ntv;
+    t.n_waves = a.n_baseline * t.g.n_tb * t.g.n_cb;
+    t.bin_blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(t.n_waves, kBlock / 64),
+                                                           1 << 20));
+    t.scan_nb = ceil_div(t.n_bins + 1, kScanChunk);
+    return t;
+}
+
+static int tiled_supported(const StdArgs &a, bool pol_identity, int64_t cgk_len) {
+    NGC_CHECK(a.support <= kMaxTiledSupport && a.support >= 2, NGC_ERR_UNSUPPORTED,
+              "tiled gridder supports 2 <= support <= 8");
+    NGC_CHECK(pol_identity, NGC_ERR_UNSUPPORTED, "tiled gridder needs pol_map = arange(n_pol)");
+    NGC_CHECK(a.n_pol <= 4, NGC_ERR_UNSUPPORTED, "tiled gridder supports up to 4 pols");
+    NGC_CHECK(a.n_u < 65536 && a.n_v < 65536, NGC_ERR_UNSUPPORTED, "grid axis >= 65536");
+    NGC_CHECK(a.os < 30000, NGC_ERR_UNSUPPORTED, "oversampling too large");
+    NGC_CHECK(cgk_len <= 512, NGC_ERR_UNSUPPORTED, "cgk_1d longer than 512");
+    return NGC_OK;
+}
+
+// FP64 tap-sum table for sum_weight; into the context workspace, or into
+// `owned` (a plan's buffer list) when given
+static int tiled_tables(ngc_context *ctx, hipStream_t s, const StdArgs &a, const double *d_cgk,
+                        const double *cgk, int64_t cgk_len, std::vector<void *> *owned,
+                        TiledBufs &bf) {
+    bf.half = (int)(a.os / 2 + 1);
+    std::vector<double> ntab;
+    norm_table(cgk, cgk_len, a.support, a.os, bf.half, ntab);
+    void *d_ntab;
+    if (owned) {
+        NGC_HIP(hipMalloc(&d_ntab, ntab.size() * sizeof(double)));
+        owned->push_back(d_ntab);
+        NGC_HIP(hipMemcpyAsync(d_ntab, ntab.data(), ntab.size() * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+        NGC_HIP(hipStreamSynchronize(s));
+    } else {
+        NGC_TRY(upload(ctx, s, "norm_tab", ntab.data(), ntab.size() * sizeof(double), &d_ntab));
+    }
+    bf.ntab = (const double *)d_ntab;
+    bf.cgk = d_cgk;
+    bf.cgk_len = cgk_len;
+    return NGC_OK;
+}
+
+static int tiled_workspace(ngc_context *ctx, const TiledGeom &tg, TiledBufs &bf) {
+    void *p;
+    NGC_TRY(workspace(ctx, "bin_counts", (tg.n_bins + 1) * 8, &p));
+    bf.counts = (unsigned long long *)p;
+    NGC_TRY(workspace(ctx, "bin_offsets", (tg.n_bins + 1) * 8, &p));
+    bf.offsets = (unsigned long long *)p;
+    NGC_TRY(workspace(ctx, "scan_blocks", tg.scan_nb * 8, &p));
+    bf.block_sums = (unsigned long long *)p;
+    NGC_TRY(workspace(ctx, "bin_order", tg.n_bins * 4 + 4, &p));
+    bf.order = (unsigned int *)p;
+    NGC_TRY(workspace(ctx, "bin_hist", 2 * kBuckets * 4, &p));
+    bf.hist = (unsigned int *)p;
+    NGC_TRY(workspace(ctx, "tiled_flags", 4 * sizeof(int), &p));
+    bf.flags = (int *)p;
+    return NGC_OK;
+}
+
+static int fill_blocks(int64_t n) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 4096));
+}
+
+// count pass and scan: offsets[n_bins] becomes the number of records
+template <int MODE, int NP, typename ACC>
+static int enqueue_count_scan(ngc_context *ctx, hipStream_t s, const StdArgs &a,
+                              const TiledGeom &tg, const TiledBufs &bf, double *sum_weight,
+                              int accumulate) {
+    const int64_t n_sw = a.n_imag_chan * a.n_imag_pol;
+    fill_kernel<unsigned long long><<<fill_blocks(tg.n_bins + 1), 256, 0, s>>>(
+        bf.counts, tg.n_bins + 1, 0ull);
+    if (!accumulate) fill_kernel<double><<<fill_blocks(n_sw), 256, 0, s>>>(sum_weight, n_sw, 0.0);
+    NGC_HIP(hipGetLastError());
+    phase_begin(ctx, s, "bin_count");
+    if (tg.n_waves > 0) {
+        std_bin_kernel<false, MODE, NP, ACC><<<(int)tg.bin_blocks, kBlock, 0, s>>>(
+            a, tg.g, bf.counts, nullptr, 0ull, nullptr, bf.ntab, bf.half, sum_weight, 0);
+        NGC_HIP(hipGetLastError());
+    }
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "bin_scan");
+    scan_reduce_kernel<<<(int)tg.scan_nb, 256, 0, s>>>(bf.counts, tg.n_bins + 1, bf.block_sums);
+    scan_blocks_kernel<<<1, 256, 0, s>>>(bf.block_sums, tg.scan_nb);
+    scan_apply_kernel<<<(int)tg.scan_nb, 256, 0, s>>>(bf.counts, tg.n_bins + 1, bf.block_sums,
+                                                      bf.offsets);
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    return NGC_OK;
+}
+
+// fill pass, bin order and the persistent gridding kernel
+template <int MODE, int NP, typename ACC, bool KAHAN, int TU>
+static int enqueue_fill_grid(ngc_context *ctx, hipStream_t s, const StdArgs &a,
+                             const TiledGeom &tg, const TiledBufs &bf, void *grid,
+                             double *sum_weight, int accumulate) {
+    const int64_t n_sw = a.n_imag_chan * a.n_imag_pol;
+    const int sw_lds = n_sw <= 4096 ? 1 : 0;
+    const size_t sw_shmem = sw_lds ? n_sw * sizeof(double) : 0;
+    copy_kernel<unsigned long long><<<fill_blocks(tg.n_bins + 1), 256, 0, s>>>(
+        bf.counts, bf.offsets, tg.n_bins + 1);
+    fill_kernel<int><<<1, 64, 0, s>>>(bf.flags, 4, 0);
+    fill_kernel<unsigned int><<<1, 64, 0, s>>>(bf.hist, 2 * kBuckets, 0u);
+    NGC_HIP(hipGetLastError());
+    phase_begin(ctx, s, "bin_fill");
+    if (tg.n_waves > 0) {
+        std_bin_kernel<true, MODE, NP, ACC><<<(int)tg.bin_blocks, kBlock, sw_shmem, s>>>(
+            a, tg.g, bf.counts, bf.entries, bf.cap, bf.flags + 1, bf.ntab, bf.half, sum_weight,
+            sw_lds);
+        NGC_HIP(hipGetLastError());
+    }
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "bin_order");
+    order_hist_kernel<<<fill_blocks(tg.n_bins), 256, 0, s>>>(bf.offsets, tg.n_bins, bf.hist);
+    order_prefix_kernel<<<1, 64, 0, s>>>(bf.hist, bf.hist + kBuckets);
+    order_scatter_kernel<<<fill_blocks(tg.n_bins), 256, 0, s>>>(bf.offsets, tg.n_bins,
+                                                                bf.hist + kBuckets, bf.order);
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "std_grid_tiled");
+    auto kern = std_grid_tiled_kernel<MODE, NP, ACC, KAHAN, TU>;
+    int per_cu = 1;
+    NGC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
+    const int64_t wgs = std::max<int64_t>(1, std::min<int64_t>(tg.n_bins,
+                                                               (int64_t)per_cu * ctx->n_cu));
+    kern<<<(unsigned)wgs, 64, 0, s>>>(a, tg.g, bf.offsets, bf.order, bf.flags, (int)tg.n_bins,
+                                      bf.cap, bf.entries, bf.cgk, (int)bf.cgk_len, grid,
+                                      accumulate);
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    return NGC_OK;
+}
+
+// Calls f(mode_c, np_c, acc_c, kahan_c, tu_c) with the compile-time form of
+// (mode, n_pol, precision, tile rows).
+template <typename F>
+static int tiled_dispatch(int mode, int64_t n_pol, int precision, int tu, F &&f) {
+    struct F32 { using type = float; };
+    struct F64 { using type = double; };
+    auto by_tu = [&](auto m, auto np, auto acc, auto k) -> int {
+        if (tu == 32) return f(m, np, acc, k, std::integral_constant<int, 32>{});
+        return f(m, np, acc, k, std::integral_constant<int, 16>{});
+    };
+    auto by_prec = [&](auto m, auto np) -> int {
+        if (precision == NGC_PREC_FP32_KAHAN) return by_tu(m, np, F32{}, std::true_type{});
+        if (precision == NGC_PREC_FP32) return by_tu(m, np, F32{}, std::false_type{});
+        return by_tu(m, np, F64{}, std::false_type{});
+    };
+    auto by_pol = [&](auto m) -> int {
+        switch (n_pol) {
+        case 1: return by_prec(m, std::integral_constant<int, 1>{});
+        case 2: return by_prec(m, std::integral_constant<int, 2>{});
+        case 3: return by_prec(m, std::integral_constant<int, 3>{});
+        default: return by_prec(m, std::integral_constant<int, 4>{});
+        }
+    };
+    if (mode == NGC_GRID_IMAGE) return by_pol(std::integral_constant<int, NGC_GRID_IMAGE>{});
+    return by_pol(std::integral_constant<int, NGC_GRID_PSF>{});
+}
+
+template <int MODE, int NP, typename ACC> constexpr int entry_bytes() {
+    return EntrySize<NP, (MODE == NGC_GRID_IMAGE) ? 2 : 1, ACC>::bytes;
+}
+
+// Eager run: count+scan, read the record count, size the entries, then
+// fill+order+grid.  With `entries_total` the record count is returned.
+static int tiled_run_eager(ngc_context *ctx, hipStream_t s, int mode, int precision,
+                           const StdArgs &a, const TiledGeom &tg, TiledBufs &bf, void *grid,
+                           double *sum_weight, int accumulate,
+                           unsigned long long *entries_total) {
+    return tiled_dispatch(mode, a.n_pol, precision, tg.cfg.tu,
+                          [&](auto m, auto np, auto acc, auto k, auto tu) -> int {
+        constexpr int MODE = decltype(m)::value;
+        constexpr int NP = decltype(np)::value;
+        using ACC = typename decltype(acc)::type;
+        constexpr bool KAHAN = decltype(k)::value;
+        constexpr int TU = decltype(tu)::value;
+        NGC_TRY((enqueue_count_scan<MODE, NP, ACC>(ctx, s, a, tg, bf, sum_weight, accumulate)));
+        unsigned long long total = 0;
+        NGC_HIP(hipMemcpyAsync(&total, bf.offsets + tg.n_bins, sizeof(total),
+                               hipMemcpyDeviceToHost, s));
+        NGC_HIP(hipStreamSynchronize(s));
+        if (entries_total) *entries_total = total;
+        void *p;
+        NGC_TRY(workspace(ctx, "bin_entries", (size_t)(total + 1) * entry_bytes<MODE, NP, ACC>(),
+                          &p));
+        bf.entries = (char *)p;
+        bf.cap = total + 1;
+        return enqueue_fill_grid<MODE, NP, ACC, KAHAN, TU>(ctx, s, a, tg, bf, grid, sum_weight,
+                                                           accumulate);
+    });
+}
+
+static int std_validate(int mode, int precision, const StdArgs &a, const double *freq,
+                        const int32_t *chan_map, const int32_t *pol_map, const double *cgk,
+                        int64_t cgk_len, const void *grid, const double *sum_weight,
+                        bool *pol_identity_out) {
     NGC_CHECK(mode == NGC_GRID_IMAGE || mode == NGC_GRID_PSF || mode == NGC_GRID_DENSITY,
               NGC_ERR_ARG, "mode must be IMAGE, PSF or DENSITY");
     NGC_CHECK(precision >= NGC_PREC_FP64 && precision <= NGC_PREC_FP32, NGC_ERR_ARG,
@@ -604,6 +997,18 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     const int64_t far = sc > a.support - 1 - sc ? sc : a.support - 1 - sc;
     const int64_t max_tap = a.os * far + (a.os + 1) / 2;
     NGC_CHECK(max_tap < cgk_len, NGC_ERR_ARG, "cgk_1d too short for support and oversampling");
+    *pol_identity_out = pol_identity;
+    return NGC_OK;
+}
+
+static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int precision,
+                              StdArgs a, const double *freq, const int32_t *chan_map,
+                              const int32_t *pol_map, const double *cgk, int64_t cgk_len,
+                              int accumulate, void *grid, double *sum_weight,
+                              double delta_l, double delta_m) {
+    bool pol_identity = false;
+    NGC_TRY(std_validate(mode, precision, a, freq, chan_map, pol_map, cgk, cgk_len, grid,
+                         sum_weight, &pol_identity));
 
     std::vector<double> scale;
     uv_scale_host(freq, a.n_chan, delta_l, delta_m, a.n_u, a.n_v, scale);
@@ -657,128 +1062,17 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     }
 
     // ---------------- tiled path ----------------
-    NGC_CHECK(a.support <= kMaxTiledSupport && a.support >= 2, NGC_ERR_UNSUPPORTED,
-              "tiled gridder supports 2 <= support <= 8");
-    NGC_CHECK(pol_identity, NGC_ERR_UNSUPPORTED, "tiled gridder needs pol_map = arange(n_pol)");
-    NGC_CHECK(a.n_pol <= 4, NGC_ERR_UNSUPPORTED, "tiled gridder supports up to 4 pols");
-    NGC_CHECK(a.n_u < 65536 && a.n_v < 65536, NGC_ERR_UNSUPPORTED, "grid axis >= 65536");
-    NGC_CHECK(a.os < 30000, NGC_ERR_UNSUPPORTED, "oversampling too large");
-    NGC_CHECK(cgk_len <= 512, NGC_ERR_UNSUPPORTED, "cgk_1d longer than 512");
-
-    const int half = (int)(a.os / 2 + 1);
-    std::vector<double> ntab;
-    norm_table(cgk, cgk_len, a.support, a.os, half, ntab);
-    void *d_ntab;
-    NGC_TRY(upload(ctx, s, "norm_tab", ntab.data(), ntab.size() * sizeof(double), &d_ntab));
-
-    BinGeom g;
-    g.tu = ctx->tile_rows == 16 ? 16 : 32;
-    g.ntu = ceil_div(a.n_u, g.tu);
-    g.ntv = ceil_div(a.n_v, kTile);
-    // two channels per integration and 32 integrations per wavefront: 64-byte
-    // contiguous visibility reads, and runs of 32 integrations per bin
-    int ch_log2 = 0;
-    while ((1 << (ch_log2 + 1)) <= std::min<int64_t>(a.n_chan, 2)) ch_log2++;
-    g.ch_log2 = ch_log2;
-    g.n_cb = ceil_div(a.n_chan, int64_t(1) << ch_log2);
-    g.n_tb = ceil_div(a.n_time, int64_t(64) >> ch_log2);
-    const int64_t n_bins = a.n_imag_chan * g.ntu * g.ntv;
-    NGC_CHECK(n_bins < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED, "too many tiles");
-    const int64_t n_waves = a.n_baseline * g.n_tb * g.n_cb;
-    int64_t bin_blocks = ceil_div(n_waves, kBlock / 64);
-    if (bin_blocks > (1 << 20)) bin_blocks = 1 << 20;
-    if (bin_blocks < 1) bin_blocks = 1;
-
-    void *d_counts, *d_off, *d_scan_tmp, *d_ent;
-    NGC_TRY(workspace(ctx, "bin_counts", (n_bins + 1) * sizeof(unsigned long long), &d_counts));
-    NGC_TRY(workspace(ctx, "bin_offsets", (n_bins + 1) * sizeof(unsigned long long), &d_off));
-    NGC_HIP(hipMemsetAsync(d_counts, 0, (n_bins + 1) * sizeof(unsigned long long), s));
-    if (!accumulate) NGC_HIP(hipMemsetAsync(sum_weight, 0, n_sw * sizeof(double), s));
-    const int sw_lds = n_sw <= 4096 ? 1 : 0;
-    const size_t sw_shmem = sw_lds ? n_sw * sizeof(double) : 0;
-
-    // one instantiation per (mode, pols, precision)
-    int rc = NGC_OK;
-    auto run = [&](auto mode_c, auto np_c, auto acc_c) -> int {
-        constexpr int MODE = decltype(mode_c)::value;
-        constexpr int NP = decltype(np_c)::value;
-        using ACC = typename decltype(acc_c)::type;
-        constexpr int NC = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
-        constexpr int EB = EntrySize<NP, NC, ACC>::bytes;
-        unsigned long long *counts = (unsigned long long *)d_counts;
-        phase_begin(ctx, s, "bin_count");
-        if (n_waves > 0) {
-            std_bin_kernel<false, MODE, NP, ACC><<<(int)bin_blocks, kBlock, 0, s>>>(
-                a, g, counts, nullptr, (const double *)d_ntab, half, sum_weight, 0);
-            NGC_HIP(hipGetLastError());
-        }
-        phase_end(ctx, s);
-        phase_begin(ctx, s, "bin_scan");
-        size_t tmp_bytes = 0;
-        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts,
-                                                 (unsigned long long *)d_off, n_bins + 1, s));
-        NGC_TRY(workspace(ctx, "scan_tmp", tmp_bytes, &d_scan_tmp));
-        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp, tmp_bytes, counts,
-                                                 (unsigned long long *)d_off, n_bins + 1, s));
-        phase_end(ctx, s);
-        unsigned long long total = 0;
-        NGC_HIP(hipMemcpyAsync(&total, (unsigned long long *)d_off + n_bins, sizeof(total),
-                               hipMemcpyDeviceToHost, s));
-        NGC_HIP(hipStreamSynchronize(s));
-        NGC_TRY(workspace(ctx, "bin_entries", (size_t)(total + 1) * EB, &d_ent));
-        // cursors start at the offsets
-        NGC_HIP(hipMemcpyAsync(d_counts, d_off, (n_bins + 1) * sizeof(unsigned long long),
-                               hipMemcpyDeviceToDevice, s));
-        phase_begin(ctx, s, "bin_fill");
-        if (n_waves > 0) {
-            std_bin_kernel<true, MODE, NP, ACC><<<(int)bin_blocks, kBlock, sw_shmem, s>>>(
-                a, g, counts, (char *)d_ent, (const double *)d_ntab, half, sum_weight, sw_lds);
-            NGC_HIP(hipGetLastError());
-        }
-        phase_end(ctx, s);
-        const unsigned long long *offp = (const unsigned long long *)d_off;
-        phase_begin(ctx, s, "bin_order");
-        const unsigned int *order = nullptr;
-        NGC_TRY(bin_order_desc(ctx, s, offp, n_bins, &order));
-        phase_end(ctx, s);
-        phase_begin(ctx, s, "std_grid_tiled");
-        const char *entp = (const char *)d_ent;
-        const double *cgkp = (const double *)d_cgk;
-        if (compensated && g.tu == 16)
-            std_grid_tiled_kernel<MODE, NP, ACC, true, 16><<<(unsigned)n_bins, 64, 0, s>>>(
-                a, g, offp, order, entp, cgkp, (int)cgk_len, grid, accumulate);
-        else if (compensated)
-            std_grid_tiled_kernel<MODE, NP, ACC, true, 32><<<(unsigned)n_bins, 64, 0, s>>>(
-                a, g, offp, order, entp, cgkp, (int)cgk_len, grid, accumulate);
-        else if (g.tu == 16)
-            std_grid_tiled_kernel<MODE, NP, ACC, false, 16><<<(unsigned)n_bins, 64, 0, s>>>(
-                a, g, offp, order, entp, cgkp, (int)cgk_len, grid, accumulate);
-        else
-            std_grid_tiled_kernel<MODE, NP, ACC, false, 32><<<(unsigned)n_bins, 64, 0, s>>>(
-                a, g, offp, order, entp, cgkp, (int)cgk_len, grid, accumulate);
-        NGC_HIP(hipGetLastError());
-        phase_end(ctx, s);
-        return NGC_OK;
-    };
-    using I0 = std::integral_constant<int, NGC_GRID_IMAGE>;
-    using I1 = std::integral_constant<int, NGC_GRID_PSF>;
-    struct F32 { using type = float; };
-    struct F64 { using type = double; };
-    auto by_pol = [&](auto mode_c, auto acc_c) -> int {
-        switch (a.n_pol) {
-        case 1: return run(mode_c, std::integral_constant<int, 1>{}, acc_c);
-        case 2: return run(mode_c, std::integral_constant<int, 2>{}, acc_c);
-        case 3: return run(mode_c, std::integral_constant<int, 3>{}, acc_c);
-        default: return run(mode_c, std::integral_constant<int, 4>{}, acc_c);
-        }
-    };
-    if (mode == NGC_GRID_IMAGE)
-        rc = single ? by_pol(I0{}, F32{}) : by_pol(I0{}, F64{});
-    else
-        rc = single ? by_pol(I1{}, F32{}) : by_pol(I1{}, F64{});
-    return rc;
+    NGC_TRY(tiled_supported(a, pol_identity, cgk_len));
+    TiledCfg cfg;
+    cfg.tu = ctx->tile_rows == 32 ? 32 : 16;
+    cfg.ch_lanes = ctx->bin_chan_lanes;
+    const TiledGeom tg = tiled_geom(a, cfg);
+    TiledBufs bf{};
+    NGC_TRY(tiled_tables(ctx, s, a, (const double *)d_cgk, cgk, cgk_len, nullptr, bf));
+    NGC_TRY(tiled_workspace(ctx, tg, bf));
+    return tiled_run_eager(ctx, s, mode, precision, a, tg, bf, grid, sum_weight, accumulate,
+                           nullptr);
 }
-
 }  // namespace ngc
 
 extern "C" int ngc_standard_grid(ngc_context *ctx, void *stream, int mode, int precision,
@@ -820,3 +1114,279 @@ extern "C" int ngc_standard_grid(ngc_context *ctx, void *stream, int mode, int p
                                    cgk_1d, cgk_len, accumulate, grid, sum_weight, delta_l,
                                    delta_m);
 }
+
+// ---------------------------------------------------------------------------
+// gridding plans: the whole tiled pipeline of one ngc_standard_grid call,
+// autotuned once and captured in a hipGraph
+// ---------------------------------------------------------------------------
+struct ngc_grid_plan {
+    ngc_context *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    std::vector<void *> owned;
+    int *flags = nullptr;
+    std::string info;
+};
+
+namespace ngc {
+
+static void plan_free(ngc_grid_plan *p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    for (void *q : p->owned) (void)hipFree(q);
+    delete p;
+}
+
+static int plan_alloc(ngc_grid_plan *p, size_t bytes, void **out) {
+    void *q = nullptr;
+    NGC_HIP(hipMalloc(&q, bytes ? bytes : 256));
+    p->owned.push_back(q);
+    *out = q;
+    return NGC_OK;
+}
+
+static int plan_upload(ngc_grid_plan *p, hipStream_t s, const void *host, size_t bytes,
+                       void **out) {
+    NGC_TRY(plan_alloc(p, bytes, out));
+    NGC_HIP(hipMemcpyAsync(*out, host, bytes, hipMemcpyHostToDevice, s));
+    return NGC_OK;
+}
+
+static std::string shape_key(int mode, int precision, const StdArgs &a) {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "%d/%d/%lld/%lld/%lld/%lld/%lld/%lld/%lld/%lld/%lld", mode,
+             precision, (long long)a.n_time, (long long)a.n_baseline, (long long)a.n_chan,
+             (long long)a.n_pol, (long long)a.n_imag_chan, (long long)a.n_u, (long long)a.n_v,
+             (long long)a.support, (long long)a.os);
+    return buf;
+}
+
+static int plan_create(ngc_context *ctx, hipStream_t s, int mode, int precision, StdArgs a,
+                       const double *freq, const int32_t *chan_map, const int32_t *pol_map,
+                       const double *cgk, int64_t cgk_len, double delta_l, double delta_m,
+                       int accumulate, void *grid, double *sum_weight, int autotune,
+                       ngc_grid_plan *p) {
+    bool pol_identity = false;
+    NGC_TRY(std_validate(mode, precision, a, freq, chan_map, pol_map, cgk, cgk_len, grid,
+                         sum_weight, &pol_identity));
+    NGC_CHECK(mode == NGC_GRID_IMAGE || mode == NGC_GRID_PSF, NGC_ERR_UNSUPPORTED,
+              "gridding plans cover image and PSF modes");
+    NGC_CHECK(precision != NGC_PREC_FP64, NGC_ERR_UNSUPPORTED,
+              "gridding plans run the tiled forms (FP64_TILED, FP32_KAHAN, FP32)");
+    NGC_TRY(tiled_supported(a, pol_identity, cgk_len));
+
+    // plan-owned tables
+    std::vector<double> scale;
+    uv_scale_host(freq, a.n_chan, delta_l, delta_m, a.n_u, a.n_v, scale);
+    void *d_scale, *d_cmap, *d_pmap, *d_cgk;
+    NGC_TRY(plan_upload(p, s, scale.data(), scale.size() * sizeof(double), &d_scale));
+    NGC_TRY(plan_upload(p, s, chan_map, a.n_chan * sizeof(int32_t) + 4, &d_cmap));
+    NGC_TRY(plan_upload(p, s, pol_map, a.n_pol * sizeof(int32_t), &d_pmap));
+    NGC_TRY(plan_upload(p, s, cgk, cgk_len * sizeof(double), &d_cgk));
+    a.uv_scale = (const double *)d_scale;
+    a.chan_map = (const int *)d_cmap;
+    a.pol_map = (const int *)d_pmap;
+    TiledBufs tables{};
+    NGC_TRY(tiled_tables(ctx, s, a, (const double *)d_cgk, cgk, cgk_len, &p->owned, tables));
+
+    const bool saved_profile = ctx->profile;
+    ctx->profile = false;
+    struct Restore {
+        ngc_context *c;
+        bool v;
+        ~Restore() { c->profile = v; }
+    } restore{ctx, saved_profile};
+
+    // autotuner: time the whole eager pipeline for every candidate shape on
+    // these inputs (one warm-up and one timed run each) and keep the fastest
+    const std::string key = shape_key(mode, precision, a);
+    TiledCfg best;
+    best.tu = ctx->tile_rows;
+    best.ch_lanes = ctx->bin_chan_lanes;
+    std::string tune_log;
+    auto it = ctx->tuned.find(key);
+    if (autotune || it == ctx->tuned.end()) {
+        if (autotune) {
+            hipEvent_t e0, e1;
+            NGC_HIP(hipEventCreate(&e0));
+            NGC_HIP(hipEventCreate(&e1));
+            float best_ms = 1e30f;
+            for (int tu : {16, 32})
+                for (int cl : {1, 2, 4, 8}) {
+                    TiledCfg cfg;
+                    cfg.tu = tu;
+                    cfg.ch_lanes = cl;
+                    const TiledGeom tg = tiled_geom(a, cfg);
+                    TiledBufs bf = tables;
+                    NGC_TRY(tiled_workspace(ctx, tg, bf));
+                    float ms = 0.f;
+                    for (int rep = 0; rep < 2; rep++) {
+                        NGC_HIP(hipEventRecord(e0, s));
+                        NGC_TRY(tiled_run_eager(ctx, s, mode, precision, a, tg, bf, grid,
+                                                sum_weight, 0, nullptr));
+                        NGC_HIP(hipEventRecord(e1, s));
+                        NGC_HIP(hipEventSynchronize(e1));
+                        NGC_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    }
+                    char b[64];
+                    snprintf(b, sizeof(b), "%s%dx32/%d:%.3f", tune_log.empty() ? "" : " ", tu,
+                             cl, ms);
+                    tune_log += b;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = cfg;
+                    }
+                }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            ctx->tuned[key] = {best.tu, best.ch_lanes};
+        }
+    } else {
+        best.tu = it->second.first;
+        best.ch_lanes = it->second.second;
+    }
+
+    // plan-owned pipeline buffers sized by one eager count pass
+    const TiledGeom tg = tiled_geom(a, best);
+    TiledBufs bf = tables;
+    void *q;
+    NGC_TRY(plan_alloc(p, (tg.n_bins + 1) * 8, &q));
+    bf.counts = (unsigned long long *)q;
+    NGC_TRY(plan_alloc(p, (tg.n_bins + 1) * 8, &q));
+    bf.offsets = (unsigned long long *)q;
+    NGC_TRY(plan_alloc(p, tg.scan_nb * 8, &q));
+    bf.block_sums = (unsigned long long *)q;
+    NGC_TRY(plan_alloc(p, tg.n_bins * 4 + 4, &q));
+    bf.order = (unsigned int *)q;
+    NGC_TRY(plan_alloc(p, 2 * kBuckets * 4, &q));
+    bf.hist = (unsigned int *)q;
+    NGC_TRY(plan_alloc(p, 4 * sizeof(int), &q));
+    bf.flags = (int *)q;
+    p->flags = bf.flags;
+    int rc = tiled_dispatch(mode, a.n_pol, precision, best.tu,
+                            [&](auto m, auto np, auto acc, auto k, auto tu) -> int {
+        constexpr int MODE = decltype(m)::value;
+        constexpr int NP = decltype(np)::value;
+        using ACC = typename decltype(acc)::type;
+        constexpr bool KAHAN = decltype(k)::value;
+        constexpr int TU = decltype(tu)::value;
+        NGC_TRY((enqueue_count_scan<MODE, NP, ACC>(ctx, s, a, tg, bf, sum_weight, accumulate)));
+        unsigned long long total = 0;
+        NGC_HIP(hipMemcpyAsync(&total, bf.offsets + tg.n_bins, sizeof(total),
+                               hipMemcpyDeviceToHost, s));
+        NGC_HIP(hipStreamSynchronize(s));
+        // room for the record count to grow by 1/16 (inputs rewritten in place)
+        bf.cap = total + total / 16 + 1024;
+        void *e;
+        NGC_TRY(plan_alloc(p, (size_t)bf.cap * entry_bytes<MODE, NP, ACC>(), &e));
+        bf.entries = (char *)e;
+        NGC_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        NGC_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        int r = enqueue_count_scan<MODE, NP, ACC>(ctx, p->cap_stream, a, tg, bf, sum_weight,
+                                                  accumulate);
+        if (r == NGC_OK)
+            r = enqueue_fill_grid<MODE, NP, ACC, KAHAN, TU>(ctx, p->cap_stream, a, tg, bf, grid,
+                                                            sum_weight, accumulate);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(p->cap_stream, &graph);
+        p->graph = graph;
+        if (r != NGC_OK) return r;
+        NGC_CHECK(ec == hipSuccess && graph, NGC_ERR_HIP, "hipStreamEndCapture failed");
+        NGC_HIP(hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0));
+        char b[256];
+        snprintf(b, sizeof(b), "tile %dx32, %d channel lanes, %lld bins, %llu records (cap %llu)",
+                 best.tu, best.ch_lanes, (long long)tg.n_bins, total, bf.cap);
+        p->info = b;
+        return NGC_OK;
+    });
+    if (!tune_log.empty()) p->info += "; tuning ms " + tune_log;
+    return rc;
+}
+
+}  // namespace ngc
+
+extern "C" {
+
+int ngc_grid_plan_create(ngc_context *ctx, void *stream, int mode, int precision,
+                         const double *uvw, const void *vis, int vis_dtype, const void *weight,
+                         int weight_dtype, const uint8_t *flag, const double *freq_chan,
+                         const int32_t *chan_map, const int32_t *pol_map, const double *cgk_1d,
+                         int64_t cgk_len, int64_t n_time, int64_t n_baseline, int64_t n_chan,
+                         int64_t n_pol, int64_t n_imag_chan, int64_t n_imag_pol, int64_t n_u,
+                         int64_t n_v, double delta_l, double delta_m, int64_t support,
+                         int64_t oversampling, int accumulate, void *grid, double *sum_weight,
+                         int autotune, ngc_grid_plan **plan) {
+    NGC_CHECK(ctx && plan, NGC_ERR_ARG, "ctx or plan is NULL");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = (hipStream_t)stream;
+    NGC_TRY(ngc::enter_stream(ctx, s));
+    ngc::StdArgs a{};
+    a.uvw = uvw;
+    a.vis = vis;
+    a.weight = weight;
+    a.flag = flag;
+    a.n_time = n_time;
+    a.n_baseline = n_baseline;
+    a.n_chan = n_chan;
+    a.n_pol = n_pol;
+    a.n_imag_chan = n_imag_chan;
+    a.n_imag_pol = n_imag_pol;
+    a.n_u = n_u;
+    a.n_v = n_v;
+    a.support = support;
+    a.os = oversampling;
+    a.vis_dtype = vis_dtype;
+    a.weight_dtype = weight_dtype;
+    ngc_grid_plan *p = new ngc_grid_plan();
+    p->ctx = ctx;
+    const int rc = ngc::plan_create(ctx, s, mode, precision, a, freq_chan, chan_map, pol_map,
+                                    cgk_1d, cgk_len, delta_l, delta_m, accumulate, grid,
+                                    sum_weight, autotune, p);
+    if (rc != NGC_OK) {
+        (void)hipStreamSynchronize(s);
+        ngc::plan_free(p);
+        return rc;
+    }
+    *plan = p;
+    return NGC_OK;
+}
+
+int ngc_grid_plan_execute(ngc_grid_plan *plan, void *stream) {
+    NGC_CHECK(plan && plan->exec, NGC_ERR_ARG, "plan is NULL");
+    NGC_HIP(hipSetDevice(plan->ctx->device));
+    NGC_HIP(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+    return NGC_OK;
+}
+
+int ngc_grid_plan_status(ngc_grid_plan *plan, void *stream) {
+    NGC_CHECK(plan, NGC_ERR_ARG, "plan is NULL");
+    NGC_HIP(hipSetDevice(plan->ctx->device));
+    int flags[2] = {0, 0};
+    NGC_HIP(hipMemcpyAsync(flags, plan->flags, sizeof(flags), hipMemcpyDeviceToHost,
+                           (hipStream_t)stream));
+    NGC_HIP(hipStreamSynchronize((hipStream_t)stream));
+    NGC_CHECK(flags[1] == 0, NGC_ERR_ARG,
+              "the inputs now bin more records than the plan holds; make a new plan");
+    return NGC_OK;
+}
+
+int ngc_grid_plan_describe(ngc_grid_plan *plan, char *buf, int64_t len) {
+    NGC_CHECK(plan && buf && len > 0, NGC_ERR_ARG, "bad arguments");
+    const size_t k = std::min<size_t>(plan->info.size(), (size_t)len - 1);
+    memcpy(buf, plan->info.data(), k);
+    buf[k] = 0;
+    return NGC_OK;
+}
+
+int ngc_grid_plan_destroy(ngc_grid_plan *plan) {
+    if (!plan) return NGC_OK;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipDeviceSynchronize();
+    ngc::plan_free(plan);
+    return NGC_OK;
+}
+
+}  // extern "C"

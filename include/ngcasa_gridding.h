@@ -280,6 +280,35 @@ int ngc_phase_gradient(ngc_context *ctx, void *stream, const double *pix, int64_
                        int64_t k_u, int64_t k_v, double *out);
 
 /*
+ * Gridding plans (no counterpart in the reference, whose graph layer rebuilds
+ * its dask graph on every call).  A plan fixes every argument of one
+ * ngc_standard_grid call in a tiled precision (IMAGE or PSF mode): the input,
+ * grid and sum_weight pointers, the shapes and the tables.  Creation runs the
+ * autotuner when `autotune` is nonzero (tile rows 16/32 x 1/2/4/8 channels per
+ * binning wavefront, each timed on these inputs; the choice is cached per
+ * shape in the context), sizes the bins with one counting pass, and captures
+ * the whole pipeline (count, scan, fill, largest-first bin order, persistent
+ * gridding kernel) in a hipGraph.  ngc_grid_plan_execute replays it on
+ * `stream` with one launch, asynchronously; the inputs may be rewritten in
+ * place between replays.  ngc_grid_plan_status synchronises `stream` and
+ * fails if the inputs now bin more records than the plan holds (1/16 spare).
+ */
+typedef struct ngc_grid_plan ngc_grid_plan;
+int ngc_grid_plan_create(ngc_context *ctx, void *stream, int mode, int precision,
+                         const double *uvw, const void *vis, int vis_dtype, const void *weight,
+                         int weight_dtype, const uint8_t *flag, const double *freq_chan,
+                         const int32_t *chan_map, const int32_t *pol_map, const double *cgk_1d,
+                         int64_t cgk_len, int64_t n_time, int64_t n_baseline, int64_t n_chan,
+                         int64_t n_pol, int64_t n_imag_chan, int64_t n_imag_pol, int64_t n_u,
+                         int64_t n_v, double delta_l, double delta_m, int64_t support,
+                         int64_t oversampling, int accumulate, void *grid, double *sum_weight,
+                         int autotune, ngc_grid_plan **plan);
+int ngc_grid_plan_execute(ngc_grid_plan *plan, void *stream);
+int ngc_grid_plan_status(ngc_grid_plan *plan, void *stream);
+int ngc_grid_plan_describe(ngc_grid_plan *plan, char *buf, int64_t len);
+int ngc_grid_plan_destroy(ngc_grid_plan *plan);
+
+/*
  * Tile shape of the uv-tiled gridders: 16 or 32 grid rows per tile (the tile
  * is always 32 columns wide).  Set by the autotuner (no counterpart in the
  * reference, whose gridder has no tiles).  Default 16.
