@@ -67,9 +67,10 @@ CONFIGS = {
     # Briggs 0.5, 4 pols, 10% flags, 2048^2 at 0.1"
     "C2": Config("C2", 50, "lognormal", 3000.0, -23.0234, -23.0, 2000, 4.0, 100e9, 10e6, 64, 4,
                  2048, 0.1, 200, flag_frac=0.10, n_blob=5, fov_deg=0.02, seed=2),
-    # batched cube: 256 chan x 2 pol, uneven per-plane flags 0-60%
+    # batched cube: 256 chan x 2 pol, uneven per-plane flags 0-60%; cell 0.05"
+    # keeps the longest (6 km) baselines on the 1228^2 grid at 100.5 GHz
     "C3": Config("C3", 50, "lognormal", 3000.0, -23.0234, -23.0, 1000, 8.0, 100e9, 2e6, 256, 2,
-                 1024, 0.2, 100, flag_per_plane_max=0.6, spectral_index=True, fov_deg=0.02,
+                 1024, 0.05, 100, flag_per_plane_max=0.6, spectral_index=True, fov_deg=0.01,
                  seed=3),
     # 25-pointing heterogeneous mosaic: 12 x 7 m + 40 x 12 m, 230 GHz, 4 h of 10 s dumps
     # (cell 0.09": the 12 m - 12 m GCF support is 17 at oversampling 20,
