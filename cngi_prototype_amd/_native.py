@@ -31,7 +31,8 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_profile_enable", "ngc_profile_read", "ngc_set_tile_rows",
            "ngc_gcf_patterns", "ngc_gcf_resize", "ngc_phase_gradient",
            "ngc_grid_plan_create", "ngc_grid_plan_execute", "ngc_grid_plan_status",
-           "ngc_grid_plan_describe", "ngc_grid_plan_destroy")
+           "ngc_grid_plan_describe", "ngc_grid_plan_destroy", "ngc_bf16_gemm",
+           "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step")
 
 
 class NativeError(RuntimeError):
@@ -83,6 +84,12 @@ _PROTOS = {
     "ngc_grid_plan_status": (_i32, [_vp, _vp]),
     "ngc_grid_plan_describe": (_i32, [_vp, ctypes.c_char_p, _i64]),
     "ngc_grid_plan_destroy": (_i32, [_vp]),
+    "ngc_bf16_gemm": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
+                             _i32, _i32, _vp, _i64, _vp]),
+    "ngc_tanh_backward": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp]),
+    "ngc_col_sum": (_i32, [_vp, _vp, _i64, _i64, _vp, _vp]),
+    "ngc_mse_grad": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "ngc_adam_step": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _i64]),
 }
 
 
@@ -523,3 +530,66 @@ def phase_gradient(pix, resize_conv_size):
                                            _ptr(out))
     _check("ngc_phase_gradient", rc)
     return out
+
+
+# ---------------------------------------------------------------------------
+# dense-layer primitives of the kernel surrogate (float32 device tensors)
+# ---------------------------------------------------------------------------
+def bf16_gemm(A, B, bias=None, act=0, out=None, pre=None, accumulate=False, trans_a=False,
+              trans_b=False):
+    """out = act(op(A) @ op(B) + bias) on the matrix cores (bf16 operands, FP32
+    accumulation).  op(X) = X.T when trans_x.  Returns out (M x N float32)."""
+    import torch
+    for t, n in ((A, "A"), (B, "B")):
+        _dev(t, n)
+        if t.dtype != torch.float32 or t.dim() != 2:
+            raise TypeError(f"{n} must be a 2-D float32 tensor")
+    M, K = (A.shape[1], A.shape[0]) if trans_a else (A.shape[0], A.shape[1])
+    Kb, N = (B.shape[1], B.shape[0]) if trans_b else (B.shape[0], B.shape[1])
+    if K != Kb:
+        raise ValueError("inner dimensions differ")
+    sam, sak = (A.stride(1), A.stride(0)) if trans_a else (A.stride(0), A.stride(1))
+    sbk, sbn = (B.stride(1), B.stride(0)) if trans_b else (B.stride(0), B.stride(1))
+    if out is None:
+        out = torch.empty((M, N), dtype=torch.float32, device=A.device)
+    if not out.is_contiguous() or tuple(out.shape) != (M, N):
+        raise ValueError("out must be a contiguous M x N tensor")
+    if pre is not None and (not pre.is_contiguous() or tuple(pre.shape) != (M, N)):
+        raise ValueError("pre must be a contiguous M x N tensor")
+    rc = load_library().ngc_bf16_gemm(context().handle, _stream(), M, N, K, _ptr(A), sam, sak,
+                                      _ptr(B), sbk, sbn, _ptr(bias), int(act),
+                                      int(bool(accumulate)), _ptr(out), N, _ptr(pre))
+    _check("ngc_bf16_gemm", rc)
+    return out
+
+
+def tanh_backward(dA, A, out=None):
+    import torch
+    out = torch.empty_like(dA) if out is None else out
+    _check("ngc_tanh_backward", load_library().ngc_tanh_backward(
+        context().handle, _stream(), dA.numel(), _ptr(dA), _ptr(A), _ptr(out)))
+    return out
+
+
+def col_sum(X, out=None):
+    import torch
+    out = torch.empty(X.shape[1], dtype=torch.float32, device=X.device) if out is None else out
+    _check("ngc_col_sum", load_library().ngc_col_sum(context().handle, _stream(), X.shape[0],
+                                                     X.shape[1], _ptr(X), _ptr(out)))
+    return out
+
+
+def mse_grad(y, t, g=None, loss=None):
+    """Returns (gradient, device scalar loss)."""
+    import torch
+    g = torch.empty_like(y) if g is None else g
+    loss = torch.empty(1, dtype=torch.float64, device=y.device) if loss is None else loss
+    _check("ngc_mse_grad", load_library().ngc_mse_grad(context().handle, _stream(), y.numel(),
+                                                       _ptr(y), _ptr(t), _ptr(g), _ptr(loss)))
+    return g, loss
+
+
+def adam_step(w, g, m, v, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
+    _check("ngc_adam_step", load_library().ngc_adam_step(
+        context().handle, _stream(), w.numel(), _ptr(w), _ptr(g), _ptr(m), _ptr(v), float(lr),
+        float(beta1), float(beta2), float(eps), int(step)))

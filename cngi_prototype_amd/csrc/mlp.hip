@@ -1,0 +1,224 @@
+// Dense layers of the learned A-term kernel surrogate, on gfx950 matrix cores.
+//
+// No counterpart in the reference: the surrogate replaces per-channel
+// evaluations of the exact FFT-of-aperture convolution function
+// (make_gridding_convolution_function.py:161-280).  The network is a plain
+// MLP; every product runs as bf16 x bf16 -> FP32 MFMA (v_mfma_f32_16x16x16_bf16):
+// operands are held in FP32 in memory (the optimiser's master copy) and rounded
+// to bf16 (round to nearest even) as they are loaded, so the weights the
+// network computes with are bf16 and every accumulation is FP32.
+//
+//   ngc_bf16_gemm     C = act(op(A) op(B) + bias), strided A and B (so X^T dY and
+//                     dY W^T need no transposed copies), optional pre-activation
+//                     output, optional accumulate into C
+//   ngc_tanh_backward dZ = dA * (1 - A^2)
+//   ngc_col_sum       column sums (bias gradients)
+//   ngc_mse_grad      d/dy of mean((y - t)^2) and the loss
+//   ngc_adam_step     Adam update of FP32 master weights
+#include <algorithm>
+
+#include "ngc_internal.hpp"
+
+namespace ngc {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16 bits, round to nearest even (NaN kept quiet)
+__device__ inline short to_bf16(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (short)0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+struct GemmArgs {
+    const float *A, *B, *bias;
+    float *C, *Z;
+    int64_t M, N, K;
+    int64_t sam, sak, sbk, sbn, ldc;
+    int act, accumulate;
+};
+
+// One wavefront computes a 32 x 32 block of C as 2 x 2 tiles of
+// v_mfma_f32_16x16x16_bf16; four wavefronts make a 64 x 64 workgroup tile.
+// Operand layout of the instruction (lane l, element j = 0..3):
+//   A[l % 16][4 (l / 16) + j],  B[4 (l / 16) + j][l % 16],
+//   D: row 4 (l / 16) + j, column l % 16.
+__global__ void __launch_bounds__(256) bf16_gemm_kernel(GemmArgs g) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t m0 = (int64_t)blockIdx.y * 64 + (wave >> 1) * 32;
+    const int64_t n0 = (int64_t)blockIdx.x * 64 + (wave & 1) * 32;
+    const int lr = lane & 15, lk = (lane >> 4) * 4;
+    f32x4 acc[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int64_t k0 = 0; k0 < g.K; k0 += 16) {
+        bf16x4 a[2], b[2];
+        for (int mi = 0; mi < 2; mi++) {
+            const int64_t row = m0 + 16 * mi + lr;
+            for (int j = 0; j < 4; j++) {
+                const int64_t k = k0 + lk + j;
+                a[mi][j] = (row < g.M && k < g.K) ? to_bf16(g.A[row * g.sam + k * g.sak])
+                                                  : (short)0;
+            }
+        }
+        for (int ni = 0; ni < 2; ni++) {
+            const int64_t col = n0 + 16 * ni + lr;
+            for (int j = 0; j < 4; j++) {
+                const int64_t k = k0 + lk + j;
+                b[ni][j] = (col < g.N && k < g.K) ? to_bf16(g.B[k * g.sbk + col * g.sbn])
+                                                  : (short)0;
+            }
+        }
+        for (int mi = 0; mi < 2; mi++)
+            for (int ni = 0; ni < 2; ni++)
+                acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mi], b[ni],
+                                                                        acc[mi][ni], 0, 0, 0);
+    }
+    for (int mi = 0; mi < 2; mi++)
+        for (int ni = 0; ni < 2; ni++)
+            for (int j = 0; j < 4; j++) {
+                const int64_t row = m0 + 16 * mi + lk + j;
+                const int64_t col = n0 + 16 * ni + lr;
+                if (row >= g.M || col >= g.N) continue;
+                float z = acc[mi][ni][j];
+                if (g.bias) z += g.bias[col];
+                if (g.Z) g.Z[row * g.ldc + col] = z;
+                const float y = g.act == 1 ? tanhf(z) : z;
+                float *c = &g.C[row * g.ldc + col];
+                *c = g.accumulate ? *c + y : y;
+            }
+}
+
+__global__ void tanh_backward_kernel(int64_t n, const float *dA, const float *A, float *dZ) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dZ[i] = dA[i] * (1.f - A[i] * A[i]);
+}
+
+// one workgroup per column block: out[c] = sum_r X[r][c]
+__global__ void __launch_bounds__(256) col_sum_kernel(int64_t M, int64_t N, const float *X,
+                                                      float *out) {
+    __shared__ float s[256];
+    const int64_t c = (int64_t)blockIdx.x;
+    float acc = 0.f;
+    for (int64_t r = threadIdx.x; r < M; r += blockDim.x) acc += X[r * N + c];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = s[0];
+}
+
+__global__ void __launch_bounds__(256) mse_grad_kernel(int64_t n, const float *y, const float *t,
+                                                       float *g, double *loss) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    const float scale = 2.f / (float)n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float d = y[i] - t[i];
+        g[i] = scale * d;
+        acc += (double)d * (double)d;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, s[0] / (double)n);
+}
+
+__global__ void adam_kernel(int64_t n, float *w, const float *g, float *m, float *v, float lr,
+                            float b1, float b2, float eps, float c1, float c2) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+__global__ void zero_f64_kernel(double *p) { *p = 0.0; }
+
+static int blocks_of(int64_t n) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 8192));
+}
+
+}  // namespace ngc
+
+using namespace ngc;
+
+extern "C" {
+
+int ngc_bf16_gemm(ngc_context *ctx, void *stream, int64_t M, int64_t N, int64_t K,
+                  const float *A, int64_t sam, int64_t sak, const float *B, int64_t sbk,
+                  int64_t sbn, const float *bias, int act, int accumulate, float *C, int64_t ldc,
+                  float *Z) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    NGC_CHECK(M >= 0 && N >= 0 && K >= 0 && ldc >= N, NGC_ERR_ARG, "bad GEMM dimensions");
+    NGC_CHECK(act == 0 || act == 1, NGC_ERR_ARG, "act must be 0 (none) or 1 (tanh)");
+    if (M == 0 || N == 0) return NGC_OK;
+    NGC_CHECK(A && B && C, NGC_ERR_ARG, "NULL GEMM operand");
+    NGC_CHECK(ceil_div(M, 64) < 65536, NGC_ERR_UNSUPPORTED, "M too large");
+    NGC_HIP(hipSetDevice(ctx->device));
+    GemmArgs g{A, B, bias, C, Z, M, N, K, sam, sak, sbk, sbn, ldc, act, accumulate};
+    dim3 grid((unsigned)ceil_div(N, 64), (unsigned)ceil_div(M, 64));
+    bf16_gemm_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(g);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+int ngc_tanh_backward(ngc_context *ctx, void *stream, int64_t n, const float *dA, const float *A,
+                      float *dZ) {
+    NGC_CHECK(ctx && (n == 0 || (dA && A && dZ)), NGC_ERR_ARG, "bad arguments");
+    if (n == 0) return NGC_OK;
+    NGC_HIP(hipSetDevice(ctx->device));
+    tanh_backward_kernel<<<blocks_of(n), 256, 0, (hipStream_t)stream>>>(n, dA, A, dZ);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+int ngc_col_sum(ngc_context *ctx, void *stream, int64_t M, int64_t N, const float *X,
+                float *out) {
+    NGC_CHECK(ctx && X && out && M >= 0 && N >= 0, NGC_ERR_ARG, "bad arguments");
+    NGC_CHECK(N < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED, "too many columns");
+    if (N == 0) return NGC_OK;
+    NGC_HIP(hipSetDevice(ctx->device));
+    col_sum_kernel<<<(unsigned)N, 256, 0, (hipStream_t)stream>>>(M, N, X, out);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+int ngc_mse_grad(ngc_context *ctx, void *stream, int64_t n, const float *y, const float *t,
+                 float *g, double *loss) {
+    NGC_CHECK(ctx && y && t && g && loss && n > 0, NGC_ERR_ARG, "bad arguments");
+    NGC_HIP(hipSetDevice(ctx->device));
+    zero_f64_kernel<<<1, 1, 0, (hipStream_t)stream>>>(loss);
+    mse_grad_kernel<<<std::min(blocks_of(n), 1024), 256, 0, (hipStream_t)stream>>>(n, y, t, g,
+                                                                                   loss);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+int ngc_adam_step(ngc_context *ctx, void *stream, int64_t n, float *w, const float *g, float *m,
+                  float *v, double lr, double beta1, double beta2, double eps, int64_t step) {
+    NGC_CHECK(ctx && w && g && m && v && n >= 0 && step >= 1, NGC_ERR_ARG, "bad arguments");
+    if (n == 0) return NGC_OK;
+    NGC_HIP(hipSetDevice(ctx->device));
+    const float c1 = (float)(1.0 - std::pow(beta1, (double)step));
+    const float c2 = (float)(1.0 - std::pow(beta2, (double)step));
+    adam_kernel<<<blocks_of(n), 256, 0, (hipStream_t)stream>>>(
+        n, w, g, m, v, (float)lr, (float)beta1, (float)beta2, (float)eps, c1, c2);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+}  // extern "C"

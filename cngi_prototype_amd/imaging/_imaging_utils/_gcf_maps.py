@@ -69,3 +69,31 @@ def field_pixel_offsets(field_dir, phase_center, cell_size):
     m = np.sin(fd[:, 1]) * np.cos(dec0) - np.cos(fd[:, 1]) * np.sin(dec0) * np.cos(dra)
     deg = 180 / np.pi
     return np.stack([(l * deg) / (cell_size[0] * deg), (m * deg) / (cell_size[1] * deg)], 1)
+
+
+def gcf_kernels(pb_freq, dish, blockage, pairs, casa, pb_cell, npad, oversampling, max_support,
+                support_cut_level, supports=None):
+    """The a_term branch's kernels on the GPU (make_gridding_convolution_function.py:
+    181-280): weight kernels (power patterns) first, whose support cut decides
+    the supports unless given, then the convolution kernels (voltage-pattern
+    products), each cropped and normalised.  Returns device float64
+    (WEIGHT_CONV_KERNEL, CONV_KERNEL) [n_pairs, n_freq, 1, K, K] and SUPPORT
+    [n_pairs, n_freq, 1, 2]."""
+    from ... import _native
+    os_ = np.asarray(oversampling)
+    resize = (np.asarray(max_support) + 1) * os_
+    kernels = []
+    for ipower in (2, 1):
+        full, amax, amin, row, col = _native.gcf_patterns(pb_freq, dish, blockage, pairs, casa,
+                                                          ipower, pb_cell, npad)
+        if ipower == 2 and supports is None:
+            supports = np.zeros((len(pairs), len(pb_freq), 1, 2), dtype=int)
+            for k in range(len(pairs)):
+                for f in range(len(pb_freq)):
+                    supports[k, f, 0] = conv_support(row[k, f], col[k, f], amax[k, f],
+                                                     amin[k, f], npad, support_cut_level, os_,
+                                                     max_support)
+        resized = _native.gcf_resize(full, supports[:, :, 0], os_, resize)
+        del full
+        kernels.append(resized.unsqueeze(2))
+    return kernels[0], kernels[1], supports

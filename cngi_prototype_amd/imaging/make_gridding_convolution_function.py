@@ -9,8 +9,8 @@ from .._utils._check_parms import _check_sel_parms
 from .._utils._io import copy_structure
 from ..xds import DataArray, Dataset
 from ._imaging_utils._check_imaging_parms import _check_gcf_parms, _check_grid_parms
-from ._imaging_utils._gcf_maps import cf_baseline_map, cf_chan_map, conv_support, \
-    field_pixel_offsets
+from ._imaging_utils._gcf_maps import cf_baseline_map, cf_chan_map, field_pixel_offsets, \
+    gcf_kernels
 
 
 def make_gridding_convolution_function(mxds, gcf_parms, grid_parms, sel_parms):
@@ -53,24 +53,23 @@ def make_gridding_convolution_function(mxds, gcf_parms, grid_parms, sel_parms):
     cfp = np.zeros(len(_gcf_parms['pol']), dtype=int)
     pb_cell = _grid_parms['cell_size'] * os_
 
-    kernels = []
-    supports = None
-    for ipower in (2, 1):  # the weight kernel first: it decides the supports
-        full, amax, amin, row, col = _native.gcf_patterns(
-            pb_freq, _gcf_parms['list_dish_diameters'], _gcf_parms['list_blockage_diameters'],
-            pairs, casa, ipower, pb_cell, npad)
-        if ipower == 2:
-            supports = np.zeros((len(pairs), len(pb_freq), 1, 2), dtype=int)
-            for k in range(len(pairs)):
-                for f in range(len(pb_freq)):
-                    supports[k, f, 0] = conv_support(row[k, f], col[k, f], amax[k, f],
-                                                     amin[k, f], npad,
-                                                     _gcf_parms['support_cut_level'], os_,
-                                                     _gcf_parms['max_support'])
-        resized = _native.gcf_resize(full, supports[:, :, 0], os_, resize)
-        del full
-        kernels.append(resized.unsqueeze(2))  # conv_pol axis of length 1
-    weight_conv_kernel, conv_kernel = kernels
+    weight_conv_kernel, conv_kernel, supports = gcf_kernels(
+        pb_freq, _gcf_parms['list_dish_diameters'], _gcf_parms['list_blockage_diameters'], pairs,
+        casa, pb_cell, npad, os_, _gcf_parms['max_support'], _gcf_parms['support_cut_level'])
+    surrogate = _gcf_parms.get('surrogate')
+    if surrogate is not None:
+        # per-channel convolution kernels from the learned surrogate (the
+        # exact kernel of the channel's bin plus the predicted change to the
+        # channel's own frequency), each checked against the exact kernel when
+        # surrogate_check is set; the weight kernels stay the bin's exact ones
+        from ._imaging_utils._kernel_surrogate import per_channel_kernels
+        conv_kernel, weight_conv_kernel, report = per_channel_kernels(
+            surrogate, _gcf_parms['freq_chan'], cfc, pb_freq, _gcf_parms['list_dish_diameters'],
+            _gcf_parms['list_blockage_diameters'], pairs, casa, pb_cell, npad, os_,
+            _gcf_parms['max_support'], supports, conv_kernel, weight_conv_kernel,
+            check=_gcf_parms.get('surrogate_check', True))
+        supports = supports[:, cfc]
+        cfc = np.arange(len(_gcf_parms['freq_chan']))
 
     pix_dist = field_pixel_offsets(field_phase_dir, _gcf_parms['phase_center'],
                                    _grid_parms['cell_size'])
@@ -95,5 +94,7 @@ def make_gridding_convolution_function(mxds, gcf_parms, grid_parms, sel_parms):
                 'm': (('m',), np.arange(_grid_parms['image_size'][1]))})
     gcf.attrs['cell_uv'] = 1 / (npad * _grid_parms['cell_size'] * os_)
     gcf.attrs['oversampling'] = os_
+    if surrogate is not None:
+        gcf.attrs['surrogate_report'] = report
     print('#########################  Created graph for make_gridding_convolution_function #########################')
     return gcf

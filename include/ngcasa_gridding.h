@@ -309,6 +309,33 @@ int ngc_grid_plan_describe(ngc_grid_plan *plan, char *buf, int64_t len);
 int ngc_grid_plan_destroy(ngc_grid_plan *plan);
 
 /*
+ * Dense layers of the learned A-term kernel surrogate (no counterpart in the
+ * reference).  All arrays are device FP32; GEMM operands are rounded to bf16
+ * as they are loaded and multiplied on the matrix cores with FP32
+ * accumulation.
+ *   ngc_bf16_gemm      C[m][n] = act(sum_k A(m,k) B(k,n) + bias[n]) (+ C[m][n] when
+ *                      accumulate), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn],
+ *                      act 0 = identity, 1 = tanh; Z (or NULL) receives the
+ *                      pre-activation values (row stride ldc)
+ *   ngc_tanh_backward  dZ = dA * (1 - A^2) for A = tanh(Z)
+ *   ngc_col_sum        out[n] = sum_m X[m][n]   (X is M x N, row major)
+ *   ngc_mse_grad       g = 2 (y - t) / n, *loss = mean (y - t)^2 (device double)
+ *   ngc_adam_step      Adam (Kingma & Ba 2015) on FP32 master weights, step >= 1
+ */
+int ngc_bf16_gemm(ngc_context *ctx, void *stream, int64_t M, int64_t N, int64_t K,
+                  const float *A, int64_t sam, int64_t sak, const float *B, int64_t sbk,
+                  int64_t sbn, const float *bias, int act, int accumulate, float *C, int64_t ldc,
+                  float *Z);
+int ngc_tanh_backward(ngc_context *ctx, void *stream, int64_t n, const float *dA, const float *A,
+                      float *dZ);
+int ngc_col_sum(ngc_context *ctx, void *stream, int64_t M, int64_t N, const float *X,
+                float *out);
+int ngc_mse_grad(ngc_context *ctx, void *stream, int64_t n, const float *y, const float *t,
+                 float *g, double *loss);
+int ngc_adam_step(ngc_context *ctx, void *stream, int64_t n, float *w, const float *g, float *m,
+                  float *v, double lr, double beta1, double beta2, double eps, int64_t step);
+
+/*
  * Tile shape of the uv-tiled gridders: 16 or 32 grid rows per tile (the tile
  * is always 32 columns wide).  Set by the autotuner (no counterpart in the
  * reference, whose gridder has no tiles).  Default 16.
