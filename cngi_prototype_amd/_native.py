@@ -32,7 +32,8 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_gcf_patterns", "ngc_gcf_resize", "ngc_phase_gradient",
            "ngc_grid_plan_create", "ngc_grid_plan_execute", "ngc_grid_plan_status",
            "ngc_grid_plan_describe", "ngc_grid_plan_destroy", "ngc_bf16_gemm",
-           "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step")
+           "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step",
+           "ngc_direction_rotate")
 
 
 class NativeError(RuntimeError):
@@ -90,6 +91,8 @@ _PROTOS = {
     "ngc_col_sum": (_i32, [_vp, _vp, _i64, _i64, _vp, _vp]),
     "ngc_mse_grad": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "ngc_adam_step": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _i64]),
+    "ngc_direction_rotate": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _vp, _i64, _vp, _vp, _vp,
+                                    _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp]),
 }
 
 
@@ -593,3 +596,32 @@ def adam_step(w, g, m, v, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
     _check("ngc_adam_step", load_library().ngc_adam_step(
         context().handle, _stream(), w.numel(), _ptr(w), _ptr(g), _ptr(m), _ptr(v), float(lr),
         float(beta1), float(beta2), float(eps), int(step)))
+
+
+def direction_rotate(uvw, vis, field, field_ids, uvw_rotmat, phase_rotation, freq_chan,
+                     common_tangent=True, single_precision=True):
+    """ngc_direction_rotate on device tensors.  Returns (uvw_rot float64,
+    vis_rot complex64 if single_precision else complex128)."""
+    import torch
+    ctx = context()
+    for t, n in ((uvw, "uvw"), (vis, "vis"), (field, "field")):
+        _dev(t, n)
+    if field.dtype != torch.int32:
+        raise TypeError("field must be int32")
+    T, B, C, P = vis.shape
+    uvw_out = torch.empty_like(uvw)
+    vis_out = torch.empty(vis.shape, dtype=torch.complex64 if single_precision else
+                          torch.complex128, device=vis.device)
+    ids, idp = _host(field_ids, np.int32)
+    m, mp = _host(np.asarray(uvw_rotmat).reshape(-1, 9), np.float64)
+    ph, php = _host(np.asarray(phase_rotation).reshape(-1, 3), np.float64)
+    fr, frp = _host(freq_chan, np.float64)
+    rc = load_library().ngc_direction_rotate(
+        ctx.handle, _stream(), _ptr(uvw), _ptr(vis), _dt_vis(vis), _ptr(field), idp, len(ids),
+        mp, php, frp, T, B, C, P, int(bool(common_tangent)), int(bool(single_precision)),
+        _ptr(uvw_out), _ptr(vis_out))
+    if rc == -4:
+        raise AssertionError("direction_rotate only supports xds where field_id remains constant "
+                             "over baseline (and is listed in FIELD)")
+    _check("ngc_direction_rotate", rc)
+    return uvw_out, vis_out

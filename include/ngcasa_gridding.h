@@ -26,6 +26,8 @@
  *                              make_gridding_convolution_function.py:181-191,245-246,394-412
  *   ngc_gcf_resize             resize_and_calc_support  :361-391 (crop/normalise)
  *   ngc_phase_gradient         make_phase_gradient  :331-359 (the table)
+ *   ngc_direction_rotate       apply_rotation_matrix + apply_phasor
+ *                              ngcasa/imaging/direction_rotate.py:213-268
  *
  * Conventions
  *   - Large arrays (uvw, visibilities, weights, flags, grids, images) are
@@ -231,6 +233,29 @@ int ngc_normalize_mosaic(ngc_context *ctx, void *stream, double *image,
                          int64_t n_planes, int64_t n_l, int64_t n_m,
                          int64_t os_u, int64_t os_v, double pb_limit,
                          int single_precision);
+
+/*
+ * Phase-centre rotation: replaces apply_rotation_matrix and apply_phasor of
+ * direction_rotate (ngcasa/imaging/direction_rotate.py:213-268); the 3 x 3
+ * matrices and phase-rotation vectors of calc_rotation_mats (:148-196) come
+ * from the host.
+ *   field            device int32 [T,B]; one valid id (>= 0) per integration,
+ *                    rows with INT32_MIN ignored, else NGC_ERR_FIELD
+ *   field_ids        host int32 [n_ids]: the ids the tables are listed for
+ *   uvw_rotmat       host float64 [n_ids][3][3]: uvw_rot = uvw (row) @ M
+ *   phase_rotation   host float64 [n_ids][3]
+ *   uvw_out          device float64 [T,B,3]
+ *   vis_out          device [T,B,C,P]: complex64 when single_precision, else
+ *                    complex128; vis * exp(2 pi i (uvw_rot . p) f / c), with
+ *                    the w term left out under common-tangent reprojection
+ * Synchronises `stream` once (the field check).
+ */
+int ngc_direction_rotate(ngc_context *ctx, void *stream, const double *uvw, const void *vis,
+                         int vis_dtype, const int32_t *field, const int32_t *field_ids,
+                         int64_t n_ids, const double *uvw_rotmat, const double *phase_rotation,
+                         const double *freq_chan, int64_t n_time, int64_t n_baseline,
+                         int64_t n_chan, int64_t n_pol, int common_tangent, int single_precision,
+                         double *uvw_out, void *vis_out);
 
 /*
  * A-term convolution-function patterns: replaces make_baseline_patterns and
