@@ -343,7 +343,7 @@ def main():
     ap.add_argument("--cpu-chans", type=int, default=6)
     ap.add_argument("--config", default="C2")
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
-    ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 16, 32],
+    ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16, 32],
                     help="uv-tile rows of the gridder (0: library default, 16)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the steps, run the grid's ifftshift (a plain copy kernel of "

@@ -472,7 +472,7 @@ def profile_read():
 
 
 def set_tile_rows(rows):
-    """Tile rows (16 or 32) of the uv-tiled gridders on the current device."""
+    """Tile rows (8, 16 or 32) of the uv-tiled gridders on the current device."""
     ctx = context()
     _check("ngc_set_tile_rows", load_library().ngc_set_tile_rows(ctx.handle, int(rows)))
 

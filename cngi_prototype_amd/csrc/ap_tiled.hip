@@ -394,8 +394,9 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     //   s_wd    [kApBatch][NP*NW]
     extern __shared__ __align__(16) char smem_raw[];
     const ApLds L = ap_lds_layout(R, W, NP, NW, (int)sizeof(ACC), KAHAN);
-    ACC *s_acc = reinterpret_cast<ACC *>(smem_raw);
-    ACC *s_cmp = s_acc + NACC * PLANE;
+    // (sum, compensation) pairs side by side with KAHAN
+    using CELL = typename std::conditional<KAHAN, float2, ACC>::type;
+    CELL *s_acc = reinterpret_cast<CELL *>(smem_raw);
     int4 *s_fo = reinterpret_cast<int4 *>(smem_raw + L.fo);
     int4 *s_pos = reinterpret_cast<int4 *>(smem_raw + L.pos);
     unsigned int *s_base = reinterpret_cast<unsigned int *>(smem_raw + L.base);
@@ -415,8 +416,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     const int org_v = (int)((trem % a.ntv) * a.T) - a.hl;
 
     for (int i = tid; i < NACC * PLANE; i += blockDim.x) {
-        s_acc[i] = (ACC)0;
-        if (KAHAN) s_cmp[i] = (ACC)0;
+        s_acc[i] = CELL{};
     }
 
     const bool active = tid < R * R;
@@ -431,19 +431,19 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
         if (cur >= 0) {
 #pragma unroll
             for (int q = 0; q < NACC; q++) {
-                ACC *dst = &s_acc[q * PLANE + cur];
-                const ACC x = acc[q];
-                if (KAHAN) {
-                    const ACC s = *dst;
-                    const ACC t = s + x;
-                    ACC *cp = &s_cmp[q * PLANE + cur];
-                    if (fabs(s) >= fabs(x))
-                        *cp += (s - t) + x;
-                    else
-                        *cp += (x - t) + s;
-                    *dst = t;
+                CELL *dst = &s_acc[q * PLANE + cur];
+                if constexpr (KAHAN) {
+                    // Knuth's TwoSum: t + e == s + x exactly, e joins the compensation
+                    const float x = acc[q];
+                    float2 sc = *dst;
+                    const float t = sc.x + x;
+                    const float z = t - sc.x;
+                    const float e = (sc.x - (t - z)) + (x - z);
+                    sc.y += e;
+                    sc.x = t;
+                    *dst = sc;
                 } else {
-                    *dst = *dst + x;
+                    *dst = *dst + acc[q];
                 }
             }
         }
@@ -558,8 +558,11 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
         const int q = i / (W * W), r = i - q * W * W;
         const int lu = r / W, lv = r - lu * W;
         const int o = q * PLANE + lu * (W + 1) + lv;
-        SC v = (SC)s_acc[o];
-        if (KAHAN) v += (SC)s_cmp[o];
+        SC v;
+        if constexpr (KAHAN)
+            v = (SC)s_acc[o].x + (SC)s_acc[o].y;
+        else
+            v = (SC)s_acc[o];
         out[i] = v;
     }
 }

@@ -381,8 +381,10 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         ACC v[NPC];
     };
 
-    __shared__ ACC s_acc[NPC * PLANE];
-    __shared__ ACC s_cmp[KAHAN ? NPC * PLANE : 1];
+    // LDS subgrid: plain sums, or (sum, compensation) pairs side by side so
+    // that a compensated flush is one 8-byte read and one 8-byte write
+    using CELL = typename std::conditional<KAHAN, float2, ACC>::type;
+    __shared__ CELL s_acc[NPC * PLANE];
     __shared__ ACC s_wu[kBatch][8];    // u weight per residue, sign = cell changed
     __shared__ ACC s_wv[kBatch][8];    // v weight per residue, sign = cell changed
     __shared__ int s_cell[kBatch][8];  // cell_u(r) | cell_v(r) << 16 (tile-relative + 64)
@@ -408,10 +410,7 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     const int64_t end = (int64_t)min(offsets[bin + 1], cap);
 
 #pragma unroll 4
-    for (int i = lane; i < NPC * PLANE; i += 64) {
-        s_acc[i] = (ACC)0;
-        if (KAHAN) s_cmp[i] = (ACC)0;
-    }
+    for (int i = lane; i < NPC * PLANE; i += 64) s_acc[i] = CELL{};
     if (end > start)
         for (int i = lane; i < 512; i += 64) s_k[i] = i < cgk_len ? (ACC)cgk[i] : (ACC)0;
     __syncthreads();
@@ -431,19 +430,19 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
             const int o = cur_u * kRow + cur_v;
 #pragma unroll
             for (int q = 0; q < NPC; q++) {
-                ACC *dst = &s_acc[q * PLANE + o];
-                const ACC x = acc[q];
-                if (KAHAN) {
-                    const ACC s = *dst;
-                    const ACC t = s + x;
-                    ACC *cp = &s_cmp[q * PLANE + o];
-                    if (fabs(s) >= fabs(x))
-                        *cp += (s - t) + x;
-                    else
-                        *cp += (x - t) + s;
-                    *dst = t;
+                CELL *dst = &s_acc[q * PLANE + o];
+                if constexpr (KAHAN) {
+                    // Knuth's TwoSum: t + e == s + x exactly, e joins the compensation
+                    const float x = acc[q];
+                    float2 sc = *dst;
+                    const float t = sc.x + x;
+                    const float z = t - sc.x;
+                    const float e = (sc.x - (t - z)) + (x - z);
+                    sc.y += e;
+                    sc.x = t;
+                    *dst = sc;
                 } else {
-                    *dst = *dst + x;
+                    *dst = *dst + acc[q];
                 }
             }
         }
@@ -535,8 +534,11 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         const int64_t gu = u_lo + lu, gv = v_lo + lv;
         if (gu >= a.n_u || gv >= a.n_v) continue;
         const int o = (p * NC + c) * PLANE + lu * kRow + lv;
-        ACC val = s_acc[o];
-        if (KAHAN) val = val + s_cmp[o];
+        ACC val;
+        if constexpr (KAHAN)
+            val = s_acc[o].x + s_acc[o].y;
+        else
+            val = s_acc[o];
         const int64_t gi = (((a_chan * a.n_imag_pol + p) * a.n_u + gu) * a.n_v + gv) * NC + c;
         ACC *gp = reinterpret_cast<ACC *>(grid_out) + gi;
         if (accumulate)
@@ -735,7 +737,7 @@ static void norm_table(const double *cgk, int64_t cgk_len, int64_t support, int6
 // and a captured hipGraph.
 // ---------------------------------------------------------------------------
 struct TiledCfg {
-    int tu = 16;        // tile rows (16 or 32); tiles are 32 columns wide
+    int tu = 16;        // tile rows (8, 16 or 32); tiles are 32 columns wide
     int ch_lanes = 2;   // channels per binning wavefront (1, 2, 4 or 8)
 };
 
@@ -908,6 +910,7 @@ static int tiled_dispatch(int mode, int64_t n_pol, int precision, int tu, F &&f)
     struct F64 { using type = double; };
     auto by_tu = [&](auto m, auto np, auto acc, auto k) -> int {
         if (tu == 32) return f(m, np, acc, k, std::integral_constant<int, 32>{});
+        if (tu == 8) return f(m, np, acc, k, std::integral_constant<int, 8>{});
         return f(m, np, acc, k, std::integral_constant<int, 16>{});
     };
     auto by_prec = [&](auto m, auto np) -> int {
@@ -1064,7 +1067,7 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     // ---------------- tiled path ----------------
     NGC_TRY(tiled_supported(a, pol_identity, cgk_len));
     TiledCfg cfg;
-    cfg.tu = ctx->tile_rows == 32 ? 32 : 16;
+    cfg.tu = ctx->tile_rows;
     cfg.ch_lanes = ctx->bin_chan_lanes;
     const TiledGeom tg = tiled_geom(a, cfg);
     TiledBufs bf{};
@@ -1214,7 +1217,7 @@ static int plan_create(ngc_context *ctx, hipStream_t s, int mode, int precision,
             NGC_HIP(hipEventCreate(&e0));
             NGC_HIP(hipEventCreate(&e1));
             float best_ms = 1e30f;
-            for (int tu : {16, 32})
+            for (int tu : {8, 16, 32})
                 for (int cl : {1, 2, 4, 8}) {
                     TiledCfg cfg;
                     cfg.tu = tu;

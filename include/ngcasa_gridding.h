@@ -309,7 +309,7 @@ int ngc_phase_gradient(ngc_context *ctx, void *stream, const double *pix, int64_
  * its dask graph on every call).  A plan fixes every argument of one
  * ngc_standard_grid call in a tiled precision (IMAGE or PSF mode): the input,
  * grid and sum_weight pointers, the shapes and the tables.  Creation runs the
- * autotuner when `autotune` is nonzero (tile rows 16/32 x 1/2/4/8 channels per
+ * autotuner when `autotune` is nonzero (tile rows 8/16/32 x 1/2/4/8 channels per
  * binning wavefront, each timed on these inputs; the choice is cached per
  * shape in the context), sizes the bins with one counting pass, and captures
  * the whole pipeline (count, scan, fill, largest-first bin order, persistent
@@ -361,8 +361,8 @@ int ngc_adam_step(ngc_context *ctx, void *stream, int64_t n, float *w, const flo
                   float *v, double lr, double beta1, double beta2, double eps, int64_t step);
 
 /*
- * Tile shape of the uv-tiled gridders: 16 or 32 grid rows per tile (the tile
- * is always 32 columns wide).  Set by the autotuner (no counterpart in the
+ * Tile shape of the uv-tiled gridders: 8, 16 or 32 grid rows per tile (the
+ * tile is always 32 columns wide).  Set by the autotuner (no counterpart in the
  * reference, whose gridder has no tiles).  Default 16.
  */
 int ngc_set_tile_rows(ngc_context *ctx, int tile_rows);

@@ -26,7 +26,7 @@ def _dev(a, dtype=None):
     return t.to(dtype) if dtype is not None else t
 
 
-@pytest.mark.parametrize("rows", [16, 32])
+@pytest.mark.parametrize("rows", [8, 16, 32])
 @pytest.mark.parametrize("precision", ["fp64", "fp64_tiled", "fp32_kahan", "fp32"])
 @pytest.mark.parametrize("mode", ["psf", "image"])
 def test_standard_grid_matches_oracle(gpu, precision, mode, rows):
