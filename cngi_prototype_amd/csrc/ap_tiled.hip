@@ -25,8 +25,11 @@
 //     cell's running sum in registers and flushes it into the LDS region
 //     (compensated with FP32_KAHAN) when its cell changes.  The region is
 //     written to a scratch slot of the bin.
-//  5. ap_merge_kernel: every grid cell sums, in a fixed order and FP64, the
-//     (at most four) regions that cover it, and is written exactly once.
+//  5. ap_merge_kernel: only non-empty bins have scratch slots; every cell of
+//     a tile next to a non-empty bin adds, in a fixed order and FP64, the (at
+//     most four) regions that cover it (the grid is zeroed first unless
+//     accumulating), so a chunk of a large cube touches only its part of the
+//     grid.
 //
 // Weight mode: every sample lands on the same centre taps of its (image
 // channel, pol, convolution function, field) key, so the weight grid is
@@ -372,7 +375,8 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                 const unsigned int *__restrict__ order, const char *__restrict__ entries,
                 const ACC *__restrict__ ckr, const Cplx<ACC> *__restrict__ pgu,
                 const Cplx<ACC> *__restrict__ pgv,
-                typename std::conditional<KAHAN, double, ACC>::type *__restrict__ scratch) {
+                typename std::conditional<KAHAN, double, ACC>::type *__restrict__ scratch,
+                const int *__restrict__ slot) {
     constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
     constexpr int NACC = NP * 2;  // the grid is complex in every mode
     using E = ApEntry<NP, NW, ACC>;
@@ -553,7 +557,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     flush();
     __syncthreads();
     // the region goes to this bin's scratch slot: [NACC][W][W]
-    SC *out = scratch + bin * (int64_t)NACC * W * W;
+    SC *out = scratch + (int64_t)slot[bin] * NACC * W * W;
     for (int i = tid; i < NACC * W * W; i += blockDim.x) {
         const int q = i / (W * W), r = i - q * W * W;
         const int lu = r / W, lv = r - lu * W;
@@ -567,46 +571,82 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     }
 }
 
-// Every grid cell sums the regions that cover it (home tile first, then the
-// neighbours in fixed order) and is written once.
+// Tiles whose 3 x 3 neighbourhood holds a non-empty bin get the regions that
+// cover their cells added (home tile first, then the neighbours in fixed
+// order, summed in FP64); every other cell keeps its value (zero unless
+// accumulating).  flags[bin]: 1 when the bin holds records; slot[bin]: its
+// scratch slot; tiles[]: the listed tiles, *n_tiles of them.
+__global__ void ap_flags_kernel(ApT a, const unsigned long long *offsets, int64_t n_bins,
+                                int *nonempty, int *near) {
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= n_bins;
+         b += (int64_t)gridDim.x * blockDim.x) {
+        if (b == n_bins) {  // the scans' sentinels
+            nonempty[b] = 0;
+            near[b] = 0;
+            continue;
+        }
+        nonempty[b] = offsets[b + 1] > offsets[b] ? 1 : 0;
+        const int64_t tiles = a.ntu * a.ntv;
+        const int64_t ch = b / tiles, r = b - ch * tiles;
+        const int64_t tu = r / a.ntv, tv = r - tu * a.ntv;
+        int any = 0;
+        for (int du = -1; du <= 1; du++)
+            for (int dv = -1; dv <= 1; dv++) {
+                const int64_t u = tu + du, v = tv + dv;
+                if (u < 0 || u >= a.ntu || v < 0 || v >= a.ntv) continue;
+                const int64_t nb = (ch * a.ntu + u) * a.ntv + v;
+                any |= offsets[nb + 1] > offsets[nb];
+            }
+        near[b] = any;
+    }
+}
+
+__global__ void ap_list_kernel(int64_t n_bins, const int *near, const int *near_pos, int *tiles) {
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n_bins;
+         b += (int64_t)gridDim.x * blockDim.x)
+        if (near[b]) tiles[near_pos[b]] = (int)b;
+}
+
 template <typename SC, typename OUT>
-__global__ void ap_merge_kernel(ApT a, const unsigned long long *__restrict__ offsets,
-                                const SC *__restrict__ scratch, OUT *grid, int accumulate) {
-    const int64_t n = a.n_imag_chan * a.n_imag_pol * a.n_u * a.n_v;
+__global__ void ap_merge_kernel(ApT a, const int *__restrict__ nonempty,
+                                const int *__restrict__ slot, const int *__restrict__ tiles,
+                                const int *__restrict__ n_tiles_p, const SC *__restrict__ scratch,
+                                OUT *grid) {
     const int W = a.W, T = a.T;
     const int NACC = (int)a.n_imag_pol * 2;
+    const int64_t per_tile = (int64_t)a.n_imag_pol * T * T;
+    const int64_t n = (int64_t)(*n_tiles_p) * per_tile;
+    const int64_t tiles_per_chan = a.ntu * a.ntv;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t v = i % a.n_v;
-        const int64_t u = (i / a.n_v) % a.n_u;
-        const int64_t pol = (i / (a.n_u * a.n_v)) % a.n_imag_pol;
-        const int64_t ch = i / (a.n_u * a.n_v * a.n_imag_pol);
+        const int64_t k = i / per_tile, r = i - k * per_tile;
+        const int64_t home = tiles[k];
+        const int64_t ch = home / tiles_per_chan, hr = home - ch * tiles_per_chan;
+        const int64_t htu = hr / a.ntv, htv = hr - htu * a.ntv;
+        const int64_t pol = r / (T * T), c = r - pol * T * T;
+        const int64_t u = htu * T + c / T, v = htv * T + c % T;
+        if (u >= a.n_u || v >= a.n_v) continue;
         double re = 0.0, im = 0.0;
-        const int64_t tu0 = u / T, tv0 = v / T;
         for (int du = 0; du < 3; du++) {
-            const int64_t tu = du == 0 ? tu0 : (du == 1 ? tu0 - 1 : tu0 + 1);
+            const int64_t tu = du == 0 ? htu : (du == 1 ? htu - 1 : htu + 1);
             if (tu < 0 || tu >= a.ntu) continue;
             const int64_t lu = u - (tu * T - a.hl);
             if (lu < 0 || lu >= W) continue;
             for (int dv = 0; dv < 3; dv++) {
-                const int64_t tv = dv == 0 ? tv0 : (dv == 1 ? tv0 - 1 : tv0 + 1);
+                const int64_t tv = dv == 0 ? htv : (dv == 1 ? htv - 1 : htv + 1);
                 if (tv < 0 || tv >= a.ntv) continue;
                 const int64_t lv = v - (tv * T - a.hl);
                 if (lv < 0 || lv >= W) continue;
                 const int64_t bin = (ch * a.ntu + tu) * a.ntv + tv;
-                if (offsets[bin + 1] <= offsets[bin]) continue;
-                const SC *s = scratch + bin * (int64_t)NACC * W * W + lu * W + lv;
-                re += (double)s[(2 * pol) * (int64_t)W * W];
-                im += (double)s[(2 * pol + 1) * (int64_t)W * W];
+                if (!nonempty[bin]) continue;
+                const SC *sp = scratch + (int64_t)slot[bin] * NACC * W * W + lu * W + lv;
+                re += (double)sp[(2 * pol) * (int64_t)W * W];
+                im += (double)sp[(2 * pol + 1) * (int64_t)W * W];
             }
         }
-        if (accumulate) {
-            grid[2 * i] = (OUT)((double)grid[2 * i] + re);
-            grid[2 * i + 1] = (OUT)((double)grid[2 * i + 1] + im);
-        } else {
-            grid[2 * i] = (OUT)re;
-            grid[2 * i + 1] = (OUT)im;
-        }
+        const int64_t gi = ((ch * a.n_imag_pol + pol) * a.n_u + u) * a.n_v + v;
+        grid[2 * gi] = (OUT)((double)grid[2 * gi] + re);
+        grid[2 * gi + 1] = (OUT)((double)grid[2 * gi + 1] + im);
     }
 }
 
@@ -964,13 +1004,37 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
         NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp, tmp_bytes, counts,
                                                  (unsigned long long *)d_off, n_bins + 1, s));
         phase_end(ctx, s);
+        // non-empty bins get scratch slots; tiles next to one are merged
+        void *d_ne, *d_near, *d_slot, *d_npos, *d_tiles, *d_itmp;
+        NGC_TRY(workspace(ctx, "ap_nonempty", (n_bins + 1) * sizeof(int), &d_ne));
+        NGC_TRY(workspace(ctx, "ap_near", (n_bins + 1) * sizeof(int), &d_near));
+        NGC_TRY(workspace(ctx, "ap_slot", (n_bins + 1) * sizeof(int), &d_slot));
+        NGC_TRY(workspace(ctx, "ap_near_pos", (n_bins + 1) * sizeof(int), &d_npos));
+        NGC_TRY(workspace(ctx, "ap_tiles", (n_bins + 1) * sizeof(int), &d_tiles));
+        ap_flags_kernel<<<grid_blocks(n_bins + 1), 256, 0, s>>>(
+            a, (const unsigned long long *)d_off, n_bins, (int *)d_ne, (int *)d_near);
+        NGC_HIP(hipGetLastError());
+        size_t itmp = 0;
+        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, itmp, (const int *)d_ne, (int *)d_slot,
+                                                 n_bins + 1, s));
+        NGC_TRY(workspace(ctx, "ap_int_scan_tmp", itmp, &d_itmp));
+        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_itmp, itmp, (const int *)d_ne, (int *)d_slot,
+                                                 n_bins + 1, s));
+        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_itmp, itmp, (const int *)d_near,
+                                                 (int *)d_npos, n_bins + 1, s));
+        ap_list_kernel<<<grid_blocks(n_bins), 256, 0, s>>>(n_bins, (const int *)d_near,
+                                                           (const int *)d_npos, (int *)d_tiles);
+        NGC_HIP(hipGetLastError());
         unsigned long long total = 0;
+        int n_nonempty = 0;
         NGC_HIP(hipMemcpyAsync(&total, (unsigned long long *)d_off + n_bins, sizeof(total),
+                               hipMemcpyDeviceToHost, s));
+        NGC_HIP(hipMemcpyAsync(&n_nonempty, (int *)d_slot + n_bins, sizeof(int),
                                hipMemcpyDeviceToHost, s));
         NGC_HIP(hipStreamSynchronize(s));
         NGC_TRY(workspace(ctx, "ap_bin_entries", (size_t)(total + 1) * EB, &d_ent));
         NGC_TRY(workspace(ctx, "ap_scratch",
-                          (size_t)n_bins * 2 * NP * a.W * a.W * sizeof(SC), &d_scr));
+                          (size_t)(n_nonempty + 1) * 2 * NP * a.W * a.W * sizeof(SC), &d_scr));
         NGC_HIP(hipMemcpyAsync(d_counts, d_off, (n_bins + 1) * sizeof(unsigned long long),
                                hipMemcpyDeviceToDevice, s));
         phase_begin(ctx, s, "ap_bin_fill");
@@ -993,19 +1057,28 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
             kern<<<(unsigned)n_bins, threads, lds, s>>>(
                 a, (const unsigned long long *)d_off, order, (const char *)d_ent,
                 (const ACC *)d_ckr, (const Cplx<ACC> *)d_pgu, (const Cplx<ACC> *)d_pgv,
-                (SC *)d_scr);
+                (SC *)d_scr, (const int *)d_slot);
             NGC_HIP(hipGetLastError());
         }
         phase_end(ctx, s);
         phase_begin(ctx, s, "ap_merge");
-        if (single)
-            ap_merge_kernel<SC, float><<<grid_blocks(cells), 256, 0, s>>>(
-                a, (const unsigned long long *)d_off, (const SC *)d_scr, (float *)c.grid,
-                c.accumulate);
-        else
-            ap_merge_kernel<SC, double><<<grid_blocks(cells), 256, 0, s>>>(
-                a, (const unsigned long long *)d_off, (const SC *)d_scr, (double *)c.grid,
-                c.accumulate);
+        const int64_t merge_items = n_bins * c.n_imag_pol * T * T;
+        const int *ntp = (const int *)d_npos + n_bins;  // number of listed tiles
+        if (single) {
+            if (!c.accumulate)
+                ap_zero_grid_kernel<float><<<grid_blocks(2 * cells), 256, 0, s>>>(
+                    (float *)c.grid, 2 * cells);
+            ap_merge_kernel<SC, float><<<grid_blocks(merge_items), 256, 0, s>>>(
+                a, (const int *)d_ne, (const int *)d_slot, (const int *)d_tiles, ntp,
+                (const SC *)d_scr, (float *)c.grid);
+        } else {
+            if (!c.accumulate)
+                ap_zero_grid_kernel<double><<<grid_blocks(2 * cells), 256, 0, s>>>(
+                    (double *)c.grid, 2 * cells);
+            ap_merge_kernel<SC, double><<<grid_blocks(merge_items), 256, 0, s>>>(
+                a, (const int *)d_ne, (const int *)d_slot, (const int *)d_tiles, ntp,
+                (const SC *)d_scr, (double *)c.grid);
+        }
         NGC_HIP(hipGetLastError());
         phase_end(ctx, s);
         return NGC_OK;
