@@ -79,6 +79,11 @@ CONFIGS = {
                  32, 2, 1024, 0.09, 60, fov_deg=0.012, lognormal_mu=np.log(150.0),
                  lognormal_sigma=0.7, seed=4,
                  extra=dict(n_field=25, dish=(7.0, 12.0), n_dish=(12, 40))),
+    # large streamed mosaic cube: 64 x 12 m antennas, 8 h of 7.43 s dumps at 100 GHz,
+    # 64 chan x 2 pol (1.0e9 visibilities), 49 pointings, 4096^2 at 0.05"
+    "C5": Config("C5", 64, "lognormal", 3000.0, -23.0234, -23.0, 3875, 7.432, 100e9, 7.8125e6,
+                 64, 2, 4096, 0.05, 80, fov_deg=0.03, seed=5,
+                 extra=dict(n_field=49, dish=(12.0,), n_dish=(64,))),
 }
 
 
