@@ -85,6 +85,10 @@ def mosaic_leg(args):
     pol, 1440 x 10 s dumps, 1024^2 at 0.09", GCF oversampling 20, supports 11/15/17).
     Times the chosen form against the plain FP64 drop-in on the same inputs and
     states the dirty-image error per plane."""
+    # the imaging API reports its parameter defaults on stdout, as the reference's
+    # does; keep stdout for the one JSON record
+    out, sys.stdout = sys.stdout, sys.stderr
+
     import torch
     from cngi_prototype_amd import _native as nat
     from cngi_prototype_amd import synthetic as syn
@@ -496,13 +500,13 @@ def main():
 
     torch.cuda.set_device(0)
     if args.mosaic_only:
-        print(json.dumps({"mosaic": mosaic_leg(args)}), flush=True)
+        print(json.dumps({"mosaic": mosaic_leg(args)}), file=out, flush=True)
         return
     if args.batched_only:
-        print(json.dumps({"batched": batched_leg(args)}), flush=True)
+        print(json.dumps({"batched": batched_leg(args)}), file=out, flush=True)
         return
     if args.streamed_only:
-        print(json.dumps({"streamed": streamed_leg(args)}), flush=True)
+        print(json.dumps({"streamed": streamed_leg(args)}), file=out, flush=True)
         return
     if args.tile_rows:
         nat.set_tile_rows(args.tile_rows)
@@ -686,7 +690,7 @@ def main():
         "batched": batched,
         "streamed": streamed,
     }
-    print(json.dumps(line), flush=True)
+    print(json.dumps(line), file=out, flush=True)
 
 
 if __name__ == "__main__":
