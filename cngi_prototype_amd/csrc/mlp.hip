@@ -38,57 +38,118 @@ struct GemmArgs {
     int64_t M, N, K;
     int64_t sam, sak, sbk, sbn, ldc;
     int act, accumulate;
+    int64_t k_chunk;   // K range of one split (a multiple of kGemmBK)
+    float *partial;    // [splits][M][N] FP32 partial products when split
 };
 
-// One wavefront computes a 32 x 32 block of C as 2 x 2 tiles of
-// v_mfma_f32_16x16x16_bf16; four wavefronts make a 64 x 64 workgroup tile.
+constexpr int kGemmBM = 64, kGemmBN = 64, kGemmBK = 32;
+constexpr int kGemmLd = kGemmBK + 8;   // LDS row pitch in bf16 (80 B, 8 B aligned)
+
+__device__ inline void gemm_store(const GemmArgs &g, int64_t row, int64_t col, float z) {
+    if (g.bias) z += g.bias[col];
+    if (g.Z) g.Z[row * g.ldc + col] = z;
+    const float y = g.act == 1 ? tanhf(z) : z;
+    float *c = &g.C[row * g.ldc + col];
+    *c = g.accumulate ? *c + y : y;
+}
+
+// One workgroup computes a 64 x 64 block of C over the K range of its split
+// (blockIdx.z).  Per 32-wide K step the four wavefronts load the A (64 x 32)
+// and B (32 x 64) tiles with the contiguous dimension of each operand across
+// consecutive lanes (so X^T and W^T operands are read coalesced too), round to
+// bf16 and store both k-contiguous in LDS; each wavefront then runs a 32 x 32
+// block as 2 x 2 tiles of v_mfma_f32_16x16x16_bf16 over two K sub-steps.
 // Operand layout of the instruction (lane l, element j = 0..3):
 //   A[l % 16][4 (l / 16) + j],  B[4 (l / 16) + j][l % 16],
 //   D: row 4 (l / 16) + j, column l % 16.
+// Unsplit launches apply bias, activation and accumulate directly; split
+// launches write FP32 partial blocks that gemm_reduce_kernel sums in split
+// order.
 __global__ void __launch_bounds__(256) bf16_gemm_kernel(GemmArgs g) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t m0 = (int64_t)blockIdx.y * 64 + (wave >> 1) * 32;
-    const int64_t n0 = (int64_t)blockIdx.x * 64 + (wave & 1) * 32;
+    __shared__ __align__(16) short As[kGemmBM * kGemmLd];
+    __shared__ __align__(16) short Bs[kGemmBN * kGemmLd];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t mt = (int64_t)blockIdx.y * kGemmBM, nt = (int64_t)blockIdx.x * kGemmBN;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int64_t kb = (int64_t)blockIdx.z * g.k_chunk;
+    const int64_t ke = g.K < kb + g.k_chunk ? g.K : kb + g.k_chunk;
+    const bool a_mc = g.sam == 1 && g.sak != 1;   // A contiguous along m
+    const bool b_kc = g.sbk == 1 && g.sbn != 1;   // B contiguous along k
     const int lr = lane & 15, lk = (lane >> 4) * 4;
     f32x4 acc[2][2];
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-    for (int64_t k0 = 0; k0 < g.K; k0 += 16) {
-        bf16x4 a[2], b[2];
-        for (int mi = 0; mi < 2; mi++) {
-            const int64_t row = m0 + 16 * mi + lr;
-            for (int j = 0; j < 4; j++) {
-                const int64_t k = k0 + lk + j;
-                a[mi][j] = (row < g.M && k < g.K) ? to_bf16(g.A[row * g.sam + k * g.sak])
-                                                  : (short)0;
+    for (int64_t k0 = kb; k0 < ke; k0 += kGemmBK) {
+        if (a_mc) {
+            const int kk = tid >> 3, mm = (tid & 7) * 8;
+            const int64_t k = k0 + kk;
+            for (int j = 0; j < 8; j++) {
+                const int64_t m = mt + mm + j;
+                As[(mm + j) * kGemmLd + kk] =
+                    (m < g.M && k < ke) ? to_bf16(g.A[m * g.sam + k * g.sak]) : (short)0;
+            }
+        } else {
+            const int mm = tid >> 2, kk = (tid & 3) * 8;
+            const int64_t m = mt + mm;
+            for (int j = 0; j < 8; j++) {
+                const int64_t k = k0 + kk + j;
+                As[mm * kGemmLd + kk + j] =
+                    (m < g.M && k < ke) ? to_bf16(g.A[m * g.sam + k * g.sak]) : (short)0;
             }
         }
-        for (int ni = 0; ni < 2; ni++) {
-            const int64_t col = n0 + 16 * ni + lr;
-            for (int j = 0; j < 4; j++) {
-                const int64_t k = k0 + lk + j;
-                b[ni][j] = (col < g.N && k < g.K) ? to_bf16(g.B[k * g.sbk + col * g.sbn])
-                                                  : (short)0;
+        if (b_kc) {
+            const int nn = tid >> 2, kk = (tid & 3) * 8;
+            const int64_t n = nt + nn;
+            for (int j = 0; j < 8; j++) {
+                const int64_t k = k0 + kk + j;
+                Bs[nn * kGemmLd + kk + j] =
+                    (n < g.N && k < ke) ? to_bf16(g.B[k * g.sbk + n * g.sbn]) : (short)0;
+            }
+        } else {
+            const int kk = tid >> 3, nn = (tid & 7) * 8;
+            const int64_t k = k0 + kk;
+            for (int j = 0; j < 8; j++) {
+                const int64_t n = nt + nn + j;
+                Bs[(nn + j) * kGemmLd + kk] =
+                    (n < g.N && k < ke) ? to_bf16(g.B[k * g.sbk + n * g.sbn]) : (short)0;
             }
         }
-        for (int mi = 0; mi < 2; mi++)
+        __syncthreads();
+        for (int ks = 0; ks < kGemmBK; ks += 16) {
+            bf16x4 a[2], b[2];
+            for (int mi = 0; mi < 2; mi++)
+                a[mi] = *(const bf16x4 *)&As[(wm + 16 * mi + lr) * kGemmLd + ks + lk];
             for (int ni = 0; ni < 2; ni++)
-                acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[mi], b[ni],
-                                                                        acc[mi][ni], 0, 0, 0);
+                b[ni] = *(const bf16x4 *)&Bs[(wn + 16 * ni + lr) * kGemmLd + ks + lk];
+            for (int mi = 0; mi < 2; mi++)
+                for (int ni = 0; ni < 2; ni++)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                        a[mi], b[ni], acc[mi][ni], 0, 0, 0);
+        }
+        __syncthreads();
     }
     for (int mi = 0; mi < 2; mi++)
         for (int ni = 0; ni < 2; ni++)
             for (int j = 0; j < 4; j++) {
-                const int64_t row = m0 + 16 * mi + lk + j;
-                const int64_t col = n0 + 16 * ni + lr;
+                const int64_t row = mt + wm + 16 * mi + lk + j;
+                const int64_t col = nt + wn + 16 * ni + lr;
                 if (row >= g.M || col >= g.N) continue;
-                float z = acc[mi][ni][j];
-                if (g.bias) z += g.bias[col];
-                if (g.Z) g.Z[row * g.ldc + col] = z;
-                const float y = g.act == 1 ? tanhf(z) : z;
-                float *c = &g.C[row * g.ldc + col];
-                *c = g.accumulate ? *c + y : y;
+                if (gridDim.z > 1)
+                    g.partial[((int64_t)blockIdx.z * g.M + row) * g.N + col] = acc[mi][ni][j];
+                else
+                    gemm_store(g, row, col, acc[mi][ni][j]);
             }
+}
+
+// Sum of the split-K partial blocks in split order, then the epilogue.
+__global__ void __launch_bounds__(256) gemm_reduce_kernel(GemmArgs g, int splits) {
+    const int64_t mn = g.M * g.N;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < mn;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float z = 0.f;
+        for (int s = 0; s < splits; s++) z += g.partial[s * mn + i];
+        gemm_store(g, i / g.N, i % g.N, z);
+    }
 }
 
 __global__ void tanh_backward_kernel(int64_t n, const float *dA, const float *A, float *dZ) {
@@ -167,11 +228,37 @@ int ngc_bf16_gemm(ngc_context *ctx, void *stream, int64_t M, int64_t N, int64_t 
     NGC_CHECK(act == 0 || act == 1, NGC_ERR_ARG, "act must be 0 (none) or 1 (tanh)");
     if (M == 0 || N == 0) return NGC_OK;
     NGC_CHECK(A && B && C, NGC_ERR_ARG, "NULL GEMM operand");
-    NGC_CHECK(ceil_div(M, 64) < 65536, NGC_ERR_UNSUPPORTED, "M too large");
+    NGC_CHECK(ceil_div(M, kGemmBM) < 65536, NGC_ERR_UNSUPPORTED, "M too large");
     NGC_HIP(hipSetDevice(ctx->device));
-    GemmArgs g{A, B, bias, C, Z, M, N, K, sam, sak, sbk, sbn, ldc, act, accumulate};
-    dim3 grid((unsigned)ceil_div(N, 64), (unsigned)ceil_div(M, 64));
-    bf16_gemm_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(g);
+    NGC_CHECK(ceil_div(N, kGemmBN) < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED, "N too large");
+    // split K when the output blocks alone cannot fill the device, keeping at
+    // least 4 K steps per split
+    const int64_t tiles = ceil_div(M, kGemmBM) * ceil_div(N, kGemmBN);
+    const int64_t ksteps = std::max<int64_t>(1, ceil_div(K, kGemmBK));
+    int64_t splits = 1;
+    if (tiles < ctx->n_cu && ksteps >= 8)
+        splits = std::min<int64_t>(ksteps / 4, ceil_div(2 * (int64_t)ctx->n_cu, tiles));
+    const int64_t k_chunk = ceil_div(ksteps, splits) * kGemmBK;
+    splits = std::max<int64_t>(1, ceil_div(K, k_chunk));
+    GemmArgs g{A, B, bias, C, Z, M, N, K, sam, sak, sbk, sbn, ldc, act, accumulate,
+               K > 0 ? k_chunk : kGemmBK, nullptr};
+    hipStream_t s = (hipStream_t)stream;
+    if (splits > 1) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        NGC_TRY(enter_stream(ctx, s));
+        void *part = nullptr;
+        NGC_TRY(workspace(ctx, "gemm_partial", (size_t)(splits * M * N) * sizeof(float), &part));
+        g.partial = (float *)part;
+        dim3 grid((unsigned)ceil_div(N, kGemmBN), (unsigned)ceil_div(M, kGemmBM),
+                  (unsigned)splits);
+        bf16_gemm_kernel<<<grid, 256, 0, s>>>(g);
+        NGC_HIP(hipGetLastError());
+        gemm_reduce_kernel<<<blocks_of(M * N), 256, 0, s>>>(g, (int)splits);
+        NGC_HIP(hipGetLastError());
+        return NGC_OK;
+    }
+    dim3 grid((unsigned)ceil_div(N, kGemmBN), (unsigned)ceil_div(M, kGemmBM), 1);
+    bf16_gemm_kernel<<<grid, 256, 0, s>>>(g);
     NGC_HIP(hipGetLastError());
     return NGC_OK;
 }

@@ -14,12 +14,18 @@ twice the parallactic angle (symmetric Airy patterns do not depend on it; the
 training set draws it at random so the network learns that), and the
 sub-cell offset pair (ou/os, ov/os).  Output: the R x R tap block (R = the
 largest support) of that offset, as the difference from the bin's exact block
-in units of the bin kernel's peak.
+in units of the bin kernel's peak, divided by the frequency offset feature
+100 (f/f_bin - 1).  The kernel of a channel is the bin's kernel stretched by
+f/f_bin, so the change is to first order proportional to that offset: the
+network learns the smooth rate of change (and its slow dependence on the
+offset), and the product with the offset is exact where the change vanishes.
 
 The exact kernel is the fallback and the judge: training and validation
 targets come from the exact GPU kernels, and with check=True every
-per-channel kernel is compared with the exact one and replaced by it when its
-largest tap error exceeds 1e-4 of the peak.
+per-channel kernel is compared with the exact one over the taps the gridder
+reads (the support window at every sub-cell offset) and replaced by it when
+its largest tap error there exceeds 1e-4 of the peak.  Outside the window,
+which no gridder reads, a surrogate kernel keeps the bin kernel's samples.
 """
 import math
 import time
@@ -54,6 +60,8 @@ class KernelSurrogate:
         self.x_mean = torch.zeros(n_in, dtype=torch.float32, device="cuda")
         self.x_std = torch.ones(n_in, dtype=torch.float32, device="cuda")
         self.t_scale = 1.0
+        # predictions are multiplied by this (unscaled) input feature, if set
+        self.scale_feature = None
 
     def params(self):
         out = []
@@ -114,7 +122,11 @@ class KernelSurrogate:
         import torch
         out = []
         for i in range(0, x.shape[0], batch):
-            out.append(self.forward(self._xs(x[i:i + batch]))[-1] * self.t_scale)
+            xb = x[i:i + batch]
+            y = self.forward(self._xs(xb))[-1] * self.t_scale
+            if self.scale_feature is not None:
+                y = y * xb[:, self.scale_feature:self.scale_feature + 1]
+            out.append(y)
         return torch.cat(out)
 
 
@@ -145,11 +157,12 @@ def tap_blocks(ck, support, oversampling, R):
     return blk * (mu[:, None, :, None] & mv[None, :, None, :])
 
 
-def blocks_to_kernel(blocks, support, oversampling, K):
+def blocks_to_kernel(blocks, support, oversampling, K, base=None):
     """Inverse of tap_blocks: scatter blocks [os_u+1, os_v+1, R, R] back into a
-    K x K kernel.  Every kernel sample inside the support window is written
-    once: with an even oversampling the offset +os/2 of tap r is the sample of
-    offset -os/2 of tap r+1, so +os/2 is kept only for the last tap."""
+    K x K kernel (zeros, or a copy of `base`, outside the support window, which
+    no gridder reads).  Every kernel sample inside the support window is
+    written once: with an even oversampling the offset +os/2 of tap r is the
+    sample of offset -os/2 of tap r+1, so +os/2 is kept only for the last tap."""
     import torch
     R = blocks.shape[-1]
     dev = blocks.device
@@ -165,7 +178,8 @@ def blocks_to_kernel(blocks, support, oversampling, K):
                 torch.as_tensor(idx[o, r], device=dev))
     ou, ru, iu = kept(support[0], oversampling[0])
     ov, rv, iv = kept(support[1], oversampling[1])
-    out = torch.zeros((K, K), dtype=blocks.dtype, device=dev)
+    out = (torch.zeros((K, K), dtype=blocks.dtype, device=dev) if base is None
+           else base.clone())
     out[iu[:, None], iv[None, :]] = blocks[ou[:, None], ov[None, :], ru[:, None], rv[None, :]]
     return out
 
@@ -243,27 +257,35 @@ def training_set(dish, blockage, pairs, casa, pb_cell, npad, oversampling, max_s
 
 def train_surrogate(dish, blockage, pairs, casa, pb_cell, npad, oversampling, max_support,
                     supports, f_bins, half_width, n_freq=24, steps=3000, batch=4096, seed=0,
-                    log_every=0):
-    """Build the exact training set on the GPU and train a KernelSurrogate.
+                    log_every=0, relative=True, lr=2e-3):
+    """Build the exact training set on the GPU and train a KernelSurrogate
+    (relative: the network predicts the change per unit frequency offset).
     Returns (model, info)."""
     t0 = time.time()
     f_ref = float(np.mean(f_bins))
     x, t, peaks = training_set(dish, blockage, pairs, casa, pb_cell, npad, oversampling,
                                max_support, supports, f_bins, half_width, n_freq, f_ref, seed)
     t_data = time.time() - t0
+    import torch
     R = int(np.max(supports))
     model = KernelSurrogate(N_IN, R * R, seed=seed)
+    if relative:
+        # learn the change per unit of the frequency offset feature (examples at
+        # the bin frequency itself carry no change and are left out)
+        model.scale_feature = 4
+        d = x[:, 4:5]
+        keep = (d.abs() > 1e-9).squeeze(1)
+        x, t = x[keep].contiguous(), (t[keep] / d[keep]).contiguous()
     model.set_scaling(x, t)
     t0 = time.time()
     hist = model.train(model._xs(x), (t / model.t_scale).contiguous(), steps=steps, batch=batch,
-                       seed=seed + 1, log_every=log_every)
-    import torch
+                       lr=lr, seed=seed + 1, log_every=log_every)
     torch.cuda.synchronize()
     model.meta = dict(f_ref=f_ref, R=R, supports=np.asarray(supports).tolist(),
                       oversampling=[int(o) for o in oversampling],
                       half_width=float(half_width), f_bins=[float(f) for f in f_bins],
                       dish=[float(d) for d in dish], blockage=[float(b) for b in blockage],
-                      pairs=np.asarray(pairs).tolist())
+                      pairs=np.asarray(pairs).tolist(), scale_feature=model.scale_feature)
     info = dict(examples=int(x.shape[0]), data_s=round(t_data, 2),
                 train_s=round(time.time() - t0, 2), steps=steps, history=hist)
     return model, info
@@ -310,7 +332,8 @@ def per_channel_kernels(model, freq_chan, cf_chan_map, pb_freq, dish, blockage, 
                                     peak, oversampling)
             for k, ch in enumerate(chans):
                 ck[p, ch, 0] = blocks_to_kernel(blocks[k].reshape(
-                    oversampling[0] + 1, oversampling[1] + 1, R, R), sup, oversampling, K)
+                    oversampling[0] + 1, oversampling[1] + 1, R, R), sup, oversampling, K,
+                    base=ck_bins[p, b, 0])
     report["surrogate_s"] = round(time.time() - t0, 3)
     if check:
         t0 = time.time()
@@ -320,13 +343,18 @@ def per_channel_kernels(model, freq_chan, cf_chan_map, pb_freq, dish, blockage, 
         report["exact_s"] = round(time.time() - t0, 3)
         for p in range(n_pairs):
             for ch in range(n_chan):
-                ex = ck_exact[p, ch, 0]
+                # over the taps the gridder reads: the support window at every
+                # sub-cell offset
+                sup = np.asarray(supports)[p, cf_chan_map[ch], 0]
+                ex = tap_blocks(ck_exact[p, ch, 0], sup, oversampling, R)
                 peak = float(ex.abs().max().item())
-                e_s = float((ck[p, ch, 0] - ex).abs().max().item()) / peak
-                e_b = float((ck_bins[p, cf_chan_map[ch], 0] - ex).abs().max().item()) / peak
+                e_s = float((tap_blocks(ck[p, ch, 0], sup, oversampling, R) - ex)
+                            .abs().max().item()) / peak
+                e_b = float((tap_blocks(ck_bins[p, cf_chan_map[ch], 0], sup, oversampling, R)
+                             - ex).abs().max().item()) / peak
                 report["max_err_surrogate"] = max(report["max_err_surrogate"], e_s)
                 report["max_err_bin"] = max(report["max_err_bin"], e_b)
                 if e_s > TOL_PEAK:
-                    ck[p, ch, 0] = ex
+                    ck[p, ch, 0] = ck_exact[p, ch, 0]
                     report["fallback"].append((int(p), int(ch), e_s))
     return ck, wck, report

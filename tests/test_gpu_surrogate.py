@@ -43,6 +43,30 @@ def test_bf16_gemm_matches_rounded_product(gpu, ta, tb, act):
     del ref
 
 
+@pytest.mark.parametrize("M,K,N,ta,tb", [
+    (256, 4096, 289, True, False),   # weight gradient X^T dY: split over K
+    (10, 3000, 256, True, False),    # first-layer weight gradient, K not a multiple of 32
+    (4096, 289, 256, False, True),   # input gradient dY W^T
+    (1000, 256, 289, False, False),  # forward layer
+])
+def test_bf16_gemm_training_shapes(gpu, M, K, N, ta, tb):
+    """The shapes of the surrogate's training step, including the split-K
+    launches (partial products summed in split order)."""
+    import cngi_prototype_amd._native as nat
+    rng = np.random.default_rng(M + K + N)
+    A = rng.normal(size=(K, M) if ta else (M, K))
+    B = rng.normal(size=(N, K) if tb else (K, N))
+    Ao = A.T if ta else A
+    Bo = B.T if tb else B
+    ref = mlp.bf16(Ao) @ mlp.bf16(Bo)
+    out = nat.bf16_gemm(_t(A), _t(B), trans_a=ta, trans_b=tb)
+    # FP32 accumulation over K terms in another order than the float64 product
+    assert rel_err(out.cpu().numpy(), ref) < 1e-5
+    # accumulate into an existing output through the split path as well
+    out2 = nat.bf16_gemm(_t(A), _t(B), trans_a=ta, trans_b=tb, out=out.clone(), accumulate=True)
+    assert rel_err(out2.cpu().numpy(), 2 * ref) < 1e-5
+
+
 def test_gradients_match_oracle(gpu):
     import torch
     from cngi_prototype_amd.imaging._imaging_utils._kernel_surrogate import KernelSurrogate
@@ -82,7 +106,8 @@ def test_surrogate_beats_bin_kernels_and_falls_back(gpu):
     """Per-channel kernels over a 1 GHz band that the reference covers with one
     kernel (chan_tolerance_factor 0.005 at 230 GHz): the surrogate's largest
     tap error is below the single bin kernel's, and every kernel returned
-    (after the fallback) is within 1e-4 of its peak of the exact one."""
+    (after the fallback) is within 1e-4 of its peak of the exact one on the
+    samples the gridder reads."""
     import torch
     from helpers import mosaic_case
     from cngi_prototype_amd.imaging._imaging_utils._gcf_maps import (cf_baseline_map,
@@ -112,9 +137,22 @@ def test_surrogate_beats_bin_kernels_and_falls_back(gpu):
     assert rep["max_err_surrogate"] < rep["max_err_bin"]
     _, exact, _ = gcf_kernels(freq, dish, blockage, pairs, True, pb_cell, npad, os_, ms, 0.0,
                               supports=sup[:, cfc])
+    K = ck.shape[-1]
     for p in range(len(pairs)):
+        # the samples a gridder reads: cf index os*i + off + K//2 with
+        # |i| <= S//2 and |off| <= os//2
+        su, sv = sup[p, cfc[0], 0]
+        hu, hv = os_[0] * (su // 2) + os_[0] // 2, os_[1] * (sv // 2) + os_[1] // 2
+        win = (slice(K // 2 - hu, K // 2 + hu + 1), slice(K // 2 - hv, K // 2 + hv + 1))
         for ch in range(len(freq)):
             ex = exact[p, ch, 0]
-            err = float((ck_ch[p, ch, 0] - ex).abs().max().item()) / float(ex.abs().max().item())
+            got = ck_ch[p, ch, 0]
+            err = float((got[win] - ex[win]).abs().max().item()) / float(ex.abs().max().item())
             assert err <= ks.TOL_PEAK
+            # outside the window: the bin kernel's samples, or the exact kernel's
+            # after a fallback
+            outside = torch.ones_like(got, dtype=torch.bool)
+            outside[win] = False
+            assert (torch.equal(got[outside], ck[p, cfc[ch], 0][outside])
+                    or torch.equal(got[outside], ex[outside]))
     assert tuple(wck_ch.shape) == tuple(ck_ch.shape)
