@@ -22,7 +22,7 @@ GRID_IMAGE, GRID_PSF, GRID_DENSITY, GRID_WEIGHT = 0, 1, 2, 3
 PREC_FP64, PREC_FP64_TILED, PREC_FP32_KAHAN, PREC_FP32 = 0, 1, 2, 3
 PRECISIONS = {"fp64": PREC_FP64, "fp64_tiled": PREC_FP64_TILED, "fp32_kahan": PREC_FP32_KAHAN,
               "fp32": PREC_FP32}
-C64, C128, F32, F64 = 0, 1, 2, 3
+C64, C128, F32, F64, I32, I64, BOOL = 0, 1, 2, 3, 4, 5, 6
 WEIGHT_UNIFORM, WEIGHT_BRIGGS = 1, 2
 
 EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_version",
@@ -33,7 +33,7 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_grid_plan_create", "ngc_grid_plan_execute", "ngc_grid_plan_status",
            "ngc_grid_plan_describe", "ngc_grid_plan_destroy", "ngc_bf16_gemm",
            "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step",
-           "ngc_direction_rotate")
+           "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags")
 
 
 class NativeError(RuntimeError):
@@ -74,6 +74,9 @@ _PROTOS = {
     "ngc_gcf_resize": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
                               _vp]),
     "ngc_phase_gradient": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _vp]),
+    "ngc_apply_flags": (_i32, [_vp, _vp, _vp, _i32, _vp, _i64, _vp]),
+    "ngc_make_pb": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i32, _i32, _i64, _i64, _i64,
+                           _i64, _d, _d, _i64, _vp]),
     "ngc_profile_read": (_i32, [_vp, ctypes.c_char_p, _i64, _vp, _i64]),
     "ngc_normalize_mosaic": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64,
                                     _i64, _d, _i32]),
@@ -532,6 +535,54 @@ def phase_gradient(pix, resize_conv_size):
     rc = load_library().ngc_phase_gradient(ctx.handle, _stream(), pxp, len(px), k_u, k_v,
                                            _ptr(out))
     _check("ngc_phase_gradient", rc)
+    return out
+
+
+def apply_flags(data, flag, out=None):
+    """ngc_apply_flags: data where flag == 0, else the dtype's NaN value
+    (NaN, integer minimum, True).  Device tensors of the same shape; returns
+    out (a new tensor unless given; out may be data)."""
+    import torch
+    _dev(data, "data")
+    _dev(flag, "flag")
+    codes = {torch.complex64: C64, torch.complex128: C128, torch.float32: F32,
+             torch.float64: F64, torch.int32: I32, torch.int64: I64, torch.bool: BOOL}
+    if data.dtype not in codes:
+        raise TypeError(f"apply_flags does not support {data.dtype}")
+    if tuple(flag.shape) != tuple(data.shape):
+        raise ValueError("flag and data shapes differ")
+    if flag.dtype not in (torch.bool, torch.uint8):
+        raise TypeError("flag must be bool or uint8")
+    if not (data.is_contiguous() and flag.is_contiguous()):
+        raise ValueError("data and flag must be contiguous")
+    out = torch.empty_like(data) if out is None else out
+    if out.dtype != data.dtype or tuple(out.shape) != tuple(data.shape) or \
+            not out.is_contiguous():
+        raise ValueError("out must be a contiguous tensor like data")
+    rc = load_library().ngc_apply_flags(context().handle, _stream(), _ptr(data),
+                                        codes[data.dtype], _ptr(flag), data.numel(), _ptr(out))
+    _check("ngc_apply_flags", rc)
+    return out
+
+
+def make_pb(freq_chan, dish, blockage, casa, ipower, image_size, image_center, cell, n_pol):
+    """ngc_make_pb: device float64 [n_l, n_m, n_chan, n_pol, n_dish] Airy
+    primary beams (cell in radians)."""
+    import torch
+    ctx = context()
+    fr, frp = _host(np.asarray(freq_chan).ravel(), np.float64)
+    di, dip = _host(np.asarray(dish).ravel(), np.float64)
+    bl, blp = _host(np.asarray(blockage).ravel(), np.float64)
+    if len(di) != len(bl):
+        raise ValueError("dish and blockage lists differ in length")
+    n_l, n_m = int(image_size[0]), int(image_size[1])
+    out = torch.empty((n_l, n_m, len(fr), int(n_pol), len(di)), dtype=torch.float64,
+                      device="cuda")
+    rc = load_library().ngc_make_pb(ctx.handle, _stream(), frp, len(fr), dip, blp, len(di),
+                                    int(bool(casa)), int(ipower), n_l, n_m,
+                                    int(image_center[0]), int(image_center[1]),
+                                    float(cell[0]), float(cell[1]), int(n_pol), _ptr(out))
+    _check("ngc_make_pb", rc)
     return out
 
 

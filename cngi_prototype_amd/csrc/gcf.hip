@@ -8,6 +8,8 @@
 //   ngc_gcf_resize       crop and normalisation of resize_and_calc_support
 //                        (make_gridding_convolution_function.py:361-391)
 //   ngc_phase_gradient   the table of make_phase_gradient (:349-359)
+//   ngc_make_pb          the primary beam image of make_pb (make_pb.py:28-123) with
+//                        _casa_airy_disk / _airy_disk (_make_pb_symmetric.py:24-130)
 #include "ngc_internal.hpp"
 
 namespace ngc {
@@ -74,6 +76,41 @@ __global__ void pattern_kernel(PatternArgs a, double2 *buf) {
             }
         }
         buf[i] = make_double2(p, 0.0);
+    }
+}
+
+struct PbArgs {
+    const double *freq;      // [n_chan]
+    const double *dish;      // [n_dish]
+    const double *blockage;  // [n_dish]
+    int64_t n_l, n_m, n_chan, n_pol, n_dish;
+    int64_t c_l, c_m;        // image centre pixel
+    double cell_l, cell_m;   // radians
+    int casa, ipower;
+};
+
+// One thread per (l, m, chan, dish): the dish's pattern at image pixel (l, m),
+// written to every pol of out[l][m][chan][pol][dish].
+__global__ void pb_kernel(PbArgs a, double *out) {
+#pragma clang fp contract(off)
+    const int64_t n = a.n_l * a.n_m * a.n_chan * a.n_dish;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = i % a.n_dish;
+        const int64_t rest = i / a.n_dish;
+        const int64_t c = rest % a.n_chan;
+        const int64_t lm = rest / a.n_chan;
+        const int64_t l = lm / a.n_m, m = lm - l * a.n_m;
+        double p = 1.0;
+        if (!(l == a.c_l && m == a.c_m)) {
+            const double xl = (double)(l - a.c_l) * a.cell_l;
+            const double ym = (double)(m - a.c_m) * a.cell_m;
+            const double k = (2.0 * M_PI * a.freq[c]) / kCLight;
+            const double r = (sqrt(xl * xl + ym * ym) * k) * (a.dish[d] / 2.0);
+            p = airy_pattern(r, a.dish[d], a.blockage[d], a.casa, a.ipower);
+        }
+        double *o = out + (lm * a.n_chan + c) * a.n_pol * a.n_dish + d;
+        for (int64_t q = 0; q < a.n_pol; q++) o[q * a.n_dish] = p;
     }
 }
 
@@ -291,6 +328,45 @@ extern "C" int ngc_phase_gradient(ngc_context *ctx, void *stream, const double *
     if (n == 0) return NGC_OK;
     phase_gradient_kernel<<<nblocks(n), 256, 0, s>>>((const double *)d_pix, n_field, k_u, k_v,
                                                      (double2 *)out);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+extern "C" int ngc_make_pb(ngc_context *ctx, void *stream, const double *freq_chan,
+                           int64_t n_chan, const double *dish, const double *blockage,
+                           int64_t n_dish, int casa, int ipower, int64_t n_l, int64_t n_m,
+                           int64_t center_l, int64_t center_m, double cell_l, double cell_m,
+                           int64_t n_pol, double *out) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    NGC_CHECK(freq_chan && dish && blockage && out, NGC_ERR_ARG, "NULL array");
+    NGC_CHECK(n_chan >= 1 && n_dish >= 1 && n_pol >= 1 && n_l >= 1 && n_m >= 1, NGC_ERR_ARG,
+              "bad dimensions");
+    NGC_CHECK(center_l >= 0 && center_l < n_l && center_m >= 0 && center_m < n_m, NGC_ERR_ARG,
+              "image centre outside the image");
+    NGC_CHECK(ipower == 1 || ipower == 2, NGC_ERR_ARG, "ipower must be 1 or 2");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = (hipStream_t)stream;
+    NGC_TRY(enter_stream(ctx, s));
+    void *d_freq, *d_dish, *d_blk;
+    NGC_TRY(upload(ctx, s, "pb_freq", freq_chan, n_chan * sizeof(double), &d_freq));
+    NGC_TRY(upload(ctx, s, "pb_dish", dish, n_dish * sizeof(double), &d_dish));
+    NGC_TRY(upload(ctx, s, "pb_blk", blockage, n_dish * sizeof(double), &d_blk));
+    PbArgs a;
+    a.freq = (const double *)d_freq;
+    a.dish = (const double *)d_dish;
+    a.blockage = (const double *)d_blk;
+    a.n_l = n_l;
+    a.n_m = n_m;
+    a.n_chan = n_chan;
+    a.n_pol = n_pol;
+    a.n_dish = n_dish;
+    a.c_l = center_l;
+    a.c_m = center_m;
+    a.cell_l = cell_l;
+    a.cell_m = cell_m;
+    a.casa = casa;
+    a.ipower = ipower;
+    pb_kernel<<<nblocks(n_l * n_m * n_chan * n_dish), 256, 0, s>>>(a, out);
     NGC_HIP(hipGetLastError());
     return NGC_OK;
 }

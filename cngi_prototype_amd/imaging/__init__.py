@@ -12,4 +12,5 @@ from .make_gridding_convolution_function import make_gridding_convolution_functi
 from .make_mosaic_pb import make_mosaic_pb
 from .make_image_with_gcf import make_image_with_gcf
 from .make_psf_with_gcf import make_psf_with_gcf
+from .make_pb import make_pb
 from .direction_rotate import direction_rotate

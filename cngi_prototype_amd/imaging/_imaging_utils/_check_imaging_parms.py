@@ -5,7 +5,7 @@ Same keys, defaults, ranges and unit conversions as the reference:
 negates cell[0] and sets ``image_size_padded = int(fft_padding*image_size)``.
 One optional key is added, ``grid_parms['precision']`` (absent by default),
 choosing the GPU form: 'fp64' (plain FP64 drop-in, the default),
-'fp64_tiled' or 'fp32_kahan'.
+'fp64_tiled', 'fp32_kahan' or 'fp32'.
 """
 import numbers
 
@@ -14,7 +14,7 @@ import numpy as np
 from ..._utils._check_parms import INT_TYPES, _check_parms
 
 ARCSEC_TO_RAD = np.pi / (3600 * 180)
-PRECISIONS = ['fp64', 'fp64_tiled', 'fp32_kahan']
+PRECISIONS = ['fp64', 'fp64_tiled', 'fp32_kahan', 'fp32']
 
 
 def _check_grid_parms(grid_parms):
@@ -84,6 +84,24 @@ def _check_gcf_parms(gcf_parms):
         for key in ('freq_chan', 'list_dish_diameters', 'list_blockage_diameters',
                     'unique_ant_indx', 'pol'):
             gcf_parms[key] = np.array(gcf_parms[key])
+    return parms_passed
+
+
+def _check_pb_parms(img_dataset, pb_parms):
+    """_check_imaging_parms.py:148-164."""
+    parms_passed = True
+    if not (_check_parms(pb_parms, 'function', [str], default='casa_airy')):
+        parms_passed = False
+    if not (_check_parms(pb_parms, 'list_dish_diameters', [list],
+                         list_acceptable_data_types=[numbers.Number], list_len=-1)):
+        parms_passed = False
+    if not (_check_parms(pb_parms, 'list_blockage_diameters', [list],
+                         list_acceptable_data_types=[numbers.Number], list_len=-1)):
+        parms_passed = False
+    if len(pb_parms['list_dish_diameters']) != len(pb_parms['list_blockage_diameters']):
+        print('######### ERROR:Parameter ', 'list_dish_diameters and list_blockage_diameters '
+              'must be the same length.')
+        parms_passed = False
     return parms_passed
 
 

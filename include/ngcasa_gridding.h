@@ -92,6 +92,9 @@ extern "C" {
 #define NGC_C128 1
 #define NGC_F32 2
 #define NGC_F64 3
+#define NGC_I32 4
+#define NGC_I64 5
+#define NGC_BOOL 6            /* one byte per element, 0 or 1 */
 
 /* weighting for ngc_briggs_factors (make_imaging_weight.py:198-213) */
 #define NGC_WEIGHT_UNIFORM 1
@@ -303,6 +306,33 @@ int ngc_gcf_resize(ngc_context *ctx, void *stream, const double *full, int64_t n
  */
 int ngc_phase_gradient(ngc_context *ctx, void *stream, const double *pix, int64_t n_field,
                        int64_t k_u, int64_t k_v, double *out);
+
+/*
+ * Flags applied to one data variable: replaces the per-variable body of
+ * apply_flags (cngi/vis/apply_flags.py:49-55), dv.where(flag == 0).astype(dv.dtype).
+ * out[i] = data[i] where flag[i] == 0, else NaN (NGC_F32/F64, both parts for
+ * NGC_C64/C128), the integer minimum (NGC_I32/I64) or 1 (NGC_BOOL).
+ *   data, out  device arrays of n elements of `dtype` (out may equal data)
+ *   flag       device uint8 [n]
+ */
+int ngc_apply_flags(ngc_context *ctx, void *stream, const void *data, int dtype,
+                    const uint8_t *flag, int64_t n, void *out);
+
+/*
+ * Primary beam image: replaces the map_blocks call of make_pb (make_pb.py:100-104)
+ * over _casa_airy_disk (casa != 0) / _airy_disk (_imaging_utils/_make_pb_symmetric.py:
+ * 24-130).  Pixel (l, m) of channel c and dish d is the Airy power (ipower 2) or
+ * voltage (ipower 1) pattern at radius argument
+ *   r = sqrt(((l - center_l) cell_l)^2 + ((m - center_m) cell_m)^2) * (2 pi f_c / c) * D_d / 2,
+ * and exactly 1 at the centre pixel; the value is repeated over the n_pol pols.
+ *   freq_chan  host float64 [n_chan];  dish, blockage  host float64 [n_dish] (metres)
+ *   cell_l, cell_m  radians
+ *   out  device float64 [n_l, n_m, n_chan, n_pol, n_dish]
+ */
+int ngc_make_pb(ngc_context *ctx, void *stream, const double *freq_chan, int64_t n_chan,
+                const double *dish, const double *blockage, int64_t n_dish, int casa,
+                int ipower, int64_t n_l, int64_t n_m, int64_t center_l, int64_t center_m,
+                double cell_l, double cell_m, int64_t n_pol, double *out);
 
 /*
  * Gridding plans (no counterpart in the reference, whose graph layer rebuilds

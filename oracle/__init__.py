@@ -399,6 +399,15 @@ def airy_rorder(freq_chan, n_pol, dish, blockage, ipower, cell, image_size,
     return np.tile(out, (1, 1, n_pol, 1, 1))
 
 
+def pb_image(freq_chan, n_pol, dish, blockage, ipower, cell, image_size, image_center,
+             casa=True):
+    """_casa_airy_disk / _airy_disk (_make_pb_symmetric.py:24-130), the
+    pattern function of make_pb (make_pb.py:86-104): the same values as the
+    _rorder forms above, laid out [Nx, Ny, n_freq, n_pol, n_dish]."""
+    return np.transpose(airy_rorder(freq_chan, n_pol, dish, blockage, ipower, cell, image_size,
+                                    image_center, casa), (3, 4, 1, 2, 0))
+
+
 def cf_baseline_map(unique_ant_indx, baseline_ant, n_unique_ant):
     """create_cf_baseline_map (make_gridding_convolution_function.py:512-528)."""
     n_pairs = int((n_unique_ant ** 2 + n_unique_ant) / 2)

@@ -38,3 +38,17 @@ def test_version_and_error_string_without_gpu():
     lib = _native.load_library()
     assert b"gfx950" in lib.ngc_version()
     assert lib.ngc_last_error() is not None
+
+
+def test_binding_covers_the_header():
+    """The ctypes binding declares a prototype for every entry point of the
+    header, and its dtype/mode/precision codes are the header's."""
+    from cngi_prototype_amd import _native
+    assert sorted(_native.EXPORTS) == declared_symbols()
+    with open(os.path.join(ROOT, "include", "ngcasa_gridding.h")) as f:
+        defines = dict((k, int(v)) for k, v in
+                       re.findall(r"^#define (NGC_\w+) (-?\d+)", f.read(), re.M))
+    for name in ("C64", "C128", "F32", "F64", "I32", "I64", "BOOL"):
+        assert getattr(_native, name) == defines["NGC_" + name], name
+    for name in ("IMAGE", "PSF", "DENSITY", "WEIGHT"):
+        assert getattr(_native, "GRID_" + name) == defines["NGC_GRID_" + name], name

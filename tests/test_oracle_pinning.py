@@ -472,6 +472,32 @@ def test_airy_pattern_limits():
     assert np.allclose(pb[0, 0, 0], ref, atol=1e-6)
 
 
+def test_pb_image_layout_and_scaling():
+    """make_pb's layout [l, m, chan, pol, dish]: 1 at the image centre (also
+    off the middle of the image), the unblocked power pattern (2 J1(r)/r)^2 at
+    a pixel straight along l and along m, the same value at twice the
+    distance for half the frequency (the pattern depends on |l| f only),
+    identical pols, and the blocked dish type in the last axis."""
+    from scipy.special import j1
+    f = np.array([1.0e11, 2.0e11])
+    nl, nm = 40, 33
+    cell = np.array([-2e-6, 2e-6])
+    c = [17, 12]
+    pb = oracle.pb_image(f, 2, [12.0, 12.0], [0.0, 0.75], 2, cell, [nl, nm], c, casa=True)
+    assert pb.shape == (nl, nm, 2, 2, 2)
+    assert np.all(pb[c[0], c[1]] == 1.0)
+    assert np.array_equal(pb[..., 0, :], pb[..., 1, :])
+    k = 2 * np.pi * f / C
+    for dl, dm in ((5, 0), (0, 7), (3, 4)):
+        r = np.hypot(dl * cell[0], dm * cell[1]) * k * 6.0
+        assert np.allclose(pb[c[0] + dl, c[1] + dm, :, 0, 0], (2 * j1(r) / r) ** 2, rtol=1e-13)
+    # |l| f scaling: pixel distance 2d at f equals distance d at 2f
+    assert np.allclose(pb[c[0] + 10, c[1], 0, 0, 0], pb[c[0] + 5, c[1], 1, 0, 0], rtol=1e-12)
+    assert np.allclose(pb[c[0], c[1] + 8, 0], pb[c[0], c[1] + 4, 1], rtol=1e-12)
+    # the blocked dish differs from the unblocked one away from the centre
+    assert np.abs(pb[c[0] + 6, c[1], :, 0, 1] - pb[c[0] + 6, c[1], :, 0, 0]).max() > 1e-6
+
+
 def test_cf_maps_hand_cases():
     """Dish-type pairs (i <= j) in row-major order; a baseline whose types
     come reversed keeps the default pair 0.  Channel bins: n = int(floor(BW /
