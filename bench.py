@@ -480,6 +480,9 @@ def main():
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16, 32],
                     help="uv-tile rows of the gridder (0: library default, 16)")
+    ap.add_argument("--tile-waves", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="wavefronts per tile of the gridder, splitting the pols "
+                         "(0: library default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the steps, run the grid's ifftshift (a plain copy kernel of "
                          "known size) so that a counter run sees a kernel of known traffic")
@@ -500,6 +503,8 @@ def main():
         cropped_correction_factors, prolate_spheroidal_kernel_1d
 
     torch.cuda.set_device(0)
+    if args.tile_waves:
+        nat.set_tile_waves(args.tile_waves)
     if args.mosaic_only:
         print(json.dumps({"mosaic": mosaic_leg(args)}), file=out, flush=True)
         return

@@ -33,7 +33,7 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_grid_plan_create", "ngc_grid_plan_execute", "ngc_grid_plan_status",
            "ngc_grid_plan_describe", "ngc_grid_plan_destroy", "ngc_bf16_gemm",
            "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step",
-           "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags")
+           "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags", "ngc_set_tile_waves")
 
 
 class NativeError(RuntimeError):
@@ -69,6 +69,7 @@ _PROTOS = {
                                  _vp, _vp]),
     "ngc_profile_enable": (_i32, [_vp, _i32]),
     "ngc_set_tile_rows": (_i32, [_vp, _i32]),
+    "ngc_set_tile_waves": (_i32, [_vp, _i32]),
     "ngc_gcf_patterns": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _i32, _i32, _d,
                                 _d, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "ngc_gcf_resize": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
@@ -478,6 +479,13 @@ def set_tile_rows(rows):
     """Tile rows (8, 16 or 32) of the uv-tiled gridders on the current device."""
     ctx = context()
     _check("ngc_set_tile_rows", load_library().ngc_set_tile_rows(ctx.handle, int(rows)))
+
+
+def set_tile_waves(waves):
+    """Wavefronts per tile of the uv-tiled standard gridder (0: default, 2 for
+    4 pols and 1 otherwise; 1, 2 or 4) on the current device."""
+    ctx = context()
+    _check("ngc_set_tile_waves", load_library().ngc_set_tile_waves(ctx.handle, int(waves)))
 
 
 def gcf_patterns(pb_freq, dish, blockage, pairs, casa, ipower, cell_lm, n_uv):

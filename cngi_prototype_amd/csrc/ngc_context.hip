@@ -106,6 +106,15 @@ int ngc_set_tile_rows(ngc_context *ctx, int tile_rows) {
     return NGC_OK;
 }
 
+int ngc_set_tile_waves(ngc_context *ctx, int waves) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    NGC_CHECK(waves == 0 || waves == 1 || waves == 2 || waves == 4, NGC_ERR_ARG,
+              "waves must be 0 (default), 1, 2 or 4");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tile_waves = waves;
+    return NGC_OK;
+}
+
 int ngc_profile_enable(ngc_context *ctx, int enable) {
     NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
     std::lock_guard<std::mutex> lock(ctx->mu);

@@ -87,6 +87,7 @@ struct ngc_context {
     int *d_error = nullptr;  // device error flags [4]
     int tile_rows = 16;      // uv-tile rows of the tiled gridder (16 or 32)
     int bin_chan_lanes = 2;  // channels per binning wavefront (1, 2, 4, 8)
+    int tile_waves = 0;      // wavefronts per tile of the tiled gridder (0: default)
     int n_cu = 256;          // compute units of the device
     // autotuned (tile rows, binning channel lanes) per gridding shape
     std::map<std::string, std::pair<int, int>> tuned;

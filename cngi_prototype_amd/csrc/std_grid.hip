@@ -15,10 +15,11 @@
 //    consecutive integrations of one baseline together).  The fill pass
 //    stores per sample the cell, the sub-cell offsets and the weighted
 //    visibilities of all pols in the accumulation type, and adds the FP64
-//    sum_weight term.  One 64-lane wavefront then grids one tile into an LDS
-//    subgrid (lane-owned residue classes, register runs; see the kernel) and
-//    writes each grid cell to HBM once.  Pixel positions and sum_weight stay
-//    FP64; FP32_KAHAN compensates every LDS flush.
+//    sum_weight term.  One workgroup of 1, 2 or 4 wavefronts (splitting the
+//    pols) then grids one tile into an LDS subgrid (lane-owned residue
+//    classes, register runs; see the kernel) and writes each grid cell to HBM
+//    once.  Pixel positions and sum_weight stay FP64; FP32_KAHAN accumulates
+//    the FP32 register runs into FP64 LDS cells.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -334,86 +335,178 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
 }
 
 // ---------------------------------------------------------------------------
-// tiled gridding kernel: one wavefront per (image channel, TU x 32 tile) bin
+// tiled gridding kernel: one workgroup per (image channel, TU x 32 tile) bin
 // ---------------------------------------------------------------------------
 // Work distribution after J. W. Romein, "An efficient work-distribution
 // strategy for gridding radio-telescope data on GPUs" (ICS 2012), applied to
-// an LDS subgrid: lane (a, b) of the wavefront owns every tile cell with
+// an LDS subgrid: lane (a, b) of a wavefront owns every tile cell with
 // u = a (mod 8) and v = b (mod 8).  A support <= 8 window covers each residue
 // pair at most once, so for every sample each lane has at most one tap, and
-// no two lanes ever touch the same cell.  A lane keeps the running sum of its
-// current cell in registers and writes it to the LDS subgrid (plain
-// read-modify-write, no atomics) only when its cell changes; consecutive
-// integrations of one baseline move slowly across the grid, so most taps
-// never leave the register file.  With KAHAN the LDS subgrid carries a
-// Kahan-Neumaier compensation word per value and every flush is a
-// compensated add.  The tile is written to HBM once at the end.
+// no two lanes of a wavefront ever touch the same cell.  A lane keeps the
+// running sum of its current cell in registers and adds it to the LDS
+// subgrid only when its cell changes; consecutive integrations of one
+// baseline move slowly across the grid, so most taps never leave the
+// register file.  The add is a no-return LDS atomic (ds_add_f32/f64): no two
+// lanes share a cell, but the read-modify-write then happens inside the LDS
+// unit and the wavefront does not wait for it.  FP32_KAHAN keeps FP64 cells
+// (one conversion and one add per flush); the tile is written to HBM once at
+// the end.  The W wavefronts of a workgroup split the pols of the tile, each
+// with its own accumulators and the same samples.
 //
-// Per-sample work is done once per sample by the lane that stages it, in
-// per-residue layout: for residue r the u tap weight of the lane owning u = r
-// (mod 8), with the sign bit set when that residue's cell differs from the
-// previous sample's (the weights are non-negative), the same for v, and the
-// residue's cell coordinates.  In the broadcast loop a lane then reads its
-// two words at a fixed offset, ORs the sign bits to decide a flush, and
-// multiplies the magnitudes (abs is an operand modifier).  TU (16 or 32 rows)
-// trades LDS per wavefront, hence occupancy, against the share of samples
-// binned to two tiles.
+// Per-sample work is done once per sample and axis by the lanes that stage
+// it, in per-residue layout: for residue r the tap weight of the lanes owning
+// u = r (mod 8), with the sign bit set when that residue's cell differs from
+// the previous sample's (the weights are non-negative), and the cell; the
+// same for v.  In the broadcast loop a lane then reads its two records at a
+// fixed offset, ORs the sign bits to decide a flush, and multiplies the
+// magnitudes (abs is an operand modifier).  Samples are read from LDS in
+// double-buffered groups, so a group's reads do not queue behind the
+// previous group's adds; the next batch's entries are loaded from HBM while
+// the current batch is gridded.  TU (8, 16 or 32 rows) trades LDS per
+// workgroup, hence occupancy, against the share of samples binned to two
+// tiles.
 constexpr int kBatch = 32;
-constexpr int kGroup = 4;
-constexpr int kRow = kTile + 1;  // LDS row stride (words)
+constexpr int kGroup = 8;
+constexpr int kBuckets = 40;       // bin-size classes of the largest-first order
+
+// size class of a bin of n records: 0 when empty, else floor(log2 n) + 1
+__device__ inline int bin_bucket(unsigned long long n) {
+    if (n == 0) return 0;
+    const int b = 64 - __clzll((long long)n);
+    return b < kBuckets ? b : kBuckets - 1;
+}
+
+constexpr int kChunk = 16;        // tiles per work ticket of small bins
+constexpr int kChunkBelow = 32;   // records below which a bin is small
+
+// LDS subgrid layout: row-major tile planes of 32 cells per row, with the
+// columns of row u XOR-swizzled: cell (u, v) at u 32 + (v ^ X (u mod 8)),
+// X = 2 for 8-byte and 4 for 4-byte cells.  The lanes of a wavefront own the
+// cells of one 8 x 8 window (u = a, v = b mod 8); in this layout the lanes
+// that the LDS serves together fall on distinct banks for every window
+// position, and so do consecutive cells of one row as the write-out reads
+// them, without padding (which would cost occupancy).
+template <int CELL_BYTES>
+__device__ inline int subgrid_index(int u, int v) {
+    constexpr int X = CELL_BYTES == 8 ? 2 : 4;
+    return u * kTile + (v ^ (X * (u & 7)));
+}
 
 // cell of residue r for a window starting at w: the first u >= w with u = r (mod 8)
 __device__ inline int residue_cell(int w, int r) { return w + ((r - w) & 7); }
 
-template <int MODE, int NP, typename ACC, bool KAHAN, int TU>
-__global__ void __launch_bounds__(64)
+template <int MODE, int NP, typename ACC, bool KAHAN, int TU, int W>
+__global__ void __launch_bounds__(64 * W)
 std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict__ offsets,
-                      const unsigned int *__restrict__ order, int *queue, int n_items,
+                      const unsigned int *__restrict__ order,
+                      const unsigned int *__restrict__ hist, int *queue, int n_items,
                       unsigned long long cap, const char *__restrict__ entries,
                       const double *__restrict__ cgk, int cgk_len, void *grid_out,
                       int accumulate) {
     constexpr int NC = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
-    constexpr int PLANE = TU * kRow;
     constexpr int NPC = NP * NC;
+    static_assert(NP % W == 0, "the pols split evenly over the wavefronts");
+    constexpr int QW = NPC / W;         // accumulators (pol, re/im) of one wavefront
+    constexpr int SPW = kBatch / W;     // samples staged by one wavefront per batch
+    constexpr int NT = 64 * W;
     using E = Entry<NP, NC, ACC>;
     constexpr int EB = EntrySize<NP, NC, ACC>::bytes;
+    // weighted visibilities of a staged sample, split by wavefront so that a
+    // wavefront loads only its own accumulators' values
     struct WD {
-        ACC v[NPC];
+        ACC v[NPC / W];
     };
 
-    // LDS subgrid: plain sums, or (sum, compensation) pairs side by side so
-    // that a compensated flush is one 8-byte read and one 8-byte write
-    using CELL = typename std::conditional<KAHAN, float2, ACC>::type;
+    // LDS subgrid: plain sums, or for the compensated form FP64 cells (the
+    // same 8 bytes as a (sum, compensation) pair of floats, and one
+    // conversion and one add per flush instead of a TwoSum)
+    using CELL = typename std::conditional<KAHAN, double, ACC>::type;
+    constexpr int PLANE = TU * kTile;
     __shared__ CELL s_acc[NPC * PLANE];
-    __shared__ ACC s_wu[kBatch][8];    // u weight per residue, sign = cell changed
-    __shared__ ACC s_wv[kBatch][8];    // v weight per residue, sign = cell changed
-    __shared__ int s_cell[kBatch][8];  // cell_u(r) | cell_v(r) << 16 (tile-relative + 64)
-    __shared__ WD s_wd[kBatch];
+    // per staged sample, axis (u, v) and residue: the tap weight (sign bit set
+    // when the residue's cell differs from the previous sample's) and the
+    // cell (tile-relative + 64); rows padded so the staging writes spread
+    // over the banks
+    struct AxW {
+        ACC w;
+        int cell;
+    };
+    constexpr int AXROW = 9;
+    __shared__ __align__(16) AxW s_ax[2][kBatch][AXROW];
+    __shared__ WD s_wd[W][kBatch];
     __shared__ ACC s_k[512];
+    __shared__ int s_item;
+    __shared__ int64_t s_tq[kChunk][3];   // bin, first and end record of each tile of a ticket
 
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int q0 = wave * QW;  // this wavefront's accumulators are q0 .. q0 + QW - 1
+    // the kernel table, once per workgroup: all loads issued before the stores
+    {
+        constexpr int KL = 512 / NT;
+        ACC kv[KL];
+#pragma unroll
+        for (int k = 0; k < KL; k++) {
+            const int i = tid + k * NT;
+            kv[k] = i < cgk_len ? (ACC)cgk[i] : (ACC)0;
+        }
+#pragma unroll
+        for (int k = 0; k < KL; k++) s_k[tid + k * NT] = kv[k];
+    }
+#pragma unroll 4
+    for (int i = tid; i < NPC * PLANE; i += NT) s_acc[i] = CELL{};
     // persistent: each workgroup takes the next tile (largest bins first) from
     // the queue until all n_items are done; every wave reaches the exit
+    // work tickets: bins of kChunkBelow records or more (the front of the
+    // largest-first order) are handed out one per ticket, for balance; the
+    // small and empty bins after them in runs of kChunk (fewer atomics on the
+    // queue counter, one round trip for their records)
+    if (tid == 0) {
+        int nb = 0;
+        for (int b = bin_bucket(kChunkBelow); b < kBuckets; b++) nb += (int)hist[b];
+        s_item = nb;
+    }
+    __syncthreads();
+    const int n_single = s_item;
+    int it_next = 0, it_end = 0, it_first = 0;
     for (;;) {
-    int item = 0;
-    if (lane == 0) item = atomicAdd(queue, 1);
-    item = __shfl(item, 0, 64);
-    if (item >= n_items) break;
-    __syncthreads();  // the previous tile's LDS reads are complete
-    const int64_t bin = order[item];
+    __syncthreads();  // the previous tile's LDS reads (and s_item, s_tq) are complete
+    if (it_next >= it_end) {
+        if (tid == 0) s_item = atomicAdd(queue, 1);
+        __syncthreads();
+        const int t = s_item;
+        if (t < n_single) {
+            it_next = t;
+            it_end = t + 1;
+        } else {
+            const int64_t first = (int64_t)n_single + (int64_t)(t - n_single) * kChunk;
+            it_next = first < n_items ? (int)first : n_items;
+            it_end = it_next + kChunk;
+        }
+        if (it_end > n_items) it_end = n_items;
+        if (it_next >= n_items) break;
+        // the chunk's bins and record ranges, loaded together
+        it_first = it_next;
+        if (tid < it_end - it_next) {
+            const int64_t b = order[it_next + tid];
+            s_tq[tid][0] = b;
+            s_tq[tid][1] = (int64_t)min(offsets[b], cap);
+            s_tq[tid][2] = (int64_t)min(offsets[b + 1], cap);
+        }
+        __syncthreads();
+    }
+    const int ci = it_next - it_first;
+    it_next++;
+    const int64_t bin = s_tq[ci][0];
     const int64_t tiles = g.ntu * g.ntv;
     const int64_t a_chan = bin / tiles;
     const int64_t trem = bin - a_chan * tiles;
     const int u_lo = (int)((trem / g.ntv) * TU);
     const int v_lo = (int)((trem % g.ntv) * kTile);
-    const int64_t start = (int64_t)min(offsets[bin], cap);
-    const int64_t end = (int64_t)min(offsets[bin + 1], cap);
+    const int64_t start = s_tq[ci][1];
+    const int64_t end = s_tq[ci][2];
 
-#pragma unroll 4
-    for (int i = lane; i < NPC * PLANE; i += 64) s_acc[i] = CELL{};
-    if (end > start)
-        for (int i = lane; i < 512; i += 64) s_k[i] = i < cgk_len ? (ACC)cgk[i] : (ACC)0;
-    __syncthreads();
+    // (the subgrid is clear here: zeroed at the start, then by each write-out)
 
     const int sc = (int)(a.support / 2);
     const int S = (int)a.support;
@@ -421,130 +514,173 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     const int ra = lane & 7, rb = lane >> 3;  // owned residues
 
     int cur_u = -64, cur_v = -64;  // current cell (tile-relative), off-tile = none
-    ACC acc[NPC];
+    ACC acc[QW];
 #pragma unroll
-    for (int q = 0; q < NPC; q++) acc[q] = (ACC)0;
+    for (int q = 0; q < QW; q++) acc[q] = (ACC)0;
 
     auto flush = [&]() {
         if ((unsigned)cur_u < (unsigned)TU && (unsigned)cur_v < (unsigned)kTile) {
-            const int o = cur_u * kRow + cur_v;
+            const int o = subgrid_index<(int)sizeof(CELL)>(cur_u, cur_v);
 #pragma unroll
-            for (int q = 0; q < NPC; q++) {
-                CELL *dst = &s_acc[q * PLANE + o];
-                if constexpr (KAHAN) {
-                    // Knuth's TwoSum: t + e == s + x exactly, e joins the compensation
-                    const float x = acc[q];
-                    float2 sc = *dst;
-                    const float t = sc.x + x;
-                    const float z = t - sc.x;
-                    const float e = (sc.x - (t - z)) + (x - z);
-                    sc.y += e;
-                    sc.x = t;
-                    *dst = sc;
-                } else {
-                    *dst = *dst + acc[q];
-                }
+            for (int q = 0; q < QW; q++) {
+                // a read-modify-write inside the LDS unit (no two lanes share a
+                // cell; the atomic only saves the wavefront the round trip)
+                __hip_atomic_fetch_add(&s_acc[(q0 + q) * PLANE + o], (CELL)acc[q],
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
     };
 
-    int prev_lu0 = -1000, prev_lv0 = -1000;  // window of the last sample of the previous batch
+    // staging lanes: lane l < 2 SPW of wavefront w stages axis l / SPW of
+    // sample j = w SPW + l % SPW (entry b0 + j) for the batch at b0; the next
+    // batch's entries are loaded while the current one is gridded
+    const int js = lane % SPW, axis = lane / SPW;
+    const int j_st = wave * SPW + js;
+    const bool stager = lane < 2 * SPW;
+    E en{};
+    unsigned int puv = 0;
+    auto load_batch = [&](int64_t b) {
+        const int64_t e = b + j_st;
+        if (stager && e < end) {
+            en = *reinterpret_cast<const E *>(entries + e * EB);
+            const int64_t pe = e > start ? e - 1 : e;
+            puv = reinterpret_cast<const E *>(entries + pe * EB)->uv;
+        }
+    };
+    load_batch(start);
     for (int64_t b0 = start; b0 < end; b0 += kBatch) {
-        // staging: lane j < kBatch unpacks sample b0 + j
-        const int64_t e = b0 + lane;
-        const bool valid = lane < kBatch && e < end;
-        int lu0 = -1000, lv0 = -1000, du = 0, dv = 0;
-        if (valid) {
-            const E en = *reinterpret_cast<const E *>(entries + e * EB);
-            lu0 = (en.uv & 0xffff) - sc - u_lo;
-            lv0 = ((en.uv >> 16) & 0xffff) - sc - v_lo;
-            du = (int)(short)(en.duv & 0xffff);
-            dv = (int)(short)((en.duv >> 16) & 0xffff);
-            WD w;
-#pragma unroll
-            for (int p = 0; p < NP; p++)
-#pragma unroll
-                for (int c = 0; c < NC; c++) w.v[p * NC + c] = en.wd[p][c];
-            s_wd[lane] = w;
-        }
-        int plu0 = __shfl_up(lu0, 1, 64), plv0 = __shfl_up(lv0, 1, 64);
-        if (lane == 0) {
-            plu0 = prev_lu0;
-            plv0 = prev_lv0;
-        }
-        if (lane < kBatch) {
+        const int64_t e = b0 + j_st;
+        const bool valid = stager && e < end;
+        if (stager) {
+            // this lane's axis: window start, previous sample's window start and
+            // sub-cell offset (previous = none for the first entry of the bin)
+            const int shift = axis ? 16 : 0;
+            const int lo = axis ? v_lo : u_lo;
+            int c0 = -1000, pc0 = -1000, d = 0;
+            if (valid) {
+                c0 = (int)((en.uv >> shift) & 0xffff) - sc - lo;
+                d = (int)(short)((en.duv >> shift) & 0xffff);
+                if (e > start) pc0 = (int)((puv >> shift) & 0xffff) - sc - lo;
+            }
+            AxW row[8];
 #pragma unroll
             for (int r = 0; r < 8; r++) {
-                const int cu = residue_cell(lu0, r), cv = residue_cell(lv0, r);
-                const int iu = cu - lu0, iv = cv - lv0;
-                const int ku = abs(os * (iu - sc) + du);
-                const int kv = abs(os * (iv - sc) + dv);
-                ACC wu = (valid && iu < S && ku < cgk_len) ? s_k[ku] : (ACC)0;
-                ACC wv = (valid && iv < S && kv < cgk_len) ? s_k[kv] : (ACC)0;
-                if (cu != residue_cell(plu0, r)) wu = -wu - (ACC)0;  // sign bit set, even for 0
-                if (cv != residue_cell(plv0, r)) wv = -wv - (ACC)0;
-                s_wu[lane][r] = wu;
-                s_wv[lane][r] = wv;
-                s_cell[lane][r] = ((cu + 64) & 0xffff) | ((cv + 64) << 16);
+                const int cell = residue_cell(c0, r);
+                const int it = cell - c0;
+                const int k = abs(os * (it - sc) + d);
+                const ACC t = s_k[k < 511 ? k : 511];   // the table is zero past cgk_len
+                ACC w = (valid && it < S && k < cgk_len) ? t : (ACC)0;
+                if (cell != residue_cell(pc0, r)) w = -w - (ACC)0;  // sign bit set, even for 0
+                row[r].w = w;
+                row[r].cell = cell + 64;
+            }
+#pragma unroll
+            for (int r = 0; r < 8; r++) s_ax[axis][j_st][r] = row[r];
+            if (axis == 0) {
+                // weighted visibilities; zero past the end of the bin, where the
+                // lanes keep (and finally flush) the last real cell
+#pragma unroll
+                for (int gw = 0; gw < W; gw++) {
+                    WD w;
+#pragma unroll
+                    for (int q = 0; q < QW; q++) {
+                        const int pq = gw * QW + q;
+                        w.v[q] = valid ? en.wd[pq / NC][pq % NC] : (ACC)0;
+                    }
+                    s_wd[gw][j_st] = w;
+                }
             }
         }
         const int64_t nleft = end - b0;
         const int nh = nleft >= kBatch ? kBatch : (int)nleft;
-        prev_lu0 = __shfl(lu0, nh - 1, 64);
-        prev_lv0 = __shfl(lv0, nh - 1, 64);
         __syncthreads();
+        if (b0 + kBatch < end) load_batch(b0 + kBatch);
 
+        // groups of kGroup samples, double-buffered: the next group's LDS reads
+        // are issued before this group's flushes, so they do not queue
+        // behind its adds
         const int ng = (nh + kGroup - 1) / kGroup;
-        for (int gi = 0; gi < ng; gi++) {
-            ACC wu[kGroup], wv[kGroup];
+        AxW au0[kGroup], av0[kGroup], au1[kGroup], av1[kGroup];
+        WD wd0[kGroup], wd1[kGroup];
+        auto load_group = [&](int gi, AxW (&au)[kGroup], AxW (&av)[kGroup], WD (&wd)[kGroup]) {
 #pragma unroll
             for (int jj = 0; jj < kGroup; jj++) {
-                wu[jj] = s_wu[gi * kGroup + jj][ra];
-                wv[jj] = s_wv[gi * kGroup + jj][rb];
+                au[jj] = s_ax[0][gi * kGroup + jj][ra];
+                av[jj] = s_ax[1][gi * kGroup + jj][rb];
+                wd[jj] = s_wd[wave][gi * kGroup + jj];
             }
+        };
+        auto run_group = [&](const AxW (&au)[kGroup], const AxW (&av)[kGroup],
+                             const WD (&wd)[kGroup]) {
 #pragma unroll
             for (int jj = 0; jj < kGroup; jj++) {
-                const int j = gi * kGroup + jj;
-                const WD w = s_wd[j];
-                if (signbit(wu[jj]) || signbit(wv[jj])) {
+                if (signbit(au[jj].w) || signbit(av[jj].w)) {
                     flush();
-                    cur_u = (s_cell[j][ra] & 0xffff) - 64;
-                    cur_v = (s_cell[j][rb] >> 16) - 64;
+                    cur_u = au[jj].cell - 64;
+                    cur_v = av[jj].cell - 64;
 #pragma unroll
-                    for (int q = 0; q < NPC; q++) acc[q] = (ACC)0;
+                    for (int q = 0; q < QW; q++) acc[q] = (ACC)0;
                 }
-                const ACC wt = fabs(wu[jj]) * fabs(wv[jj]);
+                const ACC wt = fabs(au[jj].w) * fabs(av[jj].w);
 #pragma unroll
-                for (int q = 0; q < NPC; q++) acc[q] += wt * w.v[q];
+                for (int q = 0; q < QW; q++) acc[q] += wt * wd[jj].v[q];
             }
+        };
+        load_group(0, au0, av0, wd0);
+        for (int gi = 0; gi < ng; gi += 2) {
+            if (gi + 1 < ng) load_group(gi + 1, au1, av1, wd1);
+            run_group(au0, av0, wd0);
+            if (gi + 1 >= ng) break;
+            if (gi + 2 < ng) load_group(gi + 2, au0, av0, wd0);
+            run_group(au1, av1, wd1);
         }
         __syncthreads();
     }
     flush();
     __syncthreads();
 
-    // write the tile: every cell of the grid is written exactly once
-    constexpr int F = NPC * TU * kTile;
+    // write the tile, lanes running along v (each row a contiguous run of the
+    // grid), and clear the subgrid for the next tile: every cell of the grid
+    // is written exactly once
+    constexpr int F = NP * TU * kTile;
+    using OUT = typename std::conditional<NC == 2,
+        typename std::conditional<std::is_same<ACC, float>::value, float2, double2>::type,
+        ACC>::type;
 #pragma unroll 4
-    for (int k = lane; k < F; k += 64) {
-        const int c = k % NC;
-        const int lv = (k / NC) % kTile;
-        const int lu = (k / (NC * kTile)) % TU;
-        const int p = k / (NC * kTile * TU);
+    for (int k = tid; k < F; k += NT) {
+        const int lv = k % kTile;
+        const int lu = (k / kTile) % TU;
+        const int p = k / (kTile * TU);
+        const int o = subgrid_index<(int)sizeof(CELL)>(lu, lv);
+        OUT val;
+        if constexpr (NC == 2) {
+            CELL *re = &s_acc[(2 * p) * PLANE + o];
+            CELL *im = &s_acc[(2 * p + 1) * PLANE + o];
+            val.x = (ACC)*re;
+            val.y = (ACC)*im;
+            *re = CELL{};
+            *im = CELL{};
+        } else {
+            CELL *c = &s_acc[p * PLANE + o];
+            val = (ACC)*c;
+            *c = CELL{};
+        }
         const int64_t gu = u_lo + lu, gv = v_lo + lv;
         if (gu >= a.n_u || gv >= a.n_v) continue;
-        const int o = (p * NC + c) * PLANE + lu * kRow + lv;
-        ACC val;
-        if constexpr (KAHAN)
-            val = s_acc[o].x + s_acc[o].y;
-        else
-            val = s_acc[o];
-        const int64_t gi = (((a_chan * a.n_imag_pol + p) * a.n_u + gu) * a.n_v + gv) * NC + c;
-        ACC *gp = reinterpret_cast<ACC *>(grid_out) + gi;
-        if (accumulate)
-            *gp = *gp + val;
-        else
+        OUT *gp = reinterpret_cast<OUT *>(grid_out) +
+                  ((a_chan * a.n_imag_pol + p) * a.n_u + gu) * a.n_v + gv;
+        if (accumulate) {
+            OUT g0 = *gp;
+            if constexpr (NC == 2) {
+                g0.x += val.x;
+                g0.y += val.y;
+            } else {
+                g0 += val;
+            }
+            *gp = g0;
+        } else {
             *gp = val;
+        }
     }
     }  // work queue
 }
@@ -554,7 +690,6 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
 // largest-first bin order (bins grouped by floor(log2(count)), descending)
 // ---------------------------------------------------------------------------
 constexpr int kScanChunk = 1024;  // elements per scan block (256 threads x 4)
-constexpr int kBuckets = 40;
 
 template <typename T> __global__ void fill_kernel(T *p, int64_t n, T v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -637,12 +772,6 @@ scan_apply_kernel(const unsigned long long *in, int64_t n, const unsigned long l
         if (base + k < n) out[base + k] = e;
         e += v[k];
     }
-}
-
-__device__ inline int bin_bucket(unsigned long long n) {
-    if (n == 0) return 0;
-    const int b = 64 - __clzll((long long)n);
-    return b < kBuckets ? b : kBuckets - 1;
 }
 
 // hist[b] = number of bins in bucket b (hist zeroed beforehand)
@@ -739,6 +868,7 @@ static void norm_table(const double *cgk, int64_t cgk_len, int64_t support, int6
 struct TiledCfg {
     int tu = 16;        // tile rows (8, 16 or 32); tiles are 32 columns wide
     int ch_lanes = 2;   // channels per binning wavefront (1, 2, 4 or 8)
+    int waves = 0;      // wavefronts per tile, splitting the pols (0: default)
 };
 
 struct TiledGeom {
@@ -861,7 +991,7 @@ static int enqueue_count_scan(ngc_context *ctx, hipStream_t s, const StdArgs &a,
 }
 
 // fill pass, bin order and the persistent gridding kernel
-template <int MODE, int NP, typename ACC, bool KAHAN, int TU>
+template <int MODE, int NP, typename ACC, bool KAHAN, int TU, int W>
 static int enqueue_fill_grid(ngc_context *ctx, hipStream_t s, const StdArgs &a,
                              const TiledGeom &tg, const TiledBufs &bf, void *grid,
                              double *sum_weight, int accumulate) {
@@ -889,12 +1019,13 @@ static int enqueue_fill_grid(ngc_context *ctx, hipStream_t s, const StdArgs &a,
     NGC_HIP(hipGetLastError());
     phase_end(ctx, s);
     phase_begin(ctx, s, "std_grid_tiled");
-    auto kern = std_grid_tiled_kernel<MODE, NP, ACC, KAHAN, TU>;
+    auto kern = std_grid_tiled_kernel<MODE, NP, ACC, KAHAN, TU, W>;
     int per_cu = 1;
-    NGC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
+    NGC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, 0));
     const int64_t wgs = std::max<int64_t>(1, std::min<int64_t>(tg.n_bins,
                                                                (int64_t)per_cu * ctx->n_cu));
-    kern<<<(unsigned)wgs, 64, 0, s>>>(a, tg.g, bf.offsets, bf.order, bf.flags, (int)tg.n_bins,
+    kern<<<(unsigned)wgs, 64 * W, 0, s>>>(a, tg.g, bf.offsets, bf.order, bf.hist, bf.flags,
+                                          (int)tg.n_bins,
                                       bf.cap, bf.entries, bf.cgk, (int)bf.cgk_len, grid,
                                       accumulate);
     NGC_HIP(hipGetLastError());
@@ -902,16 +1033,32 @@ static int enqueue_fill_grid(ngc_context *ctx, hipStream_t s, const StdArgs &a,
     return NGC_OK;
 }
 
-// Calls f(mode_c, np_c, acc_c, kahan_c, tu_c) with the compile-time form of
-// (mode, n_pol, precision, tile rows).
+// Wavefronts per tile for n_pol pols: `req` when it divides the pols into
+// equal groups (1, 2 or 4), else the measured best: 2 for 4 pols, 1 otherwise.
+static int tile_waves(int64_t n_pol, int req) {
+    if ((req == 1 || req == 2 || req == 4) && n_pol % req == 0 && n_pol <= 4) return req;
+    return n_pol == 4 ? 2 : 1;
+}
+
+// Calls f(mode_c, np_c, acc_c, kahan_c, tu_c, w_c) with the compile-time form of
+// (mode, n_pol, precision, tile rows, wavefronts per tile).
 template <typename F>
-static int tiled_dispatch(int mode, int64_t n_pol, int precision, int tu, F &&f) {
+static int tiled_dispatch(int mode, int64_t n_pol, int precision, int tu, int waves, F &&f) {
     struct F32 { using type = float; };
     struct F64 { using type = double; };
+    auto by_w = [&](auto m, auto np, auto acc, auto k, auto t) -> int {
+        constexpr int NP = decltype(np)::value;
+        const int w = tile_waves(NP, waves);
+        if constexpr (NP % 4 == 0)
+            if (w == 4) return f(m, np, acc, k, t, std::integral_constant<int, 4>{});
+        if constexpr (NP % 2 == 0)
+            if (w == 2) return f(m, np, acc, k, t, std::integral_constant<int, 2>{});
+        return f(m, np, acc, k, t, std::integral_constant<int, 1>{});
+    };
     auto by_tu = [&](auto m, auto np, auto acc, auto k) -> int {
-        if (tu == 32) return f(m, np, acc, k, std::integral_constant<int, 32>{});
-        if (tu == 8) return f(m, np, acc, k, std::integral_constant<int, 8>{});
-        return f(m, np, acc, k, std::integral_constant<int, 16>{});
+        if (tu == 32) return by_w(m, np, acc, k, std::integral_constant<int, 32>{});
+        if (tu == 8) return by_w(m, np, acc, k, std::integral_constant<int, 8>{});
+        return by_w(m, np, acc, k, std::integral_constant<int, 16>{});
     };
     auto by_prec = [&](auto m, auto np) -> int {
         if (precision == NGC_PREC_FP32_KAHAN) return by_tu(m, np, F32{}, std::true_type{});
@@ -940,13 +1087,14 @@ static int tiled_run_eager(ngc_context *ctx, hipStream_t s, int mode, int precis
                            const StdArgs &a, const TiledGeom &tg, TiledBufs &bf, void *grid,
                            double *sum_weight, int accumulate,
                            unsigned long long *entries_total) {
-    return tiled_dispatch(mode, a.n_pol, precision, tg.cfg.tu,
-                          [&](auto m, auto np, auto acc, auto k, auto tu) -> int {
+    return tiled_dispatch(mode, a.n_pol, precision, tg.cfg.tu, tg.cfg.waves,
+                          [&](auto m, auto np, auto acc, auto k, auto tu, auto w) -> int {
         constexpr int MODE = decltype(m)::value;
         constexpr int NP = decltype(np)::value;
         using ACC = typename decltype(acc)::type;
         constexpr bool KAHAN = decltype(k)::value;
         constexpr int TU = decltype(tu)::value;
+        constexpr int W = decltype(w)::value;
         NGC_TRY((enqueue_count_scan<MODE, NP, ACC>(ctx, s, a, tg, bf, sum_weight, accumulate)));
         unsigned long long total = 0;
         NGC_HIP(hipMemcpyAsync(&total, bf.offsets + tg.n_bins, sizeof(total),
@@ -958,7 +1106,7 @@ static int tiled_run_eager(ngc_context *ctx, hipStream_t s, int mode, int precis
                           &p));
         bf.entries = (char *)p;
         bf.cap = total + 1;
-        return enqueue_fill_grid<MODE, NP, ACC, KAHAN, TU>(ctx, s, a, tg, bf, grid, sum_weight,
+        return enqueue_fill_grid<MODE, NP, ACC, KAHAN, TU, W>(ctx, s, a, tg, bf, grid, sum_weight,
                                                            accumulate);
     });
 }
@@ -1069,6 +1217,7 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     TiledCfg cfg;
     cfg.tu = ctx->tile_rows;
     cfg.ch_lanes = ctx->bin_chan_lanes;
+    cfg.waves = ctx->tile_waves;
     const TiledGeom tg = tiled_geom(a, cfg);
     TiledBufs bf{};
     NGC_TRY(tiled_tables(ctx, s, a, (const double *)d_cgk, cgk, cgk_len, nullptr, bf));
@@ -1209,6 +1358,7 @@ static int plan_create(ngc_context *ctx, hipStream_t s, int mode, int precision,
     TiledCfg best;
     best.tu = ctx->tile_rows;
     best.ch_lanes = ctx->bin_chan_lanes;
+    best.waves = ctx->tile_waves;
     std::string tune_log;
     auto it = ctx->tuned.find(key);
     if (autotune || it == ctx->tuned.end()) {
@@ -1222,6 +1372,7 @@ static int plan_create(ngc_context *ctx, hipStream_t s, int mode, int precision,
                     TiledCfg cfg;
                     cfg.tu = tu;
                     cfg.ch_lanes = cl;
+                    cfg.waves = ctx->tile_waves;
                     const TiledGeom tg = tiled_geom(a, cfg);
                     TiledBufs bf = tables;
                     NGC_TRY(tiled_workspace(ctx, tg, bf));
@@ -1269,13 +1420,14 @@ static int plan_create(ngc_context *ctx, hipStream_t s, int mode, int precision,
     NGC_TRY(plan_alloc(p, 4 * sizeof(int), &q));
     bf.flags = (int *)q;
     p->flags = bf.flags;
-    int rc = tiled_dispatch(mode, a.n_pol, precision, best.tu,
-                            [&](auto m, auto np, auto acc, auto k, auto tu) -> int {
+    int rc = tiled_dispatch(mode, a.n_pol, precision, best.tu, best.waves,
+                            [&](auto m, auto np, auto acc, auto k, auto tu, auto w) -> int {
         constexpr int MODE = decltype(m)::value;
         constexpr int NP = decltype(np)::value;
         using ACC = typename decltype(acc)::type;
         constexpr bool KAHAN = decltype(k)::value;
         constexpr int TU = decltype(tu)::value;
+        constexpr int W = decltype(w)::value;
         NGC_TRY((enqueue_count_scan<MODE, NP, ACC>(ctx, s, a, tg, bf, sum_weight, accumulate)));
         unsigned long long total = 0;
         NGC_HIP(hipMemcpyAsync(&total, bf.offsets + tg.n_bins, sizeof(total),
@@ -1291,7 +1443,7 @@ static int plan_create(ngc_context *ctx, hipStream_t s, int mode, int precision,
         int r = enqueue_count_scan<MODE, NP, ACC>(ctx, p->cap_stream, a, tg, bf, sum_weight,
                                                   accumulate);
         if (r == NGC_OK)
-            r = enqueue_fill_grid<MODE, NP, ACC, KAHAN, TU>(ctx, p->cap_stream, a, tg, bf, grid,
+            r = enqueue_fill_grid<MODE, NP, ACC, KAHAN, TU, W>(ctx, p->cap_stream, a, tg, bf, grid,
                                                             sum_weight, accumulate);
         hipGraph_t graph = nullptr;
         const hipError_t ec = hipStreamEndCapture(p->cap_stream, &graph);
@@ -1300,8 +1452,10 @@ static int plan_create(ngc_context *ctx, hipStream_t s, int mode, int precision,
         NGC_CHECK(ec == hipSuccess && graph, NGC_ERR_HIP, "hipStreamEndCapture failed");
         NGC_HIP(hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0));
         char b[256];
-        snprintf(b, sizeof(b), "tile %dx32, %d channel lanes, %lld bins, %llu records (cap %llu)",
-                 best.tu, best.ch_lanes, (long long)tg.n_bins, total, bf.cap);
+        snprintf(b, sizeof(b),
+                 "tile %dx32, %d wavefronts per tile, %d channel lanes, %lld bins, %llu records "
+                 "(cap %llu)",
+                 best.tu, W, best.ch_lanes, (long long)tg.n_bins, total, bf.cap);
         p->info = b;
         return NGC_OK;
     });

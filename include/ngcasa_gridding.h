@@ -398,6 +398,15 @@ int ngc_adam_step(ngc_context *ctx, void *stream, int64_t n, float *w, const flo
 int ngc_set_tile_rows(ngc_context *ctx, int tile_rows);
 
 /*
+ * Wavefronts per tile of the uv-tiled standard gridder (no counterpart in the
+ * reference): the pols of a tile are split evenly over 1, 2 or 4 wavefronts
+ * of one workgroup, which share the tile's LDS subgrid and staged samples; a
+ * count that does not divide the pols falls back to the default.  0 (the
+ * default) = 2 wavefronts for 4 pols, else 1.
+ */
+int ngc_set_tile_waves(ngc_context *ctx, int waves);
+
+/*
  * Phase timing (instrumentation; no counterpart in the reference, which has
  * no timing inside its kernels).  When enabled, each gridding call records
  * HIP events on its stream around its phases; ngc_profile_read synchronises

@@ -41,9 +41,24 @@ def test_standard_grid_matches_oracle(gpu, precision, mode, rows):
         nat.set_tile_rows(16)
 
 
-def _standard_case(nat, precision, mode):
+@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("n_pol", [1, 2, 4])
+@pytest.mark.parametrize("precision", ["fp64_tiled", "fp32_kahan"])
+def test_standard_grid_wavefronts_per_tile(gpu, precision, n_pol, waves):
+    """The pols of a tile split over 1, 2 or 4 wavefronts (a count that does
+    not divide the pols falls back to the default) give the oracle's grid."""
+    nat = _native()
+    nat.set_tile_waves(waves)
+    try:
+        _standard_case(nat, precision, "image", n_pol=n_pol)
+        _standard_case(nat, precision, "psf", n_pol=n_pol)
+    finally:
+        nat.set_tile_waves(0)
+
+
+def _standard_case(nat, precision, mode, n_pol=2):
     import torch
-    c = small_case(n_time=24, n_chan=3, n_pol=2, nan_frac=0.05)
+    c = small_case(n_time=24, n_chan=3, n_pol=n_pol, nan_frac=0.05)
     cgk = oracle.ps_kernel_1d(100, 7)
     npad = c["npad"]
     g_ref, sw_ref = oracle.standard_grid(c["vis"], c["uvw"], c["weight"], c["freq"], cgk,
@@ -54,7 +69,7 @@ def _standard_case(nat, precision, mode):
     vis = _dev(c["vis"], vis_dt) if mode == "image" else None
     n_chan = c["weight"].shape[2]
     g, sw = nat.standard_grid(_dev(c["uvw"]), _dev(c["weight"], w_dt), c["freq"],
-                              np.arange(n_chan), np.arange(2), cgk, n_chan, [npad, npad],
+                              np.arange(n_chan), np.arange(n_pol), cgk, n_chan, [npad, npad],
                               c["delta"], 7, 100,
                               nat.GRID_IMAGE if mode == "image" else nat.GRID_PSF, precision,
                               vis=vis)
