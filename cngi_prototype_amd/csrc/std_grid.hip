@@ -203,22 +203,17 @@ __device__ inline bool sample_at(const StdArgs &a, const BinGeom &g, int64_t wav
     return true;
 }
 
-// centre cell, offsets and on-grid test of a sample; false when it is not gridded
-__device__ inline bool place(const StdArgs &a, const Sample &sm, Pix &px) {
-    const int64_t row = sm.t * a.n_baseline + sm.b;
-    if (!pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[sm.ch],
-                        a.uv_scale[a.n_chan + sm.ch], a.n_u, a.n_v, a.os, px))
+// centre cell, offsets and on-grid test of a sample at (u, v) metres, with
+// `any_unflagged` from its flags; false when it is not gridded
+__device__ inline bool place(const StdArgs &a, const Sample &sm, double u, double v,
+                             bool any_unflagged, Pix &px) {
+    if (!any_unflagged) return false;
+    if (!pixel_position(u, v, a.uv_scale[sm.ch], a.uv_scale[a.n_chan + sm.ch], a.n_u, a.n_v,
+                        a.os, px))
         return false;
     const int64_t sc = a.support / 2;
-    if (!((px.uc + sc < a.n_u) && (px.vc + sc < a.n_v) && (px.uc - sc >= 0) &&
-          (px.vc - sc >= 0)))
-        return false;
-    if (a.flag) {
-        bool any = false;
-        for (int p = 0; p < a.n_pol; p++) any |= (a.flag[sm.rc * a.n_pol + p] == 0);
-        if (!any) return false;
-    }
-    return true;
+    return (px.uc + sc < a.n_u) && (px.vc + sc < a.n_v) && (px.uc - sc >= 0) &&
+           (px.vc - sc >= 0);
 }
 
 // k-th overlapped tile (k = 0..3) of a placed sample, or -1
@@ -271,7 +266,59 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
         const int64_t wave_id = w0 + (threadIdx.x >> 6);
         Sample sm{};
         Pix px{};
-        const bool ok = (wave_id < n_waves) && sample_at(a, g, wave_id, lane, sm) && place(a, sm, px);
+        const bool have = (wave_id < n_waves) && sample_at(a, g, wave_id, lane, sm);
+        // all of the sample's loads are issued before the placement arithmetic
+        const int64_t base = sm.rc * NP;
+        double uu = 0.0, vv = 0.0;
+        bool fl[NP];
+        double wts[NP];
+        double2 vis[NP];
+        bool any_unflagged = false;
+        if (have) {
+            const int64_t row = sm.t * a.n_baseline + sm.b;
+            uu = a.uvw[row * 3];
+            vv = a.uvw[row * 3 + 1];
+            // 4 pols: one word of flags, one 16-byte load of FP32 weights and two
+            // of complex64 visibilities when the arrays are aligned for it
+            if (NP == 4 && a.flag && ((uintptr_t)a.flag & 3) == 0) {
+                const uint32_t f4 = *reinterpret_cast<const uint32_t *>(a.flag + base);
+#pragma unroll
+                for (int p = 0; p < NP; p++) fl[p] = ((f4 >> (8 * p)) & 0xffu) != 0;
+            } else {
+#pragma unroll
+                for (int p = 0; p < NP; p++) fl[p] = a.flag ? a.flag[base + p] != 0 : false;
+            }
+#pragma unroll
+            for (int p = 0; p < NP; p++) any_unflagged |= !fl[p];
+            if (FILL) {
+                if (NP == 4 && a.weight_dtype == NGC_F32 && ((uintptr_t)a.weight & 15) == 0) {
+                    const float4 w4 = *reinterpret_cast<const float4 *>(
+                        reinterpret_cast<const float *>(a.weight) + base);
+                    wts[0] = w4.x;
+                    wts[NP > 1 ? 1 : 0] = w4.y;
+                    wts[NP > 2 ? 2 : 0] = w4.z;
+                    wts[NP > 3 ? 3 : 0] = w4.w;
+                } else {
+#pragma unroll
+                    for (int p = 0; p < NP; p++) wts[p] = load_w(a.weight, base + p, a.weight_dtype);
+                }
+                if (MODE == NGC_GRID_IMAGE) {
+                    if (NP == 4 && a.vis_dtype == NGC_C64 && ((uintptr_t)a.vis & 15) == 0) {
+                        const float4 *vp = reinterpret_cast<const float4 *>(
+                            reinterpret_cast<const float2 *>(a.vis) + base);
+                        const float4 v01 = vp[0], v23 = vp[1];
+                        vis[0] = make_double2(v01.x, v01.y);
+                        vis[NP > 1 ? 1 : 0] = make_double2(v01.z, v01.w);
+                        vis[NP > 2 ? 2 : 0] = make_double2(v23.x, v23.y);
+                        vis[NP > 3 ? 3 : 0] = make_double2(v23.z, v23.w);
+                    } else {
+#pragma unroll
+                        for (int p = 0; p < NP; p++) vis[p] = load_vis(a.vis, base + p, a.vis_dtype);
+                    }
+                }
+            }
+        }
+        const bool ok = have && place(a, sm, uu, vv, any_unflagged, px);
         const int64_t a_chan = ok ? a.chan_map[sm.ch] : 0;
         E rec;
         if (FILL && ok) {
@@ -280,15 +327,14 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
             const int ndu = min(max((int)px.du + norm_half, 0), nw - 1);
             const int ndv = min(max((int)px.dv + norm_half, 0), nw - 1);
             const double norm = norm_tab[ndu * nw + ndv];
-            const int64_t base = sm.rc * NP;
 #pragma unroll
             for (int p = 0; p < NP; p++) {
-                const double wv = weight_of(a, base, p);
+                // the weight is NaN when flagged
+                const double wv = fl[p] ? __longlong_as_double(0x7ff8000000000000LL) : wts[p];
                 double wr = wv, wi = 0.0;
                 if (MODE == NGC_GRID_IMAGE) {
-                    const double2 v = load_vis(a.vis, base + p, a.vis_dtype);
-                    wr = v.x * wv;
-                    wi = v.y * wv;
+                    wr = vis[p].x * wv;
+                    wi = vis[p].y * wv;
                 }
                 const bool use = !(isnan(wr) || isnan(wi)) && !(wr == 0.0 && wi == 0.0);
                 rec.wd[p][0] = use ? (ACC)wr : (ACC)0;
@@ -302,28 +348,48 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
                 }
             }
         }
+        // group the lanes by tile for each of the (up to 4) tiles a sample
+        // overlaps: the group's lowest lane leads, every lane knows its rank
+        int64_t key[4];
+        int leader[4], rank[4], cnt[4];
+#pragma unroll
         for (int k = 0; k < 4; k++) {
-            int64_t key = ok ? tile_key(a, g, px, a_chan, k) : -1;
-            uint64_t pending = __ballot(key >= 0);
+            key[k] = ok ? tile_key(a, g, px, a_chan, k) : -1;
+            leader[k] = lane;
+            rank[k] = 0;
+            cnt[k] = 0;
+            uint64_t pending = __ballot(key[k] >= 0);
             while (pending) {
-                const int leader = __ffsll((long long)pending) - 1;
-                const int64_t lkey = __shfl(key, leader, 64);
-                const uint64_t same = __ballot(key == lkey);
-                unsigned long long basev = 0;
-                if (lane == leader)
-                    basev = atomicAdd(&counts[lkey], (unsigned long long)__popcll(same));
-                if (FILL) {
-                    basev = __shfl(basev, leader, 64);
-                    if (key == lkey) {
-                        const unsigned long long pos = basev + __popcll(same & below);
-                        if (pos < cap)
-                            *reinterpret_cast<E *>(entries + pos * EB) = rec;
-                        else
-                            atomicOr(overflow, 1);
-                    }
+                const int l = __ffsll((long long)pending) - 1;
+                const int64_t lkey = __shfl(key[k], l, 64);
+                const uint64_t same = __ballot(key[k] == lkey);
+                if (key[k] == lkey) {
+                    leader[k] = l;
+                    rank[k] = __popcll(same & below);
+                    cnt[k] = __popcll(same);
                 }
-                if (key == lkey) key = -1;
                 pending &= ~same;
+            }
+        }
+        // one counter atomic per group, all issued before any result is used
+        unsigned long long basev[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            basev[k] = 0;
+            if (key[k] >= 0 && rank[k] == 0)
+                basev[k] = atomicAdd(&counts[key[k]], (unsigned long long)cnt[k]);
+        }
+        if (FILL) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const unsigned long long b0 = __shfl(basev[k], leader[k], 64);
+                if (key[k] >= 0) {
+                    const unsigned long long pos = b0 + (unsigned long long)rank[k];
+                    if (pos < cap)
+                        *reinterpret_cast<E *>(entries + pos * EB) = rec;
+                    else
+                        atomicOr(overflow, 1);
+                }
             }
         }
     }
