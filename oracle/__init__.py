@@ -340,6 +340,57 @@ def grid_to_image(grid_kernel_orient, image_size):
     return remove_padding(img, image_size).real * (npad[0] * npad[1])
 
 
+def model_to_grid(model, n_uv):
+    """Model image -> uv grid for prediction, the inverse placement of
+    grid_to_image: pad the image [N_l, N_m, n_chan, n_pol] (Jy/pixel),
+    divided by the cropped correction image, into the centre of the n_uv
+    grid, then fftshift(fft2(ifftshift(.))) (unnormalised forward transform,
+    so a pixel of flux S gives |G| = S).  Returns [n_chan, n_pol, n_u, n_v]
+    complex in the gridder's orientation.  No reference counterpart:
+    predict_modelvis_image.py:20 is a stub."""
+    model = np.asarray(model, dtype=np.float64)
+    n_u, n_v = int(n_uv[0]), int(n_uv[1])
+    size = np.array(model.shape[0:2])
+    corr = remove_padding(ps_correction_image(n_uv), size)
+    pad = np.zeros((n_u, n_v) + model.shape[2:], dtype=np.float64)
+    start = np.array([n_u, n_v]) // 2 - size // 2
+    pad[start[0]:start[0] + size[0], start[1]:start[1] + size[1]] = \
+        model / corr[:, :, None, None]
+    g = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(pad, axes=(0, 1)), axes=(0, 1)),
+                        axes=(0, 1))
+    return np.ascontiguousarray(np.moveaxis(g, [-2, -1], [0, 1]))
+
+
+def standard_degrid(grid, uvw, freq_chan, cgk_1d, n_uv, delta_lm, support, oversampling,
+                    chan_mode="cube", normalize=True, n_pol=None):
+    """Visibility prediction by degridding: the adjoint of standard_grid's
+    loop with unit weights (same cells, offsets, bounds and taps), divided by
+    the tap sum when normalize.  grid [n_imag_chan, n_pol, n_u, n_v] complex;
+    returns vis [T, B, C, P] complex128 (0 for samples off the grid)."""
+    grid = np.ascontiguousarray(grid, dtype=np.complex128)
+    uvw = _f64(uvw)
+    freq_chan = _f64(freq_chan)
+    n_time, n_baseline = uvw.shape[0], uvw.shape[1]
+    n_chan = len(freq_chan)
+    n_imag_chan, n_imag_pol = grid.shape[0], grid.shape[1]
+    n_pol = n_imag_pol if n_pol is None else int(n_pol)
+    chan_map = (np.arange(n_chan, dtype=np.int64) if chan_mode == "cube"
+                else np.zeros(n_chan, dtype=np.int64))
+    pol_map = np.arange(n_pol, dtype=np.int64)
+    vis = np.zeros((n_time, n_baseline, n_chan, n_pol), dtype=np.complex128)
+    L = _lib()
+    L.oracle_standard_degrid.restype = None
+    L.oracle_standard_degrid(
+        _p(vis.view(np.float64)), _p(grid.view(np.float64)), _p(uvw), _p(freq_chan),
+        _p(chan_map), _p(pol_map), _p(_f64(cgk_1d)), ctypes.c_int64(n_time),
+        ctypes.c_int64(n_baseline), ctypes.c_int64(n_chan), ctypes.c_int64(n_pol),
+        ctypes.c_int64(n_imag_chan), ctypes.c_int64(n_imag_pol), ctypes.c_int64(int(n_uv[0])),
+        ctypes.c_int64(int(n_uv[1])), ctypes.c_double(float(delta_lm[0])),
+        ctypes.c_double(float(delta_lm[1])), ctypes.c_int64(int(support)),
+        ctypes.c_int64(int(oversampling)), ctypes.c_int(int(bool(normalize))))
+    return vis
+
+
 def correct_standard_image(uncorrected, sum_weight, n_uv, image_size):
     """correct_image of make_psf.py:124-131 / make_image.py:123-130."""
     corr = remove_padding(ps_correction_image(n_uv), image_size)

@@ -176,6 +176,99 @@ void oracle_standard_grid(double *grid, double *sum_weight, int do_psf,
 }
 
 /* ----------------------------------------------------------------------- */
+/* Visibility degridding (prediction): the gather counterpart of the loop    */
+/* above.  The reference has no implementation (_standard_grid.py:429-430    */
+/* prints that it is missing; predict_modelvis_image.py:20 is a stub), so    */
+/* this restatement is defined as the adjoint of oracle_standard_grid with   */
+/* unit weights: the same uv_scale, centre cell, sub-cell offset, bounds and */
+/* tap index, gathering instead of scattering.                               */
+/* ----------------------------------------------------------------------- */
+/*
+ * vis   : out [n_time, n_baseline, n_chan, n_pol] complex (interleaved);
+ *         V = sum_taps conv * G[cell] (divided by sum_taps conv when
+ *         normalize), 0 for samples that are not placed on the grid
+ * grid  : [n_imag_chan, n_imag_pol, n_u, n_v] complex (interleaved)
+ */
+void oracle_standard_degrid(double *vis, const double *grid, const double *uvw,
+                            const double *freq_chan, const int64_t *chan_map,
+                            const int64_t *pol_map, const double *cgk_1d,
+                            int64_t n_time, int64_t n_baseline, int64_t n_chan,
+                            int64_t n_pol, int64_t n_imag_chan, int64_t n_imag_pol,
+                            int64_t n_u, int64_t n_v, double delta_l, double delta_m,
+                            int64_t support, int64_t oversampling, int normalize)
+{
+    (void)n_imag_chan;
+    double *uv_scale0 = (double *)malloc(sizeof(double) * (size_t)n_chan);
+    double *uv_scale1 = (double *)malloc(sizeof(double) * (size_t)n_chan);
+    for (int64_t c = 0; c < n_chan; c++) {
+        uv_scale0[c] = -((freq_chan[c] * delta_l) * (double)n_u) / C_LIGHT;
+        uv_scale1[c] = -((freq_chan[c] * delta_m) * (double)n_v) / C_LIGHT;
+    }
+    const int64_t support_center = support / 2;
+    const int64_t uv_center0 = n_u / 2, uv_center1 = n_v / 2;
+    const int64_t start_support = -support_center;
+    const int64_t end_support = support - support_center;
+    const int64_t plane = n_u * n_v;
+    for (int64_t t = 0; t < n_time; t++) {
+        for (int64_t b = 0; b < n_baseline; b++) {
+            for (int64_t ch = 0; ch < n_chan; ch++) {
+                const int64_t vis_base = ((t * n_baseline + b) * n_chan + ch) * n_pol;
+                for (int64_t p = 0; p < n_pol; p++) {
+                    vis[2 * (vis_base + p)] = 0.0;
+                    vis[2 * (vis_base + p) + 1] = 0.0;
+                }
+                const int64_t a_chan = chan_map[ch];
+                const double *uvw_tb = uvw + (t * n_baseline + b) * 3;
+                const double u = uvw_tb[0] * uv_scale0[ch];
+                const double v = uvw_tb[1] * uv_scale1[ch];
+                if (is_nan(u) || is_nan(v))
+                    continue;
+                const double u_pos = u + (double)uv_center0;
+                const double v_pos = v + (double)uv_center1;
+                const int64_t u_center_indx = (int64_t)(u_pos + 0.5);
+                const int64_t v_center_indx = (int64_t)(v_pos + 0.5);
+                if (!((u_center_indx + support_center < n_u) &&
+                      (v_center_indx + support_center < n_v) &&
+                      (u_center_indx - support_center >= 0) &&
+                      (v_center_indx - support_center >= 0)))
+                    continue;
+                const int64_t u_center_offset_indx = (int64_t)floor(
+                    ((double)u_center_indx - u_pos) * (double)oversampling + 0.5);
+                const int64_t v_center_offset_indx = (int64_t)floor(
+                    ((double)v_center_indx - v_pos) * (double)oversampling + 0.5);
+                for (int64_t p = 0; p < n_pol; p++) {
+                    const int64_t pbase = (a_chan * n_imag_pol + pol_map[p]) * plane;
+                    double re = 0.0, im = 0.0, norm = 0.0;
+                    for (int64_t iv = start_support; iv < end_support; iv++) {
+                        const int64_t v_indx = v_center_indx + iv;
+                        const double conv_v =
+                            cgk_1d[llabs(oversampling * iv + v_center_offset_indx)];
+                        for (int64_t iu = start_support; iu < end_support; iu++) {
+                            const int64_t u_indx = u_center_indx + iu;
+                            const double conv_u =
+                                cgk_1d[llabs(oversampling * iu + u_center_offset_indx)];
+                            const double conv = conv_u * conv_v;
+                            const int64_t cell = pbase + u_indx * n_v + v_indx;
+                            re = re + conv * grid[2 * cell];
+                            im = im + conv * grid[2 * cell + 1];
+                            norm = norm + conv;
+                        }
+                    }
+                    if (normalize && norm != 0.0) {
+                        re = re / norm;
+                        im = im / norm;
+                    }
+                    vis[2 * (vis_base + p)] = re;
+                    vis[2 * (vis_base + p) + 1] = im;
+                }
+            }
+        }
+    }
+    free(uv_scale0);
+    free(uv_scale1);
+}
+
+/* ----------------------------------------------------------------------- */
 /* _standard_imaging_weight_degrid_jit (_standard_grid.py:467-518)          */
 /* ----------------------------------------------------------------------- */
 /*

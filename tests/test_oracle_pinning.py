@@ -706,3 +706,84 @@ def test_rotation_onto_a_point_source_removes_its_phase():
                                      False, False)
     assert np.abs(np.angle(out)).max() < 1e-9
     assert np.allclose(np.abs(out), 1.0)
+
+
+# --------------------------------------------------------------------------
+# prediction (degridding): no reference implementation exists
+# (_standard_grid.py:429-430, predict_modelvis_image.py:20), so the oracle is
+# pinned by mathematics: adjointness with the pinned gridder, closed forms,
+# and the visibilities of a point source
+# --------------------------------------------------------------------------
+def test_degrid_is_the_adjoint_of_grid():
+    """<degrid(G), V> = <G, grid(V)> with unit weights: the same cells, offsets
+    and taps in both directions (an index, sign or offset slip breaks it)."""
+    rng = np.random.default_rng(21)
+    vis, uvw, w, freq, delta = _tiny_case(9, T=12, fill=0.9)
+    n_chan, n_pol = w.shape[2], w.shape[3]
+    V = rng.normal(size=vis.shape) + 1j * rng.normal(size=vis.shape)
+    cgk = oracle.ps_kernel_1d(100, 7)
+    n_uv = [48, 48]
+    G = rng.normal(size=(n_chan, n_pol) + tuple(n_uv)) + 1j * rng.normal(
+        size=(n_chan, n_pol) + tuple(n_uv))
+    gv, _ = oracle.standard_grid(V, uvw, np.ones(w.shape), freq, cgk, n_uv, delta, 7, 100,
+                                 do_psf=False)
+    dg = oracle.standard_degrid(G, uvw, freq, cgk, n_uv, delta, 7, 100, normalize=False)
+    lhs = np.vdot(dg, V)
+    rhs = np.vdot(G, gv)
+    assert abs(lhs - rhs) < 1e-11 * abs(rhs)
+    assert np.count_nonzero(dg) > 0
+
+
+def test_degrid_of_a_constant_grid_is_one():
+    """Normalised by the tap sum, a constant grid predicts the constant for
+    every sample on the grid, and 0 for the samples off it."""
+    vis, uvw, w, freq, delta = _tiny_case(4, T=10, fill=1.3)
+    cgk = oracle.ps_kernel_1d(100, 7)
+    n_uv = [48, 48]
+    G = np.full((w.shape[2], w.shape[3]) + tuple(n_uv), 2.5 - 1.0j)
+    dg = oracle.standard_degrid(G, uvw, freq, cgk, n_uv, delta, 7, 100)
+    on = dg != 0
+    assert on.any() and (~on).any()
+    assert np.allclose(dg[on], 2.5 - 1.0j, rtol=1e-13)
+    _, sw = oracle.standard_grid(None, uvw, np.ones(w.shape), freq, cgk, n_uv, delta, 7, 100,
+                                 do_psf=True)
+    # the samples the gridder places are exactly the ones predicted
+    assert np.count_nonzero(on) == np.count_nonzero(
+        oracle.standard_degrid(np.ones_like(G), uvw, freq, cgk, n_uv, delta, 7, 100))
+
+
+def test_model_to_grid_single_pixel_closed_form():
+    """A pixel of flux S at (l0, m0) of an N x N model gives
+    G[u][v] = S / corr(l0, m0) exp(-2 pi i ((u - n_u//2)(l0 - N//2)/n_u +
+    (v - n_v//2)(m0 - N//2)/n_v)), the direct Fourier sum."""
+    N, n_uv = 20, [26, 25]
+    l0, m0, S = 13, 4, 2.75
+    model = np.zeros((N, N, 1, 1))
+    model[l0, m0] = S
+    G = oracle.model_to_grid(model, n_uv)
+    corr = oracle.remove_padding(oracle.ps_correction_image(n_uv), [N, N])
+    u = np.arange(n_uv[0])[:, None] - n_uv[0] // 2
+    v = np.arange(n_uv[1])[None, :] - n_uv[1] // 2
+    ref = S / corr[l0, m0] * np.exp(-2j * np.pi * (u * (l0 - N // 2) / n_uv[0] +
+                                                   v * (m0 - N // 2) / n_uv[1]))
+    assert np.allclose(G[0, 0], ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("lc,mc,tol", [(0, 0, 1e-6), (5.0, -9.0, 1e-2)])
+def test_point_source_prediction_matches_its_visibilities(lc, mc, tol):
+    """A 1 Jy model pixel where make_image puts a point source predicts that
+    source's visibilities exp(2 pi i (u l + v m)): exactly at the phase centre
+    (up to grdsf(0)), and within the spheroidal kernel's aliasing error away
+    from it (a sign or shift error would give O(1) differences)."""
+    vis, uvw, w, freq, delta, n, npad = _point_source_case(lc, mc)
+    cgk = oracle.ps_kernel_1d(100, 7)
+    g, sw = oracle.standard_grid(vis, uvw, w, freq, cgk, [npad, npad], delta, 7, 100,
+                                 do_psf=False)
+    img = oracle.correct_standard_image(oracle.grid_to_image(g, [n, n]), sw, [npad, npad], [n, n])
+    pk = np.unravel_index(np.argmax(img[:, :, 0, 0]), (n, n))
+    model = np.zeros((n, n, 1, 1))
+    model[pk] = 1.0
+    pv = oracle.standard_degrid(oracle.model_to_grid(model, [npad, npad]), uvw, freq, cgk,
+                                [npad, npad], delta, 7, 100)
+    assert np.all(pv != 0)
+    assert np.abs(pv - vis).max() < tol
