@@ -33,7 +33,8 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_grid_plan_create", "ngc_grid_plan_execute", "ngc_grid_plan_status",
            "ngc_grid_plan_describe", "ngc_grid_plan_destroy", "ngc_bf16_gemm",
            "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step",
-           "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags", "ngc_set_tile_waves")
+           "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags", "ngc_set_tile_waves",
+           "ngc_model_to_grid", "ngc_standard_degrid")
 
 
 class NativeError(RuntimeError):
@@ -76,6 +77,11 @@ _PROTOS = {
                               _vp]),
     "ngc_phase_gradient": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _vp]),
     "ngc_apply_flags": (_i32, [_vp, _vp, _vp, _i32, _vp, _i64, _vp]),
+    "ngc_model_to_grid": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _vp,
+                                 _i32]),
+    "ngc_standard_degrid": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64,
+                                   _i64, _i64, _i64, _i64, _i64, _d, _d, _i64, _i64, _vp, _i32,
+                                   _vp, _i32, _i32]),
     "ngc_make_pb": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i32, _i32, _i64, _i64, _i64,
                            _i64, _d, _d, _i64, _vp]),
     "ngc_profile_read": (_i32, [_vp, ctypes.c_char_p, _i64, _vp, _i64]),
@@ -570,6 +576,65 @@ def apply_flags(data, flag, out=None):
     rc = load_library().ngc_apply_flags(context().handle, _stream(), _ptr(data),
                                         codes[data.dtype], _ptr(flag), data.numel(), _ptr(out))
     _check("ngc_apply_flags", rc)
+    return out
+
+
+def model_to_grid(image, n_uv, corr_l=None, corr_m=None, single=True, out=None):
+    """ngc_model_to_grid: image device float64 [n_chan, n_pol, n_l, n_m] (Jy/pixel)
+    -> grid [n_chan, n_pol, n_u, n_v] complex64 (single) or complex128."""
+    import torch
+    _dev(image, "image")
+    if image.dtype != torch.float64 or image.dim() != 4 or not image.is_contiguous():
+        raise TypeError("image must be a contiguous 4-D float64 tensor")
+    nc, npol, n_l, n_m = image.shape
+    n_u, n_v = int(n_uv[0]), int(n_uv[1])
+    dt = torch.complex64 if single else torch.complex128
+    if out is None:
+        out = torch.empty((nc, npol, n_u, n_v), dtype=dt, device=image.device)
+    if out.dtype != dt or tuple(out.shape) != (nc, npol, n_u, n_v) or not out.is_contiguous():
+        raise ValueError("out must be a contiguous grid of the requested precision")
+    cl = cm = None
+    if corr_l is not None:
+        cl, clp = _host(np.asarray(corr_l).ravel(), np.float64)
+        cm, cmp_ = _host(np.asarray(corr_m).ravel(), np.float64)
+        if len(cl) != n_l or len(cm) != n_m:
+            raise ValueError("correction vectors must match the image size")
+    rc = load_library().ngc_model_to_grid(
+        context().handle, _stream(), _ptr(image), nc * npol, n_l, n_m, n_u, n_v,
+        clp if cl is not None else None, cmp_ if cm is not None else None, _ptr(out),
+        C64 if single else C128)
+    _check("ngc_model_to_grid", rc)
+    return out
+
+
+def standard_degrid(uvw, grid, freq_chan, chan_map, pol_map, cgk_1d, delta_lm, support,
+                    oversampling, n_pol=None, normalize=True, single=True, out=None):
+    """ngc_standard_degrid: predicted visibilities [T, B, C, P] (complex64 when
+    single, else complex128) from grid [n_imag_chan, n_imag_pol, n_u, n_v]."""
+    import torch
+    _dev(uvw, "uvw")
+    _dev(grid, "grid")
+    if grid.dtype not in (torch.complex64, torch.complex128) or not grid.is_contiguous():
+        raise TypeError("grid must be a contiguous complex64/complex128 tensor")
+    T, B = uvw.shape[0], uvw.shape[1]
+    fr, frp = _host(np.asarray(freq_chan).ravel(), np.float64)
+    cm, cmp_ = _host(np.asarray(chan_map).ravel(), np.int32)
+    pm, pmp = _host(np.asarray(pol_map).ravel(), np.int32)
+    cg, cgp = _host(np.asarray(cgk_1d).ravel(), np.float64)
+    C = len(fr)
+    P = len(pm) if n_pol is None else int(n_pol)
+    nic, nip, n_u, n_v = grid.shape
+    dt = torch.complex64 if single else torch.complex128
+    if out is None:
+        out = torch.empty((T, B, C, P), dtype=dt, device=uvw.device)
+    if out.dtype != dt or tuple(out.shape) != (T, B, C, P) or not out.is_contiguous():
+        raise ValueError("out must be a contiguous [T, B, C, P] tensor of the requested dtype")
+    rc = load_library().ngc_standard_degrid(
+        context().handle, _stream(), _ptr(uvw), frp, cmp_, pmp, cgp, len(cg), T, B, C, P, nic,
+        nip, n_u, n_v, float(delta_lm[0]), float(delta_lm[1]), int(support), int(oversampling),
+        _ptr(grid), C64 if grid.dtype == torch.complex64 else C128, _ptr(out),
+        C64 if single else C128, int(bool(normalize)))
+    _check("ngc_standard_degrid", rc)
     return out
 
 

@@ -1,0 +1,289 @@
+// Visibility prediction (the major-cycle inverse of the gridding path).
+//
+// The reference has no implementation: its degridding graph prints that
+// "Degridding of visibilities and psf still needs to be implemented"
+// (ngcasa/imaging/_imaging_utils/_standard_grid.py:429-430) and
+// predict_modelvis_image (ngcasa/imaging/predict_modelvis_image.py:20) is a
+// stub whose docstring names the steps: model image -> gridding correction ->
+// Fourier transform -> degrid.  Here:
+//
+//   ngc_model_to_grid     model image [planes, n_l, n_m] (Jy/pixel) divided by
+//                         the cropped correction image, padded to the centre of
+//                         the n_u x n_v grid, fftshift(fft2(ifftshift(.))) with
+//                         rocFFT (unnormalised forward): the inverse placement
+//                         of ngc_grid_to_image
+//   ngc_standard_degrid   V = sum_taps conv * G[cell] (/ sum_taps conv), the
+//                         adjoint of the standard gridder's loop with unit
+//                         weights: same uv scale, centre cell, sub-cell offset,
+//                         bounds and tap index (_standard_grid.py:273-360)
+//
+// The degrid kernel is a gather: one thread per (baseline, channel, block of
+// kDegridT integrations), channels fastest, so a wavefront writes each
+// integration's visibilities as one contiguous run and each thread's
+// footprint moves slowly from one integration to the next (its grid lines
+// stay in cache).  Taps and sums are FP64; the grid may be complex64.
+#include "ngc_internal.hpp"
+
+namespace ngc {
+
+constexpr int kDegridT = 16;  // integrations per thread
+
+// buf[pl][ku][kv] = X[l][m] / (corr_l[l] corr_m[m]) for the model pixel
+// (l, m) = ((ku + n_l//2) mod n_u, (kv + n_m//2) mod n_v) when inside the
+// model, else 0 (the padded image, ifftshift-ed)
+template <typename T>
+__global__ void model_pad_kernel(const double *image, int64_t n_planes, int64_t n_l,
+                                 int64_t n_m, int64_t n_u, int64_t n_v, const double *corr,
+                                 T *buf) {
+#pragma clang fp contract(off)
+    const int64_t plane = n_u * n_v;
+    const int64_t n = n_planes * plane;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t pl = i / plane;
+        const int64_t r = i - pl * plane;
+        const int64_t ku = r / n_v, kv = r - ku * n_v;
+        int64_t l = ku + n_l / 2;
+        if (l >= n_u) l -= n_u;
+        int64_t m = kv + n_m / 2;
+        if (m >= n_v) m -= n_v;
+        double x = 0.0;
+        if (l < n_l && m < n_m) {
+            x = image[(pl * n_l + l) * n_m + m];
+            if (corr) x = x / (corr[l] * corr[n_l + m]);
+        }
+        buf[2 * i] = (T)x;
+        buf[2 * i + 1] = (T)0;
+    }
+}
+
+// grid[pl][u][v] = buf[pl][(u - n_u//2) mod n_u][(v - n_v//2) mod n_v]  (fftshift)
+template <typename T, typename G>
+__global__ void fftshift_complex_kernel(const T *buf, int64_t n_planes, int64_t n_u,
+                                        int64_t n_v, G *grid) {
+    const int64_t plane = n_u * n_v;
+    const int64_t n = n_planes * plane;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t pl = i / plane;
+        const int64_t r = i - pl * plane;
+        const int64_t u = r / n_v, v = r - u * n_v;
+        int64_t ku = u - n_u / 2;
+        if (ku < 0) ku += n_u;
+        int64_t kv = v - n_v / 2;
+        if (kv < 0) kv += n_v;
+        const int64_t src = pl * plane + ku * n_v + kv;
+        grid[2 * i] = (G)buf[2 * src];
+        grid[2 * i + 1] = (G)buf[2 * src + 1];
+    }
+}
+
+struct DegridStdArgs {
+    const double *uvw;
+    const double *uv_scale;  // [2][n_chan]
+    const int *chan_map, *pol_map;
+    const double *cgk;
+    int64_t cgk_len;
+    int64_t n_time, n_baseline, n_chan, n_pol, n_imag_pol, n_u, n_v, support, os;
+    int normalize;
+};
+
+template <typename G, typename V>
+__global__ void __launch_bounds__(256) standard_degrid_kernel(DegridStdArgs a, const G *grid,
+                                                              V *vis) {
+#pragma clang fp contract(off)
+    __shared__ double s_k[512];
+    for (int i = threadIdx.x; i < 512; i += blockDim.x)
+        s_k[i] = i < a.cgk_len ? a.cgk[i] : 0.0;
+    __syncthreads();
+    const int64_t n_tb = (a.n_time + kDegridT - 1) / kDegridT;
+    const int64_t n_thr = a.n_baseline * n_tb * a.n_chan;
+    const int64_t sc = a.support / 2;
+    const int64_t plane = a.n_u * a.n_v;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_thr;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ch = i % a.n_chan;
+        const int64_t r = i / a.n_chan;
+        const int64_t tb = r % n_tb, b = r / n_tb;
+        const int64_t a_chan = a.chan_map[ch];
+        const double s0 = a.uv_scale[ch], s1 = a.uv_scale[a.n_chan + ch];
+        const int64_t t_end = min((tb + 1) * kDegridT, a.n_time);
+        for (int64_t t = tb * kDegridT; t < t_end; t++) {
+            const int64_t row = t * a.n_baseline + b;
+            V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol);
+            Pix px;
+            bool on = pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], s0, s1, a.n_u, a.n_v,
+                                     a.os, px);
+            on = on && (px.uc + sc < a.n_u) && (px.vc + sc < a.n_v) && (px.uc - sc >= 0) &&
+                 (px.vc - sc >= 0);
+            for (int64_t p = 0; p < a.n_pol; p++) {
+                double re = 0.0, im = 0.0, norm = 0.0;
+                if (on) {
+                    const G *g = grid + 2 * (a_chan * a.n_imag_pol + a.pol_map[p]) * plane;
+                    for (int64_t iv = -sc; iv < a.support - sc; iv++) {
+                        const int64_t kv = a.os * iv + px.dv;
+                        const double cv = s_k[min(kv < 0 ? -kv : kv, (int64_t)511)];
+                        const int64_t v_indx = px.vc + iv;
+                        for (int64_t iu = -sc; iu < a.support - sc; iu++) {
+                            const int64_t ku = a.os * iu + px.du;
+                            const double c = s_k[min(ku < 0 ? -ku : ku, (int64_t)511)] * cv;
+                            const int64_t cell = (px.uc + iu) * a.n_v + v_indx;
+                            re = re + c * (double)g[2 * cell];
+                            im = im + c * (double)g[2 * cell + 1];
+                            norm = norm + c;
+                        }
+                    }
+                    if (a.normalize && norm != 0.0) {
+                        re = re / norm;
+                        im = im / norm;
+                    }
+                }
+                out[2 * p] = (V)re;
+                out[2 * p + 1] = (V)im;
+            }
+        }
+    }
+}
+
+static int pblocks(int64_t n) {
+    int64_t b = ceil_div(n, 256);
+    if (b > (1 << 20)) b = 1 << 20;
+    return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace ngc
+
+using namespace ngc;
+
+extern "C" int ngc_model_to_grid(ngc_context *ctx, void *stream, const double *image,
+                                 int64_t n_planes, int64_t n_l, int64_t n_m, int64_t n_u,
+                                 int64_t n_v, const double *corr_l, const double *corr_m,
+                                 void *grid, int grid_dtype) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    NGC_CHECK(image && grid, NGC_ERR_ARG, "NULL image or grid");
+    NGC_CHECK(n_planes >= 1 && n_l >= 1 && n_m >= 1 && n_l <= n_u && n_m <= n_v, NGC_ERR_ARG,
+              "bad dimensions (the model must fit the grid)");
+    NGC_CHECK(grid_dtype == NGC_C64 || grid_dtype == NGC_C128, NGC_ERR_ARG,
+              "grid dtype must be C64 or C128");
+    NGC_CHECK((corr_l == nullptr) == (corr_m == nullptr), NGC_ERR_ARG,
+              "corr_l and corr_m must both be given or both NULL");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = (hipStream_t)stream;
+    NGC_TRY(enter_stream(ctx, s));
+    const bool dbl = grid_dtype == NGC_C128;
+    const int64_t cells = n_planes * n_u * n_v;
+    void *buf;
+    NGC_TRY(workspace(ctx, "fft_buffer", cells * (dbl ? 16 : 8), &buf));
+    double *d_corr = nullptr;
+    if (corr_l) {
+        std::vector<double> c(n_l + n_m);
+        for (int64_t i = 0; i < n_l; i++) c[i] = corr_l[i];
+        for (int64_t i = 0; i < n_m; i++) c[n_l + i] = corr_m[i];
+        void *p;
+        NGC_TRY(upload(ctx, s, "corr", c.data(), c.size() * sizeof(double), &p));
+        d_corr = (double *)p;
+    }
+    const int nb = pblocks(cells);
+    profile_new_call(ctx, s);
+    phase_begin(ctx, s, "model_pad");
+    if (dbl)
+        model_pad_kernel<double><<<nb, 256, 0, s>>>(image, n_planes, n_l, n_m, n_u, n_v, d_corr,
+                                                    (double *)buf);
+    else
+        model_pad_kernel<float><<<nb, 256, 0, s>>>(image, n_planes, n_l, n_m, n_u, n_v, d_corr,
+                                                   (float *)buf);
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "rocfft_forward");
+    NGC_TRY(fft_execute(ctx, s, buf, n_u, n_v, n_planes, dbl, true));
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "fftshift");
+    if (dbl)
+        fftshift_complex_kernel<double, double><<<nb, 256, 0, s>>>(
+            (const double *)buf, n_planes, n_u, n_v, (double *)grid);
+    else
+        fftshift_complex_kernel<float, float><<<nb, 256, 0, s>>>(
+            (const float *)buf, n_planes, n_u, n_v, (float *)grid);
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    return NGC_OK;
+}
+
+extern "C" int ngc_standard_degrid(ngc_context *ctx, void *stream, const double *uvw,
+                                   const double *freq_chan, const int32_t *chan_map,
+                                   const int32_t *pol_map, const double *cgk_1d, int64_t cgk_len,
+                                   int64_t n_time, int64_t n_baseline, int64_t n_chan,
+                                   int64_t n_pol, int64_t n_imag_chan, int64_t n_imag_pol,
+                                   int64_t n_u, int64_t n_v, double delta_l, double delta_m,
+                                   int64_t support, int64_t oversampling, const void *grid,
+                                   int grid_dtype, void *vis, int vis_dtype, int normalize) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    NGC_CHECK(n_time >= 0 && n_baseline >= 0 && n_chan >= 1 && n_pol >= 1 &&
+                  n_imag_chan >= 1 && n_imag_pol >= 1 && n_u >= 1 && n_v >= 1 && support >= 1 &&
+                  oversampling >= 0,
+              NGC_ERR_ARG, "bad dimensions");
+    NGC_CHECK(freq_chan && chan_map && pol_map && cgk_1d && cgk_len >= 1, NGC_ERR_ARG,
+              "NULL parameter array");
+    NGC_CHECK(cgk_len <= 512, NGC_ERR_UNSUPPORTED, "cgk_1d longer than 512");
+    NGC_CHECK(grid_dtype == NGC_C64 || grid_dtype == NGC_C128, NGC_ERR_ARG,
+              "grid dtype must be C64 or C128");
+    NGC_CHECK(vis_dtype == NGC_C64 || vis_dtype == NGC_C128, NGC_ERR_ARG,
+              "vis dtype must be C64 or C128");
+    for (int64_t c = 0; c < n_chan; c++)
+        NGC_CHECK(chan_map[c] >= 0 && chan_map[c] < n_imag_chan, NGC_ERR_ARG,
+                  "chan_map entry out of range");
+    for (int64_t p = 0; p < n_pol; p++)
+        NGC_CHECK(pol_map[p] >= 0 && pol_map[p] < n_imag_pol, NGC_ERR_ARG,
+                  "pol_map entry out of range");
+    // every tap index must fall inside the table: |os*i + d| <= os*(S//2 + 1/2)
+    NGC_CHECK(oversampling * (support / 2) + oversampling / 2 < cgk_len || support == 1,
+              NGC_ERR_ARG, "cgk_1d shorter than the support needs");
+    if (n_time == 0 || n_baseline == 0) return NGC_OK;
+    NGC_CHECK(uvw && grid && vis, NGC_ERR_ARG, "NULL uvw, grid or vis");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = (hipStream_t)stream;
+    NGC_TRY(enter_stream(ctx, s));
+    std::vector<double> scale;
+    uv_scale_host(freq_chan, n_chan, delta_l, delta_m, n_u, n_v, scale);
+    void *d_scale, *d_cm, *d_pm, *d_cgk;
+    NGC_TRY(upload(ctx, s, "dg_scale", scale.data(), scale.size() * sizeof(double), &d_scale));
+    NGC_TRY(upload(ctx, s, "dg_chan_map", chan_map, n_chan * sizeof(int32_t), &d_cm));
+    NGC_TRY(upload(ctx, s, "dg_pol_map", pol_map, n_pol * sizeof(int32_t), &d_pm));
+    NGC_TRY(upload(ctx, s, "dg_cgk", cgk_1d, cgk_len * sizeof(double), &d_cgk));
+    DegridStdArgs a;
+    a.uvw = uvw;
+    a.uv_scale = (const double *)d_scale;
+    a.chan_map = (const int *)d_cm;
+    a.pol_map = (const int *)d_pm;
+    a.cgk = (const double *)d_cgk;
+    a.cgk_len = cgk_len;
+    a.n_time = n_time;
+    a.n_baseline = n_baseline;
+    a.n_chan = n_chan;
+    a.n_pol = n_pol;
+    a.n_imag_pol = n_imag_pol;
+    a.n_u = n_u;
+    a.n_v = n_v;
+    a.support = support;
+    a.os = oversampling;
+    a.normalize = normalize ? 1 : 0;
+    const int64_t n_thr = n_baseline * ceil_div(n_time, kDegridT) * n_chan;
+    const int nb = pblocks(n_thr);
+    profile_new_call(ctx, s);
+    phase_begin(ctx, s, "standard_degrid");
+    if (grid_dtype == NGC_C64 && vis_dtype == NGC_C64)
+        standard_degrid_kernel<float, float><<<nb, 256, 0, s>>>(a, (const float *)grid,
+                                                                (float *)vis);
+    else if (grid_dtype == NGC_C64)
+        standard_degrid_kernel<float, double><<<nb, 256, 0, s>>>(a, (const float *)grid,
+                                                                 (double *)vis);
+    else if (vis_dtype == NGC_C64)
+        standard_degrid_kernel<double, float><<<nb, 256, 0, s>>>(a, (const double *)grid,
+                                                                 (float *)vis);
+    else
+        standard_degrid_kernel<double, double><<<nb, 256, 0, s>>>(a, (const double *)grid,
+                                                                  (double *)vis);
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    return NGC_OK;
+}

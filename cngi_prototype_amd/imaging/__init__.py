@@ -13,4 +13,5 @@ from .make_mosaic_pb import make_mosaic_pb
 from .make_image_with_gcf import make_image_with_gcf
 from .make_psf_with_gcf import make_psf_with_gcf
 from .make_pb import make_pb
+from .predict_modelvis_image import predict_modelvis_image
 from .direction_rotate import direction_rotate

@@ -335,6 +335,41 @@ int ngc_make_pb(ngc_context *ctx, void *stream, const double *freq_chan, int64_t
                 double cell_l, double cell_m, int64_t n_pol, double *out);
 
 /*
+ * Visibility prediction.  The reference has no implementation: its degridding
+ * graph prints that degridding is still to be implemented
+ * (_imaging_utils/_standard_grid.py:429-430) and predict_modelvis_image.py:20
+ * is a stub naming the steps (correction, Fourier transform, degrid).
+ *
+ * ngc_model_to_grid: model image [n_planes, n_l, n_m] (device float64,
+ * Jy/pixel), divided by corr_l[l] corr_m[m] (host; both NULL: none), padded to
+ * the centre of an n_u x n_v grid (start n_u//2 - n_l//2) and transformed as
+ * fftshift(fft2(ifftshift(.))) (unnormalised forward): the inverse placement
+ * of ngc_grid_to_image.  grid: device [n_planes, n_u, n_v] of grid_dtype
+ * (NGC_C64|NGC_C128).
+ *
+ * ngc_standard_degrid: for every (time, baseline, chan, pol) sample placed on
+ * the grid as the standard gridder places it (same uv scale, centre cell,
+ * sub-cell offset, bounds and taps, _standard_grid.py:273-360),
+ *   V = sum_taps conv * grid[chan_map[c], pol_map[p], u0+i, v0+j]
+ * divided by sum_taps conv when `normalize`; 0 for samples off the grid or
+ * with NaN uvw.  The adjoint of ngc_standard_grid with unit weights.
+ *   grid  device [n_imag_chan, n_imag_pol, n_u, n_v] of grid_dtype (C64|C128)
+ *   vis   device out [T, B, C, P] of vis_dtype (C64|C128)
+ *   uvw device; freq_chan, chan_map, pol_map, cgk_1d host
+ */
+int ngc_model_to_grid(ngc_context *ctx, void *stream, const double *image, int64_t n_planes,
+                      int64_t n_l, int64_t n_m, int64_t n_u, int64_t n_v, const double *corr_l,
+                      const double *corr_m, void *grid, int grid_dtype);
+int ngc_standard_degrid(ngc_context *ctx, void *stream, const double *uvw,
+                        const double *freq_chan, const int32_t *chan_map,
+                        const int32_t *pol_map, const double *cgk_1d, int64_t cgk_len,
+                        int64_t n_time, int64_t n_baseline, int64_t n_chan, int64_t n_pol,
+                        int64_t n_imag_chan, int64_t n_imag_pol, int64_t n_u, int64_t n_v,
+                        double delta_l, double delta_m, int64_t support, int64_t oversampling,
+                        const void *grid, int grid_dtype, void *vis, int vis_dtype,
+                        int normalize);
+
+/*
  * Gridding plans (no counterpart in the reference, whose graph layer rebuilds
  * its dask graph on every call).  A plan fixes every argument of one
  * ngc_standard_grid call in a tiled precision (IMAGE or PSF mode): the input,
