@@ -456,6 +456,90 @@ def streamed_leg(args):
                                    rel_rms_median=float(np.median(rel)), planes=int(n_planes)))
 
 
+def predict_leg(args):
+    """Visibility prediction at the C2 shapes: a model cube of 64 channels x
+    4 pols at 2048^2 (2000 random point sources per plane, Jy/pixel) is
+    transformed to the 2457^2 grid (correction, padding, rocFFT) and degridded
+    at all 627.2M samples, complex64 grid and visibilities with FP64 taps and
+    sums.  Compared with the FP64 path on the first 4 channels.  Roofline of
+    the degrid kernel: the grid read once + uvw + the visibilities written."""
+    import torch
+    from cngi_prototype_amd import _native as nat
+    from cngi_prototype_amd import synthetic as syn
+    from cngi_prototype_amd.imaging._imaging_utils._gridding_convolutional_kernels import \
+        cropped_correction_factors, prolate_spheroidal_kernel_1d
+
+    cfg = syn.CONFIGS["C2"]
+    N = cfg.image_size
+    npad = int(1.2 * N)
+    cell = cfg.cell_arcsec * syn.ARCSEC
+    delta = np.array([-cell, cell])
+    C, P = cfg.n_chan, cfg.n_pol
+    enu = syn.antenna_enu(cfg)
+    uvw_h = syn.uvw_tracks(enu, cfg.lat_deg, cfg.dec_deg, cfg.n_time, cfg.dt)
+    uvw = torch.as_tensor(uvw_h).cuda()
+    T, B = uvw_h.shape[0], uvw_h.shape[1]
+    freq = syn.freqs(cfg)
+    rng = np.random.default_rng(cfg.seed + 31)
+    model = torch.zeros((C, P, N, N), dtype=torch.float64, device="cuda")
+    n_src = 2000
+    li = torch.as_tensor(rng.integers(N // 8, N - N // 8, size=n_src), device="cuda")
+    mi = torch.as_tensor(rng.integers(N // 8, N - N // 8, size=n_src), device="cuda")
+    fl = torch.as_tensor(rng.uniform(1e-3, 1.0, size=n_src), device="cuda")
+    model[:, :, li, mi] = fl
+    cgk = prolate_spheroidal_kernel_1d(100, 7)
+    cl, cm = cropped_correction_factors([npad, npad], [N, N])
+    grid = torch.empty((C, P, npad, npad), dtype=torch.complex64, device="cuda")
+    vis = torch.empty((T, B, C, P), dtype=torch.complex64, device="cuda")
+
+    def step():
+        ph = {}
+        nat.model_to_grid(model, [npad, npad], cl, cm, single=True, out=grid)
+        ph.update(dict(nat.profile_read()))
+        nat.standard_degrid(uvw, grid, freq, np.arange(C), np.arange(P), cgk, delta, 7, 100,
+                            single=True, out=vis)
+        ph.update(dict(nat.profile_read()))
+        return ph
+
+    nat.profile_enable(True)
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    phases = {}
+    t0 = time.time()
+    for _ in range(args.steps):
+        for k, v in step().items():
+            phases[k] = phases.get(k, 0.0) + v / args.steps
+    torch.cuda.synchronize()
+    ms_step = (time.time() - t0) / args.steps * 1e3
+    nat.profile_enable(False)
+    # FP64 path on the first 4 channels
+    nc = 4
+    g64 = nat.model_to_grid(model[:nc].contiguous(), [npad, npad], cl, cm, single=False)
+    v64 = nat.standard_degrid(uvw, g64, freq[:nc], np.arange(nc), np.arange(P), cgk, delta, 7,
+                              100, single=False)
+    d = (vis[:, :, :nc].to(torch.complex128) - v64)
+    rel = float(torch.linalg.norm(d) / torch.linalg.norm(v64))
+    del g64, v64, d, model, grid
+    n_samples = T * B * C * P
+    t_dg = phases.get("standard_degrid", float("nan")) / 1e3
+    alg = C * P * npad * npad * 8 + T * B * 24 + n_samples * 8
+    gbs = alg / t_dg / 1e9
+    log(f"predict: step {ms_step:.2f} ms, phases {phases}")
+    return dict(workload=f"C2 shapes: model cube {C} chan x {P} pol at {N}^2 (2000 point "
+                         f"sources per plane) -> {npad}^2 grid -> {n_samples / 1e6:.1f}M "
+                         f"predicted samples", samples=int(n_samples),
+                value=round(n_samples / (ms_step / 1e3) / 1e6, 1), unit="Mvis/s",
+                ms_per_step=round(ms_step, 3),
+                phases_ms={k: round(v, 3) for k, v in phases.items()},
+                degrid_roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=8000.0,
+                                     unit="GB/s", frac=round(gbs / 8000.0, 4),
+                                     algorithmic_bytes=int(alg),
+                                     kernel_ms=round(t_dg * 1e3, 3)),
+                precision="complex64 grid and visibilities, FP64 taps",
+                rel_err_vs_fp64_4chan=rel)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -475,6 +559,10 @@ def main():
                     help="skip the C5 streamed-mosaic leg (reported under 'streamed')")
     ap.add_argument("--streamed-only", action="store_true",
                     help="run only the C5 streamed-mosaic leg and print its record")
+    ap.add_argument("--no-predict", action="store_true",
+                    help="skip the C2-shape visibility prediction leg (reported under 'predict')")
+    ap.add_argument("--predict-only", action="store_true",
+                    help="run only the prediction leg and print its record")
     ap.add_argument("--cpu-chans", type=int, default=6)
     ap.add_argument("--config", default="C2")
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
@@ -513,6 +601,9 @@ def main():
         return
     if args.streamed_only:
         print(json.dumps({"streamed": streamed_leg(args)}), file=out, flush=True)
+        return
+    if args.predict_only:
+        print(json.dumps({"predict": predict_leg(args)}), file=out, flush=True)
         return
     if args.tile_rows:
         nat.set_tile_rows(args.tile_rows)
@@ -663,6 +754,16 @@ def main():
             log(f"streamed leg failed: {e!r}")
             streamed = {"error": repr(e)}
 
+    predict = None
+    if not args.no_predict:
+        torch.cuda.empty_cache()
+        try:
+            predict = predict_leg(args)
+            log(f"predict {predict}")
+        except Exception as e:  # report, never fake
+            log(f"predict leg failed: {e!r}")
+            predict = {"error": repr(e)}
+
     line = {
         "metric": "Mvis/s gridded (std+mosaic) at <=1e-5 rel err vs FP64; % of HBM roofline",
         "value": round(value, 1),
@@ -695,6 +796,7 @@ def main():
         "mosaic": mosaic,
         "batched": batched,
         "streamed": streamed,
+        "predict": predict,
     }
     print(json.dumps(line), file=out, flush=True)
 

@@ -17,16 +17,15 @@
 //                         weights: same uv scale, centre cell, sub-cell offset,
 //                         bounds and tap index (_standard_grid.py:273-360)
 //
-// The degrid kernel is a gather: one thread per (baseline, channel, block of
-// kDegridT integrations), channels fastest, so a wavefront writes each
-// integration's visibilities as one contiguous run and each thread's
-// footprint moves slowly from one integration to the next (its grid lines
-// stay in cache).  Taps and sums are FP64; the grid may be complex64.
+// The degrid kernel is a gather: one thread per sample; the 64 lanes of a
+// wavefront are 64 consecutive integrations of one (baseline, channel), whose
+// footprints in one grid plane overlap, so their tap reads share cache lines.
+// Taps and sums are FP64; the grid may be complex64.
+#include <type_traits>
+
 #include "ngc_internal.hpp"
 
 namespace ngc {
-
-constexpr int kDegridT = 16;  // integrations per thread
 
 // buf[pl][ku][kv] = X[l][m] / (corr_l[l] corr_m[m]) for the model pixel
 // (l, m) = ((ku + n_l//2) mod n_u, (kv + n_m//2) mod n_v) when inside the
@@ -88,7 +87,23 @@ struct DegridStdArgs {
     int normalize;
 };
 
-template <typename G, typename V>
+// Tap weights of one axis: c[i] = cgk[|os (i - S//2) + d|] for i < S.
+template <int SMAX>
+__device__ inline void axis_taps(const double *s_k, int64_t S, int64_t os, int64_t d,
+                                 double (&c)[SMAX]) {
+#pragma unroll
+    for (int i = 0; i < SMAX; i++) {
+        const int64_t k = os * (i - S / 2) + d;
+        c[i] = i < S ? s_k[min(k < 0 ? -k : k, (int64_t)511)] : 0.0;
+    }
+}
+
+// One thread per sample (all pols); the 64 lanes of a wavefront are 64
+// consecutive integrations of one (baseline, channel), whose footprints in the
+// same grid plane overlap, so their tap reads share cache lines.  SMAX is the
+// compile-time bound of the support (taps beyond the support weigh 0 and are
+// skipped).
+template <typename G, typename V, int SMAX>
 __global__ void __launch_bounds__(256) standard_degrid_kernel(DegridStdArgs a, const G *grid,
                                                               V *vis) {
 #pragma clang fp contract(off)
@@ -96,51 +111,65 @@ __global__ void __launch_bounds__(256) standard_degrid_kernel(DegridStdArgs a, c
     for (int i = threadIdx.x; i < 512; i += blockDim.x)
         s_k[i] = i < a.cgk_len ? a.cgk[i] : 0.0;
     __syncthreads();
-    const int64_t n_tb = (a.n_time + kDegridT - 1) / kDegridT;
-    const int64_t n_thr = a.n_baseline * n_tb * a.n_chan;
+    const int64_t n_tb = (a.n_time + 63) / 64;
+    const int64_t n_thr = a.n_baseline * a.n_chan * n_tb * 64;
     const int64_t sc = a.support / 2;
     const int64_t plane = a.n_u * a.n_v;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_thr;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t ch = i % a.n_chan;
-        const int64_t r = i / a.n_chan;
-        const int64_t tb = r % n_tb, b = r / n_tb;
-        const int64_t a_chan = a.chan_map[ch];
-        const double s0 = a.uv_scale[ch], s1 = a.uv_scale[a.n_chan + ch];
-        const int64_t t_end = min((tb + 1) * kDegridT, a.n_time);
-        for (int64_t t = tb * kDegridT; t < t_end; t++) {
-            const int64_t row = t * a.n_baseline + b;
-            V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol);
-            Pix px;
-            bool on = pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], s0, s1, a.n_u, a.n_v,
-                                     a.os, px);
-            on = on && (px.uc + sc < a.n_u) && (px.vc + sc < a.n_v) && (px.uc - sc >= 0) &&
-                 (px.vc - sc >= 0);
+        const int64_t w = i >> 6;
+        const int64_t t = (w % n_tb) * 64 + (i & 63);
+        const int64_t r = w / n_tb;
+        const int64_t ch = r % a.n_chan, b = r / a.n_chan;
+        if (t >= a.n_time) continue;
+        const int64_t row = t * a.n_baseline + b;
+        V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol);
+        Pix px;
+        bool on = pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
+                                 a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os, px);
+        on = on && (px.uc + sc < a.n_u) && (px.vc + sc < a.n_v) && (px.uc - sc >= 0) &&
+             (px.vc - sc >= 0);
+        if (!on) {
             for (int64_t p = 0; p < a.n_pol; p++) {
-                double re = 0.0, im = 0.0, norm = 0.0;
-                if (on) {
-                    const G *g = grid + 2 * (a_chan * a.n_imag_pol + a.pol_map[p]) * plane;
-                    for (int64_t iv = -sc; iv < a.support - sc; iv++) {
-                        const int64_t kv = a.os * iv + px.dv;
-                        const double cv = s_k[min(kv < 0 ? -kv : kv, (int64_t)511)];
-                        const int64_t v_indx = px.vc + iv;
-                        for (int64_t iu = -sc; iu < a.support - sc; iu++) {
-                            const int64_t ku = a.os * iu + px.du;
-                            const double c = s_k[min(ku < 0 ? -ku : ku, (int64_t)511)] * cv;
-                            const int64_t cell = (px.uc + iu) * a.n_v + v_indx;
-                            re = re + c * (double)g[2 * cell];
-                            im = im + c * (double)g[2 * cell + 1];
-                            norm = norm + c;
-                        }
-                    }
-                    if (a.normalize && norm != 0.0) {
-                        re = re / norm;
-                        im = im / norm;
+                out[2 * p] = (V)0;
+                out[2 * p + 1] = (V)0;
+            }
+            continue;
+        }
+        double cu[SMAX], cv[SMAX];
+        axis_taps<SMAX>(s_k, a.support, a.os, px.du, cu);
+        axis_taps<SMAX>(s_k, a.support, a.os, px.dv, cv);
+        // tap sum in the gridder's loop order (v outer, u inner)
+        double norm = 0.0;
+#pragma unroll
+        for (int iv = 0; iv < SMAX; iv++)
+#pragma unroll
+            for (int iu = 0; iu < SMAX; iu++)
+                if (iv < a.support && iu < a.support) norm = norm + cu[iu] * cv[iv];
+        const int64_t a_chan = a.chan_map[ch];
+        const int64_t u0 = px.uc - sc, v0 = px.vc - sc;
+        for (int64_t p = 0; p < a.n_pol; p++) {
+            const G *g = grid + 2 * ((a_chan * a.n_imag_pol + a.pol_map[p]) * plane +
+                                     u0 * a.n_v + v0);
+            double re = 0.0, im = 0.0;
+#pragma unroll
+            for (int iv = 0; iv < SMAX; iv++) {
+#pragma unroll
+                for (int iu = 0; iu < SMAX; iu++) {
+                    if (iv < a.support && iu < a.support) {
+                        const double c = cu[iu] * cv[iv];
+                        const int64_t o = 2 * ((int64_t)iu * a.n_v + iv);
+                        re = re + c * (double)g[o];
+                        im = im + c * (double)g[o + 1];
                     }
                 }
-                out[2 * p] = (V)re;
-                out[2 * p + 1] = (V)im;
             }
+            if (a.normalize && norm != 0.0) {
+                re = re / norm;
+                im = im / norm;
+            }
+            out[2 * p] = (V)re;
+            out[2 * p + 1] = (V)im;
         }
     }
 }
@@ -225,6 +254,7 @@ extern "C" int ngc_standard_degrid(ngc_context *ctx, void *stream, const double 
     NGC_CHECK(freq_chan && chan_map && pol_map && cgk_1d && cgk_len >= 1, NGC_ERR_ARG,
               "NULL parameter array");
     NGC_CHECK(cgk_len <= 512, NGC_ERR_UNSUPPORTED, "cgk_1d longer than 512");
+    NGC_CHECK(support <= 16, NGC_ERR_UNSUPPORTED, "support above 16");
     NGC_CHECK(grid_dtype == NGC_C64 || grid_dtype == NGC_C128, NGC_ERR_ARG,
               "grid dtype must be C64 or C128");
     NGC_CHECK(vis_dtype == NGC_C64 || vis_dtype == NGC_C128, NGC_ERR_ARG,
@@ -267,22 +297,29 @@ extern "C" int ngc_standard_degrid(ngc_context *ctx, void *stream, const double 
     a.support = support;
     a.os = oversampling;
     a.normalize = normalize ? 1 : 0;
-    const int64_t n_thr = n_baseline * ceil_div(n_time, kDegridT) * n_chan;
+    const int64_t n_thr = n_baseline * n_chan * ceil_div(n_time, 64) * 64;
     const int nb = pblocks(n_thr);
     profile_new_call(ctx, s);
     phase_begin(ctx, s, "standard_degrid");
-    if (grid_dtype == NGC_C64 && vis_dtype == NGC_C64)
-        standard_degrid_kernel<float, float><<<nb, 256, 0, s>>>(a, (const float *)grid,
-                                                                (float *)vis);
-    else if (grid_dtype == NGC_C64)
-        standard_degrid_kernel<float, double><<<nb, 256, 0, s>>>(a, (const float *)grid,
-                                                                 (double *)vis);
-    else if (vis_dtype == NGC_C64)
-        standard_degrid_kernel<double, float><<<nb, 256, 0, s>>>(a, (const double *)grid,
-                                                                 (float *)vis);
+    auto launch = [&](auto smax) {
+        constexpr int SM = decltype(smax)::value;
+        if (grid_dtype == NGC_C64 && vis_dtype == NGC_C64)
+            standard_degrid_kernel<float, float, SM><<<nb, 256, 0, s>>>(a, (const float *)grid,
+                                                                        (float *)vis);
+        else if (grid_dtype == NGC_C64)
+            standard_degrid_kernel<float, double, SM><<<nb, 256, 0, s>>>(
+                a, (const float *)grid, (double *)vis);
+        else if (vis_dtype == NGC_C64)
+            standard_degrid_kernel<double, float, SM><<<nb, 256, 0, s>>>(
+                a, (const double *)grid, (float *)vis);
+        else
+            standard_degrid_kernel<double, double, SM><<<nb, 256, 0, s>>>(
+                a, (const double *)grid, (double *)vis);
+    };
+    if (support <= 8)
+        launch(std::integral_constant<int, 8>{});
     else
-        standard_degrid_kernel<double, double><<<nb, 256, 0, s>>>(a, (const double *)grid,
-                                                                  (double *)vis);
+        launch(std::integral_constant<int, 16>{});
     NGC_HIP(hipGetLastError());
     phase_end(ctx, s);
     return NGC_OK;
