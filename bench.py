@@ -55,7 +55,10 @@ def measured_traffic(precision, config):
 
 def cpu_baseline(cfg, data, npad, delta, cgk, n_chan_sample):
     """The oracle (FP64 C restatement of the reference loop) timed on this
-    host, one thread, on the first n_chan_sample channels of the workload."""
+    host on the first n_chan_sample channels of the workload: on every host
+    core, shaped as the reference's dask graph (one task per time chunk with a
+    private grid, then a pairwise tree-sum of the grids), and on one thread.
+    Returns (all-cores record, one-thread record)."""
     import oracle
     oracle.build()
     t0 = time.time()
@@ -65,19 +68,28 @@ def cpu_baseline(cfg, data, npad, delta, cgk, n_chan_sample):
     flag = data["flag"][:, :, :n_chan_sample].cpu().numpy().astype(bool)
     w[flag] = np.nan
     fr = data["freq"][:n_chan_sample]
-    log(f"cpu baseline: host copy {time.time() - t0:.1f}s")
-    t0 = time.time()
-    grid, sw, used = oracle.standard_grid_chunked(vis, uvw, w, fr, cgk, [npad, npad], delta, 7,
-                                                  100, do_psf=False, time_chunk=uvw.shape[0],
-                                                  n_threads=1)
-    dt = time.time() - t0
     n_unflagged = int((~flag).sum())
-    del grid
-    return dict(value=round(n_unflagged / dt / 1e6, 3), unit="Mvis/s", cores=int(used),
-                kind="port",
-                sample=f"FP64 C oracle (reference loop order), 1 thread, image mode, first "
-                       f"{n_chan_sample} of {cfg.n_chan} channels ({n_unflagged} unflagged "
-                       f"samples), {dt:.1f} s")
+    log(f"cpu baseline: host copy {time.time() - t0:.1f}s")
+    out = []
+    for threads in (0, 1):
+        n_thr = threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        chunk = uvw.shape[0] if threads == 1 else -(-uvw.shape[0] // n_thr)
+        t0 = time.time()
+        grid, sw, used = oracle.standard_grid_chunked(vis, uvw, w, fr, cgk, [npad, npad], delta,
+                                                      7, 100, do_psf=False, time_chunk=chunk,
+                                                      n_threads=threads)
+        dt = time.time() - t0
+        del grid
+        how = ("1 thread" if threads == 1 else
+               f"{used} threads, {-(-uvw.shape[0] // chunk)} time chunks with private grids and "
+               f"a pairwise tree-sum (the reference's dask graph shape)")
+        out.append(dict(value=round(n_unflagged / dt / 1e6, 3), unit="Mvis/s", cores=int(used),
+                        kind="port",
+                        sample=f"FP64 C oracle (reference loop order), {how}, image mode, first "
+                               f"{n_chan_sample} of {cfg.n_chan} channels ({n_unflagged} "
+                               f"unflagged samples), {dt:.1f} s"))
+        log(f"cpu baseline {out[-1]}")
+    return out[0], out[1]
 
 
 def mosaic_leg(args):
@@ -715,11 +727,10 @@ def main():
         torch.cuda.synchronize()
         log(f"error check {time.time() - te:.1f}s: {err}")
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(cfg, data, npad, delta, cgk, args.cpu_chans)
-            log(f"cpu baseline {cpu}")
+            cpu, cpu1 = cpu_baseline(cfg, data, npad, delta, cgk, args.cpu_chans)
         except Exception as e:  # report, never fake
             log(f"cpu baseline failed: {e!r}")
 
@@ -789,6 +800,7 @@ def main():
                            f"classes, register runs flushed into an LDS subgrid)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "error_vs_fp64": err,
         "phases_ms": phase_ms,
         "briggs_ms": round(t_briggs * 1e3, 2),
