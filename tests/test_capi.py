@@ -52,3 +52,47 @@ def test_binding_covers_the_header():
         assert getattr(_native, name) == defines["NGC_" + name], name
     for name in ("IMAGE", "PSF", "DENSITY", "WEIGHT"):
         assert getattr(_native, "GRID_" + name) == defines["NGC_GRID_" + name], name
+
+
+def _zero_args(fn):
+    out = []
+    for t in fn.argtypes:
+        if t in (ctypes.c_double, ctypes.c_float):
+            out.append(0.0)
+        elif t in (ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64):
+            out.append(0)
+        else:
+            out.append(None)
+    return out
+
+
+def test_entry_points_reject_a_null_handle_without_a_gpu():
+    """Every entry point that takes a context or plan returns NGC_ERR_ARG (-1)
+    for a NULL one, with a message, before any device call; the destroy
+    functions accept NULL, as free() does."""
+    from cngi_prototype_amd import _native
+    lib = _native.load_library()
+    lib.ngc_last_error.restype = ctypes.c_char_p
+    null_ok = {"ngc_context_destroy", "ngc_grid_plan_destroy"}
+    skip = {"ngc_context_create", "ngc_last_error", "ngc_version"}
+    checked = 0
+    for name in _native.EXPORTS:
+        if name in skip:
+            continue
+        fn = getattr(lib, name)
+        rc = fn(*_zero_args(fn))
+        if name in null_ok:
+            assert rc == 0, name
+        else:
+            assert rc == -1, name
+            assert lib.ngc_last_error(), name
+        checked += 1
+    assert checked >= 30
+
+
+def test_context_create_rejects_a_null_out_pointer():
+    from cngi_prototype_amd import _native
+    lib = _native.load_library()
+    lib.ngc_last_error.restype = ctypes.c_char_p
+    assert lib.ngc_context_create(0, None) == -1
+    assert b"NULL" in lib.ngc_last_error()
