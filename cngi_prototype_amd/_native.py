@@ -34,7 +34,8 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_grid_plan_describe", "ngc_grid_plan_destroy", "ngc_bf16_gemm",
            "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step",
            "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags", "ngc_set_tile_waves",
-           "ngc_model_to_grid", "ngc_standard_degrid")
+           "ngc_model_to_grid", "ngc_standard_degrid", "ngc_mlp_sample_rows",
+           "ngc_adam_schedule", "ngc_adam_step_sched")
 
 
 class NativeError(RuntimeError):
@@ -101,6 +102,10 @@ _PROTOS = {
     "ngc_col_sum": (_i32, [_vp, _vp, _i64, _i64, _vp, _vp]),
     "ngc_mse_grad": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "ngc_adam_step": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _d, _d, _d, _d, _i64]),
+    "ngc_mlp_sample_rows": (_i32, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, ctypes.c_uint64,
+                                   _vp, _vp, _vp]),
+    "ngc_adam_schedule": (_i32, [_vp, _vp, _vp, _vp, _d, _d, _i64, _i64, _d, _d]),
+    "ngc_adam_step_sched": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _d, _d, _d]),
     "ngc_direction_rotate": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _vp, _i64, _vp, _vp, _vp,
                                     _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp]),
 }
@@ -720,6 +725,28 @@ def adam_step(w, g, m, v, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
     _check("ngc_adam_step", load_library().ngc_adam_step(
         context().handle, _stream(), w.numel(), _ptr(w), _ptr(g), _ptr(m), _ptr(v), float(lr),
         float(beta1), float(beta2), float(eps), int(step)))
+
+
+def mlp_sample_rows(x, t, batch, seed, counter, xb, tb):
+    """ngc_mlp_sample_rows: the minibatch of the step in `counter` (device int64)."""
+    _check("ngc_mlp_sample_rows", load_library().ngc_mlp_sample_rows(
+        context().handle, _stream(), _ptr(x), _ptr(t), x.shape[0], x.shape[1], t.shape[1],
+        int(batch), int(seed) & ((1 << 64) - 1), _ptr(counter), _ptr(xb), _ptr(tb)))
+
+
+def adam_schedule(counter, sched, lr, lr_min, steps, step0, beta1=0.9, beta2=0.999):
+    """ngc_adam_schedule: rate and bias corrections of the step in `counter`,
+    written to sched (device float [3]); advances the counter."""
+    _check("ngc_adam_schedule", load_library().ngc_adam_schedule(
+        context().handle, _stream(), _ptr(counter), _ptr(sched), float(lr), float(lr_min),
+        int(steps), int(step0), float(beta1), float(beta2)))
+
+
+def adam_step_sched(w, g, m, v, sched, beta1=0.9, beta2=0.999, eps=1e-8):
+    """ngc_adam_step_sched: Adam with the rate and corrections read from sched."""
+    _check("ngc_adam_step_sched", load_library().ngc_adam_step_sched(
+        context().handle, _stream(), w.numel(), _ptr(w), _ptr(g), _ptr(m), _ptr(v),
+        _ptr(sched), float(beta1), float(beta2), float(eps)))
 
 
 def direction_rotate(uvw, vis, field, field_ids, uvw_rotmat, phase_rotation, freq_chan,

@@ -209,6 +209,65 @@ __global__ void adam_kernel(int64_t n, float *w, const float *g, float *m, float
 
 __global__ void zero_f64_kernel(double *p) { *p = 0.0; }
 
+// Device-side training schedule, so that one training step can be captured in
+// a graph and replayed: `counter` holds the number of steps taken in this run.
+__device__ inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// rows r_i = hash(seed, step, i) mod n of x [n, dx] and t [n, dt] into the
+// minibatch xb [batch, dx], tb [batch, dt]
+__global__ void sample_rows_kernel(const float *x, const float *t, int64_t n, int64_t dx,
+                                   int64_t dt, int64_t batch, uint64_t seed,
+                                   const int64_t *counter, float *xb, float *tb) {
+    const uint64_t k = (uint64_t)*counter;
+    const int64_t width = dx + dt;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < batch * width;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = i / width, c = i - j * width;
+        const int64_t r = (int64_t)(splitmix64(seed ^ splitmix64(k * (uint64_t)batch +
+                                                                  (uint64_t)j)) %
+                                    (uint64_t)n);
+        if (c < dx)
+            xb[j * dx + c] = x[r * dx + c];
+        else
+            tb[j * dt + c - dx] = t[r * dt + c - dx];
+    }
+}
+
+// sched = {learning rate (cosine decay from lr to lr_min over `steps`),
+// 1 - b1^s, 1 - b2^s} for Adam step s = step0 + k + 1, k = *counter; then ++k.
+__global__ void adam_schedule_kernel(int64_t *counter, float *sched, double lr, double lr_min,
+                                     int64_t steps, int64_t step0, double b1, double b2) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int64_t k = *counter;
+        const double span = steps > 1 ? (double)(steps - 1) : 1.0;
+        const double rate = lr_min + 0.5 * (lr - lr_min) * (1.0 + cos(M_PI * (double)k / span));
+        const double s = (double)(step0 + k + 1);
+        sched[0] = (float)rate;
+        sched[1] = (float)(1.0 - pow(b1, s));
+        sched[2] = (float)(1.0 - pow(b2, s));
+        *counter = k + 1;
+    }
+}
+
+__global__ void adam_sched_kernel(int64_t n, float *w, const float *g, float *m, float *v,
+                                  const float *sched, float b1, float b2, float eps) {
+    const float lr = sched[0], c1 = sched[1], c2 = sched[2];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
 static int blocks_of(int64_t n) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 8192));
 }
@@ -304,6 +363,42 @@ int ngc_adam_step(ngc_context *ctx, void *stream, int64_t n, float *w, const flo
     const float c2 = (float)(1.0 - std::pow(beta2, (double)step));
     adam_kernel<<<blocks_of(n), 256, 0, (hipStream_t)stream>>>(
         n, w, g, m, v, (float)lr, (float)beta1, (float)beta2, (float)eps, c1, c2);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+int ngc_mlp_sample_rows(ngc_context *ctx, void *stream, const float *x, const float *t,
+                        int64_t n, int64_t dx, int64_t dt, int64_t batch, uint64_t seed,
+                        const int64_t *counter, float *xb, float *tb) {
+    NGC_CHECK(ctx && x && t && counter && xb && tb, NGC_ERR_ARG, "bad arguments");
+    NGC_CHECK(n >= 1 && dx >= 1 && dt >= 1 && batch >= 1, NGC_ERR_ARG, "bad dimensions");
+    NGC_HIP(hipSetDevice(ctx->device));
+    sample_rows_kernel<<<blocks_of(batch * (dx + dt)), 256, 0, (hipStream_t)stream>>>(
+        x, t, n, dx, dt, batch, seed, counter, xb, tb);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+int ngc_adam_schedule(ngc_context *ctx, void *stream, int64_t *counter, float *sched,
+                      double lr, double lr_min, int64_t steps, int64_t step0, double beta1,
+                      double beta2) {
+    NGC_CHECK(ctx && counter && sched && steps >= 1 && step0 >= 0, NGC_ERR_ARG,
+              "bad arguments");
+    NGC_HIP(hipSetDevice(ctx->device));
+    adam_schedule_kernel<<<1, 64, 0, (hipStream_t)stream>>>(counter, sched, lr, lr_min, steps,
+                                                            step0, beta1, beta2);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+int ngc_adam_step_sched(ngc_context *ctx, void *stream, int64_t n, float *w, const float *g,
+                        float *m, float *v, const float *sched, double beta1, double beta2,
+                        double eps) {
+    NGC_CHECK(ctx && w && g && m && v && sched && n >= 0, NGC_ERR_ARG, "bad arguments");
+    if (n == 0) return NGC_OK;
+    NGC_HIP(hipSetDevice(ctx->device));
+    adam_sched_kernel<<<blocks_of(n), 256, 0, (hipStream_t)stream>>>(
+        n, w, g, m, v, sched, (float)beta1, (float)beta2, (float)eps);
     NGC_HIP(hipGetLastError());
     return NGC_OK;
 }

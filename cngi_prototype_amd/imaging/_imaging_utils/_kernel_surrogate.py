@@ -90,22 +90,57 @@ class KernelSurrogate:
                 dz = _native.tanh_backward(da, acts[i])
         return loss, grads
 
-    def train(self, x, t, steps=3000, batch=4096, lr=2e-3, lr_min=2e-5, seed=1, log_every=0):
-        """Adam with a cosine learning-rate decay over `steps` minibatches."""
+    def train(self, x, t, steps=3000, batch=4096, lr=2e-3, lr_min=2e-5, seed=1, log_every=0,
+              graph=True):
+        """Adam with a cosine learning-rate decay over `steps` minibatches drawn
+        on the device.  The step (sampling, forward, backward, schedule, Adam)
+        keeps its step number in device memory, so with graph=True it runs
+        once eagerly and is then captured as a HIP graph and replayed; with
+        graph=False every step is launched eagerly (the same kernels and
+        arithmetic)."""
         import torch
-        gen = torch.Generator(device="cuda").manual_seed(seed)
         n = x.shape[0]
-        hist = []
-        for k in range(steps):
-            idx = torch.randint(0, n, (min(batch, n),), generator=gen, device="cuda")
-            xb, tb = x[idx].contiguous(), t[idx].contiguous()
+        b = min(batch, n)
+        dev = x.device
+        counter = torch.zeros(1, dtype=torch.int64, device=dev)
+        sched = torch.zeros(3, dtype=torch.float32, device=dev)
+        xb = torch.empty((b, x.shape[1]), dtype=torch.float32, device=dev)
+        tb = torch.empty((b, t.shape[1]), dtype=torch.float32, device=dev)
+        step0 = self.steps
+        seed = int(seed) * 0x9E3779B97F4A7C15 + 1
+
+        def one_step():
+            _native.mlp_sample_rows(x, t, b, seed, counter, xb, tb)
             loss, grads = self.loss_and_grads(xb, tb)
-            self.steps += 1
-            rate = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * k / max(1, steps - 1)))
+            _native.adam_schedule(counter, sched, lr, lr_min, steps, step0)
             for p, g, m, v in zip(self.params(), grads, self.m, self.v):
-                _native.adam_step(p, g, m, v, rate, self.steps)
+                _native.adam_step_sched(p, g, m, v, sched)
+            return loss
+
+        hist = []
+
+        def log(k, loss):
             if log_every and (k % log_every == 0 or k == steps - 1):
                 hist.append((k, float(loss.item())))
+
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            loss = one_step()   # eager: sizes the workspaces, binds this stream
+            log(0, loss)
+            if graph and steps > 1:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
+                    gloss = one_step()
+                for k in range(1, steps):
+                    g.replay()
+                    log(k, gloss)
+            else:
+                for k in range(1, steps):
+                    log(k, one_step())
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.steps = step0 + steps
         return hist
 
     def set_scaling(self, x, t):

@@ -426,6 +426,30 @@ int ngc_adam_step(ngc_context *ctx, void *stream, int64_t n, float *w, const flo
                   float *v, double lr, double beta1, double beta2, double eps, int64_t step);
 
 /*
+ * Training with a device-side step counter (no counterpart in the reference),
+ * so that a whole training step can be captured once in a graph and
+ * replayed: nothing step-dependent is passed from the host.
+ *   ngc_mlp_sample_rows   minibatch of step k = *counter: rows
+ *                         splitmix64(seed ^ splitmix64(k*batch + j)) mod n of
+ *                         x [n, dx] and t [n, dt] into xb [batch, dx], tb [batch, dt]
+ *   ngc_adam_schedule     sched = {cosine-decayed rate lr -> lr_min over `steps`
+ *                         at step k, 1 - beta1^s, 1 - beta2^s} with
+ *                         s = step0 + k + 1; then *counter = k + 1
+ *   ngc_adam_step_sched   the Adam update with the rate and bias corrections
+ *                         read from sched (device float [3])
+ * counter: device int64; all arrays device FP32.
+ */
+int ngc_mlp_sample_rows(ngc_context *ctx, void *stream, const float *x, const float *t,
+                        int64_t n, int64_t dx, int64_t dt, int64_t batch, uint64_t seed,
+                        const int64_t *counter, float *xb, float *tb);
+int ngc_adam_schedule(ngc_context *ctx, void *stream, int64_t *counter, float *sched,
+                      double lr, double lr_min, int64_t steps, int64_t step0, double beta1,
+                      double beta2);
+int ngc_adam_step_sched(ngc_context *ctx, void *stream, int64_t n, float *w, const float *g,
+                        float *m, float *v, const float *sched, double beta1, double beta2,
+                        double eps);
+
+/*
  * Tile shape of the uv-tiled gridders: 8, 16 or 32 grid rows per tile (the
  * tile is always 32 columns wide).  Set by the autotuner (no counterpart in the
  * reference, whose gridder has no tiles).  Default 16.

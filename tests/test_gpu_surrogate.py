@@ -102,6 +102,31 @@ def test_training_fits_a_smooth_map(gpu):
     assert hist[-1][1] < 0.05 * hist[0][1]
 
 
+def test_graph_training_equals_eager(gpu):
+    """Training replayed from one captured HIP graph gives the same weights,
+    bit for bit, as launching every step eagerly (the step number, sampling
+    and schedule live on the device), and a second training run continues
+    the Adam step count."""
+    import torch
+    from cngi_prototype_amd.imaging._imaging_utils._kernel_surrogate import KernelSurrogate
+    rng = np.random.default_rng(5)
+    x = _t(rng.uniform(-1, 1, size=(3000, 10)))
+    t = _t(np.sin(rng.uniform(-1, 1, size=(3000, 10)) @ rng.normal(size=(10, 24))))
+    models = []
+    for graph in (False, True):
+        m = KernelSurrogate(10, 24, width=64, depth=4, seed=3)
+        m.train(x, t, steps=40, batch=512, lr=3e-3, seed=7, graph=graph)
+        m.train(x, t, steps=25, batch=512, lr=1e-3, seed=8, graph=graph)
+        models.append(m)
+    torch.cuda.synchronize()
+    assert models[0].steps == models[1].steps == 65
+    for a, b in zip(models[0].params() + models[0].m + models[0].v,
+                    models[1].params() + models[1].m + models[1].v):
+        assert torch.equal(a, b)
+    assert not torch.equal(models[1].W[0], KernelSurrogate(10, 24, width=64, depth=4,
+                                                           seed=3).W[0])
+
+
 def test_surrogate_beats_bin_kernels_and_falls_back(gpu):
     """Per-channel kernels over a 1 GHz band that the reference covers with one
     kernel (chan_tolerance_factor 0.005 at 230 GHz): the surrogate's largest
