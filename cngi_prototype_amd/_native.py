@@ -97,7 +97,7 @@ _PROTOS = {
     "ngc_grid_plan_describe": (_i32, [_vp, ctypes.c_char_p, _i64]),
     "ngc_grid_plan_destroy": (_i32, [_vp]),
     "ngc_bf16_gemm": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp,
-                             _i32, _i32, _vp, _i64, _vp]),
+                             _i32, _i32, _vp, _i64, _vp, _vp]),
     "ngc_tanh_backward": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp]),
     "ngc_col_sum": (_i32, [_vp, _vp, _i64, _i64, _vp, _vp]),
     "ngc_mse_grad": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp]),
@@ -668,9 +668,10 @@ def make_pb(freq_chan, dish, blockage, casa, ipower, image_size, image_center, c
 # dense-layer primitives of the kernel surrogate (float32 device tensors)
 # ---------------------------------------------------------------------------
 def bf16_gemm(A, B, bias=None, act=0, out=None, pre=None, accumulate=False, trans_a=False,
-              trans_b=False):
+              trans_b=False, aux=None):
     """out = act(op(A) @ op(B) + bias) on the matrix cores (bf16 operands, FP32
-    accumulation).  op(X) = X.T when trans_x.  Returns out (M x N float32)."""
+    accumulation).  op(X) = X.T when trans_x; act 0 none, 1 tanh, 2 times
+    1 - aux^2 (aux M x N).  Returns out (M x N float32)."""
     import torch
     for t, n in ((A, "A"), (B, "B")):
         _dev(t, n)
@@ -688,9 +689,14 @@ def bf16_gemm(A, B, bias=None, act=0, out=None, pre=None, accumulate=False, tran
         raise ValueError("out must be a contiguous M x N tensor")
     if pre is not None and (not pre.is_contiguous() or tuple(pre.shape) != (M, N)):
         raise ValueError("pre must be a contiguous M x N tensor")
+    if (act == 2) != (aux is not None):
+        raise ValueError("aux is given exactly with act=2")
+    if aux is not None and (not aux.is_contiguous() or tuple(aux.shape) != (M, N)):
+        raise ValueError("aux must be a contiguous M x N tensor")
     rc = load_library().ngc_bf16_gemm(context().handle, _stream(), M, N, K, _ptr(A), sam, sak,
                                       _ptr(B), sbk, sbn, _ptr(bias), int(act),
-                                      int(bool(accumulate)), _ptr(out), N, _ptr(pre))
+                                      int(bool(accumulate)), _ptr(out), N, _ptr(pre),
+                                      _ptr(aux))
     _check("ngc_bf16_gemm", rc)
     return out
 

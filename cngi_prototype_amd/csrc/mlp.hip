@@ -10,7 +10,8 @@
 //
 //   ngc_bf16_gemm     C = act(op(A) op(B) + bias), strided A and B (so X^T dY and
 //                     dY W^T need no transposed copies), optional pre-activation
-//                     output, optional accumulate into C
+//                     output, optional accumulate into C; act 2 multiplies by
+//                     the tanh derivative 1 - aux^2 (the backward step fused)
 //   ngc_tanh_backward dZ = dA * (1 - A^2)
 //   ngc_col_sum       column sums (bias gradients)
 //   ngc_mse_grad      d/dy of mean((y - t)^2) and the loss
@@ -35,6 +36,7 @@ __device__ inline short to_bf16(float x) {
 struct GemmArgs {
     const float *A, *B, *bias;
     float *C, *Z;
+    const float *aux;  // act 2: activations whose tanh derivative scales the result
     int64_t M, N, K;
     int64_t sam, sak, sbk, sbn, ldc;
     int act, accumulate;
@@ -48,7 +50,13 @@ constexpr int kGemmLd = kGemmBK + 8;   // LDS row pitch in bf16 (80 B, 8 B align
 __device__ inline void gemm_store(const GemmArgs &g, int64_t row, int64_t col, float z) {
     if (g.bias) z += g.bias[col];
     if (g.Z) g.Z[row * g.ldc + col] = z;
-    const float y = g.act == 1 ? tanhf(z) : z;
+    float y = z;
+    if (g.act == 1) {
+        y = tanhf(z);
+    } else if (g.act == 2) {
+        const float a = g.aux[row * g.ldc + col];
+        y = z * (1.f - a * a);
+    }
     float *c = &g.C[row * g.ldc + col];
     *c = g.accumulate ? *c + y : y;
 }
@@ -281,10 +289,12 @@ extern "C" {
 int ngc_bf16_gemm(ngc_context *ctx, void *stream, int64_t M, int64_t N, int64_t K,
                   const float *A, int64_t sam, int64_t sak, const float *B, int64_t sbk,
                   int64_t sbn, const float *bias, int act, int accumulate, float *C, int64_t ldc,
-                  float *Z) {
+                  float *Z, const float *aux) {
     NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
     NGC_CHECK(M >= 0 && N >= 0 && K >= 0 && ldc >= N, NGC_ERR_ARG, "bad GEMM dimensions");
-    NGC_CHECK(act == 0 || act == 1, NGC_ERR_ARG, "act must be 0 (none) or 1 (tanh)");
+    NGC_CHECK(act == 0 || act == 1 || act == 2, NGC_ERR_ARG,
+              "act must be 0 (none), 1 (tanh) or 2 (times 1 - aux^2)");
+    NGC_CHECK(act != 2 || aux, NGC_ERR_ARG, "act 2 needs aux");
     if (M == 0 || N == 0) return NGC_OK;
     NGC_CHECK(A && B && C, NGC_ERR_ARG, "NULL GEMM operand");
     NGC_CHECK(ceil_div(M, kGemmBM) < 65536, NGC_ERR_UNSUPPORTED, "M too large");
@@ -299,7 +309,7 @@ int ngc_bf16_gemm(ngc_context *ctx, void *stream, int64_t M, int64_t N, int64_t 
         splits = std::min<int64_t>(ksteps / 4, ceil_div(2 * (int64_t)ctx->n_cu, tiles));
     const int64_t k_chunk = ceil_div(ksteps, splits) * kGemmBK;
     splits = std::max<int64_t>(1, ceil_div(K, k_chunk));
-    GemmArgs g{A, B, bias, C, Z, M, N, K, sam, sak, sbk, sbn, ldc, act, accumulate,
+    GemmArgs g{A, B, bias, C, Z, aux, M, N, K, sam, sak, sbk, sbn, ldc, act, accumulate,
                K > 0 ? k_chunk : kGemmBK, nullptr};
     hipStream_t s = (hipStream_t)stream;
     if (splits > 1) {

@@ -46,14 +46,33 @@ class KernelSurrogate:
         import torch
         dims = [n_in] + [width] * (depth - 1) + [n_out]
         gen = torch.Generator().manual_seed(seed)
-        self.W, self.b = [], []
+        # parameters, Adam moments and gradients each live in one flat buffer
+        # (W_1, b_1, W_2, ...), so one Adam update covers the whole network
+        sizes = []
+        for i in range(depth):
+            sizes += [(dims[i], dims[i + 1]), (dims[i + 1],)]
+        total = sum(math.prod(sz) for sz in sizes)
+        self.flat = torch.zeros(total, dtype=torch.float32, device="cuda")
+        self.m_flat = torch.zeros_like(self.flat)
+        self.v_flat = torch.zeros_like(self.flat)
+        self.g_flat = torch.zeros_like(self.flat)
+
+        def views(buf):
+            out, o = [], 0
+            for sz in sizes:
+                n = math.prod(sz)
+                out.append(buf[o:o + n].view(sz))
+                o += n
+            return out
+        pv = views(self.flat)
+        self.W, self.b = pv[0::2], pv[1::2]
         for i in range(depth):
             lim = math.sqrt(6.0 / (dims[i] + dims[i + 1]))
             w = (torch.rand((dims[i], dims[i + 1]), generator=gen, dtype=torch.float32) * 2 - 1)
-            self.W.append((w * lim).cuda())
-            self.b.append(torch.zeros(dims[i + 1], dtype=torch.float32, device="cuda"))
-        self.m = [torch.zeros_like(p) for p in self.params()]
-        self.v = [torch.zeros_like(p) for p in self.params()]
+            self.W[i].copy_((w * lim).cuda())
+        self.m = views(self.m_flat)
+        self.v = views(self.v_flat)
+        self.grads = views(self.g_flat)
         self.steps = 0
         self.dims = dims
         self.meta = {}
@@ -77,17 +96,20 @@ class KernelSurrogate:
             acts.append(_native.bf16_gemm(acts[-1], w, bias=b, act=0 if last else 1))
         return acts
 
-    def loss_and_grads(self, x, t):
-        """Mean squared error (device scalar) and gradients [dW1, db1, ...]."""
+    def loss_and_grads(self, x, t, out=None):
+        """Mean squared error (device scalar) and gradients [dW1, db1, ...]
+        (written into `out`, a list of tensors of the parameters' shapes, when
+        given).  The tanh derivative of each hidden layer is applied in the
+        epilogue of the GEMM that propagates the gradient to it."""
         acts = self.forward(x)
         dz, loss = _native.mse_grad(acts[-1], t)
-        grads = [None] * (2 * len(self.W))
+        grads = [None] * (2 * len(self.W)) if out is None else out
         for i in range(len(self.W) - 1, -1, -1):
-            grads[2 * i] = _native.bf16_gemm(acts[i], dz, trans_a=True)
-            grads[2 * i + 1] = _native.col_sum(dz)
+            grads[2 * i] = _native.bf16_gemm(acts[i], dz, trans_a=True,
+                                             out=None if out is None else out[2 * i])
+            grads[2 * i + 1] = _native.col_sum(dz, out=None if out is None else out[2 * i + 1])
             if i > 0:
-                da = _native.bf16_gemm(dz, self.W[i], trans_b=True)
-                dz = _native.tanh_backward(da, acts[i])
+                dz = _native.bf16_gemm(dz, self.W[i], trans_b=True, act=2, aux=acts[i])
         return loss, grads
 
     def train(self, x, t, steps=3000, batch=4096, lr=2e-3, lr_min=2e-5, seed=1, log_every=0,
@@ -111,10 +133,9 @@ class KernelSurrogate:
 
         def one_step():
             _native.mlp_sample_rows(x, t, b, seed, counter, xb, tb)
-            loss, grads = self.loss_and_grads(xb, tb)
+            loss, _ = self.loss_and_grads(xb, tb, out=self.grads)
             _native.adam_schedule(counter, sched, lr, lr_min, steps, step0)
-            for p, g, m, v in zip(self.params(), grads, self.m, self.v):
-                _native.adam_step_sched(p, g, m, v, sched)
+            _native.adam_step_sched(self.flat, self.g_flat, self.m_flat, self.v_flat, sched)
             return loss
 
         hist = []

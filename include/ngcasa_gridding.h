@@ -405,8 +405,12 @@ int ngc_grid_plan_destroy(ngc_grid_plan *plan);
  * accumulation.
  *   ngc_bf16_gemm      C[m][n] = act(sum_k A(m,k) B(k,n) + bias[n]) (+ C[m][n] when
  *                      accumulate), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn],
- *                      act 0 = identity, 1 = tanh; Z (or NULL) receives the
- *                      pre-activation values (row stride ldc)
+ *                      act 0 = identity, 1 = tanh, 2 = times the tanh derivative
+ *                      1 - aux[m][n]^2 (aux: activations, row stride ldc; the
+ *                      backward step of a tanh layer fused); Z (or NULL)
+ *                      receives the pre-activation values (row stride ldc).
+ *                      Output blocks that cannot fill the device split K; the
+ *                      partial products are summed in split order.
  *   ngc_tanh_backward  dZ = dA * (1 - A^2) for A = tanh(Z)
  *   ngc_col_sum        out[n] = sum_m X[m][n]   (X is M x N, row major)
  *   ngc_mse_grad       g = 2 (y - t) / n, *loss = mean (y - t)^2 (device double)
@@ -415,7 +419,7 @@ int ngc_grid_plan_destroy(ngc_grid_plan *plan);
 int ngc_bf16_gemm(ngc_context *ctx, void *stream, int64_t M, int64_t N, int64_t K,
                   const float *A, int64_t sam, int64_t sak, const float *B, int64_t sbk,
                   int64_t sbn, const float *bias, int act, int accumulate, float *C, int64_t ldc,
-                  float *Z);
+                  float *Z, const float *aux);
 int ngc_tanh_backward(ngc_context *ctx, void *stream, int64_t n, const float *dA, const float *A,
                       float *dZ);
 int ngc_col_sum(ngc_context *ctx, void *stream, int64_t M, int64_t N, const float *X,
