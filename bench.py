@@ -13,6 +13,13 @@ binning.  Briggs imaging weights are computed once, on the GPU, before the
 timed region (also timed and reported).  The dirty-image error of the chosen
 form is measured against the FP64 form on the same inputs, per plane.
 
+Further legs, reported in the same JSON line under their own keys (each can
+be skipped with --no-<leg> or run alone with --<leg>-only): 'mosaic' (C4,
+tiled aperture gridder against the plain FP64 one, and the learned A-term
+surrogate), 'batched' (C3, the autotuned hipGraph gridding plan), 'streamed'
+(the C5 one-GPU share from pinned host memory) and 'predict' (visibility
+prediction at the C2 shapes).
+
 Prints one JSON line (see the keys below).  --gpus N > 1 is refused: this
 build runs on one GPU.
 """
@@ -795,9 +802,11 @@ def main():
                                f"support 7, oversampling 100, 10% flags",
                    "samples": int(cfg.n_vis), "unflagged": n_unflagged,
                    "planes": int(n_planes),
-                   "form": f"{args.precision} (uv-tiled bins, one wavefront per "
-                           f"{args.tile_rows or 16}x32-cell tile, lane-owned residue "
-                           f"classes, register runs flushed into an LDS subgrid)"},
+                   "form": f"{args.precision} (uv-tiled bins, a workgroup of "
+                           f"{args.tile_waves or 2} wavefront(s) per "
+                           f"{args.tile_rows or 16}x32-cell tile splitting the pols, "
+                           f"lane-owned residue classes, FP32 register runs added to an "
+                           f"LDS subgrid (FP64 cells for fp32_kahan))"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baseline_1thread": cpu1,
