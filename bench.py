@@ -590,6 +590,8 @@ def main():
     ap.add_argument("--tile-waves", type=int, default=0, choices=[0, 1, 2, 4],
                     help="wavefronts per tile of the gridder, splitting the pols "
                          "(0: library default)")
+    ap.add_argument("--bin-lanes", type=int, default=0, choices=[0, 1, 2, 4, 8],
+                    help="channels per binning wavefront (0: library default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the steps, run the grid's ifftshift (a plain copy kernel of "
                          "known size) so that a counter run sees a kernel of known traffic")
@@ -612,6 +614,8 @@ def main():
     torch.cuda.set_device(0)
     if args.tile_waves:
         nat.set_tile_waves(args.tile_waves)
+    if args.bin_lanes:
+        nat.set_bin_lanes(args.bin_lanes)
     if args.mosaic_only:
         print(json.dumps({"mosaic": mosaic_leg(args)}), file=out, flush=True)
         return

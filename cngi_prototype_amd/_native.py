@@ -35,7 +35,7 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step",
            "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags", "ngc_set_tile_waves",
            "ngc_model_to_grid", "ngc_standard_degrid", "ngc_mlp_sample_rows",
-           "ngc_adam_schedule", "ngc_adam_step_sched")
+           "ngc_adam_schedule", "ngc_adam_step_sched", "ngc_set_bin_lanes")
 
 
 class NativeError(RuntimeError):
@@ -72,6 +72,7 @@ _PROTOS = {
     "ngc_profile_enable": (_i32, [_vp, _i32]),
     "ngc_set_tile_rows": (_i32, [_vp, _i32]),
     "ngc_set_tile_waves": (_i32, [_vp, _i32]),
+    "ngc_set_bin_lanes": (_i32, [_vp, _i32]),
     "ngc_gcf_patterns": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _i32, _i32, _d,
                                 _d, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "ngc_gcf_resize": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
@@ -497,6 +498,13 @@ def set_tile_waves(waves):
     4 pols and 1 otherwise; 1, 2 or 4) on the current device."""
     ctx = context()
     _check("ngc_set_tile_waves", load_library().ngc_set_tile_waves(ctx.handle, int(waves)))
+
+
+def set_bin_lanes(lanes):
+    """Channels per binning wavefront of the uv-tiled standard gridder (1, 2, 4
+    or 8) on the current device."""
+    ctx = context()
+    _check("ngc_set_bin_lanes", load_library().ngc_set_bin_lanes(ctx.handle, int(lanes)))
 
 
 def gcf_patterns(pb_freq, dish, blockage, pairs, casa, ipower, cell_lm, n_uv):

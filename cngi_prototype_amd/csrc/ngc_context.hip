@@ -106,6 +106,15 @@ int ngc_set_tile_rows(ngc_context *ctx, int tile_rows) {
     return NGC_OK;
 }
 
+int ngc_set_bin_lanes(ngc_context *ctx, int lanes) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    NGC_CHECK(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8, NGC_ERR_ARG,
+              "lanes must be 1, 2, 4 or 8");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->bin_chan_lanes = lanes;
+    return NGC_OK;
+}
+
 int ngc_set_tile_waves(ngc_context *ctx, int waves) {
     NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
     NGC_CHECK(waves == 0 || waves == 1 || waves == 2 || waves == 4, NGC_ERR_ARG,

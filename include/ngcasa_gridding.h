@@ -472,6 +472,14 @@ int ngc_set_tile_rows(ngc_context *ctx, int tile_rows);
 int ngc_set_tile_waves(ngc_context *ctx, int waves);
 
 /*
+ * Channels per binning wavefront of the uv-tiled standard gridder (no
+ * counterpart in the reference): a wavefront bins 64/lanes integrations x
+ * `lanes` adjacent channels of one baseline (1, 2, 4 or 8; default 2).  The
+ * plan autotuner chooses it per shape.
+ */
+int ngc_set_bin_lanes(ngc_context *ctx, int lanes);
+
+/*
  * Phase timing (instrumentation; no counterpart in the reference, which has
  * no timing inside its kernels).  When enabled, each gridding call records
  * HIP events on its stream around its phases; ngc_profile_read synchronises
