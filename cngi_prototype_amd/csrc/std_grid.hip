@@ -500,7 +500,8 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     constexpr int AXROW = 9;
     __shared__ __align__(16) AxW s_ax[2][kBatch][AXROW];
     __shared__ WD s_wd[W][kBatch];
-    __shared__ ACC s_k[512];
+    __shared__ ACC s_k[513];          // the kernel table; zero from cgk_len on
+    __shared__ unsigned s_chg[W];     // per wavefront: its samples whose window moved
     __shared__ int s_item;
     __shared__ int64_t s_tq[kChunk][3];   // bin, first and end record of each tile of a ticket
 
@@ -518,6 +519,7 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         }
 #pragma unroll
         for (int k = 0; k < KL; k++) s_k[tid + k * NT] = kv[k];
+        if (tid == 0) s_k[512] = (ACC)0;   // where taps past the table clamp
     }
 #pragma unroll 4
     for (int i = tid; i < NPC * PLANE; i += NT) s_acc[i] = CELL{};
@@ -597,69 +599,105 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         }
     };
 
-    // staging lanes: lane l < 2 SPW of wavefront w stages axis l / SPW of
-    // sample j = w SPW + l % SPW (entry b0 + j) for the batch at b0; the next
-    // batch's entries are loaded while the current one is gridded
-    const int js = lane % SPW, axis = lane / SPW;
+    // staging: every lane of every wavefront stages taps it = h RPL ..
+    // h RPL + RPL - 1 of one axis of one sample of the batch at b0: sample
+    // j = w SPW + js (entry b0 + j) of wavefront w, with js = l % SPW,
+    // axis = (l / SPW) % 2 and h = l / (2 SPW) for lane l.  Tap `it` of a
+    // window starting at c0 is the cell c0 + it, owned by residue
+    // (c0 + it) mod 8.  The next batch's entries are loaded while the current
+    // one is gridded.
+    constexpr int H = 64 / (2 * SPW);   // lanes per (sample, axis) = W
+    constexpr int RPL = 8 / H;          // taps per lane
+    static_assert(H * RPL == 8, "the taps of an axis split evenly over its lanes");
+    const int js = lane % SPW, axis = (lane / SPW) & 1, h = lane / (2 * SPW);
     const int j_st = wave * SPW + js;
-    const bool stager = lane < 2 * SPW;
-    E en{};
-    unsigned int puv = 0;
+    const bool wd_lane = axis == 0 && h == 0;   // also stages the weighted visibilities
+    const int shift = axis ? 16 : 0;
+    const int lo = (axis ? v_lo : u_lo) + sc;
+    const int it0 = h * RPL;
+    int2 hdr = make_int2(0, 0);   // (uv, duv) of the lane's entry
+    unsigned int puv = 0;         // uv of the entry before it
+    WD wdl[W];
     auto load_batch = [&](int64_t b) {
         const int64_t e = b + j_st;
-        if (stager && e < end) {
-            en = *reinterpret_cast<const E *>(entries + e * EB);
-            const int64_t pe = e > start ? e - 1 : e;
-            puv = reinterpret_cast<const E *>(entries + pe * EB)->uv;
+        if (e < end) {
+            const char *p = entries + e * EB;
+            hdr = *reinterpret_cast<const int2 *>(p);
+            puv = e > start ? *reinterpret_cast<const unsigned int *>(p - EB) : 0u;
+            if (wd_lane) {
+                const E *ep = reinterpret_cast<const E *>(p);
+#pragma unroll
+                for (int gw = 0; gw < W; gw++)
+#pragma unroll
+                    for (int q = 0; q < QW; q++) {
+                        const int pq = gw * QW + q;
+                        wdl[gw].v[q] = ep->wd[pq / NC][pq % NC];
+                    }
+            }
         }
     };
     load_batch(start);
     for (int64_t b0 = start; b0 < end; b0 += kBatch) {
         const int64_t e = b0 + j_st;
-        const bool valid = stager && e < end;
-        if (stager) {
-            // this lane's axis: window start, previous sample's window start and
-            // sub-cell offset (previous = none for the first entry of the bin)
-            const int shift = axis ? 16 : 0;
-            const int lo = axis ? v_lo : u_lo;
-            int c0 = -1000, pc0 = -1000, d = 0;
+        const bool valid = e < end;
+        {
+            // this lane's axis: window start c0, its shift from the previous
+            // sample's window (any nonzero value for the first entry of the
+            // bin) and the sub-cell offset.  Past the end of the bin the
+            // window does not move and the weighted visibilities are zero, so
+            // the lanes keep (and finally flush) the last real cell.
+            int c0 = -1000, dlt = 0, d = 0;
             if (valid) {
-                c0 = (int)((en.uv >> shift) & 0xffff) - sc - lo;
-                d = (int)(short)((en.duv >> shift) & 0xffff);
-                if (e > start) pc0 = (int)((puv >> shift) & 0xffff) - sc - lo;
+                c0 = (int)(((unsigned)hdr.x >> shift) & 0xffffu) - lo;
+                d = (int)(short)(((unsigned)hdr.y >> shift) & 0xffffu);
+                dlt = e > start ? c0 - ((int)((puv >> shift) & 0xffffu) - lo) : 1000;
             }
-            AxW row[8];
+            // residue (c0 + it) mod 8 changes cell iff the shift moves the
+            // window by it or more cells: unsigned(dlt + it) >= 8
+            // all table reads first (k <= 512 is always in bounds), then the
+            // records; s_k is zero from cgk_len on
+            const int x0 = os * (it0 - sc) + d;
+            ACC tk[RPL];
 #pragma unroll
-            for (int r = 0; r < 8; r++) {
-                const int cell = residue_cell(c0, r);
-                const int it = cell - c0;
-                const int k = abs(os * (it - sc) + d);
-                const ACC t = s_k[k < 511 ? k : 511];   // the table is zero past cgk_len
-                ACC w = (valid && it < S && k < cgk_len) ? t : (ACC)0;
-                if (cell != residue_cell(pc0, r)) w = -w - (ACC)0;  // sign bit set, even for 0
-                row[r].w = w;
-                row[r].cell = cell + 64;
+            for (int i = 0; i < RPL; i++) {
+                const int x = x0 + os * i;
+                tk[i] = s_k[min(x < 0 ? -x : x, 512)];
             }
 #pragma unroll
-            for (int r = 0; r < 8; r++) s_ax[axis][j_st][r] = row[r];
-            if (axis == 0) {
-                // weighted visibilities; zero past the end of the bin, where the
-                // lanes keep (and finally flush) the last real cell
+            for (int i = 0; i < RPL; i++) {
+                const int it = it0 + i;
+                const ACC t = it < S ? tk[i] : (ACC)0;
+                const bool moved = (unsigned)(dlt + it) >= 8u;
+                AxW rec;
+                rec.w = moved ? -t - (ACC)0 : t;   // sign bit set, even for 0
+                rec.cell = c0 + it + 64;
+                s_ax[axis][j_st][(c0 + it) & 7] = rec;
+            }
+            if (wd_lane) {
 #pragma unroll
                 for (int gw = 0; gw < W; gw++) {
                     WD w;
 #pragma unroll
-                    for (int q = 0; q < QW; q++) {
-                        const int pq = gw * QW + q;
-                        w.v[q] = valid ? en.wd[pq / NC][pq % NC] : (ACC)0;
-                    }
+                    for (int q = 0; q < QW; q++) w.v[q] = valid ? wdl[gw].v[q] : (ACC)0;
                     s_wd[gw][j_st] = w;
                 }
+            }
+            // samples whose window moved on either axis: one bit per sample
+            // of this wavefront's share, from the it = 0 lanes
+            const uint64_t mv = __ballot(h == 0 && dlt != 0);
+            if (lane == 0) {
+                const uint64_t m = (mv | (mv >> SPW)) & (~0ull >> (64 - SPW));
+                s_chg[wave] = (unsigned)(m << (wave * SPW));
             }
         }
         const int64_t nleft = end - b0;
         const int nh = nleft >= kBatch ? kBatch : (int)nleft;
         __syncthreads();
+        unsigned chg_v = 0;
+#pragma unroll
+        for (int gw = 0; gw < W; gw++) chg_v |= s_chg[gw];
+        // wave-uniform: the per-sample cell checks below are scalar branches
+        const unsigned chg = __builtin_amdgcn_readfirstlane(chg_v);
         if (b0 + kBatch < end) load_batch(b0 + kBatch);
 
         // groups of kGroup samples, double-buffered: the next group's LDS reads
@@ -676,29 +714,36 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
                 wd[jj] = s_wd[wave][gi * kGroup + jj];
             }
         };
-        auto run_group = [&](const AxW (&au)[kGroup], const AxW (&av)[kGroup],
+        auto run_group = [&](int gi, const AxW (&au)[kGroup], const AxW (&av)[kGroup],
                              const WD (&wd)[kGroup]) {
 #pragma unroll
             for (int jj = 0; jj < kGroup; jj++) {
-                if (signbit(au[jj].w) || signbit(av[jj].w)) {
-                    flush();
-                    cur_u = au[jj].cell - 64;
-                    cur_v = av[jj].cell - 64;
+                if ((chg >> (gi * kGroup + jj)) & 1u) {
+                    if (signbit(au[jj].w) || signbit(av[jj].w)) {
+                        flush();
+                        cur_u = au[jj].cell - 64;
+                        cur_v = av[jj].cell - 64;
 #pragma unroll
-                    for (int q = 0; q < QW; q++) acc[q] = (ACC)0;
+                        for (int q = 0; q < QW; q++) acc[q] = (ACC)0;
+                    }
                 }
                 const ACC wt = fabs(au[jj].w) * fabs(av[jj].w);
 #pragma unroll
                 for (int q = 0; q < QW; q++) acc[q] += wt * wd[jj].v[q];
             }
         };
+        // The prefetch is unconditional (past the last group it reloads
+        // group kBatch / kGroup - 1, whose staged records are in bounds and
+        // unused), so the compiler knows how many LDS reads are in flight and
+        // waits only for the current group's, not for the prefetch.
+        constexpr int kLastGroup = kBatch / kGroup - 1;
         load_group(0, au0, av0, wd0);
         for (int gi = 0; gi < ng; gi += 2) {
-            if (gi + 1 < ng) load_group(gi + 1, au1, av1, wd1);
-            run_group(au0, av0, wd0);
+            load_group(min(gi + 1, kLastGroup), au1, av1, wd1);
+            run_group(gi, au0, av0, wd0);
             if (gi + 1 >= ng) break;
-            if (gi + 2 < ng) load_group(gi + 2, au0, av0, wd0);
-            run_group(au1, av1, wd1);
+            load_group(min(gi + 2, kLastGroup), au0, av0, wd0);
+            run_group(gi + 1, au1, av1, wd1);
         }
         __syncthreads();
     }
