@@ -489,16 +489,14 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     using CELL = typename std::conditional<KAHAN, double, ACC>::type;
     constexpr int PLANE = TU * kTile;
     __shared__ CELL s_acc[NPC * PLANE];
-    // per staged sample, axis (u, v) and residue: the tap weight (sign bit set
-    // when the residue's cell differs from the previous sample's) and the
-    // cell (tile-relative + 64); rows padded so the staging writes spread
-    // over the banks
-    struct AxW {
-        ACC w;
-        int cell;
-    };
+    // per staged sample, axis (u, v) and residue: the tap weight, sign bit
+    // set when the residue's cell differs from the previous sample's; rows
+    // padded so the staging writes spread over the banks.  The cells follow
+    // from the sample's window start (tile-relative, per axis), which the
+    // lanes need only when the window moved.
     constexpr int AXROW = 9;
-    __shared__ __align__(16) AxW s_ax[2][kBatch][AXROW];
+    __shared__ ACC s_ax[2][kBatch][AXROW];
+    __shared__ int2 s_c0[kBatch];
     __shared__ WD s_wd[W][kBatch];
     __shared__ ACC s_k[513];          // the kernel table; zero from cgk_len on
     __shared__ unsigned s_chg[W];     // per wavefront: its samples whose window moved
@@ -668,11 +666,10 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
                 const int it = it0 + i;
                 const ACC t = it < S ? tk[i] : (ACC)0;
                 const bool moved = (unsigned)(dlt + it) >= 8u;
-                AxW rec;
-                rec.w = moved ? -t - (ACC)0 : t;   // sign bit set, even for 0
-                rec.cell = c0 + it + 64;
-                s_ax[axis][j_st][(c0 + it) & 7] = rec;
+                // sign bit set, even for 0
+                s_ax[axis][j_st][(c0 + it) & 7] = moved ? -t - (ACC)0 : t;
             }
+            if (h == 0) reinterpret_cast<int *>(&s_c0[j_st])[axis] = c0;
             if (wd_lane) {
 #pragma unroll
                 for (int gw = 0; gw < W; gw++) {
@@ -704,30 +701,40 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         // are issued before this group's flushes, so they do not queue
         // behind its adds
         const int ng = (nh + kGroup - 1) / kGroup;
-        AxW au0[kGroup], av0[kGroup], au1[kGroup], av1[kGroup];
+        // (lane l also holds the window start of sample l % kGroup of the
+        // group, which a moved sample's new cells are taken from)
+        ACC au0[kGroup], av0[kGroup], au1[kGroup], av1[kGroup];
         WD wd0[kGroup], wd1[kGroup];
-        auto load_group = [&](int gi, AxW (&au)[kGroup], AxW (&av)[kGroup], WD (&wd)[kGroup]) {
+        int2 c00, c01;
+        auto load_group = [&](int gi, ACC (&au)[kGroup], ACC (&av)[kGroup], WD (&wd)[kGroup],
+                              int2 &c0l) {
 #pragma unroll
             for (int jj = 0; jj < kGroup; jj++) {
                 au[jj] = s_ax[0][gi * kGroup + jj][ra];
                 av[jj] = s_ax[1][gi * kGroup + jj][rb];
                 wd[jj] = s_wd[wave][gi * kGroup + jj];
             }
+            c0l = s_c0[gi * kGroup + (lane & (kGroup - 1))];
         };
-        auto run_group = [&](int gi, const AxW (&au)[kGroup], const AxW (&av)[kGroup],
-                             const WD (&wd)[kGroup]) {
+        auto run_group = [&](int gi, const ACC (&au)[kGroup], const ACC (&av)[kGroup],
+                             const WD (&wd)[kGroup], const int2 &c0l) {
 #pragma unroll
             for (int jj = 0; jj < kGroup; jj++) {
                 if ((chg >> (gi * kGroup + jj)) & 1u) {
-                    if (signbit(au[jj].w) || signbit(av[jj].w)) {
+                    // the new window; reading it first also keeps this a
+                    // scalar branch around the per-lane test below
+                    const int wu = __builtin_amdgcn_readlane(c0l.x, jj);
+                    const int wv = __builtin_amdgcn_readlane(c0l.y, jj);
+                    if (signbit(au[jj]) || signbit(av[jj])) {
                         flush();
-                        cur_u = au[jj].cell - 64;
-                        cur_v = av[jj].cell - 64;
+                        // the cell of residue (ra, rb) in the new window
+                        cur_u = residue_cell(wu, ra);
+                        cur_v = residue_cell(wv, rb);
 #pragma unroll
                         for (int q = 0; q < QW; q++) acc[q] = (ACC)0;
                     }
                 }
-                const ACC wt = fabs(au[jj].w) * fabs(av[jj].w);
+                const ACC wt = fabs(au[jj]) * fabs(av[jj]);
 #pragma unroll
                 for (int q = 0; q < QW; q++) acc[q] += wt * wd[jj].v[q];
             }
@@ -737,13 +744,13 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         // unused), so the compiler knows how many LDS reads are in flight and
         // waits only for the current group's, not for the prefetch.
         constexpr int kLastGroup = kBatch / kGroup - 1;
-        load_group(0, au0, av0, wd0);
+        load_group(0, au0, av0, wd0, c00);
         for (int gi = 0; gi < ng; gi += 2) {
-            load_group(min(gi + 1, kLastGroup), au1, av1, wd1);
-            run_group(gi, au0, av0, wd0);
+            load_group(min(gi + 1, kLastGroup), au1, av1, wd1, c01);
+            run_group(gi, au0, av0, wd0, c00);
             if (gi + 1 >= ng) break;
-            load_group(min(gi + 2, kLastGroup), au0, av0, wd0);
-            run_group(gi + 1, au1, av1, wd1);
+            load_group(min(gi + 2, kLastGroup), au0, av0, wd0, c00);
+            run_group(gi + 1, au1, av1, wd1, c01);
         }
         __syncthreads();
     }
