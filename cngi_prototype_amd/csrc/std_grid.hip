@@ -690,17 +690,12 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         const int64_t nleft = end - b0;
         const int nh = nleft >= kBatch ? kBatch : (int)nleft;
         __syncthreads();
-        unsigned chg_v = 0;
-#pragma unroll
-        for (int gw = 0; gw < W; gw++) chg_v |= s_chg[gw];
-        // wave-uniform: the per-sample cell checks below are scalar branches
-        const unsigned chg = __builtin_amdgcn_readfirstlane(chg_v);
-        if (b0 + kBatch < end) load_batch(b0 + kBatch);
 
         // groups of kGroup samples, double-buffered: the next group's LDS reads
         // are issued before this group's flushes, so they do not queue
         // behind its adds
         const int ng = (nh + kGroup - 1) / kGroup;
+        unsigned chg = 0;   // samples whose window moved, one bit each
         // (lane l also holds the window start of sample l % kGroup of the
         // group, which a moved sample's new cells are taken from)
         ACC au0[kGroup], av0[kGroup], au1[kGroup], av1[kGroup];
@@ -745,6 +740,13 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         // waits only for the current group's, not for the prefetch.
         constexpr int kLastGroup = kBatch / kGroup - 1;
         load_group(0, au0, av0, wd0, c00);
+        // (read with the first group's records: one wait for both)
+        unsigned chg_v = 0;
+#pragma unroll
+        for (int gw = 0; gw < W; gw++) chg_v |= s_chg[gw];
+        // wave-uniform: the per-sample cell checks below are scalar branches
+        chg = __builtin_amdgcn_readfirstlane(chg_v);
+        if (b0 + kBatch < end) load_batch(b0 + kBatch);
         for (int gi = 0; gi < ng; gi += 2) {
             load_group(min(gi + 1, kLastGroup), au1, av1, wd1, c01);
             run_group(gi, au0, av0, wd0, c00);
