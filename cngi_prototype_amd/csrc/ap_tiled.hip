@@ -66,6 +66,7 @@ struct ApT {
     int64_t ntu, ntv;
     int ch_log2;
     int64_t n_tb, n_cb;
+    FastDiv div_cb, div_tb;  // by n_cb and n_tb (n_baseline n_tb n_cb < 2^32)
     int half_u, half_v;    // sub-cell offset index range [-half, half]
     int vis_dtype, weight_dtype;
     int *error;
@@ -198,11 +199,13 @@ struct ApSample {
 __device__ inline bool ap_sample_at(const ApT &a, int64_t wave_id, int lane, ApSample &sm) {
     const int chl = 1 << a.ch_log2;
     const int tl = 64 >> a.ch_log2;
-    const int64_t cb = wave_id % a.n_cb;
-    const int64_t r = wave_id / a.n_cb;
-    const int64_t tb = r % a.n_tb;
-    const int64_t b = r / a.n_tb;
-    sm.t = tb * tl + (lane >> a.ch_log2);
+    // wave_id is the same in every lane: the divisions are scalar work
+    const uint32_t wid = (uint32_t)wave_id;
+    const uint32_t r = a.div_cb.div(wid);
+    const uint32_t cb = wid - r * a.div_cb.d;
+    const uint32_t b = a.div_tb.div(r);
+    const uint32_t tb = r - b * a.div_tb.d;
+    sm.t = (int64_t)tb * tl + (lane >> a.ch_log2);
     sm.ch = cb * chl + (lane & (chl - 1));
     sm.b = b;
     if (b >= a.n_baseline || sm.t >= a.n_time || sm.ch >= a.n_chan) return false;
@@ -264,7 +267,7 @@ ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *n
     const int nwu = 2 * a.half_u + 1, nwv = 2 * a.half_v + 1;
     for (int64_t w0 = (int64_t)blockIdx.x * waves_per_block; w0 < n_waves;
          w0 += (int64_t)gridDim.x * waves_per_block) {
-        const int64_t wave_id = w0 + (threadIdx.x >> 6);
+        const int64_t wave_id = w0 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
         ApSample sm{};
         Pix px{}, pxv{};
         int fidx = 0;
@@ -855,6 +858,10 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     a.ch_log2 = ch_log2;
     a.n_cb = ceil_div(c.n_chan, int64_t(1) << ch_log2);
     a.n_tb = ceil_div(c.n_time, int64_t(64) >> ch_log2);
+    NGC_CHECK(c.n_baseline * a.n_tb * a.n_cb < (int64_t(1) << 32), NGC_ERR_UNSUPPORTED,
+              "too many (baseline, time block, channel block) slots for one call");
+    a.div_cb = FastDiv::of((uint32_t)a.n_cb);
+    a.div_tb = FastDiv::of((uint32_t)a.n_tb);
 
     const int64_t n_sw = c.n_imag_chan * c.n_imag_pol;
     const int64_t n_rc = c.n_time * c.n_baseline * c.n_chan;

@@ -182,6 +182,33 @@ template <typename T> __device__ inline double ld_weight(const void *p, int64_t 
     return (double)reinterpret_cast<const T *>(p)[i];
 }
 
+// Unsigned 32-bit division by a divisor fixed at launch, as a multiply-high,
+// a subtract and two shifts (T. Granlund and P. L. Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, the round-up method of
+// their figure 4.1): with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0) where t = mulhi(m, n),
+// exact for every n < 2^32.
+struct FastDiv {
+    uint32_t d = 1, m = 1;
+    int s1 = 0, s2 = 0;
+    // d = 0 (an empty axis, where nothing is divided) is taken as 1
+    static FastDiv of(uint32_t d) {
+        FastDiv f;
+        if (d == 0) d = 1;
+        f.d = d;
+        int l = 0;
+        while ((uint64_t(1) << l) < d) l++;
+        f.m = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+        f.s1 = l < 1 ? l : 1;
+        f.s2 = l > 1 ? l - 1 : 0;
+        return f;
+    }
+    __host__ __device__ uint32_t div(uint32_t n) const {
+        const uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
+        return (t + ((n - t) >> s1)) >> s2;
+    }
+};
+
 // Pixel position of one (row, channel) sample, with the reference's exact
 // FP64 operation order and rounding (_standard_grid.py:299-324):
 //   u = uvw_u*scale; u_pos = u + n_u//2; centre = int(u_pos+0.5) (truncation);

@@ -31,6 +31,8 @@
 namespace ngc {
 
 constexpr int kTile = 32;          // tile edge in cells (both axes)
+constexpr int kTileLog2 = 5;
+static_assert(kTile == 1 << kTileLog2, "kTileLog2");
 constexpr int kBlock = 256;        // threads per workgroup
 constexpr int kMaxTiledSupport = 8;
 
@@ -179,8 +181,10 @@ std_grid_plain_kernel(StdArgs a, const double *cgk, int64_t cgk_len, double *gri
 struct BinGeom {
     int64_t ntu, ntv;      // tiles per axis
     int tu;                // tile rows (u extent); the v extent is kTile
+    int tu_log2;           // log2(tu)
     int ch_log2;           // log2(ch_lanes)
     int64_t n_tb, n_cb;    // time blocks, channel blocks
+    FastDiv div_cb, div_tb;  // by n_cb and n_tb (n_baseline n_tb n_cb < 2^32)
 };
 
 struct Sample {
@@ -191,11 +195,13 @@ __device__ inline bool sample_at(const StdArgs &a, const BinGeom &g, int64_t wav
                                  Sample &sm) {
     const int chl = 1 << g.ch_log2;
     const int tl = 64 >> g.ch_log2;
-    const int64_t cb = wave_id % g.n_cb;
-    const int64_t r = wave_id / g.n_cb;
-    const int64_t tb = r % g.n_tb;
-    const int64_t b = r / g.n_tb;
-    sm.t = tb * tl + (lane >> g.ch_log2);
+    // wave_id is the same in every lane: the divisions are scalar work
+    const uint32_t wid = (uint32_t)wave_id;
+    const uint32_t r = g.div_cb.div(wid);
+    const uint32_t cb = wid - r * g.div_cb.d;
+    const uint32_t b = g.div_tb.div(r);
+    const uint32_t tb = r - b * g.div_tb.d;
+    sm.t = (int64_t)tb * tl + (lane >> g.ch_log2);
     sm.ch = cb * chl + (lane & (chl - 1));
     sm.b = b;
     if (b >= a.n_baseline || sm.t >= a.n_time || sm.ch >= a.n_chan) return false;
@@ -219,10 +225,12 @@ __device__ inline bool place(const StdArgs &a, const Sample &sm, double u, doubl
 // k-th overlapped tile (k = 0..3) of a placed sample, or -1
 __device__ inline int64_t tile_key(const StdArgs &a, const BinGeom &g, const Pix &px,
                                    int64_t a_chan, int k) {
-    const int64_t sc = a.support / 2;
-    const int64_t tu0 = (px.uc - sc) / g.tu, tu1 = (px.uc + sc) / g.tu;
-    const int64_t tv0 = (px.vc - sc) / kTile, tv1 = (px.vc + sc) / kTile;
-    const int64_t tu = tu0 + (k & 1), tv = tv0 + (k >> 1);
+    // placed samples have 0 <= uc - sc and uc + sc < n_u < 65536 (same for v)
+    const int sc = (int)(a.support / 2);
+    const int uc = (int)px.uc, vc = (int)px.vc;
+    const int tu0 = (uc - sc) >> g.tu_log2, tu1 = (uc + sc) >> g.tu_log2;
+    const int tv0 = (vc - sc) >> kTileLog2, tv1 = (vc + sc) >> kTileLog2;
+    const int tu = tu0 + (k & 1), tv = tv0 + (k >> 1);
     if (tu > tu1 || tv > tv1) return -1;
     return (a_chan * g.ntu + tu) * g.ntv + tv;
 }
@@ -263,7 +271,7 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
     const int nw = 2 * norm_half + 1;
     for (int64_t w0 = (int64_t)blockIdx.x * waves_per_block; w0 < n_waves;
          w0 += (int64_t)gridDim.x * waves_per_block) {
-        const int64_t wave_id = w0 + (threadIdx.x >> 6);
+        const int64_t wave_id = w0 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
         Sample sm{};
         Pix px{};
         const bool have = (wave_id < n_waves) && sample_at(a, g, wave_id, lane, sm);
@@ -1012,6 +1020,8 @@ static TiledGeom tiled_geom(const StdArgs &a, const TiledCfg &cfg) {
     TiledGeom t;
     t.cfg = cfg;
     t.g.tu = cfg.tu;
+    t.g.tu_log2 = 0;
+    while ((1 << t.g.tu_log2) < cfg.tu) t.g.tu_log2++;
     t.g.ntu = ceil_div(a.n_u, cfg.tu);
     t.g.ntv = ceil_div(a.n_v, kTile);
     int ch_log2 = 0;
@@ -1021,6 +1031,8 @@ static TiledGeom tiled_geom(const StdArgs &a, const TiledCfg &cfg) {
     t.g.n_tb = ceil_div(a.n_time, int64_t(64) >> ch_log2);
     t.n_bins = a.n_imag_chan * t.g.ntu * t.g.ntv;
     t.n_waves = a.n_baseline * t.g.n_tb * t.g.n_cb;
+    t.g.div_cb = FastDiv::of((uint32_t)t.g.n_cb);
+    t.g.div_tb = FastDiv::of((uint32_t)t.g.n_tb);
     t.bin_blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(t.n_waves, kBlock / 64),
                                                            1 << 20));
     t.scan_nb = ceil_div(t.n_bins + 1, kScanChunk);
@@ -1063,6 +1075,8 @@ static int tiled_tables(ngc_context *ctx, hipStream_t s, const StdArgs &a, const
 }
 
 static int tiled_workspace(ngc_context *ctx, const TiledGeom &tg, TiledBufs &bf) {
+    NGC_CHECK(tg.n_waves < (int64_t(1) << 32), NGC_ERR_UNSUPPORTED,
+              "too many (baseline, time block, channel block) slots for one call");
     void *p;
     NGC_TRY(workspace(ctx, "bin_counts", (tg.n_bins + 1) * 8, &p));
     bf.counts = (unsigned long long *)p;
