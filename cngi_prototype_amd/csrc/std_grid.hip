@@ -211,12 +211,11 @@ __device__ inline bool sample_at(const StdArgs &a, const BinGeom &g, int64_t wav
 
 // centre cell, offsets and on-grid test of a sample at (u, v) metres, with
 // `any_unflagged` from its flags; false when it is not gridded
-__device__ inline bool place(const StdArgs &a, const Sample &sm, double u, double v,
+// (s0, s1 = the channel's uv scales, loaded with the sample's data)
+__device__ inline bool place(const StdArgs &a, double u, double v, double s0, double s1,
                              bool any_unflagged, Pix &px) {
     if (!any_unflagged) return false;
-    if (!pixel_position(u, v, a.uv_scale[sm.ch], a.uv_scale[a.n_chan + sm.ch], a.n_u, a.n_v,
-                        a.os, px))
-        return false;
+    if (!pixel_position(u, v, s0, s1, a.n_u, a.n_v, a.os, px)) return false;
     const int64_t sc = a.support / 2;
     return (px.uc + sc < a.n_u) && (px.vc + sc < a.n_v) && (px.uc - sc >= 0) &&
            (px.vc - sc >= 0);
@@ -247,10 +246,72 @@ template <int NP, int NC, typename ACC> struct EntrySize {
     static constexpr int bytes = (raw + 15) / 16 * 16;
 };
 
+// The fast load path of the binning passes (FAST = true): uint8 flags, FP32
+// weights and, in image mode, complex64 visibilities, each array aligned to
+// one sample's NP elements (NP = 1, 2 or 4), so that a sample's flags,
+// weights and visibilities are one vector load each, issued together.  The
+// general path branches on the dtypes and alignment, and the compiler then
+// waits for each load before issuing the next.
+template <int NP> struct FastSample {
+    static_assert(NP == 1 || NP == 2 || NP == 4, "fast loads for 1, 2 or 4 pols");
+    using FT = typename std::conditional<
+        NP == 4, uint32_t, typename std::conditional<NP == 2, uint16_t, uint8_t>::type>::type;
+    FT f;
+    float w[NP];
+    float2 v[NP];
+    template <bool FILLV, bool IMAGE> __device__ void load(const StdArgs &a, int64_t base) {
+        f = *reinterpret_cast<const FT *>(a.flag + base);
+        if constexpr (FILLV) {
+            const float *wp = reinterpret_cast<const float *>(a.weight) + base;
+            if constexpr (NP == 4) {
+                const float4 t = *reinterpret_cast<const float4 *>(wp);
+                w[0] = t.x; w[NP > 1 ? 1 : 0] = t.y; w[NP > 2 ? 2 : 0] = t.z; w[NP - 1] = t.w;
+            } else if constexpr (NP == 2) {
+                const float2 t = *reinterpret_cast<const float2 *>(wp);
+                w[0] = t.x; w[NP - 1] = t.y;
+            } else {
+                w[0] = *wp;
+            }
+            if constexpr (IMAGE) {
+                const float2 *vp = reinterpret_cast<const float2 *>(a.vis) + base;
+                if constexpr (NP == 1) {
+                    v[0] = *vp;
+                } else {
+#pragma unroll
+                    for (int h = 0; h < NP / 2; h++) {
+                        const float4 t = reinterpret_cast<const float4 *>(vp)[h];
+                        v[2 * h] = make_float2(t.x, t.y);
+                        v[2 * h + 1] = make_float2(t.z, t.w);
+                    }
+                }
+            }
+        }
+    }
+    __device__ void unpack(bool (&fl)[NP], double (&wts)[NP], double2 (&vis)[NP]) const {
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            fl[p] = ((f >> (8 * p)) & 0xffu) != 0;
+            wts[p] = w[p];
+            vis[p] = make_double2(v[p].x, v[p].y);
+        }
+    }
+};
+
+// whether the fast load path applies to a call's arrays
+static bool bin_fast_loads(const StdArgs &a, int mode, int64_t np) {
+    if (!(np == 1 || np == 2 || np == 4) || !a.flag) return false;
+    if ((uintptr_t)a.flag % np != 0) return false;
+    if (a.weight_dtype != NGC_F32 || (uintptr_t)a.weight % (4 * np) != 0) return false;
+    if (mode == NGC_GRID_IMAGE &&
+        (a.vis_dtype != NGC_C64 || (uintptr_t)a.vis % (np == 1 ? 8 : 16) != 0))
+        return false;
+    return true;
+}
+
 // Count pass (FILL = false) and fill pass (FILL = true).  The fill pass also
 // accumulates sum_weight in FP64: sel_weight * norm(du, dv) per valid pol,
 // the reference's per-sample term (_standard_grid.py:366).
-template <bool FILL, int MODE, int NP, typename ACC>
+template <bool FILL, int MODE, int NP, typename ACC, bool FAST>
 __global__ void __launch_bounds__(kBlock)
 std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
                unsigned long long cap, int *overflow, const double *norm_tab, int norm_half,
@@ -277,7 +338,8 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
         const bool have = (wave_id < n_waves) && sample_at(a, g, wave_id, lane, sm);
         // all of the sample's loads are issued before the placement arithmetic
         const int64_t base = sm.rc * NP;
-        double uu = 0.0, vv = 0.0;
+        double uu = 0.0, vv = 0.0, s0 = 0.0, s1 = 0.0;
+        int64_t a_chan_ld = 0;
         bool fl[NP];
         double wts[NP];
         double2 vis[NP];
@@ -286,55 +348,71 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
             const int64_t row = sm.t * a.n_baseline + sm.b;
             uu = a.uvw[row * 3];
             vv = a.uvw[row * 3 + 1];
-            // 4 pols: one word of flags, one 16-byte load of FP32 weights and two
-            // of complex64 visibilities when the arrays are aligned for it
-            if (NP == 4 && a.flag && ((uintptr_t)a.flag & 3) == 0) {
-                const uint32_t f4 = *reinterpret_cast<const uint32_t *>(a.flag + base);
+            s0 = a.uv_scale[sm.ch];
+            s1 = a.uv_scale[a.n_chan + sm.ch];
+            a_chan_ld = a.chan_map[sm.ch];
+            if constexpr (FAST) {
+                // one vector load per array, all issued before any is used
+                FastSample<NP> fs;
+                fs.template load<FILL, MODE == NGC_GRID_IMAGE>(a, base);
+                fs.unpack(fl, wts, vis);
 #pragma unroll
-                for (int p = 0; p < NP; p++) fl[p] = ((f4 >> (8 * p)) & 0xffu) != 0;
+                for (int p = 0; p < NP; p++) any_unflagged |= !fl[p];
             } else {
+                // 4 pols: one word of flags, one 16-byte load of FP32 weights and two
+                // of complex64 visibilities when the arrays are aligned for it
+                if (NP == 4 && a.flag && ((uintptr_t)a.flag & 3) == 0) {
+                    const uint32_t f4 = *reinterpret_cast<const uint32_t *>(a.flag + base);
 #pragma unroll
-                for (int p = 0; p < NP; p++) fl[p] = a.flag ? a.flag[base + p] != 0 : false;
-            }
-#pragma unroll
-            for (int p = 0; p < NP; p++) any_unflagged |= !fl[p];
-            if (FILL) {
-                if (NP == 4 && a.weight_dtype == NGC_F32 && ((uintptr_t)a.weight & 15) == 0) {
-                    const float4 w4 = *reinterpret_cast<const float4 *>(
-                        reinterpret_cast<const float *>(a.weight) + base);
-                    wts[0] = w4.x;
-                    wts[NP > 1 ? 1 : 0] = w4.y;
-                    wts[NP > 2 ? 2 : 0] = w4.z;
-                    wts[NP > 3 ? 3 : 0] = w4.w;
+                    for (int p = 0; p < NP; p++) fl[p] = ((f4 >> (8 * p)) & 0xffu) != 0;
                 } else {
 #pragma unroll
-                    for (int p = 0; p < NP; p++) wts[p] = load_w(a.weight, base + p, a.weight_dtype);
+                    for (int p = 0; p < NP; p++) fl[p] = a.flag ? a.flag[base + p] != 0 : false;
                 }
-                if (MODE == NGC_GRID_IMAGE) {
-                    if (NP == 4 && a.vis_dtype == NGC_C64 && ((uintptr_t)a.vis & 15) == 0) {
-                        const float4 *vp = reinterpret_cast<const float4 *>(
-                            reinterpret_cast<const float2 *>(a.vis) + base);
-                        const float4 v01 = vp[0], v23 = vp[1];
-                        vis[0] = make_double2(v01.x, v01.y);
-                        vis[NP > 1 ? 1 : 0] = make_double2(v01.z, v01.w);
-                        vis[NP > 2 ? 2 : 0] = make_double2(v23.x, v23.y);
-                        vis[NP > 3 ? 3 : 0] = make_double2(v23.z, v23.w);
+#pragma unroll
+                for (int p = 0; p < NP; p++) any_unflagged |= !fl[p];
+                if (FILL) {
+                    if (NP == 4 && a.weight_dtype == NGC_F32 && ((uintptr_t)a.weight & 15) == 0) {
+                        const float4 w4 = *reinterpret_cast<const float4 *>(
+                            reinterpret_cast<const float *>(a.weight) + base);
+                        wts[0] = w4.x;
+                        wts[NP > 1 ? 1 : 0] = w4.y;
+                        wts[NP > 2 ? 2 : 0] = w4.z;
+                        wts[NP > 3 ? 3 : 0] = w4.w;
                     } else {
 #pragma unroll
-                        for (int p = 0; p < NP; p++) vis[p] = load_vis(a.vis, base + p, a.vis_dtype);
+                        for (int p = 0; p < NP; p++) wts[p] = load_w(a.weight, base + p, a.weight_dtype);
+                    }
+                    if (MODE == NGC_GRID_IMAGE) {
+                        if (NP == 4 && a.vis_dtype == NGC_C64 && ((uintptr_t)a.vis & 15) == 0) {
+                            const float4 *vp = reinterpret_cast<const float4 *>(
+                                reinterpret_cast<const float2 *>(a.vis) + base);
+                            const float4 v01 = vp[0], v23 = vp[1];
+                            vis[0] = make_double2(v01.x, v01.y);
+                            vis[NP > 1 ? 1 : 0] = make_double2(v01.z, v01.w);
+                            vis[NP > 2 ? 2 : 0] = make_double2(v23.x, v23.y);
+                            vis[NP > 3 ? 3 : 0] = make_double2(v23.z, v23.w);
+                        } else {
+#pragma unroll
+                            for (int p = 0; p < NP; p++) vis[p] = load_vis(a.vis, base + p, a.vis_dtype);
+                        }
                     }
                 }
             }
         }
-        const bool ok = have && place(a, sm, uu, vv, any_unflagged, px);
-        const int64_t a_chan = ok ? a.chan_map[sm.ch] : 0;
+        const bool ok = have && place(a, uu, vv, s0, s1, any_unflagged, px);
+        const int64_t a_chan = ok ? a_chan_ld : 0;
         E rec;
+        double norm = 0.0;
+        bool use[NP];
+        double wvs[NP];
         if (FILL && ok) {
             rec.uv = (int)((px.uc & 0xffff) | (px.vc << 16));
             rec.duv = (int)((px.du & 0xffff) | ((px.dv & 0xffff) << 16));
             const int ndu = min(max((int)px.du + norm_half, 0), nw - 1);
             const int ndv = min(max((int)px.dv + norm_half, 0), nw - 1);
-            const double norm = norm_tab[ndu * nw + ndv];
+            // (used after the counter atomics are issued, which hides its latency)
+            norm = norm_tab[ndu * nw + ndv];
 #pragma unroll
             for (int p = 0; p < NP; p++) {
                 // the weight is NaN when flagged
@@ -344,16 +422,10 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
                     wr = vis[p].x * wv;
                     wi = vis[p].y * wv;
                 }
-                const bool use = !(isnan(wr) || isnan(wi)) && !(wr == 0.0 && wi == 0.0);
-                rec.wd[p][0] = use ? (ACC)wr : (ACC)0;
-                if (NC == 2) rec.wd[p][NC - 1] = use ? (ACC)wi : (ACC)0;
-                if (use) {
-                    const int64_t swi = a_chan * a.n_imag_pol + p;
-                    if (sw_in_lds)
-                        atomicAdd(&s_sw[swi], wv * norm);
-                    else
-                        atomicAdd(&sum_weight[swi], wv * norm);
-                }
+                use[p] = !(isnan(wr) || isnan(wi)) && !(wr == 0.0 && wi == 0.0);
+                wvs[p] = wv;
+                rec.wd[p][0] = use[p] ? (ACC)wr : (ACC)0;
+                if (NC == 2) rec.wd[p][NC - 1] = use[p] ? (ACC)wi : (ACC)0;
             }
         }
         // group the lanes by tile for each of the (up to 4) tiles a sample
@@ -386,6 +458,19 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
             basev[k] = 0;
             if (key[k] >= 0 && rank[k] == 0)
                 basev[k] = atomicAdd(&counts[key[k]], (unsigned long long)cnt[k]);
+        }
+        if (FILL && ok) {
+            // the FP64 sum_weight terms, while the counter atomics are in flight
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                if (use[p]) {
+                    const int64_t swi = a_chan * a.n_imag_pol + p;
+                    if (sw_in_lds)
+                        atomicAdd(&s_sw[swi], wvs[p] * norm);
+                    else
+                        atomicAdd(&sum_weight[swi], wvs[p] * norm);
+                }
+            }
         }
         if (FILL) {
 #pragma unroll
@@ -1109,8 +1194,19 @@ static int enqueue_count_scan(ngc_context *ctx, hipStream_t s, const StdArgs &a,
     NGC_HIP(hipGetLastError());
     phase_begin(ctx, s, "bin_count");
     if (tg.n_waves > 0) {
-        std_bin_kernel<false, MODE, NP, ACC><<<(int)tg.bin_blocks, kBlock, 0, s>>>(
-            a, tg.g, bf.counts, nullptr, 0ull, nullptr, bf.ntab, bf.half, sum_weight, 0);
+        auto launch = [&](auto fast_c) {
+            std_bin_kernel<false, MODE, NP, ACC, decltype(fast_c)::value>
+                <<<(int)tg.bin_blocks, kBlock, 0, s>>>(a, tg.g, bf.counts, nullptr, 0ull, nullptr,
+                                                       bf.ntab, bf.half, sum_weight, 0);
+        };
+        if constexpr (NP == 1 || NP == 2 || NP == 4) {
+            if (bin_fast_loads(a, MODE, NP))
+                launch(std::true_type{});
+            else
+                launch(std::false_type{});
+        } else {
+            launch(std::false_type{});
+        }
         NGC_HIP(hipGetLastError());
     }
     phase_end(ctx, s);
@@ -1139,9 +1235,20 @@ static int enqueue_fill_grid(ngc_context *ctx, hipStream_t s, const StdArgs &a,
     NGC_HIP(hipGetLastError());
     phase_begin(ctx, s, "bin_fill");
     if (tg.n_waves > 0) {
-        std_bin_kernel<true, MODE, NP, ACC><<<(int)tg.bin_blocks, kBlock, sw_shmem, s>>>(
-            a, tg.g, bf.counts, bf.entries, bf.cap, bf.flags + 1, bf.ntab, bf.half, sum_weight,
-            sw_lds);
+        auto launch = [&](auto fast_c) {
+            std_bin_kernel<true, MODE, NP, ACC, decltype(fast_c)::value>
+                <<<(int)tg.bin_blocks, kBlock, sw_shmem, s>>>(
+                    a, tg.g, bf.counts, bf.entries, bf.cap, bf.flags + 1, bf.ntab, bf.half,
+                    sum_weight, sw_lds);
+        };
+        if constexpr (NP == 1 || NP == 2 || NP == 4) {
+            if (bin_fast_loads(a, MODE, NP))
+                launch(std::true_type{});
+            else
+                launch(std::false_type{});
+        } else {
+            launch(std::false_type{});
+        }
         NGC_HIP(hipGetLastError());
     }
     phase_end(ctx, s);
