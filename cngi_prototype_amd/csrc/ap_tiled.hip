@@ -413,6 +413,8 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     Cplx<ACC> *s_pv = reinterpret_cast<Cplx<ACC> *>(smem_raw + L.pv);
     ACC *s_wd = reinterpret_cast<ACC *>(smem_raw + L.wd);
 
+    __shared__ unsigned s_mv;   // batch samples whose window or support moved
+
     const int tid = threadIdx.x;
     const int64_t bin = order[blockIdx.x];
     const int64_t start = (int64_t)offsets[bin], end = (int64_t)offsets[bin + 1];
@@ -476,6 +478,23 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                 for (int c = 0; c < NW; c++) s_wd[tid * NP * NW + q * NW + c] = en.wd[q][c];
         }
         __syncthreads();
+        // samples whose window start or support differs from the previous
+        // sample's (and the first of the batch): only there can a lane's cell
+        // or tap validity change, so only there do the lanes test their cell
+        if (tid < 64) {
+            bool m = false;
+            if (tid < nh) {
+                const int4 p = s_pos[tid], f = s_fo[tid];
+                if (tid == 0) {
+                    m = true;
+                } else {
+                    const int4 pp = s_pos[tid - 1], fp = s_fo[tid - 1];
+                    m = p.x != pp.x || p.z != pp.z || f.w != fp.w;
+                }
+            }
+            const uint64_t bm = __ballot(m);
+            if (tid == 0) s_mv = (unsigned)bm;
+        }
         // per-sample, per-residue tables
         for (int idx = tid; idx < nh * 2 * R; idx += blockDim.x) {
             const int j = idx / (2 * R), q = idx - j * 2 * R;
@@ -524,36 +543,81 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
             }
         }
         __syncthreads();
+        const unsigned moved = __builtin_amdgcn_readfirstlane(s_mv);
         if (active) {
-            // one sample of look-ahead on the tap load
-            int ui_n = s_ui[ra], vi_n = s_vi[rb];
-            ACC k_n = (ui_n | vi_n) >= 0 ? ckr[s_base[0] + (unsigned)((ui_n >> 16) + (vi_n >> 16))]
-                                         : (ACC)0;
-            for (int j = 0; j < nh; j++) {
-                const int ui = ui_n, vi = vi_n;
-                const ACC k = k_n;
-                if (j + 1 < nh) {
-                    ui_n = s_ui[(j + 1) * R + ra];
-                    vi_n = s_vi[(j + 1) * R + rb];
-                    if ((ui_n | vi_n) >= 0)
-                        k_n = ckr[s_base[j + 1] + (unsigned)((ui_n >> 16) + (vi_n >> 16))];
-                }
-                if ((ui | vi) < 0) continue;
-                const int cell = (ui & 0xffff) + (vi & 0xffff);
-                const Cplx<ACC> pv = s_pv[j * R + rb];
-                const ACC tr = k * pv.x, ti = k * pv.y;
-                if (cell != cur) {
-                    flush();
-                    cur = cell;
+            // Groups of G samples.  A group's tap-table loads (global memory,
+            // one per lane and sample) are issued together one group ahead of
+            // its arithmetic, branch-free: an invalid tap (residue outside
+            // the sample's support) loads element 0 and is zeroed at use, and
+            // past the batch the last sample is reloaded.  The next group's
+            // loads are unconditional so that the wait for this group's taps
+            // counts only the loads issued after them.
+            constexpr int G = kApGroup;
+            const int ng = (nh + G - 1) / G;
+            int cA[G], cB[G];
+            ACC tA[G], tB[G];
+            auto load = [&](int j0, int (&cellv)[G], ACC (&t)[G]) {
+                int ui[G], vi[G];
+                unsigned base[G];
 #pragma unroll
-                    for (int q = 0; q < NACC; q++) acc[q] = (ACC)0;
+                for (int g = 0; g < G; g++) {
+                    const int j = min(j0 + g, nh - 1);
+                    ui[g] = s_ui[j * R + ra];
+                    vi[g] = s_vi[j * R + rb];
+                    base[g] = s_base[j];
                 }
 #pragma unroll
-                for (int p = 0; p < NP; p++) {
-                    const Cplx<ACC> qv = s_q[(j * R + ra) * NP + p];
-                    acc[2 * p] += tr * qv.x - ti * qv.y;
-                    acc[2 * p + 1] += tr * qv.y + ti * qv.x;
+                for (int g = 0; g < G; g++) {
+                    const bool valid = (ui[g] | vi[g]) >= 0;
+                    cellv[g] = valid ? (ui[g] & 0xffff) + (vi[g] & 0xffff) : -1;
+                    // (a 32-bit byte offset from the table's base: the load's
+                    // address is the scalar base plus one vector offset)
+                    const unsigned off =
+                        (valid ? base[g] + (unsigned)((ui[g] >> 16) + (vi[g] >> 16)) : 0u) *
+                        (unsigned)sizeof(ACC);
+                    t[g] = *reinterpret_cast<const ACC *>(reinterpret_cast<const char *>(ckr) + off);
                 }
+            };
+            auto run = [&](int j0, const int (&cellv)[G], const ACC (&t)[G]) {
+                Cplx<ACC> pv[G], qv[G][NP];
+#pragma unroll
+                for (int g = 0; g < G; g++) {
+                    const int j = min(j0 + g, nh - 1);
+                    pv[g] = s_pv[j * R + rb];
+#pragma unroll
+                    for (int p = 0; p < NP; p++) qv[g][p] = s_q[(j * R + ra) * NP + p];
+                }
+#pragma unroll
+                for (int g = 0; g < G; g++) {
+                    const int j = j0 + g;
+                    if (j >= nh) break;
+                    if ((moved >> j) & 1u) {
+                        if (cellv[g] >= 0 && cellv[g] != cur) {
+                            flush();
+                            cur = cellv[g];
+#pragma unroll
+                            for (int q = 0; q < NACC; q++) acc[q] = (ACC)0;
+                        }
+                    }
+                    // (an invalid tap adds exact zeros to the current cell)
+                    const ACC k = cellv[g] >= 0 ? t[g] : (ACC)0;
+                    const ACC tr = k * pv[g].x, ti = k * pv[g].y;
+#pragma unroll
+                    for (int p = 0; p < NP; p++) {
+                        acc[2 * p] = fma(tr, qv[g][p].x, acc[2 * p]);
+                        acc[2 * p] = fma(-ti, qv[g][p].y, acc[2 * p]);
+                        acc[2 * p + 1] = fma(tr, qv[g][p].y, acc[2 * p + 1]);
+                        acc[2 * p + 1] = fma(ti, qv[g][p].x, acc[2 * p + 1]);
+                    }
+                }
+            };
+            load(0, cA, tA);
+            for (int gi = 0; gi < ng; gi += 2) {
+                load((gi + 1) * G, cB, tB);
+                run(gi * G, cA, tA);
+                if (gi + 1 >= ng) break;
+                load((gi + 2) * G, cA, tA);
+                run((gi + 1) * G, cB, tB);
             }
         }
     }
@@ -949,6 +1013,9 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     // re-laid taps and the separable phase-gradient factors
     void *d_ckr, *d_pgu, *d_pgv;
     const int64_t n_ckr = c.n_cf * nwu * nwv * a.R * a.R;
+    // the tiled kernel addresses the re-laid taps with 32-bit byte offsets
+    NGC_CHECK(n_ckr * acc_bytes < (int64_t(1) << 32), NGC_ERR_UNSUPPORTED,
+              "re-laid convolution kernels exceed 4 GiB");
     NGC_TRY(workspace(ctx, "ap_ckr", n_ckr * acc_bytes, &d_ckr));
     NGC_TRY(workspace(ctx, "ap_pgu", c.n_field * c.k_u * 2 * acc_bytes, &d_pgu));
     NGC_TRY(workspace(ctx, "ap_pgv", c.n_field * c.k_v * 2 * acc_bytes, &d_pgv));
