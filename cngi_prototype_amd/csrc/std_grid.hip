@@ -536,6 +536,7 @@ __device__ inline int bin_bucket(unsigned long long n) {
 }
 
 constexpr int kChunk = 16;        // tiles per work ticket of small bins
+constexpr int kEmptyChunk = 4;    // empty tiles zeroed per ticket of the empty-tile queue
 constexpr int kChunkBelow = 32;   // records below which a bin is small
 
 // LDS subgrid layout: row-major tile planes of 32 cells per row, with the
@@ -594,6 +595,7 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     __shared__ ACC s_k[513];          // the kernel table; zero from cgk_len on
     __shared__ unsigned s_chg[W];     // per wavefront: its samples whose window moved
     __shared__ int s_item;
+    __shared__ int s_nz, s_e;   // non-empty bins; the empty-tile ticket
     __shared__ int64_t s_tq[kChunk][3];   // bin, first and end record of each tile of a ticket
 
     const int tid = threadIdx.x;
@@ -618,15 +620,55 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     // the queue until all n_items are done; every wave reaches the exit
     // work tickets: bins of kChunkBelow records or more (the front of the
     // largest-first order) are handed out one per ticket, for balance; the
-    // small and empty bins after them in runs of kChunk (fewer atomics on the
-    // queue counter, one round trip for their records)
+    // small non-empty bins after them in runs of kChunk (fewer atomics on the
+    // queue counter, one round trip for their records).  The empty bins, last
+    // in the order, have a queue of their own (queue[2], kEmptyChunk tiles a
+    // ticket): after each non-empty tile a workgroup zeroes one chunk of them
+    // straight from registers, so their stores overlap the gridding instead of
+    // forming a store-bound tail; in accumulate mode they are left untouched.
     if (tid == 0) {
         int nb = 0;
         for (int b = bin_bucket(kChunkBelow); b < kBuckets; b++) nb += (int)hist[b];
         s_item = nb;
+        s_nz = n_items - (int)hist[0];
     }
     __syncthreads();
     const int n_single = s_item;
+    const int n_nz = s_nz;
+    using OUT = typename std::conditional<NC == 2,
+        typename std::conditional<std::is_same<ACC, float>::value, float2, double2>::type,
+        ACC>::type;
+    constexpr int F = NP * TU * kTile;   // grid values of one tile
+    bool zero_more = !accumulate;
+    // one ticket of the empty-tile queue; false when it is exhausted
+    auto zero_chunk = [&]() -> bool {
+        __syncthreads();  // every read of s_e is done
+        if (tid == 0) s_e = atomicAdd(queue + 2, 1);
+        __syncthreads();
+        const int64_t first = (int64_t)n_nz + (int64_t)s_e * kEmptyChunk;
+        if (first >= n_items) return false;
+        const int64_t last = min(first + kEmptyChunk, (int64_t)n_items);
+        unsigned eb[kEmptyChunk];
+#pragma unroll
+        for (int c = 0; c < kEmptyChunk; c++) eb[c] = order[min(first + c, last - 1)];
+        const int64_t tiles = g.ntu * g.ntv;
+#pragma unroll
+        for (int c = 0; c < kEmptyChunk; c++) {
+            if (first + c >= last) break;
+            const int64_t ach = eb[c] / tiles;
+            const int64_t trem = eb[c] - ach * tiles;
+            const int64_t u0 = (trem / g.ntv) * TU, v0 = (trem % g.ntv) * kTile;
+#pragma unroll 4
+            for (int k = tid; k < F; k += NT) {
+                const int64_t gu = u0 + (k / kTile) % TU, gv = v0 + k % kTile;
+                const int p = k / (kTile * TU);
+                if (gu < a.n_u && gv < a.n_v)
+                    reinterpret_cast<OUT *>(grid_out)[((ach * a.n_imag_pol + p) * a.n_u + gu) *
+                                                          a.n_v + gv] = OUT{};
+            }
+        }
+        return true;
+    };
     int it_next = 0, it_end = 0, it_first = 0;
     for (;;) {
     __syncthreads();  // the previous tile's LDS reads (and s_item, s_tq) are complete
@@ -639,11 +681,11 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
             it_end = t + 1;
         } else {
             const int64_t first = (int64_t)n_single + (int64_t)(t - n_single) * kChunk;
-            it_next = first < n_items ? (int)first : n_items;
+            it_next = first < n_nz ? (int)first : n_nz;
             it_end = it_next + kChunk;
         }
-        if (it_end > n_items) it_end = n_items;
-        if (it_next >= n_items) break;
+        if (it_end > n_nz) it_end = n_nz;
+        if (it_next >= n_nz) break;
         // the chunk's bins and record ranges, loaded together
         it_first = it_next;
         if (tid < it_end - it_next) {
@@ -854,11 +896,7 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
 
     // write the tile, lanes running along v (each row a contiguous run of the
     // grid), and clear the subgrid for the next tile: every cell of the grid
-    // is written exactly once
-    constexpr int F = NP * TU * kTile;
-    using OUT = typename std::conditional<NC == 2,
-        typename std::conditional<std::is_same<ACC, float>::value, float2, double2>::type,
-        ACC>::type;
+    // is written exactly once (empty tiles by the empty-tile queue)
 #pragma unroll 4
     for (int k = tid; k < F; k += NT) {
         const int lv = k % kTile;
@@ -895,7 +933,9 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
             *gp = val;
         }
     }
+    if (zero_more) zero_more = zero_chunk();
     }  // work queue
+    while (zero_more) zero_more = zero_chunk();
 }
 
 // ---------------------------------------------------------------------------
@@ -1095,7 +1135,7 @@ struct TiledBufs {
     char *entries = nullptr;
     unsigned long long cap = 0;           // entries capacity (records)
     unsigned int *order = nullptr, *hist = nullptr;  // hist: [2][kBuckets] (counts, cursors)
-    int *flags = nullptr;                 // [0] work queue, [1] entries overflow
+    int *flags = nullptr;                 // [0] work queue, [1] entries overflow, [2] empty-tile queue
     const double *ntab = nullptr, *cgk = nullptr;
     int64_t cgk_len = 0;
     int half = 0;

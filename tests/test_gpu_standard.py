@@ -90,6 +90,56 @@ def _standard_case(nat, precision, mode, n_pol=2):
         assert rel_err(g, g_ref) < 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64_tiled", "fp32_kahan", "fp32"])
+@pytest.mark.parametrize("mode", ["psf", "image"])
+def test_unreached_tiles_written_and_kept_when_accumulating(gpu, precision, mode):
+    """Tiles that no sample reaches are written too: an output grid filled
+    with NaN comes back equal to the oracle's, zeros included.  In accumulate
+    mode those tiles keep their values exactly and the reached ones add."""
+    import torch
+    nat = _native()
+    c = small_case(n_time=4, n_chan=2, n_pol=2, image_size=512)
+    cgk = oracle.ps_kernel_1d(100, 7)
+    npad = c["npad"]
+    single = precision in ("fp32_kahan", "fp32")
+    visr, wr = c["vis"], c["weight"]
+    if single:
+        visr = visr.astype(np.complex64).astype(np.complex128)
+        wr = wr.astype(np.float32).astype(np.float64)
+    g_ref, _ = oracle.standard_grid(visr, c["uvw"], wr, c["freq"], cgk, [npad, npad],
+                                    c["delta"], 7, 100, do_psf=(mode == "psf"))
+    # most of the grid is out of reach of the 4 integrations
+    assert np.mean(g_ref == 0) > 0.5
+    if mode == "image":
+        gdt = torch.complex64 if single else torch.complex128
+        fill = complex(float("nan"), float("nan"))
+    else:
+        gdt = torch.float32 if single else torch.float64
+        fill = float("nan")
+    tol = 1e-6 if single else 1e-12
+    args = (_dev(c["uvw"]), _dev(c["weight"], torch.float32 if single else torch.float64),
+            c["freq"], np.arange(2), np.arange(2), cgk, 2, [npad, npad], c["delta"], 7, 100,
+            nat.GRID_IMAGE if mode == "image" else nat.GRID_PSF, precision)
+    vis = _dev(c["vis"], torch.complex64 if single else torch.complex128) \
+        if mode == "image" else None
+    grid = torch.full((2, 2, npad, npad), fill, dtype=gdt, device="cuda")
+    g, _ = nat.standard_grid(*args, vis=vis, grid=grid)
+    torch.cuda.synchronize()
+    assert not bool(torch.isnan(g).any())
+    assert rel_err(g.cpu().numpy(), g_ref) < tol
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    base = torch.rand((2, 2, npad, npad), generator=gen, device="cuda",
+                      dtype=torch.float64).to(gdt)
+    g2 = base.clone()
+    sw2 = torch.zeros((2, 2), dtype=torch.float64, device="cuda")
+    nat.standard_grid(*args, vis=vis, grid=g2, sum_weight=sw2, accumulate=True)
+    torch.cuda.synchronize()
+    d = (g2 - base).cpu().numpy()
+    assert rel_err(d, g_ref) < (1e-5 if single else 1e-12)
+    untouched = torch.from_numpy(g_ref == 0).cuda()
+    assert bool((g2[untouched] == base[untouched]).all())
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32_kahan"])
 def test_grid_to_image_matches_oracle(gpu, precision):
     import torch
