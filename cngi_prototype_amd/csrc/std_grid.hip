@@ -536,7 +536,7 @@ __device__ inline int bin_bucket(unsigned long long n) {
 }
 
 constexpr int kChunk = 16;        // tiles per work ticket of small bins
-constexpr int kEmptyChunk = 4;    // empty tiles zeroed per ticket of the empty-tile queue
+constexpr int kEmptyChunk = 8;    // empty tiles zeroed per ticket of the empty-tile queue
 constexpr int kChunkBelow = 32;   // records below which a bin is small
 
 // LDS subgrid layout: row-major tile planes of 32 cells per row, with the
