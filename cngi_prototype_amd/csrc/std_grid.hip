@@ -235,7 +235,10 @@ __device__ inline int64_t tile_key(const StdArgs &a, const BinGeom &g, const Pix
 }
 
 // Entry record of one sample in one bin: packed centre cell and offsets, then
-// the weighted visibilities of every pol (ACC precision), padded to 16 B.
+// the weighted visibilities of every pol (ACC precision), padded to 8 B (the
+// header is read as one 8-byte word).  The fill pass's entry stores are the
+// largest part of its time, so records carry no further padding: 40 B for
+// 4 pols and 24 B for 2 pols of complex FP32.
 template <int NP, int NC, typename ACC> struct Entry {
     int uv;      // u0 | v0 << 16
     int duv;     // du & 0xffff | dv << 16
@@ -243,7 +246,7 @@ template <int NP, int NC, typename ACC> struct Entry {
 };
 template <int NP, int NC, typename ACC> struct EntrySize {
     static constexpr int raw = (int)sizeof(Entry<NP, NC, ACC>);
-    static constexpr int bytes = (raw + 15) / 16 * 16;
+    static constexpr int bytes = (raw + 7) / 8 * 8;
 };
 
 // The fast load path of the binning passes (FAST = true): uint8 flags, FP32
