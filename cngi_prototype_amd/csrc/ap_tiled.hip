@@ -394,8 +394,10 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     //   s_fo    [kApBatch] int4   staging: field, offsets, supports
     //   s_pos   [kApBatch] int4   staging: start cells and their residues
     //   s_base  [kApBatch] uint   first tap index of the sample's R x R block
-    //   s_ui    [kApBatch][R] int u-residue a: cell_u*(W+1) | ru*R << 16, -1 = no tap
-    //   s_vi    [kApBatch][R] int v-residue b: cell_v | rv << 16, -1 = no tap
+    //   s_ui    [kApBatch][R] int u-residue a: cell_u*(W+1) << 16 | ru*R, sign bit = no tap
+    //   s_vi    [kApBatch][R] int v-residue b: cell_v << 16 | rv, sign bit = no tap
+    //   (the sum of the two records of a valid tap is cell << 16 | tap index
+    //   in the sample's R x R block: cell < W*(W+1) < 2^15, index < R*R < 2^16)
     //   s_q     [kApBatch][R][NP] u phase factor x weighted visibility per pol
     //   s_pv    [kApBatch][R] v phase factor
     //   s_wd    [kApBatch][NP*NW]
@@ -512,7 +514,8 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                 pu.y = (ACC)0;
                 if (valid) pu = pgu[(int64_t)fo.x * a.k_u + a.os_u * (ru - su / 2) + fo.y +
                                     a.k_u / 2];
-                s_ui[j * R + r] = valid ? (((ps.x + ru) * (W + 1)) | ((ru * R) << 16)) : -1;
+                s_ui[j * R + r] = valid ? ((((ps.x + ru) * (W + 1)) << 16) | (ru * R))
+                                        : (int)0x80000000;
 #pragma unroll
                 for (int p = 0; p < NP; p++) {
                     Cplx<ACC> qv;
@@ -538,7 +541,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                 pv.y = (ACC)0;
                 if (valid) pv = pgv[(int64_t)fo.x * a.k_v + a.os_v * (rv - sv / 2) + fo.z +
                                     a.k_v / 2];
-                s_vi[j * R + r] = valid ? ((ps.z + rv) | (rv << 16)) : -1;
+                s_vi[j * R + r] = valid ? (((ps.z + rv) << 16) | rv) : (int)0x80000000;
                 s_pv[j * R + r] = pv;
             }
         }
@@ -569,12 +572,15 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
 #pragma unroll
                 for (int g = 0; g < G; g++) {
                     const bool valid = (ui[g] | vi[g]) >= 0;
-                    cellv[g] = valid ? (ui[g] & 0xffff) + (vi[g] & 0xffff) : -1;
-                    // (a 32-bit byte offset from the table's base: the load's
-                    // address is the scalar base plus one vector offset)
+                    const int sum = (int)((unsigned)ui[g] + (unsigned)vi[g]);
+                    cellv[g] = valid ? sum : -1;   // cell << 16 | tap index, -1 = no tap
+                    // The tap's byte offset from the table's base (the load's
+                    // address is the scalar base plus one vector offset).  For
+                    // an invalid tap the low half is another in-block index
+                    // or 0, so the load stays inside the sample's block; its
+                    // value is zeroed at use.
                     const unsigned off =
-                        (valid ? base[g] + (unsigned)((ui[g] >> 16) + (vi[g] >> 16)) : 0u) *
-                        (unsigned)sizeof(ACC);
+                        (base[g] + ((unsigned)sum & 0xffffu)) * (unsigned)sizeof(ACC);
                     t[g] = *reinterpret_cast<const ACC *>(reinterpret_cast<const char *>(ckr) + off);
                 }
             };
@@ -592,9 +598,10 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                     const int j = j0 + g;
                     if (j >= nh) break;
                     if ((moved >> j) & 1u) {
-                        if (cellv[g] >= 0 && cellv[g] != cur) {
+                        const int cell = cellv[g] >> 16;
+                        if (cellv[g] >= 0 && cell != cur) {
                             flush();
-                            cur = cellv[g];
+                            cur = cell;
 #pragma unroll
                             for (int q = 0; q < NACC; q++) acc[q] = (ACC)0;
                         }
