@@ -129,8 +129,14 @@ __global__ void ap_taps_kernel(ApT a, const double *ck, ACC *ckr) {
     const int R = a.R;
     const int64_t n = a.n_cf * nwu * nwv * R * R;
     const int64_t kplane = a.k_u * a.k_v;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+    // element n is a zero tap, where the tiled kernel points the taps that
+    // fall outside a sample's support
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n;
          i += (int64_t)gridDim.x * blockDim.x) {
+        if (i == n) {
+            ckr[i] = (ACC)0;
+            continue;
+        }
         const int rv = (int)(i % R), ru = (int)((i / R) % R);
         const int64_t o = i / ((int64_t)R * R);
         const int64_t ov = o % nwv - a.half_v;
@@ -349,7 +355,7 @@ ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *n
 // Byte offsets of the tiled kernel's dynamic LDS block (shared by the
 // kernel and the host's size computation).
 struct ApLds {
-    int fo, pos, base, ui, vi, q, pv, wd, bytes;
+    int fo, pos, base, ui, vi, uc, vc, q, pv, wd, bytes;
 };
 __host__ __device__ inline int ap_align16(int x) { return (x + 15) & ~15; }
 __host__ __device__ inline ApLds ap_lds_layout(int R, int W, int NP, int NW, int acc_bytes,
@@ -361,7 +367,9 @@ __host__ __device__ inline ApLds ap_lds_layout(int R, int W, int NP, int NW, int
     L.base = L.pos + kApBatch * 16;
     L.ui = L.base + kApBatch * 4;
     L.vi = L.ui + kApBatch * R * 4;
-    L.q = ap_align16(L.vi + kApBatch * R * 4);
+    L.uc = L.vi + kApBatch * R * 4;
+    L.vc = L.uc + kApBatch * R * 4;
+    L.q = ap_align16(L.vc + kApBatch * R * 4);
     L.pv = L.q + kApBatch * R * NP * 2 * acc_bytes;
     L.wd = L.pv + kApBatch * R * 2 * acc_bytes;
     L.bytes = ap_align16(L.wd + kApBatch * NP * NW * acc_bytes);
@@ -394,10 +402,12 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     //   s_fo    [kApBatch] int4   staging: field, offsets, supports
     //   s_pos   [kApBatch] int4   staging: start cells and their residues
     //   s_base  [kApBatch] uint   first tap index of the sample's R x R block
-    //   s_ui    [kApBatch][R] int u-residue a: cell_u*(W+1) << 16 | ru*R, sign bit = no tap
-    //   s_vi    [kApBatch][R] int v-residue b: cell_v << 16 | rv, sign bit = no tap
-    //   (the sum of the two records of a valid tap is cell << 16 | tap index
-    //   in the sample's R x R block: cell < W*(W+1) < 2^15, index < R*R < 2^16)
+    //   s_ui    [kApBatch][R] uint u-residue a: byte offset of the sample's tap
+    //                         block plus ru*R taps; the zero tap's offset Z if no tap
+    //   s_vi    [kApBatch][R] uint v-residue b: rv taps in bytes; Z if no tap
+    //                         (min(ui + vi, Z) is the tap's byte offset)
+    //   s_uc    [kApBatch][R] int cell_u*(W+1), or -2^30 if no tap
+    //   s_vc    [kApBatch][R] int cell_v, or -2^30 if no tap (uc + vc < 0: no tap)
     //   s_q     [kApBatch][R][NP] u phase factor x weighted visibility per pol
     //   s_pv    [kApBatch][R] v phase factor
     //   s_wd    [kApBatch][NP*NW]
@@ -409,8 +419,13 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     int4 *s_fo = reinterpret_cast<int4 *>(smem_raw + L.fo);
     int4 *s_pos = reinterpret_cast<int4 *>(smem_raw + L.pos);
     unsigned int *s_base = reinterpret_cast<unsigned int *>(smem_raw + L.base);
-    int *s_ui = reinterpret_cast<int *>(smem_raw + L.ui);
-    int *s_vi = reinterpret_cast<int *>(smem_raw + L.vi);
+    unsigned *s_ui = reinterpret_cast<unsigned *>(smem_raw + L.ui);
+    unsigned *s_vi = reinterpret_cast<unsigned *>(smem_raw + L.vi);
+    int *s_uc = reinterpret_cast<int *>(smem_raw + L.uc);
+    int *s_vc = reinterpret_cast<int *>(smem_raw + L.vc);
+    // byte offset of the zero tap (the host keeps the table below 2^31 bytes)
+    const unsigned zb = (unsigned)(a.n_cf * (2 * a.half_u + 1) * (2 * a.half_v + 1) * R * R) *
+                        (unsigned)sizeof(ACC);
     Cplx<ACC> *s_q = reinterpret_cast<Cplx<ACC> *>(smem_raw + L.q);
     Cplx<ACC> *s_pv = reinterpret_cast<Cplx<ACC> *>(smem_raw + L.pv);
     ACC *s_wd = reinterpret_cast<ACC *>(smem_raw + L.wd);
@@ -514,8 +529,9 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                 pu.y = (ACC)0;
                 if (valid) pu = pgu[(int64_t)fo.x * a.k_u + a.os_u * (ru - su / 2) + fo.y +
                                     a.k_u / 2];
-                s_ui[j * R + r] = valid ? ((((ps.x + ru) * (W + 1)) << 16) | (ru * R))
-                                        : (int)0x80000000;
+                s_ui[j * R + r] = valid ? (s_base[j] + (unsigned)(ru * R)) * (unsigned)sizeof(ACC)
+                                        : zb;
+                s_uc[j * R + r] = valid ? (ps.x + ru) * (W + 1) : -(1 << 30);
 #pragma unroll
                 for (int p = 0; p < NP; p++) {
                     Cplx<ACC> qv;
@@ -541,7 +557,8 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                 pv.y = (ACC)0;
                 if (valid) pv = pgv[(int64_t)fo.x * a.k_v + a.os_v * (rv - sv / 2) + fo.z +
                                     a.k_v / 2];
-                s_vi[j * R + r] = valid ? (((ps.z + rv) << 16) | rv) : (int)0x80000000;
+                s_vi[j * R + r] = valid ? (unsigned)rv * (unsigned)sizeof(ACC) : zb;
+                s_vc[j * R + r] = valid ? ps.z + rv : -(1 << 30);
                 s_pv[j * R + r] = pv;
             }
         }
@@ -560,27 +577,23 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
             int cA[G], cB[G];
             ACC tA[G], tB[G];
             auto load = [&](int j0, int (&cellv)[G], ACC (&t)[G]) {
-                int ui[G], vi[G];
-                unsigned base[G];
+                unsigned ui[G], vi[G];
+                int uc[G], vc[G];
 #pragma unroll
                 for (int g = 0; g < G; g++) {
                     const int j = min(j0 + g, nh - 1);
                     ui[g] = s_ui[j * R + ra];
                     vi[g] = s_vi[j * R + rb];
-                    base[g] = s_base[j];
+                    uc[g] = s_uc[j * R + ra];
+                    vc[g] = s_vc[j * R + rb];
                 }
 #pragma unroll
                 for (int g = 0; g < G; g++) {
-                    const bool valid = (ui[g] | vi[g]) >= 0;
-                    const int sum = (int)((unsigned)ui[g] + (unsigned)vi[g]);
-                    cellv[g] = valid ? sum : -1;   // cell << 16 | tap index, -1 = no tap
+                    cellv[g] = uc[g] + vc[g];   // < 0: no tap
                     // The tap's byte offset from the table's base (the load's
-                    // address is the scalar base plus one vector offset).  For
-                    // an invalid tap the low half is another in-block index
-                    // or 0, so the load stays inside the sample's block; its
-                    // value is zeroed at use.
-                    const unsigned off =
-                        (base[g] + ((unsigned)sum & 0xffffu)) * (unsigned)sizeof(ACC);
+                    // address is the scalar base plus one vector offset); a
+                    // tap outside the support reads the zero tap.
+                    const unsigned off = min(ui[g] + vi[g], zb);
                     t[g] = *reinterpret_cast<const ACC *>(reinterpret_cast<const char *>(ckr) + off);
                 }
             };
@@ -598,16 +611,17 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                     const int j = j0 + g;
                     if (j >= nh) break;
                     if ((moved >> j) & 1u) {
-                        const int cell = cellv[g] >> 16;
-                        if (cellv[g] >= 0 && cell != cur) {
+                        const int cell = cellv[g];
+                        if (cell >= 0 && cell != cur) {
                             flush();
                             cur = cell;
 #pragma unroll
                             for (int q = 0; q < NACC; q++) acc[q] = (ACC)0;
                         }
                     }
-                    // (an invalid tap adds exact zeros to the current cell)
-                    const ACC k = cellv[g] >= 0 ? t[g] : (ACC)0;
+                    // (a tap outside the support is the zero tap: it adds exact
+                    // zeros to the current cell)
+                    const ACC k = t[g];
                     const ACC tr = k * pv[g].x, ti = k * pv[g].y;
 #pragma unroll
                     for (int p = 0; p < NP; p++) {
@@ -1021,22 +1035,22 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     void *d_ckr, *d_pgu, *d_pgv;
     const int64_t n_ckr = c.n_cf * nwu * nwv * a.R * a.R;
     // the tiled kernel addresses the re-laid taps with 32-bit byte offsets
-    NGC_CHECK(n_ckr * acc_bytes < (int64_t(1) << 32), NGC_ERR_UNSUPPORTED,
-              "re-laid convolution kernels exceed 4 GiB");
-    NGC_TRY(workspace(ctx, "ap_ckr", n_ckr * acc_bytes, &d_ckr));
+    NGC_CHECK((n_ckr + 1) * acc_bytes < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED,
+              "re-laid convolution kernels exceed 2 GiB");
+    NGC_TRY(workspace(ctx, "ap_ckr", (n_ckr + 1) * acc_bytes, &d_ckr));
     NGC_TRY(workspace(ctx, "ap_pgu", c.n_field * c.k_u * 2 * acc_bytes, &d_pgu));
     NGC_TRY(workspace(ctx, "ap_pgv", c.n_field * c.k_v * 2 * acc_bytes, &d_pgv));
     const int64_t n_pg = c.n_field * c.k_u * c.k_v;
     NGC_HIP(hipMemsetAsync(ctx->d_error + 1, 0, sizeof(int), s));
     phase_begin(ctx, s, "ap_tables");
     if (single) {
-        ap_taps_kernel<float><<<grid_blocks(n_ckr), 256, 0, s>>>(a, c.conv_kernel,
+        ap_taps_kernel<float><<<grid_blocks(n_ckr + 1), 256, 0, s>>>(a, c.conv_kernel,
                                                                  (float *)d_ckr);
         ap_pg_kernel<float2><<<grid_blocks(n_pg), 256, 0, s>>>(
             a, (const double2 *)c.phase_gradient, (float2 *)d_pgu, (float2 *)d_pgv,
             ctx->d_error + 1);
     } else {
-        ap_taps_kernel<double><<<grid_blocks(n_ckr), 256, 0, s>>>(a, c.conv_kernel,
+        ap_taps_kernel<double><<<grid_blocks(n_ckr + 1), 256, 0, s>>>(a, c.conv_kernel,
                                                                   (double *)d_ckr);
         ap_pg_kernel<double2><<<grid_blocks(n_pg), 256, 0, s>>>(
             a, (const double2 *)c.phase_gradient, (double2 *)d_pgu, (double2 *)d_pgv,
