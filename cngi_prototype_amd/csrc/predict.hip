@@ -102,11 +102,15 @@ __device__ inline void axis_taps(const double *s_k, int64_t S, int64_t os, int64
 // consecutive integrations of one (baseline, channel), whose footprints in the
 // same grid plane overlap, so their tap reads share cache lines.  SMAX is the
 // compile-time bound of the support (taps beyond the support weigh 0 and are
-// skipped).
-template <typename G, typename V, int SMAX>
+// skipped); NP the number of pols (0: n_pol at run time).
+//
+// The kernel is separable, so each pol's sum is taken as
+// sum_u cu[u] (sum_v cv[v] g[u][v]): the inner sums along v in the grid's
+// precision (FP32 taps for a complex64 grid, 7 terms each), the outer sums
+// and the normalisation by the tap sum (sum cu)(sum cv) in FP64.
+template <typename G, typename V, int SMAX, int NP>
 __global__ void __launch_bounds__(256) standard_degrid_kernel(DegridStdArgs a, const G *grid,
                                                               V *vis) {
-#pragma clang fp contract(off)
     __shared__ double s_k[512];
     for (int i = threadIdx.x; i < 512; i += blockDim.x)
         s_k[i] = i < a.cgk_len ? a.cgk[i] : 0.0;
@@ -139,30 +143,38 @@ __global__ void __launch_bounds__(256) standard_degrid_kernel(DegridStdArgs a, c
         double cu[SMAX], cv[SMAX];
         axis_taps<SMAX>(s_k, a.support, a.os, px.du, cu);
         axis_taps<SMAX>(s_k, a.support, a.os, px.dv, cv);
-        // tap sum in the gridder's loop order (v outer, u inner)
-        double norm = 0.0;
+        G cvg[SMAX];
+        double su = 0.0, sv = 0.0;
 #pragma unroll
-        for (int iv = 0; iv < SMAX; iv++)
-#pragma unroll
-            for (int iu = 0; iu < SMAX; iu++)
-                if (iv < a.support && iu < a.support) norm = norm + cu[iu] * cv[iv];
+        for (int i = 0; i < SMAX; i++) {
+            cvg[i] = (G)cv[i];
+            su += cu[i];
+            sv += cv[i];
+        }
+        const double norm = su * sv;   // the sum of all taps
         const int64_t a_chan = a.chan_map[ch];
         const int64_t u0 = px.uc - sc, v0 = px.vc - sc;
-        for (int64_t p = 0; p < a.n_pol; p++) {
+        const int n_pol = NP > 0 ? NP : (int)a.n_pol;
+        const int S = (int)a.support;
+#pragma unroll
+        for (int p = 0; p < n_pol; p++) {
             const G *g = grid + 2 * ((a_chan * a.n_imag_pol + a.pol_map[p]) * plane +
                                      u0 * a.n_v + v0);
             double re = 0.0, im = 0.0;
 #pragma unroll
-            for (int iv = 0; iv < SMAX; iv++) {
+            for (int iu = 0; iu < SMAX; iu++) {
+                if (iu >= S) break;
+                const G *gr = g + 2 * (int64_t)iu * a.n_v;
+                G ir = (G)0, ii = (G)0;
 #pragma unroll
-                for (int iu = 0; iu < SMAX; iu++) {
-                    if (iv < a.support && iu < a.support) {
-                        const double c = cu[iu] * cv[iv];
-                        const int64_t o = 2 * ((int64_t)iu * a.n_v + iv);
-                        re = re + c * (double)g[o];
-                        im = im + c * (double)g[o + 1];
+                for (int iv = 0; iv < SMAX; iv++) {
+                    if (iv < S) {
+                        ir = fma(cvg[iv], gr[2 * iv], ir);
+                        ii = fma(cvg[iv], gr[2 * iv + 1], ii);
                     }
                 }
+                re = fma(cu[iu], (double)ir, re);
+                im = fma(cu[iu], (double)ii, im);
             }
             if (a.normalize && norm != 0.0) {
                 re = re / norm;
@@ -301,25 +313,36 @@ extern "C" int ngc_standard_degrid(ngc_context *ctx, void *stream, const double 
     const int nb = pblocks(n_thr);
     profile_new_call(ctx, s);
     phase_begin(ctx, s, "standard_degrid");
-    auto launch = [&](auto smax) {
+    auto launch = [&](auto smax, auto np_c) {
         constexpr int SM = decltype(smax)::value;
+        constexpr int NP = decltype(np_c)::value;
         if (grid_dtype == NGC_C64 && vis_dtype == NGC_C64)
-            standard_degrid_kernel<float, float, SM><<<nb, 256, 0, s>>>(a, (const float *)grid,
-                                                                        (float *)vis);
+            standard_degrid_kernel<float, float, SM, NP><<<nb, 256, 0, s>>>(
+                a, (const float *)grid, (float *)vis);
         else if (grid_dtype == NGC_C64)
-            standard_degrid_kernel<float, double, SM><<<nb, 256, 0, s>>>(
+            standard_degrid_kernel<float, double, SM, NP><<<nb, 256, 0, s>>>(
                 a, (const float *)grid, (double *)vis);
         else if (vis_dtype == NGC_C64)
-            standard_degrid_kernel<double, float, SM><<<nb, 256, 0, s>>>(
+            standard_degrid_kernel<double, float, SM, NP><<<nb, 256, 0, s>>>(
                 a, (const double *)grid, (float *)vis);
         else
-            standard_degrid_kernel<double, double, SM><<<nb, 256, 0, s>>>(
+            standard_degrid_kernel<double, double, SM, NP><<<nb, 256, 0, s>>>(
                 a, (const double *)grid, (double *)vis);
     };
+    auto by_pol = [&](auto smax) {
+        if (n_pol == 1)
+            launch(smax, std::integral_constant<int, 1>{});
+        else if (n_pol == 2)
+            launch(smax, std::integral_constant<int, 2>{});
+        else if (n_pol == 4)
+            launch(smax, std::integral_constant<int, 4>{});
+        else
+            launch(smax, std::integral_constant<int, 0>{});
+    };
     if (support <= 8)
-        launch(std::integral_constant<int, 8>{});
+        by_pol(std::integral_constant<int, 8>{});
     else
-        launch(std::integral_constant<int, 16>{});
+        by_pol(std::integral_constant<int, 16>{});
     NGC_HIP(hipGetLastError());
     phase_end(ctx, s);
     return NGC_OK;
