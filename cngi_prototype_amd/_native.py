@@ -34,7 +34,7 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_grid_plan_describe", "ngc_grid_plan_destroy", "ngc_bf16_gemm",
            "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step",
            "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags", "ngc_set_tile_waves",
-           "ngc_model_to_grid", "ngc_standard_degrid", "ngc_mlp_sample_rows",
+           "ngc_model_to_grid", "ngc_standard_degrid", "ngc_aperture_degrid", "ngc_mlp_sample_rows",
            "ngc_adam_schedule", "ngc_adam_step_sched", "ngc_set_bin_lanes")
 
 
@@ -84,6 +84,10 @@ _PROTOS = {
     "ngc_standard_degrid": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64,
                                    _i64, _i64, _i64, _i64, _i64, _d, _d, _i64, _i64, _vp, _i32,
                                    _vp, _i32, _i32]),
+    "ngc_aperture_degrid": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    _vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _i64, _i64,
+                                    _i64, _i64, _i64, _i64, _i64, _i64, _d, _d, _i64, _i64,
+                                    _vp, _i32, _vp, _i32, _i32]),
     "ngc_make_pb": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i32, _i32, _i64, _i64, _i64,
                            _i64, _d, _d, _i64, _vp]),
     "ngc_profile_read": (_i32, [_vp, ctypes.c_char_p, _i64, _vp, _i64]),
@@ -648,6 +652,58 @@ def standard_degrid(uvw, grid, freq_chan, chan_map, pol_map, cgk_1d, delta_lm, s
         _ptr(grid), C64 if grid.dtype == torch.complex64 else C128, _ptr(out),
         C64 if single else C128, int(bool(normalize)))
     _check("ngc_standard_degrid", rc)
+    return out
+
+
+def aperture_degrid(uvw, field, field_id, grid, freq_chan, chan_map, pol_map, cf_baseline_map,
+                    cf_chan_map, cf_pol_map, conv_kernel, support, phase_gradient, delta_lm,
+                    oversampling, n_pol=None, normalize=True, single=True, out=None):
+    """ngc_aperture_degrid: visibilities [T, B, C, P] (complex64 when single,
+    else complex128) predicted from grid [n_imag_chan, n_imag_pol, n_u, n_v]
+    with the A-term kernels (conv_kernel float64 [n_cfb, n_cfc, n_cfp, K_u,
+    K_v], phase_gradient complex128 [n_field, K_u, K_v], device tensors)."""
+    import torch
+    for t, n in ((uvw, "uvw"), (field, "field"), (grid, "grid"), (conv_kernel, "conv_kernel"),
+                 (phase_gradient, "phase_gradient")):
+        _dev(t, n)
+    if field.dtype != torch.int32:
+        raise TypeError("field must be int32")
+    if conv_kernel.dtype != torch.float64 or phase_gradient.dtype != torch.complex128:
+        raise TypeError("conv_kernel must be float64 and phase_gradient complex128")
+    if grid.dtype not in (torch.complex64, torch.complex128) or not grid.is_contiguous():
+        raise TypeError("grid must be a contiguous complex64/complex128 tensor")
+    T, B = uvw.shape[0], uvw.shape[1]
+    fid, fidp = _host(np.asarray(field_id).ravel(), np.int32)
+    fr, frp = _host(np.asarray(freq_chan).ravel(), np.float64)
+    cm, cmp_ = _host(np.asarray(chan_map).ravel(), np.int32)
+    pm, pmp = _host(np.asarray(pol_map).ravel(), np.int32)
+    cb, cbp = _host(np.asarray(cf_baseline_map).ravel(), np.int32)
+    cc, ccp = _host(np.asarray(cf_chan_map).ravel(), np.int32)
+    cpm, cpp = _host(np.asarray(cf_pol_map).ravel(), np.int32)
+    sp_, spp = _host(np.asarray(support).ravel(), np.int32)
+    n_cfb, n_cfc, n_cfp, k_u, k_v = conv_kernel.shape
+    if tuple(phase_gradient.shape[1:]) != (k_u, k_v):
+        raise ValueError("phase_gradient must be [n_field, K_u, K_v]")
+    if sp_.size != n_cfb * n_cfc * n_cfp * 2:
+        raise ValueError("support must be [n_cfb, n_cfc, n_cfp, 2]")
+    C = len(fr)
+    P = len(pm) if n_pol is None else int(n_pol)
+    nic, nip, n_u, n_v = grid.shape
+    dt = torch.complex64 if single else torch.complex128
+    if out is None:
+        out = torch.empty((T, B, C, P), dtype=dt, device=uvw.device)
+    if out.dtype != dt or tuple(out.shape) != (T, B, C, P) or not out.is_contiguous():
+        raise ValueError("out must be a contiguous [T, B, C, P] tensor of the requested dtype")
+    rc = load_library().ngc_aperture_degrid(
+        context().handle, _stream(), _ptr(uvw), _ptr(field), fidp, len(fid), frp, cmp_, pmp,
+        cbp, ccp, cpp, _ptr(conv_kernel), n_cfb, n_cfc, n_cfp, k_u, k_v, spp,
+        _ptr(phase_gradient), T, B, C, P, nic, nip, n_u, n_v, float(delta_lm[0]),
+        float(delta_lm[1]), int(oversampling[0]), int(oversampling[1]), _ptr(grid),
+        C64 if grid.dtype == torch.complex64 else C128, _ptr(out), C64 if single else C128,
+        int(bool(normalize)))
+    if rc == -4:
+        raise IndexError("a row's FIELD_ID is not in the gcf's field_id list")
+    _check("ngc_aperture_degrid", rc)
     return out
 
 

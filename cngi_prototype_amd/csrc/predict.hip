@@ -347,3 +347,236 @@ extern "C" int ngc_standard_degrid(ngc_context *ctx, void *stream, const double 
     phase_end(ctx, s);
     return NGC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Aperture (A-term) degridding: the adjoint of the aperture gridder's loop
+// (_aperture_grid.py:377-513) with unit weights.  One thread per (row,
+// channel); the 64 lanes of a wavefront are 64 consecutive integrations of
+// one (baseline, channel).  Per pol, V = sum conj(CK*PG) G[cell] over the
+// gridder's cells, offsets and taps (v outer, u inner, FP64), divided by
+// conj(sum CK*PG) when normalize.  Taps whose kernel index falls outside the
+// K_u x K_v table weigh 0, as in the tiled gridder.
+// ---------------------------------------------------------------------------
+namespace ngc {
+
+struct DegridApArgs {
+    const double *uvw;
+    const int *field;       // device [T, B]
+    const int *field_id;    // [n_field]
+    int n_field;
+    const double *uv_scale;  // [2][n_chan]
+    const int *chan_map, *pol_map, *cfb, *cfc, *cfp, *support;
+    const double *ck;        // [n_cfb, n_cfc, n_cfp, k_u, k_v]
+    const double2 *pg;       // [n_field, k_u, k_v]
+    int64_t n_cfc, n_cfp, k_u, k_v;
+    int64_t n_time, n_baseline, n_chan, n_pol, n_imag_pol, n_u, n_v, os_u, os_v;
+    int64_t max_sc;          // the largest support (the gridder's bounds margin)
+    int normalize;
+    int *error;
+};
+
+template <typename G, typename V>
+__global__ void __launch_bounds__(256) aperture_degrid_kernel(DegridApArgs a, const G *grid,
+                                                              V *vis) {
+    const int64_t n_tb = (a.n_time + 63) / 64;
+    const int64_t n_thr = a.n_baseline * a.n_chan * n_tb * 64;
+    const int64_t plane = a.n_u * a.n_v;
+    const int64_t kplane = a.k_u * a.k_v;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_thr;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t w = i >> 6;
+        const int64_t t = (w % n_tb) * 64 + (i & 63);
+        const int64_t r = w / n_tb;
+        const int64_t ch = r % a.n_chan, b = r / a.n_chan;
+        if (t >= a.n_time) continue;
+        const int64_t row = t * a.n_baseline + b;
+        V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol);
+        for (int64_t p = 0; p < a.n_pol; p++) {
+            out[2 * p] = (V)0;
+            out[2 * p + 1] = (V)0;
+        }
+        const int fval = a.field[row];
+        if (!(fval > -1)) continue;
+        int fidx = -1;
+        for (int f = 0; f < a.n_field; f++)
+            if (a.field_id[f] == fval) {
+                fidx = f;
+                break;
+            }
+        if (fidx < 0) {
+            atomicOr(a.error, 1);
+            continue;
+        }
+        Pix px, pxv;
+        if (!pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
+                            a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os_u, px))
+            continue;
+        pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
+                       a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os_v, pxv);
+        if (!((px.uc + a.max_sc < a.n_u) && (px.vc + a.max_sc < a.n_v) &&
+              (px.uc - a.max_sc >= 0) && (px.vc - a.max_sc >= 0)))
+            continue;
+        const int64_t u_off = px.du + a.k_u / 2, v_off = pxv.dv + a.k_v / 2;
+        const int64_t a_chan = a.chan_map[ch];
+        const double2 *pg = a.pg + fidx * kplane;
+        for (int64_t p = 0; p < a.n_pol; p++) {
+            const int64_t cf = (a.cfb[b] * a.n_cfc + a.cfc[ch]) * a.n_cfp + a.cfp[p];
+            const int64_t su = a.support[2 * cf], sv = a.support[2 * cf + 1];
+            const int64_t scu = su / 2, scv = sv / 2;
+            const double *ck = a.ck + cf * kplane;
+            const G *g = grid + 2 * ((a_chan * a.n_imag_pol + a.pol_map[p]) * plane);
+            double re = 0.0, im = 0.0, nre = 0.0, nim = 0.0;
+            for (int64_t iv = -scv; iv < sv - scv; iv++) {
+                const int64_t kv = a.os_v * iv + v_off;
+                const int64_t gv = px.vc + iv;
+                for (int64_t iu = -scu; iu < su - scu; iu++) {
+                    const int64_t ku = a.os_u * iu + u_off;
+                    if (ku < 0 || ku >= a.k_u || kv < 0 || kv >= a.k_v) continue;
+                    const int64_t kidx = ku * a.k_v + kv;
+                    const double c = ck[kidx];
+                    const double cr = c * pg[kidx].x, ci = c * pg[kidx].y;
+                    const int64_t cell = (px.uc + iu) * a.n_v + gv;
+                    const double gr = (double)g[2 * cell], gi = (double)g[2 * cell + 1];
+                    re += cr * gr + ci * gi;   // conj(c) g
+                    im += cr * gi - ci * gr;
+                    nre += cr;
+                    nim += ci;
+                }
+            }
+            if (a.normalize) {
+                const double d = nre * nre + nim * nim;   // divide by conj(norm)
+                if (d != 0.0) {
+                    const double r2 = (re * nre - im * nim) / d;
+                    const double i2 = (im * nre + re * nim) / d;
+                    re = r2;
+                    im = i2;
+                }
+            }
+            out[2 * p] = (V)re;
+            out[2 * p + 1] = (V)im;
+        }
+    }
+}
+
+}  // namespace ngc
+
+extern "C" int ngc_aperture_degrid(ngc_context *ctx, void *stream, const double *uvw,
+                                   const int32_t *field, const int32_t *field_id,
+                                   int64_t n_field, const double *freq_chan,
+                                   const int32_t *chan_map, const int32_t *pol_map,
+                                   const int32_t *cf_baseline_map, const int32_t *cf_chan_map,
+                                   const int32_t *cf_pol_map, const double *conv_kernel,
+                                   int64_t n_cfb, int64_t n_cfc, int64_t n_cfp, int64_t k_u,
+                                   int64_t k_v, const int32_t *support,
+                                   const void *phase_gradient, int64_t n_time,
+                                   int64_t n_baseline, int64_t n_chan, int64_t n_pol,
+                                   int64_t n_imag_chan, int64_t n_imag_pol, int64_t n_u,
+                                   int64_t n_v, double delta_l, double delta_m, int64_t os_u,
+                                   int64_t os_v, const void *grid, int grid_dtype, void *vis,
+                                   int vis_dtype, int normalize) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    NGC_CHECK(n_time >= 0 && n_baseline >= 0 && n_chan >= 1 && n_pol >= 1 &&
+                  n_imag_chan >= 1 && n_imag_pol >= 1 && n_u >= 1 && n_v >= 1 && n_field >= 1 &&
+                  n_cfb >= 1 && n_cfc >= 1 && n_cfp >= 1 && k_u >= 1 && k_v >= 1 &&
+                  os_u >= 1 && os_v >= 1,
+              NGC_ERR_ARG, "bad dimensions");
+    NGC_CHECK(freq_chan && chan_map && pol_map && field_id && cf_baseline_map && cf_chan_map &&
+                  cf_pol_map && support,
+              NGC_ERR_ARG, "NULL parameter array");
+    NGC_CHECK(grid_dtype == NGC_C64 || grid_dtype == NGC_C128, NGC_ERR_ARG,
+              "grid dtype must be C64 or C128");
+    NGC_CHECK(vis_dtype == NGC_C64 || vis_dtype == NGC_C128, NGC_ERR_ARG,
+              "vis dtype must be C64 or C128");
+    for (int64_t c = 0; c < n_chan; c++) {
+        NGC_CHECK(chan_map[c] >= 0 && chan_map[c] < n_imag_chan, NGC_ERR_ARG,
+                  "chan_map entry out of range");
+        NGC_CHECK(cf_chan_map[c] >= 0 && cf_chan_map[c] < n_cfc, NGC_ERR_ARG,
+                  "cf_chan_map entry out of range");
+    }
+    for (int64_t p = 0; p < n_pol; p++) {
+        NGC_CHECK(pol_map[p] >= 0 && pol_map[p] < n_imag_pol, NGC_ERR_ARG,
+                  "pol_map entry out of range");
+        NGC_CHECK(cf_pol_map[p] >= 0 && cf_pol_map[p] < n_cfp, NGC_ERR_ARG,
+                  "cf_pol_map entry out of range");
+    }
+    for (int64_t b = 0; b < n_baseline; b++)
+        NGC_CHECK(cf_baseline_map[b] >= 0 && cf_baseline_map[b] < n_cfb, NGC_ERR_ARG,
+                  "cf_baseline_map entry out of range");
+    int64_t max_sc = 0;
+    for (int64_t i = 0; i < n_cfb * n_cfc * n_cfp * 2; i++) {
+        NGC_CHECK(support[i] >= 1, NGC_ERR_ARG, "support entries must be >= 1");
+        max_sc = std::max<int64_t>(max_sc, support[i]);
+    }
+    if (n_time == 0 || n_baseline == 0) return NGC_OK;
+    NGC_CHECK(uvw && field && conv_kernel && phase_gradient && grid && vis, NGC_ERR_ARG,
+              "NULL device array");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = (hipStream_t)stream;
+    NGC_TRY(enter_stream(ctx, s));
+    std::vector<double> scale;
+    uv_scale_host(freq_chan, n_chan, delta_l, delta_m, n_u, n_v, scale);
+    void *d_scale, *d_fid, *d_cm, *d_pm, *d_cfb, *d_cfc, *d_cfp, *d_sup;
+    NGC_TRY(upload(ctx, s, "adg_scale", scale.data(), scale.size() * sizeof(double), &d_scale));
+    NGC_TRY(upload(ctx, s, "adg_field_id", field_id, n_field * sizeof(int32_t), &d_fid));
+    NGC_TRY(upload(ctx, s, "adg_chan_map", chan_map, n_chan * sizeof(int32_t), &d_cm));
+    NGC_TRY(upload(ctx, s, "adg_pol_map", pol_map, n_pol * sizeof(int32_t), &d_pm));
+    NGC_TRY(upload(ctx, s, "adg_cfb", cf_baseline_map, n_baseline * sizeof(int32_t), &d_cfb));
+    NGC_TRY(upload(ctx, s, "adg_cfc", cf_chan_map, n_chan * sizeof(int32_t), &d_cfc));
+    NGC_TRY(upload(ctx, s, "adg_cfp", cf_pol_map, n_pol * sizeof(int32_t), &d_cfp));
+    NGC_TRY(upload(ctx, s, "adg_support", support, n_cfb * n_cfc * n_cfp * 2 * sizeof(int32_t),
+                   &d_sup));
+    DegridApArgs a;
+    a.uvw = uvw;
+    a.field = field;
+    a.field_id = (const int *)d_fid;
+    a.n_field = (int)n_field;
+    a.uv_scale = (const double *)d_scale;
+    a.chan_map = (const int *)d_cm;
+    a.pol_map = (const int *)d_pm;
+    a.cfb = (const int *)d_cfb;
+    a.cfc = (const int *)d_cfc;
+    a.cfp = (const int *)d_cfp;
+    a.support = (const int *)d_sup;
+    a.ck = conv_kernel;
+    a.pg = (const double2 *)phase_gradient;
+    a.n_cfc = n_cfc;
+    a.n_cfp = n_cfp;
+    a.k_u = k_u;
+    a.k_v = k_v;
+    a.n_time = n_time;
+    a.n_baseline = n_baseline;
+    a.n_chan = n_chan;
+    a.n_pol = n_pol;
+    a.n_imag_pol = n_imag_pol;
+    a.n_u = n_u;
+    a.n_v = n_v;
+    a.os_u = os_u;
+    a.os_v = os_v;
+    a.max_sc = max_sc;
+    a.normalize = normalize ? 1 : 0;
+    a.error = ctx->d_error;
+    NGC_HIP(hipMemsetAsync(ctx->d_error, 0, sizeof(int), s));
+    const int64_t n_thr = n_baseline * n_chan * ceil_div(n_time, 64) * 64;
+    const int nb = pblocks(n_thr);
+    profile_new_call(ctx, s);
+    phase_begin(ctx, s, "aperture_degrid");
+    if (grid_dtype == NGC_C64 && vis_dtype == NGC_C64)
+        aperture_degrid_kernel<float, float><<<nb, 256, 0, s>>>(a, (const float *)grid,
+                                                                (float *)vis);
+    else if (grid_dtype == NGC_C64)
+        aperture_degrid_kernel<float, double><<<nb, 256, 0, s>>>(a, (const float *)grid,
+                                                                 (double *)vis);
+    else if (vis_dtype == NGC_C64)
+        aperture_degrid_kernel<double, float><<<nb, 256, 0, s>>>(a, (const double *)grid,
+                                                                 (float *)vis);
+    else
+        aperture_degrid_kernel<double, double><<<nb, 256, 0, s>>>(a, (const double *)grid,
+                                                                  (double *)vis);
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    int err = 0;
+    NGC_HIP(hipMemcpyAsync(&err, ctx->d_error, sizeof(int), hipMemcpyDeviceToHost, s));
+    NGC_HIP(hipStreamSynchronize(s));
+    NGC_CHECK(err == 0, NGC_ERR_FIELD, "a row's field id is not in field_id");
+    return NGC_OK;
+}

@@ -372,6 +372,36 @@ int ngc_standard_degrid(ngc_context *ctx, void *stream, const double *uvw,
                         int normalize);
 
 /*
+ * ngc_aperture_degrid: visibility prediction with the mosaic A-term kernels,
+ * the adjoint of ngc_aperture_grid (image mode, _aperture_grid.py:377-513)
+ * with unit weights.  For every (time, baseline, chan, pol) sample that the
+ * aperture gridder places (row field id found in field_id, same uv scale,
+ * centre cell, sub-cell offsets in os_u / os_v, bounds by the largest
+ * support, CK * PG[field] taps),
+ *   V = sum_taps conj(CK * PG) * grid[chan_map[c], pol_map[p], cell]
+ * divided by conj(sum_taps CK * PG) when `normalize` (a constant grid then
+ * predicts the constant); 0 for rows on field <= -1 and samples off the grid.
+ * Arrays as for ngc_aperture_grid: field device int32 [T,B]; field_id,
+ * freq_chan, chan_map, pol_map, cf_*_map and support host; conv_kernel device
+ * float64 [n_cfb, n_cfc, n_cfp, k_u, k_v]; phase_gradient device complex128
+ * [n_field, k_u, k_v]; grid device [n_imag_chan, n_imag_pol, n_u, n_v] of
+ * grid_dtype (C64|C128); vis device out [T, B, C, P] of vis_dtype (C64|C128).
+ * Returns NGC_ERR_FIELD if a row's field id is not in field_id (synchronises
+ * `stream`).
+ */
+int ngc_aperture_degrid(ngc_context *ctx, void *stream, const double *uvw, const int32_t *field,
+                        const int32_t *field_id, int64_t n_field, const double *freq_chan,
+                        const int32_t *chan_map, const int32_t *pol_map,
+                        const int32_t *cf_baseline_map, const int32_t *cf_chan_map,
+                        const int32_t *cf_pol_map, const double *conv_kernel, int64_t n_cfb,
+                        int64_t n_cfc, int64_t n_cfp, int64_t k_u, int64_t k_v,
+                        const int32_t *support, const void *phase_gradient, int64_t n_time,
+                        int64_t n_baseline, int64_t n_chan, int64_t n_pol, int64_t n_imag_chan,
+                        int64_t n_imag_pol, int64_t n_u, int64_t n_v, double delta_l,
+                        double delta_m, int64_t os_u, int64_t os_v, const void *grid,
+                        int grid_dtype, void *vis, int vis_dtype, int normalize);
+
+/*
  * Gridding plans (no counterpart in the reference, whose graph layer rebuilds
  * its dask graph on every call).  A plan fixes every argument of one
  * ngc_standard_grid call in a tiled precision (IMAGE or PSF mode): the input,

@@ -233,6 +233,42 @@ def aperture_grid(vis, uvw, weight, field, field_id, cf_baseline_map, cf_chan_ma
     return grid, sw
 
 
+def aperture_degrid(grid, uvw, field, field_id, cf_baseline_map, cf_chan_map, cf_pol_map,
+                    conv_kernel, weight_support, phase_gradient, freq_chan, n_uv, delta_lm,
+                    oversampling, chan_mode="cube", normalize=True, n_pol=None):
+    """Visibility prediction with the A-term kernels: the adjoint of
+    aperture_grid's loop with unit weights (same rows, fields, cells, offsets
+    and CK*PG taps), V = sum conj(CK*PG) G[cell], divided by conj(sum CK*PG)
+    when normalize.  grid [n_imag_chan, n_pol, n_u, n_v] complex; returns vis
+    [T, B, C, P] complex128 (0 for rows and samples that are not gridded)."""
+    grid = np.ascontiguousarray(grid, dtype=np.complex128)
+    uvw = _f64(uvw)
+    freq_chan = _f64(freq_chan)
+    n_time, n_baseline, n_chan = uvw.shape[0], uvw.shape[1], len(freq_chan)
+    n_pol = grid.shape[1] if n_pol is None else int(n_pol)
+    c = _aperture_common(uvw, np.zeros((n_time, n_baseline, n_chan, n_pol)), field, field_id,
+                         cf_baseline_map, cf_chan_map, cf_pol_map, conv_kernel,
+                         weight_support, phase_gradient, chan_mode)
+    k = c["kernel"]
+    vis = np.zeros((n_time, n_baseline, n_chan, n_pol), dtype=np.complex128)
+    L = _lib()
+    L.oracle_aperture_degrid.restype = ctypes.c_int
+    rc = L.oracle_aperture_degrid(
+        _p(vis.view(np.float64)), _p(grid.view(np.float64)), _p(c["uvw"]), _p(freq_chan),
+        _p(c["chan_map"]), _p(c["pol_map"]), _p(c["cfb"]), _p(c["cfc"]), _p(c["cfp"]), _p(k),
+        ctypes.c_int64(k.shape[0]), ctypes.c_int64(k.shape[1]), ctypes.c_int64(k.shape[2]),
+        ctypes.c_int64(k.shape[3]), ctypes.c_int64(k.shape[4]), ctypes.c_int64(n_time),
+        ctypes.c_int64(n_baseline), ctypes.c_int64(n_chan), ctypes.c_int64(n_pol),
+        ctypes.c_int64(grid.shape[1]), ctypes.c_int64(int(n_uv[0])), ctypes.c_int64(int(n_uv[1])),
+        ctypes.c_double(float(delta_lm[0])), ctypes.c_double(float(delta_lm[1])),
+        _p(c["support"]), ctypes.c_int64(int(oversampling[0])),
+        ctypes.c_int64(int(oversampling[1])), _p(c["field"]), _p(c["field_id"]),
+        ctypes.c_int64(len(c["field_id"])), _p(c["pg"]), ctypes.c_int(int(bool(normalize))))
+    if rc != 0:
+        raise IndexError("field id not found in the gcf field_id list")
+    return vis
+
+
 def aperture_weight_grid(uvw, weight, field, field_id, cf_baseline_map,
                          cf_chan_map, cf_pol_map, weight_conv_kernel,
                          weight_support, phase_gradient, freq_chan, n_uv,

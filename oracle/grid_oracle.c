@@ -480,6 +480,134 @@ int oracle_aperture_grid(double *grid, double *sum_weight, int do_psf,
 }
 
 /* ----------------------------------------------------------------------- */
+/* Aperture degridding (no counterpart in the reference, whose prediction    */
+/* is a stub: ngcasa/imaging/predict_modelvis_image.py:20).  Defined as the  */
+/* adjoint of oracle_aperture_grid's loop with unit weights: the same rows,  */
+/* fields, cells, offsets and CK*PG taps, each sample reading               */
+/*   V = sum_taps conj(CK*PG) G[cell]                                        */
+/* and, when normalize, V / conj(sum_taps CK*PG) (a constant grid c then     */
+/* predicts c).  Samples that are not placed on the grid get 0.             */
+/* ----------------------------------------------------------------------- */
+int oracle_aperture_degrid(double *vis, const double *grid, const double *uvw,
+                           const double *freq_chan, const int64_t *chan_map,
+                           const int64_t *pol_map, const int64_t *cf_baseline_map,
+                           const int64_t *cf_chan_map, const int64_t *cf_pol_map,
+                           const double *conv_kernel, int64_t n_cfb, int64_t n_cfc,
+                           int64_t n_cfp, int64_t k_u, int64_t k_v, int64_t n_time,
+                           int64_t n_baseline, int64_t n_chan, int64_t n_pol,
+                           int64_t n_imag_pol, int64_t n_u, int64_t n_v,
+                           double delta_l, double delta_m, const int64_t *weight_support,
+                           int64_t os_u, int64_t os_v, const int64_t *field,
+                           const int64_t *field_id, int64_t n_field,
+                           const double *phase_gradient, int normalize)
+{
+    double *uv_scale0 = (double *)malloc(sizeof(double) * (size_t)n_chan);
+    double *uv_scale1 = (double *)malloc(sizeof(double) * (size_t)n_chan);
+    for (int64_t c = 0; c < n_chan; c++) {
+        uv_scale0[c] = -((freq_chan[c] * delta_l) * (double)n_u) / C_LIGHT;
+        uv_scale1[c] = -((freq_chan[c] * delta_m) * (double)n_v) / C_LIGHT;
+    }
+    const int64_t uv_center0 = n_u / 2, uv_center1 = n_v / 2;
+    int64_t max_support_center = weight_support[0];
+    for (int64_t i = 1; i < n_cfb * n_cfc * n_cfp * 2; i++)
+        if (weight_support[i] > max_support_center)
+            max_support_center = weight_support[i];
+    const int64_t conv_v_center = k_v / 2;
+    const int64_t conv_u_center = k_u / 2;
+    const int64_t kplane = k_u * k_v;
+    const int64_t plane = n_u * n_v;
+    for (int64_t i = 0; i < 2 * n_time * n_baseline * n_chan * n_pol; i++)
+        vis[i] = 0.0;
+
+    for (int64_t t = 0; t < n_time; t++) {
+        for (int64_t b = 0; b < n_baseline; b++) {
+            const int64_t fval = field[t * n_baseline + b];
+            if (!(fval > -1))
+                continue;
+            int64_t field_indx = -1;
+            for (int64_t f = 0; f < n_field; f++)
+                if (field_id[f] == fval) { field_indx = f; break; }
+            if (field_indx < 0) {
+                free(uv_scale0);
+                free(uv_scale1);
+                return -1;
+            }
+            const double *pg = phase_gradient + 2 * field_indx * kplane;
+            const int64_t cf_baseline = cf_baseline_map[b];
+            for (int64_t ch = 0; ch < n_chan; ch++) {
+                const int64_t cf_chan = cf_chan_map[ch];
+                const int64_t a_chan = chan_map[ch];
+                const double *uvw_tb = uvw + (t * n_baseline + b) * 3;
+                const double u = uvw_tb[0] * uv_scale0[ch];
+                const double v = uvw_tb[1] * uv_scale1[ch];
+                if (is_nan(u) || is_nan(v))
+                    continue;
+                const double u_pos = u + (double)uv_center0;
+                const double v_pos = v + (double)uv_center1;
+                const int64_t u_center_indx = (int64_t)(u_pos + 0.5);
+                const int64_t v_center_indx = (int64_t)(v_pos + 0.5);
+                if (!((u_center_indx + max_support_center < n_u) &&
+                      (v_center_indx + max_support_center < n_v) &&
+                      (u_center_indx - max_support_center >= 0) &&
+                      (v_center_indx - max_support_center >= 0)))
+                    continue;
+                const double u_offset = (double)u_center_indx - u_pos;
+                const int64_t u_center_offset_indx =
+                    (int64_t)floor(u_offset * (double)os_u + 0.5) + conv_u_center;
+                const double v_offset = (double)v_center_indx - v_pos;
+                const int64_t v_center_offset_indx =
+                    (int64_t)floor(v_offset * (double)os_v + 0.5) + conv_v_center;
+                const int64_t base = ((t * n_baseline + b) * n_chan + ch) * n_pol;
+                for (int64_t p = 0; p < n_pol; p++) {
+                    const int64_t cf_pol = cf_pol_map[p];
+                    const int64_t a_pol = pol_map[p];
+                    const int64_t sidx = ((cf_baseline * n_cfc + cf_chan) * n_cfp + cf_pol) * 2;
+                    const int64_t support_u = weight_support[sidx + 0];
+                    const int64_t support_v = weight_support[sidx + 1];
+                    const int64_t scu = support_u / 2, scv = support_v / 2;
+                    const double *ck = conv_kernel + ((cf_baseline * n_cfc + cf_chan) * n_cfp + cf_pol) * kplane;
+                    const int64_t pbase = (a_chan * n_imag_pol + a_pol) * plane;
+                    double re = 0.0, im = 0.0, norm_re = 0.0, norm_im = 0.0;
+                    for (int64_t iv = -scv; iv < support_v - scv; iv++) {
+                        const int64_t v_indx = v_center_indx + iv;
+                        const int64_t cf_v_indx = os_v * iv + v_center_offset_indx;
+                        for (int64_t iu = -scu; iu < support_u - scu; iu++) {
+                            const int64_t u_indx = u_center_indx + iu;
+                            const int64_t cf_u_indx = os_u * iu + u_center_offset_indx;
+                            const int64_t kidx = cf_u_indx * k_v + cf_v_indx;
+                            const double c_re = ck[kidx] * pg[2 * kidx];
+                            const double c_im = ck[kidx] * pg[2 * kidx + 1];
+                            const int64_t cell = pbase + u_indx * n_v + v_indx;
+                            const double g_re = grid[2 * cell], g_im = grid[2 * cell + 1];
+                            /* conj(c) * g */
+                            re = re + (c_re * g_re + c_im * g_im);
+                            im = im + (c_re * g_im - c_im * g_re);
+                            norm_re = norm_re + c_re;
+                            norm_im = norm_im + c_im;
+                        }
+                    }
+                    if (normalize) {
+                        /* divide by conj(norm) */
+                        const double d = norm_re * norm_re + norm_im * norm_im;
+                        if (d != 0.0) {
+                            const double r2 = (re * norm_re - im * norm_im) / d;
+                            const double i2 = (im * norm_re + re * norm_im) / d;
+                            re = r2;
+                            im = i2;
+                        }
+                    }
+                    vis[2 * (base + p)] = re;
+                    vis[2 * (base + p) + 1] = im;
+                }
+            }
+        }
+    }
+    free(uv_scale0);
+    free(uv_scale1);
+    return 0;
+}
+
+/* ----------------------------------------------------------------------- */
 /* _aperture_weight_grid_jit (_aperture_grid.py:181-291)                    */
 /* ----------------------------------------------------------------------- */
 int oracle_aperture_weight_grid(double *grid, double *sum_weight,

@@ -108,12 +108,16 @@ def test_aperture_grid_matches_oracle(gpu, case, mode, chan_mode, precision):
         ref["SUPPORT"], _dev(ref["PHASE_GRADIENT"]), n_imag, npad, case["delta"], OS,
         native_mode, precision, vis=vd)
     assert g.dtype == (torch.complex64 if single else torch.complex128)
-    # The plain FP32 form adds its register runs into uncompensated FP32 cells;
-    # the rounding of those sums depends on the order of the records inside a
-    # bin, which the binning's atomic cursors leave open, so it varies from run
-    # to run (measured up to 1.2e-6 in continuum mode, where every channel
-    # lands in one plane).  The compensated form stays well below 1e-6.
-    tol = {"fp32": 3e-6, "fp32_kahan": 1e-6}.get(precision, 1e-12)
+    # The single-precision forms sum each lane's register run of taps in plain
+    # FP32 (the compensated form then adds the runs into compensated cells,
+    # the plain one into FP32 cells).  The rounding depends on the order of the
+    # records inside a bin, which the binning's atomic cursors leave open, so
+    # it varies from run to run; in continuum mode, where every channel lands
+    # in one plane, both forms have measured up to 1.3e-6.
+    if precision in ("fp32", "fp32_kahan"):
+        tol = 3e-6 if (chan_mode == "continuum" or precision == "fp32") else 1e-6
+    else:
+        tol = 1e-12
     assert rel_err(g.cpu().numpy(), g_ref) < tol
     assert rel_err(sw.cpu().numpy(), sw_ref) < 1e-12
 

@@ -1,5 +1,5 @@
 """Visibility prediction on the GPU (ngc_model_to_grid, ngc_standard_degrid,
-predict_modelvis_image) against the oracle restatement, which the CPU tests
+ngc_aperture_degrid, predict_modelvis_image) against the oracle restatement, which the CPU tests
 pin by adjointness with the gridder, closed forms and point-source
 visibilities (the reference itself has no implementation)."""
 import numpy as np
@@ -95,3 +95,56 @@ def test_predict_modelvis_image_point_sources(gpu):
     assert np.abs(v[..., 1] - exp2).max() < 1e-2
     out2 = predict_modelvis_image(img, out, gp, {"incremental": True})
     assert np.allclose(out2["MODEL"].data.cpu().numpy(), 2 * v, rtol=1e-12, atol=1e-12)
+
+
+def _ap_case(seed):
+    """Two convolution functions (supports 5 and 7), three fields with random
+    unit phase gradients, random real CK, a row on field -1."""
+    rng = np.random.default_rng(seed)
+    c = small_case(n_time=16, n_chan=3, n_pol=2)
+    T, B = c["uvw"].shape[:2]
+    os_, K = [4, 5], [40, 48]
+    ck = rng.uniform(0.1, 1.0, size=(2, 1, 1, K[0], K[1]))
+    sup = np.array([[5, 5], [7, 7]]).reshape(2, 1, 1, 2)
+    pg = np.exp(1j * rng.uniform(-np.pi, np.pi, size=(3, K[0], K[1])))
+    field = rng.integers(0, 3, size=(T, B)) + 10
+    field[2, 3] = -1
+    maps = (rng.integers(0, 2, size=B), np.zeros(3, int), np.zeros(2, int))
+    return c, ck, sup, pg, field, [10, 11, 12], maps, os_
+
+
+@pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("normalize", [True, False])
+def test_aperture_degrid_matches_oracle(gpu, single, normalize):
+    import torch
+    import cngi_prototype_amd._native as nat
+    c, ck, sup, pg, field, fid, maps, os_ = _ap_case(6)
+    npad = c["npad"]
+    rng = np.random.default_rng(9)
+    G = rng.normal(size=(3, 2, npad, npad)) + 1j * rng.normal(size=(3, 2, npad, npad))
+    if single:
+        G = G.astype(np.complex64).astype(np.complex128)
+    ref = oracle.aperture_degrid(G, c["uvw"], field, fid, *maps, ck, sup, pg, c["freq"],
+                                 [npad, npad], c["delta"], os_, normalize=normalize)
+    got = nat.aperture_degrid(_t(c["uvw"]), _t(field, torch.int32), fid,
+                              _t(G, torch.complex64 if single else torch.complex128), c["freq"],
+                              np.arange(3), np.arange(2), *maps, _t(ck), sup, _t(pg), c["delta"],
+                              os_, normalize=normalize, single=single)
+    got = got.cpu().numpy()
+    assert np.count_nonzero(ref) > 0
+    assert np.array_equal(got == 0, ref == 0)
+    assert rel_err(got, ref) < (1e-6 if single else 1e-12)
+
+
+def test_aperture_degrid_unknown_field_raises(gpu):
+    import torch
+    import cngi_prototype_amd._native as nat
+    c, ck, sup, pg, field, fid, maps, os_ = _ap_case(6)
+    npad = c["npad"]
+    field = field.copy()
+    field[1, 1] = 99
+    G = np.ones((3, 2, npad, npad), dtype=np.complex128)
+    with pytest.raises(IndexError):
+        nat.aperture_degrid(_t(c["uvw"]), _t(field, torch.int32), fid, _t(G), c["freq"],
+                            np.arange(3), np.arange(2), *maps, _t(ck), sup, _t(pg), c["delta"],
+                            os_, single=False)

@@ -752,6 +752,85 @@ def test_degrid_of_a_constant_grid_is_one():
         oracle.standard_degrid(np.ones_like(G), uvw, freq, cgk, n_uv, delta, 7, 100))
 
 
+def _ap_degrid_case(seed, T=10, fill=0.8):
+    """Two convolution functions (supports 5 and 7), three fields with random
+    unit phase gradients, random real CK, one row on field -1 (not gridded)."""
+    rng = np.random.default_rng(seed)
+    vis, uvw, w, freq, delta = _tiny_case(seed, T=T, fill=fill)
+    T, B, NC, P = w.shape
+    os_ = [4, 5]
+    K = [40, 48]
+    ck = rng.uniform(0.1, 1.0, size=(2, 1, 1, K[0], K[1]))
+    sup = np.array([[5, 5], [7, 7]]).reshape(2, 1, 1, 2)
+    pg = np.exp(1j * rng.uniform(-np.pi, np.pi, size=(3, K[0], K[1])))
+    field = rng.integers(0, 3, size=(T, B)) + 10
+    field[2, 3] = -1
+    maps = (rng.integers(0, 2, size=B), np.zeros(NC, int), np.zeros(P, int))
+    return vis, uvw, freq, delta, ck, sup, pg, field, [10, 11, 12], maps, os_
+
+
+def test_aperture_degrid_is_the_adjoint_of_aperture_grid():
+    """<degrid(G), V> = <G, grid(V)> with unit weights for complex CK*PG taps:
+    the degridder reads conj(CK*PG) on the gridder's cells, offsets and taps
+    (a missing conjugate, a swapped os_u/os_v or a tap slip breaks it)."""
+    rng = np.random.default_rng(31)
+    vis, uvw, freq, delta, ck, sup, pg, field, fid, maps, os_ = _ap_degrid_case(3)
+    T, B, NC, P = vis.shape
+    n_uv = [52, 48]
+    G = rng.normal(size=(NC, P) + tuple(n_uv)) + 1j * rng.normal(size=(NC, P) + tuple(n_uv))
+    gv, _ = oracle.aperture_grid(vis, uvw, np.ones(vis.shape), field, fid, *maps, ck, sup, pg,
+                                 freq, n_uv, delta, os_, do_psf=False)
+    dg = oracle.aperture_degrid(G, uvw, field, fid, *maps, ck, sup, pg, freq, n_uv, delta, os_,
+                                normalize=False)
+    lhs = np.vdot(dg, vis)
+    rhs = np.vdot(G, gv)
+    assert abs(lhs - rhs) < 1e-11 * abs(rhs)
+    assert np.count_nonzero(dg) > 0
+    assert np.all(dg[2, 3] == 0)   # the row on field -1
+
+
+def test_aperture_degrid_of_a_constant_grid():
+    """Normalised by conj(sum CK*PG), a constant grid predicts the constant on
+    exactly the samples the aperture gridder places, and 0 elsewhere."""
+    vis, uvw, freq, delta, ck, sup, pg, field, fid, maps, os_ = _ap_degrid_case(5, fill=1.4)
+    T, B, NC, P = vis.shape
+    n_uv = [52, 48]
+    G = np.full((NC, P) + tuple(n_uv), 0.5 + 2.0j)
+    dg = oracle.aperture_degrid(G, uvw, field, fid, *maps, ck, sup, pg, freq, n_uv, delta, os_)
+    on = dg != 0
+    assert on.any() and (~on).any()
+    assert np.allclose(dg[on], 0.5 + 2.0j, rtol=1e-12)
+    # placed samples: those the gridder gives a nonzero footprint with unit data
+    one = np.ones(vis.shape, dtype=np.complex128)
+    for t in range(T):
+        for b in range(B):
+            for c in range(NC):
+                sel = np.zeros(vis.shape)
+                sel[t, b, c, 0] = 1.0
+                g, _ = oracle.aperture_grid(one, uvw, sel, field, fid, *maps, ck, sup, pg, freq,
+                                            n_uv, delta, os_, do_psf=False)
+                assert bool(np.any(g != 0)) == bool(on[t, b, c, 0])
+
+
+def test_aperture_degrid_reduces_to_standard_degrid():
+    """With a separable prolate-spheroidal CK, unit phase gradient and
+    SUPPORT = 7 the A-term degridder is the standard one."""
+    rng = np.random.default_rng(8)
+    vis, uvw, w, freq, delta = _tiny_case(7, T=6, fill=0.35)
+    T, B, NC, P = w.shape
+    os_, K = 100, 800
+    ck = _separable_ps_kernel(os_, K)[None, None, None]
+    n_uv = [44, 44]
+    G = rng.normal(size=(NC, P) + tuple(n_uv)) + 1j * rng.normal(size=(NC, P) + tuple(n_uv))
+    da = oracle.aperture_degrid(G, uvw, np.zeros((T, B), int), [0], np.zeros(B, int),
+                                np.zeros(NC, int), np.zeros(P, int), ck, np.full((1, 1, 1, 2), 7),
+                                np.ones((1, K, K), dtype=np.complex128), freq, n_uv, delta,
+                                [os_, os_])
+    ds = oracle.standard_degrid(G, uvw, freq, oracle.ps_kernel_1d(os_, 7), n_uv, delta, 7, os_)
+    assert np.count_nonzero(ds) > 0
+    assert np.allclose(da, ds, rtol=0, atol=1e-12 * np.abs(ds).max())
+
+
 def test_model_to_grid_single_pixel_closed_form():
     """A pixel of flux S at (l0, m0) of an N x N model gives
     G[u][v] = S / corr(l0, m0) exp(-2 pi i ((u - n_u//2)(l0 - N//2)/n_u +
