@@ -22,6 +22,13 @@ pol)); vis_dataset holds UVW and the chan coordinate; grid_parms as for
 make_image (image_size, cell_size, fft_padding, chan_mode, optional
 precision); storage_parms: arr_name (default 'MODEL'), incremental
 (default False: overwrite; True: add to the existing variable).
+
+With `gcf_dataset` (the output of make_gridding_convolution_function) the
+prediction uses the mosaic A-term kernels instead: the model is transformed
+without the prolate-spheroidal correction and every visibility is
+sum conj(CK PG) G / conj(sum CK PG) over the cells, offsets and taps the
+aperture gridder would give it (ngc_aperture_degrid, the adjoint of the
+aperture gridder with unit weights); vis_dataset then needs FIELD_ID.
 """
 import copy
 
@@ -36,9 +43,10 @@ from ._imaging_utils._gridding_convolutional_kernels import cropped_correction_f
     prolate_spheroidal_kernel_1d
 
 
-def predict_modelvis_image(img_dataset, vis_dataset, grid_parms, storage_parms):
+def predict_modelvis_image(img_dataset, vis_dataset, grid_parms, storage_parms, gcf_dataset=None):
     """Model visibilities [time, baseline, chan, pol] from a model image cube
-    (or a one-channel continuum model with chan_mode 'continuum')."""
+    (or a one-channel continuum model with chan_mode 'continuum'); with
+    gcf_dataset, through the A-term kernels."""
     import torch
     print('######################### Start predict_modelvis_image #########################')
     _grid_parms = copy.deepcopy(grid_parms)
@@ -71,15 +79,31 @@ def predict_modelvis_image(img_dataset, vis_dataset, grid_parms, storage_parms):
         chan_map = np.zeros(n_chan, dtype=int)
 
     single = _grid_parms.get('precision', 'fp64') in ('fp32_kahan', 'fp32')
-    oversampling, support = 100, 7
-    cgk_1d = prolate_spheroidal_kernel_1d(oversampling, support)
-    corr_l, corr_m = cropped_correction_factors(_grid_parms['image_size_padded'],
-                                                _grid_parms['image_size'])
-    grid = _native.model_to_grid(m, _grid_parms['image_size_padded'], corr_l, corr_m,
-                                 single=single)
-    vis = _native.standard_degrid(uvw, grid, freq, chan_map, np.arange(n_mpol), cgk_1d,
-                                  _grid_parms['cell_size'], support, oversampling,
-                                  single=single)
+    if gcf_dataset is None:
+        oversampling, support = 100, 7
+        cgk_1d = prolate_spheroidal_kernel_1d(oversampling, support)
+        corr_l, corr_m = cropped_correction_factors(_grid_parms['image_size_padded'],
+                                                    _grid_parms['image_size'])
+        grid = _native.model_to_grid(m, _grid_parms['image_size_padded'], corr_l, corr_m,
+                                     single=single)
+        vis = _native.standard_degrid(uvw, grid, freq, chan_map, np.arange(n_mpol), cgk_1d,
+                                      _grid_parms['cell_size'], support, oversampling,
+                                      single=single)
+    else:
+        assert 'FIELD_ID' in vis_dataset, \
+            "######### ERROR: A-term prediction needs FIELD_ID in the visibility dataset"
+        grid = _native.model_to_grid(m, _grid_parms['image_size_padded'], single=single)
+        vis = _native.aperture_degrid(
+            uvw, to_device(vis_dataset['FIELD_ID'].data, torch.int32),
+            np.asarray(gcf_dataset.coords['field_id'].values), grid, freq, chan_map,
+            np.arange(n_mpol), np.asarray(gcf_dataset['CF_BASELINE_MAP'].values),
+            np.asarray(gcf_dataset['CF_CHAN_MAP'].values),
+            np.asarray(gcf_dataset['CF_POL_MAP'].values),
+            to_device(gcf_dataset['CONV_KERNEL'].data, torch.float64),
+            np.asarray(gcf_dataset['SUPPORT'].values),
+            to_device(gcf_dataset['PHASE_GRADIENT'].data, torch.complex128),
+            _grid_parms['cell_size'], np.asarray(gcf_dataset.attrs['oversampling']),
+            single=single)
     del grid
     out = copy_structure(vis_dataset)
     if incremental and arr_name in out.data_vars:

@@ -148,3 +148,44 @@ def test_aperture_degrid_unknown_field_raises(gpu):
         nat.aperture_degrid(_t(c["uvw"]), _t(field, torch.int32), fid, _t(G), c["freq"],
                             np.arange(3), np.arange(2), *maps, _t(ck), sup, _t(pg), c["delta"],
                             os_, single=False)
+
+
+def test_predict_modelvis_image_with_gcf(gpu):
+    """predict_modelvis_image with a gcf dataset: the model, transformed
+    without the spheroidal correction, is degridded with the A-term kernels,
+    fields and maps of the gcf, as the oracle degrids the same grid."""
+    import torch
+    import cngi_prototype_amd._native as nat
+    from cngi_prototype_amd.imaging import predict_modelvis_image
+    from cngi_prototype_amd.xds import Dataset
+    c, ck, sup, pg, field, fid, maps, os_ = _ap_case(12)
+    n, npad = c["image_size"], c["npad"]
+    rng = np.random.default_rng(13)
+    T, B = c["uvw"].shape[:2]
+    vis_ds = Dataset(data_vars={"UVW": (["time", "baseline", "uvw_index"], _t(c["uvw"])),
+                                "FIELD_ID": (["time", "baseline"], field)},
+                     coords={"chan": (["chan"], c["freq"]), "pol": (["pol"], np.array([9, 12]))})
+    gcf = Dataset(data_vars={"CONV_KERNEL": (["cf_b", "cf_c", "cf_p", "k_u", "k_v"], ck),
+                             "SUPPORT": (["cf_b", "cf_c", "cf_p", "d"], sup),
+                             "PHASE_GRADIENT": (["field_id", "k_u", "k_v"], pg),
+                             "CF_BASELINE_MAP": (["baseline"], maps[0]),
+                             "CF_CHAN_MAP": (["chan"], maps[1]),
+                             "CF_POL_MAP": (["pol"], maps[2])},
+                  coords={"field_id": (["field_id"], np.array(fid))},
+                  attrs={"oversampling": np.array(os_)})
+    model = np.zeros((n, n, 1, 3, 2))
+    for _ in range(6):
+        model[rng.integers(n // 4, 3 * n // 4), rng.integers(n // 4, 3 * n // 4), 0, :, :] += \
+            rng.uniform(0.1, 2.0)
+    img = Dataset(data_vars={"MODEL": (["l", "m", "time", "chan", "pol"], model)})
+    cell = c["cell_arcsec"]
+    gp = {"image_size": [n, n], "cell_size": [cell, cell], "fft_padding": 1.2,
+          "chan_mode": "cube"}
+    out = predict_modelvis_image(img, vis_ds, gp, {}, gcf_dataset=gcf)
+    v = out["MODEL"].data.cpu().numpy()
+    m = _t(np.ascontiguousarray(np.moveaxis(model[:, :, 0], [0, 1], [-2, -1])))
+    G = nat.model_to_grid(m, [npad, npad], single=False).cpu().numpy()
+    ref = oracle.aperture_degrid(G, c["uvw"], field, fid, *maps, ck, sup, pg, c["freq"],
+                                 [npad, npad], c["delta"], os_)
+    assert np.count_nonzero(ref) > 0
+    assert rel_err(v, ref) < 1e-12
