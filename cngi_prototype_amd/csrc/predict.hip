@@ -458,6 +458,226 @@ __global__ void __launch_bounds__(256) aperture_degrid_kernel(DegridApArgs a, co
     }
 }
 
+// ---------------------------------------------------------------------------
+// Separable form of the same sum, for phase gradients of the form
+// exp(i(x px + y py)) (the gcf builder's), PG[x][y] = PGu[x] PGv[y] with
+// PGu[x] = PG[x][cv], PGv[y] = PG[cu][y] (so PG[cu][cv] = 1).  Then
+//   conj(CK PG) G = conj(PGu) conj(CK PGv) G,
+// so per u row the inner sum S_u = sum_v conj(CK PGv) G and the outer sum
+// sum_u conj(PGu) S_u take 4 + 4 instead of 2 + 8 multiply-adds per tap and
+// pol.  All sums stay in FP64 with FP64 tables, whatever the grid's
+// precision: towards an offset pointing CK PG turns many times across the
+// support, the sums cancel to a small fraction of sum |CK PG|, and FP32
+// sums lose that fraction's digits (1.5e-4 relative error on a 25-pointing
+// mosaic).  FP64 multiply-adds issue at the rate of unpacked FP32 ones on
+// gfx950, so this costs only the tables' width.  Tables (rebuilt per call):
+//  - adg_taps: CK re-laid per (convolution function, sub-cell offset pair) as
+//    an R x R block, R the largest support, ckr[cf][ou][ov][ru][rv] =
+//    CK[os_u (ru - su/2) + ou + K_u/2][os_v (rv - sv/2) + ov + K_v/2], zero
+//    where that index leaves the K_u x K_v table (a zero weight, as in the
+//    general form);
+//  - adg_pgu / adg_pgv: the factors over an index range widened by E_u / E_v
+//    on both sides (zero outside the table) so no tap index needs a clamp.
+// A wavefront is 64 integrations of one (baseline, channel), so the
+// convolution function and the loop bounds are wave-uniform; pols that share
+// a convolution function (cf_pol_map constant) share the tap weights.
+// ---------------------------------------------------------------------------
+struct AdgTab {
+    int R, hu, hv;            // largest support; offset half-ranges
+    int64_t eu, ev;           // factor-table margins
+    int64_t n_cf;
+};
+
+template <typename ACC>
+__global__ void adg_taps_kernel(DegridApArgs a, AdgTab t, ACC *ckr) {
+    const int nwu = 2 * t.hu + 1, nwv = 2 * t.hv + 1;
+    const int R = t.R;
+    const int64_t n = t.n_cf * nwu * nwv * R * R;
+    const int64_t kplane = a.k_u * a.k_v;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int rv = (int)(i % R), ru = (int)((i / R) % R);
+        const int64_t o = i / ((int64_t)R * R);
+        const int64_t ov = o % nwv - t.hv;
+        const int64_t ou = (o / nwv) % nwu - t.hu;
+        const int64_t cf = o / ((int64_t)nwu * nwv);
+        const int64_t su = a.support[cf * 2], sv = a.support[cf * 2 + 1];
+        double v = 0.0;
+        if (ru < su && rv < sv) {
+            const int64_t x = a.os_u * (ru - su / 2) + ou + a.k_u / 2;
+            const int64_t y = a.os_v * (rv - sv / 2) + ov + a.k_v / 2;
+            if (x >= 0 && x < a.k_u && y >= 0 && y < a.k_v) v = a.ck[cf * kplane + x * a.k_v + y];
+        }
+        ckr[i] = (ACC)v;
+    }
+}
+
+// sets *bad when some PG[x][y] differs from PG[x][cv] PG[cu][y] by more than 1e-9
+__global__ void adg_pg_check_kernel(DegridApArgs a, int *bad) {
+#pragma clang fp contract(off)
+    const int64_t cu = a.k_u / 2, cv = a.k_v / 2;
+    const int64_t kplane = a.k_u * a.k_v;
+    const int64_t n = (int64_t)a.n_field * kplane;
+    int b = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t f = i / kplane, r = i - f * kplane;
+        const int64_t x = r / a.k_v, y = r - x * a.k_v;
+        const double2 gx = a.pg[f * kplane + x * a.k_v + cv];
+        const double2 gy = a.pg[f * kplane + cu * a.k_v + y];
+        const double2 g = a.pg[i];
+        const double pr = gx.x * gy.x - gx.y * gy.y, pi = gx.x * gy.y + gx.y * gy.x;
+        b |= !(fabs(pr - g.x) <= 1e-9 && fabs(pi - g.y) <= 1e-9);
+    }
+    if (b) atomicOr(bad, 1);
+}
+
+// pgu[f][x + eu] = PG[f][x][cv] for x in [0, K_u), 0 for the margins; pgv alike
+template <typename ACC>
+__global__ void adg_pg_factor_kernel(DegridApArgs a, AdgTab t, ACC *pgu, ACC *pgv) {
+    const int64_t cu = a.k_u / 2, cv = a.k_v / 2;
+    const int64_t kplane = a.k_u * a.k_v;
+    const int64_t lu = a.k_u + 2 * t.eu, lv = a.k_v + 2 * t.ev;
+    const int64_t nu = (int64_t)a.n_field * lu, n = nu + (int64_t)a.n_field * lv;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < nu) {
+            const int64_t f = i / lu, x = i - f * lu - t.eu;
+            double2 g = make_double2(0.0, 0.0);
+            if (x >= 0 && x < a.k_u) g = a.pg[f * kplane + x * a.k_v + cv];
+            pgu[2 * i] = (ACC)g.x;
+            pgu[2 * i + 1] = (ACC)g.y;
+        } else {
+            const int64_t j = i - nu;
+            const int64_t f = j / lv, y = j - f * lv - t.ev;
+            double2 g = make_double2(0.0, 0.0);
+            if (y >= 0 && y < a.k_v) g = a.pg[f * kplane + cu * a.k_v + y];
+            pgv[2 * j] = (ACC)g.x;
+            pgv[2 * j + 1] = (ACC)g.y;
+        }
+    }
+}
+
+// NP pols per pass sharing one convolution function (NP = 1 when the pols'
+// functions differ); G: the grid's element type.
+template <typename G, typename V, int NP>
+__global__ void __launch_bounds__(256) aperture_degrid_sep_kernel(DegridApArgs a, AdgTab t,
+                                                                  const double *__restrict__ ckr,
+                                                                  const double *__restrict__ pgu,
+                                                                  const double *__restrict__ pgv,
+                                                                  const G *__restrict__ grid,
+                                                                  V *vis) {
+    using ACC = double;
+    const int64_t n_tb = (a.n_time + 63) / 64;
+    const int64_t n_thr = a.n_baseline * a.n_chan * n_tb * 64;
+    const int64_t plane = a.n_u * a.n_v;
+    const int nwu = 2 * t.hu + 1, nwv = 2 * t.hv + 1;
+    const int64_t lu = a.k_u + 2 * t.eu, lv = a.k_v + 2 * t.ev;
+    const int RR = t.R * t.R;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_thr;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t w = i >> 6;
+        const int64_t tm = (w % n_tb) * 64 + (i & 63);
+        const int64_t r = w / n_tb;
+        const int64_t ch = r % a.n_chan, b = r / a.n_chan;
+        if (tm >= a.n_time) continue;
+        const int64_t row = tm * a.n_baseline + b;
+        V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol);
+        for (int64_t p = 0; p < a.n_pol; p++) {
+            out[2 * p] = (V)0;
+            out[2 * p + 1] = (V)0;
+        }
+        const int fval = a.field[row];
+        if (!(fval > -1)) continue;
+        int fidx = -1;
+        for (int f = 0; f < a.n_field; f++)
+            if (a.field_id[f] == fval) {
+                fidx = f;
+                break;
+            }
+        if (fidx < 0) {
+            atomicOr(a.error, 1);
+            continue;
+        }
+        Pix px, pxv;
+        if (!pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
+                            a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os_u, px))
+            continue;
+        pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
+                       a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os_v, pxv);
+        if (!((px.uc + a.max_sc < a.n_u) && (px.vc + a.max_sc < a.n_v) &&
+              (px.uc - a.max_sc >= 0) && (px.vc - a.max_sc >= 0)))
+            continue;
+        const int64_t du = px.du, dv = pxv.dv;
+        // |du| <= os_u/2 + 1 by construction of the offset; kept as a guard
+        if (du < -t.hu || du > t.hu || dv < -t.hv || dv > t.hv) {
+            atomicOr(a.error, 2);
+            continue;
+        }
+        const int64_t a_chan = a.chan_map[ch];
+        const int64_t cfbc = (a.cfb[b] * a.n_cfc + a.cfc[ch]) * a.n_cfp;
+        for (int64_t p0 = 0; p0 < a.n_pol; p0 += NP) {
+            const int64_t cf = cfbc + a.cfp[p0];
+            const int su = a.support[2 * cf], sv = a.support[2 * cf + 1];
+            const int scu = su / 2, scv = sv / 2;
+            const ACC *kb = ckr + ((cf * nwu + du + t.hu) * nwv + dv + t.hv) * RR;
+            const ACC *pu = pgu + 2 * (fidx * lu + t.eu + a.k_u / 2 + du - (int64_t)a.os_u * scu);
+            const ACC *pv = pgv + 2 * (fidx * lv + t.ev + a.k_v / 2 + dv - (int64_t)a.os_v * scv);
+            const int su2 = 2 * (int)a.os_u, sv2 = 2 * (int)a.os_v;
+            const G *gp[NP];
+#pragma unroll
+            for (int q = 0; q < NP; q++)
+                gp[q] = grid + 2 * ((a_chan * a.n_imag_pol + a.pol_map[p0 + q]) * plane +
+                                    (px.uc - scu) * a.n_v + (px.vc - scv));
+            double vr[NP], vi[NP], nr = 0.0, ni = 0.0;
+#pragma unroll
+            for (int q = 0; q < NP; q++) vr[q] = vi[q] = 0.0;
+            for (int ru = 0; ru < su; ru++) {
+                ACC sr[NP], si[NP], tr = 0, ti = 0;
+#pragma unroll
+                for (int q = 0; q < NP; q++) sr[q] = si[q] = 0;
+                const ACC *kr = kb + ru * t.R;
+                const int64_t go = 2 * (int64_t)ru * a.n_v;
+                for (int rv = 0; rv < sv; rv++) {
+                    const ACC k = kr[rv];
+                    const ACC cr = k * pv[rv * sv2], ci = k * pv[rv * sv2 + 1];
+                    tr += cr;
+                    ti += ci;
+#pragma unroll
+                    for (int q = 0; q < NP; q++) {
+                        const ACC gr = gp[q][go + 2 * rv], gi = gp[q][go + 2 * rv + 1];
+                        sr[q] += cr * gr + ci * gi;   // conj(c) g
+                        si[q] += cr * gi - ci * gr;
+                    }
+                }
+                const double wr = (double)pu[ru * su2], wi = (double)pu[ru * su2 + 1];
+#pragma unroll
+                for (int q = 0; q < NP; q++) {   // conj(w) S
+                    vr[q] += wr * (double)sr[q] + wi * (double)si[q];
+                    vi[q] += wr * (double)si[q] - wi * (double)sr[q];
+                }
+                nr += wr * (double)tr - wi * (double)ti;   // w T
+                ni += wr * (double)ti + wi * (double)tr;
+            }
+#pragma unroll
+            for (int q = 0; q < NP; q++) {
+                double re = vr[q], im = vi[q];
+                if (a.normalize) {
+                    const double d = nr * nr + ni * ni;   // divide by conj(norm)
+                    if (d != 0.0) {
+                        const double r2 = (re * nr - im * ni) / d;
+                        const double i2 = (im * nr + re * ni) / d;
+                        re = r2;
+                        im = i2;
+                    }
+                }
+                out[2 * (p0 + q)] = (V)re;
+                out[2 * (p0 + q) + 1] = (V)im;
+            }
+        }
+    }
+}
+
 }  // namespace ngc
 
 extern "C" int ngc_aperture_degrid(ngc_context *ctx, void *stream, const double *uvw,
@@ -555,28 +775,91 @@ extern "C" int ngc_aperture_degrid(ngc_context *ctx, void *stream, const double 
     a.max_sc = max_sc;
     a.normalize = normalize ? 1 : 0;
     a.error = ctx->d_error;
-    NGC_HIP(hipMemsetAsync(ctx->d_error, 0, sizeof(int), s));
     const int64_t n_thr = n_baseline * n_chan * ceil_div(n_time, 64) * 64;
     const int nb = pblocks(n_thr);
     profile_new_call(ctx, s);
-    phase_begin(ctx, s, "aperture_degrid");
-    if (grid_dtype == NGC_C64 && vis_dtype == NGC_C64)
-        aperture_degrid_kernel<float, float><<<nb, 256, 0, s>>>(a, (const float *)grid,
-                                                                (float *)vis);
-    else if (grid_dtype == NGC_C64)
-        aperture_degrid_kernel<float, double><<<nb, 256, 0, s>>>(a, (const float *)grid,
-                                                                 (double *)vis);
-    else if (vis_dtype == NGC_C64)
-        aperture_degrid_kernel<double, float><<<nb, 256, 0, s>>>(a, (const double *)grid,
-                                                                 (float *)vis);
-    else
-        aperture_degrid_kernel<double, double><<<nb, 256, 0, s>>>(a, (const double *)grid,
-                                                                  (double *)vis);
+
+    // the separable form needs PG = PGu PGv and re-laid taps of modest size
+    AdgTab t;
+    t.R = (int)max_sc;
+    t.hu = (int)(os_u / 2 + 1);
+    t.hv = (int)(os_v / 2 + 1);
+    t.eu = os_u * t.R + t.hu + 1;
+    t.ev = os_v * t.R + t.hv + 1;
+    t.n_cf = n_cfb * n_cfc * n_cfp;
+    const int64_t elem = 8;
+    const int64_t n_ckr = t.n_cf * (2 * t.hu + 1) * (2 * t.hv + 1) * t.R * t.R;
+    bool sep = n_ckr * elem <= (int64_t(1) << 30);
+    if (sep) {
+        NGC_HIP(hipMemsetAsync(ctx->d_error + 1, 0, sizeof(int), s));
+        adg_pg_check_kernel<<<pblocks(n_field * k_u * k_v), 256, 0, s>>>(a, ctx->d_error + 1);
+        NGC_HIP(hipGetLastError());
+        int bad = 0;
+        NGC_HIP(hipMemcpyAsync(&bad, ctx->d_error + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        NGC_HIP(hipStreamSynchronize(s));
+        sep = bad == 0;
+    }
+    NGC_HIP(hipMemsetAsync(ctx->d_error, 0, sizeof(int), s));
+    if (sep) {
+        const int64_t lu = k_u + 2 * t.eu, lv = k_v + 2 * t.ev;
+        void *d_ckr, *d_pgu, *d_pgv;
+        NGC_TRY(workspace(ctx, "adg_ckr", n_ckr * elem, &d_ckr));
+        NGC_TRY(workspace(ctx, "adg_pgu", n_field * lu * 2 * elem, &d_pgu));
+        NGC_TRY(workspace(ctx, "adg_pgv", n_field * lv * 2 * elem, &d_pgv));
+        bool shared = true;
+        for (int64_t p = 1; p < n_pol; p++) shared = shared && cf_pol_map[p] == cf_pol_map[0];
+        const int np = shared && (n_pol == 1 || n_pol == 2 || n_pol == 4) ? (int)n_pol : 1;
+        phase_begin(ctx, s, "aperture_degrid_tables");
+        adg_taps_kernel<double><<<pblocks(n_ckr), 256, 0, s>>>(a, t, (double *)d_ckr);
+        adg_pg_factor_kernel<double><<<pblocks(n_field * (lu + lv)), 256, 0, s>>>(
+            a, t, (double *)d_pgu, (double *)d_pgv);
+        NGC_HIP(hipGetLastError());
+        phase_end(ctx, s);
+        phase_begin(ctx, s, "aperture_degrid");
+        auto launch = [&](auto g, auto v, auto npc) {
+            using G = decltype(g);
+            using V = decltype(v);
+            aperture_degrid_sep_kernel<G, V, decltype(npc)::value><<<nb, 256, 0, s>>>(
+                a, t, (const double *)d_ckr, (const double *)d_pgu, (const double *)d_pgv,
+                (const G *)grid, (V *)vis);
+        };
+        auto by_np = [&](auto g, auto v) {
+            if (np == 4)
+                launch(g, v, std::integral_constant<int, 4>{});
+            else if (np == 2)
+                launch(g, v, std::integral_constant<int, 2>{});
+            else
+                launch(g, v, std::integral_constant<int, 1>{});
+        };
+        if (grid_dtype == NGC_C64 && vis_dtype == NGC_C64)
+            by_np(float{}, float{});
+        else if (grid_dtype == NGC_C64)
+            by_np(float{}, double{});
+        else if (vis_dtype == NGC_C64)
+            by_np(double{}, float{});
+        else
+            by_np(double{}, double{});
+    } else {
+        phase_begin(ctx, s, "aperture_degrid");
+        if (grid_dtype == NGC_C64 && vis_dtype == NGC_C64)
+            aperture_degrid_kernel<float, float><<<nb, 256, 0, s>>>(a, (const float *)grid,
+                                                                    (float *)vis);
+        else if (grid_dtype == NGC_C64)
+            aperture_degrid_kernel<float, double><<<nb, 256, 0, s>>>(a, (const float *)grid,
+                                                                     (double *)vis);
+        else if (vis_dtype == NGC_C64)
+            aperture_degrid_kernel<double, float><<<nb, 256, 0, s>>>(a, (const double *)grid,
+                                                                     (float *)vis);
+        else
+            aperture_degrid_kernel<double, double><<<nb, 256, 0, s>>>(a, (const double *)grid,
+                                                                      (double *)vis);
+    }
     NGC_HIP(hipGetLastError());
     phase_end(ctx, s);
     int err = 0;
     NGC_HIP(hipMemcpyAsync(&err, ctx->d_error, sizeof(int), hipMemcpyDeviceToHost, s));
     NGC_HIP(hipStreamSynchronize(s));
+    NGC_CHECK(!(err & 2), NGC_ERR_UNSUPPORTED, "sub-cell offset outside the re-laid taps");
     NGC_CHECK(err == 0, NGC_ERR_FIELD, "a row's field id is not in field_id");
     return NGC_OK;
 }

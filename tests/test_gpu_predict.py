@@ -97,28 +97,45 @@ def test_predict_modelvis_image_point_sources(gpu):
     assert np.allclose(out2["MODEL"].data.cpu().numpy(), 2 * v, rtol=1e-12, atol=1e-12)
 
 
-def _ap_case(seed):
-    """Two convolution functions (supports 5 and 7), three fields with random
-    unit phase gradients, random real CK, a row on field -1."""
+def _ap_case(seed, pg_form="random", cf_per_pol=False):
+    """Two convolution functions per pol function (supports 5 and 7, or 5/7
+    and 3/7 with a function per pol), three fields, random real CK, a row on
+    field -1.  pg_form "random": unit phase gradients of random phase (no
+    u/v factorisation, the general kernel); "separable": the gcf builder's
+    exp(i((x - K_u/2) px + (y - K_v/2) py)) per field (the separable kernel)."""
     rng = np.random.default_rng(seed)
     c = small_case(n_time=16, n_chan=3, n_pol=2)
     T, B = c["uvw"].shape[:2]
     os_, K = [4, 5], [40, 48]
-    ck = rng.uniform(0.1, 1.0, size=(2, 1, 1, K[0], K[1]))
-    sup = np.array([[5, 5], [7, 7]]).reshape(2, 1, 1, 2)
-    pg = np.exp(1j * rng.uniform(-np.pi, np.pi, size=(3, K[0], K[1])))
+    n_cfp = 2 if cf_per_pol else 1
+    ck = rng.uniform(0.1, 1.0, size=(2, 1, n_cfp, K[0], K[1]))
+    sup = np.array([[5, 5], [7, 7], [3, 7], [7, 5]])[:2 * n_cfp].reshape(2, n_cfp, 2)
+    sup = np.ascontiguousarray(np.swapaxes(sup.reshape(n_cfp, 2, 2), 0, 1)).reshape(2, 1, n_cfp, 2)
+    if pg_form == "random":
+        pg = np.exp(1j * rng.uniform(-np.pi, np.pi, size=(3, K[0], K[1])))
+    else:
+        x = np.arange(K[0])[:, None] - K[0] // 2
+        y = np.arange(K[1])[None, :] - K[1] // 2
+        slopes = rng.uniform(-0.3, 0.3, size=(3, 2))
+        pg = np.exp(1j * (x[None] * slopes[:, 0, None, None] + y[None] * slopes[:, 1, None, None]))
     field = rng.integers(0, 3, size=(T, B)) + 10
     field[2, 3] = -1
-    maps = (rng.integers(0, 2, size=B), np.zeros(3, int), np.zeros(2, int))
+    maps = (rng.integers(0, 2, size=B), np.zeros(3, int), np.arange(2) if cf_per_pol else np.zeros(2, int))
     return c, ck, sup, pg, field, [10, 11, 12], maps, os_
 
 
 @pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("normalize", [True, False])
-def test_aperture_degrid_matches_oracle(gpu, single, normalize):
+@pytest.mark.parametrize("form", ["random", "separable", "separable_cf_per_pol"])
+def test_aperture_degrid_matches_oracle(gpu, single, normalize, form):
+    """Both kernels against the oracle: the general one (a phase gradient
+    that does not factor) and the separable one, with pols sharing one
+    convolution function or each with its own; the profile phases show
+    which kernel ran."""
     import torch
     import cngi_prototype_amd._native as nat
-    c, ck, sup, pg, field, fid, maps, os_ = _ap_case(6)
+    c, ck, sup, pg, field, fid, maps, os_ = _ap_case(6, pg_form=form.split("_")[0],
+                                                     cf_per_pol=form.endswith("per_pol"))
     npad = c["npad"]
     rng = np.random.default_rng(9)
     G = rng.normal(size=(3, 2, npad, npad)) + 1j * rng.normal(size=(3, 2, npad, npad))
@@ -126,10 +143,16 @@ def test_aperture_degrid_matches_oracle(gpu, single, normalize):
         G = G.astype(np.complex64).astype(np.complex128)
     ref = oracle.aperture_degrid(G, c["uvw"], field, fid, *maps, ck, sup, pg, c["freq"],
                                  [npad, npad], c["delta"], os_, normalize=normalize)
-    got = nat.aperture_degrid(_t(c["uvw"]), _t(field, torch.int32), fid,
-                              _t(G, torch.complex64 if single else torch.complex128), c["freq"],
-                              np.arange(3), np.arange(2), *maps, _t(ck), sup, _t(pg), c["delta"],
-                              os_, normalize=normalize, single=single)
+    nat.profile_enable(True)
+    try:
+        got = nat.aperture_degrid(_t(c["uvw"]), _t(field, torch.int32), fid,
+                                  _t(G, torch.complex64 if single else torch.complex128),
+                                  c["freq"], np.arange(3), np.arange(2), *maps, _t(ck), sup,
+                                  _t(pg), c["delta"], os_, normalize=normalize, single=single)
+        phases = [name for name, _ in nat.profile_read()]
+    finally:
+        nat.profile_enable(False)
+    assert ("aperture_degrid_tables" in phases) == (form != "random"), phases
     got = got.cpu().numpy()
     assert np.count_nonzero(ref) > 0
     assert np.array_equal(got == 0, ref == 0)
