@@ -459,56 +459,70 @@ __global__ void __launch_bounds__(256) aperture_degrid_kernel(DegridApArgs a, co
 }
 
 // ---------------------------------------------------------------------------
-// Separable form of the same sum, for phase gradients of the form
-// exp(i(x px + y py)) (the gcf builder's), PG[x][y] = PGu[x] PGv[y] with
-// PGu[x] = PG[x][cv], PGv[y] = PG[cu][y] (so PG[cu][cv] = 1).  Then
-//   conj(CK PG) G = conj(PGu) conj(CK PGv) G,
-// so per u row the inner sum S_u = sum_v conj(CK PGv) G and the outer sum
-// sum_u conj(PGu) S_u take 4 + 4 instead of 2 + 8 multiply-adds per tap and
-// pol.  All sums stay in FP64 with FP64 tables, whatever the grid's
-// precision: towards an offset pointing CK PG turns many times across the
-// support, the sums cancel to a small fraction of sum |CK PG|, and FP32
-// sums lose that fraction's digits (1.5e-4 relative error on a 25-pointing
-// mosaic).  FP64 multiply-adds issue at the rate of unpacked FP32 ones on
-// gfx950, so this costs only the tables' width.  Tables (rebuilt per call):
-//  - adg_taps: CK re-laid per (convolution function, sub-cell offset pair) as
-//    an R x R block, R the largest support, ckr[cf][ou][ov][ru][rv] =
+// Tap-parallel form of the same sum, for phase gradients of the form
+// exp(i(x px + y py)) (the gcf builder's): PG[x][y] = PGu[x] PGv[y] with
+// PGu[x] = PG[x][cv], PGv[y] = PG[cu][y] (so PG[cu][cv] = 1), so a tap's
+// weight is CK * PGu * PGv from two short factor tables instead of a
+// [field][K_u][K_v] table.
+//
+// A wavefront takes 64 integrations of one (baseline, channel), so the
+// convolution function and its support are wave-uniform.  Each lane first
+// places one integration (cell, sub-cell offset, field); then, in 16 rounds,
+// 16-lane groups each take one integration and run over its su * sv taps in
+// order, lane l taking taps l, l + 16, ...  The taps of one integration are
+// one contiguous block of the re-laid kernel, so a group's kernel reads are
+// 128 contiguous bytes and its grid reads follow rows of the footprint.  (One
+// lane per integration had every lane read its own block: 64 cache lines per
+// load, and the kernel ran at the L2's bandwidth.)  The group's partial sums
+// are added across its 16 lanes.  Pols that share a convolution function
+// (cf_pol_map constant) share the tap weights.
+//
+// All sums and tables are FP64, whatever the grid's precision: towards an
+// offset pointing CK PG turns many times across the support, the sums cancel
+// to a small fraction of sum |CK PG|, and FP32 sums lose that fraction's
+// digits (1.5e-4 relative error on a 25-pointing mosaic).  FP64 multiply-adds
+// issue at the rate of unpacked FP32 ones on gfx950.
+//
+// Tables (rebuilt per call):
+//  - adg_taps: CK re-laid per (convolution function, sub-cell offset pair)
+//    as a block of su * sv taps, row-major in (ru, rv):
+//    ckr[cf_base[cf] + ((ou + hu) nwv + ov + hv) su sv + ru sv + rv] =
 //    CK[os_u (ru - su/2) + ou + K_u/2][os_v (rv - sv/2) + ov + K_v/2], zero
 //    where that index leaves the K_u x K_v table (a zero weight, as in the
 //    general form);
-//  - adg_pgu / adg_pgv: the factors over an index range widened by E_u / E_v
-//    on both sides (zero outside the table) so no tap index needs a clamp.
-// A wavefront is 64 integrations of one (baseline, channel), so the
-// convolution function and the loop bounds are wave-uniform; pols that share
-// a convolution function (cf_pol_map constant) share the tap weights.
+//  - adg_pgu / adg_pgv: the factors re-laid the same way, per (field,
+//    sub-cell offset) as W = R + 1 entries (R the largest support),
+//    pur[(f nwu + ou + hu) W + iu + R/2] = PGu[os_u iu + ou + K_u/2] for the
+//    tap offset iu = ru - su/2, zero where the index leaves the table, so a
+//    group's factor reads are contiguous as well.
 // ---------------------------------------------------------------------------
 struct AdgTab {
-    int R, hu, hv;            // largest support; offset half-ranges
-    int64_t eu, ev;           // factor-table margins
+    int hu, hv;               // sub-cell offset half-ranges
+    int R, W;                 // largest support; factor entries per offset (R + 1)
     int64_t n_cf;
+    int max_block;            // largest su * sv
+    const int64_t *cf_base;   // [n_cf] first element of each function's blocks
 };
 
-template <typename ACC>
-__global__ void adg_taps_kernel(DegridApArgs a, AdgTab t, ACC *ckr) {
+__global__ void adg_taps_kernel(DegridApArgs a, AdgTab t, double *ckr) {
     const int nwu = 2 * t.hu + 1, nwv = 2 * t.hv + 1;
-    const int R = t.R;
-    const int64_t n = t.n_cf * nwu * nwv * R * R;
+    const int64_t nwo = (int64_t)nwu * nwv;
+    const int64_t n = t.n_cf * nwo * t.max_block;
     const int64_t kplane = a.k_u * a.k_v;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int rv = (int)(i % R), ru = (int)((i / R) % R);
-        const int64_t o = i / ((int64_t)R * R);
-        const int64_t ov = o % nwv - t.hv;
-        const int64_t ou = (o / nwv) % nwu - t.hu;
-        const int64_t cf = o / ((int64_t)nwu * nwv);
-        const int64_t su = a.support[cf * 2], sv = a.support[cf * 2 + 1];
+        const int j = (int)(i % t.max_block);
+        const int64_t o = (i / t.max_block) % nwo;
+        const int64_t cf = i / (nwo * t.max_block);
+        const int su = a.support[cf * 2], sv = a.support[cf * 2 + 1];
+        if (j >= su * sv) continue;
+        const int ru = j / sv, rv = j - ru * sv;
+        const int64_t ov = o % nwv - t.hv, ou = o / nwv - t.hu;
+        const int64_t x = a.os_u * (ru - su / 2) + ou + a.k_u / 2;
+        const int64_t y = a.os_v * (rv - sv / 2) + ov + a.k_v / 2;
         double v = 0.0;
-        if (ru < su && rv < sv) {
-            const int64_t x = a.os_u * (ru - su / 2) + ou + a.k_u / 2;
-            const int64_t y = a.os_v * (rv - sv / 2) + ov + a.k_v / 2;
-            if (x >= 0 && x < a.k_u && y >= 0 && y < a.k_v) v = a.ck[cf * kplane + x * a.k_v + y];
-        }
-        ckr[i] = (ACC)v;
+        if (x >= 0 && x < a.k_u && y >= 0 && y < a.k_v) v = a.ck[cf * kplane + x * a.k_v + y];
+        ckr[t.cf_base[cf] + o * (int64_t)(su * sv) + j] = v;
     }
 }
 
@@ -532,87 +546,108 @@ __global__ void adg_pg_check_kernel(DegridApArgs a, int *bad) {
     if (b) atomicOr(bad, 1);
 }
 
-// pgu[f][x + eu] = PG[f][x][cv] for x in [0, K_u), 0 for the margins; pgv alike
-template <typename ACC>
-__global__ void adg_pg_factor_kernel(DegridApArgs a, AdgTab t, ACC *pgu, ACC *pgv) {
+// pur[f][ou][iu] = PG[f][os_u (iu - R/2) + ou + K_u/2][cv], 0 off the table;
+// pvr alike with PG[f][cu][.]
+__global__ void adg_pg_factor_kernel(DegridApArgs a, AdgTab t, double *pur, double *pvr) {
     const int64_t cu = a.k_u / 2, cv = a.k_v / 2;
     const int64_t kplane = a.k_u * a.k_v;
-    const int64_t lu = a.k_u + 2 * t.eu, lv = a.k_v + 2 * t.ev;
-    const int64_t nu = (int64_t)a.n_field * lu, n = nu + (int64_t)a.n_field * lv;
+    const int64_t nwu = 2 * t.hu + 1, nwv = 2 * t.hv + 1;
+    const int64_t nu = (int64_t)a.n_field * nwu * t.W, n = nu + (int64_t)a.n_field * nwv * t.W;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        if (i < nu) {
-            const int64_t f = i / lu, x = i - f * lu - t.eu;
-            double2 g = make_double2(0.0, 0.0);
-            if (x >= 0 && x < a.k_u) g = a.pg[f * kplane + x * a.k_v + cv];
-            pgu[2 * i] = (ACC)g.x;
-            pgu[2 * i + 1] = (ACC)g.y;
-        } else {
-            const int64_t j = i - nu;
-            const int64_t f = j / lv, y = j - f * lv - t.ev;
-            double2 g = make_double2(0.0, 0.0);
-            if (y >= 0 && y < a.k_v) g = a.pg[f * kplane + cu * a.k_v + y];
-            pgv[2 * j] = (ACC)g.x;
-            pgv[2 * j + 1] = (ACC)g.y;
-        }
+        const bool is_u = i < nu;
+        const int64_t j = is_u ? i : i - nu;
+        const int64_t nw = is_u ? nwu : nwv;
+        const int64_t e = j % t.W - t.R / 2;
+        const int64_t o = (j / t.W) % nw - (is_u ? t.hu : t.hv);
+        const int64_t f = j / (t.W * nw);
+        const int64_t x = (is_u ? a.os_u : a.os_v) * e + o + (is_u ? a.k_u : a.k_v) / 2;
+        double2 g = make_double2(0.0, 0.0);
+        if (is_u && x >= 0 && x < a.k_u) g = a.pg[f * kplane + x * a.k_v + cv];
+        if (!is_u && x >= 0 && x < a.k_v) g = a.pg[f * kplane + cu * a.k_v + x];
+        double *dst = is_u ? pur : pvr;
+        dst[2 * j] = g.x;
+        dst[2 * j + 1] = g.y;
     }
+}
+
+__device__ inline double group16_sum(double v) {
+    v += __shfl_xor(v, 8);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 1);
+    return v;
 }
 
 // NP pols per pass sharing one convolution function (NP = 1 when the pols'
 // functions differ); G: the grid's element type.
 template <typename G, typename V, int NP>
-__global__ void __launch_bounds__(256) aperture_degrid_sep_kernel(DegridApArgs a, AdgTab t,
-                                                                  const double *__restrict__ ckr,
-                                                                  const double *__restrict__ pgu,
-                                                                  const double *__restrict__ pgv,
-                                                                  const G *__restrict__ grid,
-                                                                  V *vis) {
-    using ACC = double;
+__global__ void __launch_bounds__(256) aperture_degrid_taps_kernel(DegridApArgs a, AdgTab t,
+                                                                   const double *__restrict__ ckr,
+                                                                   const double *__restrict__ pgu,
+                                                                   const double *__restrict__ pgv,
+                                                                   const G *__restrict__ grid,
+                                                                   V *vis) {
     const int64_t n_tb = (a.n_time + 63) / 64;
-    const int64_t n_thr = a.n_baseline * a.n_chan * n_tb * 64;
+    const int64_t n_w = a.n_baseline * a.n_chan * n_tb;
+    const int lane = (int)(threadIdx.x & 63);
+    const int grp = lane >> 4, l = lane & 15;
     const int64_t plane = a.n_u * a.n_v;
     const int nwu = 2 * t.hu + 1, nwv = 2 * t.hv + 1;
-    const int64_t lu = a.k_u + 2 * t.eu, lv = a.k_v + 2 * t.ev;
-    const int RR = t.R * t.R;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_thr;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t w = i >> 6;
-        const int64_t tm = (w % n_tb) * 64 + (i & 63);
+    const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t ws = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    // this wave's tap offsets for the support it last used: {2 ru, 2 rv,
+    // 2 (ru n_v + rv)} per tap, so the tap loop reads them instead of
+    // dividing its tap index by sv
+    extern __shared__ int4 s_taps[];
+    int4 *taps = s_taps + (threadIdx.x >> 6) * t.max_block;
+    int tab_su = 0, tab_sv = 0;
+    for (int64_t w = w0; w < n_w; w += ws) {   // wave-uniform
+        const int64_t t0 = (w % n_tb) * 64;
         const int64_t r = w / n_tb;
         const int64_t ch = r % a.n_chan, b = r / a.n_chan;
-        if (tm >= a.n_time) continue;
-        const int64_t row = tm * a.n_baseline + b;
-        V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol);
-        for (int64_t p = 0; p < a.n_pol; p++) {
-            out[2 * p] = (V)0;
-            out[2 * p + 1] = (V)0;
-        }
-        const int fval = a.field[row];
-        if (!(fval > -1)) continue;
-        int fidx = -1;
-        for (int f = 0; f < a.n_field; f++)
-            if (a.field_id[f] == fval) {
-                fidx = f;
-                break;
+        // placement: lane = integration t0 + lane; fidx < 0 marks nothing to predict
+        int uc = 0, vc = 0, du = 0, dv = 0, fidx = -1;
+        const int64_t tm = t0 + lane;
+        if (tm < a.n_time) {
+            const int64_t row = tm * a.n_baseline + b;
+            const int fval = a.field[row];
+            int f_i = -1;
+            if (fval > -1) {
+                for (int f = 0; f < a.n_field; f++)
+                    if (a.field_id[f] == fval) {
+                        f_i = f;
+                        break;
+                    }
+                if (f_i < 0) atomicOr(a.error, 1);
             }
-        if (fidx < 0) {
-            atomicOr(a.error, 1);
-            continue;
-        }
-        Pix px, pxv;
-        if (!pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
-                            a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os_u, px))
-            continue;
-        pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
-                       a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os_v, pxv);
-        if (!((px.uc + a.max_sc < a.n_u) && (px.vc + a.max_sc < a.n_v) &&
-              (px.uc - a.max_sc >= 0) && (px.vc - a.max_sc >= 0)))
-            continue;
-        const int64_t du = px.du, dv = pxv.dv;
-        // |du| <= os_u/2 + 1 by construction of the offset; kept as a guard
-        if (du < -t.hu || du > t.hu || dv < -t.hv || dv > t.hv) {
-            atomicOr(a.error, 2);
-            continue;
+            Pix px, pxv;
+            if (f_i >= 0 &&
+                pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
+                               a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os_u, px)) {
+                pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
+                               a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os_v, pxv);
+                if ((px.uc + a.max_sc < a.n_u) && (px.vc + a.max_sc < a.n_v) &&
+                    (px.uc - a.max_sc >= 0) && (px.vc - a.max_sc >= 0)) {
+                    // |du| <= os_u/2 + 1 by construction of the offset; kept as a guard
+                    if (px.du < -t.hu || px.du > t.hu || pxv.dv < -t.hv || pxv.dv > t.hv) {
+                        atomicOr(a.error, 2);
+                    } else {
+                        uc = (int)px.uc;
+                        vc = (int)px.vc;
+                        du = (int)px.du;
+                        dv = (int)pxv.dv;
+                        fidx = f_i;
+                    }
+                }
+            }
+            if (fidx < 0) {
+                V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol);
+                for (int64_t p = 0; p < a.n_pol; p++) {
+                    out[2 * p] = (V)0;
+                    out[2 * p + 1] = (V)0;
+                }
+            }
         }
         const int64_t a_chan = a.chan_map[ch];
         const int64_t cfbc = (a.cfb[b] * a.n_cfc + a.cfc[ch]) * a.n_cfp;
@@ -620,59 +655,90 @@ __global__ void __launch_bounds__(256) aperture_degrid_sep_kernel(DegridApArgs a
             const int64_t cf = cfbc + a.cfp[p0];
             const int su = a.support[2 * cf], sv = a.support[2 * cf + 1];
             const int scu = su / 2, scv = sv / 2;
-            const ACC *kb = ckr + ((cf * nwu + du + t.hu) * nwv + dv + t.hv) * RR;
-            const ACC *pu = pgu + 2 * (fidx * lu + t.eu + a.k_u / 2 + du - (int64_t)a.os_u * scu);
-            const ACC *pv = pgv + 2 * (fidx * lv + t.ev + a.k_v / 2 + dv - (int64_t)a.os_v * scv);
-            const int su2 = 2 * (int)a.os_u, sv2 = 2 * (int)a.os_v;
-            const G *gp[NP];
+            const int n_tap = su * sv;
+            if (su != tab_su || sv != tab_sv) {   // wave-uniform
+                __builtin_amdgcn_wave_barrier();
+                for (int j = lane; j < n_tap; j += 64) {
+                    const int ru = j / sv, rv = j - ru * sv;
+                    taps[j] = make_int4(2 * ru, 2 * rv, 2 * (ru * (int)a.n_v + rv), 0);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                tab_su = su;
+                tab_sv = sv;
+            }
+            const double *kcf = ckr + t.cf_base[cf];
+            const G *gpl[NP];
 #pragma unroll
             for (int q = 0; q < NP; q++)
-                gp[q] = grid + 2 * ((a_chan * a.n_imag_pol + a.pol_map[p0 + q]) * plane +
-                                    (px.uc - scu) * a.n_v + (px.vc - scv));
-            double vr[NP], vi[NP], nr = 0.0, ni = 0.0;
+                gpl[q] = grid + 2 * ((a_chan * a.n_imag_pol + a.pol_map[p0 + q]) * plane);
+            for (int rd = 0; rd < 16; rd++) {
+                const int src = 4 * rd + grp;
+                const int s_f = __shfl(fidx, src);
+                const int s_uc = __shfl(uc, src), s_vc = __shfl(vc, src);
+                const int s_du = __shfl(du, src), s_dv = __shfl(dv, src);
+                if (s_f < 0) continue;   // the whole group skips
+                const double *kb = kcf + ((int64_t)(s_du + t.hu) * nwv + s_dv + t.hv) * n_tap;
+                const double *pu =
+                    pgu + 2 * (((int64_t)s_f * nwu + s_du + t.hu) * t.W + t.R / 2 - scu);
+                const double *pv =
+                    pgv + 2 * (((int64_t)s_f * nwv + s_dv + t.hv) * t.W + t.R / 2 - scv);
+                const int64_t g0 = (int64_t)(s_uc - scu) * a.n_v + (s_vc - scv);
+                const G *gs[NP];
 #pragma unroll
-            for (int q = 0; q < NP; q++) vr[q] = vi[q] = 0.0;
-            for (int ru = 0; ru < su; ru++) {
-                ACC sr[NP], si[NP], tr = 0, ti = 0;
+                for (int q = 0; q < NP; q++) gs[q] = gpl[q] + 2 * g0;
+                double sr[NP], si[NP], nr = 0.0, ni = 0.0;
 #pragma unroll
-                for (int q = 0; q < NP; q++) sr[q] = si[q] = 0;
-                const ACC *kr = kb + ru * t.R;
-                const int64_t go = 2 * (int64_t)ru * a.n_v;
-                for (int rv = 0; rv < sv; rv++) {
-                    const ACC k = kr[rv];
-                    const ACC cr = k * pv[rv * sv2], ci = k * pv[rv * sv2 + 1];
-                    tr += cr;
-                    ti += ci;
+                for (int q = 0; q < NP; q++) sr[q] = si[q] = 0.0;
+                auto tap = [&](int j) {
+                    const int4 o = taps[j];
+                    const double k = kb[j];
+                    const double ur = pu[o.x], ui = pu[o.x + 1];
+                    const double vr = pv[o.y], vi = pv[o.y + 1];
+                    const double cr = k * fma(ur, vr, -ui * vi);
+                    const double ci = k * fma(ur, vi, ui * vr);
+                    nr += cr;
+                    ni += ci;
+#pragma unroll
+                    for (int q = 0; q < NP; q++) {   // conj(c) g
+                        const double gr = (double)gs[q][o.z], gi = (double)gs[q][o.z + 1];
+                        sr[q] = fma(cr, gr, sr[q]);
+                        sr[q] = fma(ci, gi, sr[q]);
+                        si[q] = fma(cr, gi, si[q]);
+                        si[q] = fma(-ci, gr, si[q]);
+                    }
+                };
+                // full passes of 16 taps (a wave-uniform count), then the rest
+                const int n_full = n_tap >> 4;
+#pragma unroll 2
+                for (int it = 0; it < n_full; it++) tap(l + 16 * it);
+                if (l + 16 * n_full < n_tap) tap(l + 16 * n_full);
+                nr = group16_sum(nr);
+                ni = group16_sum(ni);
+#pragma unroll
+                for (int q = 0; q < NP; q++) {
+                    sr[q] = group16_sum(sr[q]);
+                    si[q] = group16_sum(si[q]);
+                }
+                if (l == 0) {
+                    const int64_t row = (t0 + src) * a.n_baseline + b;
+                    V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol + p0);
 #pragma unroll
                     for (int q = 0; q < NP; q++) {
-                        const ACC gr = gp[q][go + 2 * rv], gi = gp[q][go + 2 * rv + 1];
-                        sr[q] += cr * gr + ci * gi;   // conj(c) g
-                        si[q] += cr * gi - ci * gr;
+                        double re = sr[q], im = si[q];
+                        if (a.normalize) {
+                            const double d = nr * nr + ni * ni;   // divide by conj(norm)
+                            if (d != 0.0) {
+                                const double r2 = (re * nr - im * ni) / d;
+                                const double i2 = (im * nr + re * ni) / d;
+                                re = r2;
+                                im = i2;
+                            }
+                        }
+                        out[2 * q] = (V)re;
+                        out[2 * q + 1] = (V)im;
                     }
                 }
-                const double wr = (double)pu[ru * su2], wi = (double)pu[ru * su2 + 1];
-#pragma unroll
-                for (int q = 0; q < NP; q++) {   // conj(w) S
-                    vr[q] += wr * (double)sr[q] + wi * (double)si[q];
-                    vi[q] += wr * (double)si[q] - wi * (double)sr[q];
-                }
-                nr += wr * (double)tr - wi * (double)ti;   // w T
-                ni += wr * (double)ti + wi * (double)tr;
-            }
-#pragma unroll
-            for (int q = 0; q < NP; q++) {
-                double re = vr[q], im = vi[q];
-                if (a.normalize) {
-                    const double d = nr * nr + ni * ni;   // divide by conj(norm)
-                    if (d != 0.0) {
-                        const double r2 = (re * nr - im * ni) / d;
-                        const double i2 = (im * nr + re * ni) / d;
-                        re = r2;
-                        im = i2;
-                    }
-                }
-                out[2 * (p0 + q)] = (V)re;
-                out[2 * (p0 + q) + 1] = (V)im;
             }
         }
     }
@@ -781,15 +847,26 @@ extern "C" int ngc_aperture_degrid(ngc_context *ctx, void *stream, const double 
 
     // the separable form needs PG = PGu PGv and re-laid taps of modest size
     AdgTab t;
-    t.R = (int)max_sc;
     t.hu = (int)(os_u / 2 + 1);
     t.hv = (int)(os_v / 2 + 1);
-    t.eu = os_u * t.R + t.hu + 1;
-    t.ev = os_v * t.R + t.hv + 1;
+    t.R = (int)max_sc;
+    t.W = t.R + 1;
     t.n_cf = n_cfb * n_cfc * n_cfp;
     const int64_t elem = 8;
-    const int64_t n_ckr = t.n_cf * (2 * t.hu + 1) * (2 * t.hv + 1) * t.R * t.R;
-    bool sep = n_ckr * elem <= (int64_t(1) << 30);
+    const int64_t nwo = (int64_t)(2 * t.hu + 1) * (2 * t.hv + 1);
+    std::vector<int64_t> cf_base(t.n_cf);
+    int64_t n_ckr = 0;
+    t.max_block = 1;
+    for (int64_t cf = 0; cf < t.n_cf; cf++) {
+        const int64_t blk = (int64_t)support[2 * cf] * support[2 * cf + 1];
+        cf_base[cf] = n_ckr;
+        n_ckr += nwo * blk;
+        t.max_block = std::max<int>(t.max_block, (int)blk);
+    }
+    // the tap-offset tables take max_block int4s per wavefront in LDS, and the
+    // offsets are 32-bit
+    bool sep = n_ckr * elem <= (int64_t(1) << 30) && t.max_block <= 1024 &&
+               2 * (max_sc * n_v + max_sc) < (int64_t(1) << 31);
     if (sep) {
         NGC_HIP(hipMemsetAsync(ctx->d_error + 1, 0, sizeof(int), s));
         adg_pg_check_kernel<<<pblocks(n_field * k_u * k_v), 256, 0, s>>>(a, ctx->d_error + 1);
@@ -801,25 +878,31 @@ extern "C" int ngc_aperture_degrid(ngc_context *ctx, void *stream, const double 
     }
     NGC_HIP(hipMemsetAsync(ctx->d_error, 0, sizeof(int), s));
     if (sep) {
-        const int64_t lu = k_u + 2 * t.eu, lv = k_v + 2 * t.ev;
+        const int64_t lu = (2 * t.hu + 1) * t.W, lv = (2 * t.hv + 1) * t.W;
         void *d_ckr, *d_pgu, *d_pgv;
         NGC_TRY(workspace(ctx, "adg_ckr", n_ckr * elem, &d_ckr));
         NGC_TRY(workspace(ctx, "adg_pgu", n_field * lu * 2 * elem, &d_pgu));
         NGC_TRY(workspace(ctx, "adg_pgv", n_field * lv * 2 * elem, &d_pgv));
+        void *d_cfbase;
+        NGC_TRY(upload(ctx, s, "adg_cf_base", cf_base.data(), t.n_cf * sizeof(int64_t),
+                       &d_cfbase));
+        t.cf_base = (const int64_t *)d_cfbase;
         bool shared = true;
         for (int64_t p = 1; p < n_pol; p++) shared = shared && cf_pol_map[p] == cf_pol_map[0];
         const int np = shared && (n_pol == 1 || n_pol == 2 || n_pol == 4) ? (int)n_pol : 1;
         phase_begin(ctx, s, "aperture_degrid_tables");
-        adg_taps_kernel<double><<<pblocks(n_ckr), 256, 0, s>>>(a, t, (double *)d_ckr);
-        adg_pg_factor_kernel<double><<<pblocks(n_field * (lu + lv)), 256, 0, s>>>(
-            a, t, (double *)d_pgu, (double *)d_pgv);
+        adg_taps_kernel<<<pblocks(t.n_cf * nwo * t.max_block), 256, 0, s>>>(a, t,
+                                                                           (double *)d_ckr);
+        adg_pg_factor_kernel<<<pblocks(n_field * (lu + lv)), 256, 0, s>>>(a, t, (double *)d_pgu,
+                                                                           (double *)d_pgv);
         NGC_HIP(hipGetLastError());
         phase_end(ctx, s);
         phase_begin(ctx, s, "aperture_degrid");
         auto launch = [&](auto g, auto v, auto npc) {
             using G = decltype(g);
             using V = decltype(v);
-            aperture_degrid_sep_kernel<G, V, decltype(npc)::value><<<nb, 256, 0, s>>>(
+            aperture_degrid_taps_kernel<G, V, decltype(npc)::value>
+                <<<nb, 256, 4 * t.max_block * sizeof(int4), s>>>(
                 a, t, (const double *)d_ckr, (const double *)d_pgu, (const double *)d_pgv,
                 (const G *)grid, (V *)vis);
         };
