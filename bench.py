@@ -192,19 +192,25 @@ def mosaic_leg(args):
 
     vis_p = degrid(grid, True)
     torch.cuda.synchronize()
-    t0 = time.time()
     n_dg = max(1, min(args.steps, 3))
+    nat.profile_enable(True)
+    dg_phases = {}
+    t0 = time.time()
     for _ in range(n_dg):
         degrid(grid, True, vis_p)
+        for name, ms in nat.profile_read():
+            dg_phases[name] = dg_phases.get(name, 0.0) + ms / n_dg
     torch.cuda.synchronize()
     ms_degrid = (time.time() - t0) / n_dg * 1e3
+    nat.profile_enable(False)
     # error of the complex64 path against FP64 throughout, on the FP64 grid
     vis64 = degrid(g64, False)
     vis32 = degrid(g64.to(torch.complex64), True)
     dg_err = float(torch.linalg.norm((vis32.to(torch.complex128) - vis64).ravel())
                    / torch.linalg.norm(vis64.ravel()))
     del vis64, vis32, vis_p, g64
-    log(f"mosaic aperture degrid: {ms_degrid:.2f} ms, rel error vs fp64 {dg_err:.2e}")
+    log(f"mosaic aperture degrid: {ms_degrid:.2f} ms, phases {dg_phases}, "
+        f"rel error vs fp64 {dg_err:.2e}")
     img32 = nat.grid_to_image(grid, [N, N], sum_weight=sw)
     n_planes = C * P
     dd = (img32 - img64).reshape(n_planes, -1)
@@ -216,6 +222,10 @@ def mosaic_leg(args):
     taps = np.asarray([sup[cfb[b], 0, 0, 0] ** 2 for b in range(cfg.n_baseline)], dtype=np.float64)
     flops = float(taps.sum()) * cfg.n_time * C * (2 + 8 * P)
     t_k = phases.get("ap_grid_tiled", float("nan")) / 1e3
+    # A-term prediction: per tap PGu*PGv (6), times CK (2), the norm (2) and
+    # conj(c) g accumulated per pol (8), in FP64
+    t_dg = dg_phases.get("aperture_degrid", float("nan")) / 1e3
+    dg_tflops = round(float(taps.sum()) * cfg.n_time * C * (10 + 8 * P) / t_dg / 1e12, 2)
     # learned A-term surrogate: per-channel kernels for the 32 channels that the
     # reference covers with one kernel per dish pair (one frequency bin)
     surrogate = None
@@ -272,6 +282,8 @@ def mosaic_leg(args):
                 fp32_peak_tflops=157.3, phases_ms=phases, gcf_s=round(t_gcf, 2),
                 aperture_degrid=dict(ms=round(ms_degrid, 3),
                                      mvis_s=round(n_vis / (ms_degrid / 1e3) / 1e6, 1),
+                                     phases_ms=dg_phases, kernel_fp64_tflops=dg_tflops,
+                                     fp64_peak_tflops=78.6,
                                      rel_err_complex64_vs_fp64=dg_err),
                 surrogate=surrogate)
 
@@ -350,29 +362,7 @@ def batched_leg(args):
     g64, sw64 = nat.standard_grid(uvw, w, fr, np.arange(C), np.arange(P), cgk, C, [npad, npad],
                                   delta, 7, 100, nat.GRID_IMAGE, "fp64", vis=vis, flag=flag)
     img64 = nat.grid_to_image(g64, [N, N], sum_weight=sw64)
-    # A-term prediction: the gridded cube degridded back to every sample with
-    # the same kernels, fields and maps (complex64 grid -> complex64 samples)
-    def degrid(g, single, out=None):
-        return nat.aperture_degrid(d["uvw"], d["field"], fid, g, d["freq"], np.arange(C),
-                                   np.arange(P), cfb, np.asarray(gcf["CF_CHAN_MAP"].values),
-                                   np.asarray(gcf["CF_POL_MAP"].values), ck, sup, pg, delta,
-                                   [20, 20], single=single, out=out)
-
-    vis_p = degrid(grid, True)
-    torch.cuda.synchronize()
-    t0 = time.time()
-    n_dg = max(1, min(args.steps, 3))
-    for _ in range(n_dg):
-        degrid(grid, True, vis_p)
-    torch.cuda.synchronize()
-    ms_degrid = (time.time() - t0) / n_dg * 1e3
-    # error of the complex64 path against FP64 throughout, on the FP64 grid
-    vis64 = degrid(g64, False)
-    vis32 = degrid(g64.to(torch.complex64), True)
-    dg_err = float(torch.linalg.norm((vis32.to(torch.complex128) - vis64).ravel())
-                   / torch.linalg.norm(vis64.ravel()))
-    del vis64, vis32, vis_p, g64
-    log(f"mosaic aperture degrid: {ms_degrid:.2f} ms, rel error vs fp64 {dg_err:.2e}")
+    del g64
     img = nat.grid_to_image(grid, [N, N], sum_weight=sw)
     n_planes = C * P
     dd = (img - img64).reshape(n_planes, -1)
@@ -498,29 +488,7 @@ def streamed_leg(args):
     t64 = time.time() - t0
     del dv
     img64 = nat.grid_to_image(g64, [N, N], sum_weight=sw64)
-    # A-term prediction: the gridded cube degridded back to every sample with
-    # the same kernels, fields and maps (complex64 grid -> complex64 samples)
-    def degrid(g, single, out=None):
-        return nat.aperture_degrid(d["uvw"], d["field"], fid, g, d["freq"], np.arange(C),
-                                   np.arange(P), cfb, np.asarray(gcf["CF_CHAN_MAP"].values),
-                                   np.asarray(gcf["CF_POL_MAP"].values), ck, sup, pg, delta,
-                                   [20, 20], single=single, out=out)
-
-    vis_p = degrid(grid, True)
-    torch.cuda.synchronize()
-    t0 = time.time()
-    n_dg = max(1, min(args.steps, 3))
-    for _ in range(n_dg):
-        degrid(grid, True, vis_p)
-    torch.cuda.synchronize()
-    ms_degrid = (time.time() - t0) / n_dg * 1e3
-    # error of the complex64 path against FP64 throughout, on the FP64 grid
-    vis64 = degrid(g64, False)
-    vis32 = degrid(g64.to(torch.complex64), True)
-    dg_err = float(torch.linalg.norm((vis32.to(torch.complex128) - vis64).ravel())
-                   / torch.linalg.norm(vis64.ravel()))
-    del vis64, vis32, vis_p, g64
-    log(f"mosaic aperture degrid: {ms_degrid:.2f} ms, rel error vs fp64 {dg_err:.2e}")
+    del g64
     img = nat.grid_to_image(grid, [N, N], sum_weight=sw)
     n_planes = C * P
     dd = (img - img64).reshape(n_planes, -1)
