@@ -599,8 +599,12 @@ __global__ void __launch_bounds__(256) aperture_degrid_taps_kernel(DegridApArgs 
     // this wave's tap offsets for the support it last used: {2 ru, 2 rv,
     // 2 (ru n_v + rv)} per tap, so the tap loop reads them instead of
     // dividing its tap index by sv
+    // Then each group's sums go to its integration's slot (2 NP + 2 doubles),
+    // and each lane normalises and writes its own integration.
     extern __shared__ int4 s_taps[];
     int4 *taps = s_taps + (threadIdx.x >> 6) * t.max_block;
+    double *res = reinterpret_cast<double *>(s_taps + 4 * t.max_block) +
+                  (threadIdx.x >> 6) * 64 * (2 * NP + 2);
     int tab_su = 0, tab_sv = 0;
     for (int64_t w = w0; w < n_w; w += ws) {   // wave-uniform
         const int64_t t0 = (w % n_tb) * 64;
@@ -667,7 +671,16 @@ __global__ void __launch_bounds__(256) aperture_degrid_taps_kernel(DegridApArgs 
                 tab_su = su;
                 tab_sv = sv;
             }
-            const double *kcf = ckr + t.cf_base[cf];
+            // this lane's integration: its taps' block, factor rows and
+            // footprint corner (32-bit element offsets; the host checks the
+            // table and plane sizes)
+            int koff = 0, puo = 0, pvo = 0, goff = 0;
+            if (fidx >= 0) {
+                koff = (int)(t.cf_base[cf] + ((du + t.hu) * nwv + dv + t.hv) * n_tap);
+                puo = 2 * ((fidx * nwu + du + t.hu) * t.W + t.R / 2 - scu);
+                pvo = 2 * ((fidx * nwv + dv + t.hv) * t.W + t.R / 2 - scv);
+                goff = 2 * ((uc - scu) * (int)a.n_v + vc - scv);
+            }
             const G *gpl[NP];
 #pragma unroll
             for (int q = 0; q < NP; q++)
@@ -675,18 +688,15 @@ __global__ void __launch_bounds__(256) aperture_degrid_taps_kernel(DegridApArgs 
             for (int rd = 0; rd < 16; rd++) {
                 const int src = 4 * rd + grp;
                 const int s_f = __shfl(fidx, src);
-                const int s_uc = __shfl(uc, src), s_vc = __shfl(vc, src);
-                const int s_du = __shfl(du, src), s_dv = __shfl(dv, src);
+                const int s_k = __shfl(koff, src), s_g = __shfl(goff, src);
+                const int s_pu = __shfl(puo, src), s_pv = __shfl(pvo, src);
                 if (s_f < 0) continue;   // the whole group skips
-                const double *kb = kcf + ((int64_t)(s_du + t.hu) * nwv + s_dv + t.hv) * n_tap;
-                const double *pu =
-                    pgu + 2 * (((int64_t)s_f * nwu + s_du + t.hu) * t.W + t.R / 2 - scu);
-                const double *pv =
-                    pgv + 2 * (((int64_t)s_f * nwv + s_dv + t.hv) * t.W + t.R / 2 - scv);
-                const int64_t g0 = (int64_t)(s_uc - scu) * a.n_v + (s_vc - scv);
+                const double *kb = ckr + s_k;
+                const double *pu = pgu + s_pu;
+                const double *pv = pgv + s_pv;
                 const G *gs[NP];
 #pragma unroll
-                for (int q = 0; q < NP; q++) gs[q] = gpl[q] + 2 * g0;
+                for (int q = 0; q < NP; q++) gs[q] = gpl[q] + s_g;
                 double sr[NP], si[NP], nr = 0.0, ni = 0.0;
 #pragma unroll
                 for (int q = 0; q < NP; q++) sr[q] = si[q] = 0.0;
@@ -721,25 +731,40 @@ __global__ void __launch_bounds__(256) aperture_degrid_taps_kernel(DegridApArgs 
                     si[q] = group16_sum(si[q]);
                 }
                 if (l == 0) {
-                    const int64_t row = (t0 + src) * a.n_baseline + b;
-                    V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol + p0);
+                    double *o = res + src * (2 * NP + 2);
 #pragma unroll
                     for (int q = 0; q < NP; q++) {
-                        double re = sr[q], im = si[q];
-                        if (a.normalize) {
-                            const double d = nr * nr + ni * ni;   // divide by conj(norm)
-                            if (d != 0.0) {
-                                const double r2 = (re * nr - im * ni) / d;
-                                const double i2 = (im * nr + re * ni) / d;
-                                re = r2;
-                                im = i2;
-                            }
-                        }
-                        out[2 * q] = (V)re;
-                        out[2 * q + 1] = (V)im;
+                        o[2 * q] = sr[q];
+                        o[2 * q + 1] = si[q];
                     }
+                    o[2 * NP] = nr;
+                    o[2 * NP + 1] = ni;
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (fidx >= 0) {
+                const double *o = res + lane * (2 * NP + 2);
+                const double nr = o[2 * NP], ni = o[2 * NP + 1];
+                const int64_t row = tm * a.n_baseline + b;
+                V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol + p0);
+#pragma unroll
+                for (int q = 0; q < NP; q++) {
+                    double re = o[2 * q], im = o[2 * q + 1];
+                    if (a.normalize) {
+                        const double d = nr * nr + ni * ni;   // divide by conj(norm)
+                        if (d != 0.0) {
+                            const double r2 = (re * nr - im * ni) / d;
+                            const double i2 = (im * nr + re * ni) / d;
+                            re = r2;
+                            im = i2;
+                        }
+                    }
+                    out[2 * q] = (V)re;
+                    out[2 * q + 1] = (V)im;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
@@ -866,7 +891,9 @@ extern "C" int ngc_aperture_degrid(ngc_context *ctx, void *stream, const double 
     // the tap-offset tables take max_block int4s per wavefront in LDS, and the
     // offsets are 32-bit
     bool sep = n_ckr * elem <= (int64_t(1) << 30) && t.max_block <= 1024 &&
-               2 * (max_sc * n_v + max_sc) < (int64_t(1) << 31);
+               2 * n_u * n_v < (int64_t(1) << 31) &&
+               2 * n_field * (2 * t.hu + 3) * (int64_t)(max_sc + 1) < (int64_t(1) << 31) &&
+               2 * n_field * (2 * t.hv + 3) * (int64_t)(max_sc + 1) < (int64_t(1) << 31);
     if (sep) {
         NGC_HIP(hipMemsetAsync(ctx->d_error + 1, 0, sizeof(int), s));
         adg_pg_check_kernel<<<pblocks(n_field * k_u * k_v), 256, 0, s>>>(a, ctx->d_error + 1);
@@ -901,8 +928,10 @@ extern "C" int ngc_aperture_degrid(ngc_context *ctx, void *stream, const double 
         auto launch = [&](auto g, auto v, auto npc) {
             using G = decltype(g);
             using V = decltype(v);
-            aperture_degrid_taps_kernel<G, V, decltype(npc)::value>
-                <<<nb, 256, 4 * t.max_block * sizeof(int4), s>>>(
+            constexpr int NPv = decltype(npc)::value;
+            aperture_degrid_taps_kernel<G, V, NPv>
+                <<<nb, 256, 4 * (t.max_block * sizeof(int4) + 64 * (2 * NPv + 2) * sizeof(double)),
+                   s>>>(
                 a, t, (const double *)d_ckr, (const double *)d_pgu, (const double *)d_pgv,
                 (const G *)grid, (V *)vis);
         };
