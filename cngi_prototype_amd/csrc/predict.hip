@@ -16,11 +16,17 @@
 //                         adjoint of the standard gridder's loop with unit
 //                         weights: same uv scale, centre cell, sub-cell offset,
 //                         bounds and tap index (_standard_grid.py:273-360)
+//   ngc_aperture_degrid   V = sum_taps conj(CK PG) G[cell] (/ conj(sum CK PG)),
+//                         the adjoint of the aperture gridder's loop with unit
+//                         weights (_aperture_grid.py:377-513), per field,
+//                         baseline, channel and pol kernel
 //
-// The degrid kernel is a gather: one thread per sample; the 64 lanes of a
-// wavefront are 64 consecutive integrations of one (baseline, channel), whose
-// footprints in one grid plane overlap, so their tap reads share cache lines.
-// Taps and sums are FP64; the grid may be complex64.
+// The standard degrid kernel is a gather: one thread per sample; the 64 lanes
+// of a wavefront are 64 consecutive integrations of one (baseline, channel),
+// whose footprints in one grid plane overlap, so their tap reads share cache
+// lines.  Its inner 7-tap sums run in the grid's precision, the outer sums and
+// the normalisation in FP64.  The aperture kernels are described where they
+// are defined.
 #include <type_traits>
 
 #include "ngc_internal.hpp"
@@ -355,7 +361,9 @@ extern "C" int ngc_standard_degrid(ngc_context *ctx, void *stream, const double 
 // one (baseline, channel).  Per pol, V = sum conj(CK*PG) G[cell] over the
 // gridder's cells, offsets and taps (v outer, u inner, FP64), divided by
 // conj(sum CK*PG) when normalize.  Taps whose kernel index falls outside the
-// K_u x K_v table weigh 0, as in the tiled gridder.
+// K_u x K_v table weigh 0, as in the tiled gridder.  This general form runs
+// when the phase gradient does not factor into u and v parts (or the tables
+// of the tap-parallel form below would be too large).
 // ---------------------------------------------------------------------------
 namespace ngc {
 
