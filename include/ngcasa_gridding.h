@@ -386,6 +386,10 @@ int ngc_standard_degrid(ngc_context *ctx, void *stream, const double *uvw,
  * float64 [n_cfb, n_cfc, n_cfp, k_u, k_v]; phase_gradient device complex128
  * [n_field, k_u, k_v]; grid device [n_imag_chan, n_imag_pol, n_u, n_v] of
  * grid_dtype (C64|C128); vis device out [T, B, C, P] of vis_dtype (C64|C128).
+ * Sums are FP64 for either grid precision.  A phase gradient that factors as
+ * PG[x][y] = PG[x][K_v/2] PG[K_u/2][y] (to 1e-9; the gcf builder's
+ * exp(i(x px + y py)) does) takes the tap-parallel kernel; any other takes a
+ * one-thread-per-sample kernel with the same result.
  * Returns NGC_ERR_FIELD if a row's field id is not in field_id (synchronises
  * `stream`).
  */
