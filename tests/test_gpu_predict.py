@@ -173,15 +173,18 @@ def test_aperture_degrid_unknown_field_raises(gpu):
                             os_, single=False)
 
 
-def test_predict_modelvis_image_with_gcf(gpu):
+@pytest.mark.parametrize("pg_form", ["random", "separable"])
+def test_predict_modelvis_image_with_gcf(gpu, pg_form):
     """predict_modelvis_image with a gcf dataset: the model, transformed
     without the spheroidal correction, is degridded with the A-term kernels,
-    fields and maps of the gcf, as the oracle degrids the same grid."""
+    fields and maps of the gcf, as the oracle degrids the same grid (both
+    A-term kernels: a phase gradient that does not factor, and the gcf
+    builder's form)."""
     import torch
     import cngi_prototype_amd._native as nat
     from cngi_prototype_amd.imaging import predict_modelvis_image
     from cngi_prototype_amd.xds import Dataset
-    c, ck, sup, pg, field, fid, maps, os_ = _ap_case(12)
+    c, ck, sup, pg, field, fid, maps, os_ = _ap_case(12, pg_form=pg_form)
     n, npad = c["image_size"], c["npad"]
     rng = np.random.default_rng(13)
     T, B = c["uvw"].shape[:2]
