@@ -48,12 +48,15 @@ def test_degrid_matches_oracle(gpu, single, chan_mode, normalize):
 
 
 @pytest.mark.parametrize("single", [False, True])
-def test_model_to_grid_matches_oracle(gpu, single):
+@pytest.mark.parametrize("n_uv", [[61, 60], [60, 61], [64, 64]])
+def test_model_to_grid_matches_oracle(gpu, single, n_uv):
+    """Odd and even grid lengths on both axes (fftshift and the padding
+    offsets differ between them)."""
     import cngi_prototype_amd._native as nat
     from cngi_prototype_amd.imaging._imaging_utils._gridding_convolutional_kernels import \
         cropped_correction_factors
     rng = np.random.default_rng(8)
-    N, n_uv = 50, [61, 60]
+    N = 50
     model = rng.normal(size=(N, N, 2, 2))
     ref = oracle.model_to_grid(model, n_uv)
     cl, cm = cropped_correction_factors(n_uv, [N, N])
