@@ -218,3 +218,39 @@ def test_predict_modelvis_image_with_gcf(gpu, pg_form):
                                  [npad, npad], c["delta"], os_)
     assert np.count_nonzero(ref) > 0
     assert rel_err(v, ref) < 1e-12
+
+
+def test_builder_phase_gradient_takes_the_tap_parallel_kernel(gpu):
+    """The phase gradient that ngc_phase_gradient builds (the gcf's
+    exp(i(x px + y py))) factors into its u and v parts to rounding, so A-term
+    prediction with a built gcf runs the tap-parallel kernel, and that kernel
+    matches the oracle on it."""
+    import torch
+    import cngi_prototype_amd._native as nat
+    c, ck, sup, _, field, fid, maps, os_ = _ap_case(21)
+    K = ck.shape[-2:]
+    rng = np.random.default_rng(22)
+    pix = rng.uniform(-400.0, 400.0, size=(3, 2))
+    pg_d = nat.phase_gradient(pix, K)
+    pg = pg_d.cpu().numpy()
+    cu, cv = K[0] // 2, K[1] // 2
+    fac = pg[:, :, cv][:, :, None] * pg[:, cu, :][:, None, :]
+    assert np.abs(fac - pg).max() < 1e-12
+    npad = c["npad"]
+    G = rng.normal(size=(3, 2, npad, npad)) + 1j * rng.normal(size=(3, 2, npad, npad))
+    nat.profile_enable(True)
+    try:
+        got = nat.aperture_degrid(_t(c["uvw"]), _t(field, torch.int32), fid, _t(G), c["freq"],
+                                  np.arange(3), np.arange(2), *maps, _t(ck), sup, pg_d,
+                                  c["delta"], os_, single=False)
+        phases = [name for name, _ in nat.profile_read()]
+    finally:
+        nat.profile_enable(False)
+    assert "aperture_degrid_tables" in phases, phases
+    ref = oracle.aperture_degrid(G, c["uvw"], field, fid, *maps, ck, sup, pg, c["freq"],
+                                 [npad, npad], c["delta"], os_)
+    assert np.count_nonzero(ref) > 0
+    # the factors round the phase as x px and y py apart, not x px + y py:
+    # about 1e-16 |phase| per tap, amplified where the steep ramps of these
+    # offsets (up to 400 pixels) make the tap sums cancel
+    assert rel_err(got.cpu().numpy(), ref) < 1e-11
