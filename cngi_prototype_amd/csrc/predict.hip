@@ -16,7 +16,7 @@
 //                         adjoint of the standard gridder's loop with unit
 //                         weights: same uv scale, centre cell, sub-cell offset,
 //                         bounds and tap index (_standard_grid.py:273-360)
-//   ngc_aperture_degrid   V = sum_taps conj(CK PG) G[cell] (/ conj(sum CK PG)),
+//   ngc_aperture_degrid   V = sum_taps conj(CK PG) G[cell] (/ sum_taps CK),
 //                         the adjoint of the aperture gridder's loop with unit
 //                         weights (_aperture_grid.py:377-513), per field,
 //                         baseline, channel and pol kernel
@@ -359,8 +359,8 @@ extern "C" int ngc_standard_degrid(ngc_context *ctx, void *stream, const double 
 // (_aperture_grid.py:377-513) with unit weights.  One thread per (row,
 // channel); the 64 lanes of a wavefront are 64 consecutive integrations of
 // one (baseline, channel).  Per pol, V = sum conj(CK*PG) G[cell] over the
-// gridder's cells, offsets and taps (v outer, u inner, FP64), divided by
-// conj(sum CK*PG) when normalize.  Taps whose kernel index falls outside the
+// gridder's cells, offsets and taps (v outer, u inner, FP64), divided by the
+// kernel's tap sum sum CK (no phase gradient) when normalize.  Taps whose kernel index falls outside the
 // K_u x K_v table weigh 0, as in the tiled gridder.  This general form runs
 // when the phase gradient does not factor into u and v parts (or the tables
 // of the tap-parallel form below would be too large).
@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(256) aperture_degrid_kernel(DegridApArgs a, co
             const int64_t scu = su / 2, scv = sv / 2;
             const double *ck = a.ck + cf * kplane;
             const G *g = grid + 2 * ((a_chan * a.n_imag_pol + a.pol_map[p]) * plane);
-            double re = 0.0, im = 0.0, nre = 0.0, nim = 0.0;
+            double re = 0.0, im = 0.0, nrm = 0.0;
             for (int64_t iv = -scv; iv < sv - scv; iv++) {
                 const int64_t kv = a.os_v * iv + v_off;
                 const int64_t gv = px.vc + iv;
@@ -447,18 +447,12 @@ __global__ void __launch_bounds__(256) aperture_degrid_kernel(DegridApArgs a, co
                     const double gr = (double)g[2 * cell], gi = (double)g[2 * cell + 1];
                     re += cr * gr + ci * gi;   // conj(c) g
                     im += cr * gi - ci * gr;
-                    nre += cr;
-                    nim += ci;
+                    nrm += c;
                 }
             }
-            if (a.normalize) {
-                const double d = nre * nre + nim * nim;   // divide by conj(norm)
-                if (d != 0.0) {
-                    const double r2 = (re * nre - im * nim) / d;
-                    const double i2 = (im * nre + re * nim) / d;
-                    re = r2;
-                    im = i2;
-                }
+            if (a.normalize && nrm != 0.0) {   // the beam is 1 at the pointing centre
+                re = re / nrm;
+                im = im / nrm;
             }
             out[2 * p] = (V)re;
             out[2 * p + 1] = (V)im;
@@ -534,7 +528,10 @@ __global__ void adg_taps_kernel(DegridApArgs a, AdgTab t, double *ckr) {
     }
 }
 
-// sets *bad when some PG[x][y] differs from PG[x][cv] PG[cu][y] by more than 1e-9
+// sets *bad when some PG[x][y] differs from PG[x][cv] PG[cu][y] by more than
+// 1e-11 (the table has unit modulus; the gcf builder's exp(i(x px + y py))
+// factors to ~1e-13).  An accepted table then moves a normalised prediction
+// by at most 1e-11 of sum |CK G| / sum CK.
 __global__ void adg_pg_check_kernel(DegridApArgs a, int *bad) {
 #pragma clang fp contract(off)
     const int64_t cu = a.k_u / 2, cv = a.k_v / 2;
@@ -549,7 +546,7 @@ __global__ void adg_pg_check_kernel(DegridApArgs a, int *bad) {
         const double2 gy = a.pg[f * kplane + cu * a.k_v + y];
         const double2 g = a.pg[i];
         const double pr = gx.x * gy.x - gx.y * gy.y, pi = gx.x * gy.y + gx.y * gy.x;
-        b |= !(fabs(pr - g.x) <= 1e-9 && fabs(pi - g.y) <= 1e-9);
+        b |= !(fabs(pr - g.x) <= 1e-11 && fabs(pi - g.y) <= 1e-11);
     }
     if (b) atomicOr(bad, 1);
 }
@@ -705,7 +702,7 @@ __global__ void __launch_bounds__(256) aperture_degrid_taps_kernel(DegridApArgs 
                 const G *gs[NP];
 #pragma unroll
                 for (int q = 0; q < NP; q++) gs[q] = gpl[q] + s_g;
-                double sr[NP], si[NP], nr = 0.0, ni = 0.0;
+                double sr[NP], si[NP], nr = 0.0;
 #pragma unroll
                 for (int q = 0; q < NP; q++) sr[q] = si[q] = 0.0;
                 auto tap = [&](int j) {
@@ -715,8 +712,7 @@ __global__ void __launch_bounds__(256) aperture_degrid_taps_kernel(DegridApArgs 
                     const double vr = pv[o.y], vi = pv[o.y + 1];
                     const double cr = k * fma(ur, vr, -ui * vi);
                     const double ci = k * fma(ur, vi, ui * vr);
-                    nr += cr;
-                    ni += ci;
+                    nr += k;
 #pragma unroll
                     for (int q = 0; q < NP; q++) {   // conj(c) g
                         const double gr = (double)gs[q][o.z], gi = (double)gs[q][o.z + 1];
@@ -732,7 +728,6 @@ __global__ void __launch_bounds__(256) aperture_degrid_taps_kernel(DegridApArgs 
                 for (int it = 0; it < n_full; it++) tap(l + 16 * it);
                 if (l + 16 * n_full < n_tap) tap(l + 16 * n_full);
                 nr = group16_sum(nr);
-                ni = group16_sum(ni);
 #pragma unroll
                 for (int q = 0; q < NP; q++) {
                     sr[q] = group16_sum(sr[q]);
@@ -746,27 +741,21 @@ __global__ void __launch_bounds__(256) aperture_degrid_taps_kernel(DegridApArgs 
                         o[2 * q + 1] = si[q];
                     }
                     o[2 * NP] = nr;
-                    o[2 * NP + 1] = ni;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             if (fidx >= 0) {
                 const double *o = res + lane * (2 * NP + 2);
-                const double nr = o[2 * NP], ni = o[2 * NP + 1];
+                const double nr = o[2 * NP];
                 const int64_t row = tm * a.n_baseline + b;
                 V *out = vis + 2 * ((row * a.n_chan + ch) * a.n_pol + p0);
 #pragma unroll
                 for (int q = 0; q < NP; q++) {
                     double re = o[2 * q], im = o[2 * q + 1];
-                    if (a.normalize) {
-                        const double d = nr * nr + ni * ni;   // divide by conj(norm)
-                        if (d != 0.0) {
-                            const double r2 = (re * nr - im * ni) / d;
-                            const double i2 = (im * nr + re * ni) / d;
-                            re = r2;
-                            im = i2;
-                        }
+                    if (a.normalize && nr != 0.0) {   // the kernel's own tap sum
+                        re = re / nr;
+                        im = im / nr;
                     }
                     out[2 * q] = (V)re;
                     out[2 * q + 1] = (V)im;

@@ -26,9 +26,11 @@ precision); storage_parms: arr_name (default 'MODEL'), incremental
 With `gcf_dataset` (the output of make_gridding_convolution_function) the
 prediction uses the mosaic A-term kernels instead: the model is transformed
 without the prolate-spheroidal correction and every visibility is
-sum conj(CK PG) G / conj(sum CK PG) over the cells, offsets and taps the
-aperture gridder would give it (ngc_aperture_degrid, the adjoint of the
-aperture gridder with unit weights); vis_dataset then needs FIELD_ID.
+sum conj(CK PG) G / sum CK over the cells, offsets and taps the aperture
+gridder would give it (ngc_aperture_degrid, the adjoint of the aperture
+gridder with unit weights, normalised so that the beam is 1 at each field's
+pointing centre: a source there predicts its flux, a source elsewhere its
+flux times the beam); vis_dataset then needs FIELD_ID.
 """
 import copy
 

@@ -379,15 +379,17 @@ int ngc_standard_degrid(ngc_context *ctx, void *stream, const double *uvw,
  * centre cell, sub-cell offsets in os_u / os_v, bounds by the largest
  * support, CK * PG[field] taps),
  *   V = sum_taps conj(CK * PG) * grid[chan_map[c], pol_map[p], cell]
- * divided by conj(sum_taps CK * PG) when `normalize` (a constant grid then
- * predicts the constant); 0 for rows on field <= -1 and samples off the grid.
+ * divided by the kernel's own tap sum, sum_taps CK (without the phase
+ * gradient), when `normalize`: the beam is 1 at the field's pointing centre,
+ * so a point source there predicts its flux and one elsewhere its flux times
+ * the beam at that offset; 0 for rows on field <= -1 and samples off the grid.
  * Arrays as for ngc_aperture_grid: field device int32 [T,B]; field_id,
  * freq_chan, chan_map, pol_map, cf_*_map and support host; conv_kernel device
  * float64 [n_cfb, n_cfc, n_cfp, k_u, k_v]; phase_gradient device complex128
  * [n_field, k_u, k_v]; grid device [n_imag_chan, n_imag_pol, n_u, n_v] of
  * grid_dtype (C64|C128); vis device out [T, B, C, P] of vis_dtype (C64|C128).
  * Sums are FP64 for either grid precision.  A phase gradient that factors as
- * PG[x][y] = PG[x][K_v/2] PG[K_u/2][y] (to 1e-9; the gcf builder's
+ * PG[x][y] = PG[x][K_v/2] PG[K_u/2][y] (to 1e-11 absolute; the gcf builder's
  * exp(i(x px + y py)) does) takes the tap-parallel kernel; any other takes a
  * one-thread-per-sample kernel with the same result.
  * Returns NGC_ERR_FIELD if a row's field id is not in field_id (synchronises

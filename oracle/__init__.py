@@ -49,6 +49,41 @@ def _lib():
     return _LIB
 
 
+def use_native_build(directory=None):
+    """Compile the restatement with -O3 -march=native for this host (into
+    `directory`, a temporary one by default) and route every call of this
+    module through that build.  Used for the timed CPU baseline only: the
+    results are those of the default build (no fast-math, no contraction).
+    Returns the library path."""
+    global _LIB
+    import subprocess
+    import tempfile
+    d = directory or tempfile.mkdtemp(prefix="oracle_native_")
+    os.makedirs(d, exist_ok=True)
+    subprocess.run(["make", "-s", "-C", _HERE, "native", f"NATIVE_DIR={d}"], check=True)
+    path = os.path.join(d, "liboracle_native.so")
+    _LIB = ctypes.CDLL(path)
+    return path
+
+
+def cpu_model():
+    """The host CPU's model name and the cores this process may use."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return name, avail
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
@@ -238,8 +273,8 @@ def aperture_degrid(grid, uvw, field, field_id, cf_baseline_map, cf_chan_map, cf
                     oversampling, chan_mode="cube", normalize=True, n_pol=None):
     """Visibility prediction with the A-term kernels: the adjoint of
     aperture_grid's loop with unit weights (same rows, fields, cells, offsets
-    and CK*PG taps), V = sum conj(CK*PG) G[cell], divided by conj(sum CK*PG)
-    when normalize.  grid [n_imag_chan, n_pol, n_u, n_v] complex; returns vis
+    and CK*PG taps), V = sum conj(CK*PG) G[cell], divided by the tap sum
+    sum CK (without the phase gradient) when normalize.  grid [n_imag_chan, n_pol, n_u, n_v] complex; returns vis
     [T, B, C, P] complex128 (0 for rows and samples that are not gridded)."""
     grid = np.ascontiguousarray(grid, dtype=np.complex128)
     uvw = _f64(uvw)
@@ -376,22 +411,23 @@ def grid_to_image(grid_kernel_orient, image_size):
     return remove_padding(img, image_size).real * (npad[0] * npad[1])
 
 
-def model_to_grid(model, n_uv):
+def model_to_grid(model, n_uv, corr=True):
     """Model image -> uv grid for prediction, the inverse placement of
     grid_to_image: pad the image [N_l, N_m, n_chan, n_pol] (Jy/pixel),
-    divided by the cropped correction image, into the centre of the n_uv
-    grid, then fftshift(fft2(ifftshift(.))) (unnormalised forward transform,
-    so a pixel of flux S gives |G| = S).  Returns [n_chan, n_pol, n_u, n_v]
-    complex in the gridder's orientation.  No reference counterpart:
+    divided by the cropped correction image (when corr; the A-term path
+    takes the model as it is), into the centre of the n_uv grid, then
+    fftshift(fft2(ifftshift(.))) (unnormalised forward transform, so a pixel
+    of flux S gives |G| = S).  Returns [n_chan, n_pol, n_u, n_v] complex in
+    the gridder's orientation.  No reference counterpart:
     predict_modelvis_image.py:20 is a stub."""
     model = np.asarray(model, dtype=np.float64)
     n_u, n_v = int(n_uv[0]), int(n_uv[1])
     size = np.array(model.shape[0:2])
-    corr = remove_padding(ps_correction_image(n_uv), size)
     pad = np.zeros((n_u, n_v) + model.shape[2:], dtype=np.float64)
     start = np.array([n_u, n_v]) // 2 - size // 2
-    pad[start[0]:start[0] + size[0], start[1]:start[1] + size[1]] = \
-        model / corr[:, :, None, None]
+    if corr:
+        model = model / remove_padding(ps_correction_image(n_uv), size)[:, :, None, None]
+    pad[start[0]:start[0] + size[0], start[1]:start[1] + size[1]] = model
     g = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(pad, axes=(0, 1)), axes=(0, 1)),
                         axes=(0, 1))
     return np.ascontiguousarray(np.moveaxis(g, [-2, -1], [0, 1]))

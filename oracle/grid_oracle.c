@@ -485,8 +485,10 @@ int oracle_aperture_grid(double *grid, double *sum_weight, int do_psf,
 /* adjoint of oracle_aperture_grid's loop with unit weights: the same rows,  */
 /* fields, cells, offsets and CK*PG taps, each sample reading               */
 /*   V = sum_taps conj(CK*PG) G[cell]                                        */
-/* and, when normalize, V / conj(sum_taps CK*PG) (a constant grid c then     */
-/* predicts c).  Samples that are not placed on the grid get 0.             */
+/* and, when normalize, V / sum_taps CK: the kernel's own tap sum without    */
+/* the phase gradient, so a point source at a field's pointing centre        */
+/* predicts its flux and one elsewhere its flux times the beam there.        */
+/* Samples that are not placed on the grid get 0.                            */
 /* ----------------------------------------------------------------------- */
 int oracle_aperture_degrid(double *vis, const double *grid, const double *uvw,
                            const double *freq_chan, const int64_t *chan_map,
@@ -567,7 +569,7 @@ int oracle_aperture_degrid(double *vis, const double *grid, const double *uvw,
                     const int64_t scu = support_u / 2, scv = support_v / 2;
                     const double *ck = conv_kernel + ((cf_baseline * n_cfc + cf_chan) * n_cfp + cf_pol) * kplane;
                     const int64_t pbase = (a_chan * n_imag_pol + a_pol) * plane;
-                    double re = 0.0, im = 0.0, norm_re = 0.0, norm_im = 0.0;
+                    double re = 0.0, im = 0.0, norm = 0.0;
                     for (int64_t iv = -scv; iv < support_v - scv; iv++) {
                         const int64_t v_indx = v_center_indx + iv;
                         const int64_t cf_v_indx = os_v * iv + v_center_offset_indx;
@@ -582,19 +584,14 @@ int oracle_aperture_degrid(double *vis, const double *grid, const double *uvw,
                             /* conj(c) * g */
                             re = re + (c_re * g_re + c_im * g_im);
                             im = im + (c_re * g_im - c_im * g_re);
-                            norm_re = norm_re + c_re;
-                            norm_im = norm_im + c_im;
+                            norm = norm + ck[kidx];
                         }
                     }
-                    if (normalize) {
-                        /* divide by conj(norm) */
-                        const double d = norm_re * norm_re + norm_im * norm_im;
-                        if (d != 0.0) {
-                            const double r2 = (re * norm_re - im * norm_im) / d;
-                            const double i2 = (im * norm_re + re * norm_im) / d;
-                            re = r2;
-                            im = i2;
-                        }
+                    if (normalize && norm != 0.0) {
+                        /* divide by the kernel's own tap sum, without the
+                           phase gradient: the beam is 1 at the pointing centre */
+                        re = re / norm;
+                        im = im / norm;
                     }
                     vis[2 * (base + p)] = re;
                     vis[2 * (base + p) + 1] = im;

@@ -36,13 +36,20 @@ def _oracle_inputs(mxds, weight_var):
             np.asarray(xds[weight_var].values), np.asarray(xds.coords["chan"].values))
 
 
-@pytest.mark.parametrize("weighting", ["natural", "briggs", "uniform"])
-def test_make_imaging_weight(gpu, weighting):
+@pytest.mark.parametrize("weighting,n_pol", [("natural", 2), ("briggs", 2), ("uniform", 2),
+                                             ("briggs", 4), ("uniform", 4)])
+def test_make_imaging_weight(gpu, weighting, n_pol):
+    """n_pol = 4 goes through the full_pol opt-in (the C2 workload's Briggs
+    chain); without it the reference's assert refuses more than 2 pols."""
     from cngi_prototype_amd.imaging import make_imaging_weight
-    cfg, mxds = _mxds()
+    cfg, mxds = _mxds(n_pol=n_pol)
     gp = _grid_parms(cfg)
-    out = make_imaging_weight(mxds, {"weighting": weighting, "robust": 0.5}, dict(gp),
-                              {"xds": "xds0"})
+    iparms = {"weighting": weighting, "robust": 0.5}
+    if n_pol > 2:
+        with pytest.raises(AssertionError, match="Full polarization"):
+            make_imaging_weight(mxds, dict(iparms), dict(gp), {"xds": "xds0"})
+        iparms["full_pol"] = True
+    out = make_imaging_weight(mxds, iparms, dict(gp), {"xds": "xds0"})
     xds = out.attrs["xds0"]
     dg = xds.attrs["data_groups"][0]["0"]
     if weighting == "natural":

@@ -48,6 +48,34 @@ def test_degrid_matches_oracle(gpu, single, chan_mode, normalize):
 
 
 @pytest.mark.parametrize("single", [False, True])
+@pytest.mark.parametrize("n_pol,support", [(1, 7), (4, 7), (4, 9), (1, 13), (2, 16)])
+def test_degrid_shapes_match_oracle(gpu, single, n_pol, support):
+    """130 integrations (two full 64-lane time blocks and a partial third),
+    1, 2 or 4 pols (the unrolled pol paths) and supports 7 to 16 (the
+    support-8 and support-16 kernel forms) against the oracle.  Supports above
+    9 use oversampling 50, which keeps the kernel table within the 512 entries
+    the degridder holds (os * (S//2 + 1))."""
+    import torch
+    import cngi_prototype_amd._native as nat
+    c = small_case(n_time=130, n_chan=2, n_pol=n_pol, image_size=96)
+    npad = c["npad"]
+    rng = np.random.default_rng(40 + support)
+    G = rng.normal(size=(2, n_pol, npad, npad)) + 1j * rng.normal(size=(2, n_pol, npad, npad))
+    if single:
+        G = G.astype(np.complex64).astype(np.complex128)
+    os_ = 100 if support <= 9 else 50
+    cgk = oracle.ps_kernel_1d(os_, support)
+    ref = oracle.standard_degrid(G, c["uvw"], c["freq"], cgk, [npad, npad], c["delta"], support,
+                                 os_, n_pol=n_pol)
+    got = nat.standard_degrid(_t(c["uvw"]), _t(G, torch.complex64 if single else torch.complex128),
+                              c["freq"], np.arange(2), np.arange(n_pol), cgk, c["delta"], support,
+                              os_, single=single).cpu().numpy()
+    assert np.count_nonzero(ref) > 0
+    assert np.array_equal(got == 0, ref == 0)
+    assert rel_err(got, ref) < (1e-6 if single else 1e-12)
+
+
+@pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("n_uv", [[61, 60], [60, 61], [64, 64]])
 def test_model_to_grid_matches_oracle(gpu, single, n_uv):
     """Odd and even grid lengths on both axes (fftshift and the padding
@@ -160,6 +188,45 @@ def test_aperture_degrid_matches_oracle(gpu, single, normalize, form):
     assert np.count_nonzero(ref) > 0
     assert np.array_equal(got == 0, ref == 0)
     assert rel_err(got, ref) < (1e-6 if single else 1e-12)
+
+
+@pytest.mark.parametrize("n_pol", [1, 4])
+def test_aperture_degrid_pols_and_time_blocks(gpu, n_pol):
+    """The separable A-term kernel with 1 pol and with 4 pols sharing one
+    convolution function (the NP = 4 pass), over 130 integrations (a partial
+    last time block), against the oracle."""
+    import torch
+    import cngi_prototype_amd._native as nat
+    rng = np.random.default_rng(50 + n_pol)
+    c = small_case(n_time=130, n_chan=2, n_pol=n_pol, image_size=96)
+    T, B = c["uvw"].shape[:2]
+    os_, K = [4, 5], [40, 48]
+    ck = rng.uniform(0.1, 1.0, size=(2, 1, 1, K[0], K[1]))
+    sup = np.array([[5, 5], [7, 7]]).reshape(2, 1, 1, 2)
+    x = np.arange(K[0])[:, None] - K[0] // 2
+    y = np.arange(K[1])[None, :] - K[1] // 2
+    slopes = rng.uniform(-0.3, 0.3, size=(3, 2))
+    pg = np.exp(1j * (x[None] * slopes[:, 0, None, None] + y[None] * slopes[:, 1, None, None]))
+    field = rng.integers(0, 3, size=(T, B)) + 10
+    field[5, 2] = -1
+    maps = (rng.integers(0, 2, size=B), np.zeros(2, int), np.zeros(n_pol, int))
+    npad = c["npad"]
+    G = rng.normal(size=(2, n_pol, npad, npad)) + 1j * rng.normal(size=(2, n_pol, npad, npad))
+    ref = oracle.aperture_degrid(G, c["uvw"], field, [10, 11, 12], *maps, ck, sup, pg, c["freq"],
+                                 [npad, npad], c["delta"], os_)
+    nat.profile_enable(True)
+    try:
+        got = nat.aperture_degrid(_t(c["uvw"]), _t(field, torch.int32), [10, 11, 12], _t(G),
+                                  c["freq"], np.arange(2), np.arange(n_pol), *maps, _t(ck), sup,
+                                  _t(pg), c["delta"], os_, single=False)
+        phases = [name for name, _ in nat.profile_read()]
+    finally:
+        nat.profile_enable(False)
+    assert "aperture_degrid_tables" in phases, phases
+    got = got.cpu().numpy()
+    assert np.count_nonzero(ref) > 0
+    assert np.array_equal(got == 0, ref == 0)
+    assert rel_err(got, ref) < 1e-12
 
 
 def test_aperture_degrid_unknown_field_raises(gpu):

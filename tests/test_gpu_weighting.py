@@ -17,8 +17,12 @@ def _dev(a):
 
 @pytest.mark.parametrize("weighting,robust", [("briggs", 0.5), ("briggs", -1.3), ("uniform", 0.0)])
 @pytest.mark.parametrize("precision", ["fp64", "fp64_tiled"])
-@pytest.mark.parametrize("n_pol", [1, 2])
+@pytest.mark.parametrize("n_pol", [1, 2, 4])
 def test_briggs_chain_matches_oracle(gpu, weighting, robust, precision, n_pol):
+    """n_pol = 4 is the full_pol opt-in the C2 workload uses (XX, XY, YX, YY):
+    every pol's density plane is gridded from (w_0 + w_1)/2 and each pol's
+    natural weight is divided by its own plane's factor, as the reference's
+    loops do for n_pol != 2 (_standard_grid.py:328-330, 506-516)."""
     import cngi_prototype_amd._native as nat
     c = small_case(n_time=20, n_chan=3, n_pol=n_pol, image_size=64, nan_frac=0.05)
     N = c["image_size"]

@@ -398,6 +398,102 @@ def test_aperture_grid_reduces_to_standard(do_psf):
         assert np.allclose(swa, sws, rtol=1e-13)
 
 
+@pytest.mark.parametrize("phase", [0.0, 0.7])
+def test_aperture_sum_weight_closed_form(phase):
+    """With a constant real CK = c over the whole table and a constant phase
+    gradient exp(i phi), every gridded sample's taps sum to norm = S_u S_v c
+    exp(i phi) whatever its sub-cell offsets.  So image mode's sum_weight is
+    sum of w Re(norm^2) = W (S_u S_v c)^2 cos(2 phi), PSF mode's is
+    W S_u S_v c cos(phi) (_aperture_grid.py:508-511); |norm|^2, Re(norm) or a
+    dropped square all give other numbers.  The grid itself is W_v c
+    exp(i phi) summed per footprint, so its total is S_u S_v c exp(i phi)
+    sum of w V."""
+    rng = np.random.default_rng(11)
+    T, B, NC, P = 5, 7, 2, 2
+    K, os_, Su, Sv, c = 90, 6, 5, 3, 0.37
+    ck = np.full((1, 1, 1, K, K), c)
+    pg = np.full((1, K, K), np.exp(1j * phase))
+    uvw = rng.uniform(-400, 400, size=(T, B, 3))
+    uvw[1, 2, 0] = np.nan
+    vis = rng.standard_normal((T, B, NC, P)) + 1j * rng.standard_normal((T, B, NC, P))
+    w = rng.uniform(0.3, 2.0, size=(T, B, NC, P))
+    w[0, 3, 1, 1] = np.nan
+    freq = np.array([1.0e9, 1.05e9])
+    n = [60, 60]
+    cell = 0.4 * 0.98 / (2 * 400.0 * freq.max() / C)
+    delta = np.array([-cell, cell])
+    args = (uvw, w, np.zeros((T, B), int), [0], np.zeros(B, int), np.zeros(NC, int),
+            np.zeros(P, int))
+    sup = np.zeros((1, 1, 1, 2), int)
+    sup[..., 0], sup[..., 1] = Su, Sv
+    g_img, sw_img = oracle.aperture_grid(vis, *args, ck, sup, pg, freq, n, delta, [os_, os_],
+                                         do_psf=False)
+    g_psf, sw_psf = oracle.aperture_grid(vis, *args, ck, sup, pg, freq, n, delta, [os_, os_],
+                                         do_psf=True)
+    valid = ~np.isnan(w) & ~np.isnan(uvw[:, :, 0])[:, :, None, None]
+    W = np.where(valid, w, 0.0).sum(axis=(0, 1))          # [NC, P]
+    WV = np.where(valid, w * vis, 0.0).sum(axis=(0, 1))
+    norm = Su * Sv * c * np.exp(1j * phase)
+    assert np.allclose(sw_img, W * (norm ** 2).real, rtol=1e-12)
+    assert np.allclose(sw_psf, W * norm.real, rtol=1e-12)
+    assert np.allclose(g_img.sum(axis=(2, 3)), norm * WV, rtol=1e-12)
+    assert np.allclose(g_psf.sum(axis=(2, 3)), norm * W, rtol=1e-12)
+    if phase != 0.0:
+        # the closed form separates Re(norm^2) from |norm|^2
+        assert not np.allclose(sw_img, W * abs(norm) ** 2, rtol=1e-3)
+
+
+def test_aperture_bounds_use_the_full_support():
+    """The aperture gridder keeps a sample only if its centre cell u0
+    satisfies u0 - max(SUPPORT) >= 0 and u0 + max(SUPPORT) < n (the full
+    support as half-width, _aperture_grid.py:397,447), where the standard
+    gridder asks only u0 +- S//2 inside the grid (_standard_grid.py:320).
+    Samples are placed at chosen centre cells on both edges of u and v; the
+    aperture grid equals the standard grid of just the samples the full-support
+    rule keeps, and the standard grid still holds the others."""
+    os_, S, K, n = 100, 7, 800, 40
+    ck = _separable_ps_kernel(os_, K)[None, None, None]
+    freq = np.array([1.0e9])
+    cell = 1e-4
+    delta = np.array([-cell, cell])
+    scale = -(freq[0] * delta * n) / C
+    # (u0, v0) centre cells; the 0.2-cell shift keeps int(pos + 0.5) away from ties
+    cells = [(3, 20), (6, 20), (7, 20), (20, 3), (20, 7), (32, 20), (33, 20), (34, 20),
+             (20, 32), (20, 33), (20, 36), (7, 32), (20, 20)]
+    keep_ap = [S <= u0 and u0 + S < n and S <= v0 and v0 + S < n for u0, v0 in cells]
+    keep_std = [S // 2 <= u0 and u0 + S // 2 < n and S // 2 <= v0 and v0 + S // 2 < n
+                for u0, v0 in cells]
+    assert sum(keep_ap) >= 4 and sum(keep_std) - sum(keep_ap) >= 5
+    uvw = np.zeros((1, len(cells), 3))
+    for b, (u0, v0) in enumerate(cells):
+        uvw[0, b, 0] = (u0 + 0.2 - n // 2) / scale[0]
+        uvw[0, b, 1] = (v0 - 0.2 - n // 2) / scale[1]
+    rng = np.random.default_rng(5)
+    vis = rng.standard_normal((1, len(cells), 1, 1)) + 1j * rng.standard_normal(
+        (1, len(cells), 1, 1))
+    w = rng.uniform(0.5, 1.5, size=(1, len(cells), 1, 1))
+    sup = np.full((1, 1, 1, 2), S)
+    pg = np.ones((1, K, K), dtype=np.complex128)
+    B = len(cells)
+    ga, swa = oracle.aperture_grid(vis, uvw, w, np.zeros((1, B), int), [0], np.zeros(B, int),
+                                   [0], [0], ck, sup, pg, freq, [n, n], delta, [os_, os_],
+                                   do_psf=True)
+    cgk = oracle.ps_kernel_1d(os_, S)
+    gs_all, sws_all = oracle.standard_grid(vis, uvw, w, freq, cgk, [n, n], delta, S, os_,
+                                           do_psf=True, complex_grid=True)
+    w_ap = np.where(np.array(keep_ap)[None, :, None, None], w, np.nan)
+    gs_ap, sws_ap = oracle.standard_grid(vis, uvw, w_ap, freq, cgk, [n, n], delta, S, os_,
+                                         do_psf=True, complex_grid=True)
+    assert np.allclose(ga, gs_ap, rtol=0, atol=1e-13 * np.abs(gs_ap).max())
+    assert np.allclose(swa, sws_ap, rtol=1e-13)
+    # the standard gridder kept the samples the aperture rule dropped
+    w_std = np.where(np.array(keep_std)[None, :, None, None], w, np.nan)
+    gs_std, sws_std = oracle.standard_grid(vis, uvw, w_std, freq, cgk, [n, n], delta, S, os_,
+                                           do_psf=True, complex_grid=True)
+    assert np.allclose(gs_all, gs_std, rtol=0, atol=1e-13 * np.abs(gs_std).max())
+    assert sws_all[0, 0] > swa[0, 0] * 1.5
+
+
 def test_aperture_weight_grid_closed_form():
     """The weight gridder puts every sample's taps at the grid centre: grid
     = (sum of valid w) * CK*PG at os i + K//2, sum_weight = W Re(sum taps)."""
@@ -470,6 +566,47 @@ def test_airy_pattern_limits():
     assert np.allclose(p1[0, 0, 1], ref, rtol=1e-13, atol=1e-15)
     assert np.allclose(p2[0, 0, 0], ref ** 2, rtol=1e-13, atol=1e-15)
     assert np.allclose(pb[0, 0, 0], ref, atol=1e-6)
+
+
+@pytest.mark.parametrize("ratio", [4.0, 8.5])
+def test_casa_airy_blocked_at_zeros_of_the_blockage_term(ratio):
+    """CASA's obscured Airy voltage pattern (_make_pb_symmetric.py:226-231) is
+    (A 2J1(r)/r - 2J1(rL)/(rL)) / (A - 1) with A = (D/b)^2 and L = D/b.
+    Where r L is a zero of J1 (scipy's jn_zeros, independent of the
+    pattern's own jn calls) the second term vanishes, so the pattern must be
+    exactly A/(A-1) 2J1(r)/r there; ipower 2 squares it.  The cell is chosen
+    so that pixels one and two steps along l, and one step along m, land on
+    the first three zeros.  A swapped L (r/L), a dropped area ratio or a
+    ratio without its square all miss these values by 1e-2 or more."""
+    from scipy.special import j1, jn_zeros
+    D = 12.0
+    b = D / ratio
+    A, L = ratio ** 2, ratio
+    f = np.array([2.3e11])
+    k = 2 * np.pi * f[0] / C
+    z = jn_zeros(1, 3)
+    n = 16
+    c = [n // 2, n // 2]
+    # pixel (c+1, c) lands r L on z[0]; pixel (c+2, c) on 2 z[0]
+    # (not a zero); pixel (c, c+1) on z[1] via its own cell along m
+    cell = np.array([-z[0] / (L * k * D / 2), z[1] / (L * k * D / 2)])
+    p1 = oracle.airy_rorder(f, 1, [D], [b], 1, cell, [n, n], c)
+    p2 = oracle.airy_rorder(f, 1, [D], [b], 2, cell, [n, n], c)
+    for (i, j), zero in (((c[0] + 1, c[1]), z[0]), ((c[0], c[1] + 1), z[1]),
+                         ((c[0] - 1, c[1]), z[0]), ((c[0], c[1] - 1), z[1])):
+        r = zero / L
+        expect = A / (A - 1) * 2 * j1(r) / r
+        assert abs(p1[0, 0, 0, i, j] - expect) < 1e-12 * max(1.0, abs(expect))
+        assert abs(p2[0, 0, 0, i, j] - expect ** 2) < 1e-12
+        # the wrong argument r / L would have given a clearly different value
+        wrong = (A * 2 * j1(r) / r - 2 * j1(r / L) / (r / L)) / (A - 1)
+        assert abs(wrong - expect) > 1e-2
+    # a pixel where r L = 2 z0 is not a zero: both terms contribute
+    r = 2 * z[0] / L
+    full = (A * 2 * j1(r) / r - 2 * j1(r * L) / (r * L)) / (A - 1)
+    assert abs(p1[0, 0, 0, c[0] + 2, c[1]] - full) < 1e-12
+    assert abs(full - A / (A - 1) * 2 * j1(r) / r) > 1e-4
+    assert p1[0, 0, 0, c[0], c[1]] == 1.0
 
 
 def test_pb_image_layout_and_scaling():
@@ -790,16 +927,23 @@ def test_aperture_degrid_is_the_adjoint_of_aperture_grid():
 
 
 def test_aperture_degrid_of_a_constant_grid():
-    """Normalised by conj(sum CK*PG), a constant grid predicts the constant on
-    exactly the samples the aperture gridder places, and 0 elsewhere."""
+    """Normalised by the kernel's tap sum sum CK, a constant grid c predicts
+    c conj(sum CK PG) / sum CK; with a constant phase gradient exp(i phi_f) per
+    field that is c exp(-i phi_f) whatever the real kernel and the sample's
+    offsets, on exactly the samples the aperture gridder places, and 0
+    elsewhere.  (Dividing by conj(sum CK PG) instead would give c.)"""
     vis, uvw, freq, delta, ck, sup, pg, field, fid, maps, os_ = _ap_degrid_case(5, fill=1.4)
     T, B, NC, P = vis.shape
+    phi = np.array([0.3, -1.1, 2.0])
+    pg = np.exp(1j * phi)[:, None, None] * np.ones(pg.shape)
     n_uv = [52, 48]
     G = np.full((NC, P) + tuple(n_uv), 0.5 + 2.0j)
     dg = oracle.aperture_degrid(G, uvw, field, fid, *maps, ck, sup, pg, freq, n_uv, delta, os_)
     on = dg != 0
     assert on.any() and (~on).any()
-    assert np.allclose(dg[on], 0.5 + 2.0j, rtol=1e-12)
+    fidx = np.where(field >= 0, field - 10, 0)
+    expect = (0.5 + 2.0j) * np.exp(-1j * phi[fidx])[:, :, None, None] * np.ones(dg.shape)
+    assert np.allclose(dg[on], expect[on], rtol=1e-12)
     # placed samples: those the gridder gives a nonzero footprint with unit data
     one = np.ones(vis.shape, dtype=np.complex128)
     for t in range(T):
@@ -810,6 +954,69 @@ def test_aperture_degrid_of_a_constant_grid():
                 g, _ = oracle.aperture_grid(one, uvw, sel, field, fid, *maps, ck, sup, pg, freq,
                                             n_uv, delta, os_, do_psf=False)
                 assert bool(np.any(g != 0)) == bool(on[t, b, c, 0])
+
+
+def _mosaic_point_case():
+    """One 12 m dish type (CASA Airy with a 0.75 m blockage) at 100 GHz
+    (beam FWHM ~52"), a 200 x 200 image of 0.75" cells padded to 240, GCF
+    oversampling 10 (support 13), one field whose pointing centre sits
+    (dx, dy) = (28, -16) pixels from the phase centre (beam 0.57 there), and 80 rows whose uv points keep the
+    kernels on the grid.  Returns the pieces the A-term prediction needs."""
+    rng = np.random.default_rng(17)
+    N, npad = 200, 240
+    cell = 0.75 * oracle.ARCSEC_TO_RAD
+    delta = np.array([-cell, cell])
+    freq = np.array([1.0e11])
+    dish, blk = [12.0], [0.75]
+    os_ = [10, 10]
+    gcf = oracle.make_gcf(freq, 1, [0, 0], np.array([[0, 1]]), dish, blk, delta, [npad, npad],
+                          oversampling=os_, max_support=[15, 15])
+    K = np.array(gcf["CONV_KERNEL"].shape[-2:])
+    ra0, dec0 = 1.0, -0.4
+    dx, dy = 28, -16
+    l, m = dx * delta[0], dy * delta[1]
+    n = np.sqrt(1 - l * l - m * m)
+    dec = np.arcsin(m * np.cos(dec0) + n * np.sin(dec0))
+    ra = ra0 + np.arctan2(l, n * np.cos(dec0) - m * np.sin(dec0))
+    pg = oracle.phase_gradient(np.array([[ra, dec]]), [ra0, dec0], delta, [npad, npad], os_, K)
+    # |u| below (npad/2 - max support - 2) cells
+    umax = (npad // 2 - 17) / (npad * cell) * C / freq[0]
+    uvw = rng.uniform(-umax, umax, size=(80, 1, 3))
+    return dict(N=N, npad=npad, delta=delta, freq=freq, gcf=gcf, pg=pg, uvw=uvw, os_=os_,
+                dish=dish, blk=blk, dx=dx, dy=dy)
+
+
+def test_aperture_prediction_of_point_sources_carries_the_beam():
+    """A 1 Jy model pixel at the field's pointing centre predicts |V| = 1 on
+    every sample (the beam is 1 there: conj(PG) undoes the source's phase
+    ramp exactly), and one at the phase centre predicts |V| = the pair beam
+    VP^2 at the pointing offset, taken from the CASA Airy power pattern
+    itself (make_pb's).  Tolerance 1e-2 for the latter: the kernel is cut
+    where the weight kernel falls to 2.5% of its peak, which leaves ~0.5% of
+    the beam out.  A source mirrored through the phase centre (a sign slip
+    in the phase gradient or the conjugate) would see the beam at twice the
+    offset, ~0.06."""
+    c = _mosaic_point_case()
+    N, npad, g = c["N"], c["npad"], c["gcf"]
+    pb = oracle.pb_image(c["freq"], 1, c["dish"], c["blk"], 2, c["delta"], [N, N],
+                         [N // 2, N // 2])[:, :, 0, 0, 0]
+    beam = pb[N // 2 + c["dx"], N // 2 + c["dy"]]
+    assert 0.2 < beam < 0.8
+    fld = np.zeros((c["uvw"].shape[0], 1), int)
+    out = {}
+    for where, (px, py) in (("pointing", (N // 2 + c["dx"], N // 2 + c["dy"])),
+                            ("phase", (N // 2, N // 2))):
+        model = np.zeros((N, N, 1, 1))
+        model[px, py] = 1.0
+        G = oracle.model_to_grid(model, [npad, npad], corr=False)
+        out[where] = oracle.aperture_degrid(G, c["uvw"], fld, [0], [0], [0], [0],
+                                            g["CONV_KERNEL"], g["SUPPORT"], c["pg"], c["freq"],
+                                            [npad, npad], c["delta"], c["os_"])[:, 0, 0, 0]
+    assert np.all(out["pointing"] != 0)
+    assert np.abs(np.abs(out["pointing"]) - 1.0).max() < 1e-12
+    assert np.abs(np.abs(out["phase"]) - beam).max() < 1e-2
+    # not the old normalisation, which predicted 1 for both
+    assert np.abs(np.abs(out["phase"]) - 1.0).min() > 0.1
 
 
 def test_aperture_degrid_reduces_to_standard_degrid():
