@@ -224,6 +224,30 @@ def _dev(t, name):
     return t
 
 
+def _uvw_rows(uvw, rows=None):
+    """(T, B) of a float64 [T, B, 3] uvw tensor; the kernels read uvw[row*3]
+    and uvw[row*3 + 1] directly, so any other dtype or shape is refused."""
+    import torch
+    if uvw.dtype != torch.float64:
+        raise TypeError(f"uvw must be float64, got {uvw.dtype}")
+    if uvw.dim() != 3 or uvw.shape[2] != 3:
+        raise ValueError(f"uvw must be [n_time, n_baseline, 3], got {tuple(uvw.shape)}")
+    T, B = int(uvw.shape[0]), int(uvw.shape[1])
+    if rows is not None and (T, B) != tuple(int(x) for x in rows):
+        raise ValueError(f"uvw is [{T}, {B}, 3] but the data have {tuple(rows)} rows")
+    return T, B
+
+
+def _field_rows(field, T, B):
+    """FIELD_ID must be int32 [T, B], one id per row, as the kernels index it."""
+    import torch
+    if field.dtype != torch.int32:
+        raise TypeError("field must be int32")
+    if tuple(field.shape) != (T, B):
+        raise ValueError(f"field must be [n_time, n_baseline] = [{T}, {B}], "
+                         f"got {tuple(field.shape)}")
+
+
 # ---------------------------------------------------------------------------
 # entry points
 # ---------------------------------------------------------------------------
@@ -236,8 +260,7 @@ def standard_grid(uvw, weight, freq_chan, chan_map, pol_map, cgk_1d, n_imag_chan
     _dev(uvw, "uvw")
     _dev(weight, "weight")
     n_time, n_baseline, n_chan, n_pol = weight.shape
-    if tuple(uvw.shape[:2]) != (n_time, n_baseline) or uvw.shape[2] < 2:
-        raise ValueError("uvw shape does not match weight")
+    _uvw_rows(uvw, (n_time, n_baseline))
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     n_imag_pol = n_pol if n_imag_pol is None else int(n_imag_pol)
     n_u, n_v = int(n_uv[0]), int(n_uv[1])
@@ -287,6 +310,7 @@ class GridPlan:
         self._ctx = context()
         self.handle = None
         n_time, n_baseline, n_chan, n_pol = weight.shape
+        _uvw_rows(_dev(uvw, "uvw"), (n_time, n_baseline))
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         single = prec in (PREC_FP32_KAHAN, PREC_FP32)
         n_u, n_v = int(n_uv[0]), int(n_uv[1])
@@ -365,6 +389,7 @@ def imaging_weight_degrid(uvw, natural_weight, density_grid, factors, freq_chan,
     _dev(natural_weight, "natural_weight")
     _dev(density_grid, "density_grid")
     n_time, n_baseline, n_chan, n_pol = natural_weight.shape
+    _uvw_rows(uvw, (n_time, n_baseline))
     nc, npol, nu, nv = density_grid.shape
     if out is None:
         out = torch.empty_like(natural_weight)
@@ -392,11 +417,11 @@ def aperture_grid(uvw, weight, field, field_id, freq_chan, chan_map, pol_map,
     for t, n in ((uvw, "uvw"), (weight, "weight"), (field, "field"),
                  (conv_kernel, "conv_kernel"), (phase_gradient, "phase_gradient")):
         _dev(t, n)
-    if field.dtype != torch.int32:
-        raise TypeError("field must be int32")
+    n_time, n_baseline, n_chan, n_pol = weight.shape
+    _uvw_rows(uvw, (n_time, n_baseline))
+    _field_rows(field, n_time, n_baseline)
     if conv_kernel.dtype != torch.float64 or phase_gradient.dtype != torch.complex128:
         raise TypeError("conv_kernel must be float64 and phase_gradient complex128")
-    n_time, n_baseline, n_chan, n_pol = weight.shape
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     n_cfb, n_cfc, n_cfp, k_u, k_v = conv_kernel.shape
     if tuple(phase_gradient.shape[1:]) != (k_u, k_v):
@@ -633,7 +658,7 @@ def standard_degrid(uvw, grid, freq_chan, chan_map, pol_map, cgk_1d, delta_lm, s
     _dev(grid, "grid")
     if grid.dtype not in (torch.complex64, torch.complex128) or not grid.is_contiguous():
         raise TypeError("grid must be a contiguous complex64/complex128 tensor")
-    T, B = uvw.shape[0], uvw.shape[1]
+    T, B = _uvw_rows(uvw)
     fr, frp = _host(np.asarray(freq_chan).ravel(), np.float64)
     cm, cmp_ = _host(np.asarray(chan_map).ravel(), np.int32)
     pm, pmp = _host(np.asarray(pol_map).ravel(), np.int32)
@@ -666,13 +691,12 @@ def aperture_degrid(uvw, field, field_id, grid, freq_chan, chan_map, pol_map, cf
     for t, n in ((uvw, "uvw"), (field, "field"), (grid, "grid"), (conv_kernel, "conv_kernel"),
                  (phase_gradient, "phase_gradient")):
         _dev(t, n)
-    if field.dtype != torch.int32:
-        raise TypeError("field must be int32")
+    T, B = _uvw_rows(uvw)
+    _field_rows(field, T, B)
     if conv_kernel.dtype != torch.float64 or phase_gradient.dtype != torch.complex128:
         raise TypeError("conv_kernel must be float64 and phase_gradient complex128")
     if grid.dtype not in (torch.complex64, torch.complex128) or not grid.is_contiguous():
         raise TypeError("grid must be a contiguous complex64/complex128 tensor")
-    T, B = uvw.shape[0], uvw.shape[1]
     fid, fidp = _host(np.asarray(field_id).ravel(), np.int32)
     fr, frp = _host(np.asarray(freq_chan).ravel(), np.float64)
     cm, cmp_ = _host(np.asarray(chan_map).ravel(), np.int32)
@@ -827,9 +851,9 @@ def direction_rotate(uvw, vis, field, field_ids, uvw_rotmat, phase_rotation, fre
     ctx = context()
     for t, n in ((uvw, "uvw"), (vis, "vis"), (field, "field")):
         _dev(t, n)
-    if field.dtype != torch.int32:
-        raise TypeError("field must be int32")
     T, B, C, P = vis.shape
+    _uvw_rows(uvw, (T, B))
+    _field_rows(field, T, B)
     uvw_out = torch.empty_like(uvw)
     vis_out = torch.empty(vis.shape, dtype=torch.complex64 if single_precision else
                           torch.complex128, device=vis.device)

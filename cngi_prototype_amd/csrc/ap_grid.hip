@@ -198,10 +198,12 @@ extern "C" int ngc_aperture_grid(
     uv_scale_host(freq_chan, n_chan, delta_l, delta_m, n_u, n_v, scale);
     void *d_scale, *d_cmap, *d_pmap, *d_cfb, *d_cfc, *d_cfp, *d_sup, *d_fid;
     NGC_TRY(upload(ctx, s, "uv_scale", scale.data(), scale.size() * sizeof(double), &d_scale));
-    NGC_TRY(upload(ctx, s, "chan_map", chan_map, n_chan * sizeof(int32_t) + 4, &d_cmap));
+    NGC_TRY(upload(ctx, s, "chan_map", padded_map(chan_map, n_chan).data(), n_chan * sizeof(int32_t) + 4, &d_cmap));
     NGC_TRY(upload(ctx, s, "pol_map", pol_map, n_pol * sizeof(int32_t), &d_pmap));
-    NGC_TRY(upload(ctx, s, "cf_bl_map", cf_baseline_map, n_baseline * sizeof(int32_t) + 4, &d_cfb));
-    NGC_TRY(upload(ctx, s, "cf_chan_map", cf_chan_map, n_chan * sizeof(int32_t) + 4, &d_cfc));
+    NGC_TRY(upload(ctx, s, "cf_bl_map", padded_map(cf_baseline_map, n_baseline).data(),
+                   n_baseline * sizeof(int32_t) + 4, &d_cfb));
+    NGC_TRY(upload(ctx, s, "cf_chan_map", padded_map(cf_chan_map, n_chan).data(),
+                   n_chan * sizeof(int32_t) + 4, &d_cfc));
     NGC_TRY(upload(ctx, s, "cf_pol_map", cf_pol_map, n_pol * sizeof(int32_t), &d_cfp));
     NGC_TRY(upload(ctx, s, "support", support, n_sup * sizeof(int32_t), &d_sup));
     NGC_TRY(upload(ctx, s, "field_id", field_id, n_field * sizeof(int32_t), &d_fid));

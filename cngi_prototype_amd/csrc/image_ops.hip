@@ -412,7 +412,8 @@ int ngc_imaging_weight_degrid(ngc_context *ctx, void *stream, const double *uvw,
     uv_scale_host(freq_chan, n_chan, delta_l, delta_m, n_u, n_v, scale);
     void *d_scale, *d_cmap, *d_pmap, *d_fac;
     NGC_TRY(upload(ctx, s, "uv_scale", scale.data(), scale.size() * sizeof(double), &d_scale));
-    NGC_TRY(upload(ctx, s, "chan_map", chan_map, n_chan * sizeof(int32_t) + 4, &d_cmap));
+    NGC_TRY(upload(ctx, s, "chan_map", padded_map(chan_map, n_chan).data(), n_chan * sizeof(int32_t) + 4,
+                   &d_cmap));
     NGC_TRY(upload(ctx, s, "pol_map", pol_map, n_pol * sizeof(int32_t), &d_pmap));
     NGC_TRY(upload(ctx, s, "briggs", briggs_factors, 2 * n_imag_chan * n_imag_pol * sizeof(double),
                    &d_fac));

@@ -108,6 +108,15 @@ int enter_stream(ngc_context *ctx, hipStream_t stream);
 int upload(ngc_context *ctx, hipStream_t stream, const char *name,
            const void *host, size_t bytes, void **dev);
 
+// An int32 map of n entries followed by one zero entry (kernels that look one
+// channel or baseline ahead read the pad); the caller's array itself is read
+// for exactly n entries.
+inline std::vector<int32_t> padded_map(const int32_t *map, int64_t n) {
+    std::vector<int32_t> v((size_t)n + 1, 0);
+    for (int64_t i = 0; i < n; i++) v[(size_t)i] = map[i];
+    return v;
+}
+
 int fft_destroy_plans(ngc_context *ctx);
 int fft_execute(ngc_context *ctx, hipStream_t s, void *buf, int64_t n_u, int64_t n_v,
                 int64_t batch, bool dbl, bool forward);

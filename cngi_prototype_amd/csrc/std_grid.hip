@@ -1449,7 +1449,8 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     uv_scale_host(freq, a.n_chan, delta_l, delta_m, a.n_u, a.n_v, scale);
     void *d_scale, *d_cmap, *d_pmap, *d_cgk;
     NGC_TRY(upload(ctx, s, "uv_scale", scale.data(), scale.size() * sizeof(double), &d_scale));
-    NGC_TRY(upload(ctx, s, "chan_map", chan_map, a.n_chan * sizeof(int32_t) + 4, &d_cmap));
+    NGC_TRY(upload(ctx, s, "chan_map", padded_map(chan_map, a.n_chan).data(),
+                   a.n_chan * sizeof(int32_t) + 4, &d_cmap));
     NGC_TRY(upload(ctx, s, "pol_map", pol_map, a.n_pol * sizeof(int32_t), &d_pmap));
     NGC_TRY(upload(ctx, s, "cgk", cgk, cgk_len * sizeof(double), &d_cgk));
     a.uv_scale = (const double *)d_scale;
@@ -1509,6 +1510,7 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     return tiled_run_eager(ctx, s, mode, precision, a, tg, bf, grid, sum_weight, accumulate,
                            nullptr);
 }
+
 }  // namespace ngc
 
 extern "C" int ngc_standard_grid(ngc_context *ctx, void *stream, int mode, int precision,
@@ -1619,7 +1621,8 @@ static int plan_create(ngc_context *ctx, hipStream_t s, int mode, int precision,
     uv_scale_host(freq, a.n_chan, delta_l, delta_m, a.n_u, a.n_v, scale);
     void *d_scale, *d_cmap, *d_pmap, *d_cgk;
     NGC_TRY(plan_upload(p, s, scale.data(), scale.size() * sizeof(double), &d_scale));
-    NGC_TRY(plan_upload(p, s, chan_map, a.n_chan * sizeof(int32_t) + 4, &d_cmap));
+    NGC_TRY(plan_upload(p, s, padded_map(chan_map, a.n_chan).data(), a.n_chan * sizeof(int32_t) + 4,
+                        &d_cmap));
     NGC_TRY(plan_upload(p, s, pol_map, a.n_pol * sizeof(int32_t), &d_pmap));
     NGC_TRY(plan_upload(p, s, cgk, cgk_len * sizeof(double), &d_cgk));
     a.uv_scale = (const double *)d_scale;

@@ -140,3 +140,24 @@ def test_mosaic_synthetic_layout():
     assert f.shape == (25, 2)
     assert np.allclose(f.mean(axis=0), pc, atol=1e-6)
     assert cfg.n_baseline == 1326 and cfg.n_vis == 1440 * 1326 * 32 * 2
+
+
+def test_uvw_and_field_shapes_are_checked():
+    """The kernels index uvw as float64 [T, B, 3] and FIELD_ID as int32 [T, B]
+    per row; the bindings refuse anything else before any device call."""
+    import torch
+    from cngi_prototype_amd import _native as nat
+    assert nat._uvw_rows(torch.zeros((4, 5, 3), dtype=torch.float64)) == (4, 5)
+    with pytest.raises(TypeError):
+        nat._uvw_rows(torch.zeros((4, 5, 3), dtype=torch.float32))
+    with pytest.raises(ValueError):
+        nat._uvw_rows(torch.zeros((4, 5, 2), dtype=torch.float64))
+    with pytest.raises(ValueError):
+        nat._uvw_rows(torch.zeros((20, 3), dtype=torch.float64))
+    with pytest.raises(ValueError):
+        nat._uvw_rows(torch.zeros((4, 5, 3), dtype=torch.float64), (4, 6))
+    nat._field_rows(torch.zeros((4, 5), dtype=torch.int32), 4, 5)
+    with pytest.raises(ValueError):
+        nat._field_rows(torch.zeros((4,), dtype=torch.int32), 4, 5)
+    with pytest.raises(TypeError):
+        nat._field_rows(torch.zeros((4, 5), dtype=torch.int64), 4, 5)
