@@ -60,43 +60,189 @@ def measured_traffic(precision, config):
         return None
 
 
-def cpu_baseline(cfg, data, npad, delta, cgk, n_chan_sample):
-    """The oracle (FP64 C restatement of the reference loop) timed on this
-    host on the first n_chan_sample channels of the workload: on every host
-    core, shaped as the reference's dask graph (one task per time chunk with a
-    private grid, then a pairwise tree-sum of the grids), and on one thread.
-    Returns (all-cores record, one-thread record)."""
+def cpu_baseline(cfg, data, npad, delta, cgk, n_chan_sample, n_chan_cube):
+    """The oracle (FP64 C restatement of the reference loop), compiled here
+    with -O3 -march=native (no fast-math, no contraction), timed on this host's
+    cores in three shapes:
+
+    - cube: the reference's cube-mode chunking by channel, one task per
+      channel of the first n_chan_cube channels, each gridding all its
+      samples into its own image planes (no private grids, no tree-sum);
+    - dask time chunks: one task per time chunk of the first n_chan_sample
+      channels with a private grid, then a pairwise tree-sum (the reference's
+      graph layer as it chunks continuum data);
+    - one thread, one chunk, on the first n_chan_sample channels.
+
+    Returns the three records; each states the CPU model and threads used."""
+    import concurrent.futures
+
     import oracle
-    oracle.build()
+    lib = oracle.use_native_build()
+    model, avail = oracle.cpu_model()
+    n_thr = int(os.environ.get("OMP_NUM_THREADS", avail))
     t0 = time.time()
+    n_c = max(n_chan_sample, n_chan_cube)
     uvw = data["uvw"].cpu().numpy()
-    vis = data["vis"][:, :, :n_chan_sample].cpu().numpy().astype(np.complex128)
-    w = data["imaging_weight"][:, :, :n_chan_sample].cpu().numpy().astype(np.float64)
-    flag = data["flag"][:, :, :n_chan_sample].cpu().numpy().astype(bool)
+    vis = data["vis"][:, :, :n_c].cpu().numpy().astype(np.complex128)
+    w = data["imaging_weight"][:, :, :n_c].cpu().numpy().astype(np.float64)
+    flag = data["flag"][:, :, :n_c].cpu().numpy().astype(bool)
     w[flag] = np.nan
-    fr = data["freq"][:n_chan_sample]
-    n_unflagged = int((~flag).sum())
-    log(f"cpu baseline: host copy {time.time() - t0:.1f}s")
+    fr = data["freq"][:n_c]
+    log(f"cpu baseline: {model}, {avail} cores available, {n_thr} threads; host copy "
+        f"{time.time() - t0:.1f}s; oracle built at {lib}")
+    host = f"{model}, {n_thr} of {avail} cores"
+
+    def rec(n_samples, dt, threads, how):
+        r = dict(value=round(n_samples / dt / 1e6, 3), unit="Mvis/s", cores=int(threads),
+                 kind="port", cpu=host,
+                 sample=f"FP64 C oracle (reference loop order, -O3 -march=native), {how}, image "
+                        f"mode ({n_samples} unflagged samples), {dt:.1f} s")
+        log(f"cpu baseline {r}")
+        return r
+
+    # cube: one task per channel (contiguous per-channel inputs, as chunked
+    # storage gives them), the GIL released inside each oracle call
+    chans = [(np.ascontiguousarray(vis[:, :, c:c + 1]), np.ascontiguousarray(w[:, :, c:c + 1]))
+             for c in range(n_chan_cube)]
+    n_cube = int((~flag[:, :, :n_chan_cube]).sum())
+    t0 = time.time()
+    with concurrent.futures.ThreadPoolExecutor(max_workers=n_thr) as ex:
+        outs = list(ex.map(lambda c: oracle.standard_grid(
+            chans[c][0], uvw, chans[c][1], fr[c:c + 1], cgk, [npad, npad], delta, 7, 100,
+            do_psf=False), range(n_chan_cube)))
+    dt = time.time() - t0
+    del outs, chans
+    cube = rec(n_cube, dt, min(n_thr, n_chan_cube),
+               f"{n_chan_cube} tasks of one channel each (cube-mode chunking, every task "
+               f"writing only its own planes) on {min(n_thr, n_chan_cube)} threads, first "
+               f"{n_chan_cube} of {cfg.n_chan} channels")
+    vis_s = np.ascontiguousarray(vis[:, :, :n_chan_sample])
+    w_s = np.ascontiguousarray(w[:, :, :n_chan_sample])
+    n_s = int((~flag[:, :, :n_chan_sample]).sum())
     out = []
     for threads in (0, 1):
-        n_thr = threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        chunk = uvw.shape[0] if threads == 1 else -(-uvw.shape[0] // n_thr)
+        nt = threads or n_thr
+        chunk = uvw.shape[0] if threads == 1 else -(-uvw.shape[0] // nt)
         t0 = time.time()
-        grid, sw, used = oracle.standard_grid_chunked(vis, uvw, w, fr, cgk, [npad, npad], delta,
-                                                      7, 100, do_psf=False, time_chunk=chunk,
-                                                      n_threads=threads)
+        grid, sw, used = oracle.standard_grid_chunked(vis_s, uvw, w_s, fr[:n_chan_sample], cgk,
+                                                      [npad, npad], delta, 7, 100,
+                                                      do_psf=False, time_chunk=chunk,
+                                                      n_threads=nt)
         dt = time.time() - t0
         del grid
-        how = ("1 thread" if threads == 1 else
-               f"{used} threads, {-(-uvw.shape[0] // chunk)} time chunks with private grids and "
-               f"a pairwise tree-sum (the reference's dask graph shape)")
-        out.append(dict(value=round(n_unflagged / dt / 1e6, 3), unit="Mvis/s", cores=int(used),
-                        kind="port",
-                        sample=f"FP64 C oracle (reference loop order), {how}, image mode, first "
-                               f"{n_chan_sample} of {cfg.n_chan} channels ({n_unflagged} "
-                               f"unflagged samples), {dt:.1f} s"))
-        log(f"cpu baseline {out[-1]}")
-    return out[0], out[1]
+        how = ("1 thread, one chunk" if threads == 1 else
+               f"{used} threads, {-(-uvw.shape[0] // chunk)} time chunks with private grids "
+               f"and a pairwise tree-sum (the reference's dask graph shape)")
+        out.append(rec(n_s, dt, used, f"{how}, first {n_chan_sample} of {cfg.n_chan} channels"))
+    return cube, out[0], out[1]
+
+
+def c1_leg(args):
+    """C1 at full size: 27 antennas in a 1 km disk, 300 x 10 s integrations at
+    1.4 GHz, 16 channels x 2 pols (3.37M samples), 512^2 at 2", natural
+    weights, make_psf + make_image.  Times the PSF and image gridding calls
+    (inputs resident in HBM) in the chosen form and the FP64 drop-in, states
+    the dirty-image error per plane, and times make_psf + make_image end to
+    end through the API (FFT, correction, PSF fit) on device-resident inputs."""
+    import torch
+    from cngi_prototype_amd import _native as nat
+    from cngi_prototype_amd import synthetic as syn
+    from cngi_prototype_amd.imaging import make_image, make_psf
+    from cngi_prototype_amd.imaging._imaging_utils._gridding_convolutional_kernels import \
+        cropped_correction_factors, prolate_spheroidal_kernel_1d
+    from cngi_prototype_amd.xds import Dataset
+
+    cfg = syn.CONFIGS["C1"]
+    N = cfg.image_size
+    npad = int(1.2 * N)
+    cell = cfg.cell_arcsec * syn.ARCSEC
+    delta = np.array([-cell, cell])
+    cgk = prolate_spheroidal_kernel_1d(100, 7)
+    C, P = cfg.n_chan, cfg.n_pol
+    d = syn.device_dataset(cfg)
+    torch.cuda.synchronize()
+    uvw, vis, w, fr = d["uvw"], d["vis"], d["weight"], d["freq"]
+    n_vis = int(cfg.n_vis)
+    n_rows = cfg.n_time * cfg.n_baseline
+    corr = cropped_correction_factors([npad, npad], [N, N])
+
+    def call(prec, mode):
+        return nat.standard_grid(uvw, w, fr, np.arange(C), np.arange(P), cgk, C, [npad, npad],
+                                 delta, 7, 100, mode, prec,
+                                 vis=vis if mode == nat.GRID_IMAGE else None)
+
+    times, kernel_ms, imgs = {}, {}, {}
+    for prec in (args.precision, "fp64"):
+        for mode, name in ((nat.GRID_PSF, "psf"), (nat.GRID_IMAGE, "image")):
+            for _ in range(max(1, args.warmup)):
+                call(prec, mode)
+            torch.cuda.synchronize()
+            nat.profile_enable(True)
+            kms = []
+            t0 = time.time()
+            for _ in range(args.steps):
+                call(prec, mode)
+                kms.append(sum(ms for n_, ms in nat.profile_read()
+                               if n_ in ("std_grid_tiled", "std_grid_plain")))
+            torch.cuda.synchronize()
+            times[(prec, name)] = (time.time() - t0) / args.steps * 1e3
+            kernel_ms[(prec, name)] = float(np.mean(kms))
+            nat.profile_enable(False)
+            g, sw = call(prec, mode)
+            imgs[(prec, name)] = nat.grid_to_image(g, [N, N], sum_weight=sw, corr=corr)
+            del g
+    errs = {}
+    for name in ("psf", "image"):
+        a, b = imgs[(args.precision, name)], imgs[("fp64", name)]
+        dd = (a - b).reshape(C * P, -1)
+        rr = b.reshape(C * P, -1)
+        errs[name] = float((torch.linalg.norm(dd, dim=1) / torch.linalg.norm(rr, dim=1)).max())
+    del imgs
+    # end to end through the API on device-resident inputs
+    dims4 = ["time", "baseline", "chan", "pol"]
+    xds = Dataset(data_vars={"DATA": (dims4, vis), "UVW": (["time", "baseline", "uvw_index"], uvw),
+                             "DATA_WEIGHT": (dims4, w),
+                             "chan_width": (["chan"], np.full(C, cfg.df))},
+                  coords={"chan": (["chan"], fr), "pol": (["pol"], np.array([9, 12]))},
+                  attrs={"data_groups": [{"0": {"id": "0", "data": "DATA", "uvw": "UVW",
+                                                "weight": "DATA_WEIGHT",
+                                                "imaging_weight": "DATA_WEIGHT"}}]})
+    mxds = Dataset(attrs={"xds0": xds})
+    gp = {"image_size": [N, N], "cell_size": [cfg.cell_arcsec] * 2, "fft_padding": 1.2,
+          "chan_mode": "cube", "phase_center": syn.phase_center(cfg),
+          "precision": args.precision}
+    make_image(mxds, make_psf(mxds, Dataset(), gp, {"xds": "xds0"}, {}), gp, {"xds": "xds0"}, {})
+    torch.cuda.synchronize()
+    t0 = time.time()
+    img = make_psf(mxds, Dataset(), gp, {"xds": "xds0"}, {})
+    img = make_image(mxds, img, gp, {"xds": "xds0"}, {})
+    torch.cuda.synchronize()
+    t_api = (time.time() - t0) * 1e3
+    del img, mxds, xds
+    ck = args.precision
+    abytes = algorithmic_bytes(n_vis, n_rows, C * P, npad)
+    t_k = kernel_ms[(ck, "image")] / 1e3
+    pair = times[(ck, "psf")] + times[(ck, "image")]
+    pair64 = times[("fp64", "psf")] + times[("fp64", "image")]
+    return dict(workload=f"C1: {cfg.n_ant} antennas x {cfg.n_time} integrations, {C} chan x {P} "
+                         f"pol at 1.4 GHz, {N}^2 at {cfg.cell_arcsec}\" (padded {npad}^2), "
+                         f"natural weights, make_psf + make_image", samples=n_vis,
+                value=round(n_vis / (times[(ck, "image")] / 1e3) / 1e6, 1), unit="Mvis/s",
+                precision=ck,
+                image_ms=round(times[(ck, "image")], 3), psf_ms=round(times[(ck, "psf")], 3),
+                psf_plus_image_mvis_s=round(n_vis / (pair / 1e3) / 1e6, 1),
+                plain_fp64_image_ms=round(times[("fp64", "image")], 3),
+                plain_fp64_psf_ms=round(times[("fp64", "psf")], 3),
+                plain_fp64_psf_plus_image_mvis_s=round(n_vis / (pair64 / 1e3) / 1e6, 1),
+                speedup_vs_plain_fp64=round(pair64 / pair, 2),
+                error_vs_fp64=dict(image_rel_rms_max=errs["image"], psf_rel_rms_max=errs["psf"],
+                                   planes=C * P),
+                roofline=dict(bound="hbm", achieved=round(abytes / t_k / 1e9, 1),
+                              peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=round(abytes / t_k / 1e9 / HBM_PEAK_GBS, 4),
+                              algorithmic_bytes=int(abytes),
+                              kernel_ms=round(kernel_ms[(ck, "image")], 4)),
+                make_psf_plus_make_image_api_ms=round(t_api, 1))
 
 
 def mosaic_leg(args):
@@ -270,6 +416,7 @@ def mosaic_leg(args):
     return dict(workload=f"C4: 25-pointing mosaic, 12x7 m + 40x12 m, {cfg.n_time} x 10 s, "
                          f"{C} chan x {P} pol at 230 GHz, {N}^2 at {cfg.cell_arcsec}\", "
                          f"GCF os 20, supports {sorted(set(sup[:, 0, 0, 0].tolist()))}",
+                samples=int(n_vis),
                 value=round(n_vis / (ms_step / 1e3) / 1e6, 1), unit="Mvis/s",
                 ms_per_step=round(ms_step, 3), precision=args.precision,
                 plain_fp64_mvis_s=round(n_vis / (ms_plain / 1e3) / 1e6, 2),
@@ -358,6 +505,37 @@ def batched_leg(args):
     del gc, swc
     log(f"batched: plan {ms_plan:.2f} ms, eager {ms_eager:.2f} ms, per-channel calls "
         f"{ms_per_chan:.1f} ms (incl. slicing)")
+    # the plan's gridding kernel alone: the same call eager, in the tuned tile
+    # shape, profiled per phase (a graph replay has no phase marks)
+    import re
+    m_tu = re.search(r"tile (\d+)x32", info)
+    m_ln = re.search(r"(\d+) channel lanes", info)
+    if m_tu and m_ln:
+        nat.set_tile_rows(int(m_tu.group(1)))
+        nat.set_bin_lanes(int(m_ln.group(1)))
+    g_k = torch.empty_like(grid)
+    sw_k = torch.empty_like(sw)
+    nat.standard_grid(*args_, vis=vis, flag=flag, grid=g_k, sum_weight=sw_k)
+    torch.cuda.synchronize()
+    nat.profile_enable(True)
+    kph = {}
+    for _ in range(args.steps):
+        nat.standard_grid(*args_, vis=vis, flag=flag, grid=g_k, sum_weight=sw_k)
+        for name, ms in nat.profile_read():
+            kph[name] = kph.get(name, 0.0) + ms / args.steps
+    nat.profile_enable(False)
+    nat.set_tile_rows(16)
+    nat.set_bin_lanes(2)
+    del g_k, sw_k
+    abytes = algorithmic_bytes(cfg.n_vis, cfg.n_time * cfg.n_baseline, C * P, npad)
+    t_k = kph.get("std_grid_tiled", float("nan")) / 1e3
+    roofline = dict(bound="hbm", achieved=round(abytes / t_k / 1e9, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(abytes / t_k / 1e9 / HBM_PEAK_GBS, 4),
+                    kernel="std_grid_tiled_kernel (tuned tile shape)",
+                    algorithmic_bytes=int(abytes), kernel_ms=round(t_k * 1e3, 3),
+                    whole_call_gbs=round(abytes / (ms_plan / 1e3) / 1e9, 1),
+                    phases_ms={k: round(v, 3) for k, v in kph.items()})
+    log(f"batched roofline {roofline}")
     # error against the plain FP64 drop-in on the same inputs, per plane
     g64, sw64 = nat.standard_grid(uvw, w, fr, np.arange(C), np.arange(P), cgk, C, [npad, npad],
                                   delta, 7, 100, nat.GRID_IMAGE, "fp64", vis=vis, flag=flag)
@@ -377,19 +555,21 @@ def batched_leg(args):
                 value=round(n_unflagged / (ms_plan / 1e3) / 1e6, 1), unit="Mvis/s",
                 ms_per_step=round(ms_plan, 3), precision=args.precision, plan=info,
                 plan_create_s=round(t_plan, 2), eager_call_ms=round(ms_eager, 3),
+                roofline=roofline,
                 per_channel_calls_ms=round(ms_per_chan, 1),
                 error_vs_fp64=dict(rel_rms_max=float(rel.max()),
                                    rel_rms_median=float(np.median(rel)), planes=int(n_planes)))
 
 
 def streamed_leg(args):
-    """C5 on one GPU's share (8 of its 64 channels, as one of 8 GPUs in cube mode
-    would grid): 64 x 12 m antennas, 3875 integrations over 8 h at 100 GHz, 2 pols
-    (125M visibilities, 1.7 GB), 49 pointings, 4096^2 at 0.05", GCF oversampling 20.
-    Inputs in pinned host memory, streamed in 256 MB chunks and gridded in the
-    chosen form (FP32-Kahan tiled aperture gridder) with per-channel kernels of the
-    learned surrogate; compared with the plain FP64 GPU gridder (inputs resident)
-    with the same kernels."""
+    """C5 in full on one GPU: 64 x 12 m antennas, 3875 integrations over 8 h at
+    100 GHz, 64 channels x 2 pols (1.0e9 visibilities, 13.2 GB with weights,
+    flags and uvw), 49 pointings, 4096^2 at 0.05" (4915^2 grid, 24.7 GB in
+    complex64), GCF oversampling 20.  Inputs in pinned host memory, streamed in
+    256 MB chunks of whole integrations into the resident grid in the chosen
+    form (FP32-Kahan tiled aperture gridder) with per-channel kernels of the
+    learned surrogate; the first n_chan_check channels are compared with the
+    plain FP64 GPU gridder (inputs resident) with the same kernels."""
     import torch
     from cngi_prototype_amd import _native as nat
     from cngi_prototype_amd import synthetic as syn
@@ -400,8 +580,9 @@ def streamed_leg(args):
         stream_aperture_grid
     from cngi_prototype_amd.xds import Dataset
 
-    full = syn.CONFIGS["C5"]
-    cfg = syn.scaled(full, n_chan=8)
+    cfg = syn.CONFIGS["C5"] if args.c5_chans >= 64 else syn.scaled(syn.CONFIGS["C5"],
+                                                                     n_chan=args.c5_chans)
+    n_check = min(8, cfg.n_chan)
     N = cfg.image_size
     npad = int(1.2 * N)
     t0 = time.time()
@@ -475,27 +656,31 @@ def streamed_leg(args):
         runs.append(st)
     st = min(runs[max(1, args.warmup):], key=lambda r: r["wall_s"])
     log(f"streamed chosen form: {st}")
-    # plain FP64 on resident inputs, same kernels
-    dv = {k: v.cuda() for k, v in host.items()}
+    # plain FP64 on resident inputs, same kernels, on the first n_check channels
+    dv = {k: (v[:, :, :n_check] if v.dim() == 4 else v).contiguous().cuda()
+          for k, v in host.items()}
     torch.cuda.synchronize()
     t0 = time.time()
-    g64, sw64 = nat.aperture_grid(dv["uvw"], dv["weight"], dv["field"], g["field_id"], fr,
-                                  np.arange(C), np.arange(P), g["cf_baseline_map"],
-                                  g["cf_chan_map"], g["cf_pol_map"], g["kernel"], sup,
-                                  g["phase_gradient"], C, [npad, npad], delta, os_,
-                                  nat.GRID_IMAGE, "fp64", vis=dv["vis"])
+    g64, sw64 = nat.aperture_grid(dv["uvw"], dv["weight"], dv["field"], g["field_id"],
+                                  fr[:n_check], np.arange(n_check), np.arange(P),
+                                  g["cf_baseline_map"], g["cf_chan_map"][:n_check],
+                                  g["cf_pol_map"], g["kernel"], sup, g["phase_gradient"],
+                                  n_check, [npad, npad], delta, os_, nat.GRID_IMAGE, "fp64",
+                                  vis=dv["vis"])
     torch.cuda.synchronize()
     t64 = time.time() - t0
+    n_vis_check = int(dv["weight"].numel())
     del dv
     img64 = nat.grid_to_image(g64, [N, N], sum_weight=sw64)
     del g64
-    img = nat.grid_to_image(grid, [N, N], sum_weight=sw)
-    n_planes = C * P
+    img = nat.grid_to_image(grid[:n_check], [N, N], sum_weight=sw[:n_check])
+    n_planes = n_check * P
     dd = (img - img64).reshape(n_planes, -1)
     rr = img64.reshape(n_planes, -1)
     rel = (torch.linalg.norm(dd, dim=1) / torch.linalg.norm(rr, dim=1)).cpu().numpy()
     del img, img64, grid
-    return dict(workload=f"C5 share: 8 of 64 channels (one of 8 GPUs), {cfg.n_ant} x 12 m, "
+    t64_per_vis = t64 / n_vis_check
+    return dict(workload=f"C5: {cfg.n_chan} of 64 channels, {cfg.n_ant} x 12 m, "
                          f"{cfg.n_time} integrations, 2 pol, 49 pointings, {N}^2 at "
                          f"{cfg.cell_arcsec}\", GCF os 20, supports "
                          f"{sorted(set(sup[..., 0].ravel().tolist()))}",
@@ -503,13 +688,17 @@ def streamed_leg(args):
                 value=round(cfg.n_vis / st["wall_s"] / 1e6, 1), unit="Mvis/s",
                 note="host-resident inputs, PCIe-inclusive (streamed in 256 MB chunks)",
                 stream=st, plain_fp64_resident_s=round(t64, 2),
-                speedup_vs_plain_fp64=round(t64 / st["wall_s"], 1), precision=args.precision,
+                plain_fp64_channels=n_check,
+                plain_fp64_mvis_s=round(n_vis_check / t64 / 1e6, 1),
+                speedup_vs_plain_fp64=round(t64_per_vis * cfg.n_vis / st["wall_s"], 1),
+                precision=args.precision,
                 surrogate=dict(max_tap_err=rep["max_err_surrogate"],
                                max_tap_err_bin=rep["max_err_bin"],
                                fallbacks=len(rep["fallback"]), kernels=C,
                                train_s=tinfo["train_s"]),
                 error_vs_fp64=dict(rel_rms_max=float(rel.max()),
-                                   rel_rms_median=float(np.median(rel)), planes=int(n_planes)))
+                                   rel_rms_median=float(np.median(rel)), planes=int(n_planes),
+                                   channels=n_check))
 
 
 def predict_leg(args):
@@ -596,6 +785,18 @@ def predict_leg(args):
                 rel_err_vs_fp64_4chan=rel)
 
 
+def combined(n_std, ms_std, mosaic):
+    """The metric's combined figure: the standard (C2) and mosaic (C4) steps'
+    samples over their summed step times."""
+    if not mosaic or "ms_per_step" not in mosaic:
+        return None
+    n_mos = mosaic["samples"]
+    ms = ms_std + mosaic["ms_per_step"]
+    return dict(value=round((n_std + n_mos) / (ms / 1e3) / 1e6, 1), unit="Mvis/s",
+                samples=int(n_std + n_mos), ms=round(ms, 3),
+                parts="C2 standard step (unflagged samples) + C4 mosaic step")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -619,7 +820,18 @@ def main():
                     help="skip the C2-shape visibility prediction leg (reported under 'predict')")
     ap.add_argument("--predict-only", action="store_true",
                     help="run only the prediction leg and print its record")
-    ap.add_argument("--cpu-chans", type=int, default=6)
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the C1 leg (reported under 'c1')")
+    ap.add_argument("--c1-only", action="store_true",
+                    help="run only the C1 leg and print its record")
+    ap.add_argument("--c5-chans", type=int, default=64,
+                    help="channels of the C5 streamed leg (64: the whole cube)")
+    ap.add_argument("--no-plain-fp64", action="store_true",
+                    help="skip the one call of the plain FP64 drop-in on the main workload")
+    ap.add_argument("--cpu-chans", type=int, default=6,
+                    help="channels of the time-chunked and one-thread CPU baseline samples")
+    ap.add_argument("--cpu-cube-chans", type=int, default=16,
+                    help="channels (one task each) of the cube-mode CPU baseline sample")
     ap.add_argument("--config", default="C2")
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16, 32],
@@ -664,6 +876,9 @@ def main():
         return
     if args.predict_only:
         print(json.dumps({"predict": predict_leg(args)}), file=out, flush=True)
+        return
+    if args.c1_only:
+        print(json.dumps({"c1": c1_leg(args)}), file=out, flush=True)
         return
     if args.tile_rows:
         nat.set_tile_rows(args.tile_rows)
@@ -775,10 +990,62 @@ def main():
         torch.cuda.synchronize()
         log(f"error check {time.time() - te:.1f}s: {err}")
 
-    cpu = cpu1 = None
+    # the imaging transform of the gridded cube (make_image.py:116-130): ifftshift,
+    # rocFFT inverse, crop and correction, profiled per phase; the transform's
+    # algorithmic traffic is two passes over the complex64 grid, each reading
+    # and writing it once
+    corr = cropped_correction_factors([npad, npad], [N, N])
+    nat.grid_to_image(grid, [N, N], sum_weight=sw, corr=corr)
+    torch.cuda.synchronize()
+    nat.profile_enable(True)
+    n_fft = max(1, min(args.steps, 3))
+    fft_ph = {}
+    t0 = time.time()
+    for _ in range(n_fft):
+        img_t = nat.grid_to_image(grid, [N, N], sum_weight=sw, corr=corr)
+        for name, ms in nat.profile_read():
+            fft_ph[name] = fft_ph.get(name, 0.0) + ms / n_fft
+        del img_t
+    torch.cuda.synchronize()
+    ms_g2i = (time.time() - t0) / n_fft * 1e3
+    nat.profile_enable(False)
+    grid_bytes = n_planes * npad * npad * 8
+    t_fft = fft_ph.get("rocfft_inverse", float("nan")) / 1e3
+    fft = dict(planes=int(n_planes), length=f"{npad} x {npad} (3^3 7 13 per axis)",
+               grid_to_image_ms=round(ms_g2i, 3),
+               phases_ms={k: round(v, 3) for k, v in fft_ph.items()},
+               fft_ms=round(t_fft * 1e3, 3),
+               fft_roofline=dict(bound="hbm", achieved=round(4 * grid_bytes / t_fft / 1e9, 1),
+                                 peak=HBM_PEAK_GBS, unit="GB/s",
+                                 frac=round(4 * grid_bytes / t_fft / 1e9 / HBM_PEAK_GBS, 4),
+                                 algorithmic_bytes=int(4 * grid_bytes)),
+               make_image_ms=round(ms_step + ms_g2i, 3))
+    log(f"imaging transform {fft}")
+
+    # the plain FP64 drop-in (reference loop per sample, FP64 atomics into a
+    # complex128 grid) on the same inputs, once
+    plain = None
+    if not args.no_plain_fp64:
+        del grid
+        torch.cuda.empty_cache()
+        torch.cuda.synchronize()
+        t0 = time.time()
+        g64, sw64 = nat.standard_grid(uvw, iw, fr, chan_map, pol_map, cgk, C, [npad, npad],
+                                      delta, 7, 100, nat.GRID_IMAGE, "fp64", vis=vis, flag=flag)
+        torch.cuda.synchronize()
+        ms_plain = (time.time() - t0) * 1e3
+        del g64, sw64
+        torch.cuda.empty_cache()
+        plain = dict(ms=round(ms_plain, 1), mvis_s=round(n_unflagged / ms_plain * 1e-3, 1),
+                     speedup_of_chosen_form=round(ms_plain / ms_step, 1))
+        log(f"plain fp64 drop-in at {cfg.name}: {plain}")
+        grid = torch.empty((C, P, npad, npad), dtype=torch.complex64, device="cuda")
+
+    cpu = cpu_dask = cpu1 = None
     if not args.no_cpu_baseline:
         try:
-            cpu, cpu1 = cpu_baseline(cfg, data, npad, delta, cgk, args.cpu_chans)
+            cpu, cpu_dask, cpu1 = cpu_baseline(cfg, data, npad, delta, cgk, args.cpu_chans,
+                                               args.cpu_cube_chans)
         except Exception as e:  # report, never fake
             log(f"cpu baseline failed: {e!r}")
 
@@ -812,6 +1079,16 @@ def main():
         except Exception as e:  # report, never fake
             log(f"streamed leg failed: {e!r}")
             streamed = {"error": repr(e)}
+
+    c1 = None
+    if not args.no_c1:
+        torch.cuda.empty_cache()
+        try:
+            c1 = c1_leg(args)
+            log(f"c1 {c1}")
+        except Exception as e:  # report, never fake
+            log(f"c1 leg failed: {e!r}")
+            c1 = {"error": repr(e)}
 
     predict = None
     if not args.no_predict:
@@ -849,12 +1126,17 @@ def main():
                            f"lane-owned residue classes, FP32 register runs added to an "
                            f"LDS subgrid (FP64 cells for fp32_kahan))"},
         "roofline": roofline,
+        "combined_std_mosaic": combined(n_unflagged, ms_step, mosaic),
+        "imaging_transform": fft,
+        "plain_fp64": plain,
         "cpu_baseline": cpu,
+        "cpu_baseline_dask_time_chunks": cpu_dask,
         "cpu_baseline_1thread": cpu1,
         "error_vs_fp64": err,
         "phases_ms": phase_ms,
         "briggs_ms": round(t_briggs * 1e3, 2),
         "tile_rows": args.tile_rows or 16,
+        "c1": c1,
         "mosaic": mosaic,
         "batched": batched,
         "streamed": streamed,
