@@ -35,7 +35,8 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step",
            "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags", "ngc_set_tile_waves",
            "ngc_model_to_grid", "ngc_standard_degrid", "ngc_aperture_degrid", "ngc_mlp_sample_rows",
-           "ngc_adam_schedule", "ngc_adam_step_sched", "ngc_set_bin_lanes")
+           "ngc_adam_schedule", "ngc_adam_step_sched", "ngc_set_bin_lanes",
+           "ngc_standard_grid_noalloc")
 
 
 class NativeError(RuntimeError):
@@ -59,6 +60,10 @@ _PROTOS = {
     "ngc_standard_grid": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp, _vp,
                                  _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
                                  _i64, _i64, _d, _d, _i64, _i64, _i32, _vp, _vp]),
+    "ngc_standard_grid_noalloc": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp, _i32, _vp,
+                                         _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
+                                         _i64, _i64, _i64, _i64, _d, _d, _i64, _i64, _i32, _vp,
+                                         _vp]),
     "ngc_briggs_factors": (_i32, [_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64, _i32, _d, _vp]),
     "ngc_imaging_weight_degrid": (_i32, [_vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
@@ -253,8 +258,12 @@ def _field_rows(field, T, B):
 # ---------------------------------------------------------------------------
 def standard_grid(uvw, weight, freq_chan, chan_map, pol_map, cgk_1d, n_imag_chan, n_uv,
                   delta_lm, support, oversampling, mode, precision, vis=None, flag=None,
-                  grid=None, sum_weight=None, accumulate=False, n_imag_pol=None):
-    """ngc_standard_grid on device tensors.  Returns (grid, sum_weight)."""
+                  grid=None, sum_weight=None, accumulate=False, n_imag_pol=None,
+                  noalloc=False):
+    """ngc_standard_grid on device tensors.  Returns (grid, sum_weight).  With
+    noalloc the call goes to ngc_standard_grid_noalloc, which allocates no
+    device memory (pass grid and sum_weight to keep the whole call
+    allocation-free)."""
     import torch
     ctx = context()
     _dev(uvw, "uvw")
@@ -288,13 +297,14 @@ def standard_grid(uvw, weight, freq_chan, chan_map, pol_map, cgk_1d, n_imag_chan
     pm, pmp = _host(pol_map, np.int32)
     cg, cgp = _host(cgk_1d, np.float64)
     lib = load_library()
-    rc = lib.ngc_standard_grid(
+    name = "ngc_standard_grid_noalloc" if noalloc else "ngc_standard_grid"
+    rc = getattr(lib, name)(
         ctx.handle, _stream(), int(mode), prec, _ptr(uvw), _ptr(vis),
         _dt_vis(vis) if vis is not None else C64, _ptr(weight), _dt_w(weight), _ptr(flag),
         frp, cmp_, pmp, cgp, len(cg), n_time, n_baseline, n_chan, n_pol, int(n_imag_chan),
         n_imag_pol, n_u, n_v, float(delta_lm[0]), float(delta_lm[1]), int(support),
         int(oversampling), int(bool(accumulate)), _ptr(grid), _ptr(sum_weight))
-    _check("ngc_standard_grid", rc)
+    _check(name, rc)
     return grid, sum_weight
 
 

@@ -173,6 +173,7 @@ int ngc_context_create(int device, ngc_context **ctx) {
     NGC_HIP(hipGetDeviceCount(&n));
     NGC_CHECK(device >= 0 && device < n, NGC_ERR_ARG, "device index out of range");
     NGC_HIP(hipSetDevice(device));
+    NGC_TRY(ngc::noalloc_scratch_setup(device));
     ngc_context *c = new ngc_context();
     c->device = device;
     int n_cu = 0;

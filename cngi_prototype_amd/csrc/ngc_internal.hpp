@@ -117,6 +117,11 @@ inline std::vector<int32_t> padded_map(const int32_t *map, int64_t n) {
     return v;
 }
 
+// Device address of the allocation-free gridder's scratch array on `device`
+// (which must be current), looked up once per device; called by
+// ngc_context_create.
+int noalloc_scratch_setup(int device);
+
 int fft_destroy_plans(ngc_context *ctx);
 int fft_execute(ngc_context *ctx, hipStream_t s, void *buf, int64_t n_u, int64_t n_v,
                 int64_t batch, bool dbl, bool forward);

@@ -1511,6 +1511,213 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
                            nullptr);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free standard gridding (ngc_standard_grid_noalloc).  Every device
+// buffer a call needs beyond its inputs and outputs -- the uploaded parameter
+// tables, the FP64 tap-sum table, the bin counts, offsets, order and queues,
+// and the bin records -- is carved out of `scratch`, one file-scope device
+// array of NGC_NOALLOC_SCRATCH_BYTES that each GPU holds from the library's
+// load until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address on each GPU comes from
+// hipGetSymbolAddress, once per device at context creation
+// (noalloc_scratch_setup, with that device current), and only that address
+// is ever passed to kernels or copies.  Calls on one GPU share the array and
+// find the previous call's contents in it; everything that must start at zero
+// is cleared by fill kernels in the call itself.  Calls that overlap on one
+// GPU (different streams or contexts without ordering between them) race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[NGC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxScratchDevices] = {};
+
+// Bump allocation of 256-byte aligned regions of one call's scratch.
+struct ScratchCarve {
+    char *base;
+    size_t cap, used;
+    void *take(size_t bytes) {
+        const size_t off = (used + 255) & ~size_t(255);
+        if (bytes > cap || off > cap - bytes) return nullptr;
+        used = off + bytes;
+        return base + off;
+    }
+    size_t left() const {
+        const size_t off = (used + 255) & ~size_t(255);
+        return off < cap ? cap - off : 0;
+    }
+};
+}  // namespace
+
+int noalloc_scratch_setup(int device) {
+    NGC_CHECK(device >= 0 && device < kMaxScratchDevices, NGC_ERR_UNSUPPORTED,
+              "no-alloc scratch: device index above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return NGC_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return NGC_ERR_HIP;
+    }
+    g_scratch_dev[device] = p;
+    return NGC_OK;
+}
+
+static void *noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+
+#define NGC_TAKE(ptr, carve, bytes, what)                                                   \
+    do {                                                                                     \
+        (ptr) = (carve).take(bytes);                                                         \
+        if (!(ptr)) {                                                                        \
+            set_error(std::string("ngc_standard_grid_noalloc: ") + (what) + " needs " +      \
+                      std::to_string((size_t)(bytes)) + " bytes beyond the " +              \
+                      std::to_string((carve).used) + " already taken of the " +             \
+                      std::to_string((size_t)NGC_NOALLOC_SCRATCH_BYTES) +                   \
+                      "-byte scratch; split the call or use ngc_standard_grid");             \
+            return NGC_ERR_ARG;                                                              \
+        }                                                                                    \
+    } while (0)
+
+static int standard_grid_noalloc_impl(ngc_context *ctx, hipStream_t s, int mode, int precision,
+                                      StdArgs a, const double *freq, const int32_t *chan_map,
+                                      const int32_t *pol_map, const double *cgk,
+                                      int64_t cgk_len, int accumulate, void *grid,
+                                      double *sum_weight, double delta_l, double delta_m) {
+    bool pol_identity = false;
+    NGC_TRY(std_validate(mode, precision, a, freq, chan_map, pol_map, cgk, cgk_len, grid,
+                         sum_weight, &pol_identity));
+    char *base = (char *)noalloc_scratch(ctx->device);
+    NGC_CHECK(base, NGC_ERR_HIP, "no-alloc scratch is not set up on this device");
+    ScratchCarve cv{base, (size_t)NGC_NOALLOC_SCRATCH_BYTES, 0};
+
+    // parameter tables (the caller's host arrays are read for their exact sizes)
+    std::vector<double> scale;
+    uv_scale_host(freq, a.n_chan, delta_l, delta_m, a.n_u, a.n_v, scale);
+    const std::vector<int32_t> cmap = padded_map(chan_map, a.n_chan);
+    void *d_scale, *d_cmap, *d_pmap, *d_cgk;
+    NGC_TAKE(d_scale, cv, scale.size() * sizeof(double), "the uv scale table");
+    NGC_TAKE(d_cmap, cv, cmap.size() * sizeof(int32_t), "chan_map");
+    NGC_TAKE(d_pmap, cv, (size_t)a.n_pol * sizeof(int32_t), "pol_map");
+    NGC_TAKE(d_cgk, cv, (size_t)cgk_len * sizeof(double), "cgk_1d");
+    const bool tiled = precision != NGC_PREC_FP64 && mode != NGC_GRID_DENSITY;
+    const bool single = (precision == NGC_PREC_FP32_KAHAN || precision == NGC_PREC_FP32);
+    NGC_CHECK(tiled || !single, NGC_ERR_UNSUPPORTED,
+              "density gridding is FP64 only (precision NGC_PREC_FP64 or FP64_TILED)");
+    TiledGeom tg{};
+    TiledBufs bf{};
+    std::vector<double> ntab;
+    void *d_ntab = nullptr;
+    if (tiled) {
+        NGC_TRY(tiled_supported(a, pol_identity, cgk_len));
+        TiledCfg cfg;
+        cfg.tu = ctx->tile_rows;
+        cfg.ch_lanes = ctx->bin_chan_lanes;
+        cfg.waves = ctx->tile_waves;
+        tg = tiled_geom(a, cfg);
+        NGC_CHECK(tg.n_waves < (int64_t(1) << 32), NGC_ERR_UNSUPPORTED,
+                  "too many (baseline, time block, channel block) slots for one call");
+        bf.half = (int)(a.os / 2 + 1);
+        norm_table(cgk, cgk_len, a.support, a.os, bf.half, ntab);
+        NGC_TAKE(d_ntab, cv, ntab.size() * sizeof(double), "the tap-sum table");
+        void *p;
+        NGC_TAKE(p, cv, (size_t)(tg.n_bins + 1) * 8, "the bin counts");
+        bf.counts = (unsigned long long *)p;
+        NGC_TAKE(p, cv, (size_t)(tg.n_bins + 1) * 8, "the bin offsets");
+        bf.offsets = (unsigned long long *)p;
+        NGC_TAKE(p, cv, (size_t)tg.scan_nb * 8, "the scan partials");
+        bf.block_sums = (unsigned long long *)p;
+        NGC_TAKE(p, cv, (size_t)tg.n_bins * 4 + 4, "the bin order");
+        bf.order = (unsigned int *)p;
+        NGC_TAKE(p, cv, 2 * kBuckets * 4, "the bin-size histogram");
+        bf.hist = (unsigned int *)p;
+        NGC_TAKE(p, cv, 4 * sizeof(int), "the work-queue counters");
+        bf.flags = (int *)p;
+        bf.ntab = (const double *)d_ntab;
+        bf.cgk = (const double *)d_cgk;
+        bf.cgk_len = cgk_len;
+    }
+    // every size is known to fit before anything is enqueued (the bin records
+    // are checked after the counting pass, which writes only scratch)
+    NGC_HIP(hipMemcpyAsync(d_scale, scale.data(), scale.size() * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+    NGC_HIP(hipMemcpyAsync(d_cmap, cmap.data(), cmap.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, s));
+    NGC_HIP(hipMemcpyAsync(d_pmap, pol_map, (size_t)a.n_pol * sizeof(int32_t),
+                           hipMemcpyHostToDevice, s));
+    NGC_HIP(hipMemcpyAsync(d_cgk, cgk, (size_t)cgk_len * sizeof(double), hipMemcpyHostToDevice,
+                           s));
+    if (tiled)
+        NGC_HIP(hipMemcpyAsync(d_ntab, ntab.data(), ntab.size() * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+    a.uv_scale = (const double *)d_scale;
+    a.chan_map = (const int *)d_cmap;
+    a.pol_map = (const int *)d_pmap;
+    const int64_t n_sw = a.n_imag_chan * a.n_imag_pol;
+    profile_new_call(ctx, s);
+
+    if (!tiled) {
+        const int64_t nc = (mode == NGC_GRID_IMAGE) ? 2 : 1;
+        const int64_t n_rc = a.n_time * a.n_baseline * a.n_chan;
+        if (!accumulate) {
+            const int64_t n_g = a.n_imag_chan * a.n_imag_pol * a.n_u * a.n_v * nc;
+            fill_kernel<double><<<fill_blocks(n_g), 256, 0, s>>>((double *)grid, n_g, 0.0);
+            fill_kernel<double><<<fill_blocks(n_sw), 256, 0, s>>>(sum_weight, n_sw, 0.0);
+            NGC_HIP(hipGetLastError());
+        }
+        if (n_rc == 0) return NGC_OK;
+        const int sw_lds = n_sw <= 4096 ? 1 : 0;
+        const size_t shmem = sw_lds ? n_sw * sizeof(double) : 0;
+        const int blocks = launch_blocks(n_rc);
+        double *g = (double *)grid;
+        if (mode == NGC_GRID_IMAGE)
+            std_grid_plain_kernel<NGC_GRID_IMAGE><<<blocks, kBlock, shmem, s>>>(
+                a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
+        else if (mode == NGC_GRID_PSF)
+            std_grid_plain_kernel<NGC_GRID_PSF><<<blocks, kBlock, shmem, s>>>(
+                a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
+        else
+            std_grid_plain_kernel<NGC_GRID_DENSITY><<<blocks, kBlock, shmem, s>>>(
+                a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
+        NGC_HIP(hipGetLastError());
+        return NGC_OK;
+    }
+
+    return tiled_dispatch(mode, a.n_pol, precision, tg.cfg.tu, tg.cfg.waves,
+                          [&](auto m, auto np, auto acc, auto k, auto tu, auto w) -> int {
+        constexpr int MODE = decltype(m)::value;
+        constexpr int NP = decltype(np)::value;
+        using ACC = typename decltype(acc)::type;
+        constexpr bool KAHAN = decltype(k)::value;
+        constexpr int TU = decltype(tu)::value;
+        constexpr int W = decltype(w)::value;
+        // counting pass with accumulate = 1: it leaves sum_weight untouched, so
+        // a call refused below has not modified the outputs
+        NGC_TRY((enqueue_count_scan<MODE, NP, ACC>(ctx, s, a, tg, bf, sum_weight, 1)));
+        unsigned long long total = 0;
+        NGC_HIP(hipMemcpyAsync(&total, bf.offsets + tg.n_bins, sizeof(total),
+                               hipMemcpyDeviceToHost, s));
+        NGC_HIP(hipStreamSynchronize(s));
+        constexpr size_t EB = (size_t)entry_bytes<MODE, NP, ACC>();
+        void *p;
+        NGC_TAKE(p, cv, (size_t)(total + 1) * EB, "the bin records");
+        bf.entries = (char *)p;
+        bf.cap = total + 1;
+        if (!accumulate) {
+            fill_kernel<double><<<fill_blocks(n_sw), 256, 0, s>>>(sum_weight, n_sw, 0.0);
+            NGC_HIP(hipGetLastError());
+        }
+        return enqueue_fill_grid<MODE, NP, ACC, KAHAN, TU, W>(ctx, s, a, tg, bf, grid,
+                                                           sum_weight, accumulate);
+    });
+}
+#undef NGC_TAKE
 }  // namespace ngc
 
 extern "C" int ngc_standard_grid(ngc_context *ctx, void *stream, int mode, int precision,
@@ -1551,6 +1758,46 @@ extern "C" int ngc_standard_grid(ngc_context *ctx, void *stream, int mode, int p
     return ngc::standard_grid_impl(ctx, s, mode, precision, a, freq_chan, chan_map, pol_map,
                                    cgk_1d, cgk_len, accumulate, grid, sum_weight, delta_l,
                                    delta_m);
+}
+
+extern "C" int ngc_standard_grid_noalloc(ngc_context *ctx, void *stream, int mode,
+                                         int precision, const double *uvw, const void *vis,
+                                         int vis_dtype, const void *weight, int weight_dtype,
+                                         const uint8_t *flag, const double *freq_chan,
+                                         const int32_t *chan_map, const int32_t *pol_map,
+                                         const double *cgk_1d, int64_t cgk_len, int64_t n_time,
+                                         int64_t n_baseline, int64_t n_chan, int64_t n_pol,
+                                         int64_t n_imag_chan, int64_t n_imag_pol, int64_t n_u,
+                                         int64_t n_v, double delta_l, double delta_m,
+                                         int64_t support, int64_t oversampling, int accumulate,
+                                         void *grid, double *sum_weight) {
+    if (!ctx) {
+        ngc::set_error("ctx is NULL");
+        return NGC_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = (hipStream_t)stream;
+    NGC_TRY(ngc::enter_stream(ctx, s));
+    ngc::StdArgs a{};
+    a.uvw = uvw;
+    a.vis = vis;
+    a.weight = weight;
+    a.flag = flag;
+    a.n_time = n_time;
+    a.n_baseline = n_baseline;
+    a.n_chan = n_chan;
+    a.n_pol = n_pol;
+    a.n_imag_chan = n_imag_chan;
+    a.n_imag_pol = n_imag_pol;
+    a.n_u = n_u;
+    a.n_v = n_v;
+    a.support = support;
+    a.os = oversampling;
+    a.vis_dtype = vis_dtype;
+    a.weight_dtype = weight_dtype;
+    return ngc::standard_grid_noalloc_impl(ctx, s, mode, precision, a, freq_chan, chan_map,
+                                           pol_map, cgk_1d, cgk_len, accumulate, grid,
+                                           sum_weight, delta_l, delta_m);
 }
 
 // ---------------------------------------------------------------------------

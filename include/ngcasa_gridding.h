@@ -140,6 +140,37 @@ int ngc_standard_grid(ngc_context *ctx, void *stream, int mode, int precision,
                       double *sum_weight);
 
 /*
+ * Allocation-free form of ngc_standard_grid (no counterpart in the reference:
+ * it is for callers that cannot allocate device memory during a call, such
+ * as code under HIP graph capture, tight serving loops and harnesses that
+ * forbid hipMalloc).  Same arguments, results and error behaviour as
+ * ngc_standard_grid, in every mode and precision.  It calls no hipMalloc,
+ * hipFree or allocating library routine: every device buffer beyond the
+ * inputs and outputs (parameter tables, tap-sum table, bin counts, offsets,
+ * order and queues, bin records) lives in one static device array of
+ * NGC_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load until the
+ * process ends and addressed once per device at ngc_context_create.  A call
+ * whose tables and bin records do not fit is refused with NGC_ERR_ARG before
+ * grid or sum_weight are touched (64 MiB holds about 1.6M bin records of
+ * 40 B, i.e. 4 pols of complex64, or 2.8M of 24 B; a record is one (row,
+ * channel, uv tile touched)).  All calls on one GPU share the array: calls
+ * that overlap on one GPU (different streams or contexts with no ordering
+ * between them) race.  Each call clears what must start at zero itself.
+ */
+#define NGC_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int ngc_standard_grid_noalloc(ngc_context *ctx, void *stream, int mode, int precision,
+                              const double *uvw, const void *vis, int vis_dtype,
+                              const void *weight, int weight_dtype, const uint8_t *flag,
+                              const double *freq_chan, const int32_t *chan_map,
+                              const int32_t *pol_map, const double *cgk_1d,
+                              int64_t cgk_len, int64_t n_time, int64_t n_baseline,
+                              int64_t n_chan, int64_t n_pol, int64_t n_imag_chan,
+                              int64_t n_imag_pol, int64_t n_u, int64_t n_v,
+                              double delta_l, double delta_m, int64_t support,
+                              int64_t oversampling, int accumulate, void *grid,
+                              double *sum_weight);
+
+/*
  * Briggs/uniform factors from a density grid: replaces calculate_briggs_parms.
  *   density_grid  device float64 [n_imag_chan, n_imag_pol, n_u, n_v]
  *   sum_weight    device float64 [n_imag_chan, n_imag_pol]
