@@ -320,6 +320,11 @@ def mosaic_leg(args):
     nat.profile_enable(False)
     n_vis = cfg.n_vis
     log(f"mosaic chosen form {args.precision}: {ms_step:.2f} ms, phases {phases}")
+    if args.mosaic_grid_only:
+        # the timed gridding steps alone (for counter runs)
+        t_k = phases.get("ap_grid_tiled", float("nan")) / 1e3
+        return dict(samples=int(n_vis), ms_per_step=round(ms_step, 3),
+                    kernel_ms=round(t_k * 1e3, 3), phases_ms=phases)
     # plain FP64 drop-in, once
     torch.cuda.synchronize()
     t0 = time.time()
@@ -824,6 +829,9 @@ def main():
                     help="skip the C1 leg (reported under 'c1')")
     ap.add_argument("--c1-only", action="store_true",
                     help="run only the C1 leg and print its record")
+    ap.add_argument("--mosaic-grid-only", action="store_true",
+                    help="with --mosaic-only: only the timed gridding steps (no plain FP64, "
+                         "prediction or surrogate), for profiling")
     ap.add_argument("--c5-chans", type=int, default=64,
                     help="channels of the C5 streamed leg (64: the whole cube)")
     ap.add_argument("--no-plain-fp64", action="store_true",

@@ -354,6 +354,14 @@ ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *n
 // ---------------------------------------------------------------------------
 // Byte offsets of the tiled kernel's dynamic LDS block (shared by the
 // kernel and the host's size computation).
+// Row strides (entries) of the residue-major staging tables.  A wavefront's
+// lanes span ~64/R u-residues and all R v-residues; the padding puts the rows
+// they read together on different LDS banks: u rows (4-byte entries, read 16
+// bytes at a time) start 36 words apart, v rows of 4-byte entries 33 words
+// and of complex entries 66 words apart, q rows (NP complex values per
+// entry) 33 entries apart.
+constexpr int kApSU = kApBatch + 4;
+constexpr int kApSV = kApBatch + 1;
 struct ApLds {
     int fo, pos, base, ui, vi, uc, vc, q, pv, wd, bytes;
 };
@@ -365,13 +373,13 @@ __host__ __device__ inline ApLds ap_lds_layout(int R, int W, int NP, int NW, int
     L.fo = ap_align16((kahan ? 2 : 1) * nacc * W * (W + 1) * acc_bytes);
     L.pos = L.fo + kApBatch * 16;
     L.base = L.pos + kApBatch * 16;
-    L.ui = L.base + kApBatch * 4;
-    L.vi = L.ui + kApBatch * R * 4;
-    L.uc = L.vi + kApBatch * R * 4;
-    L.vc = L.uc + kApBatch * R * 4;
-    L.q = ap_align16(L.vc + kApBatch * R * 4);
-    L.pv = L.q + kApBatch * R * NP * 2 * acc_bytes;
-    L.wd = L.pv + kApBatch * R * 2 * acc_bytes;
+    L.ui = ap_align16(L.base + kApBatch * 4);
+    L.vi = ap_align16(L.ui + kApSU * R * 4);
+    L.uc = ap_align16(L.vi + kApSV * R * 4);
+    L.vc = ap_align16(L.uc + kApSU * R * 4);
+    L.q = ap_align16(L.vc + kApSV * R * 4);
+    L.pv = ap_align16(L.q + kApSV * R * NP * 2 * acc_bytes);
+    L.wd = ap_align16(L.pv + kApSV * R * 2 * acc_bytes);
     L.bytes = ap_align16(L.wd + kApBatch * NP * NW * acc_bytes);
     return L;
 }
@@ -402,15 +410,18 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     //   s_fo    [kApBatch] int4   staging: field, offsets, supports
     //   s_pos   [kApBatch] int4   staging: start cells and their residues
     //   s_base  [kApBatch] uint   first tap index of the sample's R x R block
-    //   s_ui    [kApBatch][R] uint u-residue a: byte offset of the sample's tap
+    //   s_ui    [R][kApBatch] uint u-residue a: byte offset of the sample's tap
     //                         block plus ru*R taps; the zero tap's offset Z if no tap
-    //   s_vi    [kApBatch][R] uint v-residue b: rv taps in bytes; Z if no tap
+    //   s_vi    [R][kApBatch] uint v-residue b: rv taps in bytes; Z if no tap
     //                         (min(ui + vi, Z) is the tap's byte offset)
-    //   s_uc    [kApBatch][R] int cell_u*(W+1), or -2^30 if no tap
-    //   s_vc    [kApBatch][R] int cell_v, or -2^30 if no tap (uc + vc < 0: no tap)
-    //   s_q     [kApBatch][R][NP] u phase factor x weighted visibility per pol
-    //   s_pv    [kApBatch][R] v phase factor
+    //   s_uc    [R][kApBatch] int cell_u*(W+1), or -2^30 if no tap
+    //   s_vc    [R][kApBatch] int cell_v, or -2^30 if no tap (uc + vc < 0: no tap)
+    //   s_q     [R][kApBatch][NP] u phase factor x weighted visibility per pol
+    //   s_pv    [R][kApBatch] v phase factor
     //   s_wd    [kApBatch][NP*NW]
+    // The per-residue tables are residue-major and always hold a whole batch
+    // (samples past the end of the bin as "no tap"), so that a lane's reads of
+    // consecutive samples are one base address plus immediate offsets.
     extern __shared__ __align__(16) char smem_raw[];
     const ApLds L = ap_lds_layout(R, W, NP, NW, (int)sizeof(ACC), KAHAN);
     // (sum, compensation) pairs side by side with KAHAN
@@ -448,6 +459,11 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     const bool active = tid < R * R;
     const int ra = active ? tid / R : 0, rb = active ? tid % R : 0;
     const int nwv = 2 * a.half_v + 1;
+    // this thread's staging items (sample j, residue r < R) of each axis are
+    // tid, tid + blockDim, ...: the first divided out once, the rest stepped
+    // by blockDim = st_dj R + st_dr
+    const int st_j0 = tid / R, st_r0 = tid - st_j0 * R;
+    const int st_dj = (int)blockDim.x / R, st_dr = (int)blockDim.x - st_dj * R;
     int cur = -1;  // current cell: u*(W+1) + v in the region, -1 = none
     ACC acc[NACC];
 #pragma unroll
@@ -485,7 +501,11 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
             const int cf = en.meta & 0xffff, fidx = (en.meta >> 16) & 0xffff;
             const int su = a.support[cf * 2], sv = a.support[cf * 2 + 1];
             const int us = uc - su / 2 - org_u, vs = vc - sv / 2 - org_v;
-            s_fo[tid] = make_int4(fidx, ou, ov, su | (sv << 16));
+            // (fo.x, fo.y: the sample's phase-gradient factor index at tap
+            // offset 0 of its support, so a residue's factor is one 32-bit add)
+            s_fo[tid] = make_int4(fidx * (int)a.k_u + ou + (int)(a.k_u / 2) - (int)a.os_u * (su / 2),
+                                  fidx * (int)a.k_v + ov + (int)(a.k_v / 2) - (int)a.os_v * (sv / 2),
+                                  0, su | (sv << 16));
             s_pos[tid] = make_int4(us, us % R, vs, vs % R);
             s_base[tid] = (unsigned)(((cf * (2 * a.half_u + 1) + ou + a.half_u) * nwv + ov +
                                       a.half_v) * R * R);
@@ -512,84 +532,93 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
             const uint64_t bm = __ballot(m);
             if (tid == 0) s_mv = (unsigned)bm;
         }
-        // per-sample, per-residue tables
-        for (int idx = tid; idx < nh * 2 * R; idx += blockDim.x) {
-            const int j = idx / (2 * R), q = idx - j * 2 * R;
-            const bool axis_v = q >= R;
-            const int r = axis_v ? q - R : q;
+        // per-sample, per-residue tables, one pass per axis (so that a
+        // wavefront's items take one branch); samples from nh on are "no tap"
+        for (int j = st_j0, r = st_r0; j < kApBatch;
+             j += st_dj + (r + st_dr >= R), r += st_dr - (r + st_dr >= R ? R : 0)) {
+            const int eu = r * kApSU + j, eq = r * kApSV + j;
+            if (j >= nh) {
+                s_ui[eu] = zb;
+                s_uc[eu] = -(1 << 30);
+#pragma unroll
+                for (int p = 0; p < NP; p++) s_q[eq * NP + p] = Cplx<ACC>{(ACC)0, (ACC)0};
+                continue;
+            }
             const int4 fo = s_fo[j];
             const int4 ps = s_pos[j];
-            if (!axis_v) {
-                const int su = fo.w & 0xffff;
-                int ru = r - ps.y;
-                ru += ru < 0 ? R : 0;
-                const bool valid = ru < su;
-                Cplx<ACC> pu;
-                pu.x = (ACC)0;
-                pu.y = (ACC)0;
-                if (valid) pu = pgu[(int64_t)fo.x * a.k_u + a.os_u * (ru - su / 2) + fo.y +
-                                    a.k_u / 2];
-                s_ui[j * R + r] = valid ? (s_base[j] + (unsigned)(ru * R)) * (unsigned)sizeof(ACC)
-                                        : zb;
-                s_uc[j * R + r] = valid ? (ps.x + ru) * (W + 1) : -(1 << 30);
+            const int su = fo.w & 0xffff;
+            int ru = r - ps.y;
+            ru += ru < 0 ? R : 0;
+            const bool valid = ru < su;
+            Cplx<ACC> pu{(ACC)0, (ACC)0};
+            if (valid) pu = pgu[fo.x + (int)a.os_u * ru];
+            s_ui[eu] = valid ? (s_base[j] + (unsigned)(ru * R)) * (unsigned)sizeof(ACC) : zb;
+            s_uc[eu] = valid ? (ps.x + ru) * (W + 1) : -(1 << 30);
 #pragma unroll
-                for (int p = 0; p < NP; p++) {
-                    Cplx<ACC> qv;
-                    if (NW == 2) {
-                        const ACC wr = s_wd[j * NP * NW + p * NW];
-                        const ACC wi = s_wd[j * NP * NW + p * NW + 1];
-                        qv.x = pu.x * wr - pu.y * wi;
-                        qv.y = pu.x * wi + pu.y * wr;
-                    } else {
-                        const ACC w = s_wd[j * NP * NW + p];
-                        qv.x = pu.x * w;
-                        qv.y = pu.y * w;
-                    }
-                    s_q[(j * R + r) * NP + p] = qv;
+            for (int p = 0; p < NP; p++) {
+                Cplx<ACC> qv;
+                if (NW == 2) {
+                    const ACC wr = s_wd[j * NP * NW + p * NW];
+                    const ACC wi = s_wd[j * NP * NW + p * NW + 1];
+                    qv.x = pu.x * wr - pu.y * wi;
+                    qv.y = pu.x * wi + pu.y * wr;
+                } else {
+                    const ACC w = s_wd[j * NP * NW + p];
+                    qv.x = pu.x * w;
+                    qv.y = pu.y * w;
                 }
-            } else {
-                const int sv = fo.w >> 16;
-                int rv = r - ps.w;
-                rv += rv < 0 ? R : 0;
-                const bool valid = rv < sv;
-                Cplx<ACC> pv;
-                pv.x = (ACC)0;
-                pv.y = (ACC)0;
-                if (valid) pv = pgv[(int64_t)fo.x * a.k_v + a.os_v * (rv - sv / 2) + fo.z +
-                                    a.k_v / 2];
-                s_vi[j * R + r] = valid ? (unsigned)rv * (unsigned)sizeof(ACC) : zb;
-                s_vc[j * R + r] = valid ? ps.z + rv : -(1 << 30);
-                s_pv[j * R + r] = pv;
+                s_q[eq * NP + p] = qv;
             }
+        }
+        for (int j = st_j0, r = st_r0; j < kApBatch;
+             j += st_dj + (r + st_dr >= R), r += st_dr - (r + st_dr >= R ? R : 0)) {
+            const int ev = r * kApSV + j;
+            if (j >= nh) {
+                s_vi[ev] = zb;
+                s_vc[ev] = -(1 << 30);
+                s_pv[ev] = Cplx<ACC>{(ACC)0, (ACC)0};
+                continue;
+            }
+            const int4 fo = s_fo[j];
+            const int4 ps = s_pos[j];
+            const int sv = fo.w >> 16;
+            int rv = r - ps.w;
+            rv += rv < 0 ? R : 0;
+            const bool valid = rv < sv;
+            Cplx<ACC> pv{(ACC)0, (ACC)0};
+            if (valid) pv = pgv[fo.y + (int)a.os_v * rv];
+            s_vi[ev] = valid ? (unsigned)rv * (unsigned)sizeof(ACC) : zb;
+            s_vc[ev] = valid ? ps.z + rv : -(1 << 30);
+            s_pv[ev] = pv;
         }
         __syncthreads();
         const unsigned moved = __builtin_amdgcn_readfirstlane(s_mv);
         if (active) {
             // Groups of G samples.  A group's tap-table loads (global memory,
             // one per lane and sample) are issued together one group ahead of
-            // its arithmetic, branch-free: an invalid tap (residue outside
-            // the sample's support) loads element 0 and is zeroed at use, and
-            // past the batch the last sample is reloaded.  The next group's
-            // loads are unconditional so that the wait for this group's taps
-            // counts only the loads issued after them.
+            // its arithmetic, branch-free: a tap outside the sample's support
+            // (or past the end of the bin) reads the zero tap.  The next
+            // group's loads are unconditional (clamped to the last group of
+            // the batch) so that the wait for this group's taps counts only
+            // the loads issued after them.  A lane's table reads are its
+            // residue row plus the sample index: one address per group.
             constexpr int G = kApGroup;
             const int ng = (nh + G - 1) / G;
-            int cA[G], cB[G];
+            const unsigned *ui_r = s_ui + ra * kApSU;
+            const unsigned *vi_r = s_vi + rb * kApSV;
+            const Cplx<ACC> *pv_r = s_pv + rb * kApSV;
+            const Cplx<ACC> *q_r = s_q + ra * kApSV * NP;
             ACC tA[G], tB[G];
-            auto load = [&](int j0, int (&cellv)[G], ACC (&t)[G]) {
+            auto load = [&](int j0, ACC (&t)[G]) {
+                j0 = min(j0, kApBatch - G);
                 unsigned ui[G], vi[G];
-                int uc[G], vc[G];
 #pragma unroll
                 for (int g = 0; g < G; g++) {
-                    const int j = min(j0 + g, nh - 1);
-                    ui[g] = s_ui[j * R + ra];
-                    vi[g] = s_vi[j * R + rb];
-                    uc[g] = s_uc[j * R + ra];
-                    vc[g] = s_vc[j * R + rb];
+                    ui[g] = ui_r[j0 + g];
+                    vi[g] = vi_r[j0 + g];
                 }
 #pragma unroll
                 for (int g = 0; g < G; g++) {
-                    cellv[g] = uc[g] + vc[g];   // < 0: no tap
                     // The tap's byte offset from the table's base (the load's
                     // address is the scalar base plus one vector offset); a
                     // tap outside the support reads the zero tap.
@@ -597,21 +626,21 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                     t[g] = *reinterpret_cast<const ACC *>(reinterpret_cast<const char *>(ckr) + off);
                 }
             };
-            auto run = [&](int j0, const int (&cellv)[G], const ACC (&t)[G]) {
+            auto run = [&](int j0, const ACC (&t)[G]) {
                 Cplx<ACC> pv[G], qv[G][NP];
 #pragma unroll
                 for (int g = 0; g < G; g++) {
-                    const int j = min(j0 + g, nh - 1);
-                    pv[g] = s_pv[j * R + rb];
+                    pv[g] = pv_r[j0 + g];
 #pragma unroll
-                    for (int p = 0; p < NP; p++) qv[g][p] = s_q[(j * R + ra) * NP + p];
+                    for (int p = 0; p < NP; p++) qv[g][p] = q_r[(j0 + g) * NP + p];
                 }
 #pragma unroll
                 for (int g = 0; g < G; g++) {
                     const int j = j0 + g;
                     if (j >= nh) break;
                     if ((moved >> j) & 1u) {
-                        const int cell = cellv[g];
+                        // the lane's cell for this sample (< 0: no tap)
+                        const int cell = s_uc[ra * kApSU + j] + s_vc[rb * kApSV + j];
                         if (cell >= 0 && cell != cur) {
                             flush();
                             cur = cell;
@@ -632,13 +661,13 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                     }
                 }
             };
-            load(0, cA, tA);
+            load(0, tA);
             for (int gi = 0; gi < ng; gi += 2) {
-                load((gi + 1) * G, cB, tB);
-                run(gi * G, cA, tA);
+                load((gi + 1) * G, tB);
+                run(gi * G, tA);
                 if (gi + 1 >= ng) break;
-                load((gi + 2) * G, cA, tA);
-                run((gi + 1) * G, cB, tB);
+                load((gi + 2) * G, tA);
+                run((gi + 1) * G, tB);
             }
         }
     }
@@ -1038,6 +1067,9 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     NGC_CHECK((n_ckr + 1) * acc_bytes < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED,
               "re-laid convolution kernels exceed 2 GiB");
     NGC_TRY(workspace(ctx, "ap_ckr", (n_ckr + 1) * acc_bytes, &d_ckr));
+    // the tiled kernel indexes the factor tables with 32-bit integers
+    NGC_CHECK(c.n_field * c.k_u < (int64_t(1) << 31) && c.n_field * c.k_v < (int64_t(1) << 31),
+              NGC_ERR_UNSUPPORTED, "phase-gradient factor tables exceed 2^31 entries");
     NGC_TRY(workspace(ctx, "ap_pgu", c.n_field * c.k_u * 2 * acc_bytes, &d_pgu));
     NGC_TRY(workspace(ctx, "ap_pgv", c.n_field * c.k_v * 2 * acc_bytes, &d_pgv));
     const int64_t n_pg = c.n_field * c.k_u * c.k_v;
