@@ -23,7 +23,7 @@
 //     pair at most once, so a lane has at most one tap per sample and no two
 //     lanes share a cell (J. W. Romein, ICS 2012).  The lane keeps its
 //     cell's running sum in registers and flushes it into the LDS region
-//     (compensated with FP32_KAHAN) when its cell changes.  The region is
+//     (FP64 cells with FP32_KAHAN) when its cell changes.  The region is
 //     written to a scratch slot of the bin.
 //  5. ap_merge_kernel: only non-empty bins have scratch slots; every cell of
 //     a tile next to a non-empty bin adds, in a fixed order and FP64, the (at
@@ -45,7 +45,7 @@
 
 namespace ngc {
 
-constexpr int kApBatch = 32;   // samples staged per batch
+constexpr int kApBatch = 16;   // samples staged per batch
 constexpr int kApGroup = 8;    // samples whose tap loads are issued together
 constexpr int kApBinBlock = 256;
 
@@ -424,8 +424,10 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     // consecutive samples are one base address plus immediate offsets.
     extern __shared__ __align__(16) char smem_raw[];
     const ApLds L = ap_lds_layout(R, W, NP, NW, (int)sizeof(ACC), KAHAN);
-    // (sum, compensation) pairs side by side with KAHAN
-    using CELL = typename std::conditional<KAHAN, float2, ACC>::type;
+    // FP64 cells for the compensated form: the same 8 bytes as a (sum,
+    // compensation) pair of floats, at least its accuracy, and a flush is one
+    // conversion and one LDS add instead of a TwoSum
+    using CELL = typename std::conditional<KAHAN, double, ACC>::type;
     CELL *s_acc = reinterpret_cast<CELL *>(smem_raw);
     int4 *s_fo = reinterpret_cast<int4 *>(smem_raw + L.fo);
     int4 *s_pos = reinterpret_cast<int4 *>(smem_raw + L.pos);
@@ -473,20 +475,10 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
         if (cur >= 0) {
 #pragma unroll
             for (int q = 0; q < NACC; q++) {
-                CELL *dst = &s_acc[q * PLANE + cur];
-                if constexpr (KAHAN) {
-                    // Knuth's TwoSum: t + e == s + x exactly, e joins the compensation
-                    const float x = acc[q];
-                    float2 sc = *dst;
-                    const float t = sc.x + x;
-                    const float z = t - sc.x;
-                    const float e = (sc.x - (t - z)) + (x - z);
-                    sc.y += e;
-                    sc.x = t;
-                    *dst = sc;
-                } else {
-                    *dst = *dst + acc[q];
-                }
+                // no two lanes share a cell: the atomic only spares the
+                // wavefront the read round trip (the add happens in the LDS)
+                __hip_atomic_fetch_add(&s_acc[q * PLANE + cur], (CELL)acc[q], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
     };
@@ -679,12 +671,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
         const int q = i / (W * W), r = i - q * W * W;
         const int lu = r / W, lv = r - lu * W;
         const int o = q * PLANE + lu * (W + 1) + lv;
-        SC v;
-        if constexpr (KAHAN)
-            v = (SC)s_acc[o].x + (SC)s_acc[o].y;
-        else
-            v = (SC)s_acc[o];
-        out[i] = v;
+        out[i] = (SC)s_acc[o];
     }
 }
 
@@ -1037,8 +1024,9 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
         return NGC_OK;
     }
 
-    // tile edge: the largest of 32, 24, 16 whose workgroup LDS lets two
-    // workgroups share a CU
+    // tile edge: of 32, 24 and 16, the one whose workgroup LDS lets the most
+    // workgroups share a CU (the kernel is latency-bound), the larger on a tie
+    // (fewer halo cells to write out and merge)
     const int acc_bytes = single ? 4 : 8;
     const int nacc = 2 * (int)c.n_pol;
     const int nw = c.mode == NGC_GRID_IMAGE ? 2 : 1;
@@ -1046,11 +1034,14 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
         return (size_t)ap_lds_layout(a.R, T + a.R - 1, (int)c.n_pol, nw, acc_bytes, kahan).bytes;
     };
     int T = 16;
-    for (int cand : {32, 24, 16})
-        if (lds_for(cand) <= 78 * 1024) {
+    size_t best_per_cu = 0;
+    for (int cand : {32, 24, 16}) {
+        const size_t per_cu = (160 * 1024) / lds_for(cand);
+        if (per_cu > best_per_cu) {
+            best_per_cu = per_cu;
             T = cand;
-            break;
         }
+    }
     const size_t lds = lds_for(T);
     NGC_CHECK(lds <= 160 * 1024, NGC_ERR_UNSUPPORTED, "aperture tile does not fit in LDS");
     a.T = T;

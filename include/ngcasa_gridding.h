@@ -82,11 +82,10 @@ extern "C" {
 #define NGC_PREC_FP64 0       /* plain FP64 drop-in: reference loop, FP64 atomics */
 #define NGC_PREC_FP64_TILED 1 /* uv-tiled binning, FP64 LDS accumulation */
 #define NGC_PREC_FP32_KAHAN 2 /* uv-tiled binning, FP32 taps and register runs
-                                 flushed into an LDS subgrid with compensated
-                                 sums: FP64 cells in the standard gridder, a
-                                 TwoSum (sum, compensation) pair per value in
-                                 the aperture gridder; FP64 for coordinates
-                                 and sum_weight */
+                                 flushed into an LDS subgrid whose cells are
+                                 FP64 (the 8 bytes of a (sum, compensation)
+                                 pair of floats, with at least its accuracy);
+                                 FP64 for coordinates and sum_weight */
 #define NGC_PREC_FP32 3       /* as FP32_KAHAN with plain FP32 LDS cells */
 
 /* element types */
