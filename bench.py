@@ -245,11 +245,60 @@ def c1_leg(args):
                 make_psf_plus_make_image_api_ms=round(t_api, 1))
 
 
+def mosaic_surrogate(d, delta, npad, C):
+    """Per-channel A-term kernels of C4 from the learned surrogate (trained on
+    exact GPU kernels at 38 frequencies x 3 dish pairs x 21^2 sub-cell offsets
+    = 50,274 examples), checked tap by tap against the exact per-channel
+    kernels (fallback above 1e-4 of the peak).  Returns (kernels, exact
+    kernels, supports per channel, report)."""
+    import torch
+    from cngi_prototype_amd.imaging._imaging_utils import _kernel_surrogate as ks
+    from cngi_prototype_amd.imaging._imaging_utils._gcf_maps import cf_chan_map, gcf_kernels
+    pairs = np.array([[0, 0], [0, 1], [1, 1]])
+    blockage = np.array([0.75, 0.75])
+    os_ = np.array([20, 20])
+    ms = np.array([18, 18])
+    pb_cell = delta * os_
+    npad2 = np.array([npad, npad])
+    cfc, pb_freq = cf_chan_map(d["freq"], 0.005)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    wck_b, ck_b, sup_b = gcf_kernels(pb_freq, d["dish"], blockage, pairs, True, pb_cell, npad2,
+                                     os_, ms, 0.025)
+    torch.cuda.synchronize()
+    t_bin = time.time() - t0
+    half = 0.5 * (d["freq"].max() - d["freq"].min()) + 50e6
+    model, tinfo = ks.train_surrogate(d["dish"], blockage, pairs, True, pb_cell, npad2, os_, ms,
+                                      sup_b[:, 0, 0, :], pb_freq, half, n_freq=38, steps=10000,
+                                      batch=4096, lr=4e-3, log_every=2500)
+    ck_c, _, rep = ks.per_channel_kernels(model, d["freq"], cfc, pb_freq, d["dish"], blockage,
+                                          pairs, True, pb_cell, npad2, os_, ms, sup_b, ck_b,
+                                          wck_b, check=True, keep_exact=True)
+    torch.cuda.synchronize()
+    sup_c = np.ascontiguousarray(sup_b[:, cfc])
+    info = dict(network="4 dense layers, width 256, bf16 matrix-core GEMMs, FP32 "
+                        "accumulation, Adam", examples=tinfo["examples"],
+                data_s=tinfo["data_s"], train_s=tinfo["train_s"],
+                loss_history=tinfo["history"], kernels=int(ck_c.shape[0] * ck_c.shape[1]),
+                bins=int(len(pb_freq)), bin_kernels_s=round(t_bin, 3),
+                surrogate_kernels_s=rep["surrogate_s"],
+                exact_per_channel_s=rep.get("exact_s"),
+                max_tap_err_bin_kernel=rep["max_err_bin"],
+                max_tap_err_surrogate=rep["max_err_surrogate"],
+                fallbacks=len(rep["fallback"]), tolerance=ks.TOL_PEAK)
+    del model
+    return ck_c, rep.pop("ck_exact"), sup_c, info
+
+
 def mosaic_leg(args):
     """C4: 25-pointing heterogeneous mosaic (12 x 7 m + 40 x 12 m, 230 GHz, 32 chan x 2
     pol, 1440 x 10 s dumps, 1024^2 at 0.09", GCF oversampling 20, supports 11/15/17).
-    Times the chosen form against the plain FP64 drop-in on the same inputs and
-    states the dirty-image error per plane."""
+
+    The timed chosen form grids with per-channel kernels from the learned
+    surrogate (the reference uses one kernel per dish pair for all 32 channels
+    of the 0.5% frequency bin).  Beside it: the chosen form with the bin
+    kernels, and the plain FP64 drop-in.  Dirty-image errors are stated per
+    plane against the FP64 image made with the exact per-channel kernels."""
     import torch
     from cngi_prototype_amd import _native as nat
     from cngi_prototype_amd import synthetic as syn
@@ -292,53 +341,109 @@ def mosaic_leg(args):
     delta = np.array([-cell, cell])
     C, P = cfg.n_chan, cfg.n_pol
     cfb = np.asarray(gcf["CF_BASELINE_MAP"].values)
-    ck = gcf["CONV_KERNEL"].data
+    cfp = np.asarray(gcf["CF_POL_MAP"].values)
     pg = gcf["PHASE_GRADIENT"].data
     fid = np.arange(len(d["fields"]))
+    # kernel sets: (CONV_KERNEL, SUPPORT, CF_CHAN_MAP)
+    bins = (gcf["CONV_KERNEL"].data, sup, np.asarray(gcf["CF_CHAN_MAP"].values))
+    chans = exact = None
+    surrogate = None
+    if not args.mosaic_grid_only:
+        try:
+            ck_c, ck_x, sup_c, surrogate = mosaic_surrogate(d, delta, npad, C)
+            chans = (ck_c, sup_c, np.arange(C))
+            exact = (ck_x, sup_c, np.arange(C))
+            log(f"mosaic surrogate: {surrogate}")
+        except Exception as e:  # report, never fake
+            log(f"surrogate failed: {e!r}")
+            surrogate = {"error": repr(e)}
+    main = chans if chans is not None else bins
 
-    def call(precision, grid=None, sw=None):
+    def call(precision, kern, grid=None, sw=None):
+        ck, sp, cfc = kern
         return nat.aperture_grid(d["uvw"], d["weight"], d["field"], fid, d["freq"], np.arange(C),
-                                 np.arange(P), cfb, np.asarray(gcf["CF_CHAN_MAP"].values),
-                                 np.asarray(gcf["CF_POL_MAP"].values), ck, sup, pg, C,
+                                 np.arange(P), cfb, cfc, cfp, ck, sp, pg, C,
                                  [npad, npad], delta, [20, 20], nat.GRID_IMAGE, precision,
                                  vis=d["vis"], grid=grid, sum_weight=sw)
 
+    def timed(kern, grid, sw, steps):
+        for _ in range(args.warmup):
+            call(args.precision, kern, grid, sw)
+        torch.cuda.synchronize()
+        nat.profile_enable(True)
+        phases = {}
+        t0 = time.time()
+        for _ in range(steps):
+            call(args.precision, kern, grid, sw)
+            for name, ms in nat.profile_read():
+                phases[name] = phases.get(name, 0.0) + ms / steps
+        torch.cuda.synchronize()
+        ms_step = (time.time() - t0) / steps * 1e3
+        nat.profile_enable(False)
+        return ms_step, phases
+
+    def plain(kern):
+        torch.cuda.synchronize()
+        t0 = time.time()
+        g, s = call("fp64", kern)
+        torch.cuda.synchronize()
+        return g, s, (time.time() - t0) * 1e3
+
+    def plane_err(img, ref):
+        dd = (img - ref).reshape(C * P, -1)
+        rr = ref.reshape(C * P, -1)
+        rel = (torch.linalg.norm(dd, dim=1) / torch.linalg.norm(rr, dim=1)).cpu().numpy()
+        return dict(rel_rms_max=float(rel.max()), rel_rms_median=float(np.median(rel)),
+                    planes=int(C * P))
+
     grid = torch.empty((C, P, npad, npad), dtype=torch.complex64, device="cuda")
     sw = torch.empty((C, P), dtype=torch.float64, device="cuda")
-    for _ in range(args.warmup):
-        call(args.precision, grid, sw)
-    torch.cuda.synchronize()
-    nat.profile_enable(True)
-    phases = {}
-    t0 = time.time()
-    for _ in range(args.steps):
-        call(args.precision, grid, sw)
-        for name, ms in nat.profile_read():
-            phases[name] = phases.get(name, 0.0) + ms / args.steps
-    torch.cuda.synchronize()
-    ms_step = (time.time() - t0) / args.steps * 1e3
-    nat.profile_enable(False)
+    ms_step, phases = timed(main, grid, sw, args.steps)
     n_vis = cfg.n_vis
-    log(f"mosaic chosen form {args.precision}: {ms_step:.2f} ms, phases {phases}")
+    kern_name = "surrogate per-channel" if chans is not None else "bin"
+    log(f"mosaic chosen form {args.precision}, {kern_name} kernels: {ms_step:.2f} ms, "
+        f"phases {phases}")
+    t_k = phases.get("ap_grid_tiled", float("nan")) / 1e3
     if args.mosaic_grid_only:
         # the timed gridding steps alone (for counter runs)
-        t_k = phases.get("ap_grid_tiled", float("nan")) / 1e3
         return dict(samples=int(n_vis), ms_per_step=round(ms_step, 3),
                     kernel_ms=round(t_k * 1e3, 3), phases_ms=phases)
-    # plain FP64 drop-in, once
-    torch.cuda.synchronize()
-    t0 = time.time()
-    g64, sw64 = call("fp64")
-    torch.cuda.synchronize()
-    ms_plain = (time.time() - t0) * 1e3
+    img_main = nat.grid_to_image(grid, [N, N], sum_weight=sw)
+    bin_leg = None
+    if chans is not None:
+        # the reference's kernels (one per bin), same form
+        gb = torch.empty_like(grid)
+        swb = torch.empty_like(sw)
+        ms_bin, ph_bin = timed(bins, gb, swb, args.steps)
+        img_bin = nat.grid_to_image(gb, [N, N], sum_weight=swb)
+        del gb
+        bin_leg = dict(ms_per_step=round(ms_bin, 3),
+                       mvis_s=round(n_vis / (ms_bin / 1e3) / 1e6, 1),
+                       kernel_ms=round(ph_bin.get("ap_grid_tiled", float("nan")), 3))
+        log(f"mosaic chosen form, bin kernels: {ms_bin:.2f} ms")
+    # plain FP64 drop-in with the timed kernel set, once
+    g64, sw64, ms_plain = plain(main)
     log(f"mosaic plain fp64: {ms_plain:.1f} ms")
     img64 = nat.grid_to_image(g64, [N, N], sum_weight=sw64)
+    del g64
+    errors = dict(vs_fp64_same_kernels=plane_err(img_main, img64))
+    if exact is not None:
+        gx, swx, ms_exact = plain(exact)
+        img_x = nat.grid_to_image(gx, [N, N], sum_weight=swx)
+        del gx
+        log(f"mosaic plain fp64, exact per-channel kernels: {ms_exact:.1f} ms")
+        errors["chosen_vs_exact_fp64"] = plane_err(img_main, img_x)
+        errors["bin_kernels_vs_exact_fp64"] = plane_err(img_bin, img_x)
+        errors["fp64_surrogate_vs_exact_fp64"] = plane_err(img64, img_x)
+        del img_x, img_bin
+    del img64, img_main
+    log(f"mosaic errors: {errors}")
     # A-term prediction: the gridded cube degridded back to every sample with
     # the same kernels, fields and maps (complex64 grid -> complex64 samples)
     def degrid(g, single, out=None):
+        ck, sp, cfc = main
         return nat.aperture_degrid(d["uvw"], d["field"], fid, g, d["freq"], np.arange(C),
-                                   np.arange(P), cfb, np.asarray(gcf["CF_CHAN_MAP"].values),
-                                   np.asarray(gcf["CF_POL_MAP"].values), ck, sup, pg, delta,
+                                   np.arange(P), cfb, cfc, cfp, ck, sp, pg, delta,
                                    [20, 20], single=single, out=out)
 
     vis_p = degrid(grid, True)
@@ -354,81 +459,34 @@ def mosaic_leg(args):
     torch.cuda.synchronize()
     ms_degrid = (time.time() - t0) / n_dg * 1e3
     nat.profile_enable(False)
-    # error of the complex64 path against FP64 throughout, on the FP64 grid
+    # error of the complex64 path against FP64 throughout, on one grid
+    g64 = grid.to(torch.complex128)
     vis64 = degrid(g64, False)
-    vis32 = degrid(g64.to(torch.complex64), True)
+    vis32 = degrid(grid, True)
     dg_err = float(torch.linalg.norm((vis32.to(torch.complex128) - vis64).ravel())
                    / torch.linalg.norm(vis64.ravel()))
     del vis64, vis32, vis_p, g64
     log(f"mosaic aperture degrid: {ms_degrid:.2f} ms, phases {dg_phases}, "
         f"rel error vs fp64 {dg_err:.2e}")
-    img32 = nat.grid_to_image(grid, [N, N], sum_weight=sw)
-    n_planes = C * P
-    dd = (img32 - img64).reshape(n_planes, -1)
-    rr = img64.reshape(n_planes, -1)
-    rel = (torch.linalg.norm(dd, dim=1) / torch.linalg.norm(rr, dim=1)).cpu().numpy()
-    del img32, img64
     # algorithmic flops: per tap CK*PG (2) + one complex multiply-add per pol (8)
-    types = d["types"]
     taps = np.asarray([sup[cfb[b], 0, 0, 0] ** 2 for b in range(cfg.n_baseline)], dtype=np.float64)
     flops = float(taps.sum()) * cfg.n_time * C * (2 + 8 * P)
-    t_k = phases.get("ap_grid_tiled", float("nan")) / 1e3
     # A-term prediction: per tap PGu*PGv (6), times CK (2), the norm (2) and
     # conj(c) g accumulated per pol (8), in FP64
     t_dg = dg_phases.get("aperture_degrid", float("nan")) / 1e3
     dg_tflops = round(float(taps.sum()) * cfg.n_time * C * (10 + 8 * P) / t_dg / 1e12, 2)
-    # learned A-term surrogate: per-channel kernels for the 32 channels that the
-    # reference covers with one kernel per dish pair (one frequency bin)
-    surrogate = None
-    try:
-        from cngi_prototype_amd.imaging._imaging_utils import _kernel_surrogate as ks
-        from cngi_prototype_amd.imaging._imaging_utils._gcf_maps import cf_chan_map, gcf_kernels
-        pairs = np.array([[0, 0], [0, 1], [1, 1]])
-        os_ = np.array([20, 20])
-        ms = np.array([18, 18])
-        pb_cell = delta * os_
-        npad2 = np.array([npad, npad])
-        cfc, pb_freq = cf_chan_map(d["freq"], 0.005)
-        torch.cuda.synchronize()
-        t0 = time.time()
-        wck_b, ck_b, sup_b = gcf_kernels(pb_freq, d["dish"], np.array([0.75, 0.75]), pairs, True,
-                                         pb_cell, npad2, os_, ms, 0.025)
-        torch.cuda.synchronize()
-        t_bin = time.time() - t0
-        half = 0.5 * (d["freq"].max() - d["freq"].min()) + 50e6
-        model, tinfo = ks.train_surrogate(d["dish"], np.array([0.75, 0.75]), pairs, True, pb_cell,
-                                          npad2, os_, ms, sup_b[:, 0, 0, :], pb_freq, half,
-                                          n_freq=24, steps=10000, batch=4096, lr=4e-3,
-                                          log_every=2500)
-        ck_c, _, rep = ks.per_channel_kernels(model, d["freq"], cfc, pb_freq, d["dish"],
-                                              np.array([0.75, 0.75]), pairs, True, pb_cell,
-                                              npad2, os_, ms, sup_b, ck_b, wck_b, check=True)
-        torch.cuda.synchronize()
-        surrogate = dict(network="4 dense layers, width 256, bf16 matrix-core GEMMs, FP32 "
-                                 "accumulation, Adam", examples=tinfo["examples"],
-                         data_s=tinfo["data_s"], train_s=tinfo["train_s"],
-                         loss_history=tinfo["history"], kernels=int(ck_c.shape[0] * ck_c.shape[1]),
-                         bin_kernels_s=round(t_bin, 3), surrogate_kernels_s=rep["surrogate_s"],
-                         exact_per_channel_s=rep.get("exact_s"),
-                         max_tap_err_bin_kernel=rep["max_err_bin"],
-                         max_tap_err_surrogate=rep["max_err_surrogate"],
-                         fallbacks=len(rep["fallback"]), tolerance=ks.TOL_PEAK)
-        log(f"mosaic surrogate: {surrogate}")
-        del ck_c, ck_b, wck_b, model
-    except Exception as e:  # report, never fake
-        log(f"surrogate failed: {e!r}")
-        surrogate = {"error": repr(e)}
     return dict(workload=f"C4: 25-pointing mosaic, 12x7 m + 40x12 m, {cfg.n_time} x 10 s, "
                          f"{C} chan x {P} pol at 230 GHz, {N}^2 at {cfg.cell_arcsec}\", "
                          f"GCF os 20, supports {sorted(set(sup[:, 0, 0, 0].tolist()))}",
                 samples=int(n_vis),
+                kernels=kern_name,
                 value=round(n_vis / (ms_step / 1e3) / 1e6, 1), unit="Mvis/s",
                 ms_per_step=round(ms_step, 3), precision=args.precision,
                 plain_fp64_mvis_s=round(n_vis / (ms_plain / 1e3) / 1e6, 2),
                 plain_fp64_ms=round(ms_plain, 1),
                 speedup_vs_plain_fp64=round(ms_plain / ms_step, 1),
-                error_vs_fp64=dict(rel_rms_max=float(rel.max()),
-                                   rel_rms_median=float(np.median(rel)), planes=int(n_planes)),
+                error_vs_fp64=errors,
+                bin_kernels=bin_leg,
                 kernel_ms=round(t_k * 1e3, 3),
                 kernel_tflops=round(flops / t_k / 1e12, 2),
                 fp32_peak_tflops=157.3, phases_ms=phases, gcf_s=round(t_gcf, 2),

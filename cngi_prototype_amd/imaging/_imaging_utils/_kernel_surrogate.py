@@ -347,25 +347,28 @@ def train_surrogate(dish, blockage, pairs, casa, pb_cell, npad, oversampling, ma
     return model, info
 
 
-def predict_blocks(model, pair, dish, blockage, freqs, f_bin, base_blocks, peak, oversampling):
+def predict_blocks(model, pair, dish, blockage, freqs, f_bin, base_blocks, peak, oversampling,
+                   pa=0.0):
     """Per-frequency tap blocks [n_freq, (os+1)^2, R*R]: the bin's exact blocks
-    plus the predicted change."""
+    plus the predicted change, at parallactic angle `pa` (radians; the exact
+    kernels of symmetric Airy patterns do not depend on it)."""
     import torch
     i, j = pair
     ou, ov = _offset_grid(oversampling)
     x = np.concatenate([features((dish[i], dish[j]), (blockage[i], blockage[j]), f, f_bin,
-                                 model.meta["f_ref"], 0.0, ou, ov, oversampling) for f in freqs])
+                                 model.meta["f_ref"], pa, ou, ov, oversampling) for f in freqs])
     y = model.predict(torch.as_tensor(x, device="cuda"))
     return base_blocks[None] + peak * y.double().reshape(len(freqs), len(ou), -1)
 
 
 def per_channel_kernels(model, freq_chan, cf_chan_map, pb_freq, dish, blockage, pairs, casa,
                         pb_cell, npad, oversampling, max_support, supports, ck_bins, wck_bins,
-                        check=True):
+                        check=True, keep_exact=False):
     """Per-channel CONV_KERNEL [n_pairs, n_chan, 1, K, K] from the surrogate,
     the bins' WEIGHT_CONV_KERNEL repeated per channel, and a report (largest
     tap error of the surrogate and of the bin kernel per channel when check,
-    and which kernels fell back to the exact one)."""
+    and which kernels fell back to the exact one; with keep_exact the exact
+    per-channel kernels of the check as report["ck_exact"])."""
     import torch
     freq_chan = np.asarray(freq_chan, dtype=np.float64)
     n_pairs, n_chan = len(pairs), len(freq_chan)
@@ -413,4 +416,6 @@ def per_channel_kernels(model, freq_chan, cf_chan_map, pb_freq, dish, blockage, 
                 if e_s > TOL_PEAK:
                     ck[p, ch, 0] = ck_exact[p, ch, 0]
                     report["fallback"].append((int(p), int(ch), e_s))
+        if keep_exact:
+            report["ck_exact"] = ck_exact
     return ck, wck, report

@@ -130,9 +130,10 @@ def test_graph_training_equals_eager(gpu):
 def test_surrogate_beats_bin_kernels_and_falls_back(gpu):
     """Per-channel kernels over a 1 GHz band that the reference covers with one
     kernel (chan_tolerance_factor 0.005 at 230 GHz): the surrogate's largest
-    tap error is below the single bin kernel's, and every kernel returned
-    (after the fallback) is within 1e-4 of its peak of the exact one on the
-    samples the gridder reads."""
+    tap error is below the single bin kernel's, every kernel returned (after
+    the fallback) is within 1e-4 of its peak of the exact one on the samples
+    the gridder reads, and the network's kernels do not depend on the
+    parallactic angle."""
     import torch
     from helpers import mosaic_case
     from cngi_prototype_amd.imaging._imaging_utils._gcf_maps import (cf_baseline_map,
@@ -181,3 +182,30 @@ def test_surrogate_beats_bin_kernels_and_falls_back(gpu):
             assert (torch.equal(got[outside], ck[p, cfc[ch], 0][outside])
                     or torch.equal(got[outside], ex[outside]))
     assert tuple(wck_ch.shape) == tuple(ck_ch.shape)
+    # parallactic angle: the exact kernels do not depend on it (symmetric Airy
+    # patterns) and the training set draws it at random; the network's
+    # kernels move with the angle by less than half the fallback tolerance
+    # and stay closer to the exact kernels than the bin kernel at any angle
+    R = int(model.meta["R"])
+
+    def read_taps(blocks, su, p):
+        # the taps a gridder reads, through the kernel the blocks make
+        return torch.stack([ks.tap_blocks(ks.blocks_to_kernel(
+            b.reshape(os_[0] + 1, os_[1] + 1, R, R), su, os_, K, base=ck[p, 0, 0]), su, os_, R)
+            for b in blocks])
+
+    for p, (i, j) in enumerate(pairs):
+        su = sup[p, cfc[0], 0]
+        base = ks.tap_blocks(ck[p, 0, 0], su, os_, R).reshape(-1, R * R)
+        peak = float(base.abs().max().item())
+        ex = torch.stack([ks.tap_blocks(exact[p, ch, 0], su, os_, R) for ch in range(len(freq))])
+        at0 = read_taps(ks.predict_blocks(model, (i, j), dish, blockage, freq, pb_freq[0], base,
+                                          peak, os_, pa=0.0), su, p)
+        print(f"pair {p} pa 0: error {float((at0 - ex).abs().max()) / peak:.2e} of the peak")
+        for pa in (0.4, 1.3, 2.2, 3.0):
+            at = read_taps(ks.predict_blocks(model, (i, j), dish, blockage, freq, pb_freq[0],
+                                             base, peak, os_, pa=pa), su, p)
+            print(f"pair {p} pa {pa}: change {float((at - at0).abs().max()) / peak:.2e}, "
+                  f"error {float((at - ex).abs().max()) / peak:.2e} of the peak", flush=True)
+            assert float((at - at0).abs().max().item()) / peak < 0.5 * ks.TOL_PEAK, (p, pa)
+            assert float((at - ex).abs().max().item()) / peak < rep["max_err_bin"], (p, pa)
