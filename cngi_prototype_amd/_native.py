@@ -30,7 +30,8 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_aperture_grid", "ngc_grid_to_image", "ngc_normalize_mosaic",
            "ngc_profile_enable", "ngc_profile_read", "ngc_set_tile_rows",
            "ngc_gcf_patterns", "ngc_gcf_resize", "ngc_phase_gradient",
-           "ngc_grid_plan_create", "ngc_grid_plan_execute", "ngc_grid_plan_status",
+           "ngc_grid_plan_create", "ngc_aperture_grid_plan_create", "ngc_grid_plan_execute",
+           "ngc_grid_plan_status",
            "ngc_grid_plan_describe", "ngc_grid_plan_destroy", "ngc_bf16_gemm",
            "ngc_tanh_backward", "ngc_col_sum", "ngc_mse_grad", "ngc_adam_step",
            "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags", "ngc_set_tile_waves",
@@ -72,6 +73,12 @@ _PROTOS = {
                                  _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64,
                                  _i64, _i64, _i64, _vp, _vp, _i64, _i64, _i64, _i64, _i64,
                                  _i64, _i64, _i64, _d, _d, _i64, _i64, _i32, _vp, _vp]),
+    "ngc_aperture_grid_plan_create": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp, _i32,
+                                             _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                             _vp, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _i64,
+                                             _i64, _i64, _i64, _i64, _i64, _i64, _i64, _d, _d,
+                                             _i64, _i64, _i32, _vp, _vp, _i32,
+                                             ctypes.POINTER(_vp)]),
     "ngc_grid_to_image": (_i32, [_vp, _vp, _vp, _i32, _i64, _i64, _i64, _i64, _i64, _vp, _vp,
                                  _vp, _vp]),
     "ngc_profile_enable": (_i32, [_vp, _i32]),
@@ -308,7 +315,42 @@ def standard_grid(uvw, weight, freq_chan, chan_map, pol_map, cgk_1d, n_imag_chan
     return grid, sum_weight
 
 
-class GridPlan:
+class _Plan:
+    """A captured gridding pipeline (ngc_grid_plan_*): execute() replays it
+    with one graph launch on the current stream."""
+
+    handle = None
+
+    def execute(self):
+        _check("ngc_grid_plan_execute",
+               load_library().ngc_grid_plan_execute(self.handle, _stream()))
+        return self.grid, self.sum_weight
+
+    def status(self):
+        rc = load_library().ngc_grid_plan_status(self.handle, _stream())
+        if rc == -4:
+            raise IndexError("a row's FIELD_ID is not in the gcf's field_id list")
+        _check("ngc_grid_plan_status", rc)
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(4096)
+        _check("ngc_grid_plan_describe",
+               load_library().ngc_grid_plan_describe(self.handle, buf, 4096))
+        return buf.value.decode()
+
+    def close(self):
+        if self.handle is not None:
+            load_library().ngc_grid_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GridPlan(_Plan):
     """A captured, autotuned ngc_standard_grid call (ngc_grid_plan_*): the
     tensors given at creation are the plan's inputs and outputs; execute()
     replays the whole tiled pipeline with one graph launch."""
@@ -348,31 +390,6 @@ class GridPlan:
             ctypes.byref(h))
         _check("ngc_grid_plan_create", rc)
         self.handle = h
-
-    def execute(self):
-        _check("ngc_grid_plan_execute",
-               load_library().ngc_grid_plan_execute(self.handle, _stream()))
-        return self.grid, self.sum_weight
-
-    def status(self):
-        _check("ngc_grid_plan_status", load_library().ngc_grid_plan_status(self.handle, _stream()))
-
-    def describe(self):
-        buf = ctypes.create_string_buffer(4096)
-        _check("ngc_grid_plan_describe",
-               load_library().ngc_grid_plan_describe(self.handle, buf, 4096))
-        return buf.value.decode()
-
-    def close(self):
-        if self.handle is not None:
-            load_library().ngc_grid_plan_destroy(self.handle)
-            self.handle = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
 
 def briggs_factors(density_grid, sum_weight, weighting, robust=0.5):
@@ -416,14 +433,13 @@ def imaging_weight_degrid(uvw, natural_weight, density_grid, factors, freq_chan,
     return out
 
 
-def aperture_grid(uvw, weight, field, field_id, freq_chan, chan_map, pol_map,
-                  cf_baseline_map, cf_chan_map, cf_pol_map, conv_kernel, support,
-                  phase_gradient, n_imag_chan, n_uv, delta_lm, oversampling, mode,
-                  precision="fp64", vis=None, flag=None, grid=None, sum_weight=None,
-                  accumulate=False):
-    """ngc_aperture_grid on device tensors.  Returns (complex grid, sum_weight)."""
+def _aperture_args(uvw, weight, field, field_id, freq_chan, chan_map, pol_map,
+                   cf_baseline_map, cf_chan_map, cf_pol_map, conv_kernel, support,
+                   phase_gradient, n_imag_chan, n_uv, delta_lm, oversampling, mode, precision,
+                   vis, flag, grid, sum_weight, accumulate):
+    """Checked ctypes arguments of ngc_aperture_grid after (ctx, stream), the
+    output tensors and the host arrays the pointers refer to."""
     import torch
-    ctx = context()
     for t, n in ((uvw, "uvw"), (weight, "weight"), (field, "field"),
                  (conv_kernel, "conv_kernel"), (phase_gradient, "phase_gradient")):
         _dev(t, n)
@@ -451,18 +467,60 @@ def aperture_grid(uvw, weight, field, field_id, freq_chan, chan_map, pol_map,
     c, cp = _host(cf_chan_map, np.int32)
     p, pp = _host(cf_pol_map, np.int32)
     s, sp = _host(support, np.int32)
-    rc = load_library().ngc_aperture_grid(
-        ctx.handle, _stream(), int(mode), prec, _ptr(uvw), _ptr(vis),
-        _dt_vis(vis) if vis is not None else C64, _ptr(weight), _dt_w(weight), _ptr(flag),
-        _ptr(field), fidp, len(fid), frp, cmp_, pmp, bp, cp, pp, _ptr(conv_kernel), n_cfb,
-        n_cfc, n_cfp, k_u, k_v, sp, _ptr(phase_gradient), n_time, n_baseline, n_chan, n_pol,
-        int(n_imag_chan), n_pol, n_u, n_v, float(delta_lm[0]), float(delta_lm[1]),
-        int(oversampling[0]), int(oversampling[1]), int(bool(accumulate)), _ptr(grid),
-        _ptr(sum_weight))
+    args = (int(mode), prec, _ptr(uvw), _ptr(vis), _dt_vis(vis) if vis is not None else C64,
+            _ptr(weight), _dt_w(weight), _ptr(flag), _ptr(field), fidp, len(fid), frp, cmp_, pmp,
+            bp, cp, pp, _ptr(conv_kernel), n_cfb, n_cfc, n_cfp, k_u, k_v, sp,
+            _ptr(phase_gradient), n_time, n_baseline, n_chan, n_pol, int(n_imag_chan), n_pol, n_u,
+            n_v, float(delta_lm[0]), float(delta_lm[1]), int(oversampling[0]),
+            int(oversampling[1]), int(bool(accumulate)), _ptr(grid), _ptr(sum_weight))
+    return args, grid, sum_weight, (fid, fr, cm, pm, b, c, p, s)
+
+
+def aperture_grid(uvw, weight, field, field_id, freq_chan, chan_map, pol_map,
+                  cf_baseline_map, cf_chan_map, cf_pol_map, conv_kernel, support,
+                  phase_gradient, n_imag_chan, n_uv, delta_lm, oversampling, mode,
+                  precision="fp64", vis=None, flag=None, grid=None, sum_weight=None,
+                  accumulate=False):
+    """ngc_aperture_grid on device tensors.  Returns (complex grid, sum_weight)."""
+    ctx = context()
+    args, grid, sum_weight, _host_arrays = _aperture_args(
+        uvw, weight, field, field_id, freq_chan, chan_map, pol_map, cf_baseline_map,
+        cf_chan_map, cf_pol_map, conv_kernel, support, phase_gradient, n_imag_chan, n_uv,
+        delta_lm, oversampling, mode, precision, vis, flag, grid, sum_weight, accumulate)
+    rc = load_library().ngc_aperture_grid(ctx.handle, _stream(), *args)
     if rc == -4:
         raise IndexError("a row's FIELD_ID is not in the gcf's field_id list")
     _check("ngc_aperture_grid", rc)
     return grid, sum_weight
+
+
+class ApertureGridPlan(_Plan):
+    """A captured ngc_aperture_grid call (ngc_aperture_grid_plan_create): every
+    pointing, channel and pol of a mosaic gridded by one graph launch.  The
+    tensors given at creation are the plan's inputs and outputs (uvw, vis,
+    weight, flag and field may be rewritten in place between replays; the
+    convolution kernels and phase gradients are read at creation)."""
+
+    def __init__(self, uvw, weight, field, field_id, freq_chan, chan_map, pol_map,
+                 cf_baseline_map, cf_chan_map, cf_pol_map, conv_kernel, support,
+                 phase_gradient, n_imag_chan, n_uv, delta_lm, oversampling, mode,
+                 precision="fp32_kahan", vis=None, flag=None, grid=None, sum_weight=None,
+                 accumulate=False, autotune=True):
+        self._ctx = context()
+        self.handle = None
+        args, self.grid, self.sum_weight, _host_arrays = _aperture_args(
+            uvw, weight, field, field_id, freq_chan, chan_map, pol_map, cf_baseline_map,
+            cf_chan_map, cf_pol_map, conv_kernel, support, phase_gradient, n_imag_chan, n_uv,
+            delta_lm, oversampling, mode, precision, vis, flag, grid, sum_weight, accumulate)
+        # the plan holds device pointers: keep the tensors alive with it
+        self._keep = (uvw, weight, field, vis, flag)
+        h = _vp()
+        rc = load_library().ngc_aperture_grid_plan_create(
+            self._ctx.handle, _stream(), *args, int(bool(autotune)), ctypes.byref(h))
+        if rc == -4:
+            raise IndexError("a row's FIELD_ID is not in the gcf's field_id list")
+        _check("ngc_aperture_grid_plan_create", rc)
+        self.handle = h
 
 
 def grid_to_image(grid, image_size, sum_weight=None, corr=None):

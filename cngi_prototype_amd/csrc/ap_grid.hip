@@ -8,6 +8,8 @@
 // The reference multiplies the whole kernel cube by the field's phase gradient
 // on every field change (:428-430); here each tap forms CK*PG on the fly from
 // the two tables, which gives the same FP64 products.
+#include <functional>
+
 #include "ngc_internal.hpp"
 
 namespace ngc {
@@ -138,6 +140,173 @@ ap_grid_plain_kernel(ApArgs a, double *grid, double *sum_weight) {
 
 using namespace ngc;
 
+namespace {
+
+// The arguments of ngc_aperture_grid.
+struct ApHost {
+    int mode, precision;
+    const double *uvw;
+    const void *vis;
+    int vis_dtype;
+    const void *weight;
+    int weight_dtype;
+    const uint8_t *flag;
+    const int32_t *field, *field_id;
+    int64_t n_field;
+    const double *freq_chan;
+    const int32_t *chan_map, *pol_map, *cf_baseline_map, *cf_chan_map, *cf_pol_map;
+    const double *conv_kernel;
+    int64_t n_cfb, n_cfc, n_cfp, k_u, k_v;
+    const int32_t *support;
+    const double *phase_gradient;
+    int64_t n_time, n_baseline, n_chan, n_pol, n_imag_chan, n_imag_pol, n_u, n_v;
+    double delta_l, delta_m;
+    int64_t os_u, os_v;
+    int accumulate;
+    void *grid;
+    double *sum_weight;
+};
+
+// Device copies of the small host tables.
+struct ApDevTables {
+    void *scale, *cmap, *pmap, *cfb, *cfc, *cfp, *sup, *fid;
+};
+
+// Argument checks; *ms = max(SUPPORT), the half-width of the reference's
+// bounds check (_aperture_grid.py:397,447), with which every tap must index
+// inside the kernel.
+int ap_validate(const ApHost &h, int64_t *ms_out) {
+    NGC_CHECK(h.mode == NGC_GRID_IMAGE || h.mode == NGC_GRID_PSF || h.mode == NGC_GRID_WEIGHT,
+              NGC_ERR_ARG, "mode must be IMAGE, PSF or WEIGHT");
+    NGC_CHECK(h.precision >= NGC_PREC_FP64 && h.precision <= NGC_PREC_FP32, NGC_ERR_ARG,
+              "unknown precision");
+    const int64_t n_samples = h.n_time * h.n_baseline * h.n_chan;
+    NGC_CHECK(h.field_id && h.freq_chan && h.chan_map && h.pol_map && h.cf_baseline_map &&
+                  h.cf_chan_map && h.cf_pol_map && h.conv_kernel && h.support &&
+                  h.phase_gradient && h.grid && h.sum_weight,
+              NGC_ERR_ARG, "NULL array");
+    NGC_CHECK((h.uvw && h.weight && h.field) || n_samples == 0, NGC_ERR_ARG,
+              "NULL uvw, weight or field");
+    NGC_CHECK(h.mode != NGC_GRID_IMAGE || h.vis || n_samples == 0, NGC_ERR_ARG,
+              "image mode needs vis");
+    NGC_CHECK(h.n_field >= 1 && h.n_cfb >= 1 && h.n_cfc >= 1 && h.n_cfp >= 1 && h.k_u >= 1 &&
+                  h.k_v >= 1,
+              NGC_ERR_ARG, "bad kernel dimensions");
+    NGC_CHECK(h.os_u >= 1 && h.os_v >= 1, NGC_ERR_ARG, "oversampling must be >= 1");
+    for (int64_t b = 0; b < h.n_baseline; b++)
+        NGC_CHECK(h.cf_baseline_map[b] >= 0 && h.cf_baseline_map[b] < h.n_cfb, NGC_ERR_ARG,
+                  "cf_baseline_map range");
+    for (int64_t c = 0; c < h.n_chan; c++) {
+        NGC_CHECK(h.cf_chan_map[c] >= 0 && h.cf_chan_map[c] < h.n_cfc, NGC_ERR_ARG,
+                  "cf_chan_map range");
+        NGC_CHECK(h.chan_map[c] >= 0 && h.chan_map[c] < h.n_imag_chan, NGC_ERR_ARG,
+                  "chan_map range");
+    }
+    for (int64_t p = 0; p < h.n_pol; p++) {
+        NGC_CHECK(h.cf_pol_map[p] >= 0 && h.cf_pol_map[p] < h.n_cfp, NGC_ERR_ARG,
+                  "cf_pol_map range");
+        NGC_CHECK(h.pol_map[p] >= 0 && h.pol_map[p] < h.n_imag_pol, NGC_ERR_ARG,
+                  "pol_map range");
+    }
+    int64_t ms = h.support[0];
+    const int64_t n_sup = h.n_cfb * h.n_cfc * h.n_cfp * 2;
+    for (int64_t i = 1; i < n_sup; i++) ms = h.support[i] > ms ? h.support[i] : ms;
+    for (int64_t i = 0; i < n_sup; i += 2) {
+        const int64_t su = h.support[i], sv = h.support[i + 1];
+        NGC_CHECK(su >= 1 && sv >= 1, NGC_ERR_ARG, "support must be >= 1");
+        const int64_t reach_u = h.os_u * (su - su / 2 - 1) + h.os_u / 2 + 1;
+        const int64_t reach_v = h.os_v * (sv - sv / 2 - 1) + h.os_v / 2 + 1;
+        const int64_t low_u = h.os_u * (su / 2) + h.os_u / 2 + 1;
+        const int64_t low_v = h.os_v * (sv / 2) + h.os_v / 2 + 1;
+        NGC_CHECK(h.k_u / 2 + reach_u < h.k_u && h.k_u / 2 - low_u >= 0 &&
+                      h.k_v / 2 + reach_v < h.k_v && h.k_v / 2 - low_v >= 0,
+                  NGC_ERR_ARG, "support x oversampling exceeds the kernel size");
+    }
+    *ms_out = ms;
+    return NGC_OK;
+}
+
+using Upload = std::function<int(const char *, const void *, size_t, void **)>;
+
+int ap_upload_tables(const ApHost &h, const Upload &up, ApDevTables &d) {
+    std::vector<double> scale;
+    uv_scale_host(h.freq_chan, h.n_chan, h.delta_l, h.delta_m, h.n_u, h.n_v, scale);
+    const int64_t n_sup = h.n_cfb * h.n_cfc * h.n_cfp * 2;
+    NGC_TRY(up("uv_scale", scale.data(), scale.size() * sizeof(double), &d.scale));
+    NGC_TRY(up("chan_map", padded_map(h.chan_map, h.n_chan).data(),
+               h.n_chan * sizeof(int32_t) + 4, &d.cmap));
+    NGC_TRY(up("pol_map", h.pol_map, h.n_pol * sizeof(int32_t), &d.pmap));
+    NGC_TRY(up("cf_bl_map", padded_map(h.cf_baseline_map, h.n_baseline).data(),
+               h.n_baseline * sizeof(int32_t) + 4, &d.cfb));
+    NGC_TRY(up("cf_chan_map", padded_map(h.cf_chan_map, h.n_chan).data(),
+               h.n_chan * sizeof(int32_t) + 4, &d.cfc));
+    NGC_TRY(up("cf_pol_map", h.cf_pol_map, h.n_pol * sizeof(int32_t), &d.cfp));
+    NGC_TRY(up("support", h.support, n_sup * sizeof(int32_t), &d.sup));
+    NGC_TRY(up("field_id", h.field_id, h.n_field * sizeof(int32_t), &d.fid));
+    return NGC_OK;
+}
+
+ApTiledCall ap_tiled_call(const ApHost &h, const ApDevTables &d, int64_t ms) {
+    bool pol_identity = h.n_imag_pol == h.n_pol, cfp_uniform = true;
+    for (int64_t p = 0; p < h.n_pol; p++) {
+        pol_identity &= h.pol_map[p] == p;
+        cfp_uniform &= h.cf_pol_map[p] == h.cf_pol_map[0];
+    }
+    ApTiledCall c{};
+    c.mode = h.mode;
+    c.precision = h.precision;
+    c.uvw = h.uvw;
+    c.vis = h.vis;
+    c.vis_dtype = h.vis_dtype;
+    c.weight = h.weight;
+    c.weight_dtype = h.weight_dtype;
+    c.flag = h.flag;
+    c.field = h.field;
+    c.d_field_id = (const int *)d.fid;
+    c.n_field = h.n_field;
+    c.d_uv_scale = (const double *)d.scale;
+    c.d_chan_map = (const int *)d.cmap;
+    c.d_cfb = (const int *)d.cfb;
+    c.d_cfc = (const int *)d.cfc;
+    c.d_support = (const int *)d.sup;
+    c.n_cfc = h.n_cfc;
+    c.n_cfp = h.n_cfp;
+    c.cfp0 = h.cf_pol_map[0];
+    c.n_cf = h.n_cfb * h.n_cfc * h.n_cfp;
+    c.conv_kernel = h.conv_kernel;
+    c.phase_gradient = h.phase_gradient;
+    c.k_u = h.k_u;
+    c.k_v = h.k_v;
+    c.n_time = h.n_time;
+    c.n_baseline = h.n_baseline;
+    c.n_chan = h.n_chan;
+    c.n_pol = h.n_pol;
+    c.n_imag_chan = h.n_imag_chan;
+    c.n_imag_pol = h.n_imag_pol;
+    c.n_u = h.n_u;
+    c.n_v = h.n_v;
+    c.os_u = h.os_u;
+    c.os_v = h.os_v;
+    c.max_support = ms;
+    c.pol_identity = pol_identity;
+    c.cfp_uniform = cfp_uniform;
+    c.accumulate = h.accumulate;
+    c.grid = h.grid;
+    c.sum_weight = h.sum_weight;
+    return c;
+}
+
+}  // namespace
+
+#define NGC_AP_HOST_ARGS                                                                      \
+    ApHost h{mode,        precision,   uvw,          vis,         vis_dtype,    weight,        \
+             weight_dtype, flag,       field,        field_id,    n_field,      freq_chan,     \
+             chan_map,    pol_map,     cf_baseline_map, cf_chan_map, cf_pol_map, conv_kernel, \
+             n_cfb,       n_cfc,       n_cfp,        k_u,         k_v,          support,       \
+             phase_gradient, n_time,   n_baseline,   n_chan,      n_pol,        n_imag_chan,   \
+             n_imag_pol,  n_u,         n_v,          delta_l,     delta_m,      os_u,          \
+             os_v,        accumulate,  grid,         sum_weight}
+
 extern "C" int ngc_aperture_grid(
     ngc_context *ctx, void *stream, int mode, int precision, const double *uvw,
     const void *vis, int vis_dtype, const void *weight, int weight_dtype, const uint8_t *flag,
@@ -150,112 +319,20 @@ extern "C" int ngc_aperture_grid(
     int64_t n_v, double delta_l, double delta_m, int64_t os_u, int64_t os_v, int accumulate,
     void *grid, double *sum_weight) {
     NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
-    NGC_CHECK(mode == NGC_GRID_IMAGE || mode == NGC_GRID_PSF || mode == NGC_GRID_WEIGHT,
-              NGC_ERR_ARG, "mode must be IMAGE, PSF or WEIGHT");
-    NGC_CHECK(precision >= NGC_PREC_FP64 && precision <= NGC_PREC_FP32, NGC_ERR_ARG,
-              "unknown precision");
-    const int64_t n_samples = n_time * n_baseline * n_chan;
-    NGC_CHECK(field_id && freq_chan && chan_map && pol_map && cf_baseline_map && cf_chan_map &&
-                  cf_pol_map && conv_kernel && support && phase_gradient && grid && sum_weight,
-              NGC_ERR_ARG, "NULL array");
-    NGC_CHECK((uvw && weight && field) || n_samples == 0, NGC_ERR_ARG,
-              "NULL uvw, weight or field");
-    NGC_CHECK(mode != NGC_GRID_IMAGE || vis || n_samples == 0, NGC_ERR_ARG, "image mode needs vis");
-    NGC_CHECK(n_field >= 1 && n_cfb >= 1 && n_cfc >= 1 && n_cfp >= 1 && k_u >= 1 && k_v >= 1,
-              NGC_ERR_ARG, "bad kernel dimensions");
-    NGC_CHECK(os_u >= 1 && os_v >= 1, NGC_ERR_ARG, "oversampling must be >= 1");
-    for (int64_t b = 0; b < n_baseline; b++)
-        NGC_CHECK(cf_baseline_map[b] >= 0 && cf_baseline_map[b] < n_cfb, NGC_ERR_ARG,
-                  "cf_baseline_map range");
-    for (int64_t c = 0; c < n_chan; c++) {
-        NGC_CHECK(cf_chan_map[c] >= 0 && cf_chan_map[c] < n_cfc, NGC_ERR_ARG, "cf_chan_map range");
-        NGC_CHECK(chan_map[c] >= 0 && chan_map[c] < n_imag_chan, NGC_ERR_ARG, "chan_map range");
-    }
-    for (int64_t p = 0; p < n_pol; p++) {
-        NGC_CHECK(cf_pol_map[p] >= 0 && cf_pol_map[p] < n_cfp, NGC_ERR_ARG, "cf_pol_map range");
-        NGC_CHECK(pol_map[p] >= 0 && pol_map[p] < n_imag_pol, NGC_ERR_ARG, "pol_map range");
-    }
-    // max(SUPPORT) is the half-width of the reference's bounds check
-    // (_aperture_grid.py:397,447); every tap must then index inside the kernel
-    int64_t ms = support[0];
-    const int64_t n_sup = n_cfb * n_cfc * n_cfp * 2;
-    for (int64_t i = 1; i < n_sup; i++) ms = support[i] > ms ? support[i] : ms;
-    for (int64_t i = 0; i < n_sup; i += 2) {
-        const int64_t su = support[i], sv = support[i + 1];
-        NGC_CHECK(su >= 1 && sv >= 1, NGC_ERR_ARG, "support must be >= 1");
-        const int64_t reach_u = os_u * (su - su / 2 - 1) + os_u / 2 + 1;
-        const int64_t reach_v = os_v * (sv - sv / 2 - 1) + os_v / 2 + 1;
-        const int64_t low_u = os_u * (su / 2) + os_u / 2 + 1;
-        const int64_t low_v = os_v * (sv / 2) + os_v / 2 + 1;
-        NGC_CHECK(k_u / 2 + reach_u < k_u && k_u / 2 - low_u >= 0 && k_v / 2 + reach_v < k_v &&
-                      k_v / 2 - low_v >= 0,
-                  NGC_ERR_ARG, "support x oversampling exceeds the kernel size");
-    }
+    const NGC_AP_HOST_ARGS;
+    int64_t ms = 0;
+    NGC_TRY(ap_validate(h, &ms));
     std::lock_guard<std::mutex> lock(ctx->mu);
     hipStream_t s = (hipStream_t)stream;
     NGC_TRY(enter_stream(ctx, s));
-    std::vector<double> scale;
-    uv_scale_host(freq_chan, n_chan, delta_l, delta_m, n_u, n_v, scale);
-    void *d_scale, *d_cmap, *d_pmap, *d_cfb, *d_cfc, *d_cfp, *d_sup, *d_fid;
-    NGC_TRY(upload(ctx, s, "uv_scale", scale.data(), scale.size() * sizeof(double), &d_scale));
-    NGC_TRY(upload(ctx, s, "chan_map", padded_map(chan_map, n_chan).data(), n_chan * sizeof(int32_t) + 4, &d_cmap));
-    NGC_TRY(upload(ctx, s, "pol_map", pol_map, n_pol * sizeof(int32_t), &d_pmap));
-    NGC_TRY(upload(ctx, s, "cf_bl_map", padded_map(cf_baseline_map, n_baseline).data(),
-                   n_baseline * sizeof(int32_t) + 4, &d_cfb));
-    NGC_TRY(upload(ctx, s, "cf_chan_map", padded_map(cf_chan_map, n_chan).data(),
-                   n_chan * sizeof(int32_t) + 4, &d_cfc));
-    NGC_TRY(upload(ctx, s, "cf_pol_map", cf_pol_map, n_pol * sizeof(int32_t), &d_cfp));
-    NGC_TRY(upload(ctx, s, "support", support, n_sup * sizeof(int32_t), &d_sup));
-    NGC_TRY(upload(ctx, s, "field_id", field_id, n_field * sizeof(int32_t), &d_fid));
+    ApDevTables d{};
+    NGC_TRY(ap_upload_tables(h, [&](const char *name, const void *host, size_t bytes, void **dev) {
+        return upload(ctx, s, name, host, bytes, dev);
+    }, d));
     const int64_t cells = n_imag_chan * n_imag_pol * n_u * n_v;
     NGC_HIP(hipMemsetAsync(ctx->d_error, 0, sizeof(int), s));
     if (precision != NGC_PREC_FP64) {
-        bool pol_identity = n_imag_pol == n_pol, cfp_uniform = true;
-        for (int64_t p = 0; p < n_pol; p++) {
-            pol_identity &= pol_map[p] == p;
-            cfp_uniform &= cf_pol_map[p] == cf_pol_map[0];
-        }
-        ApTiledCall c{};
-        c.mode = mode;
-        c.precision = precision;
-        c.uvw = uvw;
-        c.vis = vis;
-        c.vis_dtype = vis_dtype;
-        c.weight = weight;
-        c.weight_dtype = weight_dtype;
-        c.flag = flag;
-        c.field = field;
-        c.d_field_id = (const int *)d_fid;
-        c.n_field = n_field;
-        c.d_uv_scale = (const double *)d_scale;
-        c.d_chan_map = (const int *)d_cmap;
-        c.d_cfb = (const int *)d_cfb;
-        c.d_cfc = (const int *)d_cfc;
-        c.d_support = (const int *)d_sup;
-        c.n_cfc = n_cfc;
-        c.n_cfp = n_cfp;
-        c.cfp0 = cf_pol_map[0];
-        c.n_cf = n_cfb * n_cfc * n_cfp;
-        c.conv_kernel = conv_kernel;
-        c.phase_gradient = phase_gradient;
-        c.k_u = k_u;
-        c.k_v = k_v;
-        c.n_time = n_time;
-        c.n_baseline = n_baseline;
-        c.n_chan = n_chan;
-        c.n_pol = n_pol;
-        c.n_imag_chan = n_imag_chan;
-        c.n_imag_pol = n_imag_pol;
-        c.n_u = n_u;
-        c.n_v = n_v;
-        c.os_u = os_u;
-        c.os_v = os_v;
-        c.max_support = ms;
-        c.pol_identity = pol_identity;
-        c.cfp_uniform = cfp_uniform;
-        c.accumulate = accumulate;
-        c.grid = grid;
-        c.sum_weight = sum_weight;
+        const ApTiledCall c = ap_tiled_call(h, d, ms);
         profile_new_call(ctx, s);
         NGC_TRY(ap_tiled_run(ctx, s, c));
         int err = 0;
@@ -264,6 +341,8 @@ extern "C" int ngc_aperture_grid(
         NGC_CHECK(err == 0, NGC_ERR_FIELD, "a row's field id is not in field_id");
         return NGC_OK;
     }
+    void *d_scale = d.scale, *d_cmap = d.cmap, *d_pmap = d.pmap, *d_cfb = d.cfb, *d_cfc = d.cfc,
+         *d_cfp = d.cfp, *d_sup = d.sup, *d_fid = d.fid;
     if (!accumulate) {
         NGC_HIP(hipMemsetAsync(grid, 0, cells * 2 * sizeof(double), s));
         NGC_HIP(hipMemsetAsync(sum_weight, 0, n_imag_chan * n_imag_pol * sizeof(double), s));
@@ -321,5 +400,44 @@ extern "C" int ngc_aperture_grid(
     NGC_HIP(hipMemcpyAsync(&err, ctx->d_error, sizeof(int), hipMemcpyDeviceToHost, s));
     NGC_HIP(hipStreamSynchronize(s));
     NGC_CHECK(err == 0, NGC_ERR_FIELD, "a row's field id is not in field_id");
+    return NGC_OK;
+}
+
+extern "C" int ngc_aperture_grid_plan_create(
+    ngc_context *ctx, void *stream, int mode, int precision, const double *uvw,
+    const void *vis, int vis_dtype, const void *weight, int weight_dtype, const uint8_t *flag,
+    const int32_t *field, const int32_t *field_id, int64_t n_field, const double *freq_chan,
+    const int32_t *chan_map, const int32_t *pol_map, const int32_t *cf_baseline_map,
+    const int32_t *cf_chan_map, const int32_t *cf_pol_map, const double *conv_kernel,
+    int64_t n_cfb, int64_t n_cfc, int64_t n_cfp, int64_t k_u, int64_t k_v,
+    const int32_t *support, const double *phase_gradient, int64_t n_time, int64_t n_baseline,
+    int64_t n_chan, int64_t n_pol, int64_t n_imag_chan, int64_t n_imag_pol, int64_t n_u,
+    int64_t n_v, double delta_l, double delta_m, int64_t os_u, int64_t os_v, int accumulate,
+    void *grid, double *sum_weight, int autotune, ngc_grid_plan **plan) {
+    NGC_CHECK(ctx && plan, NGC_ERR_ARG, "ctx or plan is NULL");
+    const NGC_AP_HOST_ARGS;
+    int64_t ms = 0;
+    NGC_TRY(ap_validate(h, &ms));
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t s = (hipStream_t)stream;
+    NGC_TRY(enter_stream(ctx, s));
+    ngc_grid_plan *p = new ngc_grid_plan();
+    p->ctx = ctx;
+    ApDevTables d{};
+    int rc = ap_upload_tables(h, [&](const char *, const void *host, size_t bytes, void **dev) {
+        return plan_upload(p, s, host, bytes, dev);
+    }, d);
+    if (rc == NGC_OK) {
+        const ApTiledCall c = ap_tiled_call(h, d, ms);
+        // the autotuner's eager runs report field errors in the context's flag
+        (void)hipMemsetAsync(ctx->d_error, 0, sizeof(int), s);
+        rc = ap_plan_create(ctx, s, c, autotune, p);
+    }
+    if (rc != NGC_OK) {
+        (void)hipStreamSynchronize(s);
+        plan_free(p);
+        return rc;
+    }
+    *plan = p;
     return NGC_OK;
 }

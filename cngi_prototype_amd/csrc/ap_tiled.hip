@@ -39,6 +39,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <functional>
 #include <type_traits>
 
 #include "ngc_internal.hpp"
@@ -255,12 +256,14 @@ __device__ inline bool ap_place(const ApT &a, const ApSample &sm, int &fidx, Pix
 template <bool FILL, int MODE, int NP, typename ACC>
 __global__ void __launch_bounds__(kApBinBlock)
 ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *ntab,
-              double *sum_weight, int sw_in_lds) {
+              double *sum_weight, int sw_in_lds, const int *halt) {
 #pragma clang fp contract(off)
     constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
     using E = ApEntry<NP, NW, ACC>;
     constexpr int EB = ApEntrySize<NP, NW, ACC>::bytes;
     extern __shared__ double s_sw[];
+    // a plan whose buffers the inputs outgrew: nothing is written
+    if (FILL && halt && *halt) return;
     const int64_t n_sw = a.n_imag_chan * a.n_imag_pol;
     if (FILL && sw_in_lds) {
         for (int64_t i = threadIdx.x; i < n_sw; i += blockDim.x) s_sw[i] = 0.0;
@@ -395,7 +398,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                 const ACC *__restrict__ ckr, const Cplx<ACC> *__restrict__ pgu,
                 const Cplx<ACC> *__restrict__ pgv,
                 typename std::conditional<KAHAN, double, ACC>::type *__restrict__ scratch,
-                const int *__restrict__ slot) {
+                const int *__restrict__ slot, const int *__restrict__ halt) {
     constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
     constexpr int NACC = NP * 2;  // the grid is complex in every mode
     using E = ApEntry<NP, NW, ACC>;
@@ -448,7 +451,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     const int tid = threadIdx.x;
     const int64_t bin = order[blockIdx.x];
     const int64_t start = (int64_t)offsets[bin], end = (int64_t)offsets[bin + 1];
-    if (end <= start) return;
+    if (end <= start || (halt && *halt)) return;
     const int64_t tiles = a.ntu * a.ntv;
     const int64_t trem = bin % tiles;
     const int org_u = (int)((trem / a.ntv) * a.T) - a.hl;
@@ -715,7 +718,8 @@ template <typename SC, typename OUT>
 __global__ void ap_merge_kernel(ApT a, const int *__restrict__ nonempty,
                                 const int *__restrict__ slot, const int *__restrict__ tiles,
                                 const int *__restrict__ n_tiles_p, const SC *__restrict__ scratch,
-                                OUT *grid) {
+                                OUT *grid, const int *__restrict__ halt) {
+    if (halt && *halt) return;
     const int W = a.W, T = a.T;
     const int NACC = (int)a.n_imag_pol * 2;
     const int64_t per_tile = (int64_t)a.n_imag_pol * T * T;
@@ -856,14 +860,90 @@ template <typename T> __global__ void ap_zero_grid_kernel(T *p, int64_t n) {
         p[i] = (T)0;
 }
 
-__global__ void bin_size_kernel(const unsigned long long *offsets, int64_t n_bins,
-                                unsigned int *size, unsigned int *idx) {
+// Largest-first tile order without a sort (no library call, so plans can
+// capture it): tiles go into size classes of 1/8 octave, classes are laid out
+// in decreasing order, and each tile takes the next place of its class.
+constexpr int kApOrderClasses = 512;
+
+// 0 for an empty tile, else 1 + 8 floor(log2 n) + the three bits below n's
+// leading one (non-decreasing in n; a class spans at most 1/8 of its sizes)
+__device__ inline int ap_size_class(unsigned long long n) {
+    if (n == 0) return 0;
+    const int l = 63 - __clzll((long long)n);
+    const unsigned long long m = l >= 3 ? (n >> (l - 3)) & 7ull : (n << (3 - l)) & 7ull;
+    const int c = 1 + 8 * l + (int)m;
+    return c < kApOrderClasses ? c : kApOrderClasses - 1;
+}
+
+// hist[c] += tiles of class c (hist zeroed beforehand)
+__global__ void __launch_bounds__(256)
+ap_order_hist_kernel(const unsigned long long *offsets, int64_t n_bins, unsigned int *hist) {
+    __shared__ unsigned int s_n[kApOrderClasses];
+    for (int k = threadIdx.x; k < kApOrderClasses; k += blockDim.x) s_n[k] = 0;
+    __syncthreads();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_bins;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned long long n = offsets[i + 1] - offsets[i];
-        size[i] = n > 0xffffffffull ? 0xffffffffu : (unsigned int)n;
-        idx[i] = (unsigned int)i;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&s_n[ap_size_class(offsets[i + 1] - offsets[i])], 1u);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kApOrderClasses; k += blockDim.x)
+        if (s_n[k]) atomicAdd(&hist[k], s_n[k]);
+}
+
+// cursor[c] = first place of class c, the largest class first
+__global__ void ap_order_start_kernel(const unsigned int *hist, unsigned int *cursor) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        unsigned int pos = 0;
+        for (int c = kApOrderClasses - 1; c >= 0; c--) {
+            cursor[c] = pos;
+            pos += hist[c];
+        }
     }
+}
+
+__global__ void __launch_bounds__(256)
+ap_order_place_kernel(const unsigned long long *offsets, int64_t n_bins, unsigned int *cursor,
+                      unsigned int *order) {
+    __shared__ unsigned int s_n[kApOrderClasses];
+    __shared__ unsigned int s_base[kApOrderClasses];
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n_bins;
+         i0 += (int64_t)gridDim.x * blockDim.x) {
+        for (int k = threadIdx.x; k < kApOrderClasses; k += blockDim.x) s_n[k] = 0;
+        __syncthreads();
+        const int64_t i = i0 + threadIdx.x;
+        int c = 0;
+        unsigned int rank = 0;
+        if (i < n_bins) {
+            c = ap_size_class(offsets[i + 1] - offsets[i]);
+            rank = atomicAdd(&s_n[c], 1u);
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < kApOrderClasses; k += blockDim.x)
+            if (s_n[k]) s_base[k] = atomicAdd(&cursor[k], s_n[k]);
+        __syncthreads();
+        if (i < n_bins) order[s_base[c] + rank] = (unsigned int)i;
+        __syncthreads();
+    }
+}
+
+template <typename T> __global__ void ap_copy_kernel(T *dst, const T *src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+template <typename T> __global__ void ap_fill_kernel(T *p, int64_t n, T v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// A plan's buffers against the current inputs: *halt = 1 when they bin more
+// records, or fill more tiles, than the plan holds (the fill, tiled and merge
+// kernels then write nothing).
+__global__ void ap_cap_kernel(const unsigned long long *offsets, const int *slot, int64_t n_bins,
+                              unsigned long long ent_cap, int64_t slot_cap, int *halt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *halt = (offsets[n_bins] > ent_cap || (int64_t)slot[n_bins] > slot_cap) ? 1 : 0;
 }
 
 static int grid_blocks(int64_t n, int bs = 256) {
@@ -876,35 +956,43 @@ static int grid_blocks(int64_t n, int bs = 256) {
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
-// Bins in decreasing order of their sample counts (the longest-processing-
-// time-first order: workgroups are dispatched roughly in block order, so the
-// heaviest tiles start first and the tail is short).
-int bin_order_desc(ngc_context *ctx, hipStream_t s, const unsigned long long *offsets,
-                   int64_t n_bins, const unsigned int **order) {
-    NGC_CHECK(n_bins < (int64_t(1) << 32), NGC_ERR_UNSUPPORTED, "too many bins");
-    void *d_k, *d_v, *d_ko, *d_vo, *d_tmp;
-    NGC_TRY(workspace(ctx, "order_keys", n_bins * 4, &d_k));
-    NGC_TRY(workspace(ctx, "order_vals", n_bins * 4, &d_v));
-    NGC_TRY(workspace(ctx, "order_keys_out", n_bins * 4, &d_ko));
-    NGC_TRY(workspace(ctx, "order_vals_out", n_bins * 4, &d_vo));
-    bin_size_kernel<<<grid_blocks(n_bins), 256, 0, s>>>(offsets, n_bins, (unsigned int *)d_k,
-                                                        (unsigned int *)d_v);
-    NGC_HIP(hipGetLastError());
-    size_t tmp = 0;
-    NGC_HIP(hipcub::DeviceRadixSort::SortPairsDescending(
-        nullptr, tmp, (const unsigned int *)d_k, (unsigned int *)d_ko, (const unsigned int *)d_v,
-        (unsigned int *)d_vo, (int)n_bins, 0, 32, s));
-    NGC_TRY(workspace(ctx, "order_tmp", tmp, &d_tmp));
-    NGC_HIP(hipcub::DeviceRadixSort::SortPairsDescending(
-        d_tmp, tmp, (const unsigned int *)d_k, (unsigned int *)d_ko, (const unsigned int *)d_v,
-        (unsigned int *)d_vo, (int)n_bins, 0, 32, s));
-    *order = (const unsigned int *)d_vo;
-    return NGC_OK;
-}
+// Sizes of one call.
+struct ApGeom {
+    bool single = false, kahan = false;
+    int acc_bytes = 8, nwu = 0, nwv = 0;
+    int64_t n_sw = 0, n_rc = 0, cells = 0, n_ntab = 0;
+    // image and PSF modes
+    int T = 0;
+    size_t lds = 0;
+    int64_t n_bins = 0, n_ckr = 0, n_pg = 0, n_waves = 0, bin_blocks = 1;
+};
 
-int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
-    const bool single = c.precision != NGC_PREC_FP64_TILED;
-    const bool kahan = c.precision == NGC_PREC_FP32_KAHAN;
+// Buffers of the tiled pipeline: the context's workspace for one call, owned
+// allocations for a plan.
+struct ApBufs {
+    double2 *ntab = nullptr;
+    void *ckr = nullptr, *pgu = nullptr, *pgv = nullptr;
+    unsigned long long *counts = nullptr, *off = nullptr;
+    void *scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    int *ne = nullptr, *near = nullptr, *slot = nullptr, *npos = nullptr, *tiles = nullptr;
+    void *itmp = nullptr;
+    size_t itmp_bytes = 0;
+    unsigned int *ord_hist = nullptr, *ord_cursor = nullptr, *order = nullptr;
+    char *ent = nullptr;
+    unsigned long long ent_cap = 0;  // records the entry buffer holds
+    void *scr = nullptr;
+    int64_t slot_cap = 0;            // scratch slots
+    int *halt = nullptr;             // a plan's capacity flag (nullptr for a call)
+};
+
+using ApAlloc = std::function<int(const char *, size_t, void **)>;
+
+// Validates the call, fills the kernel arguments and the sizes; the tile edge
+// is T_req (16, 24 or 32) or, for 0, the one whose workgroup LDS lets the most
+// workgroups share a CU (the kernel is latency-bound), the larger on a tie
+// (fewer halo cells to write out and merge).
+static int ap_setup(const ApTiledCall &c, int T_req, ApT &a, ApGeom &g) {
     NGC_CHECK(c.pol_identity, NGC_ERR_UNSUPPORTED,
               "tiled aperture gridder needs pol_map = arange(n_pol)");
     NGC_CHECK(c.cfp_uniform, NGC_ERR_UNSUPPORTED,
@@ -917,8 +1005,9 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     NGC_CHECK(c.n_cf < 65536 && c.n_field < 65536, NGC_ERR_UNSUPPORTED,
               "too many convolution functions or fields");
     NGC_CHECK(c.os_u < 16384 && c.os_v < 16384, NGC_ERR_UNSUPPORTED, "oversampling too large");
+    NGC_CHECK(T_req == 0 || T_req == 16 || T_req == 24 || T_req == 32, NGC_ERR_ARG,
+              "tile edge must be 16, 24 or 32");
 
-    ApT a{};
     a.uvw = c.uvw;
     a.vis = c.vis;
     a.weight = c.weight;
@@ -953,7 +1042,6 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     a.half_v = (int)(c.os_v / 2 + 1);
     a.vis_dtype = c.vis_dtype;
     a.weight_dtype = c.weight_dtype;
-    a.error = ctx->d_error;
     int ch_log2 = 0;
     while ((1 << (ch_log2 + 1)) <= std::min<int64_t>(c.n_chan, 2)) ch_log2++;
     a.ch_log2 = ch_log2;
@@ -964,243 +1052,208 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     a.div_cb = FastDiv::of((uint32_t)a.n_cb);
     a.div_tb = FastDiv::of((uint32_t)a.n_tb);
 
-    const int64_t n_sw = c.n_imag_chan * c.n_imag_pol;
-    const int64_t n_rc = c.n_time * c.n_baseline * c.n_chan;
-    const int64_t cells = n_sw * c.n_u * c.n_v;
-    const int nwu = 2 * a.half_u + 1, nwv = 2 * a.half_v + 1;
+    g.single = c.precision != NGC_PREC_FP64_TILED;
+    g.kahan = c.precision == NGC_PREC_FP32_KAHAN;
+    g.acc_bytes = g.single ? 4 : 8;
+    g.n_sw = c.n_imag_chan * c.n_imag_pol;
+    g.n_rc = c.n_time * c.n_baseline * c.n_chan;
+    g.cells = g.n_sw * c.n_u * c.n_v;
+    g.nwu = 2 * a.half_u + 1;
+    g.nwv = 2 * a.half_v + 1;
+    g.n_ntab = c.n_cf * c.n_field * g.nwu * g.nwv;
+    if (c.mode == NGC_GRID_WEIGHT) return NGC_OK;
 
-    // FP64 tap-sum table (sum_weight) for every mode
-    void *d_ntab;
-    const int64_t n_ntab = c.n_cf * c.n_field * nwu * nwv;
-    NGC_TRY(workspace(ctx, "ap_ntab", n_ntab * sizeof(double2), &d_ntab));
-    if (!c.accumulate) {
-        ap_zero_kernel<<<grid_blocks(n_sw), 256, 0, s>>>(c.sum_weight, n_sw);
-        NGC_HIP(hipGetLastError());
-    }
-    phase_begin(ctx, s, "ap_tables");
-    ap_norm_kernel<<<grid_blocks(n_ntab), 256, 0, s>>>(a, c.conv_kernel,
-                                                       (const double2 *)c.phase_gradient,
-                                                       (double2 *)d_ntab);
-    NGC_HIP(hipGetLastError());
-    phase_end(ctx, s);
-
-    if (c.mode == NGC_GRID_WEIGHT) {
-        const int64_t n_keys = n_sw * c.n_cf * c.n_field;
-        void *d_w;
-        NGC_TRY(workspace(ctx, "ap_wkey", n_keys * sizeof(double), &d_w));
-        ap_zero_kernel<<<grid_blocks(n_keys), 256, 0, s>>>((double *)d_w, n_keys);
-        NGC_HIP(hipGetLastError());
-        const int in_lds = n_keys <= 4096 ? 1 : 0;
-        phase_begin(ctx, s, "ap_weight_sum");
-        if (n_rc > 0) {
-            ap_wsum_kernel<<<grid_blocks(n_rc, 256) > 4096 ? 4096 : grid_blocks(n_rc, 256), 256,
-                             in_lds ? n_keys * sizeof(double) : 0, s>>>(
-                a, (const double2 *)d_ntab, (double *)d_w, c.sum_weight, in_lds, n_keys);
-            NGC_HIP(hipGetLastError());
-        }
-        phase_end(ctx, s);
-        phase_begin(ctx, s, "ap_weight_stamp");
-        if (single) {
-            if (!c.accumulate) {
-                ap_zero_grid_kernel<float><<<grid_blocks(2 * cells), 256, 0, s>>>(
-                    (float *)c.grid, 2 * cells);
-                NGC_HIP(hipGetLastError());
-            }
-            ap_wstamp_kernel<float><<<grid_blocks(n_sw * a.R * a.R), 256, 0, s>>>(
-                a, c.conv_kernel, (const double2 *)c.phase_gradient, (const double *)d_w,
-                (float *)c.grid, c.accumulate);
-        } else {
-            if (!c.accumulate) {
-                ap_zero_grid_kernel<double><<<grid_blocks(2 * cells), 256, 0, s>>>(
-                    (double *)c.grid, 2 * cells);
-                NGC_HIP(hipGetLastError());
-            }
-            ap_wstamp_kernel<double><<<grid_blocks(n_sw * a.R * a.R), 256, 0, s>>>(
-                a, c.conv_kernel, (const double2 *)c.phase_gradient, (const double *)d_w,
-                (double *)c.grid, c.accumulate);
-        }
-        NGC_HIP(hipGetLastError());
-        phase_end(ctx, s);
-        return NGC_OK;
-    }
-
-    // tile edge: of 32, 24 and 16, the one whose workgroup LDS lets the most
-    // workgroups share a CU (the kernel is latency-bound), the larger on a tie
-    // (fewer halo cells to write out and merge)
-    const int acc_bytes = single ? 4 : 8;
-    const int nacc = 2 * (int)c.n_pol;
     const int nw = c.mode == NGC_GRID_IMAGE ? 2 : 1;
     auto lds_for = [&](int T) {
-        return (size_t)ap_lds_layout(a.R, T + a.R - 1, (int)c.n_pol, nw, acc_bytes, kahan).bytes;
+        return (size_t)ap_lds_layout(a.R, T + a.R - 1, (int)c.n_pol, nw, g.acc_bytes, g.kahan)
+            .bytes;
     };
-    int T = 16;
-    size_t best_per_cu = 0;
-    for (int cand : {32, 24, 16}) {
-        const size_t per_cu = (160 * 1024) / lds_for(cand);
-        if (per_cu > best_per_cu) {
-            best_per_cu = per_cu;
-            T = cand;
+    int T = T_req;
+    if (T == 0) {
+        size_t best_per_cu = 0;
+        for (int cand : {32, 24, 16}) {
+            const size_t per_cu = (160 * 1024) / lds_for(cand);
+            if (per_cu > best_per_cu) {
+                best_per_cu = per_cu;
+                T = cand;
+            }
         }
     }
-    const size_t lds = lds_for(T);
-    NGC_CHECK(lds <= 160 * 1024, NGC_ERR_UNSUPPORTED, "aperture tile does not fit in LDS");
+    g.T = T;
+    g.lds = lds_for(T);
+    NGC_CHECK(g.lds <= 160 * 1024, NGC_ERR_UNSUPPORTED, "aperture tile does not fit in LDS");
     a.T = T;
     a.W = T + a.R - 1;
     a.ntu = ceil_div(c.n_u, T);
     a.ntv = ceil_div(c.n_v, T);
-    const int64_t n_bins = c.n_imag_chan * a.ntu * a.ntv;
-    NGC_CHECK(n_bins < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED, "too many tiles");
-
-    // re-laid taps and the separable phase-gradient factors
-    void *d_ckr, *d_pgu, *d_pgv;
-    const int64_t n_ckr = c.n_cf * nwu * nwv * a.R * a.R;
+    g.n_bins = c.n_imag_chan * a.ntu * a.ntv;
+    NGC_CHECK(g.n_bins < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED, "too many tiles");
+    g.n_ckr = c.n_cf * g.nwu * g.nwv * a.R * a.R;
     // the tiled kernel addresses the re-laid taps with 32-bit byte offsets
-    NGC_CHECK((n_ckr + 1) * acc_bytes < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED,
+    NGC_CHECK((g.n_ckr + 1) * g.acc_bytes < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED,
               "re-laid convolution kernels exceed 2 GiB");
-    NGC_TRY(workspace(ctx, "ap_ckr", (n_ckr + 1) * acc_bytes, &d_ckr));
     // the tiled kernel indexes the factor tables with 32-bit integers
     NGC_CHECK(c.n_field * c.k_u < (int64_t(1) << 31) && c.n_field * c.k_v < (int64_t(1) << 31),
               NGC_ERR_UNSUPPORTED, "phase-gradient factor tables exceed 2^31 entries");
-    NGC_TRY(workspace(ctx, "ap_pgu", c.n_field * c.k_u * 2 * acc_bytes, &d_pgu));
-    NGC_TRY(workspace(ctx, "ap_pgv", c.n_field * c.k_v * 2 * acc_bytes, &d_pgv));
-    const int64_t n_pg = c.n_field * c.k_u * c.k_v;
-    NGC_HIP(hipMemsetAsync(ctx->d_error + 1, 0, sizeof(int), s));
-    phase_begin(ctx, s, "ap_tables");
-    if (single) {
-        ap_taps_kernel<float><<<grid_blocks(n_ckr + 1), 256, 0, s>>>(a, c.conv_kernel,
-                                                                 (float *)d_ckr);
-        ap_pg_kernel<float2><<<grid_blocks(n_pg), 256, 0, s>>>(
-            a, (const double2 *)c.phase_gradient, (float2 *)d_pgu, (float2 *)d_pgv,
-            ctx->d_error + 1);
+    g.n_pg = c.n_field * c.k_u * c.k_v;
+    g.n_waves = c.n_baseline * a.n_tb * a.n_cb;
+    g.bin_blocks = std::max<int64_t>(
+        1, std::min<int64_t>(ceil_div(g.n_waves, kApBinBlock / 64), 1 << 20));
+    return NGC_OK;
+}
+
+// Every buffer of the image/PSF pipeline except the entries and the scratch
+// slots (sized from the inputs' counts).
+static int ap_alloc(const ApTiledCall &c, const ApGeom &g, const ApAlloc &alloc, ApBufs &b) {
+    void *q;
+    const int64_t nb1 = g.n_bins + 1;
+    NGC_TRY(alloc("ap_ntab", g.n_ntab * sizeof(double2), &q));
+    b.ntab = (double2 *)q;
+    NGC_TRY(alloc("ap_ckr", (g.n_ckr + 1) * g.acc_bytes, &b.ckr));
+    NGC_TRY(alloc("ap_pgu", c.n_field * c.k_u * 2 * g.acc_bytes, &b.pgu));
+    NGC_TRY(alloc("ap_pgv", c.n_field * c.k_v * 2 * g.acc_bytes, &b.pgv));
+    NGC_TRY(alloc("ap_bin_counts", nb1 * sizeof(unsigned long long), &q));
+    b.counts = (unsigned long long *)q;
+    NGC_TRY(alloc("ap_bin_offsets", nb1 * sizeof(unsigned long long), &q));
+    b.off = (unsigned long long *)q;
+    NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b.scan_tmp_bytes, b.counts, b.off, nb1));
+    NGC_TRY(alloc("ap_scan_tmp", b.scan_tmp_bytes, &b.scan_tmp));
+    int **ints[] = {&b.ne, &b.near, &b.slot, &b.npos, &b.tiles};
+    const char *names[] = {"ap_nonempty", "ap_near", "ap_slot", "ap_near_pos", "ap_tiles"};
+    for (int i = 0; i < 5; i++) {
+        NGC_TRY(alloc(names[i], nb1 * sizeof(int), &q));
+        *ints[i] = (int *)q;
+    }
+    NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b.itmp_bytes, (const int *)b.ne, b.slot,
+                                             nb1));
+    NGC_TRY(alloc("ap_int_scan_tmp", b.itmp_bytes, &b.itmp));
+    NGC_TRY(alloc("ap_order_hist", 2 * kApOrderClasses * sizeof(unsigned int), &q));
+    b.ord_hist = (unsigned int *)q;
+    b.ord_cursor = b.ord_hist + kApOrderClasses;
+    NGC_TRY(alloc("ap_order", g.n_bins * sizeof(unsigned int) + 4, &q));
+    b.order = (unsigned int *)q;
+    return NGC_OK;
+}
+
+// The re-laid taps and the separable phase-gradient factors; *bad is set when
+// the phase gradient does not factor.
+static int ap_enqueue_tables(hipStream_t s, const ApTiledCall &c, const ApT &a, const ApGeom &g,
+                             const ApBufs &b, int *bad) {
+    if (g.single) {
+        ap_taps_kernel<float><<<grid_blocks(g.n_ckr + 1), 256, 0, s>>>(a, c.conv_kernel,
+                                                                   (float *)b.ckr);
+        ap_pg_kernel<float2><<<grid_blocks(g.n_pg), 256, 0, s>>>(
+            a, (const double2 *)c.phase_gradient, (float2 *)b.pgu, (float2 *)b.pgv, bad);
     } else {
-        ap_taps_kernel<double><<<grid_blocks(n_ckr + 1), 256, 0, s>>>(a, c.conv_kernel,
-                                                                  (double *)d_ckr);
-        ap_pg_kernel<double2><<<grid_blocks(n_pg), 256, 0, s>>>(
-            a, (const double2 *)c.phase_gradient, (double2 *)d_pgu, (double2 *)d_pgv,
-            ctx->d_error + 1);
+        ap_taps_kernel<double><<<grid_blocks(g.n_ckr + 1), 256, 0, s>>>(a, c.conv_kernel,
+                                                                    (double *)b.ckr);
+        ap_pg_kernel<double2><<<grid_blocks(g.n_pg), 256, 0, s>>>(
+            a, (const double2 *)c.phase_gradient, (double2 *)b.pgu, (double2 *)b.pgv, bad);
+    }
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+// Count pass, bin offsets, the non-empty tiles' scratch slots and the list of
+// tiles the merge covers.  No host synchronisation (captured by plans).
+template <int MODE, int NP, typename ACC>
+static int ap_enqueue_count(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, const ApT &a,
+                            const ApGeom &g, const ApBufs &b) {
+    const int64_t nb1 = g.n_bins + 1;
+    ap_fill_kernel<unsigned long long><<<grid_blocks(nb1), 256, 0, s>>>(b.counts, nb1, 0ull);
+    NGC_HIP(hipGetLastError());
+    phase_begin(ctx, s, "ap_bin_count");
+    if (g.n_waves > 0) {
+        ap_bin_kernel<false, MODE, NP, ACC><<<(int)g.bin_blocks, kApBinBlock, 0, s>>>(
+            a, b.counts, nullptr, b.ntab, c.sum_weight, 0, nullptr);
+        NGC_HIP(hipGetLastError());
+    }
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "ap_bin_scan");
+    size_t tb = b.scan_tmp_bytes;
+    NGC_HIP(hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, tb, b.counts, b.off, nb1, s));
+    phase_end(ctx, s);
+    ap_flags_kernel<<<grid_blocks(nb1), 256, 0, s>>>(a, b.off, g.n_bins, b.ne, b.near);
+    NGC_HIP(hipGetLastError());
+    size_t it = b.itmp_bytes;
+    NGC_HIP(hipcub::DeviceScan::ExclusiveSum(b.itmp, it, (const int *)b.ne, b.slot, nb1, s));
+    it = b.itmp_bytes;
+    NGC_HIP(hipcub::DeviceScan::ExclusiveSum(b.itmp, it, (const int *)b.near, b.npos, nb1, s));
+    ap_list_kernel<<<grid_blocks(g.n_bins), 256, 0, s>>>(g.n_bins, (const int *)b.near,
+                                                         (const int *)b.npos, b.tiles);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+// Fill pass, largest-first tile order (the longest-processing-time-first
+// order: workgroups are dispatched roughly in block order, so the heaviest
+// tiles start first and the tail is short), tiled gridding and merge.  No
+// host synchronisation; set_attr = false while a plan is being captured (the
+// LDS attribute is set before the capture).
+template <int MODE, int NP, typename ACC, bool KAHAN>
+static int ap_enqueue_fill(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, const ApT &a,
+                           const ApGeom &g, const ApBufs &b, bool set_attr) {
+    using SC = typename std::conditional<KAHAN, double, ACC>::type;
+    const int64_t nb1 = g.n_bins + 1;
+    const int sw_lds = g.n_sw <= 4096 ? 1 : 0;
+    const size_t sw_shmem = sw_lds ? g.n_sw * sizeof(double) : 0;
+    ap_copy_kernel<unsigned long long><<<grid_blocks(nb1), 256, 0, s>>>(b.counts, b.off, nb1);
+    NGC_HIP(hipGetLastError());
+    phase_begin(ctx, s, "ap_bin_fill");
+    if (g.n_waves > 0) {
+        ap_bin_kernel<true, MODE, NP, ACC><<<(int)g.bin_blocks, kApBinBlock, sw_shmem, s>>>(
+            a, b.counts, b.ent, b.ntab, c.sum_weight, sw_lds, b.halt);
+        NGC_HIP(hipGetLastError());
+    }
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "ap_bin_order");
+    ap_fill_kernel<unsigned int><<<1, 256, 0, s>>>(b.ord_hist, kApOrderClasses, 0u);
+    ap_order_hist_kernel<<<grid_blocks(g.n_bins), 256, 0, s>>>(b.off, g.n_bins, b.ord_hist);
+    ap_order_start_kernel<<<1, 64, 0, s>>>(b.ord_hist, b.ord_cursor);
+    ap_order_place_kernel<<<grid_blocks(g.n_bins), 256, 0, s>>>(b.off, g.n_bins, b.ord_cursor,
+                                                                b.order);
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "ap_grid_tiled");
+    const int threads = (a.R * a.R + 63) / 64 * 64;
+    auto kern = ap_tiled_kernel<MODE, NP, ACC, KAHAN>;
+    if (set_attr)
+        NGC_HIP(hipFuncSetAttribute((const void *)kern,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.lds));
+    kern<<<(unsigned)g.n_bins, threads, g.lds, s>>>(
+        a, b.off, b.order, (const char *)b.ent, (const ACC *)b.ckr, (const Cplx<ACC> *)b.pgu,
+        (const Cplx<ACC> *)b.pgv, (SC *)b.scr, (const int *)b.slot, b.halt);
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "ap_merge");
+    const int64_t merge_items = g.n_bins * c.n_imag_pol * g.T * g.T;
+    const int *ntp = (const int *)b.npos + g.n_bins;  // number of listed tiles
+    if (g.single) {
+        if (!c.accumulate)
+            ap_zero_grid_kernel<float><<<grid_blocks(2 * g.cells), 256, 0, s>>>(
+                (float *)c.grid, 2 * g.cells);
+        ap_merge_kernel<SC, float><<<grid_blocks(merge_items), 256, 0, s>>>(
+            a, (const int *)b.ne, (const int *)b.slot, (const int *)b.tiles, ntp,
+            (const SC *)b.scr, (float *)c.grid, b.halt);
+    } else {
+        if (!c.accumulate)
+            ap_zero_grid_kernel<double><<<grid_blocks(2 * g.cells), 256, 0, s>>>(
+                (double *)c.grid, 2 * g.cells);
+        ap_merge_kernel<SC, double><<<grid_blocks(merge_items), 256, 0, s>>>(
+            a, (const int *)b.ne, (const int *)b.slot, (const int *)b.tiles, ntp,
+            (const SC *)b.scr, (double *)c.grid, b.halt);
     }
     NGC_HIP(hipGetLastError());
     phase_end(ctx, s);
-    int bad = 0;
-    NGC_HIP(hipMemcpyAsync(&bad, ctx->d_error + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    NGC_HIP(hipStreamSynchronize(s));
-    NGC_CHECK(bad == 0, NGC_ERR_UNSUPPORTED,
-              "tiled aperture gridder needs a phase gradient of the form exp(i(x px + y py))");
+    return NGC_OK;
+}
 
-    void *d_counts, *d_off, *d_scan_tmp, *d_ent, *d_scr;
-    NGC_TRY(workspace(ctx, "ap_bin_counts", (n_bins + 1) * sizeof(unsigned long long),
-                      &d_counts));
-    NGC_TRY(workspace(ctx, "ap_bin_offsets", (n_bins + 1) * sizeof(unsigned long long), &d_off));
-    NGC_HIP(hipMemsetAsync(d_counts, 0, (n_bins + 1) * sizeof(unsigned long long), s));
-    const int sw_lds = n_sw <= 4096 ? 1 : 0;
-    const size_t sw_shmem = sw_lds ? n_sw * sizeof(double) : 0;
-    const int64_t n_waves = c.n_baseline * a.n_tb * a.n_cb;
-    int64_t bin_blocks = ceil_div(n_waves, kApBinBlock / 64);
-    bin_blocks = std::max<int64_t>(1, std::min<int64_t>(bin_blocks, 1 << 20));
+template <int NP, int NW, typename ACC> constexpr int ap_entry_bytes() {
+    return ApEntrySize<NP, NW, ACC>::bytes;
+}
 
-    auto run = [&](auto mode_c, auto np_c, auto acc_c, auto kahan_c) -> int {
-        constexpr int MODE = decltype(mode_c)::value;
-        constexpr int NP = decltype(np_c)::value;
-        using ACC = typename decltype(acc_c)::type;
-        constexpr bool KAHAN = decltype(kahan_c)::value;
-        constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
-        constexpr int EB = ApEntrySize<NP, NW, ACC>::bytes;
-        using SC = typename std::conditional<KAHAN, double, ACC>::type;
-        unsigned long long *counts = (unsigned long long *)d_counts;
-        phase_begin(ctx, s, "ap_bin_count");
-        if (n_waves > 0) {
-            ap_bin_kernel<false, MODE, NP, ACC><<<(int)bin_blocks, kApBinBlock, 0, s>>>(
-                a, counts, nullptr, (const double2 *)d_ntab, c.sum_weight, 0);
-            NGC_HIP(hipGetLastError());
-        }
-        phase_end(ctx, s);
-        phase_begin(ctx, s, "ap_bin_scan");
-        size_t tmp_bytes = 0;
-        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts,
-                                                 (unsigned long long *)d_off, n_bins + 1, s));
-        NGC_TRY(workspace(ctx, "ap_scan_tmp", tmp_bytes, &d_scan_tmp));
-        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_scan_tmp, tmp_bytes, counts,
-                                                 (unsigned long long *)d_off, n_bins + 1, s));
-        phase_end(ctx, s);
-        // non-empty bins get scratch slots; tiles next to one are merged
-        void *d_ne, *d_near, *d_slot, *d_npos, *d_tiles, *d_itmp;
-        NGC_TRY(workspace(ctx, "ap_nonempty", (n_bins + 1) * sizeof(int), &d_ne));
-        NGC_TRY(workspace(ctx, "ap_near", (n_bins + 1) * sizeof(int), &d_near));
-        NGC_TRY(workspace(ctx, "ap_slot", (n_bins + 1) * sizeof(int), &d_slot));
-        NGC_TRY(workspace(ctx, "ap_near_pos", (n_bins + 1) * sizeof(int), &d_npos));
-        NGC_TRY(workspace(ctx, "ap_tiles", (n_bins + 1) * sizeof(int), &d_tiles));
-        ap_flags_kernel<<<grid_blocks(n_bins + 1), 256, 0, s>>>(
-            a, (const unsigned long long *)d_off, n_bins, (int *)d_ne, (int *)d_near);
-        NGC_HIP(hipGetLastError());
-        size_t itmp = 0;
-        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, itmp, (const int *)d_ne, (int *)d_slot,
-                                                 n_bins + 1, s));
-        NGC_TRY(workspace(ctx, "ap_int_scan_tmp", itmp, &d_itmp));
-        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_itmp, itmp, (const int *)d_ne, (int *)d_slot,
-                                                 n_bins + 1, s));
-        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(d_itmp, itmp, (const int *)d_near,
-                                                 (int *)d_npos, n_bins + 1, s));
-        ap_list_kernel<<<grid_blocks(n_bins), 256, 0, s>>>(n_bins, (const int *)d_near,
-                                                           (const int *)d_npos, (int *)d_tiles);
-        NGC_HIP(hipGetLastError());
-        unsigned long long total = 0;
-        int n_nonempty = 0;
-        NGC_HIP(hipMemcpyAsync(&total, (unsigned long long *)d_off + n_bins, sizeof(total),
-                               hipMemcpyDeviceToHost, s));
-        NGC_HIP(hipMemcpyAsync(&n_nonempty, (int *)d_slot + n_bins, sizeof(int),
-                               hipMemcpyDeviceToHost, s));
-        NGC_HIP(hipStreamSynchronize(s));
-        NGC_TRY(workspace(ctx, "ap_bin_entries", (size_t)(total + 1) * EB, &d_ent));
-        NGC_TRY(workspace(ctx, "ap_scratch",
-                          (size_t)(n_nonempty + 1) * 2 * NP * a.W * a.W * sizeof(SC), &d_scr));
-        NGC_HIP(hipMemcpyAsync(d_counts, d_off, (n_bins + 1) * sizeof(unsigned long long),
-                               hipMemcpyDeviceToDevice, s));
-        phase_begin(ctx, s, "ap_bin_fill");
-        if (n_waves > 0) {
-            ap_bin_kernel<true, MODE, NP, ACC><<<(int)bin_blocks, kApBinBlock, sw_shmem, s>>>(
-                a, counts, (char *)d_ent, (const double2 *)d_ntab, c.sum_weight, sw_lds);
-            NGC_HIP(hipGetLastError());
-        }
-        phase_end(ctx, s);
-        const int threads = (a.R * a.R + 63) / 64 * 64;
-        phase_begin(ctx, s, "ap_bin_order");
-        const unsigned int *order = nullptr;
-        NGC_TRY(bin_order_desc(ctx, s, (const unsigned long long *)d_off, n_bins, &order));
-        phase_end(ctx, s);
-        phase_begin(ctx, s, "ap_grid_tiled");
-        if (total > 0) {
-            auto kern = ap_tiled_kernel<MODE, NP, ACC, KAHAN>;
-            NGC_HIP(hipFuncSetAttribute((const void *)kern,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            kern<<<(unsigned)n_bins, threads, lds, s>>>(
-                a, (const unsigned long long *)d_off, order, (const char *)d_ent,
-                (const ACC *)d_ckr, (const Cplx<ACC> *)d_pgu, (const Cplx<ACC> *)d_pgv,
-                (SC *)d_scr, (const int *)d_slot);
-            NGC_HIP(hipGetLastError());
-        }
-        phase_end(ctx, s);
-        phase_begin(ctx, s, "ap_merge");
-        const int64_t merge_items = n_bins * c.n_imag_pol * T * T;
-        const int *ntp = (const int *)d_npos + n_bins;  // number of listed tiles
-        if (single) {
-            if (!c.accumulate)
-                ap_zero_grid_kernel<float><<<grid_blocks(2 * cells), 256, 0, s>>>(
-                    (float *)c.grid, 2 * cells);
-            ap_merge_kernel<SC, float><<<grid_blocks(merge_items), 256, 0, s>>>(
-                a, (const int *)d_ne, (const int *)d_slot, (const int *)d_tiles, ntp,
-                (const SC *)d_scr, (float *)c.grid);
-        } else {
-            if (!c.accumulate)
-                ap_zero_grid_kernel<double><<<grid_blocks(2 * cells), 256, 0, s>>>(
-                    (double *)c.grid, 2 * cells);
-            ap_merge_kernel<SC, double><<<grid_blocks(merge_items), 256, 0, s>>>(
-                a, (const int *)d_ne, (const int *)d_slot, (const int *)d_tiles, ntp,
-                (const SC *)d_scr, (double *)c.grid);
-        }
-        NGC_HIP(hipGetLastError());
-        phase_end(ctx, s);
-        return NGC_OK;
-    };
+// f(mode, pols, accumulator type, compensated) with compile-time arguments
+template <typename F> static int ap_dispatch(const ApTiledCall &c, F &&f) {
     using I0 = std::integral_constant<int, NGC_GRID_IMAGE>;
     using I1 = std::integral_constant<int, NGC_GRID_PSF>;
     struct F32 {
@@ -1212,9 +1265,9 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
     using KT = std::true_type;
     using KF = std::false_type;
     auto by_prec = [&](auto mode_c, auto np_c) -> int {
-        if (c.precision == NGC_PREC_FP32_KAHAN) return run(mode_c, np_c, F32{}, KT{});
-        if (c.precision == NGC_PREC_FP32) return run(mode_c, np_c, F32{}, KF{});
-        return run(mode_c, np_c, F64{}, KF{});
+        if (c.precision == NGC_PREC_FP32_KAHAN) return f(mode_c, np_c, F32{}, KT{});
+        if (c.precision == NGC_PREC_FP32) return f(mode_c, np_c, F32{}, KF{});
+        return f(mode_c, np_c, F64{}, KF{});
     };
     auto by_pol = [&](auto mode_c) -> int {
         switch (c.n_pol) {
@@ -1224,6 +1277,276 @@ int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
         }
     };
     return c.mode == NGC_GRID_IMAGE ? by_pol(I0{}) : by_pol(I1{});
+}
+
+static int ap_weight_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, const ApT &a,
+                         const ApGeom &g, const double2 *ntab) {
+    const int64_t n_keys = g.n_sw * c.n_cf * c.n_field;
+    void *d_w;
+    NGC_TRY(workspace(ctx, "ap_wkey", n_keys * sizeof(double), &d_w));
+    ap_zero_kernel<<<grid_blocks(n_keys), 256, 0, s>>>((double *)d_w, n_keys);
+    NGC_HIP(hipGetLastError());
+    const int in_lds = n_keys <= 4096 ? 1 : 0;
+    phase_begin(ctx, s, "ap_weight_sum");
+    if (g.n_rc > 0) {
+        ap_wsum_kernel<<<grid_blocks(g.n_rc, 256) > 4096 ? 4096 : grid_blocks(g.n_rc, 256), 256,
+                         in_lds ? n_keys * sizeof(double) : 0, s>>>(a, ntab, (double *)d_w,
+                                                                   c.sum_weight, in_lds, n_keys);
+        NGC_HIP(hipGetLastError());
+    }
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "ap_weight_stamp");
+    if (g.single) {
+        if (!c.accumulate) {
+            ap_zero_grid_kernel<float><<<grid_blocks(2 * g.cells), 256, 0, s>>>(
+                (float *)c.grid, 2 * g.cells);
+            NGC_HIP(hipGetLastError());
+        }
+        ap_wstamp_kernel<float><<<grid_blocks(g.n_sw * a.R * a.R), 256, 0, s>>>(
+            a, c.conv_kernel, (const double2 *)c.phase_gradient, (const double *)d_w,
+            (float *)c.grid, c.accumulate);
+    } else {
+        if (!c.accumulate) {
+            ap_zero_grid_kernel<double><<<grid_blocks(2 * g.cells), 256, 0, s>>>(
+                (double *)c.grid, 2 * g.cells);
+            NGC_HIP(hipGetLastError());
+        }
+        ap_wstamp_kernel<double><<<grid_blocks(g.n_sw * a.R * a.R), 256, 0, s>>>(
+            a, c.conv_kernel, (const double2 *)c.phase_gradient, (const double *)d_w,
+            (double *)c.grid, c.accumulate);
+    }
+    NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    return NGC_OK;
+}
+
+// One eager call with tile edge T_req (0: by occupancy).
+static int ap_tiled_run_t(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, int T_req) {
+    ApT a{};
+    ApGeom g{};
+    NGC_TRY(ap_setup(c, T_req, a, g));
+    a.error = ctx->d_error;
+    const ApAlloc ws = [ctx](const char *name, size_t bytes, void **q) {
+        return workspace(ctx, name, bytes, q);
+    };
+    if (!c.accumulate) {
+        ap_zero_kernel<<<grid_blocks(g.n_sw), 256, 0, s>>>(c.sum_weight, g.n_sw);
+        NGC_HIP(hipGetLastError());
+    }
+    ApBufs b{};
+    if (c.mode == NGC_GRID_WEIGHT) {
+        void *q;
+        NGC_TRY(ws("ap_ntab", g.n_ntab * sizeof(double2), &q));
+        b.ntab = (double2 *)q;
+    } else {
+        NGC_TRY(ap_alloc(c, g, ws, b));
+    }
+    // FP64 tap-sum table (sum_weight) for every mode
+    phase_begin(ctx, s, "ap_tables");
+    ap_norm_kernel<<<grid_blocks(g.n_ntab), 256, 0, s>>>(a, c.conv_kernel,
+                                                         (const double2 *)c.phase_gradient,
+                                                         b.ntab);
+    NGC_HIP(hipGetLastError());
+    if (c.mode == NGC_GRID_WEIGHT) {
+        phase_end(ctx, s);
+        return ap_weight_run(ctx, s, c, a, g, b.ntab);
+    }
+    NGC_HIP(hipMemsetAsync(ctx->d_error + 1, 0, sizeof(int), s));
+    NGC_TRY(ap_enqueue_tables(s, c, a, g, b, ctx->d_error + 1));
+    phase_end(ctx, s);
+    int bad = 0;
+    NGC_HIP(hipMemcpyAsync(&bad, ctx->d_error + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    NGC_HIP(hipStreamSynchronize(s));
+    NGC_CHECK(bad == 0, NGC_ERR_UNSUPPORTED,
+              "tiled aperture gridder needs a phase gradient of the form exp(i(x px + y py))");
+
+    return ap_dispatch(c, [&](auto mode_c, auto np_c, auto acc_c, auto kahan_c) -> int {
+        constexpr int MODE = decltype(mode_c)::value;
+        constexpr int NP = decltype(np_c)::value;
+        using ACC = typename decltype(acc_c)::type;
+        constexpr bool KAHAN = decltype(kahan_c)::value;
+        constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
+        using SC = typename std::conditional<KAHAN, double, ACC>::type;
+        NGC_TRY((ap_enqueue_count<MODE, NP, ACC>(ctx, s, c, a, g, b)));
+        unsigned long long total = 0;
+        int n_nonempty = 0;
+        NGC_HIP(hipMemcpyAsync(&total, b.off + g.n_bins, sizeof(total), hipMemcpyDeviceToHost,
+                               s));
+        NGC_HIP(hipMemcpyAsync(&n_nonempty, b.slot + g.n_bins, sizeof(int),
+                               hipMemcpyDeviceToHost, s));
+        NGC_HIP(hipStreamSynchronize(s));
+        void *q;
+        NGC_TRY(ws("ap_bin_entries", (size_t)(total + 1) * ap_entry_bytes<NP, NW, ACC>(), &q));
+        b.ent = (char *)q;
+        b.ent_cap = total + 1;
+        NGC_TRY(ws("ap_scratch", (size_t)(n_nonempty + 1) * 2 * NP * a.W * a.W * sizeof(SC),
+                   &b.scr));
+        b.slot_cap = n_nonempty + 1;
+        return ap_enqueue_fill<MODE, NP, ACC, KAHAN>(ctx, s, c, a, g, b, true);
+    });
+}
+
+int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c) {
+    return ap_tiled_run_t(ctx, s, c, 0);
+}
+
+static std::string ap_shape_key(const ApTiledCall &c) {
+    char buf[320];
+    snprintf(buf, sizeof(buf), "ap/%d/%d/%lld/%lld/%lld/%lld/%lld/%lld/%lld/%lld/%lld/%lld/%lld",
+             c.mode, c.precision, (long long)c.n_time, (long long)c.n_baseline,
+             (long long)c.n_chan, (long long)c.n_pol, (long long)c.n_imag_chan,
+             (long long)c.n_u, (long long)c.n_v, (long long)c.max_support, (long long)c.os_u,
+             (long long)c.os_v, (long long)c.n_cf);
+    return buf;
+}
+
+int ap_plan_create(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, int autotune,
+                   ngc_grid_plan *p) {
+    NGC_CHECK(c.mode == NGC_GRID_IMAGE || c.mode == NGC_GRID_PSF, NGC_ERR_UNSUPPORTED,
+              "gridding plans cover image and PSF modes");
+    NGC_CHECK(c.precision != NGC_PREC_FP64, NGC_ERR_UNSUPPORTED,
+              "gridding plans run the tiled forms (FP64_TILED, FP32_KAHAN, FP32)");
+    {
+        ApT a0{};
+        ApGeom g0{};
+        NGC_TRY(ap_setup(c, 0, a0, g0));
+    }
+    const bool saved_profile = ctx->profile;
+    ctx->profile = false;
+    struct Restore {
+        ngc_context *c;
+        bool v;
+        ~Restore() { c->profile = v; }
+    } restore{ctx, saved_profile};
+
+    // tile edge: timed on these inputs (the whole eager pipeline, one warm-up
+    // and one timed run per candidate) or taken from the context's cache
+    const std::string key = ap_shape_key(c);
+    int T = 0;
+    std::string tune_log;
+    auto it = ctx->tuned.find(key);
+    if (autotune) {
+        hipEvent_t e0, e1;
+        NGC_HIP(hipEventCreate(&e0));
+        NGC_HIP(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        for (int cand : {32, 24, 16}) {
+            ApT at{};
+            ApGeom gt{};
+            if (ap_setup(c, cand, at, gt) != NGC_OK) continue;  // e.g. the tile exceeds LDS
+            float ms = 0.f;
+            for (int rep = 0; rep < 2; rep++) {
+                NGC_HIP(hipEventRecord(e0, s));
+                const int rc = ap_tiled_run_t(ctx, s, c, cand);
+                if (rc != NGC_OK) {
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    return rc;
+                }
+                NGC_HIP(hipEventRecord(e1, s));
+                NGC_HIP(hipEventSynchronize(e1));
+                NGC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            }
+            char bb[48];
+            snprintf(bb, sizeof(bb), "%s%d:%.3f", tune_log.empty() ? "" : " ", cand, ms);
+            tune_log += bb;
+            if (ms < best_ms) {
+                best_ms = ms;
+                T = cand;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->tuned[key] = {T, 0};
+    } else if (it != ctx->tuned.end()) {
+        T = it->second.first;
+    }
+
+    ApT a{};
+    ApGeom g{};
+    NGC_TRY(ap_setup(c, T, a, g));
+    void *q;
+    // flags: [1] capacity exceeded, [3] a row's field id not in field_id,
+    // [4] the phase gradient does not factor
+    NGC_TRY(plan_alloc(p, 8 * sizeof(int), &q));
+    p->flags = (int *)q;
+    ap_fill_kernel<int><<<1, 64, 0, s>>>(p->flags, 8, 0);
+    NGC_HIP(hipGetLastError());
+    a.error = p->flags + 3;
+    const ApAlloc own = [p](const char *, size_t bytes, void **out) {
+        return plan_alloc(p, bytes, out);
+    };
+    ApBufs b{};
+    NGC_TRY(ap_alloc(c, g, own, b));
+    // the kernels' tables are built once, here: the plan reads conv_kernel and
+    // phase_gradient at creation only
+    ap_norm_kernel<<<grid_blocks(g.n_ntab), 256, 0, s>>>(a, c.conv_kernel,
+                                                         (const double2 *)c.phase_gradient,
+                                                         b.ntab);
+    NGC_HIP(hipGetLastError());
+    NGC_TRY(ap_enqueue_tables(s, c, a, g, b, p->flags + 4));
+    int bad = 0;
+    NGC_HIP(hipMemcpyAsync(&bad, p->flags + 4, sizeof(int), hipMemcpyDeviceToHost, s));
+    NGC_HIP(hipStreamSynchronize(s));
+    NGC_CHECK(bad == 0, NGC_ERR_UNSUPPORTED,
+              "tiled aperture gridder needs a phase gradient of the form exp(i(x px + y py))");
+
+    return ap_dispatch(c, [&](auto mode_c, auto np_c, auto acc_c, auto kahan_c) -> int {
+        constexpr int MODE = decltype(mode_c)::value;
+        constexpr int NP = decltype(np_c)::value;
+        using ACC = typename decltype(acc_c)::type;
+        constexpr bool KAHAN = decltype(kahan_c)::value;
+        constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
+        using SC = typename std::conditional<KAHAN, double, ACC>::type;
+        // buffers sized by one eager count pass, with room for the records and
+        // the non-empty tiles to grow by 1/16 (inputs rewritten in place)
+        NGC_TRY((ap_enqueue_count<MODE, NP, ACC>(ctx, s, c, a, g, b)));
+        unsigned long long total = 0;
+        int n_nonempty = 0;
+        NGC_HIP(hipMemcpyAsync(&total, b.off + g.n_bins, sizeof(total), hipMemcpyDeviceToHost,
+                               s));
+        NGC_HIP(hipMemcpyAsync(&n_nonempty, b.slot + g.n_bins, sizeof(int),
+                               hipMemcpyDeviceToHost, s));
+        NGC_HIP(hipStreamSynchronize(s));
+        b.ent_cap = total + total / 16 + 1024;
+        void *e;
+        NGC_TRY(plan_alloc(p, (size_t)b.ent_cap * ap_entry_bytes<NP, NW, ACC>(), &e));
+        b.ent = (char *)e;
+        b.slot_cap = std::min<int64_t>(g.n_bins, n_nonempty + n_nonempty / 16 + 64);
+        NGC_TRY(plan_alloc(p, (size_t)b.slot_cap * 2 * NP * a.W * a.W * sizeof(SC), &b.scr));
+        b.halt = p->flags + 1;
+        auto kern = ap_tiled_kernel<MODE, NP, ACC, KAHAN>;
+        NGC_HIP(hipFuncSetAttribute((const void *)kern,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.lds));
+        NGC_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        NGC_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        hipStream_t cs = p->cap_stream;
+        if (!c.accumulate)
+            ap_fill_kernel<double><<<grid_blocks(g.n_sw), 256, 0, cs>>>(c.sum_weight, g.n_sw,
+                                                                        0.0);
+        ap_fill_kernel<int><<<1, 64, 0, cs>>>(p->flags + 3, 1, 0);
+        int r = ap_enqueue_count<MODE, NP, ACC>(ctx, cs, c, a, g, b);
+        if (r == NGC_OK) {
+            ap_cap_kernel<<<1, 64, 0, cs>>>(b.off, b.slot, g.n_bins, b.ent_cap, b.slot_cap,
+                                            b.halt);
+            r = ap_enqueue_fill<MODE, NP, ACC, KAHAN>(ctx, cs, c, a, g, b, false);
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(cs, &graph);
+        p->graph = graph;
+        if (r != NGC_OK) return r;
+        NGC_CHECK(ec == hipSuccess && graph, NGC_ERR_HIP, "hipStreamEndCapture failed");
+        NGC_HIP(hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0));
+        char info[320];
+        snprintf(info, sizeof(info),
+                 "aperture: tile %dx%d, %d lanes per tile, %lld bins, %llu records (cap %llu), "
+                 "%d non-empty tiles (cap %lld)",
+                 g.T, g.T, a.R * a.R, (long long)g.n_bins, total, b.ent_cap, n_nonempty,
+                 (long long)b.slot_cap);
+        p->info = info;
+        if (!tune_log.empty()) p->info += "; tuning ms " + tune_log;
+        return NGC_OK;
+    });
 }
 
 }  // namespace ngc

@@ -93,7 +93,24 @@ struct ngc_context {
     std::map<std::string, std::pair<int, int>> tuned;
 };
 
+// A captured gridding pipeline: its graph, the device buffers it owns and its
+// device flags ([1]: the inputs bin more records than the buffers hold;
+// [3]: a row's field id is not in field_id).
+struct ngc_grid_plan {
+    ngc_context *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    std::vector<void *> owned;
+    int *flags = nullptr;
+    std::string info;
+};
+
 namespace ngc {
+
+void plan_free(ngc_grid_plan *p);
+int plan_alloc(ngc_grid_plan *p, size_t bytes, void **out);
+int plan_upload(ngc_grid_plan *p, hipStream_t s, const void *host, size_t bytes, void **out);
 
 // Returns a device buffer of at least `bytes` under `name`; contents are
 // undefined.  The context must be locked.
@@ -159,11 +176,10 @@ struct ApTiledCall {
     double *sum_weight;
 };
 int ap_tiled_run(ngc_context *ctx, hipStream_t s, const ApTiledCall &c);
-
-// Bin indices sorted by decreasing sample count (offsets: n_bins + 1 exclusive
-// prefix sums).  The returned device array lives in the context workspace.
-int bin_order_desc(ngc_context *ctx, hipStream_t s, const unsigned long long *offsets,
-                   int64_t n_bins, const unsigned int **order);
+// The tiled pipeline of one call captured into plan p (image and PSF modes;
+// the tables of c must be owned by p).
+int ap_plan_create(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, int autotune,
+                   ngc_grid_plan *p);
 
 // uv scale per channel, in the reference's operation order:
 // -(freq*delta*n_uv)/c  (_standard_grid.py:274-276)

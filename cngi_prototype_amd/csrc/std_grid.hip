@@ -1804,19 +1804,9 @@ extern "C" int ngc_standard_grid_noalloc(ngc_context *ctx, void *stream, int mod
 // gridding plans: the whole tiled pipeline of one ngc_standard_grid call,
 // autotuned once and captured in a hipGraph
 // ---------------------------------------------------------------------------
-struct ngc_grid_plan {
-    ngc_context *ctx = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipStream_t cap_stream = nullptr;
-    std::vector<void *> owned;
-    int *flags = nullptr;
-    std::string info;
-};
-
 namespace ngc {
 
-static void plan_free(ngc_grid_plan *p) {
+void plan_free(ngc_grid_plan *p) {
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -1825,7 +1815,7 @@ static void plan_free(ngc_grid_plan *p) {
     delete p;
 }
 
-static int plan_alloc(ngc_grid_plan *p, size_t bytes, void **out) {
+int plan_alloc(ngc_grid_plan *p, size_t bytes, void **out) {
     void *q = nullptr;
     NGC_HIP(hipMalloc(&q, bytes ? bytes : 256));
     p->owned.push_back(q);
@@ -1833,8 +1823,8 @@ static int plan_alloc(ngc_grid_plan *p, size_t bytes, void **out) {
     return NGC_OK;
 }
 
-static int plan_upload(ngc_grid_plan *p, hipStream_t s, const void *host, size_t bytes,
-                       void **out) {
+int plan_upload(ngc_grid_plan *p, hipStream_t s, const void *host, size_t bytes,
+                void **out) {
     NGC_TRY(plan_alloc(p, bytes, out));
     NGC_HIP(hipMemcpyAsync(*out, host, bytes, hipMemcpyHostToDevice, s));
     return NGC_OK;
@@ -2055,12 +2045,13 @@ int ngc_grid_plan_execute(ngc_grid_plan *plan, void *stream) {
 int ngc_grid_plan_status(ngc_grid_plan *plan, void *stream) {
     NGC_CHECK(plan, NGC_ERR_ARG, "plan is NULL");
     NGC_HIP(hipSetDevice(plan->ctx->device));
-    int flags[2] = {0, 0};
+    int flags[4] = {0, 0, 0, 0};
     NGC_HIP(hipMemcpyAsync(flags, plan->flags, sizeof(flags), hipMemcpyDeviceToHost,
                            (hipStream_t)stream));
     NGC_HIP(hipStreamSynchronize((hipStream_t)stream));
     NGC_CHECK(flags[1] == 0, NGC_ERR_ARG,
               "the inputs now bin more records than the plan holds; make a new plan");
+    NGC_CHECK(flags[3] == 0, NGC_ERR_FIELD, "a row's field id is not in field_id");
     return NGC_OK;
 }
 

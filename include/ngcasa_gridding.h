@@ -461,6 +461,30 @@ int ngc_grid_plan_create(ngc_context *ctx, void *stream, int mode, int precision
                          int64_t n_v, double delta_l, double delta_m, int64_t support,
                          int64_t oversampling, int accumulate, void *grid, double *sum_weight,
                          int autotune, ngc_grid_plan **plan);
+/*
+ * ngc_aperture_grid_plan_create: the same for one ngc_aperture_grid call in a
+ * tiled precision (IMAGE or PSF mode), so that a mosaic's pointings x
+ * channels x pols replay as one captured graph (the reference builds one
+ * dask task per chunk, ngcasa/imaging/_imaging_utils/_aperture_grid.py:25-142).
+ * Arguments as ngc_aperture_grid, plus `autotune` (tile edge 16/24/32 timed
+ * on these inputs; cached per shape) and the returned plan.  The convolution
+ * kernels and phase gradients are read at creation (their re-laid tables are
+ * built once); uvw, vis, weight, flag and field may be rewritten in place
+ * between replays.  ngc_grid_plan_status also fails with NGC_ERR_FIELD when a
+ * replay met a row whose field id is not in field_id.  Execute, status,
+ * describe and destroy are the ngc_grid_plan_* calls.
+ */
+int ngc_aperture_grid_plan_create(
+    ngc_context *ctx, void *stream, int mode, int precision, const double *uvw,
+    const void *vis, int vis_dtype, const void *weight, int weight_dtype, const uint8_t *flag,
+    const int32_t *field, const int32_t *field_id, int64_t n_field, const double *freq_chan,
+    const int32_t *chan_map, const int32_t *pol_map, const int32_t *cf_baseline_map,
+    const int32_t *cf_chan_map, const int32_t *cf_pol_map, const double *conv_kernel,
+    int64_t n_cfb, int64_t n_cfc, int64_t n_cfp, int64_t k_u, int64_t k_v,
+    const int32_t *support, const double *phase_gradient, int64_t n_time, int64_t n_baseline,
+    int64_t n_chan, int64_t n_pol, int64_t n_imag_chan, int64_t n_imag_pol, int64_t n_u,
+    int64_t n_v, double delta_l, double delta_m, int64_t os_u, int64_t os_v, int accumulate,
+    void *grid, double *sum_weight, int autotune, ngc_grid_plan **plan);
 int ngc_grid_plan_execute(ngc_grid_plan *plan, void *stream);
 int ngc_grid_plan_status(ngc_grid_plan *plan, void *stream);
 int ngc_grid_plan_describe(ngc_grid_plan *plan, char *buf, int64_t len);
