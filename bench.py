@@ -408,6 +408,35 @@ def mosaic_leg(args):
         # the timed gridding steps alone (for counter runs)
         return dict(samples=int(n_vis), ms_per_step=round(ms_step, 3),
                     kernel_ms=round(t_k * 1e3, 3), phases_ms=phases)
+    # the same call as an aperture gridding plan: every pointing, channel and
+    # pol replayed as one captured graph (tile edge autotuned at creation)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    ck_m, sp_m, cfc_m = main
+    gp = torch.empty_like(grid)
+    swp = torch.empty_like(sw)
+    plan = nat.ApertureGridPlan(d["uvw"], d["weight"], d["field"], fid, d["freq"], np.arange(C),
+                                np.arange(P), cfb, cfc_m, cfp, ck_m, sp_m, pg, C, [npad, npad],
+                                delta, [20, 20], nat.GRID_IMAGE, args.precision, vis=d["vis"],
+                                grid=gp, sum_weight=swp, autotune=True)
+    torch.cuda.synchronize()
+    t_plan_make = time.time() - t0
+    for _ in range(args.warmup):
+        plan.execute()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(args.steps):
+        plan.execute()
+    torch.cuda.synchronize()
+    ms_plan = (time.time() - t0) / args.steps * 1e3
+    plan.status()
+    plan_err = float(torch.linalg.norm((gp - grid).ravel()) / torch.linalg.norm(grid.ravel()))
+    plan_leg = dict(ms_per_step=round(ms_plan, 3), mvis_s=round(n_vis / (ms_plan / 1e3) / 1e6, 1),
+                    create_s=round(t_plan_make, 2), describe=plan.describe(),
+                    rel_diff_vs_eager=plan_err)
+    plan.close()
+    del gp, swp
+    log(f"mosaic plan replay: {ms_plan:.2f} ms ({plan_leg['describe']})")
     img_main = nat.grid_to_image(grid, [N, N], sum_weight=sw)
     bin_leg = None
     if chans is not None:
@@ -487,6 +516,7 @@ def mosaic_leg(args):
                 speedup_vs_plain_fp64=round(ms_plain / ms_step, 1),
                 error_vs_fp64=errors,
                 bin_kernels=bin_leg,
+                plan=plan_leg,
                 kernel_ms=round(t_k * 1e3, 3),
                 kernel_tflops=round(flops / t_k / 1e12, 2),
                 fp32_peak_tflops=157.3, phases_ms=phases, gcf_s=round(t_gcf, 2),

@@ -269,6 +269,7 @@ ApTiledCall ap_tiled_call(const ApHost &h, const ApDevTables &d, int64_t ms) {
     c.d_cfb = (const int *)d.cfb;
     c.d_cfc = (const int *)d.cfc;
     c.d_support = (const int *)d.sup;
+    c.support_host = h.support;
     c.n_cfc = h.n_cfc;
     c.n_cfp = h.n_cfp;
     c.cfp0 = h.cf_pol_map[0];

@@ -46,9 +46,10 @@
 
 namespace ngc {
 
-constexpr int kApBatch = 16;   // samples staged per batch
+constexpr int kApBatch = 32;   // samples staged per batch
 constexpr int kApGroup = 8;    // samples whose tap loads are issued together
 constexpr int kApBinBlock = 256;
+constexpr int kApMaxCls = 16;  // support classes (lane wavefronts 1..16)
 
 struct ApT {
     const double *uvw;
@@ -63,8 +64,12 @@ struct ApT {
     int64_t n_field, n_cfc, n_cfp, cfp0, n_cf;
     int64_t n_time, n_baseline, n_chan, n_pol, n_imag_chan, n_imag_pol, n_u, n_v;
     int64_t os_u, os_v, k_u, k_v;
-    int R, T, W, hl;
+    int R, T, W, hl;       // R: max(SUPPORT), the tap blocks' edge
+    int Rl;                // the tiled kernel's lanes per axis (its class's support)
     int64_t ntu, ntv;
+    int64_t per_cls;       // bins per support class: n_imag_chan ntu ntv
+    int64_t slot_elems;    // elements of a scratch slot: 2 n_pol W_max^2
+    const int *cf_cls;     // [n_cf] support class of each convolution function
     int ch_log2;
     int64_t n_tb, n_cb;
     FastDiv div_cb, div_tb;  // by n_cb and n_tb (n_baseline n_tb n_cb < 2^32)
@@ -199,6 +204,14 @@ template <int NP, int NW, typename ACC> struct ApEntrySize {
     static constexpr int bytes = ((int)sizeof(ApEntry<NP, NW, ACC>) + 15) / 16 * 16;
 };
 
+// Support classes of the tiled kernel: the convolution functions whose
+// supports need the same number of lane wavefronts (ceil(R^2 / 64)) share a
+// class, gridded by R x R lanes with R the class's largest support.
+struct ApCls {
+    int n;
+    int R[kApMaxCls], W[kApMaxCls], hl[kApMaxCls];
+};
+
 struct ApSample {
     int64_t t, b, ch, row, rc;
 };
@@ -286,10 +299,9 @@ ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *n
         E rec;
         if (ok) {
             const int64_t a_chan = a.chan_map[sm.ch];
-            key = (a_chan * a.ntu + px.uc / a.T) * a.ntv + pxv.vc / a.T;
+            const int64_t cf = ((int64_t)a.cfb[sm.b] * a.n_cfc + a.cfc[sm.ch]) * a.n_cfp + a.cfp0;
+            key = a.cf_cls[cf] * a.per_cls + (a_chan * a.ntu + px.uc / a.T) * a.ntv + pxv.vc / a.T;
             if (FILL) {
-                const int64_t cf = ((int64_t)a.cfb[sm.b] * a.n_cfc + a.cfc[sm.ch]) * a.n_cfp +
-                                   a.cfp0;
                 // |offset| <= 1/2 cell, so |ou| <= os_u/2 < half_u; the clamp
                 // only bounds the table indices
                 const int ou = (int)min(max(px.du, (int64_t)-a.half_u), (int64_t)a.half_u);
@@ -404,7 +416,8 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     using E = ApEntry<NP, NW, ACC>;
     constexpr int EB = ApEntrySize<NP, NW, ACC>::bytes;
     using SC = typename std::conditional<KAHAN, double, ACC>::type;
-    const int R = a.R, W = a.W;
+    const int R = a.Rl, W = a.W;  // lanes per axis and region edge of this class
+    const int RT = a.R;            // edge of the tap blocks (max support)
     const int PLANE = W * (W + 1);
 
     // One dynamic LDS block, carved by integer byte offsets (no pointer round
@@ -440,7 +453,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     int *s_uc = reinterpret_cast<int *>(smem_raw + L.uc);
     int *s_vc = reinterpret_cast<int *>(smem_raw + L.vc);
     // byte offset of the zero tap (the host keeps the table below 2^31 bytes)
-    const unsigned zb = (unsigned)(a.n_cf * (2 * a.half_u + 1) * (2 * a.half_v + 1) * R * R) *
+    const unsigned zb = (unsigned)(a.n_cf * (2 * a.half_u + 1) * (2 * a.half_v + 1) * RT * RT) *
                         (unsigned)sizeof(ACC);
     Cplx<ACC> *s_q = reinterpret_cast<Cplx<ACC> *>(smem_raw + L.q);
     Cplx<ACC> *s_pv = reinterpret_cast<Cplx<ACC> *>(smem_raw + L.pv);
@@ -486,11 +499,16 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
         }
     };
 
-    for (int64_t b0 = start; b0 < end; b0 += kApBatch) {
-        const int nh = (int)min((int64_t)kApBatch, end - b0);
-        __syncthreads();  // previous batch consumed (and the zeroing done)
-        if (tid < nh) {
-            const E en = *reinterpret_cast<const E *>(entries + (b0 + tid) * EB);
+    // A batch's records are read into registers (threads tid < kApBatch)
+    // while the batch before it builds its tables, so that the record load
+    // and the tables' phase-gradient loads are in flight together.
+    E en{};
+    auto fetch = [&](int64_t b0) {
+        if (tid < kApBatch && b0 + tid < end)
+            en = *reinterpret_cast<const E *>(entries + (b0 + tid) * EB);
+    };
+    auto stage = [&](int64_t b0) {
+        if (tid < kApBatch && b0 + tid < end) {
             const int uc = en.uv & 0xffff, vc = (en.uv >> 16) & 0xffff;
             const int ou = (int)(short)(en.off & 0xffff), ov = (int)(short)(en.off >> 16);
             const int cf = en.meta & 0xffff, fidx = (en.meta >> 16) & 0xffff;
@@ -503,13 +521,22 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                                   0, su | (sv << 16));
             s_pos[tid] = make_int4(us, us % R, vs, vs % R);
             s_base[tid] = (unsigned)(((cf * (2 * a.half_u + 1) + ou + a.half_u) * nwv + ov +
-                                      a.half_v) * R * R);
+                                      a.half_v) * RT * RT);
 #pragma unroll
             for (int q = 0; q < NP; q++)
 #pragma unroll
                 for (int c = 0; c < NW; c++) s_wd[tid * NP * NW + q * NW + c] = en.wd[q][c];
         }
+    };
+    fetch(start);
+    stage(start);
+
+    for (int64_t b0 = start; b0 < end; b0 += kApBatch) {
+        const int nh = (int)min((int64_t)kApBatch, end - b0);
+        // this batch's staging slots written; the previous batch's tables
+        // consumed (and the zeroing done)
         __syncthreads();
+        fetch(b0 + kApBatch);
         // samples whose window start or support differs from the previous
         // sample's (and the first of the batch): only there can a lane's cell
         // or tap validity change, so only there do the lanes test their cell
@@ -527,28 +554,38 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
             const uint64_t bm = __ballot(m);
             if (tid == 0) s_mv = (unsigned)bm;
         }
-        // per-sample, per-residue tables, one pass per axis (so that a
-        // wavefront's items take one branch); samples from nh on are "no tap"
+        // per-sample, per-residue tables of both axes in one pass (the two
+        // phase-gradient factor loads of an item in flight together); samples
+        // from nh on are "no tap"
         for (int j = st_j0, r = st_r0; j < kApBatch;
              j += st_dj + (r + st_dr >= R), r += st_dr - (r + st_dr >= R ? R : 0)) {
-            const int eu = r * kApSU + j, eq = r * kApSV + j;
+            const int eu = r * kApSU + j, ev = r * kApSV + j;
             if (j >= nh) {
                 s_ui[eu] = zb;
                 s_uc[eu] = -(1 << 30);
+                s_vi[ev] = zb;
+                s_vc[ev] = -(1 << 30);
+                s_pv[ev] = Cplx<ACC>{(ACC)0, (ACC)0};
 #pragma unroll
-                for (int p = 0; p < NP; p++) s_q[eq * NP + p] = Cplx<ACC>{(ACC)0, (ACC)0};
+                for (int p = 0; p < NP; p++) s_q[ev * NP + p] = Cplx<ACC>{(ACC)0, (ACC)0};
                 continue;
             }
             const int4 fo = s_fo[j];
             const int4 ps = s_pos[j];
-            const int su = fo.w & 0xffff;
+            const int su = fo.w & 0xffff, sv = fo.w >> 16;
             int ru = r - ps.y;
             ru += ru < 0 ? R : 0;
-            const bool valid = ru < su;
-            Cplx<ACC> pu{(ACC)0, (ACC)0};
-            if (valid) pu = pgu[fo.x + (int)a.os_u * ru];
-            s_ui[eu] = valid ? (s_base[j] + (unsigned)(ru * R)) * (unsigned)sizeof(ACC) : zb;
-            s_uc[eu] = valid ? (ps.x + ru) * (W + 1) : -(1 << 30);
+            int rv = r - ps.w;
+            rv += rv < 0 ? R : 0;
+            const bool valid_u = ru < su, valid_v = rv < sv;
+            Cplx<ACC> pu{(ACC)0, (ACC)0}, pv{(ACC)0, (ACC)0};
+            if (valid_u) pu = pgu[fo.x + (int)a.os_u * ru];
+            if (valid_v) pv = pgv[fo.y + (int)a.os_v * rv];
+            s_ui[eu] = valid_u ? (s_base[j] + (unsigned)(ru * RT)) * (unsigned)sizeof(ACC) : zb;
+            s_uc[eu] = valid_u ? (ps.x + ru) * (W + 1) : -(1 << 30);
+            s_vi[ev] = valid_v ? (unsigned)rv * (unsigned)sizeof(ACC) : zb;
+            s_vc[ev] = valid_v ? ps.z + rv : -(1 << 30);
+            s_pv[ev] = pv;
 #pragma unroll
             for (int p = 0; p < NP; p++) {
                 Cplx<ACC> qv;
@@ -562,31 +599,13 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                     qv.x = pu.x * w;
                     qv.y = pu.y * w;
                 }
-                s_q[eq * NP + p] = qv;
+                s_q[ev * NP + p] = qv;
             }
         }
-        for (int j = st_j0, r = st_r0; j < kApBatch;
-             j += st_dj + (r + st_dr >= R), r += st_dr - (r + st_dr >= R ? R : 0)) {
-            const int ev = r * kApSV + j;
-            if (j >= nh) {
-                s_vi[ev] = zb;
-                s_vc[ev] = -(1 << 30);
-                s_pv[ev] = Cplx<ACC>{(ACC)0, (ACC)0};
-                continue;
-            }
-            const int4 fo = s_fo[j];
-            const int4 ps = s_pos[j];
-            const int sv = fo.w >> 16;
-            int rv = r - ps.w;
-            rv += rv < 0 ? R : 0;
-            const bool valid = rv < sv;
-            Cplx<ACC> pv{(ACC)0, (ACC)0};
-            if (valid) pv = pgv[fo.y + (int)a.os_v * rv];
-            s_vi[ev] = valid ? (unsigned)rv * (unsigned)sizeof(ACC) : zb;
-            s_vc[ev] = valid ? ps.z + rv : -(1 << 30);
-            s_pv[ev] = pv;
-        }
+        // tables visible; the staging slots consumed, so the next batch's
+        // records go in now
         __syncthreads();
+        stage(b0 + kApBatch);
         const unsigned moved = __builtin_amdgcn_readfirstlane(s_mv);
         if (active) {
             // Groups of G samples.  A group's tap-table loads (global memory,
@@ -669,7 +688,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     flush();
     __syncthreads();
     // the region goes to this bin's scratch slot: [NACC][W][W]
-    SC *out = scratch + (int64_t)slot[bin] * NACC * W * W;
+    SC *out = scratch + (int64_t)slot[bin] * a.slot_elems;
     for (int i = tid; i < NACC * W * W; i += blockDim.x) {
         const int q = i / (W * W), r = i - q * W * W;
         const int lu = r / W, lv = r - lu * W;
@@ -678,78 +697,86 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     }
 }
 
-// Tiles whose 3 x 3 neighbourhood holds a non-empty bin get the regions that
-// cover their cells added (home tile first, then the neighbours in fixed
-// order, summed in FP64); every other cell keeps its value (zero unless
-// accumulating).  flags[bin]: 1 when the bin holds records; slot[bin]: its
-// scratch slot; tiles[]: the listed tiles, *n_tiles of them.
-__global__ void ap_flags_kernel(ApT a, const unsigned long long *offsets, int64_t n_bins,
+// Tiles whose 3 x 3 neighbourhood holds a non-empty bin (of any support
+// class) get the regions that cover their cells added (class by class; in
+// each, home tile first, then the neighbours in fixed order; summed in FP64);
+// every other cell keeps its value (zero unless accumulating).
+// nonempty[bin]: 1 when the bin holds records (bins of every class);
+// slot[bin]: its scratch slot; near[tile]: 1 when the tile is merged (tiles
+// of one class's index range); tiles[]: the listed tiles, *n_tiles of them.
+__global__ void ap_flags_kernel(ApT a, ApCls k, const unsigned long long *offsets, int64_t n_bins,
                                 int *nonempty, int *near) {
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= n_bins;
+    const int64_t n = n_bins > a.per_cls ? n_bins : a.per_cls;
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= n;
          b += (int64_t)gridDim.x * blockDim.x) {
-        if (b == n_bins) {  // the scans' sentinels
-            nonempty[b] = 0;
+        // the scans' sentinels are zero
+        if (b <= n_bins) nonempty[b] = b < n_bins && offsets[b + 1] > offsets[b] ? 1 : 0;
+        if (b > a.per_cls) continue;
+        if (b == a.per_cls) {
             near[b] = 0;
             continue;
         }
-        nonempty[b] = offsets[b + 1] > offsets[b] ? 1 : 0;
         const int64_t tiles = a.ntu * a.ntv;
         const int64_t ch = b / tiles, r = b - ch * tiles;
         const int64_t tu = r / a.ntv, tv = r - tu * a.ntv;
         int any = 0;
-        for (int du = -1; du <= 1; du++)
-            for (int dv = -1; dv <= 1; dv++) {
-                const int64_t u = tu + du, v = tv + dv;
-                if (u < 0 || u >= a.ntu || v < 0 || v >= a.ntv) continue;
-                const int64_t nb = (ch * a.ntu + u) * a.ntv + v;
-                any |= offsets[nb + 1] > offsets[nb];
-            }
+        for (int c = 0; c < k.n; c++)
+            for (int du = -1; du <= 1; du++)
+                for (int dv = -1; dv <= 1; dv++) {
+                    const int64_t u = tu + du, v = tv + dv;
+                    if (u < 0 || u >= a.ntu || v < 0 || v >= a.ntv) continue;
+                    const int64_t nb = c * a.per_cls + (ch * a.ntu + u) * a.ntv + v;
+                    any |= offsets[nb + 1] > offsets[nb];
+                }
         near[b] = any;
     }
 }
 
-__global__ void ap_list_kernel(int64_t n_bins, const int *near, const int *near_pos, int *tiles) {
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n_bins;
+__global__ void ap_list_kernel(int64_t n_tiles, const int *near, const int *near_pos, int *tiles) {
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n_tiles;
          b += (int64_t)gridDim.x * blockDim.x)
         if (near[b]) tiles[near_pos[b]] = (int)b;
 }
 
 template <typename SC, typename OUT>
-__global__ void ap_merge_kernel(ApT a, const int *__restrict__ nonempty,
+__global__ void ap_merge_kernel(ApT a, ApCls k, const int *__restrict__ nonempty,
                                 const int *__restrict__ slot, const int *__restrict__ tiles,
                                 const int *__restrict__ n_tiles_p, const SC *__restrict__ scratch,
                                 OUT *grid, const int *__restrict__ halt) {
     if (halt && *halt) return;
-    const int W = a.W, T = a.T;
-    const int NACC = (int)a.n_imag_pol * 2;
+    const int T = a.T;
     const int64_t per_tile = (int64_t)a.n_imag_pol * T * T;
     const int64_t n = (int64_t)(*n_tiles_p) * per_tile;
     const int64_t tiles_per_chan = a.ntu * a.ntv;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t k = i / per_tile, r = i - k * per_tile;
-        const int64_t home = tiles[k];
+        const int64_t t = i / per_tile, r = i - t * per_tile;
+        const int64_t home = tiles[t];
         const int64_t ch = home / tiles_per_chan, hr = home - ch * tiles_per_chan;
         const int64_t htu = hr / a.ntv, htv = hr - htu * a.ntv;
         const int64_t pol = r / (T * T), c = r - pol * T * T;
         const int64_t u = htu * T + c / T, v = htv * T + c % T;
         if (u >= a.n_u || v >= a.n_v) continue;
         double re = 0.0, im = 0.0;
-        for (int du = 0; du < 3; du++) {
-            const int64_t tu = du == 0 ? htu : (du == 1 ? htu - 1 : htu + 1);
-            if (tu < 0 || tu >= a.ntu) continue;
-            const int64_t lu = u - (tu * T - a.hl);
-            if (lu < 0 || lu >= W) continue;
-            for (int dv = 0; dv < 3; dv++) {
-                const int64_t tv = dv == 0 ? htv : (dv == 1 ? htv - 1 : htv + 1);
-                if (tv < 0 || tv >= a.ntv) continue;
-                const int64_t lv = v - (tv * T - a.hl);
-                if (lv < 0 || lv >= W) continue;
-                const int64_t bin = (ch * a.ntu + tu) * a.ntv + tv;
-                if (!nonempty[bin]) continue;
-                const SC *sp = scratch + (int64_t)slot[bin] * NACC * W * W + lu * W + lv;
-                re += (double)sp[(2 * pol) * (int64_t)W * W];
-                im += (double)sp[(2 * pol + 1) * (int64_t)W * W];
+        for (int cl = 0; cl < k.n; cl++) {
+            const int W = k.W[cl], hl = k.hl[cl];
+            const int64_t plane = (int64_t)W * W;
+            for (int du = 0; du < 3; du++) {
+                const int64_t tu = du == 0 ? htu : (du == 1 ? htu - 1 : htu + 1);
+                if (tu < 0 || tu >= a.ntu) continue;
+                const int64_t lu = u - (tu * T - hl);
+                if (lu < 0 || lu >= W) continue;
+                for (int dv = 0; dv < 3; dv++) {
+                    const int64_t tv = dv == 0 ? htv : (dv == 1 ? htv - 1 : htv + 1);
+                    if (tv < 0 || tv >= a.ntv) continue;
+                    const int64_t lv = v - (tv * T - hl);
+                    if (lv < 0 || lv >= W) continue;
+                    const int64_t bin = cl * a.per_cls + (ch * a.ntu + tu) * a.ntv + tv;
+                    if (!nonempty[bin]) continue;
+                    const SC *sp = scratch + (int64_t)slot[bin] * a.slot_elems + lu * W + lv;
+                    re += (double)sp[(2 * pol) * plane];
+                    im += (double)sp[(2 * pol + 1) * plane];
+                }
             }
         }
         const int64_t gi = ((ch * a.n_imag_pol + pol) * a.n_u + u) * a.n_v + v;
@@ -875,52 +902,60 @@ __device__ inline int ap_size_class(unsigned long long n) {
     return c < kApOrderClasses ? c : kApOrderClasses - 1;
 }
 
-// hist[c] += tiles of class c (hist zeroed beforehand)
+// hist[k][c] += bins of support class k (blockIdx.y) in size class c (hist
+// zeroed beforehand)
 __global__ void __launch_bounds__(256)
-ap_order_hist_kernel(const unsigned long long *offsets, int64_t n_bins, unsigned int *hist) {
+ap_order_hist_kernel(const unsigned long long *offsets, int64_t per_cls, unsigned int *hist) {
     __shared__ unsigned int s_n[kApOrderClasses];
+    const int64_t b0 = (int64_t)blockIdx.y * per_cls;
     for (int k = threadIdx.x; k < kApOrderClasses; k += blockDim.x) s_n[k] = 0;
     __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_bins;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < per_cls;
          i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&s_n[ap_size_class(offsets[i + 1] - offsets[i])], 1u);
+        atomicAdd(&s_n[ap_size_class(offsets[b0 + i + 1] - offsets[b0 + i])], 1u);
     __syncthreads();
     for (int k = threadIdx.x; k < kApOrderClasses; k += blockDim.x)
-        if (s_n[k]) atomicAdd(&hist[k], s_n[k]);
+        if (s_n[k]) atomicAdd(&hist[blockIdx.y * kApOrderClasses + k], s_n[k]);
 }
 
-// cursor[c] = first place of class c, the largest class first
-__global__ void ap_order_start_kernel(const unsigned int *hist, unsigned int *cursor) {
+// cursor[k][c] = first place of size class c in support class k's segment
+// (k per_cls onwards), the largest size class first
+__global__ void ap_order_start_kernel(const unsigned int *hist, unsigned int *cursor, int n_cls,
+                                      int64_t per_cls) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        unsigned int pos = 0;
-        for (int c = kApOrderClasses - 1; c >= 0; c--) {
-            cursor[c] = pos;
-            pos += hist[c];
+        for (int k = 0; k < n_cls; k++) {
+            unsigned int pos = (unsigned int)(k * per_cls);
+            for (int c = kApOrderClasses - 1; c >= 0; c--) {
+                cursor[k * kApOrderClasses + c] = pos;
+                pos += hist[k * kApOrderClasses + c];
+            }
         }
     }
 }
 
 __global__ void __launch_bounds__(256)
-ap_order_place_kernel(const unsigned long long *offsets, int64_t n_bins, unsigned int *cursor,
+ap_order_place_kernel(const unsigned long long *offsets, int64_t per_cls, unsigned int *cursor,
                       unsigned int *order) {
     __shared__ unsigned int s_n[kApOrderClasses];
     __shared__ unsigned int s_base[kApOrderClasses];
-    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n_bins;
+    const int64_t b0 = (int64_t)blockIdx.y * per_cls;
+    unsigned int *cur = cursor + blockIdx.y * kApOrderClasses;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < per_cls;
          i0 += (int64_t)gridDim.x * blockDim.x) {
         for (int k = threadIdx.x; k < kApOrderClasses; k += blockDim.x) s_n[k] = 0;
         __syncthreads();
         const int64_t i = i0 + threadIdx.x;
         int c = 0;
         unsigned int rank = 0;
-        if (i < n_bins) {
-            c = ap_size_class(offsets[i + 1] - offsets[i]);
+        if (i < per_cls) {
+            c = ap_size_class(offsets[b0 + i + 1] - offsets[b0 + i]);
             rank = atomicAdd(&s_n[c], 1u);
         }
         __syncthreads();
         for (int k = threadIdx.x; k < kApOrderClasses; k += blockDim.x)
-            if (s_n[k]) s_base[k] = atomicAdd(&cursor[k], s_n[k]);
+            if (s_n[k]) s_base[k] = atomicAdd(&cur[k], s_n[k]);
         __syncthreads();
-        if (i < n_bins) order[s_base[c] + rank] = (unsigned int)i;
+        if (i < per_cls) order[s_base[c] + rank] = (unsigned int)(b0 + i);
         __syncthreads();
     }
 }
@@ -963,7 +998,10 @@ struct ApGeom {
     int64_t n_sw = 0, n_rc = 0, cells = 0, n_ntab = 0;
     // image and PSF modes
     int T = 0;
-    size_t lds = 0;
+    size_t lds = 0;                  // the largest class's workgroup LDS
+    size_t lds_cls[kApMaxCls] = {};
+    ApCls cls{};
+    std::vector<int> cf_cls;         // support class of each convolution function
     int64_t n_bins = 0, n_ckr = 0, n_pg = 0, n_waves = 0, bin_blocks = 1;
 };
 
@@ -976,6 +1014,7 @@ struct ApBufs {
     void *scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     int *ne = nullptr, *near = nullptr, *slot = nullptr, *npos = nullptr, *tiles = nullptr;
+    int *cf_cls = nullptr;
     void *itmp = nullptr;
     size_t itmp_bytes = 0;
     unsigned int *ord_hist = nullptr, *ord_cursor = nullptr, *order = nullptr;
@@ -1063,6 +1102,31 @@ static int ap_setup(const ApTiledCall &c, int T_req, ApT &a, ApGeom &g) {
     g.n_ntab = c.n_cf * c.n_field * g.nwu * g.nwv;
     if (c.mode == NGC_GRID_WEIGHT) return NGC_OK;
 
+    // support classes: convolution functions whose largest support needs
+    // the same number of lane wavefronts share one, gridded with R the
+    // class's largest support
+    {
+        int cls_of_waves[kApMaxCls + 1];
+        int r_of_waves[kApMaxCls + 1] = {};
+        for (int64_t f = 0; f < c.n_cf; f++) {
+            const int r = std::max(c.support_host[2 * f], c.support_host[2 * f + 1]);
+            const int wv = (r * r + 63) / 64;
+            r_of_waves[wv] = std::max(r_of_waves[wv], r);
+        }
+        g.cls.n = 0;
+        for (int wv = 1; wv <= kApMaxCls; wv++) {
+            cls_of_waves[wv] = -1;
+            if (r_of_waves[wv] > 0) {
+                cls_of_waves[wv] = g.cls.n;
+                g.cls.R[g.cls.n++] = r_of_waves[wv];
+            }
+        }
+        g.cf_cls.resize((size_t)c.n_cf);
+        for (int64_t f = 0; f < c.n_cf; f++) {
+            const int r = std::max(c.support_host[2 * f], c.support_host[2 * f + 1]);
+            g.cf_cls[(size_t)f] = cls_of_waves[(r * r + 63) / 64];
+        }
+    }
     const int nw = c.mode == NGC_GRID_IMAGE ? 2 : 1;
     auto lds_for = [&](int T) {
         return (size_t)ap_lds_layout(a.R, T + a.R - 1, (int)c.n_pol, nw, g.acc_bytes, g.kahan)
@@ -1084,9 +1148,18 @@ static int ap_setup(const ApTiledCall &c, int T_req, ApT &a, ApGeom &g) {
     NGC_CHECK(g.lds <= 160 * 1024, NGC_ERR_UNSUPPORTED, "aperture tile does not fit in LDS");
     a.T = T;
     a.W = T + a.R - 1;
+    a.Rl = a.R;
     a.ntu = ceil_div(c.n_u, T);
     a.ntv = ceil_div(c.n_v, T);
-    g.n_bins = c.n_imag_chan * a.ntu * a.ntv;
+    a.per_cls = c.n_imag_chan * a.ntu * a.ntv;
+    a.slot_elems = 2 * c.n_pol * (int64_t)a.W * a.W;
+    for (int k = 0; k < g.cls.n; k++) {
+        g.cls.W[k] = T + g.cls.R[k] - 1;
+        g.cls.hl[k] = g.cls.R[k] / 2;
+        g.lds_cls[k] = (size_t)ap_lds_layout(g.cls.R[k], g.cls.W[k], (int)c.n_pol, nw,
+                                             g.acc_bytes, g.kahan).bytes;
+    }
+    g.n_bins = g.cls.n * a.per_cls;
     NGC_CHECK(g.n_bins < (int64_t(1) << 31), NGC_ERR_UNSUPPORTED, "too many tiles");
     g.n_ckr = c.n_cf * g.nwu * g.nwv * a.R * a.R;
     // the tiled kernel addresses the re-laid taps with 32-bit byte offsets
@@ -1127,9 +1200,11 @@ static int ap_alloc(const ApTiledCall &c, const ApGeom &g, const ApAlloc &alloc,
     NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b.itmp_bytes, (const int *)b.ne, b.slot,
                                              nb1));
     NGC_TRY(alloc("ap_int_scan_tmp", b.itmp_bytes, &b.itmp));
-    NGC_TRY(alloc("ap_order_hist", 2 * kApOrderClasses * sizeof(unsigned int), &q));
+    NGC_TRY(alloc("ap_order_hist", 2 * g.cls.n * kApOrderClasses * sizeof(unsigned int), &q));
     b.ord_hist = (unsigned int *)q;
-    b.ord_cursor = b.ord_hist + kApOrderClasses;
+    b.ord_cursor = b.ord_hist + g.cls.n * kApOrderClasses;
+    NGC_TRY(alloc("ap_cf_cls", c.n_cf * sizeof(int), &q));
+    b.cf_cls = (int *)q;
     NGC_TRY(alloc("ap_order", g.n_bins * sizeof(unsigned int) + 4, &q));
     b.order = (unsigned int *)q;
     return NGC_OK;
@@ -1173,14 +1248,15 @@ static int ap_enqueue_count(ngc_context *ctx, hipStream_t s, const ApTiledCall &
     size_t tb = b.scan_tmp_bytes;
     NGC_HIP(hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, tb, b.counts, b.off, nb1, s));
     phase_end(ctx, s);
-    ap_flags_kernel<<<grid_blocks(nb1), 256, 0, s>>>(a, b.off, g.n_bins, b.ne, b.near);
+    ap_flags_kernel<<<grid_blocks(nb1), 256, 0, s>>>(a, g.cls, b.off, g.n_bins, b.ne, b.near);
     NGC_HIP(hipGetLastError());
     size_t it = b.itmp_bytes;
     NGC_HIP(hipcub::DeviceScan::ExclusiveSum(b.itmp, it, (const int *)b.ne, b.slot, nb1, s));
     it = b.itmp_bytes;
-    NGC_HIP(hipcub::DeviceScan::ExclusiveSum(b.itmp, it, (const int *)b.near, b.npos, nb1, s));
-    ap_list_kernel<<<grid_blocks(g.n_bins), 256, 0, s>>>(g.n_bins, (const int *)b.near,
-                                                         (const int *)b.npos, b.tiles);
+    NGC_HIP(hipcub::DeviceScan::ExclusiveSum(b.itmp, it, (const int *)b.near, b.npos,
+                                             a.per_cls + 1, s));
+    ap_list_kernel<<<grid_blocks(a.per_cls), 256, 0, s>>>(a.per_cls, (const int *)b.near,
+                                                          (const int *)b.npos, b.tiles);
     NGC_HIP(hipGetLastError());
     return NGC_OK;
 }
@@ -1207,40 +1283,49 @@ static int ap_enqueue_fill(ngc_context *ctx, hipStream_t s, const ApTiledCall &c
     }
     phase_end(ctx, s);
     phase_begin(ctx, s, "ap_bin_order");
-    ap_fill_kernel<unsigned int><<<1, 256, 0, s>>>(b.ord_hist, kApOrderClasses, 0u);
-    ap_order_hist_kernel<<<grid_blocks(g.n_bins), 256, 0, s>>>(b.off, g.n_bins, b.ord_hist);
-    ap_order_start_kernel<<<1, 64, 0, s>>>(b.ord_hist, b.ord_cursor);
-    ap_order_place_kernel<<<grid_blocks(g.n_bins), 256, 0, s>>>(b.off, g.n_bins, b.ord_cursor,
-                                                                b.order);
+    ap_fill_kernel<unsigned int><<<grid_blocks(g.cls.n * kApOrderClasses), 256, 0, s>>>(
+        b.ord_hist, g.cls.n * kApOrderClasses, 0u);
+    const dim3 og((unsigned)std::min<int64_t>(grid_blocks(a.per_cls), 4096), (unsigned)g.cls.n);
+    ap_order_hist_kernel<<<og, 256, 0, s>>>(b.off, a.per_cls, b.ord_hist);
+    ap_order_start_kernel<<<1, 64, 0, s>>>(b.ord_hist, b.ord_cursor, g.cls.n, a.per_cls);
+    ap_order_place_kernel<<<og, 256, 0, s>>>(b.off, a.per_cls, b.ord_cursor, b.order);
     NGC_HIP(hipGetLastError());
     phase_end(ctx, s);
     phase_begin(ctx, s, "ap_grid_tiled");
-    const int threads = (a.R * a.R + 63) / 64 * 64;
     auto kern = ap_tiled_kernel<MODE, NP, ACC, KAHAN>;
     if (set_attr)
         NGC_HIP(hipFuncSetAttribute((const void *)kern,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.lds));
-    kern<<<(unsigned)g.n_bins, threads, g.lds, s>>>(
-        a, b.off, b.order, (const char *)b.ent, (const ACC *)b.ckr, (const Cplx<ACC> *)b.pgu,
-        (const Cplx<ACC> *)b.pgv, (SC *)b.scr, (const int *)b.slot, b.halt);
-    NGC_HIP(hipGetLastError());
+    // one launch per support class, each with its own lanes and region
+    for (int k = 0; k < g.cls.n; k++) {
+        ApT ak = a;
+        ak.Rl = g.cls.R[k];
+        ak.W = g.cls.W[k];
+        ak.hl = g.cls.hl[k];
+        const int threads = (ak.Rl * ak.Rl + 63) / 64 * 64;
+        kern<<<(unsigned)a.per_cls, threads, g.lds_cls[k], s>>>(
+            ak, b.off, b.order + k * a.per_cls, (const char *)b.ent, (const ACC *)b.ckr,
+            (const Cplx<ACC> *)b.pgu, (const Cplx<ACC> *)b.pgv, (SC *)b.scr, (const int *)b.slot,
+            b.halt);
+        NGC_HIP(hipGetLastError());
+    }
     phase_end(ctx, s);
     phase_begin(ctx, s, "ap_merge");
-    const int64_t merge_items = g.n_bins * c.n_imag_pol * g.T * g.T;
-    const int *ntp = (const int *)b.npos + g.n_bins;  // number of listed tiles
+    const int64_t merge_items = a.per_cls * c.n_imag_pol * g.T * g.T;
+    const int *ntp = (const int *)b.npos + a.per_cls;  // number of listed tiles
     if (g.single) {
         if (!c.accumulate)
             ap_zero_grid_kernel<float><<<grid_blocks(2 * g.cells), 256, 0, s>>>(
                 (float *)c.grid, 2 * g.cells);
         ap_merge_kernel<SC, float><<<grid_blocks(merge_items), 256, 0, s>>>(
-            a, (const int *)b.ne, (const int *)b.slot, (const int *)b.tiles, ntp,
+            a, g.cls, (const int *)b.ne, (const int *)b.slot, (const int *)b.tiles, ntp,
             (const SC *)b.scr, (float *)c.grid, b.halt);
     } else {
         if (!c.accumulate)
             ap_zero_grid_kernel<double><<<grid_blocks(2 * g.cells), 256, 0, s>>>(
                 (double *)c.grid, 2 * g.cells);
         ap_merge_kernel<SC, double><<<grid_blocks(merge_items), 256, 0, s>>>(
-            a, (const int *)b.ne, (const int *)b.slot, (const int *)b.tiles, ntp,
+            a, g.cls, (const int *)b.ne, (const int *)b.slot, (const int *)b.tiles, ntp,
             (const SC *)b.scr, (double *)c.grid, b.halt);
     }
     NGC_HIP(hipGetLastError());
@@ -1340,6 +1425,9 @@ static int ap_tiled_run_t(ngc_context *ctx, hipStream_t s, const ApTiledCall &c,
         b.ntab = (double2 *)q;
     } else {
         NGC_TRY(ap_alloc(c, g, ws, b));
+        NGC_HIP(hipMemcpyAsync(b.cf_cls, g.cf_cls.data(), c.n_cf * sizeof(int),
+                               hipMemcpyHostToDevice, s));
+        a.cf_cls = b.cf_cls;
     }
     // FP64 tap-sum table (sum_weight) for every mode
     phase_begin(ctx, s, "ap_tables");
@@ -1379,8 +1467,7 @@ static int ap_tiled_run_t(ngc_context *ctx, hipStream_t s, const ApTiledCall &c,
         NGC_TRY(ws("ap_bin_entries", (size_t)(total + 1) * ap_entry_bytes<NP, NW, ACC>(), &q));
         b.ent = (char *)q;
         b.ent_cap = total + 1;
-        NGC_TRY(ws("ap_scratch", (size_t)(n_nonempty + 1) * 2 * NP * a.W * a.W * sizeof(SC),
-                   &b.scr));
+        NGC_TRY(ws("ap_scratch", (size_t)(n_nonempty + 1) * a.slot_elems * sizeof(SC), &b.scr));
         b.slot_cap = n_nonempty + 1;
         return ap_enqueue_fill<MODE, NP, ACC, KAHAN>(ctx, s, c, a, g, b, true);
     });
@@ -1478,6 +1565,9 @@ int ap_plan_create(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, int au
     };
     ApBufs b{};
     NGC_TRY(ap_alloc(c, g, own, b));
+    NGC_HIP(hipMemcpyAsync(b.cf_cls, g.cf_cls.data(), c.n_cf * sizeof(int),
+                           hipMemcpyHostToDevice, s));
+    a.cf_cls = b.cf_cls;
     // the kernels' tables are built once, here: the plan reads conv_kernel and
     // phase_gradient at creation only
     ap_norm_kernel<<<grid_blocks(g.n_ntab), 256, 0, s>>>(a, c.conv_kernel,
@@ -1513,7 +1603,7 @@ int ap_plan_create(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, int au
         NGC_TRY(plan_alloc(p, (size_t)b.ent_cap * ap_entry_bytes<NP, NW, ACC>(), &e));
         b.ent = (char *)e;
         b.slot_cap = std::min<int64_t>(g.n_bins, n_nonempty + n_nonempty / 16 + 64);
-        NGC_TRY(plan_alloc(p, (size_t)b.slot_cap * 2 * NP * a.W * a.W * sizeof(SC), &b.scr));
+        NGC_TRY(plan_alloc(p, (size_t)b.slot_cap * a.slot_elems * sizeof(SC), &b.scr));
         b.halt = p->flags + 1;
         auto kern = ap_tiled_kernel<MODE, NP, ACC, KAHAN>;
         NGC_HIP(hipFuncSetAttribute((const void *)kern,
@@ -1539,11 +1629,13 @@ int ap_plan_create(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, int au
         NGC_HIP(hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0));
         char info[320];
         snprintf(info, sizeof(info),
-                 "aperture: tile %dx%d, %d lanes per tile, %lld bins, %llu records (cap %llu), "
-                 "%d non-empty tiles (cap %lld)",
-                 g.T, g.T, a.R * a.R, (long long)g.n_bins, total, b.ent_cap, n_nonempty,
+                 "aperture: tile %dx%d, %lld bins, %llu records (cap %llu), "
+                 "%d non-empty tiles (cap %lld); lanes per support class:",
+                 g.T, g.T, (long long)g.n_bins, total, b.ent_cap, n_nonempty,
                  (long long)b.slot_cap);
         p->info = info;
+        for (int k = 0; k < g.cls.n; k++)
+            p->info += " " + std::to_string(g.cls.R[k]) + "x" + std::to_string(g.cls.R[k]);
         if (!tune_log.empty()) p->info += "; tuning ms " + tune_log;
         return NGC_OK;
     });

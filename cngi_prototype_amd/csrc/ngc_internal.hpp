@@ -165,6 +165,7 @@ struct ApTiledCall {
     int64_t n_field;
     const double *d_uv_scale;
     const int *d_chan_map, *d_cfb, *d_cfc, *d_support;
+    const int32_t *support_host;  // [n_cf][2], the caller's host copy
     int64_t n_cfc, n_cfp, cfp0, n_cf;
     const double *conv_kernel, *phase_gradient;
     int64_t k_u, k_v;
