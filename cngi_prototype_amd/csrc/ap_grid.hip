@@ -8,7 +8,9 @@
 // The reference multiplies the whole kernel cube by the field's phase gradient
 // on every field change (:428-430); here each tap forms CK*PG on the fly from
 // the two tables, which gives the same FP64 products.
+#include <algorithm>
 #include <functional>
+#include <vector>
 
 #include "ngc_internal.hpp"
 
@@ -169,7 +171,8 @@ struct ApHost {
 
 // Device copies of the small host tables.
 struct ApDevTables {
-    void *scale, *cmap, *pmap, *cfb, *cfc, *cfp, *sup, *fid;
+    void *scale, *cmap, *pmap, *cfb, *cfc, *cfp, *sup, *fid, *fmap;
+    int64_t fmap_len;
 };
 
 // Argument checks; *ms = max(SUPPORT), the half-width of the reference's
@@ -243,6 +246,22 @@ int ap_upload_tables(const ApHost &h, const Upload &up, ApDevTables &d) {
     NGC_TRY(up("cf_pol_map", h.cf_pol_map, h.n_pol * sizeof(int32_t), &d.cfp));
     NGC_TRY(up("support", h.support, n_sup * sizeof(int32_t), &d.sup));
     NGC_TRY(up("field_id", h.field_id, h.n_field * sizeof(int32_t), &d.fid));
+    // field id -> its first index in field_id (-1 if absent), when the ids are
+    // small non-negative integers; otherwise the kernels search field_id
+    int64_t id_max = -1;
+    bool small = true;
+    for (int64_t f = 0; f < h.n_field; f++) {
+        small &= h.field_id[f] >= 0 && h.field_id[f] < (1 << 20);
+        id_max = std::max<int64_t>(id_max, h.field_id[f]);
+    }
+    d.fmap = nullptr;
+    d.fmap_len = 0;
+    if (small && id_max >= 0) {
+        std::vector<int32_t> map((size_t)id_max + 1, -1);
+        for (int64_t f = h.n_field - 1; f >= 0; f--) map[(size_t)h.field_id[f]] = (int32_t)f;
+        d.fmap_len = id_max + 1;
+        NGC_TRY(up("field_map", map.data(), map.size() * sizeof(int32_t), &d.fmap));
+    }
     return NGC_OK;
 }
 
@@ -263,6 +282,8 @@ ApTiledCall ap_tiled_call(const ApHost &h, const ApDevTables &d, int64_t ms) {
     c.flag = h.flag;
     c.field = h.field;
     c.d_field_id = (const int *)d.fid;
+    c.d_field_map = (const int *)d.fmap;
+    c.field_map_len = d.fmap_len;
     c.n_field = h.n_field;
     c.d_uv_scale = (const double *)d.scale;
     c.d_chan_map = (const int *)d.cmap;

@@ -15,21 +15,24 @@
 //     (PG[x][y] = PG[x][c]*PG[c][y] for the reference's exp(i(x px + y py)));
 //     a table that does not factor is refused.
 //  3. ap_bin_kernel (count, scan, fill): every gridded (row, channel) goes to
-//     the bin of the (image channel, T x T tile) holding its centre cell, in
-//     time order per baseline; the fill pass adds the FP64 sum_weight term.
-//  4. ap_tiled_kernel: one workgroup of R*R lanes per bin.  Lane (a, b) owns
-//     every cell of the tile's halo region (T + R - 1 cells square) with
-//     u = a, v = b (mod R); a footprint of support <= R meets each residue
-//     pair at most once, so a lane has at most one tap per sample and no two
-//     lanes share a cell (J. W. Romein, ICS 2012).  The lane keeps its
-//     cell's running sum in registers and flushes it into the LDS region
-//     (FP64 cells with FP32_KAHAN) when its cell changes.  The region is
-//     written to a scratch slot of the bin.
+//     the bin of the (support class, image channel, T x T tile) holding its
+//     centre cell, in time order per baseline; the fill pass adds the FP64
+//     sum_weight term.  A support class gathers the convolution functions
+//     whose largest support needs the same number of lane wavefronts.
+//  4. ap_tiled_kernel, one launch per support class: one workgroup of R*R
+//     lanes per bin, R the class's largest support.  Lane (a, b) owns every
+//     cell of the tile's halo region (T + R - 1 cells square) with u = a,
+//     v = b (mod R); a footprint of support <= R meets each residue pair at
+//     most once, so a lane has at most one tap per sample and no two lanes
+//     share a cell (J. W. Romein, ICS 2012).  The lane keeps its cell's
+//     running sum in registers and flushes it into the LDS region (FP64
+//     cells with FP32_KAHAN) when its cell changes.  The region is written,
+//     in the grid's precision, to a scratch slot of the bin.
 //  5. ap_merge_kernel: only non-empty bins have scratch slots; every cell of
-//     a tile next to a non-empty bin adds, in a fixed order and FP64, the (at
-//     most four) regions that cover it (the grid is zeroed first unless
-//     accumulating), so a chunk of a large cube touches only its part of the
-//     grid.
+//     a tile next to a non-empty bin adds, in a fixed order and FP64, the
+//     regions that cover it (at most four per support class; the grid is
+//     zeroed first unless accumulating), so a chunk of a large cube touches
+//     only its part of the grid.
 //
 // Weight mode: every sample lands on the same centre taps of its (image
 // channel, pol, convolution function, field) key, so the weight grid is
@@ -50,6 +53,10 @@ constexpr int kApBatch = 32;   // samples staged per batch
 constexpr int kApGroup = 8;    // samples whose tap loads are issued together
 constexpr int kApBinBlock = 256;
 constexpr int kApMaxCls = 16;  // support classes (lane wavefronts 1..16)
+// Dword 3 of a raw buffer resource on gfx9 parts (32-bit data format, no
+// index stride or swizzle): range checking then compares the byte offset with
+// num_records, and a load past it returns zero.
+constexpr int kBufferWord3 = 0x00020000;
 
 struct ApT {
     const double *uvw;
@@ -58,6 +65,8 @@ struct ApT {
     const uint8_t *flag;
     const int *field;      // [T, B]
     const int *field_id;   // [n_field]
+    const int *field_map;  // [field_map_len]: index in field_id of each id, or -1
+    int64_t field_map_len; // 0: search field_id instead
     const double *uv_scale;  // [2][n_chan]
     const int *chan_map, *cfb, *cfc;
     const int *support;    // [n_cf][2]
@@ -135,8 +144,8 @@ __global__ void ap_taps_kernel(ApT a, const double *ck, ACC *ckr) {
     const int R = a.R;
     const int64_t n = a.n_cf * nwu * nwv * R * R;
     const int64_t kplane = a.k_u * a.k_v;
-    // element n is a zero tap, where the tiled kernel points the taps that
-    // fall outside a sample's support
+    // element n is zero: the end of the table, where the tiled kernel's taps
+    // outside a sample's support point (its loads past the end read zero)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n;
          i += (int64_t)gridDim.x * blockDim.x) {
         if (i == n) {
@@ -239,11 +248,15 @@ __device__ inline bool ap_place(const ApT &a, const ApSample &sm, int &fidx, Pix
     const int fval = a.field[sm.row];
     if (!(fval > -1)) return false;
     fidx = -1;
-    for (int64_t f = 0; f < a.n_field; f++)
-        if (a.field_id[f] == fval) {
-            fidx = (int)f;
-            break;
-        }
+    if (a.field_map_len > 0) {
+        if (fval < a.field_map_len) fidx = a.field_map[fval];
+    } else {
+        for (int64_t f = 0; f < a.n_field; f++)
+            if (a.field_id[f] == fval) {
+                fidx = (int)f;
+                break;
+            }
+    }
     if (fidx < 0) {
         atomicOr(a.error, 1);
         return false;
@@ -269,7 +282,7 @@ __device__ inline bool ap_place(const ApT &a, const ApSample &sm, int &fidx, Pix
 template <bool FILL, int MODE, int NP, typename ACC>
 __global__ void __launch_bounds__(kApBinBlock)
 ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *ntab,
-              double *sum_weight, int sw_in_lds, const int *halt) {
+              double *sum_weight, int sw_copies, const int *halt) {
 #pragma clang fp contract(off)
     constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
     using E = ApEntry<NP, NW, ACC>;
@@ -278,8 +291,11 @@ ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *n
     // a plan whose buffers the inputs outgrew: nothing is written
     if (FILL && halt && *halt) return;
     const int64_t n_sw = a.n_imag_chan * a.n_imag_pol;
-    if (FILL && sw_in_lds) {
-        for (int64_t i = threadIdx.x; i < n_sw; i += blockDim.x) s_sw[i] = 0.0;
+    // sum_weight terms: sw_copies > 0 copies in LDS (a lane adds to copy
+    // lane mod sw_copies, so fewer lanes of a wavefront meet on one address),
+    // summed into sum_weight at the end; 0: atomics on sum_weight itself
+    if (FILL && sw_copies) {
+        for (int64_t i = threadIdx.x; i < n_sw * sw_copies; i += blockDim.x) s_sw[i] = 0.0;
         __syncthreads();
     }
     const int lane = threadIdx.x & 63;
@@ -330,8 +346,8 @@ ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *n
                                                 ? w * (nm.x * nm.x - nm.y * nm.y)
                                                 : w * nm.x;
                         const int64_t swi = a_chan * a.n_imag_pol + p;
-                        if (sw_in_lds)
-                            atomicAdd(&s_sw[swi], term);
+                        if (sw_copies)
+                            atomicAdd(&s_sw[swi * sw_copies + (lane & (sw_copies - 1))], term);
                         else
                             atomicAdd(&sum_weight[swi], term);
                     }
@@ -357,10 +373,13 @@ ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *n
             pending &= ~same;
         }
     }
-    if (FILL && sw_in_lds) {
+    if (FILL && sw_copies) {
         __syncthreads();
-        for (int64_t i = threadIdx.x; i < n_sw; i += blockDim.x)
-            if (s_sw[i] != 0.0) atomicAdd(&sum_weight[i], s_sw[i]);
+        for (int64_t i = threadIdx.x; i < n_sw; i += blockDim.x) {
+            double t = 0.0;
+            for (int c = 0; c < sw_copies; c++) t += s_sw[i * sw_copies + c];
+            if (t != 0.0) atomicAdd(&sum_weight[i], t);
+        }
     }
 }
 
@@ -409,13 +428,13 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                 const unsigned int *__restrict__ order, const char *__restrict__ entries,
                 const ACC *__restrict__ ckr, const Cplx<ACC> *__restrict__ pgu,
                 const Cplx<ACC> *__restrict__ pgv,
-                typename std::conditional<KAHAN, double, ACC>::type *__restrict__ scratch,
+                ACC *__restrict__ scratch,
                 const int *__restrict__ slot, const int *__restrict__ halt) {
     constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
     constexpr int NACC = NP * 2;  // the grid is complex in every mode
     using E = ApEntry<NP, NW, ACC>;
     constexpr int EB = ApEntrySize<NP, NW, ACC>::bytes;
-    using SC = typename std::conditional<KAHAN, double, ACC>::type;
+    using SC = ACC;  // scratch regions in the grid's precision
     const int R = a.Rl, W = a.W;  // lanes per axis and region edge of this class
     const int RT = a.R;            // edge of the tap blocks (max support)
     const int PLANE = W * (W + 1);
@@ -427,9 +446,9 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     //   s_pos   [kApBatch] int4   staging: start cells and their residues
     //   s_base  [kApBatch] uint   first tap index of the sample's R x R block
     //   s_ui    [R][kApBatch] uint u-residue a: byte offset of the sample's tap
-    //                         block plus ru*R taps; the zero tap's offset Z if no tap
+    //                         block plus ru*R taps; the table's end Z if no tap
     //   s_vi    [R][kApBatch] uint v-residue b: rv taps in bytes; Z if no tap
-    //                         (min(ui + vi, Z) is the tap's byte offset)
+    //                         (ui + vi is the tap's byte offset; >= Z reads zero)
     //   s_uc    [R][kApBatch] int cell_u*(W+1), or -2^30 if no tap
     //   s_vc    [R][kApBatch] int cell_v, or -2^30 if no tap (uc + vc < 0: no tap)
     //   s_q     [R][kApBatch][NP] u phase factor x weighted visibility per pol
@@ -452,9 +471,14 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
     unsigned *s_vi = reinterpret_cast<unsigned *>(smem_raw + L.vi);
     int *s_uc = reinterpret_cast<int *>(smem_raw + L.uc);
     int *s_vc = reinterpret_cast<int *>(smem_raw + L.vc);
-    // byte offset of the zero tap (the host keeps the table below 2^31 bytes)
+    // byte size of the tap table up to its end element (the host keeps the
+    // table below 2^31 bytes, so ui + vi < 2^32)
     const unsigned zb = (unsigned)(a.n_cf * (2 * a.half_u + 1) * (2 * a.half_v + 1) * RT * RT) *
                         (unsigned)sizeof(ACC);
+    // the re-laid taps as a raw buffer of zb bytes: loads at offsets >= zb
+    // return zero (kBufferWord3: 32-bit data format, no stride or swizzle)
+    const __amdgpu_buffer_rsrc_t ck_rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)ckr, 0, (int)zb, kBufferWord3);
     Cplx<ACC> *s_q = reinterpret_cast<Cplx<ACC> *>(smem_raw + L.q);
     Cplx<ACC> *s_pv = reinterpret_cast<Cplx<ACC> *>(smem_raw + L.pv);
     ACC *s_wd = reinterpret_cast<ACC *>(smem_raw + L.wd);
@@ -611,7 +635,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
             // Groups of G samples.  A group's tap-table loads (global memory,
             // one per lane and sample) are issued together one group ahead of
             // its arithmetic, branch-free: a tap outside the sample's support
-            // (or past the end of the bin) reads the zero tap.  The next
+            // (or past the end of the bin) reads zero.  The next
             // group's loads are unconditional (clamped to the last group of
             // the batch) so that the wait for this group's taps counts only
             // the loads issued after them.  A lane's table reads are its
@@ -633,11 +657,17 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                 }
 #pragma unroll
                 for (int g = 0; g < G; g++) {
-                    // The tap's byte offset from the table's base (the load's
-                    // address is the scalar base plus one vector offset); a
-                    // tap outside the support reads the zero tap.
-                    const unsigned off = min(ui[g] + vi[g], zb);
-                    t[g] = *reinterpret_cast<const ACC *>(reinterpret_cast<const char *>(ckr) + off);
+                    // The tap's byte offset from the table's base: a buffer
+                    // load, so that a tap outside the support (offset >= zb:
+                    // past the resource's end) reads zero without a clamp.
+                    const unsigned off = ui[g] + vi[g];
+                    if constexpr (sizeof(ACC) == 4) {
+                        t[g] = __builtin_bit_cast(
+                            ACC, __builtin_amdgcn_raw_buffer_load_b32(ck_rsrc, off, 0, 0));
+                    } else {
+                        t[g] = __builtin_bit_cast(
+                            ACC, __builtin_amdgcn_raw_buffer_load_b64(ck_rsrc, off, 0, 0));
+                    }
                 }
             };
             auto run = [&](int j0, const ACC (&t)[G]) {
@@ -662,8 +692,8 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
                             for (int q = 0; q < NACC; q++) acc[q] = (ACC)0;
                         }
                     }
-                    // (a tap outside the support is the zero tap: it adds exact
-                    // zeros to the current cell)
+                    // (a tap outside the support is zero: it adds exact zeros
+                    // to the current cell)
                     const ACC k = t[g];
                     const ACC tr = k * pv[g].x, ti = k * pv[g].y;
 #pragma unroll
@@ -738,50 +768,66 @@ __global__ void ap_list_kernel(int64_t n_tiles, const int *near, const int *near
         if (near[b]) tiles[near_pos[b]] = (int)b;
 }
 
+// One workgroup per listed tile (grid-stride): the slots of the tile's 3 x 3
+// neighbourhood in every support class are looked up once, then each thread
+// sums its cells.
 template <typename SC, typename OUT>
-__global__ void ap_merge_kernel(ApT a, ApCls k, const int *__restrict__ nonempty,
-                                const int *__restrict__ slot, const int *__restrict__ tiles,
-                                const int *__restrict__ n_tiles_p, const SC *__restrict__ scratch,
-                                OUT *grid, const int *__restrict__ halt) {
+__global__ void __launch_bounds__(256)
+ap_merge_kernel(ApT a, ApCls k, const int *__restrict__ nonempty, const int *__restrict__ slot,
+                const int *__restrict__ tiles, const int *__restrict__ n_tiles_p,
+                const SC *__restrict__ scratch, OUT *grid, const int *__restrict__ halt) {
     if (halt && *halt) return;
+    __shared__ int s_slot[kApMaxCls * 9];  // [class][du + 1][dv + 1]: slot, or -1
     const int T = a.T;
-    const int64_t per_tile = (int64_t)a.n_imag_pol * T * T;
-    const int64_t n = (int64_t)(*n_tiles_p) * per_tile;
+    const int n_tiles = *n_tiles_p;
     const int64_t tiles_per_chan = a.ntu * a.ntv;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t t = i / per_tile, r = i - t * per_tile;
+    const int cells = (int)a.n_imag_pol * T * T;
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const int64_t home = tiles[t];
         const int64_t ch = home / tiles_per_chan, hr = home - ch * tiles_per_chan;
-        const int64_t htu = hr / a.ntv, htv = hr - htu * a.ntv;
-        const int64_t pol = r / (T * T), c = r - pol * T * T;
-        const int64_t u = htu * T + c / T, v = htv * T + c % T;
-        if (u >= a.n_u || v >= a.n_v) continue;
-        double re = 0.0, im = 0.0;
-        for (int cl = 0; cl < k.n; cl++) {
-            const int W = k.W[cl], hl = k.hl[cl];
-            const int64_t plane = (int64_t)W * W;
-            for (int du = 0; du < 3; du++) {
-                const int64_t tu = du == 0 ? htu : (du == 1 ? htu - 1 : htu + 1);
-                if (tu < 0 || tu >= a.ntu) continue;
-                const int64_t lu = u - (tu * T - hl);
-                if (lu < 0 || lu >= W) continue;
-                for (int dv = 0; dv < 3; dv++) {
-                    const int64_t tv = dv == 0 ? htv : (dv == 1 ? htv - 1 : htv + 1);
-                    if (tv < 0 || tv >= a.ntv) continue;
-                    const int64_t lv = v - (tv * T - hl);
-                    if (lv < 0 || lv >= W) continue;
-                    const int64_t bin = cl * a.per_cls + (ch * a.ntu + tu) * a.ntv + tv;
-                    if (!nonempty[bin]) continue;
-                    const SC *sp = scratch + (int64_t)slot[bin] * a.slot_elems + lu * W + lv;
-                    re += (double)sp[(2 * pol) * plane];
-                    im += (double)sp[(2 * pol + 1) * plane];
+        const int htu = (int)(hr / a.ntv), htv = (int)(hr - (int64_t)htu * a.ntv);
+        __syncthreads();  // the previous tile's slots consumed
+        for (int i = threadIdx.x; i < k.n * 9; i += blockDim.x) {
+            const int cl = i / 9, d = i - cl * 9;
+            const int tu = htu + d / 3 - 1, tv = htv + d % 3 - 1;
+            int sl = -1;
+            if (tu >= 0 && tu < a.ntu && tv >= 0 && tv < a.ntv) {
+                const int64_t bin = cl * a.per_cls + (ch * a.ntu + tu) * a.ntv + tv;
+                if (nonempty[bin]) sl = slot[bin];
+            }
+            s_slot[i] = sl;
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+            const int pol = c / (T * T), r = c - pol * T * T;
+            const int lu0 = r / T, lv0 = r - lu0 * T;
+            const int64_t u = (int64_t)htu * T + lu0, v = (int64_t)htv * T + lv0;
+            if (u >= a.n_u || v >= a.n_v) continue;
+            double re = 0.0, im = 0.0;
+            // class by class; home tile first, then the neighbours
+            for (int cl = 0; cl < k.n; cl++) {
+                const int W = k.W[cl], hl = k.hl[cl];
+                const int64_t plane = (int64_t)W * W;
+                for (int di = 0; di < 3; di++) {
+                    const int du = di == 0 ? 0 : (di == 1 ? -1 : 1);
+                    const int lu = lu0 + hl - du * T;  // u - ((htu + du) T - hl)
+                    if (lu < 0 || lu >= W) continue;
+                    for (int dj = 0; dj < 3; dj++) {
+                        const int dv = dj == 0 ? 0 : (dj == 1 ? -1 : 1);
+                        const int lv = lv0 + hl - dv * T;
+                        if (lv < 0 || lv >= W) continue;
+                        const int sl = s_slot[cl * 9 + (du + 1) * 3 + dv + 1];
+                        if (sl < 0) continue;
+                        const SC *sp = scratch + (int64_t)sl * a.slot_elems + lu * W + lv;
+                        re += (double)sp[(2 * pol) * plane];
+                        im += (double)sp[(2 * pol + 1) * plane];
+                    }
                 }
             }
+            const int64_t gi = ((ch * a.n_imag_pol + pol) * a.n_u + u) * a.n_v + v;
+            grid[2 * gi] = (OUT)((double)grid[2 * gi] + re);
+            grid[2 * gi + 1] = (OUT)((double)grid[2 * gi + 1] + im);
         }
-        const int64_t gi = ((ch * a.n_imag_pol + pol) * a.n_u + u) * a.n_v + v;
-        grid[2 * gi] = (OUT)((double)grid[2 * gi] + re);
-        grid[2 * gi + 1] = (OUT)((double)grid[2 * gi + 1] + im);
     }
 }
 
@@ -1053,6 +1099,8 @@ static int ap_setup(const ApTiledCall &c, int T_req, ApT &a, ApGeom &g) {
     a.flag = c.flag;
     a.field = c.field;
     a.field_id = c.d_field_id;
+    a.field_map = c.d_field_map;
+    a.field_map_len = c.field_map_len;
     a.uv_scale = c.d_uv_scale;
     a.chan_map = c.d_chan_map;
     a.cfb = c.d_cfb;
@@ -1269,16 +1317,16 @@ static int ap_enqueue_count(ngc_context *ctx, hipStream_t s, const ApTiledCall &
 template <int MODE, int NP, typename ACC, bool KAHAN>
 static int ap_enqueue_fill(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, const ApT &a,
                            const ApGeom &g, const ApBufs &b, bool set_attr) {
-    using SC = typename std::conditional<KAHAN, double, ACC>::type;
+    using SC = ACC;  // scratch regions in the grid's precision
     const int64_t nb1 = g.n_bins + 1;
-    const int sw_lds = g.n_sw <= 4096 ? 1 : 0;
-    const size_t sw_shmem = sw_lds ? g.n_sw * sizeof(double) : 0;
+    const int sw_copies = g.n_sw * 8 <= 4096 ? 8 : (g.n_sw <= 4096 ? 1 : 0);
+    const size_t sw_shmem = g.n_sw * sw_copies * sizeof(double);
     ap_copy_kernel<unsigned long long><<<grid_blocks(nb1), 256, 0, s>>>(b.counts, b.off, nb1);
     NGC_HIP(hipGetLastError());
     phase_begin(ctx, s, "ap_bin_fill");
     if (g.n_waves > 0) {
         ap_bin_kernel<true, MODE, NP, ACC><<<(int)g.bin_blocks, kApBinBlock, sw_shmem, s>>>(
-            a, b.counts, b.ent, b.ntab, c.sum_weight, sw_lds, b.halt);
+            a, b.counts, b.ent, b.ntab, c.sum_weight, sw_copies, b.halt);
         NGC_HIP(hipGetLastError());
     }
     phase_end(ctx, s);
@@ -1311,20 +1359,20 @@ static int ap_enqueue_fill(ngc_context *ctx, hipStream_t s, const ApTiledCall &c
     }
     phase_end(ctx, s);
     phase_begin(ctx, s, "ap_merge");
-    const int64_t merge_items = a.per_cls * c.n_imag_pol * g.T * g.T;
+    const int merge_blocks = (int)std::min<int64_t>(a.per_cls, 8192);
     const int *ntp = (const int *)b.npos + a.per_cls;  // number of listed tiles
     if (g.single) {
         if (!c.accumulate)
             ap_zero_grid_kernel<float><<<grid_blocks(2 * g.cells), 256, 0, s>>>(
                 (float *)c.grid, 2 * g.cells);
-        ap_merge_kernel<SC, float><<<grid_blocks(merge_items), 256, 0, s>>>(
+        ap_merge_kernel<SC, float><<<merge_blocks, 256, 0, s>>>(
             a, g.cls, (const int *)b.ne, (const int *)b.slot, (const int *)b.tiles, ntp,
             (const SC *)b.scr, (float *)c.grid, b.halt);
     } else {
         if (!c.accumulate)
             ap_zero_grid_kernel<double><<<grid_blocks(2 * g.cells), 256, 0, s>>>(
                 (double *)c.grid, 2 * g.cells);
-        ap_merge_kernel<SC, double><<<grid_blocks(merge_items), 256, 0, s>>>(
+        ap_merge_kernel<SC, double><<<merge_blocks, 256, 0, s>>>(
             a, g.cls, (const int *)b.ne, (const int *)b.slot, (const int *)b.tiles, ntp,
             (const SC *)b.scr, (double *)c.grid, b.halt);
     }
@@ -1454,7 +1502,7 @@ static int ap_tiled_run_t(ngc_context *ctx, hipStream_t s, const ApTiledCall &c,
         using ACC = typename decltype(acc_c)::type;
         constexpr bool KAHAN = decltype(kahan_c)::value;
         constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
-        using SC = typename std::conditional<KAHAN, double, ACC>::type;
+        using SC = ACC;  // scratch regions in the grid's precision
         NGC_TRY((ap_enqueue_count<MODE, NP, ACC>(ctx, s, c, a, g, b)));
         unsigned long long total = 0;
         int n_nonempty = 0;
@@ -1587,7 +1635,7 @@ int ap_plan_create(ngc_context *ctx, hipStream_t s, const ApTiledCall &c, int au
         using ACC = typename decltype(acc_c)::type;
         constexpr bool KAHAN = decltype(kahan_c)::value;
         constexpr int NW = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
-        using SC = typename std::conditional<KAHAN, double, ACC>::type;
+        using SC = ACC;  // scratch regions in the grid's precision
         // buffers sized by one eager count pass, with room for the records and
         // the non-empty tiles to grow by 1/16 (inputs rewritten in place)
         NGC_TRY((ap_enqueue_count<MODE, NP, ACC>(ctx, s, c, a, g, b)));

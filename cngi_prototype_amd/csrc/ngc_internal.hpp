@@ -162,6 +162,8 @@ struct ApTiledCall {
     int weight_dtype;
     const uint8_t *flag;
     const int *field, *d_field_id;
+    const int *d_field_map;   // index in field_id of each field id (first match), or -1
+    int64_t field_map_len;    // 0 when there is no map (ids too large)
     int64_t n_field;
     const double *d_uv_scale;
     const int *d_chan_map, *d_cfb, *d_cfc, *d_support;
