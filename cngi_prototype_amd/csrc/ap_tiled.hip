@@ -391,11 +391,13 @@ ap_bin_kernel(ApT a, unsigned long long *counts, char *entries, const double2 *n
 // Row strides (entries) of the residue-major staging tables.  A wavefront's
 // lanes span ~64/R u-residues and all R v-residues; the padding puts the rows
 // they read together on different LDS banks: u rows (4-byte entries, read 16
-// bytes at a time) start 36 words apart, v rows of 4-byte entries 33 words
-// and of complex entries 66 words apart, q rows (NP complex values per
-// entry) 33 entries apart.
+// bytes at a time) start 36 words apart, v rows of 4-byte entries (read 8
+// bytes at a time) 33 words apart, v rows of complex entries (read 16 bytes
+// at a time) 68 words apart, q rows (NP complex values per entry) 33 entries
+// apart.
 constexpr int kApSU = kApBatch + 4;
 constexpr int kApSV = kApBatch + 1;
+constexpr int kApSP = kApBatch + 2;
 struct ApLds {
     int fo, pos, base, ui, vi, uc, vc, q, pv, wd, bytes;
 };
@@ -413,7 +415,7 @@ __host__ __device__ inline ApLds ap_lds_layout(int R, int W, int NP, int NW, int
     L.vc = ap_align16(L.uc + kApSU * R * 4);
     L.q = ap_align16(L.vc + kApSV * R * 4);
     L.pv = ap_align16(L.q + kApSV * R * NP * 2 * acc_bytes);
-    L.wd = ap_align16(L.pv + kApSV * R * 2 * acc_bytes);
+    L.wd = ap_align16(L.pv + kApSP * R * 2 * acc_bytes);
     L.bytes = ap_align16(L.wd + kApBatch * NP * NW * acc_bytes);
     return L;
 }
@@ -583,13 +585,13 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
         // from nh on are "no tap"
         for (int j = st_j0, r = st_r0; j < kApBatch;
              j += st_dj + (r + st_dr >= R), r += st_dr - (r + st_dr >= R ? R : 0)) {
-            const int eu = r * kApSU + j, ev = r * kApSV + j;
+            const int eu = r * kApSU + j, ev = r * kApSV + j, ep = r * kApSP + j;
             if (j >= nh) {
                 s_ui[eu] = zb;
                 s_uc[eu] = -(1 << 30);
                 s_vi[ev] = zb;
                 s_vc[ev] = -(1 << 30);
-                s_pv[ev] = Cplx<ACC>{(ACC)0, (ACC)0};
+                s_pv[ep] = Cplx<ACC>{(ACC)0, (ACC)0};
 #pragma unroll
                 for (int p = 0; p < NP; p++) s_q[ev * NP + p] = Cplx<ACC>{(ACC)0, (ACC)0};
                 continue;
@@ -609,7 +611,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
             s_uc[eu] = valid_u ? (ps.x + ru) * (W + 1) : -(1 << 30);
             s_vi[ev] = valid_v ? (unsigned)rv * (unsigned)sizeof(ACC) : zb;
             s_vc[ev] = valid_v ? ps.z + rv : -(1 << 30);
-            s_pv[ev] = pv;
+            s_pv[ep] = pv;
 #pragma unroll
             for (int p = 0; p < NP; p++) {
                 Cplx<ACC> qv;
@@ -644,7 +646,7 @@ ap_tiled_kernel(ApT a, const unsigned long long *__restrict__ offsets,
             const int ng = (nh + G - 1) / G;
             const unsigned *ui_r = s_ui + ra * kApSU;
             const unsigned *vi_r = s_vi + rb * kApSV;
-            const Cplx<ACC> *pv_r = s_pv + rb * kApSV;
+            const Cplx<ACC> *pv_r = s_pv + rb * kApSP;
             const Cplx<ACC> *q_r = s_q + ra * kApSV * NP;
             ACC tA[G], tB[G];
             auto load = [&](int j0, ACC (&t)[G]) {
@@ -1075,8 +1077,9 @@ using ApAlloc = std::function<int(const char *, size_t, void **)>;
 
 // Validates the call, fills the kernel arguments and the sizes; the tile edge
 // is T_req (16, 24 or 32) or, for 0, the one whose workgroup LDS lets the most
-// workgroups share a CU (the kernel is latency-bound), the larger on a tie
-// (fewer halo cells to write out and merge).
+// workgroups share a CU (the kernel is latency-bound), the smaller on a tie
+// (more tiles to balance over the CUs; measured faster than the larger one's
+// fewer halo cells on the 25-pointing mosaic).
 static int ap_setup(const ApTiledCall &c, int T_req, ApT &a, ApGeom &g) {
     NGC_CHECK(c.pol_identity, NGC_ERR_UNSUPPORTED,
               "tiled aperture gridder needs pol_map = arange(n_pol)");
@@ -1183,7 +1186,7 @@ static int ap_setup(const ApTiledCall &c, int T_req, ApT &a, ApGeom &g) {
     int T = T_req;
     if (T == 0) {
         size_t best_per_cu = 0;
-        for (int cand : {32, 24, 16}) {
+        for (int cand : {16, 24, 32}) {
             const size_t per_cu = (160 * 1024) / lds_for(cand);
             if (per_cu > best_per_cu) {
                 best_per_cu = per_cu;
