@@ -120,3 +120,26 @@ def test_make_grid(gpu):
     got = np.asarray(out["GRID"].values)[:, :, 0]
     assert rel_err(got, np.moveaxis(g, [0, 1], [-2, -1])) < 1e-12
     assert rel_err(np.asarray(out["SUM_WEIGHT"].values)[0], sw) < 1e-12
+
+
+def test_make_image_from_a_vis_zarr_store(gpu, tmp_path):
+    """The imaging chain on a multi-dataset read back from a vis.zarr store
+    (write_vis -> read_vis, zstd chunks over time and channel) gives the
+    image of the in-memory multi-dataset (to the FP64 atomics' summation
+    order)."""
+    from cngi_prototype_amd.dio import read_vis, write_vis
+    from cngi_prototype_amd.imaging import make_image, make_imaging_weight, make_psf
+    from cngi_prototype_amd.xds import Dataset
+    cfg, mxds = _mxds()
+    store = str(tmp_path / "c1slice.vis.zarr")
+    write_vis(mxds, store, chunks_on_disk={"time": 5, "chan": 2})
+    back = read_vis(store)
+    gp = _grid_parms(cfg)
+    imgs = []
+    for m in (mxds, back):
+        m = make_imaging_weight(m, {"weighting": "briggs", "robust": 0.5}, dict(gp),
+                                {"xds": "xds0"})
+        img = make_psf(m, Dataset(), dict(gp), {"xds": "xds0"}, {})
+        imgs.append(make_image(m, img, dict(gp), {"xds": "xds0"}, {}))
+    for var in ("IMAGE", "PSF", "SUM_WEIGHT"):
+        assert rel_err(np.asarray(imgs[1][var].values), np.asarray(imgs[0][var].values)) < 1e-12, var
