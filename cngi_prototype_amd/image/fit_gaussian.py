@@ -17,8 +17,7 @@ def _gaussian(params, sampling):
     width_x, width_y, rotation = params
     rot = np.deg2rad(90 - rotation)
     xs, ys = sampling[0] * 2 + 1, sampling[1] * 2 + 1
-    x = np.repeat(np.arange(xs), ys).reshape((xs, ys)) - sampling[0]
-    y = np.repeat(np.arange(ys), xs).reshape((ys, xs)).T - sampling[1]
+    x, y = np.meshgrid(np.arange(xs) - sampling[0], np.arange(ys) - sampling[1], indexing="ij")
     xp = x * np.cos(rot) - y * np.sin(rot)
     yp = x * np.sin(rot) + y * np.cos(rot)
     return np.exp(-((xp / width_x) ** 2 + (yp / width_y) ** 2) / 2.0)
@@ -55,8 +54,7 @@ def fit_planes(img, npix_window, sampling, cutoff, delta):
                     out[t, ch, p] = np.nan
                     continue
                 spl = RectBivariateSpline(d0, d1, plane)
-                xp = np.repeat(i0, len(i1)).reshape(len(i0), len(i1))
-                yp = np.repeat(i1, len(i0)).reshape(len(i1), len(i0)).T
+                xp, yp = np.meshgrid(i0, i1, indexing="ij")
                 vals = spl.ev(np.ravel(xp), np.ravel(yp))
                 interp = np.reshape(vals, [sampling[1], sampling[0]]).T
                 interp[interp < cutoff] = np.nan
