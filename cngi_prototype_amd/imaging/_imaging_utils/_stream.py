@@ -2,9 +2,10 @@
 
 The visibilities, weights, uvw and field ids of a cube live in pinned host
 memory; they are copied to the GPU in chunks of whole integrations (about
-`chunk_bytes` of visibilities each) on a copy stream while the previous chunk
-is gridded, and every chunk is gridded into the same device grid (accumulate).
-Two device buffers alternate.  This replaces the reference's dask graph over
+`chunk_bytes` of visibilities each, 2 GiB by default: large enough that a
+chunk's samples fill the gridder's tiles densely, small against HBM) on a copy
+stream while the previous chunk is gridded, and every chunk is gridded into
+the same device grid (accumulate).  Two device buffers alternate.  This replaces the reference's dask graph over
 time chunks with private grids and a tree-sum (_aperture_grid.py:25-142):
 the grid stays resident in HBM and each chunk adds to it in place.
 """
@@ -24,7 +25,7 @@ def pinned_copy(t):
 
 
 def stream_aperture_grid(host, freq, chan_map, pol_map, gcf, n_imag_chan, n_uv, delta_lm,
-                         oversampling, mode, precision, chunk_bytes=256 << 20, grid=None,
+                         oversampling, mode, precision, chunk_bytes=2 << 30, grid=None,
                          sum_weight=None):
     """host: dict of pinned CPU tensors uvw [T, B, 3] float64, weight [T, B, C, P]
     float32, field [T, B] int32 and (image mode) vis [T, B, C, P] complex64.
