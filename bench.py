@@ -1040,8 +1040,10 @@ def main():
         # ifftshift_kernel<float, float, 2> is a plain permuted copy of the
         # complex64 grid: planes * Npad^2 * 8 bytes in and the same out
         log(f"copy calibration: ifftshift of {n_planes * npad * npad * 8} bytes in and out")
+        nat.set_fft_mode(0)   # the rocFFT path runs the ifftshift copy kernel
         nat.grid_to_image(grid, [N, N], sum_weight=sw)
         torch.cuda.synchronize()
+        nat.set_fft_mode(1)
     ms_step = elapsed / args.steps * 1e3
     value = n_unflagged / (elapsed / args.steps) / 1e6
     phase_ms = {k: float(np.mean(v)) for k, v in phases.items()}
@@ -1090,10 +1092,11 @@ def main():
         torch.cuda.synchronize()
         log(f"error check {time.time() - te:.1f}s: {err}")
 
-    # the imaging transform of the gridded cube (make_image.py:116-130): ifftshift,
-    # rocFFT inverse, crop and correction, profiled per phase; the transform's
+    # the imaging transform of the gridded cube (make_image.py:116-130): the LDS
+    # row FFTs (ifftshift folded into the first), the tiled transpose, the column
+    # FFTs as rows, then crop and correction, profiled per phase; the transform's
     # algorithmic traffic is two passes over the complex64 grid, each reading
-    # and writing it once
+    # and writing it once.  rocFFT is timed beside it.
     corr = cropped_correction_factors([npad, npad], [N, N])
     nat.grid_to_image(grid, [N, N], sum_weight=sw, corr=corr)
     torch.cuda.synchronize()
@@ -1110,11 +1113,29 @@ def main():
     ms_g2i = (time.time() - t0) / n_fft * 1e3
     nat.profile_enable(False)
     grid_bytes = n_planes * npad * npad * 8
-    t_fft = fft_ph.get("rocfft_inverse", float("nan")) / 1e3
+    fft_phases = [k for k in fft_ph if k in ("fft_rows", "fft_transpose", "fft_cols",
+                                              "ifftshift", "rocfft_inverse")]
+    t_fft = sum(fft_ph[k] for k in fft_phases) / 1e3 if fft_phases else float("nan")
+    # the same transform through rocFFT (ifftshift kernel + library FFT)
+    nat.set_fft_mode(0)
+    nat.grid_to_image(grid, [N, N], sum_weight=sw, corr=corr)
+    torch.cuda.synchronize()
+    nat.profile_enable(True)
+    roc_ph = {}
+    for _ in range(n_fft):
+        img_t = nat.grid_to_image(grid, [N, N], sum_weight=sw, corr=corr)
+        for name, ms in nat.profile_read():
+            roc_ph[name] = roc_ph.get(name, 0.0) + ms / n_fft
+        del img_t
+    torch.cuda.synchronize()
+    nat.profile_enable(False)
+    nat.set_fft_mode(1)
     fft = dict(planes=int(n_planes), length=f"{npad} x {npad} (3^3 7 13 per axis)",
                grid_to_image_ms=round(ms_g2i, 3),
                phases_ms={k: round(v, 3) for k, v in fft_ph.items()},
                fft_ms=round(t_fft * 1e3, 3),
+               fft_phases=fft_phases,
+               rocfft_phases_ms={k: round(v, 3) for k, v in roc_ph.items()},
                fft_roofline=dict(bound="hbm", achieved=round(4 * grid_bytes / t_fft / 1e9, 1),
                                  peak=HBM_PEAK_GBS, unit="GB/s",
                                  frac=round(4 * grid_bytes / t_fft / 1e9 / HBM_PEAK_GBS, 4),

@@ -37,7 +37,7 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags", "ngc_set_tile_waves",
            "ngc_model_to_grid", "ngc_standard_degrid", "ngc_aperture_degrid", "ngc_mlp_sample_rows",
            "ngc_adam_schedule", "ngc_adam_step_sched", "ngc_set_bin_lanes",
-           "ngc_standard_grid_noalloc")
+           "ngc_standard_grid_noalloc", "ngc_set_fft_mode")
 
 
 class NativeError(RuntimeError):
@@ -85,6 +85,7 @@ _PROTOS = {
     "ngc_set_tile_rows": (_i32, [_vp, _i32]),
     "ngc_set_tile_waves": (_i32, [_vp, _i32]),
     "ngc_set_bin_lanes": (_i32, [_vp, _i32]),
+    "ngc_set_fft_mode": (_i32, [_vp, _i32]),
     "ngc_gcf_patterns": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _i32, _i32, _d,
                                 _d, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "ngc_gcf_resize": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
@@ -582,6 +583,13 @@ def profile_read():
         _check("ngc_profile_read", n)
     parts = names.value.decode().split(";") if n > 0 else []
     return list(zip(parts, ms[:n].tolist()))
+
+
+def set_fft_mode(mode):
+    """Image transform of single-precision grids on the current device: 1 (the
+    default) the LDS row FFTs where the lengths allow, 0 always rocFFT."""
+    ctx = context()
+    _check("ngc_set_fft_mode", load_library().ngc_set_fft_mode(ctx.handle, int(mode)))
 
 
 def set_tile_rows(rows):

@@ -149,10 +149,24 @@ static int grid_to_image_impl(ngc_context *ctx, hipStream_t s, const void *grid,
     const bool dbl = (dt == NGC_C128 || dt == NGC_F64);
     const int64_t cells = n_planes * n_u * n_v;
     const size_t elem = dbl ? 2 * sizeof(double) : 2 * sizeof(float);
+    double *d_corr = nullptr;
+    if (corr_l) {
+        std::vector<double> c(n_l + n_m);
+        for (int64_t i = 0; i < n_l; i++) c[i] = corr_l[i];
+        for (int64_t i = 0; i < n_m; i++) c[n_l + i] = corr_m[i];
+        void *p;
+        NGC_TRY(upload(ctx, s, "corr", c.data(), c.size() * sizeof(double), &p));
+        d_corr = (double *)p;
+    }
+    profile_new_call(ctx, s);
+    // single-precision grids of smooth lengths: the LDS transform (shift,
+    // rows, transpose, rows, crop); otherwise rocFFT
+    if (!dbl && ctx->fft_lds && lds_fft_supported(n_u) && lds_fft_supported(n_v))
+        return lds_grid_to_image(ctx, s, grid, dt, n_planes, n_u, n_v, n_l, n_m, sum_weight,
+                                 d_corr, image);
     void *buf;
     NGC_TRY(workspace(ctx, "fft_buffer", cells * elem, &buf));
     const int nb = blocks_for(cells);
-    profile_new_call(ctx, s);
     phase_begin(ctx, s, "ifftshift");
     switch (dt) {
     case NGC_C64:
@@ -185,15 +199,6 @@ static int grid_to_image_impl(ngc_context *ctx, hipStream_t s, const void *grid,
     rocfft_status st = rocfft_execute(plan->plan, bufs, nullptr, plan->info);
     NGC_CHECK(st == rocfft_status_success, NGC_ERR_FFT, "rocfft_execute failed");
     phase_end(ctx, s);
-    double *d_corr = nullptr;
-    if (corr_l) {
-        std::vector<double> c(n_l + n_m);
-        for (int64_t i = 0; i < n_l; i++) c[i] = corr_l[i];
-        for (int64_t i = 0; i < n_m; i++) c[n_l + i] = corr_m[i];
-        void *p;
-        NGC_TRY(upload(ctx, s, "corr", c.data(), c.size() * sizeof(double), &p));
-        d_corr = (double *)p;
-    }
     const int nb2 = blocks_for(n_planes * n_l * n_m);
     phase_begin(ctx, s, "crop_correct");
     if (dbl)
@@ -340,6 +345,14 @@ int ngc_grid_to_image(ngc_context *ctx, void *stream, const void *grid, int grid
     NGC_TRY(enter_stream(ctx, s));
     return grid_to_image_impl(ctx, s, grid, grid_dtype, n_planes, n_u, n_v, n_l, n_m,
                               sum_weight, corr_l, corr_m, image);
+}
+
+int ngc_set_fft_mode(ngc_context *ctx, int mode) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    NGC_CHECK(mode == 0 || mode == 1, NGC_ERR_ARG, "mode must be 0 (rocFFT) or 1 (LDS)");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->fft_lds = mode;
+    return NGC_OK;
 }
 
 int ngc_briggs_factors(ngc_context *ctx, void *stream, const double *density_grid,

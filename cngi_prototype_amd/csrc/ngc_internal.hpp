@@ -89,6 +89,7 @@ struct ngc_context {
     int bin_chan_lanes = 2;  // channels per binning wavefront (1, 2, 4, 8)
     int tile_waves = 0;      // wavefronts per tile of the tiled gridder (0: default)
     int n_cu = 256;          // compute units of the device
+    int fft_lds = 1;         // single-precision image transforms: LDS kernels where supported
     // autotuned (tile rows, binning channel lanes) per gridding shape
     std::map<std::string, std::pair<int, int>> tuned;
 };
@@ -140,6 +141,12 @@ inline std::vector<int32_t> padded_map(const int32_t *map, int64_t n) {
 int noalloc_scratch_setup(int device);
 
 int fft_destroy_plans(ngc_context *ctx);
+// The LDS image transform (lds_fft.hip): axis lengths with prime factors
+// <= 13 up to 8192; single-precision grids.  d_corr: device [n_l + n_m] or NULL.
+bool lds_fft_supported(int64_t n);
+int lds_grid_to_image(ngc_context *ctx, hipStream_t s, const void *grid, int dt,
+                      int64_t n_planes, int64_t n_u, int64_t n_v, int64_t n_l, int64_t n_m,
+                      const double *sum_weight, const double *d_corr, double *image);
 int fft_execute(ngc_context *ctx, hipStream_t s, void *buf, int64_t n_u, int64_t n_v,
                 int64_t batch, bool dbl, bool forward);
 

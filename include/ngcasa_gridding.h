@@ -234,9 +234,12 @@ int ngc_aperture_grid(ngc_context *ctx, void *stream, int mode, int precision,
                       double *sum_weight);
 
 /*
- * Grid -> image: ifftshift, unnormalised inverse 2-D FFT (rocFFT), fftshift,
- * crop of the centre n_l x n_m, real part x (n_u*n_v), then division by
- * sum_weight (0 -> 1) and by corr_l[l]*corr_m[m] when those are given.
+ * Grid -> image: ifftshift, unnormalised inverse 2-D FFT, fftshift, crop of
+ * the centre n_l x n_m, real part x (n_u*n_v), then division by sum_weight
+ * (0 -> 1) and by corr_l[l]*corr_m[m] when those are given.  The transform is
+ * rocFFT, or for single-precision grids whose axis lengths are products of 2,
+ * 3, 5, 7, 11 and 13 up to 8192 the library's own LDS row FFTs with a tiled
+ * transpose (see ngc_set_fft_mode).
  *   grid        device [n_planes, n_u, n_v] of grid_dtype (C64, C128, F32, F64);
  *               single-precision grids are transformed in FP32, double in FP64
  *   sum_weight  device float64 [n_planes] or NULL
@@ -577,6 +580,13 @@ int ngc_set_bin_lanes(ngc_context *ctx, int lanes);
  * call: names are written as a ';'-separated list into `names` (capacity
  * `names_len`), durations into `ms`.  Returns the number of phases.
  */
+/*
+ * Image transform of single-precision grids (no counterpart in the reference,
+ * which uses numpy/dask FFTs): mode 1 (default) uses the LDS kernels where
+ * the lengths allow, mode 0 always rocFFT.
+ */
+int ngc_set_fft_mode(ngc_context *ctx, int mode);
+
 int ngc_profile_enable(ngc_context *ctx, int enable);
 int ngc_profile_read(ngc_context *ctx, char *names, int64_t names_len, double *ms,
                      int64_t max);
