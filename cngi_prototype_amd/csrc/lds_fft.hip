@@ -8,13 +8,16 @@
 //                        (radices 8, 4, 2, 13, 11, 7, 5, 3): stage s with
 //                        radix R reads R values N/R apart, twiddles them by
 //                        the powers of exp(i 2 pi k / (Ns R)) (Ns = product of
-//                        the earlier radices, k = j mod Ns), takes their
-//                        R-point DFT against the R-th roots of unity held in
-//                        registers, and writes the results Ns apart from
-//                        (j - k) R + k; two LDS buffers alternate.  The
+//                        the earlier radices, k = j mod Ns by a multiply-high
+//                        with a host-computed reciprocal), takes their R-point
+//                        DFT with the R-th roots of unity as compile-time
+//                        constants (odd R in the paired form, (R-1)^2 real
+//                        multiply-adds instead of 4 (R-1)^2; R = 2, 4, 8 as
+//                        radix-2 butterflies) and writes the results Ns apart
+//                        from (j - k) R + k; two LDS buffers alternate.  The
 //                        twiddles come from one table exp(i 2 pi t / N), t < N,
 //                        computed in FP64 on the host.
-//   fft_transpose_kernel the planes transposed through 32 x 32 LDS tiles, so
+//   fft_transpose_kernel the planes transposed through 64 x 64 LDS tiles, so
 //                        that the second axis is again a row pass
 //   fft_crop_kernel      crop to the image, fftshift, real part, sum_weight and
 //                        correction (as crop_correct_kernel), reading the
@@ -35,20 +38,33 @@ constexpr int kFftMaxN = 8192;
 constexpr int kFftMaxStages = 16;
 constexpr int kFftBlock = 256;
 
+// Stage s: radix r, Ns = product of the earlier radices, step = N / (Ns r)
+// (twiddle table stride), magic = floor(2^32 / Ns) + 1, so that
+// j / Ns = umulhi(j, magic) for every j < N <= 8192 (j Ns < 2^32).
+struct FftStage {
+    int r, ns, step;
+    uint32_t magic;
+};
+
 struct FftRadices {
     int n, count;
-    int r[kFftMaxStages];
+    FftStage st[kFftMaxStages];
 };
 
 static bool fft_factor(int64_t n, FftRadices &f) {
     if (n < 2 || n > kFftMaxN) return false;
     f.n = (int)n;
     f.count = 0;
-    int64_t m = n;
+    int64_t m = n, ns = 1;
     for (int r : {8, 4, 2, 13, 11, 7, 5, 3}) {
         while (m % r == 0) {
             if (f.count == kFftMaxStages) return false;
-            f.r[f.count++] = r;
+            FftStage &s = f.st[f.count++];
+            s.r = r;
+            s.ns = (int)ns;
+            s.step = (int)(n / (ns * r));
+            s.magic = ns > 1 ? (uint32_t)((uint64_t(1) << 32) / (uint64_t)ns + 1) : 0u;
+            ns *= r;
             m /= r;
         }
     }
@@ -64,120 +80,244 @@ __device__ inline float2 fft_cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// One Stockham stage of radix R over the N values in A, written to B.
-template <int R>
-__device__ void fft_stage(const float2 *__restrict__ A, float2 *__restrict__ B, int N, int Ns,
-                          const float2 *__restrict__ tw) {
-    const int NR = N / R;
-    const int step = NR / Ns;  // N / (Ns R): twiddle index per unit of r k
-    float2 w[R];               // w[t] = exp(i 2 pi t / R) (sign of the table)
+// cos and sin of 2 pi t / R in double, by Taylor series on [-pi, pi], so that
+// the R-th roots of unity are compile-time constants of the butterflies
+constexpr double kPi = 3.14159265358979323846;
+
+constexpr double taylor_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 30; k++) {
+        term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+        sum += term;
+    }
+    return sum;
+}
+
+constexpr double taylor_sin(double x) {
+    double term = x, sum = x;
+    for (int k = 1; k < 30; k++) {
+        term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+
+// c[t] + i s[t] = exp(SIGN i 2 pi t / R)
+template <int R, int SIGN>
+struct DftRoots {
+    float c[R], s[R];
+    constexpr DftRoots() : c(), s() {
+        for (int t = 0; t < R; t++) {
+            double x = 2.0 * kPi * t / R;
+            if (x > kPi) x -= 2.0 * kPi;
+            c[t] = (float)taylor_cos(x);
+            s[t] = (float)(SIGN * taylor_sin(x));
+        }
+    }
+};
+
+// y_q = sum_m v_m exp(SIGN i 2 pi m q / R), in place.  Odd R pairs m with
+// R - m: with a_m = v_m + v_{R-m}, b_m = v_m - v_{R-m}, P = v_0 + sum a_m c_{mq}
+// and Q = sum s_{mq} (i b_m), y_q = P + Q and y_{R-q} = P - Q.  R = 2, 4, 8
+// are radix-2 butterflies whose multiplications by +-i are swaps.
+template <int SIGN>
+__device__ inline float2 fft_mul_i(float2 z) {   // z exp(SIGN i pi / 2)
+    return SIGN > 0 ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
+}
+
+template <int R, int SIGN>
+__device__ inline void fft_dft(float2 (&v)[R]) {
+    if constexpr (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = make_float2(a.x + b.x, a.y + b.y);
+        v[1] = make_float2(a.x - b.x, a.y - b.y);
+    } else if constexpr (R == 4) {
+        const float2 s02 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+        const float2 d02 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+        const float2 s13 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+        const float2 d13 = fft_mul_i<SIGN>(make_float2(v[1].x - v[3].x, v[1].y - v[3].y));
+        v[0] = make_float2(s02.x + s13.x, s02.y + s13.y);
+        v[2] = make_float2(s02.x - s13.x, s02.y - s13.y);
+        v[1] = make_float2(d02.x + d13.x, d02.y + d13.y);
+        v[3] = make_float2(d02.x - d13.x, d02.y - d13.y);
+    } else if constexpr (R == 8) {
+        float2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        fft_dft<4, SIGN>(e);
+        fft_dft<4, SIGN>(o);
+        constexpr float h = 0.70710678118654752f;
+        // o_q exp(SIGN i 2 pi q / 8)
+        o[1] = make_float2(h * (o[1].x - SIGN * o[1].y), h * (o[1].y + SIGN * o[1].x));
+        o[2] = fft_mul_i<SIGN>(o[2]);
+        o[3] = make_float2(h * (-o[3].x - SIGN * o[3].y), h * (-o[3].y + SIGN * o[3].x));
 #pragma unroll
-    for (int t = 0; t < R; t++) w[t] = tw[t * NR];
-    for (int j = threadIdx.x; j < NR; j += blockDim.x) {
-        const int k = j % Ns;
+        for (int q = 0; q < 4; q++) {
+            v[q] = make_float2(e[q].x + o[q].x, e[q].y + o[q].y);
+            v[q + 4] = make_float2(e[q].x - o[q].x, e[q].y - o[q].y);
+        }
+    } else {
+        static_assert(R % 2 == 1, "odd radix");
+        constexpr int H = (R - 1) / 2;
+        constexpr DftRoots<R, SIGN> W{};
+        float2 a[H], b[H];
+#pragma unroll
+        for (int m = 1; m <= H; m++) {
+            a[m - 1] = make_float2(v[m].x + v[R - m].x, v[m].y + v[R - m].y);
+            b[m - 1] = make_float2(v[m].x - v[R - m].x, v[m].y - v[R - m].y);
+        }
+        const float2 v0 = v[0];
+        float2 y0 = v0;
+#pragma unroll
+        for (int m = 0; m < H; m++) y0 = make_float2(y0.x + a[m].x, y0.y + a[m].y);
+        v[0] = y0;
+#pragma unroll
+        for (int q = 1; q <= H; q++) {
+            float2 P = v0, Q = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int m = 1; m <= H; m++) {
+                const float c = W.c[(m * q) % R], s = W.s[(m * q) % R];
+                P.x += a[m - 1].x * c;
+                P.y += a[m - 1].y * c;
+                Q.x -= b[m - 1].y * s;
+                Q.y += b[m - 1].x * s;
+            }
+            v[q] = make_float2(P.x + Q.x, P.y + Q.y);
+            v[R - q] = make_float2(P.x - Q.x, P.y - Q.y);
+        }
+    }
+}
+
+// One Stockham stage of radix R over the N values in A, written to B: the
+// butterfly j (k = j mod Ns) reads A[j + r N/R], twiddles input r by
+// tw[r k step] = exp(SIGN i 2 pi r k / (Ns R)) and writes output q to
+// B[(j - k) R + k + q Ns].
+template <int R, int SIGN>
+__device__ void fft_stage(const float2 *__restrict__ A, float2 *__restrict__ B, int N,
+                          const FftStage st, const float2 *__restrict__ tw) {
+    const int NR = N / R;
+    const int Ns = st.ns;
+    for (int j = threadIdx.x; j < NR; j += kFftBlock) {
         float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = A[j + r * NR];
+        int jd = j, k = 0;
         if (Ns > 1) {
+            jd = (int)__umulhi((uint32_t)j, st.magic);
+            k = j - jd * Ns;
+            const int ks = k * st.step;
 #pragma unroll
-            for (int r = 1; r < R; r++) v[r] = fft_cmul(v[r], tw[r * k * step]);
+            for (int r = 1; r < R; r++) v[r] = fft_cmul(v[r], tw[r * ks]);
         }
-        const int o = (j - k) * R + k;
+        fft_dft<R, SIGN>(v);
+        float2 *o = B + jd * Ns * R + k;
 #pragma unroll
-        for (int q = 0; q < R; q++) {
-            float2 acc = v[0];
-#pragma unroll
-            for (int m = 1; m < R; m++) {
-                const float2 c = w[(m * q) % R];
-                acc.x += v[m].x * c.x - v[m].y * c.y;
-                acc.y += v[m].x * c.y + v[m].y * c.x;
-            }
-            B[o + q * Ns] = acc;
-        }
+        for (int q = 0; q < R; q++) o[q * Ns] = v[q];
     }
 }
 
 // The whole transform of the N values in buffer A (scratch B); returns the
 // buffer that holds the result.
+template <int SIGN>
 __device__ float2 *fft_run(float2 *A, float2 *B, const FftRadices &f,
                            const float2 *__restrict__ tw) {
-    int Ns = 1;
     for (int s = 0; s < f.count; s++) {
-        const int R = f.r[s];
-        switch (R) {
-        case 2: fft_stage<2>(A, B, f.n, Ns, tw); break;
-        case 3: fft_stage<3>(A, B, f.n, Ns, tw); break;
-        case 4: fft_stage<4>(A, B, f.n, Ns, tw); break;
-        case 5: fft_stage<5>(A, B, f.n, Ns, tw); break;
-        case 7: fft_stage<7>(A, B, f.n, Ns, tw); break;
-        case 8: fft_stage<8>(A, B, f.n, Ns, tw); break;
-        case 11: fft_stage<11>(A, B, f.n, Ns, tw); break;
-        default: fft_stage<13>(A, B, f.n, Ns, tw); break;
+        const FftStage st = f.st[s];
+        switch (st.r) {
+        case 2: fft_stage<2, SIGN>(A, B, f.n, st, tw); break;
+        case 3: fft_stage<3, SIGN>(A, B, f.n, st, tw); break;
+        case 4: fft_stage<4, SIGN>(A, B, f.n, st, tw); break;
+        case 5: fft_stage<5, SIGN>(A, B, f.n, st, tw); break;
+        case 7: fft_stage<7, SIGN>(A, B, f.n, st, tw); break;
+        case 8: fft_stage<8, SIGN>(A, B, f.n, st, tw); break;
+        case 11: fft_stage<11, SIGN>(A, B, f.n, st, tw); break;
+        default: fft_stage<13, SIGN>(A, B, f.n, st, tw); break;
         }
         __syncthreads();
         float2 *t = A;
         A = B;
         B = t;
-        Ns *= R;
     }
     return A;
 }
 
 // IN: 0 complex rows of `in` (transformed in place is allowed), 1 complex64
 // grid, 2 float32 grid; for 1 and 2 the row is read through the ifftshift:
-// A[ku][kv] = grid[(ku + n_u/2) mod n_u][(kv + N/2) mod N].
-template <int IN>
-__global__ void __launch_bounds__(kFftBlock)
+// A[ku][kv] = grid[(ku + n_u/2) mod n_u][(kv + N/2) mod N], i.e. grid value w of
+// the row goes to A[(w - N/2) mod N].  Each thread issues its PER loads of a
+// row (contiguous, from the row's uniform base) before it writes any of them
+// to LDS.
+template <int IN, int PER>
+__global__ void __launch_bounds__(kFftBlock) __attribute__((amdgpu_waves_per_eu(PER <= 16 ? 4 : 2)))
 fft_rows_kernel(const float *in, float2 *out, int64_t n_rows, int64_t n_u, FftRadices f,
                 const float2 *__restrict__ tw) {
     extern __shared__ float2 s_fft[];
     const int N = f.n;
+    const int half = IN == 0 ? 0 : N / 2;
     for (int64_t row = blockIdx.x; row < n_rows; row += gridDim.x) {
         float2 *A = s_fft, *B = s_fft + N;
-        if (IN == 0) {
-            const float2 *src = reinterpret_cast<const float2 *>(in) + row * N;
-            for (int v = threadIdx.x; v < N; v += blockDim.x) A[v] = src[v];
-        } else {
+        int64_t src_row = row;
+        if (IN != 0) {
             const int64_t pl = row / n_u, ku = row - pl * n_u;
             int64_t su = ku + n_u / 2;
             if (su >= n_u) su -= n_u;
-            const int64_t base = (pl * n_u + su) * N;
-            const int half = N / 2;
-            for (int v = threadIdx.x; v < N; v += blockDim.x) {
-                int sv = v + half;
-                if (sv >= N) sv -= N;
-                if (IN == 1)
-                    A[v] = reinterpret_cast<const float2 *>(in)[base + sv];
+            src_row = pl * n_u + su;
+        }
+        float2 x[PER];
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int w = threadIdx.x + i * kFftBlock;
+            if (w < N) {
+                if (IN == 2)
+                    x[i] = make_float2((in + src_row * N)[w], 0.f);
                 else
-                    A[v] = make_float2(in[base + sv], 0.f);
+                    x[i] = (reinterpret_cast<const float2 *>(in) + src_row * N)[w];
             }
         }
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int w = threadIdx.x + i * kFftBlock;
+            int a = w - half;
+            if (a < 0) a += N;
+            if (w < N) A[a] = x[i];
+        }
         __syncthreads();
-        const float2 *res = fft_run(A, B, f, tw);
+        const float2 *res = fft_run<+1>(A, B, f, tw);
         float2 *dst = out + row * N;
-        for (int v = threadIdx.x; v < N; v += blockDim.x) dst[v] = res[v];
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int v = threadIdx.x + i * kFftBlock;
+            if (v < N) dst[v] = res[v];
+        }
         __syncthreads();  // the row's LDS is free for the next one
     }
 }
 
-// out[pl][c][r] = in[pl][r][c], [rows x cols] planes, 32 x 32 tiles
+// out[pl][c][r] = in[pl][r][c], [rows x cols] planes, 64 x 64 tiles: each
+// thread issues its 16 loads of a tile before it writes any of them to LDS
+constexpr int kTrTile = 64;
+
 __global__ void __launch_bounds__(256)
 fft_transpose_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int64_t n_planes,
                      int64_t rows, int64_t cols) {
-    __shared__ float2 tile[32][33];
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
-    const int64_t c0 = (int64_t)blockIdx.x * 32, r0 = (int64_t)blockIdx.y * 32;
+    __shared__ float2 tile[kTrTile][kTrTile + 1];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 4
+    const int64_t c0 = (int64_t)blockIdx.x * kTrTile, r0 = (int64_t)blockIdx.y * kTrTile;
+    constexpr int K = kTrTile / 4;
     for (int64_t pl = blockIdx.z; pl < n_planes; pl += gridDim.z) {
         const float2 *ip = in + pl * rows * cols;
         float2 *op = out + pl * rows * cols;
-        for (int k = ty; k < 32; k += 8) {
-            const int64_t r = r0 + k, c = c0 + tx;
-            if (r < rows && c < cols) tile[k][tx] = ip[r * cols + c];
+        float2 x[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            const int64_t r = r0 + ty + 4 * i, c = c0 + tx;
+            if (r < rows && c < cols) x[i] = ip[r * cols + c];
         }
+#pragma unroll
+        for (int i = 0; i < K; i++) tile[ty + 4 * i][tx] = x[i];
         __syncthreads();
-        for (int k = ty; k < 32; k += 8) {
-            const int64_t c = c0 + k, r = r0 + tx;
-            if (c < cols && r < rows) op[c * rows + r] = tile[tx][k];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            const int64_t c = c0 + ty + 4 * i, r = r0 + tx;
+            if (c < cols && r < rows) op[c * rows + r] = tile[tx][ty + 4 * i];
         }
         __syncthreads();
     }
@@ -239,17 +379,28 @@ static int fft_twiddles(ngc_context *ctx, hipStream_t s, int64_t n, int sign, co
     return NGC_OK;
 }
 
-template <int IN>
-static int fft_rows(hipStream_t s, const float *in, float2 *out, int64_t n_rows, int64_t n_u,
-                    const FftRadices &f, const float2 *tw) {
+template <int IN, int PER>
+static int fft_rows_launch(hipStream_t s, const float *in, float2 *out, int64_t n_rows,
+                           int64_t n_u, const FftRadices &f, const float2 *tw) {
     const size_t lds = 2 * (size_t)f.n * sizeof(float2);
-    auto kern = fft_rows_kernel<IN>;
+    auto kern = fft_rows_kernel<IN, PER>;
     NGC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
     const int64_t blocks = std::min<int64_t>(n_rows, int64_t(1) << 20);
     kern<<<(unsigned)blocks, kFftBlock, lds, s>>>(in, out, n_rows, n_u, f, tw);
     NGC_HIP(hipGetLastError());
     return NGC_OK;
+}
+
+// PER = loads per thread of one row, ceil(N / 256) rounded up to 4, 8, 12, 16, 32
+template <int IN>
+static int fft_rows(hipStream_t s, const float *in, float2 *out, int64_t n_rows, int64_t n_u,
+                    const FftRadices &f, const float2 *tw) {
+    if (f.n <= 4 * kFftBlock) return fft_rows_launch<IN, 4>(s, in, out, n_rows, n_u, f, tw);
+    if (f.n <= 8 * kFftBlock) return fft_rows_launch<IN, 8>(s, in, out, n_rows, n_u, f, tw);
+    if (f.n <= 12 * kFftBlock) return fft_rows_launch<IN, 12>(s, in, out, n_rows, n_u, f, tw);
+    if (f.n <= 16 * kFftBlock) return fft_rows_launch<IN, 16>(s, in, out, n_rows, n_u, f, tw);
+    return fft_rows_launch<IN, 32>(s, in, out, n_rows, n_u, f, tw);
 }
 
 int lds_grid_to_image(ngc_context *ctx, hipStream_t s, const void *grid, int dt,
@@ -273,7 +424,7 @@ int lds_grid_to_image(ngc_context *ctx, hipStream_t s, const void *grid, int dt,
         NGC_TRY(fft_rows<2>(s, (const float *)grid, (float2 *)b0, n_planes * n_u, n_u, fv, tw_v));
     phase_end(ctx, s);
     phase_begin(ctx, s, "fft_transpose");
-    const dim3 tg((unsigned)ceil_div(n_v, 32), (unsigned)ceil_div(n_u, 32),
+    const dim3 tg((unsigned)ceil_div(n_v, kTrTile), (unsigned)ceil_div(n_u, kTrTile),
                   (unsigned)std::min<int64_t>(n_planes, 65535));
     fft_transpose_kernel<<<tg, 256, 0, s>>>((const float2 *)b0, (float2 *)b1, n_planes, n_u, n_v);
     NGC_HIP(hipGetLastError());
