@@ -799,7 +799,7 @@ def streamed_leg(args):
 def predict_leg(args):
     """Visibility prediction at the C2 shapes: a model cube of 64 channels x
     4 pols at 2048^2 (2000 random point sources per plane, Jy/pixel) is
-    transformed to the 2457^2 grid (correction, padding, rocFFT) and degridded
+    transformed to the 2457^2 grid (correction, padding, LDS FFTs) and degridded
     at all 627.2M samples, complex64 grid and visibilities with FP64 taps and
     sums.  Compared with the FP64 path on the first 4 channels.  Roofline of
     the degrid kernel: the grid read once + uvw + the visibilities written."""

@@ -239,37 +239,88 @@ __device__ float2 *fft_run(float2 *A, float2 *B, const FftRadices &f,
     return A;
 }
 
-// IN: 0 complex rows of `in` (transformed in place is allowed), 1 complex64
-// grid, 2 float32 grid; for 1 and 2 the row is read through the ifftshift:
-// A[ku][kv] = grid[(ku + n_u/2) mod n_u][(kv + N/2) mod N], i.e. grid value w of
-// the row goes to A[(w - N/2) mod N].  Each thread issues its PER loads of a
-// row (contiguous, from the row's uniform base) before it writes any of them
-// to LDS.
-template <int IN, int PER>
+// Where a row pass reads and writes.  IN: 0 complex rows of `in` (in place
+// is allowed); 1 complex64 grid, 2 float32 grid, both read through the
+// ifftshift: A[ku][kv] = grid[(ku + n_u/2) mod n_u][(kv + N/2) mod N], i.e.
+// grid value w of the row goes to A[(w - N/2) mod N]; 3 the FP64 model image
+// [planes, n_l, n_m] padded to the rows_per_plane x N plane and ifftshift-ed,
+// divided by the correction image:
+// A[ku][kv] = X[l][m] / (corr[l] corr[n_l + m]) with l = (ku + n_l/2) mod
+// rows_per_plane, m = (kv + n_m/2) mod N, zero outside the model.  OUT: 0 the
+// row to the same row of `out`; 1 through the fftshift, row r of a plane to
+// row (r + rows_per_plane/2) mod rows_per_plane and value v to (v + N/2) mod N.
+struct FftRowArgs {
+    const void *in;
+    float2 *out;
+    int64_t n_rows, rows_per_plane;
+    int64_t n_l, n_m;        // IN 3
+    const double *corr;      // IN 3: [n_l + n_m] or NULL
+};
+
+// Each thread issues its PER loads of a row (contiguous from the row's
+// uniform base, but for the model's wrap) before it writes any of them to LDS.
+template <int IN, int OUT, int SIGN, int PER>
 __global__ void __launch_bounds__(kFftBlock) __attribute__((amdgpu_waves_per_eu(PER <= 16 ? 4 : 2)))
-fft_rows_kernel(const float *in, float2 *out, int64_t n_rows, int64_t n_u, FftRadices f,
-                const float2 *__restrict__ tw) {
+fft_rows_kernel(FftRowArgs g, FftRadices f, const float2 *__restrict__ tw) {
+    static_assert(IN != 3 || OUT == 0, "model rows are written unshifted");
     extern __shared__ float2 s_fft[];
     const int N = f.n;
-    const int half = IN == 0 ? 0 : N / 2;
-    for (int64_t row = blockIdx.x; row < n_rows; row += gridDim.x) {
+    const int half = (IN == 1 || IN == 2) ? N / 2 : 0;
+    const int64_t rpp = g.rows_per_plane;
+    for (int64_t row = blockIdx.x; row < g.n_rows; row += gridDim.x) {
         float2 *A = s_fft, *B = s_fft + N;
-        int64_t src_row = row;
-        if (IN != 0) {
-            const int64_t pl = row / n_u, ku = row - pl * n_u;
-            int64_t su = ku + n_u / 2;
-            if (su >= n_u) su -= n_u;
-            src_row = pl * n_u + su;
-        }
+        const int64_t pl = row / rpp, ku = row - pl * rpp;
         float2 x[PER];
+        if (IN == 3) {
+            int64_t l = ku + g.n_l / 2;
+            if (l >= rpp) l -= rpp;
+            const bool in_model = l < g.n_l;
+            if (!in_model) {   // a padding row: its transform is zero
+                float2 *dst = g.out + row * N;
+                for (int w = threadIdx.x; w < N; w += kFftBlock) dst[w] = make_float2(0.f, 0.f);
+                continue;
+            }
+            const double *src = reinterpret_cast<const double *>(g.in) + (pl * g.n_l + l) * g.n_m;
+            const int hm = (int)(g.n_m / 2);
+            // all loads first (model values and correction factors), then the
+            // divisions, so that no load waits behind another's arithmetic
+            double v[PER], c[PER];
 #pragma unroll
-        for (int i = 0; i < PER; i++) {
-            const int w = threadIdx.x + i * kFftBlock;
-            if (w < N) {
-                if (IN == 2)
-                    x[i] = make_float2((in + src_row * N)[w], 0.f);
-                else
-                    x[i] = (reinterpret_cast<const float2 *>(in) + src_row * N)[w];
+            for (int i = 0; i < PER; i++) {
+                const int a = threadIdx.x + i * kFftBlock;
+                int m = a + hm;
+                if (m >= N) m -= N;
+                v[i] = 0.0;
+                c[i] = 1.0;
+                if (a < N && m < g.n_m) {
+                    v[i] = src[m];
+                    if (g.corr) c[i] = g.corr[g.n_l + m];
+                }
+            }
+            const double cl = g.corr ? g.corr[l] : 1.0;
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                double y = v[i];
+                if (g.corr) y = y / (cl * c[i]);
+                x[i] = make_float2((float)y, 0.f);
+            }
+        } else {
+            int64_t src_row = row;
+            if (IN != 0) {
+                int64_t su = ku + rpp / 2;
+                if (su >= rpp) su -= rpp;
+                src_row = pl * rpp + su;
+            }
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                const int w = threadIdx.x + i * kFftBlock;
+                if (w < N) {
+                    if (IN == 2)
+                        x[i] = make_float2((reinterpret_cast<const float *>(g.in) + src_row * N)[w],
+                                           0.f);
+                    else
+                        x[i] = (reinterpret_cast<const float2 *>(g.in) + src_row * N)[w];
+                }
             }
         }
 #pragma unroll
@@ -280,27 +331,51 @@ fft_rows_kernel(const float *in, float2 *out, int64_t n_rows, int64_t n_u, FftRa
             if (w < N) A[a] = x[i];
         }
         __syncthreads();
-        const float2 *res = fft_run<+1>(A, B, f, tw);
-        float2 *dst = out + row * N;
+        const float2 *res = fft_run<SIGN>(A, B, f, tw);
+        int64_t dst_row = row;
+        if (OUT == 1) {
+            int64_t r = ku + rpp / 2;
+            if (r >= rpp) r -= rpp;
+            dst_row = pl * rpp + r;
+        }
+        float2 *dst = g.out + dst_row * N;
+        const int oh = OUT == 1 ? N / 2 : 0;   // dst[w] = res[(w - N/2) mod N]
 #pragma unroll
         for (int i = 0; i < PER; i++) {
-            const int v = threadIdx.x + i * kFftBlock;
-            if (v < N) dst[v] = res[v];
+            const int w = threadIdx.x + i * kFftBlock;
+            int v = w - oh;
+            if (v < 0) v += N;
+            if (w < N) dst[w] = res[v];
         }
         __syncthreads();  // the row's LDS is free for the next one
     }
 }
 
 // out[pl][c][r] = in[pl][r][c], [rows x cols] planes, 64 x 64 tiles: each
-// thread issues its 16 loads of a tile before it writes any of them to LDS
+// thread issues its 16 loads of a tile before it writes any of them to LDS.
+// Workgroups go round-robin to the 8 XCDs, so the tile index is remapped to
+// give each XCD a contiguous run of tiles: neighbouring tiles share the
+// cache lines at their edges (rows are not 128-byte aligned for odd lengths),
+// which are then fetched and written back through one L2.
 constexpr int kTrTile = 64;
+
+__device__ inline void xcd_tile(unsigned &tx, unsigned &ty) {
+    const unsigned nx = gridDim.x, total = gridDim.x * gridDim.y;
+    const unsigned id = blockIdx.y * nx + blockIdx.x;
+    const unsigned per = total / 8;
+    const unsigned t = id < per * 8 ? (id % 8) * per + id / 8 : id;
+    tx = t % nx;
+    ty = t / nx;
+}
 
 __global__ void __launch_bounds__(256)
 fft_transpose_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int64_t n_planes,
                      int64_t rows, int64_t cols) {
     __shared__ float2 tile[kTrTile][kTrTile + 1];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 4
-    const int64_t c0 = (int64_t)blockIdx.x * kTrTile, r0 = (int64_t)blockIdx.y * kTrTile;
+    unsigned bx, by;
+    xcd_tile(bx, by);
+    const int64_t c0 = (int64_t)bx * kTrTile, r0 = (int64_t)by * kTrTile;
     constexpr int K = kTrTile / 4;
     for (int64_t pl = blockIdx.z; pl < n_planes; pl += gridDim.z) {
         const float2 *ip = in + pl * rows * cols;
@@ -333,7 +408,9 @@ fft_crop_kernel(const float2 *__restrict__ T, int64_t n_planes, int64_t n_u, int
 #pragma clang fp contract(off)
     __shared__ float tile[32][33];   // [m - m0][l - l0]
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const int64_t l0 = (int64_t)blockIdx.x * 32, m0 = (int64_t)blockIdx.y * 32;
+    unsigned bx, by;
+    xcd_tile(bx, by);
+    const int64_t l0 = (int64_t)bx * 32, m0 = (int64_t)by * 32;
     for (int64_t pl = blockIdx.z; pl < n_planes; pl += gridDim.z) {
         const float2 *tp = T + pl * n_u * n_v;
         for (int k = ty; k < 32; k += 8) {
@@ -379,28 +456,46 @@ static int fft_twiddles(ngc_context *ctx, hipStream_t s, int64_t n, int sign, co
     return NGC_OK;
 }
 
-template <int IN, int PER>
-static int fft_rows_launch(hipStream_t s, const float *in, float2 *out, int64_t n_rows,
-                           int64_t n_u, const FftRadices &f, const float2 *tw) {
+template <int IN, int OUT, int SIGN, int PER>
+static int fft_rows_launch(hipStream_t s, const FftRowArgs &g, const FftRadices &f,
+                           const float2 *tw) {
     const size_t lds = 2 * (size_t)f.n * sizeof(float2);
-    auto kern = fft_rows_kernel<IN, PER>;
+    auto kern = fft_rows_kernel<IN, OUT, SIGN, PER>;
     NGC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds));
-    const int64_t blocks = std::min<int64_t>(n_rows, int64_t(1) << 20);
-    kern<<<(unsigned)blocks, kFftBlock, lds, s>>>(in, out, n_rows, n_u, f, tw);
+    const int64_t blocks = std::min<int64_t>(g.n_rows, int64_t(1) << 20);
+    kern<<<(unsigned)blocks, kFftBlock, lds, s>>>(g, f, tw);
     NGC_HIP(hipGetLastError());
     return NGC_OK;
 }
 
 // PER = loads per thread of one row, ceil(N / 256) rounded up to 4, 8, 12, 16, 32
-template <int IN>
-static int fft_rows(hipStream_t s, const float *in, float2 *out, int64_t n_rows, int64_t n_u,
-                    const FftRadices &f, const float2 *tw) {
-    if (f.n <= 4 * kFftBlock) return fft_rows_launch<IN, 4>(s, in, out, n_rows, n_u, f, tw);
-    if (f.n <= 8 * kFftBlock) return fft_rows_launch<IN, 8>(s, in, out, n_rows, n_u, f, tw);
-    if (f.n <= 12 * kFftBlock) return fft_rows_launch<IN, 12>(s, in, out, n_rows, n_u, f, tw);
-    if (f.n <= 16 * kFftBlock) return fft_rows_launch<IN, 16>(s, in, out, n_rows, n_u, f, tw);
-    return fft_rows_launch<IN, 32>(s, in, out, n_rows, n_u, f, tw);
+template <int IN, int OUT, int SIGN>
+static int fft_rows(hipStream_t s, const FftRowArgs &g, const FftRadices &f, const float2 *tw) {
+    if (f.n <= 4 * kFftBlock) return fft_rows_launch<IN, OUT, SIGN, 4>(s, g, f, tw);
+    if (f.n <= 8 * kFftBlock) return fft_rows_launch<IN, OUT, SIGN, 8>(s, g, f, tw);
+    if (f.n <= 12 * kFftBlock) return fft_rows_launch<IN, OUT, SIGN, 12>(s, g, f, tw);
+    if (f.n <= 16 * kFftBlock) return fft_rows_launch<IN, OUT, SIGN, 16>(s, g, f, tw);
+    return fft_rows_launch<IN, OUT, SIGN, 32>(s, g, f, tw);
+}
+
+static FftRowArgs row_args(const void *in, void *out, int64_t n_rows, int64_t rows_per_plane) {
+    FftRowArgs g{};
+    g.in = in;
+    g.out = (float2 *)out;
+    g.n_rows = n_rows;
+    g.rows_per_plane = rows_per_plane;
+    return g;
+}
+
+static int fft_transpose(hipStream_t s, const void *in, void *out, int64_t n_planes,
+                         int64_t rows, int64_t cols) {
+    const dim3 tg((unsigned)ceil_div(cols, kTrTile), (unsigned)ceil_div(rows, kTrTile),
+                  (unsigned)std::min<int64_t>(n_planes, 65535));
+    fft_transpose_kernel<<<tg, 256, 0, s>>>((const float2 *)in, (float2 *)out, n_planes, rows,
+                                            cols);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
 }
 
 int lds_grid_to_image(ngc_context *ctx, hipStream_t s, const void *grid, int dt,
@@ -418,19 +513,17 @@ int lds_grid_to_image(ngc_context *ctx, hipStream_t s, const void *grid, int dt,
     NGC_TRY(fft_twiddles(ctx, s, n_v, +1, "fft_tw_rows", &tw_v));
     NGC_TRY(fft_twiddles(ctx, s, n_u, +1, "fft_tw_cols", &tw_u));
     phase_begin(ctx, s, "fft_rows");
+    const FftRowArgs g1 = row_args(grid, b0, n_planes * n_u, n_u);
     if (dt == NGC_C64)
-        NGC_TRY(fft_rows<1>(s, (const float *)grid, (float2 *)b0, n_planes * n_u, n_u, fv, tw_v));
+        NGC_TRY((fft_rows<1, 0, +1>(s, g1, fv, tw_v)));
     else
-        NGC_TRY(fft_rows<2>(s, (const float *)grid, (float2 *)b0, n_planes * n_u, n_u, fv, tw_v));
+        NGC_TRY((fft_rows<2, 0, +1>(s, g1, fv, tw_v)));
     phase_end(ctx, s);
     phase_begin(ctx, s, "fft_transpose");
-    const dim3 tg((unsigned)ceil_div(n_v, kTrTile), (unsigned)ceil_div(n_u, kTrTile),
-                  (unsigned)std::min<int64_t>(n_planes, 65535));
-    fft_transpose_kernel<<<tg, 256, 0, s>>>((const float2 *)b0, (float2 *)b1, n_planes, n_u, n_v);
-    NGC_HIP(hipGetLastError());
+    NGC_TRY(fft_transpose(s, b0, b1, n_planes, n_u, n_v));
     phase_end(ctx, s);
     phase_begin(ctx, s, "fft_cols");
-    NGC_TRY(fft_rows<0>(s, (const float *)b1, (float2 *)b1, n_planes * n_v, n_v, fu, tw_u));
+    NGC_TRY((fft_rows<0, 0, +1>(s, row_args(b1, b1, n_planes * n_v, n_v), fu, tw_u)));
     phase_end(ctx, s);
     phase_begin(ctx, s, "crop_correct");
     const dim3 cg((unsigned)ceil_div(n_l, 32), (unsigned)ceil_div(n_m, 32),
@@ -438,6 +531,42 @@ int lds_grid_to_image(ngc_context *ctx, hipStream_t s, const void *grid, int dt,
     fft_crop_kernel<<<cg, 256, 0, s>>>((const float2 *)b1, n_planes, n_u, n_v, n_l, n_m,
                                        sum_weight, d_corr, image);
     NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
+    return NGC_OK;
+}
+
+// The forward placement of the model (the inverse of lds_grid_to_image):
+// grid = fftshift(fft2(ifftshift(padded model / correction))) for complex64
+// grids: rows of the padded model (IN 3), transpose, rows through the fftshift
+// (OUT 1), transpose into the grid.
+int lds_model_to_grid(ngc_context *ctx, hipStream_t s, const double *image, int64_t n_planes,
+                      int64_t n_l, int64_t n_m, int64_t n_u, int64_t n_v, const double *d_corr,
+                      void *grid) {
+    FftRadices fu, fv;
+    NGC_CHECK(fft_factor(n_u, fu) && fft_factor(n_v, fv), NGC_ERR_UNSUPPORTED,
+              "axis length not a product of 2, 3, 5, 7, 11, 13 up to 8192");
+    const size_t cells = (size_t)n_planes * n_u * n_v;
+    void *b0, *b1;
+    NGC_TRY(workspace(ctx, "fft_buffer", cells * sizeof(float2), &b0));
+    NGC_TRY(workspace(ctx, "fft_transposed", cells * sizeof(float2), &b1));
+    const float2 *tw_v, *tw_u;
+    NGC_TRY(fft_twiddles(ctx, s, n_v, -1, "fft_tw_rows_forward", &tw_v));
+    NGC_TRY(fft_twiddles(ctx, s, n_u, -1, "fft_tw_cols_forward", &tw_u));
+    phase_begin(ctx, s, "fft_rows");
+    FftRowArgs g1 = row_args(image, b0, n_planes * n_u, n_u);
+    g1.n_l = n_l;
+    g1.n_m = n_m;
+    g1.corr = d_corr;
+    NGC_TRY((fft_rows<3, 0, -1>(s, g1, fv, tw_v)));
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "fft_transpose");
+    NGC_TRY(fft_transpose(s, b0, b1, n_planes, n_u, n_v));
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "fft_cols");
+    NGC_TRY((fft_rows<0, 1, -1>(s, row_args(b1, b0, n_planes * n_v, n_v), fu, tw_u)));
+    phase_end(ctx, s);
+    phase_begin(ctx, s, "fft_transpose_out");
+    NGC_TRY(fft_transpose(s, b0, grid, n_planes, n_v, n_u));
     phase_end(ctx, s);
     return NGC_OK;
 }

@@ -147,6 +147,9 @@ bool lds_fft_supported(int64_t n);
 int lds_grid_to_image(ngc_context *ctx, hipStream_t s, const void *grid, int dt,
                       int64_t n_planes, int64_t n_u, int64_t n_v, int64_t n_l, int64_t n_m,
                       const double *sum_weight, const double *d_corr, double *image);
+int lds_model_to_grid(ngc_context *ctx, hipStream_t s, const double *image, int64_t n_planes,
+                      int64_t n_l, int64_t n_m, int64_t n_u, int64_t n_v, const double *d_corr,
+                      void *grid);
 int fft_execute(ngc_context *ctx, hipStream_t s, void *buf, int64_t n_u, int64_t n_v,
                 int64_t batch, bool dbl, bool forward);
 

@@ -9,9 +9,11 @@
 //
 //   ngc_model_to_grid     model image [planes, n_l, n_m] (Jy/pixel) divided by
 //                         the cropped correction image, padded to the centre of
-//                         the n_u x n_v grid, fftshift(fft2(ifftshift(.))) with
-//                         rocFFT (unnormalised forward): the inverse placement
-//                         of ngc_grid_to_image
+//                         the n_u x n_v grid, fftshift(fft2(ifftshift(.)))
+//                         (unnormalised forward): the inverse placement of
+//                         ngc_grid_to_image; complex64 grids of lengths with
+//                         prime factors up to 13 by the LDS row FFTs, others
+//                         by rocFFT
 //   ngc_standard_degrid   V = sum_taps conv * G[cell] (/ sum_taps conv), the
 //                         adjoint of the standard gridder's loop with unit
 //                         weights: same uv scale, centre cell, sub-cell offset,
@@ -219,8 +221,6 @@ extern "C" int ngc_model_to_grid(ngc_context *ctx, void *stream, const double *i
     NGC_TRY(enter_stream(ctx, s));
     const bool dbl = grid_dtype == NGC_C128;
     const int64_t cells = n_planes * n_u * n_v;
-    void *buf;
-    NGC_TRY(workspace(ctx, "fft_buffer", cells * (dbl ? 16 : 8), &buf));
     double *d_corr = nullptr;
     if (corr_l) {
         std::vector<double> c(n_l + n_m);
@@ -230,8 +230,14 @@ extern "C" int ngc_model_to_grid(ngc_context *ctx, void *stream, const double *i
         NGC_TRY(upload(ctx, s, "corr", c.data(), c.size() * sizeof(double), &p));
         d_corr = (double *)p;
     }
-    const int nb = pblocks(cells);
     profile_new_call(ctx, s);
+    // complex64 grids of smooth lengths: the LDS transform (padded rows,
+    // transpose, shifted rows, transpose); otherwise pad, rocFFT, fftshift
+    if (!dbl && ctx->fft_lds && lds_fft_supported(n_u) && lds_fft_supported(n_v))
+        return lds_model_to_grid(ctx, s, image, n_planes, n_l, n_m, n_u, n_v, d_corr, grid);
+    void *buf;
+    NGC_TRY(workspace(ctx, "fft_buffer", cells * (dbl ? 16 : 8), &buf));
+    const int nb = pblocks(cells);
     phase_begin(ctx, s, "model_pad");
     if (dbl)
         model_pad_kernel<double><<<nb, 256, 0, s>>>(image, n_planes, n_l, n_m, n_u, n_v, d_corr,

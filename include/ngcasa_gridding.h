@@ -380,7 +380,9 @@ int ngc_make_pb(ngc_context *ctx, void *stream, const double *freq_chan, int64_t
  * the centre of an n_u x n_v grid (start n_u//2 - n_l//2) and transformed as
  * fftshift(fft2(ifftshift(.))) (unnormalised forward): the inverse placement
  * of ngc_grid_to_image.  grid: device [n_planes, n_u, n_v] of grid_dtype
- * (NGC_C64|NGC_C128).
+ * (NGC_C64|NGC_C128).  Complex64 grids whose lengths are products of 2, 3, 5,
+ * 7, 11 and 13 up to 8192 use the LDS row FFTs, others rocFFT (see
+ * ngc_set_fft_mode).
  *
  * ngc_standard_degrid: for every (time, baseline, chan, pol) sample placed on
  * the grid as the standard gridder places it (same uv scale, centre cell,
@@ -581,9 +583,9 @@ int ngc_set_bin_lanes(ngc_context *ctx, int lanes);
  * `names_len`), durations into `ms`.  Returns the number of phases.
  */
 /*
- * Image transform of single-precision grids (no counterpart in the reference,
- * which uses numpy/dask FFTs): mode 1 (default) uses the LDS kernels where
- * the lengths allow, mode 0 always rocFFT.
+ * FFTs of single-precision grids in ngc_grid_to_image and ngc_model_to_grid
+ * (no counterpart in the reference, which uses numpy/dask FFTs): mode 1
+ * (default) uses the LDS kernels where the lengths allow, mode 0 always rocFFT.
  */
 int ngc_set_fft_mode(ngc_context *ctx, int mode);
 

@@ -1,7 +1,9 @@
-"""The LDS image transform of single-precision grids (lds_fft.hip) against the
-oracle's numpy transform and against rocFFT on the same grids: every radix
-(2, 3, 4, 5, 7, 8, 11, 13), odd and non-square planes (ifftshift != fftshift),
-complex64 and float32 grids, sum_weight and the correction factors."""
+"""The LDS transforms of single-precision grids (lds_fft.hip) against the
+oracle's numpy transforms and against rocFFT on the same inputs: every radix
+(2, 3, 4, 5, 7, 8, 11, 13), odd and non-square planes (ifftshift != fftshift).
+Inverse (grid_to_image): complex64 and float32 grids, sum_weight and the
+correction factors; forward (model_to_grid): padded odd and even models,
+correction division, point sources."""
 import numpy as np
 import pytest
 
@@ -73,3 +75,54 @@ def test_lds_transform_of_a_point_source(gpu):
     l = np.arange(N) - N // 2
     want = np.cos(2 * np.pi * (du * l[:, None] / n_u + dv * l[None, :] / n_v))
     assert np.abs(img - want).max() < 2e-5
+
+
+@pytest.mark.parametrize("n_u,n_v,N_l,N_m", [
+    (120, 120, 100, 100),      # 8 3 5, even model
+    (63, 65, 50, 41),          # 7 9 and 5 13, odd grid and model
+    (1001, 77, 800, 60),       # 7 11 13 and 7 11
+    (64, 1024, 63, 1000),      # 8 8 and 8 8 8 2
+    (2457, 2457, 2048, 2048),  # the C2 plane
+])
+def test_lds_model_to_grid_matches_oracle_and_rocfft(gpu, n_u, n_v, N_l, N_m):
+    """The forward placement (pad, ifftshift, fft2, fftshift) of the LDS path
+    against the oracle's numpy FFT and against rocFFT, with the correction
+    division."""
+    import cngi_prototype_amd._native as nat
+    from cngi_prototype_amd.imaging._imaging_utils._gridding_convolutional_kernels import \
+        cropped_correction_factors
+    rng = np.random.default_rng(n_u * 7 + n_v)
+    n_planes = 2 if n_u * n_v > 1e6 else 3
+    model = rng.normal(size=(N_l, N_m, n_planes, 1))
+    cl, cm = cropped_correction_factors([n_u, n_v], [N_l, N_m])
+    ref = oracle.model_to_grid(model, [n_u, n_v])   # divides by the correction image
+    md = _dev(np.moveaxis(model, [0, 1], [-2, -1]))
+    out = {}
+    for mode in (1, 0):
+        nat.set_fft_mode(mode)
+        out[mode] = nat.model_to_grid(md, [n_u, n_v], cl, cm, single=True).cpu().numpy()
+    nat.set_fft_mode(1)
+    assert out[1].shape == ref.shape
+    assert rel_err(out[1], ref) < 2e-6
+    assert rel_err(out[0], ref) < 2e-6
+    assert rel_err(out[1], out[0]) < 2e-6
+
+
+def test_lds_model_to_grid_of_a_point_source(gpu):
+    """A unit pixel at the image centre is a flat grid of ones; one at
+    (dl, dm) from the centre the plane wave exp(-2 pi i (du dl / n_u +
+    dv dm / n_v)) in (du, dv) from the grid centre."""
+    import cngi_prototype_amd._native as nat
+    n_u, n_v, N = 2457, 2457, 2048
+    m = np.zeros((1, 1, N, N))
+    m[0, 0, N // 2, N // 2] = 1.0
+    g = nat.model_to_grid(_dev(m), [n_u, n_v], single=True).cpu().numpy()[0, 0]
+    assert np.allclose(g, 1.0, rtol=0, atol=1e-6)
+    m[:] = 0
+    dl, dm = 7, -4
+    m[0, 0, N // 2 + dl, N // 2 + dm] = 1.0
+    g = nat.model_to_grid(_dev(m), [n_u, n_v], single=True).cpu().numpy()[0, 0]
+    du = np.arange(n_u) - n_u // 2
+    dv = np.arange(n_v) - n_v // 2
+    want = np.exp(-2j * np.pi * (du[:, None] * dl / n_u + dv[None, :] * dm / n_v))
+    assert np.abs(g - want).max() < 2e-5
