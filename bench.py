@@ -1093,10 +1093,10 @@ def main():
         log(f"error check {time.time() - te:.1f}s: {err}")
 
     # the imaging transform of the gridded cube (make_image.py:116-130): the LDS
-    # row FFTs (ifftshift folded into the first), the tiled transpose, the column
-    # FFTs as rows, then crop and correction, profiled per phase; the transform's
-    # algorithmic traffic is two passes over the complex64 grid, each reading
-    # and writing it once.  rocFFT is timed beside it.
+    # row FFTs (ifftshift folded in), then the columns in two in-place steps
+    # whose second writes the cropped, corrected image, profiled per phase; the
+    # transform's algorithmic traffic is two passes over the complex64 grid,
+    # each reading and writing it once.  rocFFT is timed beside it.
     corr = cropped_correction_factors([npad, npad], [N, N])
     nat.grid_to_image(grid, [N, N], sum_weight=sw, corr=corr)
     torch.cuda.synchronize()
@@ -1113,8 +1113,9 @@ def main():
     ms_g2i = (time.time() - t0) / n_fft * 1e3
     nat.profile_enable(False)
     grid_bytes = n_planes * npad * npad * 8
-    fft_phases = [k for k in fft_ph if k in ("fft_rows", "fft_transpose", "fft_cols",
-                                              "ifftshift", "rocfft_inverse")]
+    # every phase but a separate crop (the four-step column pass writes the
+    # cropped, corrected image itself)
+    fft_phases = [k for k in fft_ph if k != "crop_correct"]
     t_fft = sum(fft_ph[k] for k in fft_phases) / 1e3 if fft_phases else float("nan")
     # the same transform through rocFFT (ifftshift kernel + library FFT)
     nat.set_fft_mode(0)

@@ -1,6 +1,7 @@
-// Single-precision 2-D inverse FFT of gridded planes for the image transform
-// (make_image.py:116-130: ifftshift, ifft2, fftshift, crop and correction),
-// for axis lengths whose prime factors are at most 13 (C2: 2457 = 3^3 7 13).
+// Single-precision 2-D FFTs of gridded planes for axis lengths whose prime
+// factors are at most 13 (C2: 2457 = 3^3 7 13): the inverse image transform
+// (make_image.py:116-130: ifftshift, ifft2, fftshift, crop and correction) and
+// the forward placement of a model image for prediction.
 //
 //   fft_rows_kernel      one workgroup per row: the row is loaded (through the
 //                        ifftshift, for the first pass) into LDS and
@@ -17,15 +18,22 @@
 //                        from (j - k) R + k; two LDS buffers alternate.  The
 //                        twiddles come from one table exp(i 2 pi t / N), t < N,
 //                        computed in FP64 on the host.
-//   fft_transpose_kernel the planes transposed through 64 x 64 LDS tiles, so
-//                        that the second axis is again a row pass
+//   fft_cols_a_kernel,   the columns without a transpose (four-step, N = N1 N2):
+//   fft_cols_b_kernel    step A transforms the N1 rows r1 N2 + r2 of a
+//                        32-column block in LDS and twiddles them in place;
+//                        step B transforms N2 consecutive rows and writes the
+//                        result rows k1 + N1 k2 straight into the cropped,
+//                        corrected image (or, forward, the fftshifted grid)
+//   fft_transpose_kernel for column lengths with no split N1 <= N2 <= 128: the
+//                        planes transposed through 64 x 64 LDS tiles, so that
+//                        the second axis is again a row pass, then
 //   fft_crop_kernel      crop to the image, fftshift, real part, sum_weight and
 //                        correction (as crop_correct_kernel), reading the
 //                        transposed planes through LDS tiles
 //
-// Passes over the planes: rows (read + write), transpose (read + write), rows
-// (read + write), crop (reads the image region); the library transform it
-// replaces took 66 ms for 256 planes of 2457^2, about 9 % of HBM.
+// Passes over the planes: rows (read + write), step A (read + write), step B
+// (read, and the image written); the library transform it replaces took 66 ms
+// for 256 planes of 2457^2, about 9 % of HBM.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -186,50 +194,55 @@ __device__ inline void fft_dft(float2 (&v)[R]) {
     }
 }
 
-// One Stockham stage of radix R over the N values in A, written to B: the
-// butterfly j (k = j mod Ns) reads A[j + r N/R], twiddles input r by
-// tw[r k step] = exp(SIGN i 2 pi r k / (Ns R)) and writes output q to
-// B[(j - k) R + k + q Ns].
-template <int R, int SIGN>
+// One Stockham stage of radix R over the N values of one transform, written
+// from A to B: the butterfly j (k = j mod Ns) reads A[j + r N/R], twiddles
+// input r by tw[r k step TS] = exp(SIGN i 2 pi r k / (Ns R)) and writes output
+// q to B[(j - k) R + k + q Ns].  Value e of the transform sits at LDS index
+// e STR + c, and the transform's L threads take butterflies lt, lt + L, ...
+// (rows: L = 256, STR = 1, c = 0; column blocks: L = 8 threads per column of
+// STR = 32 columns).  TS is the stride of the twiddle table: a table for N TS
+// values serves a transform of N.
+template <int R, int SIGN, int L, int STR>
 __device__ void fft_stage(const float2 *__restrict__ A, float2 *__restrict__ B, int N,
-                          const FftStage st, const float2 *__restrict__ tw) {
+                          const FftStage st, const float2 *__restrict__ tw, int TS, int lt,
+                          int c) {
     const int NR = N / R;
     const int Ns = st.ns;
-    for (int j = threadIdx.x; j < NR; j += kFftBlock) {
+    for (int j = lt; j < NR; j += L) {
         float2 v[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) v[r] = A[j + r * NR];
+        for (int r = 0; r < R; r++) v[r] = A[(j + r * NR) * STR + c];
         int jd = j, k = 0;
         if (Ns > 1) {
             jd = (int)__umulhi((uint32_t)j, st.magic);
             k = j - jd * Ns;
-            const int ks = k * st.step;
+            const int ks = k * st.step * TS;
 #pragma unroll
             for (int r = 1; r < R; r++) v[r] = fft_cmul(v[r], tw[r * ks]);
         }
         fft_dft<R, SIGN>(v);
-        float2 *o = B + jd * Ns * R + k;
+        float2 *o = B + (jd * Ns * R + k) * STR + c;
 #pragma unroll
-        for (int q = 0; q < R; q++) o[q * Ns] = v[q];
+        for (int q = 0; q < R; q++) o[q * Ns * STR] = v[q];
     }
 }
 
-// The whole transform of the N values in buffer A (scratch B); returns the
-// buffer that holds the result.
-template <int SIGN>
+// The whole transform in buffer A (scratch B); returns the buffer that holds
+// the result.
+template <int SIGN, int L, int STR>
 __device__ float2 *fft_run(float2 *A, float2 *B, const FftRadices &f,
-                           const float2 *__restrict__ tw) {
+                           const float2 *__restrict__ tw, int TS, int lt, int c) {
     for (int s = 0; s < f.count; s++) {
         const FftStage st = f.st[s];
         switch (st.r) {
-        case 2: fft_stage<2, SIGN>(A, B, f.n, st, tw); break;
-        case 3: fft_stage<3, SIGN>(A, B, f.n, st, tw); break;
-        case 4: fft_stage<4, SIGN>(A, B, f.n, st, tw); break;
-        case 5: fft_stage<5, SIGN>(A, B, f.n, st, tw); break;
-        case 7: fft_stage<7, SIGN>(A, B, f.n, st, tw); break;
-        case 8: fft_stage<8, SIGN>(A, B, f.n, st, tw); break;
-        case 11: fft_stage<11, SIGN>(A, B, f.n, st, tw); break;
-        default: fft_stage<13, SIGN>(A, B, f.n, st, tw); break;
+        case 2: fft_stage<2, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
+        case 3: fft_stage<3, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
+        case 4: fft_stage<4, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
+        case 5: fft_stage<5, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
+        case 7: fft_stage<7, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
+        case 8: fft_stage<8, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
+        case 11: fft_stage<11, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
+        default: fft_stage<13, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
         }
         __syncthreads();
         float2 *t = A;
@@ -331,7 +344,7 @@ fft_rows_kernel(FftRowArgs g, FftRadices f, const float2 *__restrict__ tw) {
             if (w < N) A[a] = x[i];
         }
         __syncthreads();
-        const float2 *res = fft_run<SIGN>(A, B, f, tw);
+        const float2 *res = fft_run<SIGN, kFftBlock, 1>(A, B, f, tw, 1, threadIdx.x, 0);
         int64_t dst_row = row;
         if (OUT == 1) {
             int64_t r = ku + rpp / 2;
@@ -442,6 +455,239 @@ fft_crop_kernel(const float2 *__restrict__ T, int64_t n_planes, int64_t n_u, int
     }
 }
 
+// ---------------------------------------------------------------------------
+// Column transforms without a transpose (four-step): the length-N transform
+// along u of the [N][n_cols] planes, N = N1 N2, input row n = r1 N2 + r2,
+// output row k = k1 + N1 k2:
+//   X[k1 + N1 k2] = sum_r2 w_N^(r2 k1) w_N2^(r2 k2) sum_r1 x[r1 N2 + r2] w_N1^(r1 k1)
+// Step A, per (plane, r2, 32-column block): the N1 rows r1 N2 + r2 through LDS,
+// their length-N1 transforms, the twiddle w_N^(r2 k1), written back in place
+// to rows k1 N2 + r2.  Step B, per (plane, k1, block): the N2 consecutive rows
+// k1 N2 + r2, their length-N2 transforms, and output k2 written as row
+// k1 + N1 k2 of the result: cropped, corrected and real into the image
+// (inverse), or through the fftshift into the grid (forward).  Each row
+// segment a workgroup reads or writes is 32 values (256 bytes) wide; 8
+// threads share a column.  The length-N1 and length-N2 twiddles are the
+// length-N table read at strides N2 and N1.
+// ---------------------------------------------------------------------------
+constexpr int kColW = 32;
+constexpr int kColL = kFftBlock / kColW;   // threads per column
+
+__device__ inline int64_t xcd_block() {
+    const int64_t total = gridDim.x, id = blockIdx.x;
+    const int64_t per = total / 8;
+    return id < per * 8 ? (id % 8) * per + id / 8 : id;
+}
+
+struct ColArgs {
+    float2 *buf;             // [n_planes][N][n_cols], after the row pass
+    int64_t n_planes;
+    int N, n_cols, N1, N2;
+    // step B outputs
+    double *image;           // inverse: [n_planes][n_l][n_m]
+    int64_t n_l, n_m;
+    const double *sum_weight, *corr;
+    float2 *grid;            // forward: [n_planes][N][n_cols]
+};
+
+template <int SIGN, int PER>
+__global__ void __launch_bounds__(kFftBlock)
+fft_cols_a_kernel(ColArgs a, FftRadices f1, const float2 *__restrict__ tw) {
+    extern __shared__ float2 s_fft[];
+    const int N1 = a.N1, N2 = a.N2, nc = a.n_cols;
+    const int n_cg = (nc + kColW - 1) / kColW;
+    const int64_t b = xcd_block();
+    const int cg = (int)(b % n_cg);
+    const int64_t rest = b / n_cg;
+    const int r2 = (int)(rest % N2);
+    const int64_t pl = rest / N2;
+    const int c = threadIdx.x % kColW, lt = threadIdx.x / kColW;
+    const int col = cg * kColW + c;
+    const bool cok = col < nc;
+    float2 *base = a.buf + (pl * a.N + r2) * nc + col;   // row r1 N2 + r2 at r1 N2 nc
+    const int64_t rs = (int64_t)N2 * nc;
+    float2 *A = s_fft, *B = s_fft + N1 * kColW;
+    float2 x[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int r1 = lt + i * kColL;
+        if (r1 < N1 && cok) x[i] = base[r1 * rs];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int r1 = lt + i * kColL;
+        if (r1 < N1) A[r1 * kColW + c] = cok ? x[i] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const float2 *res = fft_run<SIGN, kColL, kColW>(A, B, f1, tw, N2, lt, c);
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int k1 = lt + i * kColL;
+        if (k1 < N1 && cok) base[k1 * rs] = fft_cmul(res[k1 * kColW + c], tw[r2 * k1]);
+    }
+}
+
+// MODE 0: image[pl][l][m] = (Re X[ku][kv] / sw') / (corr[l] corr[n_l + m]) for
+// l = (ku + n_l/2) mod N < n_l, m = (kv + n_m/2) mod n_cols < n_m, as
+// crop_correct_kernel computes it.  MODE 1: grid[pl][u][v] = X[ku][kv] for
+// u = (ku + N/2) mod N, v = (kv + n_cols/2) mod n_cols (the fftshift).
+template <int SIGN, int PER, int MODE>
+__global__ void __launch_bounds__(kFftBlock)
+fft_cols_b_kernel(ColArgs a, FftRadices f2, const float2 *__restrict__ tw) {
+#pragma clang fp contract(off)
+    extern __shared__ float2 s_fft[];
+    const int N1 = a.N1, N2 = a.N2, nc = a.n_cols, N = a.N;
+    const int n_cg = (nc + kColW - 1) / kColW;
+    const int64_t b = xcd_block();
+    const int cg = (int)(b % n_cg);
+    const int64_t rest = b / n_cg;
+    const int k1 = (int)(rest % N1);
+    const int64_t pl = rest / N1;
+    const int c = threadIdx.x % kColW, lt = threadIdx.x / kColW;
+    const int col = cg * kColW + c;
+    const bool cok = col < nc;
+    const float2 *base = a.buf + (pl * N + (int64_t)k1 * N2) * nc + col;
+    float2 *A = s_fft, *B = s_fft + N2 * kColW;
+    float2 x[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int r2 = lt + i * kColL;
+        if (r2 < N2 && cok) x[i] = base[(int64_t)r2 * nc];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int r2 = lt + i * kColL;
+        if (r2 < N2) A[r2 * kColW + c] = cok ? x[i] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const float2 *res = fft_run<SIGN, kColL, kColW>(A, B, f2, tw, N1, lt, c);
+    if (MODE == 0) {
+        int m = col + (int)(a.n_m / 2);
+        if (m >= nc) m -= nc;
+        const bool mok = cok && m < a.n_m;
+        double sw = 1.0;
+        if (a.sum_weight) {
+            sw = a.sum_weight[pl];
+            if (sw == 0.0) sw = 1.0;
+        }
+        const double cm = (a.corr && mok) ? a.corr[a.n_l + m] : 1.0;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int k2 = lt + i * kColL;
+            if (k2 >= N2) continue;
+            int l = k1 + N1 * k2 + (int)(a.n_l / 2);
+            if (l >= N) l -= N;
+            if (mok && l < a.n_l) {
+                double v = (double)res[k2 * kColW + c].x;
+                if (a.sum_weight) v = v / sw;
+                if (a.corr) v = v / (a.corr[l] * cm);
+                a.image[(pl * a.n_l + l) * a.n_m + m] = v;
+            }
+        }
+    } else {
+        int v = col + nc / 2;
+        if (v >= nc) v -= nc;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int k2 = lt + i * kColL;
+            if (k2 >= N2) continue;
+            int u = k1 + N1 * k2 + N / 2;
+            if (u >= N) u -= N;
+            if (cok) a.grid[(pl * N + u) * nc + v] = res[k2 * kColW + c];
+        }
+    }
+}
+
+// N = N1 N2 with N1 <= N2 <= 128 and N2 as small as possible, or N1 = 1 for
+// N <= 128 (step A is then skipped); false when no such split exists.
+static bool col_split(int64_t N, int &N1, int &N2) {
+    if (N <= 128) {
+        N1 = 1;
+        N2 = (int)N;
+        return true;
+    }
+    for (int n2 = 2; n2 <= 128; n2++) {
+        if (N % n2 == 0 && N / n2 <= n2) {
+            N1 = (int)(N / n2);
+            N2 = n2;
+            return true;
+        }
+    }
+    return false;
+}
+
+template <int SIGN, int PER>
+static int cols_a_launch(hipStream_t s, const ColArgs &a, const FftRadices &f1,
+                         const float2 *tw, int64_t blocks) {
+    const size_t lds = 2 * (size_t)a.N1 * kColW * sizeof(float2);
+    auto kern = fft_cols_a_kernel<SIGN, PER>;
+    NGC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    kern<<<(unsigned)blocks, kFftBlock, lds, s>>>(a, f1, tw);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+template <int SIGN, int PER, int MODE>
+static int cols_b_launch(hipStream_t s, const ColArgs &a, const FftRadices &f2,
+                         const float2 *tw, int64_t blocks) {
+    const size_t lds = 2 * (size_t)a.N2 * kColW * sizeof(float2);
+    auto kern = fft_cols_b_kernel<SIGN, PER, MODE>;
+    NGC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds));
+    kern<<<(unsigned)blocks, kFftBlock, lds, s>>>(a, f2, tw);
+    NGC_HIP(hipGetLastError());
+    return NGC_OK;
+}
+
+// PER = rows per thread, ceil(N1 or N2 / 8) rounded up to 2, 4, 8, 16
+template <int SIGN>
+static int cols_a(hipStream_t s, const ColArgs &a, const FftRadices &f1, const float2 *tw,
+                  int64_t blocks) {
+    if (a.N1 <= 2 * kColL) return cols_a_launch<SIGN, 2>(s, a, f1, tw, blocks);
+    if (a.N1 <= 4 * kColL) return cols_a_launch<SIGN, 4>(s, a, f1, tw, blocks);
+    if (a.N1 <= 8 * kColL) return cols_a_launch<SIGN, 8>(s, a, f1, tw, blocks);
+    return cols_a_launch<SIGN, 16>(s, a, f1, tw, blocks);
+}
+
+template <int SIGN, int MODE>
+static int cols_b(hipStream_t s, const ColArgs &a, const FftRadices &f2, const float2 *tw,
+                  int64_t blocks) {
+    if (a.N2 <= 2 * kColL) return cols_b_launch<SIGN, 2, MODE>(s, a, f2, tw, blocks);
+    if (a.N2 <= 4 * kColL) return cols_b_launch<SIGN, 4, MODE>(s, a, f2, tw, blocks);
+    if (a.N2 <= 8 * kColL) return cols_b_launch<SIGN, 8, MODE>(s, a, f2, tw, blocks);
+    return cols_b_launch<SIGN, 16, MODE>(s, a, f2, tw, blocks);
+}
+
+// Steps A and B over the row-pass output in `buf` (phases "fft_cols_a",
+// "fft_cols_b"); false in *done when the column length has no split or the
+// block counts exceed one launch (the caller then transposes).
+template <int SIGN, int MODE>
+static int cols_four_step(ngc_context *ctx, hipStream_t s, ColArgs a, const float2 *tw,
+                          bool *done) {
+    *done = false;
+    int N1, N2;
+    if (!col_split(a.N, N1, N2)) return NGC_OK;
+    a.N1 = N1;
+    a.N2 = N2;
+    const int64_t n_cg = ceil_div((int64_t)a.n_cols, (int64_t)kColW);
+    const int64_t blocks_a = a.n_planes * N2 * n_cg, blocks_b = a.n_planes * N1 * n_cg;
+    if (blocks_a >= (int64_t(1) << 31) || blocks_b >= (int64_t(1) << 31)) return NGC_OK;
+    FftRadices f1, f2;
+    NGC_CHECK(fft_factor(N2, f2) && (N1 == 1 || fft_factor(N1, f1)), NGC_ERR_UNSUPPORTED,
+              "column split not factorable");
+    if (N1 > 1) {
+        phase_begin(ctx, s, "fft_cols_a");
+        NGC_TRY((cols_a<SIGN>(s, a, f1, tw, blocks_a)));
+        phase_end(ctx, s);
+    }
+    phase_begin(ctx, s, "fft_cols_b");
+    NGC_TRY((cols_b<SIGN, MODE>(s, a, f2, tw, blocks_b)));
+    phase_end(ctx, s);
+    *done = true;
+    return NGC_OK;
+}
+
 static int fft_twiddles(ngc_context *ctx, hipStream_t s, int64_t n, int sign, const char *name,
                         const float2 **out) {
     std::vector<float> t(2 * (size_t)n);
@@ -508,7 +754,6 @@ int lds_grid_to_image(ngc_context *ctx, hipStream_t s, const void *grid, int dt,
     const size_t cells = (size_t)n_planes * n_u * n_v;
     void *b0, *b1;
     NGC_TRY(workspace(ctx, "fft_buffer", cells * sizeof(float2), &b0));
-    NGC_TRY(workspace(ctx, "fft_transposed", cells * sizeof(float2), &b1));
     const float2 *tw_v, *tw_u;
     NGC_TRY(fft_twiddles(ctx, s, n_v, +1, "fft_tw_rows", &tw_v));
     NGC_TRY(fft_twiddles(ctx, s, n_u, +1, "fft_tw_cols", &tw_u));
@@ -519,6 +764,22 @@ int lds_grid_to_image(ngc_context *ctx, hipStream_t s, const void *grid, int dt,
     else
         NGC_TRY((fft_rows<2, 0, +1>(s, g1, fv, tw_v)));
     phase_end(ctx, s);
+    // columns in place, the crop and correction fused into their last step
+    ColArgs ca{};
+    ca.buf = (float2 *)b0;
+    ca.n_planes = n_planes;
+    ca.N = (int)n_u;
+    ca.n_cols = (int)n_v;
+    ca.image = image;
+    ca.n_l = n_l;
+    ca.n_m = n_m;
+    ca.sum_weight = sum_weight;
+    ca.corr = d_corr;
+    bool done = false;
+    NGC_TRY((cols_four_step<+1, 0>(ctx, s, ca, tw_u, &done)));
+    if (done) return NGC_OK;
+    // otherwise transpose, rows again, crop
+    NGC_TRY(workspace(ctx, "fft_transposed", cells * sizeof(float2), &b1));
     phase_begin(ctx, s, "fft_transpose");
     NGC_TRY(fft_transpose(s, b0, b1, n_planes, n_u, n_v));
     phase_end(ctx, s);
@@ -537,8 +798,9 @@ int lds_grid_to_image(ngc_context *ctx, hipStream_t s, const void *grid, int dt,
 
 // The forward placement of the model (the inverse of lds_grid_to_image):
 // grid = fftshift(fft2(ifftshift(padded model / correction))) for complex64
-// grids: rows of the padded model (IN 3), transpose, rows through the fftshift
-// (OUT 1), transpose into the grid.
+// grids: rows of the padded model (IN 3), then the four-step columns with the
+// fftshift in their last step; for column lengths without a split, transpose,
+// rows through the fftshift (OUT 1) and transpose into the grid.
 int lds_model_to_grid(ngc_context *ctx, hipStream_t s, const double *image, int64_t n_planes,
                       int64_t n_l, int64_t n_m, int64_t n_u, int64_t n_v, const double *d_corr,
                       void *grid) {
@@ -548,7 +810,6 @@ int lds_model_to_grid(ngc_context *ctx, hipStream_t s, const double *image, int6
     const size_t cells = (size_t)n_planes * n_u * n_v;
     void *b0, *b1;
     NGC_TRY(workspace(ctx, "fft_buffer", cells * sizeof(float2), &b0));
-    NGC_TRY(workspace(ctx, "fft_transposed", cells * sizeof(float2), &b1));
     const float2 *tw_v, *tw_u;
     NGC_TRY(fft_twiddles(ctx, s, n_v, -1, "fft_tw_rows_forward", &tw_v));
     NGC_TRY(fft_twiddles(ctx, s, n_u, -1, "fft_tw_cols_forward", &tw_u));
@@ -559,6 +820,18 @@ int lds_model_to_grid(ngc_context *ctx, hipStream_t s, const double *image, int6
     g1.corr = d_corr;
     NGC_TRY((fft_rows<3, 0, -1>(s, g1, fv, tw_v)));
     phase_end(ctx, s);
+    // columns in place, the fftshift fused into their last step
+    ColArgs ca{};
+    ca.buf = (float2 *)b0;
+    ca.n_planes = n_planes;
+    ca.N = (int)n_u;
+    ca.n_cols = (int)n_v;
+    ca.grid = (float2 *)grid;
+    bool done = false;
+    NGC_TRY((cols_four_step<-1, 1>(ctx, s, ca, tw_u, &done)));
+    if (done) return NGC_OK;
+    // otherwise transpose, shifted rows, transpose into the grid
+    NGC_TRY(workspace(ctx, "fft_transposed", cells * sizeof(float2), &b1));
     phase_begin(ctx, s, "fft_transpose");
     NGC_TRY(fft_transpose(s, b0, b1, n_planes, n_u, n_v));
     phase_end(ctx, s);

@@ -126,3 +126,27 @@ def test_lds_model_to_grid_of_a_point_source(gpu):
     dv = np.arange(n_v) - n_v // 2
     want = np.exp(-2j * np.pi * (du[:, None] * dl / n_u + dv[None, :] * dm / n_v))
     assert np.abs(g - want).max() < 2e-5
+
+
+@pytest.mark.parametrize("n_u,n_v,N_l,N_m", [
+    (2197, 64, 2000, 60),      # 13^3: no column split N1 <= N2 <= 128, transposes
+    (8192, 40, 8000, 33),      # 64 x 128 split, 8 8 8 8 2 rows of 40 = 8 5
+])
+def test_lds_transforms_at_long_columns(gpu, n_u, n_v, N_l, N_m):
+    """Column lengths at the ends of the four-step split and past it (the
+    transpose path), both directions, against the oracle."""
+    import cngi_prototype_amd._native as nat
+    from cngi_prototype_amd.imaging._imaging_utils._gridding_convolutional_kernels import \
+        cropped_correction_factors
+    rng = np.random.default_rng(n_u + 3 * n_v)
+    g = (rng.standard_normal((2, 1, n_u, n_v)) +
+         1j * rng.standard_normal((2, 1, n_u, n_v))).astype(np.complex64)
+    ref = np.moveaxis(oracle.grid_to_image(g.astype(np.complex128), [N_l, N_m]), [0, 1], [-2, -1])
+    got = nat.grid_to_image(_dev(g), [N_l, N_m]).cpu().numpy()
+    assert rel_err(got, ref) < 1e-5
+    model = rng.normal(size=(N_l, N_m, 2, 1))
+    cl, cm = cropped_correction_factors([n_u, n_v], [N_l, N_m])
+    ref = oracle.model_to_grid(model, [n_u, n_v])
+    got = nat.model_to_grid(_dev(np.moveaxis(model, [0, 1], [-2, -1])), [n_u, n_v], cl, cm,
+                            single=True).cpu().numpy()
+    assert rel_err(got, ref) < 2e-6
