@@ -863,6 +863,9 @@ def predict_leg(args):
     del g64, v64, d, model, grid
     n_samples = T * B * C * P
     t_dg = phases.get("standard_degrid", float("nan")) / 1e3
+    # the whole degrid call: binning passes (tiled path) and the kernel
+    t_call = sum(v for k, v in phases.items()
+                 if k == "standard_degrid" or k.startswith("degrid_bin")) / 1e3
     alg = C * P * npad * npad * 8 + T * B * 24 + n_samples * 8
     gbs = alg / t_dg / 1e9
     log(f"predict: step {ms_step:.2f} ms, phases {phases}")
@@ -875,7 +878,9 @@ def predict_leg(args):
                 degrid_roofline=dict(bound="hbm", achieved=round(gbs, 1), peak=8000.0,
                                      unit="GB/s", frac=round(gbs / 8000.0, 4),
                                      algorithmic_bytes=int(alg),
-                                     kernel_ms=round(t_dg * 1e3, 3)),
+                                     kernel_ms=round(t_dg * 1e3, 3),
+                                     call_ms=round(t_call * 1e3, 3),
+                                     call_frac=round(alg / t_call / 1e9 / 8000.0, 4)),
                 precision="complex64 grid and visibilities, FP64 taps",
                 rel_err_vs_fp64_4chan=rel)
 

@@ -37,7 +37,7 @@ EXPORTS = ("ngc_context_create", "ngc_context_destroy", "ngc_last_error", "ngc_v
            "ngc_direction_rotate", "ngc_make_pb", "ngc_apply_flags", "ngc_set_tile_waves",
            "ngc_model_to_grid", "ngc_standard_degrid", "ngc_aperture_degrid", "ngc_mlp_sample_rows",
            "ngc_adam_schedule", "ngc_adam_step_sched", "ngc_set_bin_lanes",
-           "ngc_standard_grid_noalloc", "ngc_set_fft_mode")
+           "ngc_standard_grid_noalloc", "ngc_set_fft_mode", "ngc_set_degrid_mode")
 
 
 class NativeError(RuntimeError):
@@ -86,6 +86,7 @@ _PROTOS = {
     "ngc_set_tile_waves": (_i32, [_vp, _i32]),
     "ngc_set_bin_lanes": (_i32, [_vp, _i32]),
     "ngc_set_fft_mode": (_i32, [_vp, _i32]),
+    "ngc_set_degrid_mode": (_i32, [_vp, _i32]),
     "ngc_gcf_patterns": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _i32, _i32, _d,
                                 _d, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "ngc_gcf_resize": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
@@ -590,6 +591,13 @@ def set_fft_mode(mode):
     default) the LDS row FFTs where the lengths allow, 0 always rocFFT."""
     ctx = context()
     _check("ngc_set_fft_mode", load_library().ngc_set_fft_mode(ctx.handle, int(mode)))
+
+
+def set_degrid_mode(mode):
+    """Standard degridding on the current device: 1 (the default) LDS tiles
+    where the call allows them, 0 always the per-sample gather."""
+    ctx = context()
+    _check("ngc_set_degrid_mode", load_library().ngc_set_degrid_mode(ctx.handle, int(mode)))
 
 
 def set_tile_rows(rows):

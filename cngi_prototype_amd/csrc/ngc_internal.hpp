@@ -90,6 +90,7 @@ struct ngc_context {
     int tile_waves = 0;      // wavefronts per tile of the tiled gridder (0: default)
     int n_cu = 256;          // compute units of the device
     int fft_lds = 1;         // single-precision image transforms: LDS kernels where supported
+    int degrid_tiled = 1;    // standard degridding: LDS tiles where supported
     // autotuned (tile rows, binning channel lanes) per gridding shape
     std::map<std::string, std::pair<int, int>> tuned;
 };

@@ -31,6 +31,8 @@
 // are defined.
 #include <type_traits>
 
+#include <hipcub/hipcub.hpp>
+
 #include "ngc_internal.hpp"
 
 namespace ngc {
@@ -194,6 +196,177 @@ __global__ void __launch_bounds__(256) standard_degrid_kernel(DegridStdArgs a, c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tiled standard degridding (complex64 grids, support <= 8, 1, 2 or 4 pols).
+// The gather above reads 49 taps per pol from L1 for every sample; here each
+// sample is binned by the 16 x 32 tile of its footprint's first cell
+// (u0 = uc - S//2, v0 = vc - S//2), and one workgroup per non-empty bin
+// stages the tile with a 7-cell halo for every pol in LDS (23 x 39 cells, the
+// footprint of every sample of the bin) and degrids its samples from there,
+// one lane per sample with the gather's own arithmetic (same taps, same sums
+// in the same order), so both paths give the same bits.
+//   degrid_bin_kernel<false>  count: one wavefront per 64 consecutive
+//                             integrations of one (baseline, channel); lanes
+//                             of equal bin form runs, one atomic per run
+//   hipcub exclusive scan     bin offsets
+//   degrid_bin_kernel<true>   fill: the same runs take their ranges from the
+//                             bin cursors and write (sample, record) entries;
+//                             samples off the grid get zero visibilities here
+//   standard_degrid_tiled_kernel  one workgroup per bin (empty bins return)
+// Record: bits 0-3 u0 - tile row, 4-8 v0 - tile column, 9-16 du + 128,
+// 17-24 dv + 128.
+// ---------------------------------------------------------------------------
+constexpr int kDgTU = 16, kDgTV = 32, kDgHalo = 7;
+constexpr int kDgRows = kDgTU + kDgHalo, kDgCols = kDgTV + kDgHalo;   // 23 x 39
+
+struct DgBins {
+    int64_t n_tu, n_tv, n_bins;
+    unsigned int *count;    // [n_bins + 1]
+    unsigned int *offset;   // [n_bins + 1], exclusive scan of count
+    unsigned int *cursor;   // [n_bins], fill cursors
+    uint2 *entries;         // (sample index row * n_chan + chan, record)
+};
+
+template <bool FILL, typename V>
+__global__ void __launch_bounds__(256) degrid_bin_kernel(DegridStdArgs a, DgBins bn, V *vis) {
+    const int64_t n_tb = (a.n_time + 63) / 64;
+    const int64_t n_thr = a.n_baseline * a.n_chan * n_tb * 64;
+    const int64_t sc = a.support / 2;
+    // n_thr is a multiple of 64 and the stride of 256, so every wavefront
+    // runs the loop a uniform number of times (the ballots need all lanes)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_thr;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int lane = (int)(i & 63);
+        const int64_t w = i >> 6;
+        const int64_t t = (w % n_tb) * 64 + lane;
+        const int64_t r = w / n_tb;
+        const int64_t ch = r % a.n_chan, b = r / a.n_chan;
+        int64_t key = -1, rc = 0;
+        uint32_t rec = 0;
+        if (t < a.n_time) {
+            const int64_t row = t * a.n_baseline + b;
+            rc = row * a.n_chan + ch;
+            Pix px;
+            bool on = pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
+                                     a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os, px);
+            on = on && (px.uc + sc < a.n_u) && (px.vc + sc < a.n_v) && (px.uc - sc >= 0) &&
+                 (px.vc - sc >= 0);
+            if (on) {
+                const int64_t u0 = px.uc - sc, v0 = px.vc - sc;
+                const int64_t tu = u0 / kDgTU, tv = v0 / kDgTV;
+                key = ((int64_t)a.chan_map[ch] * bn.n_tu + tu) * bn.n_tv + tv;
+                rec = (uint32_t)(u0 - tu * kDgTU) | (uint32_t)(v0 - tv * kDgTV) << 4 |
+                      (uint32_t)(px.du + 128) << 9 | (uint32_t)(px.dv + 128) << 17;
+            } else if (FILL) {
+                V *out = vis + 2 * rc * a.n_pol;
+                for (int64_t p = 0; p < 2 * a.n_pol; p++) out[p] = (V)0;
+            }
+        }
+        // runs of equal keys among consecutive lanes
+        const int64_t prev = __shfl_up(key, 1, 64);
+        const bool boundary = key < 0 || lane == 0 || prev != key;
+        const unsigned long long bounds = __ballot(boundary);
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+        const int start = 63 - __clzll(bounds & upto);          // this lane's run head
+        const unsigned long long after = bounds & ~upto;
+        const int next = after ? __ffsll(after) - 1 : 64;       // first lane of the next run
+        unsigned int base = 0;
+        if (key >= 0 && lane == start) {
+            if (FILL)
+                base = atomicAdd(&bn.cursor[key], (unsigned int)(next - start));
+            else
+                atomicAdd(&bn.count[key], (unsigned int)(next - start));
+        }
+        if (FILL) {
+            base = __shfl(base, start, 64);
+            if (key >= 0) bn.entries[base + (lane - start)] = make_uint2((uint32_t)rc, rec);
+        }
+    }
+}
+
+template <int NP, typename V>
+__global__ void __launch_bounds__(256)
+standard_degrid_tiled_kernel(DegridStdArgs a, const float *grid, V *vis, DgBins bn) {
+    __shared__ double s_k[512];
+    __shared__ float2 s_g[NP][kDgRows][kDgCols];
+    const int64_t bin = blockIdx.x;
+    const unsigned int beg = bn.offset[bin], end = bn.offset[bin + 1];
+    if (beg == end) return;   // uniform: the workgroup's bin is empty
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) s_k[i] = i < a.cgk_len ? a.cgk[i] : 0.0;
+    const int64_t tv = bin % bn.n_tv, rest = bin / bn.n_tv;
+    const int64_t tu = rest % bn.n_tu, a_chan = rest / bn.n_tu;
+    const int64_t r0 = tu * kDgTU, c0 = tv * kDgTV;
+    const int64_t plane = a.n_u * a.n_v;
+    const float2 *g2 = reinterpret_cast<const float2 *>(grid);
+    // stage the tile and its halo for every pol: all loads first, then LDS
+    constexpr int kCells = NP * kDgRows * kDgCols;
+    constexpr int kPer = (kCells + 255) / 256;
+    float2 x[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int idx = threadIdx.x + k * 256;
+        const int p = idx / (kDgRows * kDgCols), rem = idx - p * (kDgRows * kDgCols);
+        const int r = rem / kDgCols, c = rem - r * kDgCols;
+        const int64_t gu = r0 + r, gv = c0 + c;
+        x[k] = make_float2(0.f, 0.f);
+        if (idx < kCells && gu < a.n_u && gv < a.n_v)
+            x[k] = g2[(a_chan * a.n_imag_pol + a.pol_map[p]) * plane + gu * a.n_v + gv];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int idx = threadIdx.x + k * 256;
+        if (idx < kCells) (&s_g[0][0][0])[idx] = x[k];
+    }
+    __syncthreads();
+    const int S = (int)a.support;
+    for (unsigned int e = beg + threadIdx.x; e < end; e += blockDim.x) {
+        const uint2 en = bn.entries[e];
+        const int64_t rc = en.x;
+        const uint32_t rec = en.y;
+        const int ru = rec & 15, rv = (rec >> 4) & 31;
+        const int64_t du = (int64_t)((rec >> 9) & 255) - 128, dv = (int64_t)((rec >> 17) & 255) - 128;
+        double cu[8], cv[8];
+        axis_taps<8>(s_k, a.support, a.os, du, cu);
+        axis_taps<8>(s_k, a.support, a.os, dv, cv);
+        float cvg[8];
+        double su = 0.0, sv = 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            cvg[i] = (float)cv[i];
+            su += cu[i];
+            sv += cv[i];
+        }
+        const double norm = su * sv;
+        V *out = vis + 2 * (rc * NP);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            double re = 0.0, im = 0.0;
+#pragma unroll
+            for (int iu = 0; iu < 8; iu++) {
+                if (iu >= S) break;
+                const float2 *gr = &s_g[p][ru + iu][rv];
+                float ir = 0.f, ii = 0.f;
+#pragma unroll
+                for (int iv = 0; iv < 8; iv++) {
+                    if (iv < S) {
+                        const float2 gval = gr[iv];
+                        ir = fmaf(cvg[iv], gval.x, ir);
+                        ii = fmaf(cvg[iv], gval.y, ii);
+                    }
+                }
+                re = fma(cu[iu], (double)ir, re);
+                im = fma(cu[iu], (double)ii, im);
+            }
+            if (a.normalize && norm != 0.0) {
+                re = re / norm;
+                im = im / norm;
+            }
+            out[2 * p] = (V)re;
+            out[2 * p + 1] = (V)im;
+        }
+    }
+}
+
 static int pblocks(int64_t n) {
     int64_t b = ceil_div(n, 256);
     if (b > (1 << 20)) b = 1 << 20;
@@ -324,6 +497,73 @@ extern "C" int ngc_standard_degrid(ngc_context *ctx, void *stream, const double 
     const int64_t n_thr = n_baseline * n_chan * ceil_div(n_time, 64) * 64;
     const int nb = pblocks(n_thr);
     profile_new_call(ctx, s);
+    const int64_t n_tu = ceil_div(n_u, (int64_t)kDgTU), n_tv = ceil_div(n_v, (int64_t)kDgTV);
+    const int64_t n_bins = n_imag_chan * n_tu * n_tv;
+    if (ctx->degrid_tiled && grid_dtype == NGC_C64 && support <= 8 && oversampling <= 200 &&
+        (n_pol == 1 || n_pol == 2 || n_pol == 4) && n_time * n_baseline * n_chan < (int64_t(1) << 32) &&
+        n_bins < (int64_t(1) << 31)) {
+        // the tiled path: bin, scan, fill, one workgroup per bin
+        const int64_t n_rc = n_time * n_baseline * n_chan;
+        DgBins bn;
+        bn.n_tu = n_tu;
+        bn.n_tv = n_tv;
+        bn.n_bins = n_bins;
+        void *p;
+        NGC_TRY(workspace(ctx, "dg_count", (n_bins + 1) * sizeof(unsigned int), &p));
+        bn.count = (unsigned int *)p;
+        NGC_TRY(workspace(ctx, "dg_offset", (n_bins + 1) * sizeof(unsigned int), &p));
+        bn.offset = (unsigned int *)p;
+        NGC_TRY(workspace(ctx, "dg_cursor", n_bins * sizeof(unsigned int), &p));
+        bn.cursor = (unsigned int *)p;
+        NGC_TRY(workspace(ctx, "dg_entries", n_rc * sizeof(uint2), &p));
+        bn.entries = (uint2 *)p;
+        size_t scan_bytes = 0;
+        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, bn.count, bn.offset,
+                                                 (int)(n_bins + 1), s));
+        void *scan_tmp;
+        NGC_TRY(workspace(ctx, "dg_scan", std::max<size_t>(scan_bytes, 8), &scan_tmp));
+        const bool v64 = vis_dtype == NGC_C64;
+        phase_begin(ctx, s, "degrid_bin_count");
+        NGC_HIP(hipMemsetAsync(bn.count, 0, (n_bins + 1) * sizeof(unsigned int), s));
+        if (v64)
+            degrid_bin_kernel<false, float><<<nb, 256, 0, s>>>(a, bn, (float *)vis);
+        else
+            degrid_bin_kernel<false, double><<<nb, 256, 0, s>>>(a, bn, (double *)vis);
+        NGC_HIP(hipGetLastError());
+        phase_end(ctx, s);
+        phase_begin(ctx, s, "degrid_bin_scan");
+        NGC_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, bn.count, bn.offset,
+                                                 (int)(n_bins + 1), s));
+        NGC_HIP(hipMemcpyAsync(bn.cursor, bn.offset, n_bins * sizeof(unsigned int),
+                               hipMemcpyDeviceToDevice, s));
+        phase_end(ctx, s);
+        phase_begin(ctx, s, "degrid_bin_fill");
+        if (v64)
+            degrid_bin_kernel<true, float><<<nb, 256, 0, s>>>(a, bn, (float *)vis);
+        else
+            degrid_bin_kernel<true, double><<<nb, 256, 0, s>>>(a, bn, (double *)vis);
+        NGC_HIP(hipGetLastError());
+        phase_end(ctx, s);
+        phase_begin(ctx, s, "standard_degrid");
+        auto tiled = [&](auto np_c) {
+            constexpr int NP = decltype(np_c)::value;
+            if (v64)
+                standard_degrid_tiled_kernel<NP, float><<<(unsigned)n_bins, 256, 0, s>>>(
+                    a, (const float *)grid, (float *)vis, bn);
+            else
+                standard_degrid_tiled_kernel<NP, double><<<(unsigned)n_bins, 256, 0, s>>>(
+                    a, (const float *)grid, (double *)vis, bn);
+        };
+        if (n_pol == 1)
+            tiled(std::integral_constant<int, 1>{});
+        else if (n_pol == 2)
+            tiled(std::integral_constant<int, 2>{});
+        else
+            tiled(std::integral_constant<int, 4>{});
+        NGC_HIP(hipGetLastError());
+        phase_end(ctx, s);
+        return NGC_OK;
+    }
     phase_begin(ctx, s, "standard_degrid");
     auto launch = [&](auto smax, auto np_c) {
         constexpr int SM = decltype(smax)::value;
@@ -976,5 +1216,13 @@ extern "C" int ngc_aperture_degrid(ngc_context *ctx, void *stream, const double 
     NGC_HIP(hipStreamSynchronize(s));
     NGC_CHECK(!(err & 2), NGC_ERR_UNSUPPORTED, "sub-cell offset outside the re-laid taps");
     NGC_CHECK(err == 0, NGC_ERR_FIELD, "a row's field id is not in field_id");
+    return NGC_OK;
+}
+
+extern "C" int ngc_set_degrid_mode(ngc_context *ctx, int mode) {
+    NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
+    NGC_CHECK(mode == 0 || mode == 1, NGC_ERR_ARG, "mode must be 0 (gather) or 1 (tiled)");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->degrid_tiled = mode;
     return NGC_OK;
 }

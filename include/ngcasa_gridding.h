@@ -393,6 +393,11 @@ int ngc_make_pb(ngc_context *ctx, void *stream, const double *freq_chan, int64_t
  *   grid  device [n_imag_chan, n_imag_pol, n_u, n_v] of grid_dtype (C64|C128)
  *   vis   device out [T, B, C, P] of vis_dtype (C64|C128)
  *   uvw device; freq_chan, chan_map, pol_map, cgk_1d host
+ * Complex64 grids with support <= 8 and 1, 2 or 4 pols are degridded from
+ * LDS tiles (samples binned by the tile of their footprint, workspace of
+ * 8 bytes per sample); the others, and every call after
+ * ngc_set_degrid_mode(ctx, 0), by a per-sample gather.  Both give the same
+ * bits.
  */
 int ngc_model_to_grid(ngc_context *ctx, void *stream, const double *image, int64_t n_planes,
                       int64_t n_l, int64_t n_m, int64_t n_u, int64_t n_v, const double *corr_l,
@@ -588,6 +593,13 @@ int ngc_set_bin_lanes(ngc_context *ctx, int lanes);
  * (default) uses the LDS kernels where the lengths allow, mode 0 always rocFFT.
  */
 int ngc_set_fft_mode(ngc_context *ctx, int mode);
+
+/*
+ * Standard degridding path (no counterpart in the reference, whose degridder
+ * is a stub): mode 1 (default) LDS tiles where ngc_standard_degrid allows
+ * them, mode 0 always the per-sample gather.
+ */
+int ngc_set_degrid_mode(ngc_context *ctx, int mode);
 
 int ngc_profile_enable(ngc_context *ctx, int enable);
 int ngc_profile_read(ngc_context *ctx, char *names, int64_t names_len, double *ms,
