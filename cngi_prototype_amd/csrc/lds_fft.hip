@@ -296,26 +296,29 @@ fft_rows_kernel(FftRowArgs g, FftRadices f, const float2 *__restrict__ tw) {
             const double *src = reinterpret_cast<const double *>(g.in) + (pl * g.n_l + l) * g.n_m;
             const int hm = (int)(g.n_m / 2);
             // all loads first (model values and correction factors), then the
-            // divisions, so that no load waits behind another's arithmetic
+            // divisions; branch-free, indices outside the model read element 0
+            // and are masked afterwards (guarded loads with their divisions
+            // had waited for one another)
             double v[PER], c[PER];
+            const double *cm = g.corr ? g.corr + g.n_l : src;
 #pragma unroll
             for (int i = 0; i < PER; i++) {
                 const int a = threadIdx.x + i * kFftBlock;
                 int m = a + hm;
                 if (m >= N) m -= N;
-                v[i] = 0.0;
-                c[i] = 1.0;
-                if (a < N && m < g.n_m) {
-                    v[i] = src[m];
-                    if (g.corr) c[i] = g.corr[g.n_l + m];
-                }
+                const int mc = (a < N && m < g.n_m) ? m : 0;
+                v[i] = src[mc];
+                c[i] = cm[mc];
             }
             const double cl = g.corr ? g.corr[l] : 1.0;
 #pragma unroll
             for (int i = 0; i < PER; i++) {
+                const int a = threadIdx.x + i * kFftBlock;
+                int m = a + hm;
+                if (m >= N) m -= N;
                 double y = v[i];
                 if (g.corr) y = y / (cl * c[i]);
-                x[i] = make_float2((float)y, 0.f);
+                x[i] = make_float2((a < N && m < g.n_m) ? (float)y : 0.f, 0.f);
             }
         } else {
             int64_t src_row = row;
