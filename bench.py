@@ -716,7 +716,7 @@ def streamed_leg(args):
                                      ms, 0.025)
     half = 0.5 * (fr.max() - fr.min()) + 20e6
     model, tinfo = ks.train_surrogate(dish, blockage, pairs, True, pb_cell, npad2, os_, ms,
-                                      sup_b[:, 0, 0, :], pb_freq, half, n_freq=16, steps=3000,
+                                      sup_b[:, 0, 0, :], pb_freq, half, n_freq=38, steps=20000,
                                       lr=4e-3, batch=4096)
     gcf = make_gridding_convolution_function(
         mxds, dict(function="casa_airy", list_dish_diameters=list(dish),
