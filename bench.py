@@ -137,6 +137,42 @@ def cpu_baseline(cfg, data, npad, delta, cgk, n_chan_sample, n_chan_cube):
     return cube, out[0], out[1]
 
 
+def cpu_baseline_c1(uvw, vis, w, fr, cgk, npad, delta):
+    """The oracle on the whole C1 image call (3.37M samples, 16 channels), on
+    one thread and in cube mode (one task per channel on up to 16 threads),
+    compiled and timed on this host as the C2 CPU baseline is."""
+    import concurrent.futures
+
+    import oracle
+    oracle.use_native_build()
+    model, avail = oracle.cpu_model()
+    n_thr = int(os.environ.get("OMP_NUM_THREADS", avail))
+    uvw_h = uvw.cpu().numpy()
+    vis_h = vis.cpu().numpy().astype(np.complex128)
+    w_h = w.cpu().numpy().astype(np.float64)
+    n = int(np.isfinite(w_h).sum())
+    C = vis_h.shape[2]
+    t0 = time.time()
+    oracle.standard_grid(vis_h, uvw_h, w_h, fr, cgk, [npad, npad], delta, 7, 100, do_psf=False)
+    t1 = time.time() - t0
+    chans = [(np.ascontiguousarray(vis_h[:, :, c:c + 1]), np.ascontiguousarray(w_h[:, :, c:c + 1]))
+             for c in range(C)]
+    nt = min(n_thr, C)
+    t0 = time.time()
+    with concurrent.futures.ThreadPoolExecutor(max_workers=nt) as ex:
+        list(ex.map(lambda c: oracle.standard_grid(chans[c][0], uvw_h, chans[c][1], fr[c:c + 1],
+                                                   cgk, [npad, npad], delta, 7, 100,
+                                                   do_psf=False), range(C)))
+    tc = time.time() - t0
+    how = "FP64 C oracle (reference loop order, -O3 -march=native), image mode, whole C1 call"
+    return dict(one_thread=dict(value=round(n / t1 / 1e6, 3), unit="Mvis/s", cores=1,
+                                kind="port", cpu=f"{model}", sample=f"{how}, 1 thread, {t1:.2f} s"),
+                cube=dict(value=round(n / tc / 1e6, 3), unit="Mvis/s", cores=nt, kind="port",
+                          cpu=f"{model}, {nt} of {avail} cores",
+                          sample=f"{how}, {C} tasks of one channel each on {nt} threads, "
+                                 f"{tc:.2f} s"))
+
+
 def c1_leg(args):
     """C1 at full size: 27 antennas in a 1 km disk, 300 x 10 s integrations at
     1.4 GHz, 16 channels x 2 pols (3.37M samples), 512^2 at 2", natural
@@ -219,6 +255,7 @@ def c1_leg(args):
     torch.cuda.synchronize()
     t_api = (time.time() - t0) * 1e3
     del img, mxds, xds
+    cpu = None if args.no_cpu_baseline else cpu_baseline_c1(uvw, vis, w, fr, cgk, npad, delta)
     ck = args.precision
     abytes = algorithmic_bytes(n_vis, n_rows, C * P, npad)
     t_k = kernel_ms[(ck, "image")] / 1e3
@@ -242,7 +279,8 @@ def c1_leg(args):
                               frac=round(abytes / t_k / 1e9 / HBM_PEAK_GBS, 4),
                               algorithmic_bytes=int(abytes),
                               kernel_ms=round(kernel_ms[(ck, "image")], 4)),
-                make_psf_plus_make_image_api_ms=round(t_api, 1))
+                make_psf_plus_make_image_api_ms=round(t_api, 1),
+                cpu_baseline=cpu)
 
 
 def mosaic_surrogate(d, delta, npad, C):
