@@ -284,7 +284,9 @@ __global__ void __launch_bounds__(256) degrid_bin_kernel(DegridStdArgs a, DgBins
     }
 }
 
-template <int NP, typename V>
+// SC: the support when known at compile time (the tap loops then have no
+// branches and the two samples' reads interleave), 0 for any support <= 8
+template <int NP, typename V, int SC>
 __global__ void __launch_bounds__(256)
 standard_degrid_tiled_kernel(DegridStdArgs a, const float *grid, V *vis, DgBins bn) {
     __shared__ double s_k[512];
@@ -318,10 +320,12 @@ standard_degrid_tiled_kernel(DegridStdArgs a, const float *grid, V *vis, DgBins 
         if (idx < kCells) (&s_g[0][0][0])[idx] = x[k];
     }
     __syncthreads();
-    const int S = (int)a.support;
-    for (unsigned int e = beg + threadIdx.x; e < end; e += blockDim.x) {
-        const uint2 en = bn.entries[e];
-        const int64_t rc = en.x;
+    const int S = SC > 0 ? SC : (int)a.support;
+    constexpr int SL = SC > 0 ? SC : 8;   // tap loop bound
+    // one sample per lane at a time, two in flight: the second sample's
+    // entry, taps and LDS reads overlap the first's (the arithmetic of each
+    // is the gather's, so the bits are the same)
+    auto degrid_one = [&](const uint2 en, double (&re)[NP], double (&im)[NP]) {
         const uint32_t rec = en.y;
         const int ru = rec & 15, rv = (rec >> 4) & 31;
         const int64_t du = (int64_t)((rec >> 9) & 255) - 128, dv = (int64_t)((rec >> 17) & 255) - 128;
@@ -337,33 +341,51 @@ standard_degrid_tiled_kernel(DegridStdArgs a, const float *grid, V *vis, DgBins 
             sv += cv[i];
         }
         const double norm = su * sv;
-        V *out = vis + 2 * (rc * NP);
 #pragma unroll
         for (int p = 0; p < NP; p++) {
-            double re = 0.0, im = 0.0;
+            double r = 0.0, m = 0.0;
 #pragma unroll
-            for (int iu = 0; iu < 8; iu++) {
-                if (iu >= S) break;
+            for (int iu = 0; iu < SL; iu++) {
+                if (SC == 0 && iu >= S) break;
                 const float2 *gr = &s_g[p][ru + iu][rv];
                 float ir = 0.f, ii = 0.f;
 #pragma unroll
-                for (int iv = 0; iv < 8; iv++) {
-                    if (iv < S) {
+                for (int iv = 0; iv < SL; iv++) {
+                    if (SC > 0 || iv < S) {
                         const float2 gval = gr[iv];
                         ir = fmaf(cvg[iv], gval.x, ir);
                         ii = fmaf(cvg[iv], gval.y, ii);
                     }
                 }
-                re = fma(cu[iu], (double)ir, re);
-                im = fma(cu[iu], (double)ii, im);
+                r = fma(cu[iu], (double)ir, r);
+                m = fma(cu[iu], (double)ii, m);
             }
             if (a.normalize && norm != 0.0) {
-                re = re / norm;
-                im = im / norm;
+                r = r / norm;
+                m = m / norm;
             }
-            out[2 * p] = (V)re;
-            out[2 * p + 1] = (V)im;
+            re[p] = r;
+            im[p] = m;
         }
+    };
+    auto store = [&](const uint2 en, const double (&re)[NP], const double (&im)[NP]) {
+        V *out = vis + 2 * ((int64_t)en.x * NP);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            out[2 * p] = (V)re[p];
+            out[2 * p + 1] = (V)im[p];
+        }
+    };
+    const unsigned int stride = blockDim.x;
+    for (unsigned int e = beg + threadIdx.x; e < end; e += 2 * stride) {
+        const bool two = e + stride < end;
+        const uint2 en0 = bn.entries[e];
+        const uint2 en1 = bn.entries[two ? e + stride : e];
+        double re0[NP], im0[NP], re1[NP], im1[NP];
+        degrid_one(en0, re0, im0);
+        degrid_one(en1, re1, im1);
+        store(en0, re0, im0);
+        if (two) store(en1, re1, im1);
     }
 }
 
@@ -547,12 +569,20 @@ extern "C" int ngc_standard_degrid(ngc_context *ctx, void *stream, const double 
         phase_begin(ctx, s, "standard_degrid");
         auto tiled = [&](auto np_c) {
             constexpr int NP = decltype(np_c)::value;
-            if (v64)
-                standard_degrid_tiled_kernel<NP, float><<<(unsigned)n_bins, 256, 0, s>>>(
+            if (support == 7) {
+                if (v64)
+                    standard_degrid_tiled_kernel<NP, float, 7><<<(unsigned)n_bins, 256, 0, s>>>(
+                        a, (const float *)grid, (float *)vis, bn);
+                else
+                    standard_degrid_tiled_kernel<NP, double, 7><<<(unsigned)n_bins, 256, 0, s>>>(
+                        a, (const float *)grid, (double *)vis, bn);
+            } else if (v64) {
+                standard_degrid_tiled_kernel<NP, float, 0><<<(unsigned)n_bins, 256, 0, s>>>(
                     a, (const float *)grid, (float *)vis, bn);
-            else
-                standard_degrid_tiled_kernel<NP, double><<<(unsigned)n_bins, 256, 0, s>>>(
+            } else {
+                standard_degrid_tiled_kernel<NP, double, 0><<<(unsigned)n_bins, 256, 0, s>>>(
                     a, (const float *)grid, (double *)vis, bn);
+            }
         };
         if (n_pol == 1)
             tiled(std::integral_constant<int, 1>{});
