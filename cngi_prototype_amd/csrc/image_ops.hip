@@ -254,12 +254,25 @@ __device__ inline double nat_w(const DegridArgs &a, int64_t i) {
     return w;
 }
 
+// A wavefront takes 8 consecutive integrations x 8 consecutive channels of
+// one baseline: the weights it reads and the imaging weights it writes are
+// runs of 8 channels, and the density cells of neighbouring integrations in
+// one channel's plane share cache lines (with the channel fastest, every lane
+// read a different plane).
 __global__ void degrid_kernel(DegridArgs a) {
 #pragma clang fp contract(off)
-    const int64_t n_rc = a.n_time * a.n_baseline * a.n_chan;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_rc;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t ch = i % a.n_chan, row = i / a.n_chan;
+    const int64_t n_cb = (a.n_chan + 7) / 8, n_tb = (a.n_time + 7) / 8;
+    const int64_t n_thr = a.n_baseline * n_tb * n_cb * 64;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n_thr;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int lane = (int)(j & 63);
+        const int64_t w = j >> 6;
+        const int64_t cb = w % n_cb, r = w / n_cb;
+        const int64_t tb = r % n_tb, bl = r / n_tb;
+        const int64_t ch = cb * 8 + (lane & 7), t = tb * 8 + (lane >> 3);
+        if (ch >= a.n_chan || t >= a.n_time) continue;
+        const int64_t row = t * a.n_baseline + bl;
+        const int64_t i = row * a.n_chan + ch;
         const int64_t base = i * a.n_pol;
         Pix px;
         bool inb = pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
@@ -452,7 +465,9 @@ int ngc_imaging_weight_degrid(ngc_context *ctx, void *stream, const double *uvw,
     a.out = imaging_weight;
     const int64_t n_rc = n_time * n_baseline * n_chan;
     if (n_rc == 0) return NGC_OK;
-    degrid_kernel<<<blocks_for(n_rc), 256, 0, s>>>(a);
+    degrid_kernel<<<blocks_for(n_baseline * ceil_div(n_time, (int64_t)8) *
+                               ceil_div(n_chan, (int64_t)8) * 64),
+                    256, 0, s>>>(a);
     NGC_HIP(hipGetLastError());
     return NGC_OK;
 }

@@ -170,6 +170,98 @@ std_grid_plain_kernel(StdArgs a, const double *cgk, int64_t cgk_len, double *gri
 }
 
 // ---------------------------------------------------------------------------
+// density grid for support 1 (make_imaging_weight grids the weights with a
+// one-tap kernel, make_imaging_weight.py:154-161): one wavefront per 64
+// consecutive integrations of one (baseline, channel), so that lanes whose
+// cell (or conjugate cell) is the same form runs, and each run adds its sum
+// with one FP64 atomic per pol instead of one per lane.  The values, the
+// bounds, the conjugate rule and the sum_weight terms are those of
+// std_grid_plain_kernel<NGC_GRID_DENSITY>; only the order of the FP64
+// additions differs (atomics have none).
+// ---------------------------------------------------------------------------
+__device__ inline double run_sum(double v, int lane, int start) {
+    // inclusive sum over lanes start .. lane
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(v, off, 64);
+        if (lane - off >= start) v += y;
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+std_density_runs_kernel(StdArgs a, const double *cgk, int64_t cgk_len, double *grid,
+                        double *sum_weight) {
+#pragma clang fp contract(off)
+    const int64_t n_tb = (a.n_time + 63) / 64;
+    const int64_t n_thr = a.n_baseline * a.n_chan * n_tb * 64;
+    const int64_t plane = a.n_u * a.n_v;
+    const int64_t cells = a.n_imag_chan * a.n_imag_pol * plane;
+    // n_thr is a multiple of 64 and the stride of kBlock, so every wavefront
+    // runs the loop a uniform number of times (the ballots need all lanes)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_thr;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int lane = (int)(i & 63);
+        const int64_t w = i >> 6;
+        const int64_t t = (w % n_tb) * 64 + lane;
+        const int64_t r = w / n_tb;
+        const int64_t ch = r % a.n_chan, b = r / a.n_chan;
+        int64_t key = -1, ckey = -1;   // cell and conjugate cell within a plane
+        double val = 0.0;              // conv * weight of this sample
+        if (t < a.n_time) {
+            const int64_t row = t * a.n_baseline + b;
+            Pix px;
+            if (pixel_position(a.uvw[row * 3], a.uvw[row * 3 + 1], a.uv_scale[ch],
+                               a.uv_scale[a.n_chan + ch], a.n_u, a.n_v, a.os, px) &&
+                px.uc < a.n_u && px.vc < a.n_v && px.uc >= 0 && px.vc >= 0) {
+                const int64_t base = (row * a.n_chan + ch) * a.n_pol;
+                const double wr = a.n_pol >= 2
+                                      ? (weight_of(a, base, 0) + weight_of(a, base, 1)) / 2.0
+                                      : weight_of(a, base, 0);
+                if (!isnan(wr) && wr != 0.0) {
+                    const int64_t ku = iabs64(px.du), kv = iabs64(px.dv);
+                    const double conv_v = kv < cgk_len ? cgk[kv] : 0.0;
+                    const double conv_u = ku < cgk_len ? cgk[ku] : 0.0;
+                    val = (conv_u * conv_v) * wr;
+                    key = px.uc * a.n_v + px.vc;
+                    ckey = px.ucj * a.n_v + px.vcj;
+                }
+            }
+        }
+        const int64_t a_chan = a.chan_map[ch];   // one channel per wavefront
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+        for (int pass = 0; pass < 2; pass++) {   // the cell, then the conjugate cell
+            const int64_t k = pass ? ckey : key;
+            const int64_t prev = __shfl_up(k, 1, 64);
+            const unsigned long long bounds = __ballot(lane == 0 || prev != k);
+            const int start = 63 - __clzll(bounds & upto);
+            const unsigned long long after = bounds & ~upto;
+            const int next = after ? __ffsll(after) - 1 : 64;
+            const double sum = run_sum(val, lane, start);
+            if (key >= 0 && lane == next - 1) {
+                for (int64_t p = 0; p < a.n_pol; p++) {
+                    const int64_t pbase = (a_chan * a.n_imag_pol + a.pol_map[p]) * plane;
+                    const int64_t cell = pbase + k;
+                    // the reference does not bounds-check the conjugate cell:
+                    // kept at its flat C-order offset when inside the grid
+                    if (pass == 0 || (cell >= 0 && cell < cells)) atomicAdd(&grid[cell], sum);
+                }
+            }
+        }
+        // sum_weight: sel * norm = conv * weight per pol, added twice (once per
+        // cell) as the reference does; one atomic pair per pol per wavefront
+        double sw = val;
+        for (int off = 32; off > 0; off >>= 1) sw += __shfl_xor(sw, off, 64);
+        if (lane == 0 && sw != 0.0) {
+            for (int64_t p = 0; p < a.n_pol; p++) {
+                double *d = &sum_weight[a_chan * a.n_imag_pol + a.pol_map[p]];
+                atomicAdd(d, sw);
+                atomicAdd(d, sw);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
 // binning: deterministic counting sort of samples into (image chan, tile) bins
 // ---------------------------------------------------------------------------
@@ -1489,6 +1581,11 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
         else if (mode == NGC_GRID_PSF)
             std_grid_plain_kernel<NGC_GRID_PSF><<<blocks, kBlock, shmem, s>>>(
                 a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
+        else if (a.support == 1)
+            std_density_runs_kernel<<<launch_blocks(a.n_baseline * a.n_chan *
+                                                    ceil_div(a.n_time, (int64_t)64) * 64),
+                                      kBlock, 0, s>>>(a, (const double *)d_cgk, cgk_len, g,
+                                                      sum_weight);
         else
             std_grid_plain_kernel<NGC_GRID_DENSITY><<<blocks, kBlock, shmem, s>>>(
                 a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
@@ -1682,6 +1779,11 @@ static int standard_grid_noalloc_impl(ngc_context *ctx, hipStream_t s, int mode,
         else if (mode == NGC_GRID_PSF)
             std_grid_plain_kernel<NGC_GRID_PSF><<<blocks, kBlock, shmem, s>>>(
                 a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
+        else if (a.support == 1)
+            std_density_runs_kernel<<<launch_blocks(a.n_baseline * a.n_chan *
+                                                    ceil_div(a.n_time, (int64_t)64) * 64),
+                                      kBlock, 0, s>>>(a, (const double *)d_cgk, cgk_len, g,
+                                                      sum_weight);
         else
             std_grid_plain_kernel<NGC_GRID_DENSITY><<<blocks, kBlock, shmem, s>>>(
                 a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
