@@ -390,8 +390,11 @@ int ngc_briggs_factors(ngc_context *ctx, void *stream, const double *density_gri
         if (bx < 1) bx = 1;
         if (bx > 1024) bx = 1024;
         dim3 grid(bx, (unsigned)np);
+        profile_new_call(ctx, s);
+        phase_begin(ctx, s, "briggs_sumsq");
         sumsq_kernel<<<grid, 256, 0, s>>>(density_grid, plane, (double *)d_sq);
         NGC_HIP(hipGetLastError());
+        phase_end(ctx, s);
         NGC_HIP(hipMemcpyAsync(sq.data(), d_sq, np * sizeof(double), hipMemcpyDeviceToHost, s));
         NGC_HIP(hipMemcpyAsync(sw.data(), sum_weight, np * sizeof(double),
                                hipMemcpyDeviceToHost, s));
@@ -465,10 +468,13 @@ int ngc_imaging_weight_degrid(ngc_context *ctx, void *stream, const double *uvw,
     a.out = imaging_weight;
     const int64_t n_rc = n_time * n_baseline * n_chan;
     if (n_rc == 0) return NGC_OK;
+    profile_new_call(ctx, s);
+    phase_begin(ctx, s, "imaging_weight_degrid");
     degrid_kernel<<<blocks_for(n_baseline * ceil_div(n_time, (int64_t)8) *
                                ceil_div(n_chan, (int64_t)8) * 64),
                     256, 0, s>>>(a);
     NGC_HIP(hipGetLastError());
+    phase_end(ctx, s);
     return NGC_OK;
 }
 
