@@ -13,25 +13,29 @@ window per plane.
 import numpy as np
 
 
-def _gaussian(params, sampling):
+def _gaussian(params, x, y):
+    """The elliptical Gaussian at the sample offsets x, y."""
     width_x, width_y, rotation = params
     rot = np.deg2rad(90 - rotation)
-    xs, ys = sampling[0] * 2 + 1, sampling[1] * 2 + 1
-    x, y = np.meshgrid(np.arange(xs) - sampling[0], np.arange(ys) - sampling[1], indexing="ij")
-    xp = x * np.cos(rot) - y * np.sin(rot)
-    yp = x * np.sin(rot) + y * np.cos(rot)
+    c, s = np.cos(rot), np.sin(rot)
+    xp = x * c - y * s
+    yp = x * s + y * c
     return np.exp(-((xp / width_x) ** 2 + (yp / width_y) ** 2) / 2.0)
 
 
-def _chi2(params, psf, sampling):
-    p = np.ravel(psf)
-    mask = ~np.isnan(p)
-    g = np.ravel(_gaussian(params, sampling))
-    return np.sum((g[mask] - p[mask]) ** 2)
+def _chi2(params, pm, xm, ym):
+    """Squared residual over the unmasked samples pm at offsets xm, ym."""
+    return np.sum((_gaussian(params, xm, ym) - pm) ** 2)
 
 
 def fit_planes(img, npix_window, sampling, cutoff, delta):
-    """img [l, m, time, chan, pol] -> [time, chan, pol, 3]."""
+    """img [l, m, time, chan, pol] -> [time, chan, pol, 3].
+
+    The sample offsets are built once, and each plane's unmasked samples once
+    (the objective then evaluates the Gaussian only there: the same values
+    summed in the same order as evaluating everywhere and masking).  Planes
+    that are bit-identical (e.g. the pols of a PSF with equal weights) share
+    one fit."""
     from scipy import optimize
     from scipy.interpolate import RectBivariateSpline
     npix_window = np.asarray(npix_window)
@@ -46,19 +50,33 @@ def fit_planes(img, npix_window, sampling, cutoff, delta):
     d1 = np.arange(0, npix_window[1]) * np.abs(delta[1])
     i0 = np.linspace(0, npix_window[0] - 1, sampling[0]) * np.abs(delta[0])
     i1 = np.linspace(0, npix_window[1] - 1, sampling[1]) * np.abs(delta[1])
+    xp, yp = np.meshgrid(i0, i1, indexing="ij")
+    xp, yp = np.ravel(xp), np.ravel(yp)
+    half = sampling // 2
+    gx, gy = np.meshgrid(np.arange(half[0] * 2 + 1) - half[0],
+                         np.arange(half[1] * 2 + 1) - half[1], indexing="ij")
+    gx, gy = np.ravel(gx), np.ravel(gy)
+    done = {}
     for t in range(win.shape[2]):
         for ch in range(win.shape[3]):
             for p in range(win.shape[4]):
-                plane = np.asarray(win[:, :, t, ch, p], dtype=np.float64)
+                plane = np.ascontiguousarray(win[:, :, t, ch, p], dtype=np.float64)
+                key = plane.tobytes()
+                if key in done:
+                    out[t, ch, p] = done[key]
+                    continue
                 if not np.all(np.isfinite(plane)):
                     out[t, ch, p] = np.nan
+                    done[key] = out[t, ch, p].copy()
                     continue
                 spl = RectBivariateSpline(d0, d1, plane)
-                xp, yp = np.meshgrid(i0, i1, indexing="ij")
-                vals = spl.ev(np.ravel(xp), np.ravel(yp))
+                vals = spl.ev(xp, yp)
                 interp = np.reshape(vals, [sampling[1], sampling[0]]).T
                 interp[interp < cutoff] = np.nan
-                res = optimize.minimize(_chi2, [2.5, 2.5, 0.0], args=(interp, sampling // 2))
+                pr = np.ravel(interp)
+                mask = ~np.isnan(pr)
+                res = optimize.minimize(_chi2, [2.5, 2.5, 0.0],
+                                        args=(pr[mask], gx[mask], gy[mask]))
                 rx = res.x
                 phi = rx[2] - 90.0
                 if phi < -90.0:
@@ -68,6 +86,7 @@ def fit_planes(img, npix_window, sampling, cutoff, delta):
                 out[t, ch, p, 1] = np.min(np.abs(rx[0:2])) * np.abs(delta[1]) * 2.355 / (
                     sampling[1] / npix_window[1])
                 out[t, ch, p, 2] = -phi
+                done[key] = out[t, ch, p].copy()
     return out
 
 
