@@ -48,3 +48,25 @@ def test_briggs_chain_matches_oracle(gpu, weighting, robust, precision, n_pol):
     both_nan = np.isnan(iw) & np.isnan(iw_ref)
     assert np.array_equal(np.isnan(iw), np.isnan(iw_ref))
     assert rel_err(np.where(both_nan, 0, iw), np.where(both_nan, 0, iw_ref)) < 1e-12
+
+
+def test_density_pols_sharing_a_plane_and_accumulating(gpu):
+    """With 2 pols both carry (w_0 + w_1)/2: mapped to one image pol, its
+    plane is twice each identity-mapped plane; accumulating into a grid adds
+    the call's planes to it (the path that takes every pol's atomics)."""
+    import cngi_prototype_amd._native as nat
+    c = small_case(n_time=70, n_chan=3, n_pol=2, image_size=64, nan_frac=0.05)
+    N = c["image_size"]
+    args = (_dev(c["uvw"]), _dev(c["weight"]), c["freq"], np.arange(3))
+    rest = (np.ones(1), 3, [N, N], c["delta"], 1, 0, nat.GRID_DENSITY, "fp64")
+    g1, sw1 = nat.standard_grid(*args, np.arange(2), *rest)
+    g2, sw2 = nat.standard_grid(*args, np.zeros(2, dtype=int), *rest, n_imag_pol=1)
+    a1, a2 = g1.cpu().numpy(), g2.cpu().numpy()
+    assert np.count_nonzero(a1) > 0
+    assert np.array_equal(a1[:, 0], a1[:, 1])
+    assert rel_err(a2[:, 0], 2 * a1[:, 0]) < 1e-12
+    assert rel_err(sw2.cpu().numpy()[:, 0], 2 * sw1.cpu().numpy()[:, 0]) < 1e-12
+    g3, sw3 = nat.standard_grid(*args, np.arange(2), *rest, grid=g1.clone(),
+                                sum_weight=sw1.clone(), accumulate=True)
+    assert rel_err(g3.cpu().numpy(), 2 * a1) < 1e-12
+    assert rel_err(sw3.cpu().numpy(), 2 * sw1.cpu().numpy()) < 1e-12
