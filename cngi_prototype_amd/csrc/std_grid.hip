@@ -188,12 +188,9 @@ __device__ inline double run_sum(double v, int lane, int start) {
     return v;
 }
 
-// n_atomic_pols: the pols whose planes take the atomics (n_pol, or 1 when all
-// pols carry the same density, 2 or more pols, and density_copy_pols_kernel
-// fills the other planes afterwards)
 __global__ void __launch_bounds__(kBlock)
 std_density_runs_kernel(StdArgs a, const double *cgk, int64_t cgk_len, double *grid,
-                        double *sum_weight, int n_atomic_pols) {
+                        double *sum_weight) {
 #pragma clang fp contract(off)
     const int64_t n_tb = (a.n_time + 63) / 64;
     const int64_t n_thr = a.n_baseline * a.n_chan * n_tb * 64;
@@ -241,7 +238,7 @@ std_density_runs_kernel(StdArgs a, const double *cgk, int64_t cgk_len, double *g
             const int next = after ? __ffsll(after) - 1 : 64;
             const double sum = run_sum(val, lane, start);
             if (key >= 0 && lane == next - 1) {
-                for (int64_t p = 0; p < n_atomic_pols; p++) {
+                for (int64_t p = 0; p < a.n_pol; p++) {
                     const int64_t pbase = (a_chan * a.n_imag_pol + a.pol_map[p]) * plane;
                     const int64_t cell = pbase + k;
                     // the reference does not bounds-check the conjugate cell:
@@ -261,29 +258,6 @@ std_density_runs_kernel(StdArgs a, const double *cgk, int64_t cgk_len, double *g
                 atomicAdd(d, sw);
             }
         }
-    }
-}
-
-// With 2 or more pols every pol of a sample carries the same density
-// ((w0 + w1) / 2), so the image pols' planes differ only by how many pols map
-// to each: plane q = count[q] x the sums gathered in plane q0 = pol_map[0]
-// (counts of at most 8 image pols).
-struct PolCounts {
-    int n[8];
-};
-
-__global__ void __launch_bounds__(kBlock)
-density_copy_pols_kernel(double *grid, int64_t n_imag_chan, int64_t n_imag_pol, int64_t plane,
-                         int q0, PolCounts cnt) {
-    const int64_t n = n_imag_chan * plane;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = i / plane, k = i - c * plane;
-        double *g = grid + c * n_imag_pol * plane + k;
-        const double base = g[q0 * plane];
-        for (int q = 0; q < (int)n_imag_pol; q++)
-            if (q != q0 && cnt.n[q] > 0) g[q * plane] = (double)cnt.n[q] * base;
-        if (cnt.n[q0] != 1) g[q0 * plane] = (double)cnt.n[q0] * base;
     }
 }
 
@@ -1208,26 +1182,13 @@ static int launch_blocks(int64_t n) {
     return (int)b;
 }
 
-// the support-1 density grid: runs kernel, then, when the pols share their
-// density and the call does not accumulate, the other pols' planes
+// the support-1 density grid
 static int density_runs(hipStream_t s, const StdArgs &a, const double *cgk, int64_t cgk_len,
-                        double *g, double *sum_weight, const int32_t *pol_map_host,
-                        int accumulate) {
-    PolCounts cnt{};
-    bool share = a.n_pol >= 2 && !accumulate && a.n_imag_pol <= 8;
-    if (share) {
-        for (int64_t p = 0; p < a.n_pol; p++) cnt.n[pol_map_host[p]]++;
-    }
+                        double *g, double *sum_weight) {
     const int64_t n_thr = a.n_baseline * a.n_chan * ceil_div(a.n_time, (int64_t)64) * 64;
-    std_density_runs_kernel<<<launch_blocks(n_thr), kBlock, 0, s>>>(
-        a, cgk, cgk_len, g, sum_weight, share ? 1 : (int)a.n_pol);
+    std_density_runs_kernel<<<launch_blocks(n_thr), kBlock, 0, s>>>(a, cgk, cgk_len, g,
+                                                                     sum_weight);
     NGC_HIP(hipGetLastError());
-    if (share) {
-        const int64_t plane = a.n_u * a.n_v;
-        density_copy_pols_kernel<<<launch_blocks(a.n_imag_chan * plane), kBlock, 0, s>>>(
-            g, a.n_imag_chan, a.n_imag_pol, plane, pol_map_host[0], cnt);
-        NGC_HIP(hipGetLastError());
-    }
     return NGC_OK;
 }
 
@@ -1632,8 +1593,7 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
             std_grid_plain_kernel<NGC_GRID_PSF><<<blocks, kBlock, shmem, s>>>(
                 a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
         else if (a.support == 1)
-            NGC_TRY(density_runs(s, a, (const double *)d_cgk, cgk_len, g, sum_weight, pol_map,
-                                 accumulate));
+            NGC_TRY(density_runs(s, a, (const double *)d_cgk, cgk_len, g, sum_weight));
         else
             std_grid_plain_kernel<NGC_GRID_DENSITY><<<blocks, kBlock, shmem, s>>>(
                 a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
@@ -1828,8 +1788,7 @@ static int standard_grid_noalloc_impl(ngc_context *ctx, hipStream_t s, int mode,
             std_grid_plain_kernel<NGC_GRID_PSF><<<blocks, kBlock, shmem, s>>>(
                 a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
         else if (a.support == 1)
-            NGC_TRY(density_runs(s, a, (const double *)d_cgk, cgk_len, g, sum_weight, pol_map,
-                                 accumulate));
+            NGC_TRY(density_runs(s, a, (const double *)d_cgk, cgk_len, g, sum_weight));
         else
             std_grid_plain_kernel<NGC_GRID_DENSITY><<<blocks, kBlock, shmem, s>>>(
                 a, (const double *)d_cgk, cgk_len, g, sum_weight, sw_lds);
