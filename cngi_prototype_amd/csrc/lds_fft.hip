@@ -1,7 +1,8 @@
 // Single-precision 2-D FFTs of gridded planes for axis lengths whose prime
-// factors are at most 13 (C2: 2457 = 3^3 7 13): the inverse image transform
-// (make_image.py:116-130: ifftshift, ifft2, fftshift, crop and correction) and
-// the forward placement of a model image for prediction.
+// factors are at most 13 (a 2048^2 image padded by 1.2: 2457 = 3^3 7 13):
+// the inverse image transform (make_image.py:116-130: ifftshift, ifft2,
+// fftshift, crop and correction) and the forward placement of a model image
+// for prediction.
 //
 //   fft_rows_kernel      one workgroup per row: the row is loaded (through the
 //                        ifftshift, for the first pass) into LDS and
@@ -32,8 +33,8 @@
 //                        transposed planes through LDS tiles
 //
 // Passes over the planes: rows (read + write), step A (read + write), step B
-// (read, and the image written); the library transform it replaces took 66 ms
-// for 256 planes of 2457^2, about 9 % of HBM.
+// (read, and the image written); rocFFT took 66 ms for the transform of 256
+// planes of 2457^2, about 9 % of HBM.
 #include <algorithm>
 #include <cmath>
 #include <vector>
