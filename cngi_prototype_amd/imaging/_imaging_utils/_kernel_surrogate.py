@@ -20,7 +20,7 @@ f/f_bin, so the change is to first order proportional to that offset: the
 network learns the smooth rate of change (and its slow dependence on the
 offset), and the product with the offset is exact where the change vanishes.
 
-The exact kernel is the fallback and the judge: training and validation
+The exact kernel is the fallback and the measure of error: training and validation
 targets come from the exact GPU kernels, and with check=True every
 per-channel kernel is compared with the exact one over the taps the gridder
 reads (the support window at every sub-cell offset) and replaced by it when

@@ -59,7 +59,9 @@ class Config:
         return self.n_time * self.n_baseline * self.n_chan * self.n_pol
 
 
-# The workloads of the benchmark (see DESIGN.md for the choices left open).
+# The workloads of the benchmark.  Where their description leaves a value open
+# (integration time, pols, antenna layout, site latitude, sky) the value below
+# is this package's choice.
 CONFIGS = {
     # plain standard gridding, make_psf + make_image, natural weighting
     "C1": Config("C1", 27, "uniform_disk", 1000.0, 34.0784, 30.0, 300, 10.0, 1.4e9, 1e6, 16, 2,

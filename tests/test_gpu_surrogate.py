@@ -1,7 +1,7 @@
 """The learned A-term kernel surrogate: bf16 matrix-core GEMM against the
 bf16-rounded float64 product, forward/backward against the CPU oracle
 (oracle/mlp.py), training, and the gain over the reference's per-bin kernels
-with the exact kernel as judge and fallback."""
+with the exact kernel as the measure of error and the fallback."""
 import numpy as np
 import pytest
 
