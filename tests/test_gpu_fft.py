@@ -131,10 +131,13 @@ def test_lds_model_to_grid_of_a_point_source(gpu):
 @pytest.mark.parametrize("n_u,n_v,N_l,N_m", [
     (2197, 64, 2000, 60),      # 13^3: no column split N1 <= N2 <= 128, transposes
     (8192, 40, 8000, 33),      # 64 x 128 split, 8 8 8 8 2 rows of 40 = 8 5
+    (48, 3969, 40, 3500),      # rows of 63^2 = 3^4 7^2 (16 loads per thread)
+    (36, 8192, 30, 8000),      # rows of 2^13 (32 loads per thread)
 ])
 def test_lds_transforms_at_long_columns(gpu, n_u, n_v, N_l, N_m):
     """Column lengths at the ends of the four-step split and past it (the
-    transpose path), both directions, against the oracle."""
+    transpose path), and rows long enough for 16 and 32 loads per thread,
+    both directions, against the oracle."""
     import cngi_prototype_amd._native as nat
     from cngi_prototype_amd.imaging._imaging_utils._gridding_convolutional_kernels import \
         cropped_correction_factors
