@@ -7,7 +7,7 @@
 //   fft_rows_kernel      one workgroup per row: the row is loaded (through the
 //                        ifftshift, for the first pass) into LDS and
 //                        transformed there by a mixed-radix Stockham FFT
-//                        (radices 8, 4, 2, 13, 11, 7, 5, 3): stage s with
+//                        (radices 8, 4, 2, 13, 11, 9, 7, 5, 3): stage s with
 //                        radix R reads R values N/R apart, twiddles them by
 //                        the powers of exp(i 2 pi k / (Ns R)) (Ns = product of
 //                        the earlier radices, k = j mod Ns by a multiply-high
@@ -65,7 +65,7 @@ static bool fft_factor(int64_t n, FftRadices &f) {
     f.n = (int)n;
     f.count = 0;
     int64_t m = n, ns = 1;
-    for (int r : {8, 4, 2, 13, 11, 7, 5, 3}) {
+    for (int r : {8, 4, 2, 13, 11, 9, 7, 5, 3}) {
         while (m % r == 0) {
             if (f.count == kFftMaxStages) return false;
             FftStage &s = f.st[f.count++];
@@ -242,6 +242,7 @@ __device__ float2 *fft_run(float2 *A, float2 *B, const FftRadices &f,
         case 5: fft_stage<5, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
         case 7: fft_stage<7, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
         case 8: fft_stage<8, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
+        case 9: fft_stage<9, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
         case 11: fft_stage<11, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
         default: fft_stage<13, SIGN, L, STR>(A, B, f.n, st, tw, TS, lt, c); break;
         }
