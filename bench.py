@@ -980,7 +980,7 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16, 32],
                     help="uv-tile rows of the gridder (0: library default, 16)")
     ap.add_argument("--tile-waves", type=int, default=0, choices=[0, 1, 2, 4],
-                    help="wavefronts per tile of the gridder, splitting the pols "
+                    help="wavefronts per tile of the gridder, splitting its records "
                          "(0: library default)")
     ap.add_argument("--bin-lanes", type=int, default=0, choices=[0, 1, 2, 4, 8],
                     help="channels per binning wavefront (0: library default)")
@@ -1303,8 +1303,8 @@ def main():
                    "samples": int(cfg.n_vis), "unflagged": n_unflagged,
                    "planes": int(n_planes),
                    "form": f"{args.precision} (uv-tiled bins, a workgroup of "
-                           f"{args.tile_waves or 2} wavefront(s) per "
-                           f"{args.tile_rows or 16}x32-cell tile splitting the pols, "
+                           f"{args.tile_waves or 4} wavefront(s) per "
+                           f"{args.tile_rows or 16}x32-cell tile splitting its records, "
                            f"lane-owned residue classes, FP32 register runs added to an "
                            f"LDS subgrid (FP64 cells for fp32_kahan))"},
         "roofline": roofline,

@@ -607,8 +607,9 @@ def set_tile_rows(rows):
 
 
 def set_tile_waves(waves):
-    """Wavefronts per tile of the uv-tiled standard gridder (0: default, 2 for
-    4 pols and 1 otherwise; 1, 2 or 4) on the current device."""
+    """Wavefronts per tile of the uv-tiled standard gridder, each gridding a
+    share of the tile's records (1, 2 or 4; 0: default, 4 for the
+    single-precision forms and 2 for FP64 tiled) on the current device."""
     ctx = context()
     _check("ngc_set_tile_waves", load_library().ngc_set_tile_waves(ctx.handle, int(waves)))
 

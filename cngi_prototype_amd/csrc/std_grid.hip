@@ -604,8 +604,22 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
 // lanes share a cell, but the read-modify-write then happens inside the LDS
 // unit and the wavefront does not wait for it.  FP32_KAHAN keeps FP64 cells
 // (one conversion and one add per flush); the tile is written to HBM once at
-// the end.  The W wavefronts of a workgroup split the pols of the tile, each
-// with its own accumulators and the same samples.
+// the end.
+//
+// The W wavefronts of a workgroup split the tile's records: wavefront w
+// grids the w-th contiguous share of the bin's records into every pol, with
+// all of the tile's accumulators in its own registers, and stages, reads and
+// runs its share on its own, so no barrier stands between sample batches
+// (the staging arrays are per wavefront; the LDS executes one wavefront's
+// instructions in order, and a wavefront-scope fence keeps the compiler from
+// moving its reads above its stores).  Cells that two shares both reach are
+// added by both wavefronts: the flush is an LDS atomic add.  The subgrid is
+// one per workgroup, so W raises the wavefronts per CU at the LDS cost of the
+// staging arrays only.  (Splitting the pols instead, every wavefront running
+// every record for its share of the accumulators, repeated the per-sample
+// product and staging and held the wavefronts together at two barriers per
+// batch: 15.0 ms against 10.9 ms for 4 record shares at 627M samples into
+// 256 planes of 2457^2.)
 //
 // Per-sample work is done once per sample and axis by the lanes that stage
 // it, in per-residue layout: for residue r the tap weight of the lanes owning
@@ -619,8 +633,6 @@ std_bin_kernel(StdArgs a, BinGeom g, unsigned long long *counts, char *entries,
 // the current batch is gridded.  TU (8, 16 or 32 rows) trades LDS per
 // workgroup, hence occupancy, against the share of samples binned to two
 // tiles.
-constexpr int kBatch = 32;
-constexpr int kGroup = 8;
 constexpr int kBuckets = 40;       // bin-size classes of the largest-first order
 
 // size class of a bin of n records: 0 when empty, else floor(log2 n) + 1
@@ -650,8 +662,17 @@ __device__ inline int subgrid_index(int u, int v) {
 // cell of residue r for a window starting at w: the first u >= w with u = r (mod 8)
 __device__ inline int residue_cell(int w, int r) { return w + ((r - w) & 7); }
 
+// Samples per staging batch of one wavefront: 16, or 8 where four
+// wavefronts' staging arrays beside a subgrid of 32 KB or more would
+// otherwise cost a workgroup per CU; read in double-buffered groups of 4.
+template <int W, int SUBGRID_BYTES> constexpr int split_batch() {
+    return (W >= 4 && SUBGRID_BYTES >= 32768) ? 8 : 16;
+}
+constexpr int kSplitGroup = 4;
+
 template <int MODE, int NP, typename ACC, bool KAHAN, int TU, int W>
 __global__ void __launch_bounds__(64 * W)
+    __attribute__((amdgpu_waves_per_eu((W >= 4 && sizeof(ACC) == 4) ? 4 : 2)))
 std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict__ offsets,
                       const unsigned int *__restrict__ order,
                       const unsigned int *__restrict__ hist, int *queue, int n_items,
@@ -660,67 +681,44 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
                       int accumulate) {
     constexpr int NC = (MODE == NGC_GRID_IMAGE) ? 2 : 1;
     constexpr int NPC = NP * NC;
-    static_assert(NP % W == 0, "the pols split evenly over the wavefronts");
-    constexpr int QW = NPC / W;         // accumulators (pol, re/im) of one wavefront
-    constexpr int SPW = kBatch / W;     // samples staged by one wavefront per batch
+    using CELL = typename std::conditional<KAHAN, double, ACC>::type;
+    constexpr int PLANE = TU * kTile;
+    constexpr int SB = split_batch<W, NPC * PLANE * (int)sizeof(CELL)>();
+    constexpr int KG = kSplitGroup;
     constexpr int NT = 64 * W;
     using E = Entry<NP, NC, ACC>;
     constexpr int EB = EntrySize<NP, NC, ACC>::bytes;
-    // weighted visibilities of a staged sample, split by wavefront so that a
-    // wavefront loads only its own accumulators' values
     struct WD {
-        ACC v[NPC / W];
+        ACC v[NPC];
     };
-
-    // LDS subgrid: plain sums, or for the compensated form FP64 cells (the
-    // same 8 bytes as a (sum, compensation) pair of floats, and one
-    // conversion and one add per flush instead of a TwoSum)
-    using CELL = typename std::conditional<KAHAN, double, ACC>::type;
-    constexpr int PLANE = TU * kTile;
     __shared__ CELL s_acc[NPC * PLANE];
-    // per staged sample, axis (u, v) and residue: the tap weight, sign bit
-    // set when the residue's cell differs from the previous sample's; rows
-    // padded so the staging writes spread over the banks.  The cells follow
-    // from the sample's window start (tile-relative, per axis), which the
-    // lanes need only when the window moved.
     constexpr int AXROW = 9;
-    __shared__ ACC s_ax[2][kBatch][AXROW];
-    __shared__ int2 s_c0[kBatch];
-    __shared__ WD s_wd[W][kBatch];
-    __shared__ ACC s_k[513];          // the kernel table; zero from cgk_len on
-    __shared__ unsigned s_chg[W];     // per wavefront: its samples whose window moved
+    __shared__ ACC s_ax[W][2][SB][AXROW];
+    __shared__ int2 s_c0[W][SB];
+    __shared__ WD s_wd[W][SB];
+    __shared__ ACC s_k[513];
     __shared__ int s_item;
-    __shared__ int s_nz, s_e;   // non-empty bins; the empty-tile ticket
-    __shared__ int64_t s_tq[kChunk][3];   // bin, first and end record of each tile of a ticket
+    __shared__ int s_nz, s_e;
+    __shared__ int64_t s_tq[kChunk][3];
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int q0 = wave * QW;  // this wavefront's accumulators are q0 .. q0 + QW - 1
-    // the kernel table, once per workgroup: all loads issued before the stores
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     {
-        constexpr int KL = 512 / NT;
+        constexpr int KL = (512 + NT - 1) / NT;
         ACC kv[KL];
 #pragma unroll
         for (int k = 0; k < KL; k++) {
             const int i = tid + k * NT;
-            kv[k] = i < cgk_len ? (ACC)cgk[i] : (ACC)0;
+            kv[k] = i < cgk_len && i < 512 ? (ACC)cgk[i] : (ACC)0;
         }
 #pragma unroll
-        for (int k = 0; k < KL; k++) s_k[tid + k * NT] = kv[k];
-        if (tid == 0) s_k[512] = (ACC)0;   // where taps past the table clamp
+        for (int k = 0; k < KL; k++)
+            if (tid + k * NT < 512) s_k[tid + k * NT] = kv[k];
+        if (tid == 0) s_k[512] = (ACC)0;
     }
 #pragma unroll 4
     for (int i = tid; i < NPC * PLANE; i += NT) s_acc[i] = CELL{};
-    // persistent: each workgroup takes the next tile (largest bins first) from
-    // the queue until all n_items are done; every wave reaches the exit
-    // work tickets: bins of kChunkBelow records or more (the front of the
-    // largest-first order) are handed out one per ticket, for balance; the
-    // small non-empty bins after them in runs of kChunk (fewer atomics on the
-    // queue counter, one round trip for their records).  The empty bins, last
-    // in the order, have a queue of their own (queue[2], kEmptyChunk tiles a
-    // ticket): after each non-empty tile a workgroup zeroes one chunk of them
-    // straight from registers, so their stores overlap the gridding instead of
-    // forming a store-bound tail; in accumulate mode they are left untouched.
     if (tid == 0) {
         int nb = 0;
         for (int b = bin_bucket(kChunkBelow); b < kBuckets; b++) nb += (int)hist[b];
@@ -733,11 +731,10 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     using OUT = typename std::conditional<NC == 2,
         typename std::conditional<std::is_same<ACC, float>::value, float2, double2>::type,
         ACC>::type;
-    constexpr int F = NP * TU * kTile;   // grid values of one tile
+    constexpr int F = NP * TU * kTile;
     bool zero_more = !accumulate;
-    // one ticket of the empty-tile queue; false when it is exhausted
     auto zero_chunk = [&]() -> bool {
-        __syncthreads();  // every read of s_e is done
+        __syncthreads();
         if (tid == 0) s_e = atomicAdd(queue + 2, 1);
         __syncthreads();
         const int64_t first = (int64_t)n_nz + (int64_t)s_e * kEmptyChunk;
@@ -764,9 +761,16 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         }
         return true;
     };
+    // no compiler motion of LDS accesses across this point; in hardware one
+    // wavefront's LDS instructions complete in order
+    auto wave_sync = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     int it_next = 0, it_end = 0, it_first = 0;
     for (;;) {
-    __syncthreads();  // the previous tile's LDS reads (and s_item, s_tq) are complete
+    __syncthreads();
     if (it_next >= it_end) {
         if (tid == 0) s_item = atomicAdd(queue, 1);
         __syncthreads();
@@ -781,7 +785,6 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         }
         if (it_end > n_nz) it_end = n_nz;
         if (it_next >= n_nz) break;
-        // the chunk's bins and record ranges, loaded together
         it_first = it_next;
         if (tid < it_end - it_next) {
             const int64_t b = order[it_next + tid];
@@ -799,55 +802,46 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     const int64_t trem = bin - a_chan * tiles;
     const int u_lo = (int)((trem / g.ntv) * TU);
     const int v_lo = (int)((trem % g.ntv) * kTile);
-    const int64_t start = s_tq[ci][1];
-    const int64_t end = s_tq[ci][2];
-
-    // (the subgrid is clear here: zeroed at the start, then by each write-out)
+    // this wavefront's share of the bin's records
+    const int64_t b_start = s_tq[ci][1], b_n = s_tq[ci][2] - b_start;
+    const int64_t start = b_start + b_n * wave / W;
+    const int64_t end = b_start + b_n * (wave + 1) / W;
 
     const int sc = (int)(a.support / 2);
     const int S = (int)a.support;
     const int os = (int)a.os;
-    const int ra = lane & 7, rb = lane >> 3;  // owned residues
+    const int ra = lane & 7, rb = lane >> 3;
 
-    int cur_u = -64, cur_v = -64;  // current cell (tile-relative), off-tile = none
-    ACC acc[QW];
+    int cur_u = -64, cur_v = -64;
+    ACC acc[NPC];
 #pragma unroll
-    for (int q = 0; q < QW; q++) acc[q] = (ACC)0;
+    for (int q = 0; q < NPC; q++) acc[q] = (ACC)0;
 
     auto flush = [&]() {
         if ((unsigned)cur_u < (unsigned)TU && (unsigned)cur_v < (unsigned)kTile) {
             const int o = subgrid_index<(int)sizeof(CELL)>(cur_u, cur_v);
 #pragma unroll
-            for (int q = 0; q < QW; q++) {
-                // a read-modify-write inside the LDS unit (no two lanes share a
-                // cell; the atomic only saves the wavefront the round trip)
-                __hip_atomic_fetch_add(&s_acc[(q0 + q) * PLANE + o], (CELL)acc[q],
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
+            for (int q = 0; q < NPC; q++)
+                __hip_atomic_fetch_add(&s_acc[q * PLANE + o], (CELL)acc[q], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     };
 
-    // staging: every lane of every wavefront stages taps it = h RPL ..
-    // h RPL + RPL - 1 of one axis of one sample of the batch at b0: sample
-    // j = w SPW + js (entry b0 + j) of wavefront w, with js = l % SPW,
-    // axis = (l / SPW) % 2 and h = l / (2 SPW) for lane l.  Tap `it` of a
-    // window starting at c0 is the cell c0 + it, owned by residue
-    // (c0 + it) mod 8.  The next batch's entries are loaded while the current
-    // one is gridded.
-    constexpr int H = 64 / (2 * SPW);   // lanes per (sample, axis) = W
-    constexpr int RPL = 8 / H;          // taps per lane
+    // staging: lane l stages taps h RPL .. h RPL + RPL - 1 of axis
+    // (l / SB) % 2 of sample l % SB of the batch, h = l / (2 SB)
+    constexpr int H = 64 / (2 * SB);
+    constexpr int RPL = 8 / H;
     static_assert(H * RPL == 8, "the taps of an axis split evenly over its lanes");
-    const int js = lane % SPW, axis = (lane / SPW) & 1, h = lane / (2 * SPW);
-    const int j_st = wave * SPW + js;
-    const bool wd_lane = axis == 0 && h == 0;   // also stages the weighted visibilities
+    const int js = lane % SB, axis = (lane / SB) & 1, h = lane / (2 * SB);
+    const bool wd_lane = axis == 0 && h == 0;
     const int shift = axis ? 16 : 0;
     const int lo = (axis ? v_lo : u_lo) + sc;
     const int it0 = h * RPL;
-    int2 hdr = make_int2(0, 0);   // (uv, duv) of the lane's entry
-    unsigned int puv = 0;         // uv of the entry before it
-    WD wdl[W];
+    int2 hdr = make_int2(0, 0);
+    unsigned int puv = 0;
+    WD wdl;
     auto load_batch = [&](int64_t b) {
-        const int64_t e = b + j_st;
+        const int64_t e = b + js;
         if (e < end) {
             const char *p = entries + e * EB;
             hdr = *reinterpret_cast<const int2 *>(p);
@@ -855,35 +849,22 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
             if (wd_lane) {
                 const E *ep = reinterpret_cast<const E *>(p);
 #pragma unroll
-                for (int gw = 0; gw < W; gw++)
-#pragma unroll
-                    for (int q = 0; q < QW; q++) {
-                        const int pq = gw * QW + q;
-                        wdl[gw].v[q] = ep->wd[pq / NC][pq % NC];
-                    }
+                for (int q = 0; q < NPC; q++) wdl.v[q] = ep->wd[q / NC][q % NC];
             }
         }
     };
     load_batch(start);
-    for (int64_t b0 = start; b0 < end; b0 += kBatch) {
-        const int64_t e = b0 + j_st;
+    for (int64_t b0 = start; b0 < end; b0 += SB) {
+        const int64_t e = b0 + js;
         const bool valid = e < end;
+        unsigned chg;
         {
-            // this lane's axis: window start c0, its shift from the previous
-            // sample's window (any nonzero value for the first entry of the
-            // bin) and the sub-cell offset.  Past the end of the bin the
-            // window does not move and the weighted visibilities are zero, so
-            // the lanes keep (and finally flush) the last real cell.
             int c0 = -1000, dlt = 0, d = 0;
             if (valid) {
                 c0 = (int)(((unsigned)hdr.x >> shift) & 0xffffu) - lo;
                 d = (int)(short)(((unsigned)hdr.y >> shift) & 0xffffu);
                 dlt = e > start ? c0 - ((int)((puv >> shift) & 0xffffu) - lo) : 1000;
             }
-            // residue (c0 + it) mod 8 changes cell iff the shift moves the
-            // window by it or more cells: unsigned(dlt + it) >= 8
-            // all table reads first (k <= 512 is always in bounds), then the
-            // records; s_k is zero from cgk_len on
             const int x0 = os * (it0 - sc) + d;
             ACC tk[RPL];
 #pragma unroll
@@ -896,87 +877,59 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
                 const int it = it0 + i;
                 const ACC t = it < S ? tk[i] : (ACC)0;
                 const bool moved = (unsigned)(dlt + it) >= 8u;
-                // sign bit set, even for 0
-                s_ax[axis][j_st][(c0 + it) & 7] = moved ? -t - (ACC)0 : t;
+                s_ax[wave][axis][js][(c0 + it) & 7] = moved ? -t - (ACC)0 : t;
             }
-            if (h == 0) reinterpret_cast<int *>(&s_c0[j_st])[axis] = c0;
+            if (h == 0) reinterpret_cast<int *>(&s_c0[wave][js])[axis] = c0;
             if (wd_lane) {
+                WD w;
 #pragma unroll
-                for (int gw = 0; gw < W; gw++) {
-                    WD w;
-#pragma unroll
-                    for (int q = 0; q < QW; q++) w.v[q] = valid ? wdl[gw].v[q] : (ACC)0;
-                    s_wd[gw][j_st] = w;
-                }
+                for (int q = 0; q < NPC; q++) w.v[q] = valid ? wdl.v[q] : (ACC)0;
+                s_wd[wave][js] = w;
             }
-            // samples whose window moved on either axis: one bit per sample
-            // of this wavefront's share, from the it = 0 lanes
+            // wave-uniform: samples of the batch whose window moved on either axis
             const uint64_t mv = __ballot(h == 0 && dlt != 0);
-            if (lane == 0) {
-                const uint64_t m = (mv | (mv >> SPW)) & (~0ull >> (64 - SPW));
-                s_chg[wave] = (unsigned)(m << (wave * SPW));
-            }
+            chg = (unsigned)((mv | (mv >> SB)) & ((1ull << SB) - 1));
         }
         const int64_t nleft = end - b0;
-        const int nh = nleft >= kBatch ? kBatch : (int)nleft;
-        __syncthreads();
+        const int nh = nleft >= SB ? SB : (int)nleft;
+        wave_sync();
 
-        // groups of kGroup samples, double-buffered: the next group's LDS reads
-        // are issued before this group's flushes, so they do not queue
-        // behind its adds
-        const int ng = (nh + kGroup - 1) / kGroup;
-        unsigned chg = 0;   // samples whose window moved, one bit each
-        // (lane l also holds the window start of sample l % kGroup of the
-        // group, which a moved sample's new cells are taken from)
-        ACC au0[kGroup], av0[kGroup], au1[kGroup], av1[kGroup];
-        WD wd0[kGroup], wd1[kGroup];
+        const int ng = (nh + KG - 1) / KG;
+        ACC au0[KG], av0[KG], au1[KG], av1[KG];
+        WD wd0[KG], wd1[KG];
         int2 c00, c01;
-        auto load_group = [&](int gi, ACC (&au)[kGroup], ACC (&av)[kGroup], WD (&wd)[kGroup],
-                              int2 &c0l) {
+        auto load_group = [&](int gi, ACC (&au)[KG], ACC (&av)[KG], WD (&wd)[KG], int2 &c0l) {
 #pragma unroll
-            for (int jj = 0; jj < kGroup; jj++) {
-                au[jj] = s_ax[0][gi * kGroup + jj][ra];
-                av[jj] = s_ax[1][gi * kGroup + jj][rb];
-                wd[jj] = s_wd[wave][gi * kGroup + jj];
+            for (int jj = 0; jj < KG; jj++) {
+                au[jj] = s_ax[wave][0][gi * KG + jj][ra];
+                av[jj] = s_ax[wave][1][gi * KG + jj][rb];
+                wd[jj] = s_wd[wave][gi * KG + jj];
             }
-            c0l = s_c0[gi * kGroup + (lane & (kGroup - 1))];
+            c0l = s_c0[wave][gi * KG + (lane & (KG - 1))];
         };
-        auto run_group = [&](int gi, const ACC (&au)[kGroup], const ACC (&av)[kGroup],
-                             const WD (&wd)[kGroup], const int2 &c0l) {
+        auto run_group = [&](int gi, const ACC (&au)[KG], const ACC (&av)[KG],
+                             const WD (&wd)[KG], const int2 &c0l) {
 #pragma unroll
-            for (int jj = 0; jj < kGroup; jj++) {
-                if ((chg >> (gi * kGroup + jj)) & 1u) {
-                    // the new window; reading it first also keeps this a
-                    // scalar branch around the per-lane test below
+            for (int jj = 0; jj < KG; jj++) {
+                if ((chg >> (gi * KG + jj)) & 1u) {
                     const int wu = __builtin_amdgcn_readlane(c0l.x, jj);
                     const int wv = __builtin_amdgcn_readlane(c0l.y, jj);
                     if (signbit(au[jj]) || signbit(av[jj])) {
                         flush();
-                        // the cell of residue (ra, rb) in the new window
                         cur_u = residue_cell(wu, ra);
                         cur_v = residue_cell(wv, rb);
 #pragma unroll
-                        for (int q = 0; q < QW; q++) acc[q] = (ACC)0;
+                        for (int q = 0; q < NPC; q++) acc[q] = (ACC)0;
                     }
                 }
                 const ACC wt = fabs(au[jj]) * fabs(av[jj]);
 #pragma unroll
-                for (int q = 0; q < QW; q++) acc[q] += wt * wd[jj].v[q];
+                for (int q = 0; q < NPC; q++) acc[q] += wt * wd[jj].v[q];
             }
         };
-        // The prefetch is unconditional (past the last group it reloads
-        // group kBatch / kGroup - 1, whose staged records are in bounds and
-        // unused), so the compiler knows how many LDS reads are in flight and
-        // waits only for the current group's, not for the prefetch.
-        constexpr int kLastGroup = kBatch / kGroup - 1;
+        constexpr int kLastGroup = SB / KG - 1;
         load_group(0, au0, av0, wd0, c00);
-        // (read with the first group's records: one wait for both)
-        unsigned chg_v = 0;
-#pragma unroll
-        for (int gw = 0; gw < W; gw++) chg_v |= s_chg[gw];
-        // wave-uniform: the per-sample cell checks below are scalar branches
-        chg = __builtin_amdgcn_readfirstlane(chg_v);
-        if (b0 + kBatch < end) load_batch(b0 + kBatch);
+        if (b0 + SB < end) load_batch(b0 + SB);
         for (int gi = 0; gi < ng; gi += 2) {
             load_group(min(gi + 1, kLastGroup), au1, av1, wd1, c01);
             run_group(gi, au0, av0, wd0, c00);
@@ -984,14 +937,11 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
             load_group(min(gi + 2, kLastGroup), au0, av0, wd0, c00);
             run_group(gi + 1, au1, av1, wd1, c01);
         }
-        __syncthreads();
+        wave_sync();
     }
     flush();
     __syncthreads();
 
-    // write the tile, lanes running along v (each row a contiguous run of the
-    // grid), and clear the subgrid for the next tile: every cell of the grid
-    // is written exactly once (empty tiles by the empty-tile queue)
 #pragma unroll 4
     for (int k = tid; k < F; k += NT) {
         const int lv = k % kTile;
@@ -1227,7 +1177,7 @@ static void norm_table(const double *cgk, int64_t cgk_len, int64_t support, int6
 struct TiledCfg {
     int tu = 16;        // tile rows (8, 16 or 32); tiles are 32 columns wide
     int ch_lanes = 2;   // channels per binning wavefront (1, 2, 4 or 8)
-    int waves = 0;      // wavefronts per tile, splitting the pols (0: default)
+    int waves = 0;      // wavefronts per tile, splitting its records (0: default)
 };
 
 struct TiledGeom {
@@ -1420,11 +1370,12 @@ static int enqueue_fill_grid(ngc_context *ctx, hipStream_t s, const StdArgs &a,
     return NGC_OK;
 }
 
-// Wavefronts per tile for n_pol pols: `req` when it divides the pols into
-// equal groups (1, 2 or 4), else the measured best: 2 for 4 pols, 1 otherwise.
-static int tile_waves(int64_t n_pol, int req) {
-    if ((req == 1 || req == 2 || req == 4) && n_pol % req == 0 && n_pol <= 4) return req;
-    return n_pol == 4 ? 2 : 1;
+// Wavefronts per tile (record shares): `req` when it is 1, 2 or 4, else the
+// measured best: 4 for the single-precision forms; 2 for FP64 tiled, whose
+// accumulators and staged values take twice the registers.
+static int tile_waves(bool single, int req) {
+    if (req == 1 || req == 2 || req == 4) return req;
+    return single ? 4 : 2;
 }
 
 // Calls f(mode_c, np_c, acc_c, kahan_c, tu_c, w_c) with the compile-time form of
@@ -1434,12 +1385,9 @@ static int tiled_dispatch(int mode, int64_t n_pol, int precision, int tu, int wa
     struct F32 { using type = float; };
     struct F64 { using type = double; };
     auto by_w = [&](auto m, auto np, auto acc, auto k, auto t) -> int {
-        constexpr int NP = decltype(np)::value;
-        const int w = tile_waves(NP, waves);
-        if constexpr (NP % 4 == 0)
-            if (w == 4) return f(m, np, acc, k, t, std::integral_constant<int, 4>{});
-        if constexpr (NP % 2 == 0)
-            if (w == 2) return f(m, np, acc, k, t, std::integral_constant<int, 2>{});
+        const int w = tile_waves(std::is_same<typename decltype(acc)::type, float>::value, waves);
+        if (w == 4) return f(m, np, acc, k, t, std::integral_constant<int, 4>{});
+        if (w == 2) return f(m, np, acc, k, t, std::integral_constant<int, 2>{});
         return f(m, np, acc, k, t, std::integral_constant<int, 1>{});
     };
     auto by_tu = [&](auto m, auto np, auto acc, auto k) -> int {

@@ -564,10 +564,11 @@ int ngc_set_tile_rows(ngc_context *ctx, int tile_rows);
 
 /*
  * Wavefronts per tile of the uv-tiled standard gridder (no counterpart in the
- * reference): the pols of a tile are split evenly over 1, 2 or 4 wavefronts
- * of one workgroup, which share the tile's LDS subgrid and staged samples; a
- * count that does not divide the pols falls back to the default.  0 (the
- * default) = 2 wavefronts for 4 pols, else 1.
+ * reference): 1, 2 or 4 wavefronts of one workgroup share the tile's LDS
+ * subgrid, each gridding a contiguous share of the tile's records into every
+ * pol.  0 (the default) = 4 for the single-precision forms, 2 for FP64 tiled.
+ * Results agree to the rounding of the register runs, whose grouping of the
+ * records follows the shares.
  */
 int ngc_set_tile_waves(ngc_context *ctx, int waves);
 

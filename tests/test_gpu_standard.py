@@ -45,8 +45,10 @@ def test_standard_grid_matches_oracle(gpu, precision, mode, rows):
 @pytest.mark.parametrize("n_pol", [1, 2, 4])
 @pytest.mark.parametrize("precision", ["fp64_tiled", "fp32_kahan"])
 def test_standard_grid_wavefronts_per_tile(gpu, precision, n_pol, waves):
-    """The pols of a tile split over 1, 2 or 4 wavefronts (a count that does
-    not divide the pols falls back to the default) give the oracle's grid."""
+    """A tile's records split into 1, 2 or 4 contiguous shares, one per
+    wavefront (whose cells overlap where a share boundary falls inside a
+    track, and some of which are empty in the small bins), give the oracle's
+    grid."""
     nat = _native()
     nat.set_tile_waves(waves)
     try:
