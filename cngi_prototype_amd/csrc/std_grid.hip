@@ -662,17 +662,17 @@ __device__ inline int subgrid_index(int u, int v) {
 // cell of residue r for a window starting at w: the first u >= w with u = r (mod 8)
 __device__ inline int residue_cell(int w, int r) { return w + ((r - w) & 7); }
 
-// Samples per staging batch of one wavefront: 16, or 8 where four
-// wavefronts' staging arrays beside a subgrid of 32 KB or more would
-// otherwise cost a workgroup per CU; read in double-buffered groups of 4.
-template <int W, int SUBGRID_BYTES> constexpr int split_batch() {
-    return (W >= 4 && SUBGRID_BYTES >= 32768) ? 8 : 16;
-}
-constexpr int kSplitGroup = 4;
+// Samples per staging batch of one wavefront: one lane per (sample, axis)
+// stages all 8 taps.  Measured at 627M samples into 256 planes of 2457^2
+// (4 wavefronts, 16-row tiles): batches of 8 samples 11.0 ms (4 workgroups
+// per CU), 16 samples 10.4 ms and 32 samples 10.0 ms (3 workgroups per CU by
+// LDS, but half or a quarter of the per-batch staging work per sample).
+constexpr int kSplitBatch = 32;
+constexpr int kSplitGroup = 4;   // samples per double-buffered read group
 
 template <int MODE, int NP, typename ACC, bool KAHAN, int TU, int W>
 __global__ void __launch_bounds__(64 * W)
-    __attribute__((amdgpu_waves_per_eu((W >= 4 && sizeof(ACC) == 4) ? 4 : 2)))
+    __attribute__((amdgpu_waves_per_eu((W >= 4 && sizeof(ACC) == 4) ? 3 : 2)))
 std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict__ offsets,
                       const unsigned int *__restrict__ order,
                       const unsigned int *__restrict__ hist, int *queue, int n_items,
@@ -683,7 +683,7 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     constexpr int NPC = NP * NC;
     using CELL = typename std::conditional<KAHAN, double, ACC>::type;
     constexpr int PLANE = TU * kTile;
-    constexpr int SB = split_batch<W, NPC * PLANE * (int)sizeof(CELL)>();
+    constexpr int SB = kSplitBatch;
     constexpr int KG = kSplitGroup;
     constexpr int NT = 64 * W;
     using E = Entry<NP, NC, ACC>;
@@ -837,6 +837,11 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     const int shift = axis ? 16 : 0;
     const int lo = (axis ? v_lo : u_lo) + sc;
     const int it0 = h * RPL;
+    // table offset of each of the lane's taps before the sub-cell shift; taps
+    // past the support get one that clamps to the zero entry s_k[512]
+    int toff[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; i++) toff[i] = it0 + i < S ? os * (it0 + i - sc) : (1 << 20);
     int2 hdr = make_int2(0, 0);
     unsigned int puv = 0;
     WD wdl;
@@ -865,29 +870,30 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
                 d = (int)(short)(((unsigned)hdr.y >> shift) & 0xffffu);
                 dlt = e > start ? c0 - ((int)((puv >> shift) & 0xffffu) - lo) : 1000;
             }
-            const int x0 = os * (it0 - sc) + d;
             ACC tk[RPL];
 #pragma unroll
             for (int i = 0; i < RPL; i++) {
-                const int x = x0 + os * i;
+                const int x = toff[i] + d;
                 tk[i] = s_k[min(x < 0 ? -x : x, 512)];
             }
 #pragma unroll
             for (int i = 0; i < RPL; i++) {
                 const int it = it0 + i;
-                const ACC t = it < S ? tk[i] : (ACC)0;
+                const ACC t = tk[i];
                 const bool moved = (unsigned)(dlt + it) >= 8u;
                 s_ax[wave][axis][js][(c0 + it) & 7] = moved ? -t - (ACC)0 : t;
             }
             if (h == 0) reinterpret_cast<int *>(&s_c0[wave][js])[axis] = c0;
             if (wd_lane) {
-                WD w;
+                if (end - b0 < SB) {   // the share's last batch: records past its end add nothing
 #pragma unroll
-                for (int q = 0; q < NPC; q++) w.v[q] = valid ? wdl.v[q] : (ACC)0;
-                s_wd[wave][js] = w;
+                    for (int q = 0; q < NPC; q++) wdl.v[q] = valid ? wdl.v[q] : (ACC)0;
+                }
+                s_wd[wave][js] = wdl;
             }
-            // wave-uniform: samples of the batch whose window moved on either axis
-            const uint64_t mv = __ballot(h == 0 && dlt != 0);
+            // wave-uniform: samples of the batch whose window moved on either
+            // axis, from the first-tap lanes (h = 0: lanes 0 .. 2 SB - 1)
+            const uint64_t mv = __ballot(dlt != 0) & (~0ull >> (64 - 2 * SB));
             chg = (unsigned)((mv | (mv >> SB)) & ((1ull << SB) - 1));
         }
         const int64_t nleft = end - b0;
