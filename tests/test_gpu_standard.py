@@ -58,6 +58,46 @@ def test_standard_grid_wavefronts_per_tile(gpu, precision, n_pol, waves):
         nat.set_tile_waves(0)
 
 
+@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("precision", ["fp64_tiled", "fp32_kahan"])
+def test_standard_grid_long_bins(gpu, precision, waves):
+    """All samples of 300 integrations squeezed into the few tiles around the
+    uv centre (uvw scaled by 0.02): bins of tens of thousands of records, so
+    every wavefront's share runs many staging batches and the share
+    boundaries fall inside long runs of one cell.  FP64 tiled must match the
+    oracle to rounding; FP32-Kahan keeps FP32 register runs of thousands of
+    samples before each flush, hence the looser bound."""
+    import torch
+    nat = _native()
+    c = small_case(n_time=300, n_chan=2, n_pol=4)
+    uvw = c["uvw"] * 0.02
+    cgk = oracle.ps_kernel_1d(100, 7)
+    npad = c["npad"]
+    single = precision == "fp32_kahan"
+    visr, wr = c["vis"], c["weight"]
+    if single:
+        visr = visr.astype(np.complex64).astype(np.complex128)
+        wr = wr.astype(np.float32).astype(np.float64)
+    g_ref, sw_ref = oracle.standard_grid(visr, uvw, wr, c["freq"], cgk, [npad, npad],
+                                         c["delta"], 7, 100, do_psf=False)
+    # the footprints cover only a few tiles of 16 x 32 cells
+    nz_u, nz_v = np.nonzero(np.abs(g_ref).sum(axis=(0, 1)))
+    assert nz_u.max() - nz_u.min() < 16 and nz_v.max() - nz_v.min() < 32
+    nat.set_tile_waves(waves)
+    try:
+        g, sw = nat.standard_grid(_dev(uvw), _dev(c["weight"], torch.float32 if single
+                                                   else torch.float64),
+                                  c["freq"], np.arange(2), np.arange(4), cgk, 2, [npad, npad],
+                                  c["delta"], 7, 100, nat.GRID_IMAGE, precision,
+                                  vis=_dev(c["vis"], torch.complex64 if single
+                                           else torch.complex128))
+        torch.cuda.synchronize()
+    finally:
+        nat.set_tile_waves(0)
+    assert rel_err(sw.cpu().numpy(), sw_ref) < 1e-12
+    assert rel_err(g.cpu().numpy(), g_ref) < (1e-4 if single else 1e-12)
+
+
 def _standard_case(nat, precision, mode, n_pol=2):
     import torch
     c = small_case(n_time=24, n_chan=3, n_pol=n_pol, nan_frac=0.05)
