@@ -655,7 +655,7 @@ def batched_leg(args):
         for name, ms in nat.profile_read():
             kph[name] = kph.get(name, 0.0) + ms / args.steps
     nat.profile_enable(False)
-    nat.set_tile_rows(16)
+    nat.set_tile_rows(32)   # the library defaults
     nat.set_bin_lanes(2)
     del g_k, sw_k
     abytes = algorithmic_bytes(cfg.n_vis, cfg.n_time * cfg.n_baseline, C * P, npad)
@@ -978,7 +978,7 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16, 32],
-                    help="uv-tile rows of the gridder (0: library default, 16)")
+                    help="uv-tile rows of the gridder (0: library default, 32)")
     ap.add_argument("--tile-waves", type=int, default=0, choices=[0, 1, 2, 4],
                     help="wavefronts per tile of the gridder, splitting its records "
                          "(0: library default)")
@@ -1304,7 +1304,7 @@ def main():
                    "planes": int(n_planes),
                    "form": f"{args.precision} (uv-tiled bins, a workgroup of "
                            f"{args.tile_waves or 4} wavefront(s) per "
-                           f"{args.tile_rows or 16}x32-cell tile splitting its records, "
+                           f"{args.tile_rows or 32}x32-cell tile splitting its records, "
                            f"lane-owned residue classes, FP32 register runs added to an "
                            f"LDS subgrid (FP64 cells for fp32_kahan))"},
         "roofline": roofline,
@@ -1319,7 +1319,7 @@ def main():
         "briggs_ms": round(t_briggs * 1e3, 2),
         "briggs_first_call_ms": round(t_briggs_first * 1e3, 2),
         "briggs_phases_ms": {k: round(v, 3) for k, v in briggs_ph.items()},
-        "tile_rows": args.tile_rows or 16,
+        "tile_rows": args.tile_rows or 32,
         "c1": c1,
         "mosaic": mosaic,
         "batched": batched,

@@ -601,7 +601,8 @@ def set_degrid_mode(mode):
 
 
 def set_tile_rows(rows):
-    """Tile rows (8, 16 or 32) of the uv-tiled gridders on the current device."""
+    """Tile rows (8, 16 or 32; default 32) of the uv-tiled standard gridder on the
+    current device."""
     ctx = context()
     _check("ngc_set_tile_rows", load_library().ngc_set_tile_rows(ctx.handle, int(rows)))
 

@@ -694,12 +694,20 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
     __shared__ CELL s_acc[NPC * PLANE];
     constexpr int AXROW = 9;
     __shared__ ACC s_ax[W][2][SB][AXROW];
-    __shared__ int2 s_c0[W][SB];
+    // per staged sample its window start, u in the low and v in the high 16 bits
+    __shared__ int s_c0[W][SB];
     __shared__ WD s_wd[W][SB];
     __shared__ ACC s_k[513];
     __shared__ int s_item;
     __shared__ int s_nz, s_e;
     __shared__ int64_t s_tq[kChunk][3];
+    // the 64 KB subgrid of 4 complex pols in 32-row FP64 cells (the default
+    // shape of 4-pol images in the single-precision forms) leaves just room
+    // for two workgroups per CU
+    static_assert(!(NPC * PLANE * sizeof(CELL) == 65536 && W == 4 && sizeof(ACC) == 4) ||
+                      sizeof(s_acc) + sizeof(s_ax) + sizeof(s_c0) + sizeof(s_wd) + sizeof(s_k) +
+                              sizeof(s_tq) + 3 * sizeof(int) <= 81920,
+                  "two workgroups of a 64 KB subgrid fit the 160 KB of a CU");
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -883,7 +891,7 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
                 const bool moved = (unsigned)(dlt + it) >= 8u;
                 s_ax[wave][axis][js][(c0 + it) & 7] = moved ? -t - (ACC)0 : t;
             }
-            if (h == 0) reinterpret_cast<int *>(&s_c0[wave][js])[axis] = c0;
+            if (h == 0) reinterpret_cast<short *>(&s_c0[wave][js])[axis] = (short)c0;
             if (wd_lane) {
                 if (end - b0 < SB) {   // the share's last batch: records past its end add nothing
 #pragma unroll
@@ -903,8 +911,8 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
         const int ng = (nh + KG - 1) / KG;
         ACC au0[KG], av0[KG], au1[KG], av1[KG];
         WD wd0[KG], wd1[KG];
-        int2 c00, c01;
-        auto load_group = [&](int gi, ACC (&au)[KG], ACC (&av)[KG], WD (&wd)[KG], int2 &c0l) {
+        int c00, c01;
+        auto load_group = [&](int gi, ACC (&au)[KG], ACC (&av)[KG], WD (&wd)[KG], int &c0l) {
 #pragma unroll
             for (int jj = 0; jj < KG; jj++) {
                 au[jj] = s_ax[wave][0][gi * KG + jj][ra];
@@ -914,12 +922,12 @@ std_grid_tiled_kernel(StdArgs a, BinGeom g, const unsigned long long *__restrict
             c0l = s_c0[wave][gi * KG + (lane & (KG - 1))];
         };
         auto run_group = [&](int gi, const ACC (&au)[KG], const ACC (&av)[KG],
-                             const WD (&wd)[KG], const int2 &c0l) {
+                             const WD (&wd)[KG], const int &c0l) {
 #pragma unroll
             for (int jj = 0; jj < KG; jj++) {
                 if ((chg >> (gi * KG + jj)) & 1u) {
-                    const int wu = __builtin_amdgcn_readlane(c0l.x, jj);
-                    const int wv = __builtin_amdgcn_readlane(c0l.y, jj);
+                    const int w2 = __builtin_amdgcn_readlane(c0l, jj);
+                    const int wu = (int)(short)(w2 & 0xffff), wv = w2 >> 16;
                     if (signbit(au[jj]) || signbit(av[jj])) {
                         flush();
                         cur_u = residue_cell(wu, ra);

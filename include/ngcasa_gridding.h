@@ -558,7 +558,7 @@ int ngc_adam_step_sched(ngc_context *ctx, void *stream, int64_t n, float *w, con
 /*
  * Tile shape of the uv-tiled gridders: 8, 16 or 32 grid rows per tile (the
  * tile is always 32 columns wide).  Set by the autotuner (no counterpart in the
- * reference, whose gridder has no tiles).  Default 16.
+ * reference, whose gridder has no tiles).  Default 32.
  */
 int ngc_set_tile_rows(ngc_context *ctx, int tile_rows);
 
