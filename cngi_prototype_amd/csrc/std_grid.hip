@@ -3,10 +3,12 @@
 // Replaces _standard_grid_jit (ngcasa/imaging/_imaging_utils/_standard_grid.py:243-371)
 // behind ngc_standard_grid().  Two implementations:
 //
-//  * plain FP64 drop-in (NGC_PREC_FP64, and every density-mode call): one
-//    thread per (time, baseline, channel) sample walks the pols and the
-//    support^2 taps in the reference's loop order and scatters with FP64
-//    atomics into the grid in HBM.
+//  * plain FP64 drop-in (NGC_PREC_FP64, and density-mode calls with supports
+//    above 1): one thread per (time, baseline, channel) sample walks the pols
+//    and the support^2 taps in the reference's loop order and scatters with
+//    FP64 atomics into the grid in HBM.  Density mode with the one-tap kernel
+//    make_imaging_weight uses goes through std_density_runs_kernel instead:
+//    the same values, one atomic per run of samples sharing a cell.
 //
 //  * uv-tiled (NGC_PREC_FP64_TILED, NGC_PREC_FP32_KAHAN, NGC_PREC_FP32): a
 //    count pass and a fill pass bin every (row, channel) sample into the
@@ -15,10 +17,10 @@
 //    consecutive integrations of one baseline together).  The fill pass
 //    stores per sample the cell, the sub-cell offsets and the weighted
 //    visibilities of all pols in the accumulation type, and adds the FP64
-//    sum_weight term.  One workgroup of 1, 2 or 4 wavefronts (splitting the
-//    pols) then grids one tile into an LDS subgrid (lane-owned residue
-//    classes, register runs; see the kernel) and writes each grid cell to HBM
-//    once.  Pixel positions and sum_weight stay FP64; FP32_KAHAN accumulates
+//    sum_weight term.  One workgroup of 1, 2 or 4 wavefronts (each taking a
+//    contiguous share of the tile's records) then grids one tile into an LDS
+//    subgrid (lane-owned residue classes, register runs; see the kernel) and
+//    writes each grid cell to HBM once.  Pixel positions and sum_weight stay FP64; FP32_KAHAN accumulates
 //    the FP32 register runs into FP64 LDS cells.
 #include <hipcub/hipcub.hpp>
 
