@@ -655,7 +655,7 @@ def batched_leg(args):
         for name, ms in nat.profile_read():
             kph[name] = kph.get(name, 0.0) + ms / args.steps
     nat.profile_enable(False)
-    nat.set_tile_rows(32)   # the library defaults
+    nat.set_tile_rows(0)   # the library defaults
     nat.set_bin_lanes(2)
     del g_k, sw_k
     abytes = algorithmic_bytes(cfg.n_vis, cfg.n_time * cfg.n_baseline, C * P, npad)
@@ -978,7 +978,7 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--precision", default="fp32_kahan", choices=["fp32_kahan", "fp32"])
     ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16, 32],
-                    help="uv-tile rows of the gridder (0: library default, 32)")
+                    help="uv-tile rows of the gridder (0: library default, 32 rows at this size)")
     ap.add_argument("--tile-waves", type=int, default=0, choices=[0, 1, 2, 4],
                     help="wavefronts per tile of the gridder, splitting its records "
                          "(0: library default)")

@@ -601,8 +601,9 @@ def set_degrid_mode(mode):
 
 
 def set_tile_rows(rows):
-    """Tile rows (8, 16 or 32; default 32) of the uv-tiled standard gridder on the
-    current device."""
+    """Tile rows (8, 16 or 32; 0, the default: 32 on calls with at least 65,536
+    channel-tile bins of 32 x 32 cells, else 16) of the uv-tiled standard
+    gridder on the current device."""
     ctx = context()
     _check("ngc_set_tile_rows", load_library().ngc_set_tile_rows(ctx.handle, int(rows)))
 

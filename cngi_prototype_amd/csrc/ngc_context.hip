@@ -99,8 +99,8 @@ extern "C" {
 
 int ngc_set_tile_rows(ngc_context *ctx, int tile_rows) {
     NGC_CHECK(ctx, NGC_ERR_ARG, "ctx is NULL");
-    NGC_CHECK(tile_rows == 8 || tile_rows == 16 || tile_rows == 32, NGC_ERR_ARG,
-              "tile_rows must be 8, 16 or 32");
+    NGC_CHECK(tile_rows == 0 || tile_rows == 8 || tile_rows == 16 || tile_rows == 32, NGC_ERR_ARG,
+              "tile_rows must be 0 (by size), 8, 16 or 32");
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->tile_rows = tile_rows;
     return NGC_OK;

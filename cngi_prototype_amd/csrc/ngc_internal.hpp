@@ -85,7 +85,7 @@ struct ngc_context {
     // rocFFT plans keyed by (n_u, n_v, batch, double_precision)
     std::map<std::tuple<int64_t, int64_t, int64_t, int>, ngc::FftPlan *> plans;
     int *d_error = nullptr;  // device error flags [4]
-    int tile_rows = 32;      // uv-tile rows of the tiled gridder (8, 16 or 32)
+    int tile_rows = 0;       // uv-tile rows of the tiled gridder (8, 16 or 32; 0: by size)
     int bin_chan_lanes = 2;  // channels per binning wavefront (1, 2, 4, 8)
     int tile_waves = 0;      // wavefronts per tile of the tiled gridder (0: default)
     int n_cu = 256;          // compute units of the device

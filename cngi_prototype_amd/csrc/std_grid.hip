@@ -1213,6 +1213,18 @@ struct TiledBufs {
     int half = 0;
 };
 
+// Tile rows of a call: the caller's choice, else 32 rows where the call has
+// at least 65,536 (image channel, tile) bins of 32 x 32 cells, else 16.  With
+// fewer bins the two workgroups per CU that a 32-row 4-pol tile allows have
+// too few tiles to balance (C1's 614^2 grid, 16 channels, 6,400 bins: image
+// call 0.89 ms with 16 rows, 1.02 ms with 32); with 380K bins (C2, C3) the
+// taller tiles bin fewer footprints twice and win.
+static int tile_rows_for(const ngc_context *ctx, const StdArgs &a) {
+    if (ctx->tile_rows) return ctx->tile_rows;
+    const int64_t bins32 = a.n_imag_chan * ceil_div(a.n_u, (int64_t)32) * ceil_div(a.n_v, (int64_t)32);
+    return bins32 >= 65536 ? 32 : 16;
+}
+
 static TiledGeom tiled_geom(const StdArgs &a, const TiledCfg &cfg) {
     TiledGeom t;
     t.cfg = cfg;
@@ -1569,7 +1581,7 @@ static int standard_grid_impl(ngc_context *ctx, hipStream_t s, int mode, int pre
     // ---------------- tiled path ----------------
     NGC_TRY(tiled_supported(a, pol_identity, cgk_len));
     TiledCfg cfg;
-    cfg.tu = ctx->tile_rows;
+    cfg.tu = tile_rows_for(ctx, a);
     cfg.ch_lanes = ctx->bin_chan_lanes;
     cfg.waves = ctx->tile_waves;
     const TiledGeom tg = tiled_geom(a, cfg);
@@ -1686,7 +1698,7 @@ static int standard_grid_noalloc_impl(ngc_context *ctx, hipStream_t s, int mode,
     if (tiled) {
         NGC_TRY(tiled_supported(a, pol_identity, cgk_len));
         TiledCfg cfg;
-        cfg.tu = ctx->tile_rows;
+        cfg.tu = tile_rows_for(ctx, a);
         cfg.ch_lanes = ctx->bin_chan_lanes;
         cfg.waves = ctx->tile_waves;
         tg = tiled_geom(a, cfg);
@@ -1951,7 +1963,7 @@ static int plan_create(ngc_context *ctx, hipStream_t s, int mode, int precision,
     // these inputs (one warm-up and one timed run each) and keep the fastest
     const std::string key = shape_key(mode, precision, a);
     TiledCfg best;
-    best.tu = ctx->tile_rows;
+    best.tu = tile_rows_for(ctx, a);
     best.ch_lanes = ctx->bin_chan_lanes;
     best.waves = ctx->tile_waves;
     std::string tune_log;

@@ -557,8 +557,10 @@ int ngc_adam_step_sched(ngc_context *ctx, void *stream, int64_t n, float *w, con
 
 /*
  * Tile shape of the uv-tiled gridders: 8, 16 or 32 grid rows per tile (the
- * tile is always 32 columns wide).  Set by the autotuner (no counterpart in the
- * reference, whose gridder has no tiles).  Default 32.
+ * tile is always 32 columns wide), or 0 (the default) for 32 rows on calls
+ * with at least 65,536 (image channel, 32 x 32 tile) bins and 16 rows below
+ * that.  Set by the autotuner (no counterpart in the reference, whose gridder
+ * has no tiles).
  */
 int ngc_set_tile_rows(ngc_context *ctx, int tile_rows);
 

@@ -38,7 +38,7 @@ def test_standard_grid_matches_oracle(gpu, precision, mode, rows):
     try:
         _standard_case(nat, precision, mode)
     finally:
-        nat.set_tile_rows(32)
+        nat.set_tile_rows(0)
 
 
 @pytest.mark.parametrize("waves", [1, 2, 4])
